@@ -1,0 +1,318 @@
+"""ctypes mirror of include/imex.h and a thin numpy-facing wrapper.
+
+The shared library ``libimex_amd.so`` is built in-tree (``make -C imex_amd``).
+Loading fails loudly if it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+GC = 3
+NVAR = 8
+
+BC_TRANSMISSIVE, BC_PERIODIC = 0, 2
+RECON_MINMOD, RECON_VANLEER, RECON_WENO5 = 1, 2, 4
+EOS_IDEAL, EOS_RK = 0, 1
+CT_CONSISTENT, CT_PLAIN = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN = 0, 1, 2
+CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
+WARN_GMRES_MAXOUTER, WARN_NEWTON_MAXITER, WARN_POSITIVITY, WARN_EOS_MAXITER = 1, 2, 4, 8
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double), ("y1", C.c_double),
+        ("gamma", C.c_double),
+        ("bc_x", C.c_int), ("bc_y", C.c_int),
+        ("recon", C.c_int), ("minmod_central_y", C.c_int),
+        ("eos", C.c_int),
+        ("rk_alfa", C.c_double), ("rk_b", C.c_double), ("rk_R", C.c_double), ("rk_cv", C.c_double),
+        ("cfl", C.c_double),
+        ("floor_on", C.c_int),
+        ("p_floor", C.c_double), ("theta", C.c_double), ("mbig_fac", C.c_double),
+        ("newton_max", C.c_int), ("newton_tol", C.c_double),
+        ("gmres_tol", C.c_double), ("gmres_restart", C.c_int), ("gmres_max_outer", C.c_int),
+        ("shock_kappa1", C.c_double), ("time_stop", C.c_double),
+        ("ct_mode", C.c_int), ("precision", C.c_int),
+        ("mixed_inner_tol", C.c_double),
+    ]
+
+
+class Scalars(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t", "dt", "adt", "a_max", "B_max_x", "B_max_y", "vx_max", "vy_max",
+                                          "v_max", "ch", "ch_x", "ch_y", "prev_dt")] + \
+               [("timestep", C.c_int), ("rks", C.c_int)]
+
+
+class StageHost(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("u_con", "Qe", "Qs", "bx", "by", "bx_qs", "by_qs", "fx", "fy")]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in ("steps", "mag_solves", "mag_gmres_iters", "mag_gmres_restarts",
+                                            "p_solves", "p_gmres_iters", "p_gmres_restarts", "newton_iters",
+                                            "op_applies_mag", "op_applies_p", "mixed_refinements")] + \
+               [("warnings", C.c_int), ("max_floored", C.c_int), ("min_p", C.c_double), ("div_b", C.c_double)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("res0", C.c_double), ("res", C.c_double),
+                ("status", C.c_int)]
+
+
+def struct_dict(s):
+    return {n: getattr(s, n) for n, _ in s._fields_}
+
+
+# ---------------------------------------------------------------- problem presets
+def default_params():
+    p = Params()
+    lib().imex_default_params(C.byref(p))
+    return p
+
+
+def vortex_params(n=64, m=None, amp=1.0, eos=EOS_IDEAL):
+    """MHD vortex on periodic [-5,5]^2, gamma 5/3, Minmod with the vortex central y-slope, StaggerCT."""
+    p = default_params()
+    p.nx, p.ny = n, (m or n)
+    p.x0, p.x1, p.y0, p.y1 = -5.0, 5.0, -5.0, 5.0
+    p.gamma = 5.0 / 3.0
+    p.bc_x = p.bc_y = BC_PERIODIC
+    p.recon, p.minmod_central_y = RECON_MINMOD, 1
+    p.eos = eos
+    p.floor_on, p.p_floor = 1, 1e-7
+    return p
+
+
+def orszag_tang_params(n=64):
+    """Orszag-Tang on periodic [0,2pi]^2, gamma 5/3, WENO5, p_floor 1e-8."""
+    p = default_params()
+    p.nx = p.ny = n
+    p.x0, p.x1, p.y0, p.y1 = 0.0, 2 * np.pi, 0.0, 2 * np.pi
+    p.gamma = 5.0 / 3.0
+    p.bc_x = p.bc_y = BC_PERIODIC
+    p.recon, p.minmod_central_y = RECON_WENO5, 0
+    p.floor_on, p.p_floor = 1, 1e-8
+    return p
+
+
+def blast_params(n=64):
+    """Magnetic blast on [-0.5,0.5]^2, outflow (transmissive), gamma 1.4, VanLeer, p_floor 1e-10."""
+    p = default_params()
+    p.nx = p.ny = n
+    p.x0, p.x1, p.y0, p.y1 = -0.5, 0.5, -0.5, 0.5
+    p.gamma = 1.4
+    p.bc_x = p.bc_y = BC_TRANSMISSIVE
+    p.recon, p.minmod_central_y = RECON_VANLEER, 0
+    p.floor_on, p.p_floor = 1, 1e-10
+    return p
+
+
+def case_of(name):
+    """(params, init kwargs) of a named workload."""
+    if name.startswith("vortex"):
+        n = int(name[6:] or 64)
+        return vortex_params(n), dict(case=CASE_VORTEX, amp=1.0, u0=1.0, v0=1.0)
+    if name.startswith("ot"):
+        n = int(name[2:] or 64)
+        return orszag_tang_params(n), dict(case=CASE_ORSZAG_TANG)
+    if name.startswith("blast"):
+        n = int(name[5:] or 64)
+        return blast_params(n), dict(case=CASE_BLAST, amp=100.0)
+    raise ValueError(name)
+
+
+# ---------------------------------------------------------------- library
+_LIB = None
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libimex_amd.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `make -C imex_amd` (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    sig = {
+        "imex_default_params": (C.c_int, [C.POINTER(Params)]),
+        "imex_init_case": (C.c_int, [C.POINTER(Params), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp]),
+        "imex_create": (C.c_int, [C.POINTER(Params), C.POINTER(vp)]),
+        "imex_destroy": (C.c_int, [vp]),
+        "imex_last_error": (C.c_char_p, []),
+        "imex_device_bytes": (C.c_longlong, [vp]),
+        "imex_set_state": (C.c_int, [vp, dp, dp, dp, C.c_double, C.c_int, C.c_double]),
+        "imex_get_state": (C.c_int, [vp, dp, dp, dp, dp, C.POINTER(Scalars)]),
+        "imex_step": (C.c_int, [vp, C.c_int, C.POINTER(Stats)]),
+        "imex_step_async": (C.c_int, [vp, C.c_int]),
+        "imex_sync": (C.c_int, [vp]),
+        "imex_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
+        "imex_reset_stats": (C.c_int, [vp]),
+        "imex_div_b": (C.c_double, [vp]),
+        "imex_set_stage": (C.c_int, [vp, C.POINTER(StageHost), C.POINTER(Scalars)]),
+        "imex_get_stage": (C.c_int, [vp, C.POINTER(StageHost), C.POINTER(Scalars)]),
+        "imex_cfl": (C.c_int, [vp, C.POINTER(Scalars)]),
+        "imex_stage_combine": (C.c_int, [vp, C.c_int]),
+        "imex_explicit_conv": (C.c_int, [vp]),
+        "imex_mag_update_b": (C.c_int, [vp]),
+        "imex_explicit_press": (C.c_int, [vp]),
+        "imex_pre_subsystem_pressure": (C.c_int, [vp]),
+        "imex_ct_update": (C.c_int, [vp]),
+        "imex_apply_op": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
+        "imex_apply_op_T": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
+        "imex_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp, C.c_double, C.POINTER(SolveInfo)]),
+        "imex_pressure_newton": (C.c_int, [vp, dp, dp, C.POINTER(SolveInfo)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def grid_shape(p):
+    return (p.ny + 2 * GC, p.nx + 2 * GC, NVAR)
+
+
+def array_shape(p):
+    return (p.ny + 2 * GC, p.nx + 2 * GC)
+
+
+def init_case(p, case, amp=1.0, u0=0.0, v0=0.0, smooth=0):
+    """Primitive initial data (Grid layout [j][i][k]) and staggered faces (Array layout [j][i])."""
+    u = np.zeros(grid_shape(p))
+    bx = np.zeros(array_shape(p))
+    by = np.zeros(array_shape(p))
+    rc = lib().imex_init_case(C.byref(p), case, amp, u0, v0, smooth, ptr(u), ptr(bx), ptr(by))
+    if rc < 0:
+        raise RuntimeError(lib().imex_last_error().decode())
+    return u, bx, by
+
+
+def new_stage(p):
+    g, a = grid_shape(p), array_shape(p)
+    return {k: np.zeros(g) for k in ("u_con", "Qe", "Qs")} | {k: np.zeros(a) for k in ("bx", "by", "bx_qs", "by_qs", "fx", "fy")}
+
+
+def stage_struct(st):
+    s = StageHost()
+    for k, _ in StageHost._fields_:
+        setattr(s, k, ptr(st[k]) if st.get(k) is not None else None)
+    return s
+
+
+class ImexError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One device context (imex_ctx) with numpy convenience methods."""
+
+    def __init__(self, params):
+        self.p = params
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.imex_create(C.byref(params), C.byref(h))
+        if rc < 0:
+            raise ImexError(f"imex_create: {self.L.imex_last_error().decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise ImexError(f"{what} failed ({rc}): {self.L.imex_last_error().decode()}")
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.imex_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, u_pri, bx, by, t=0.0, timestep=0, prev_dt=0.0):
+        return self._chk(self.L.imex_set_state(self.h, ptr(np.ascontiguousarray(u_pri)), ptr(np.ascontiguousarray(bx)),
+                                               ptr(np.ascontiguousarray(by)), t, timestep, prev_dt), "set_state")
+
+    def get_state(self):
+        p = self.p
+        u_con, u_pri = np.zeros(grid_shape(p)), np.zeros(grid_shape(p))
+        bx, by = np.zeros(array_shape(p)), np.zeros(array_shape(p))
+        sc = Scalars()
+        self._chk(self.L.imex_get_state(self.h, ptr(u_con), ptr(u_pri), ptr(bx), ptr(by), C.byref(sc)), "get_state")
+        return u_con, u_pri, bx, by, struct_dict(sc)
+
+    def step(self, n=1):
+        st = Stats()
+        rc = self._chk(self.L.imex_step(self.h, n, C.byref(st)), "step")
+        return rc, struct_dict(st)
+
+    def step_async(self, n=1):
+        return self._chk(self.L.imex_step_async(self.h, n), "step_async")
+
+    def sync(self):
+        return self._chk(self.L.imex_sync(self.h), "sync")
+
+    def stats(self):
+        st = Stats()
+        self._chk(self.L.imex_get_stats(self.h, C.byref(st)), "get_stats")
+        return struct_dict(st)
+
+    def reset_stats(self):
+        self.L.imex_reset_stats(self.h)
+
+    def div_b(self):
+        return self.L.imex_div_b(self.h)
+
+    def set_stage(self, st, sc):
+        s = Scalars()
+        for k, v in sc.items():
+            setattr(s, k, v)
+        return self._chk(self.L.imex_set_stage(self.h, C.byref(stage_struct(st)), C.byref(s)), "set_stage")
+
+    def get_stage(self):
+        st = new_stage(self.p)
+        sc = Scalars()
+        self._chk(self.L.imex_get_stage(self.h, C.byref(stage_struct(st)), C.byref(sc)), "get_stage")
+        return st, struct_dict(sc)
+
+    def sub(self, name):
+        return self._chk(getattr(self.L, "imex_" + name)(self.h), name)
+
+    def apply_op(self, op, x, p_freeze=None, mbig=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
+        self._chk(self.L.imex_apply_op(self.h, op, ptr(x), ptr(y), ptr(pf), mbig), "apply_op")
+        return y
+
+    def gmres(self, op, b, x0, precond=0, p_freeze=None, mbig=0.0):
+        x = np.array(x0, dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
+        info = SolveInfo()
+        rc = self._chk(self.L.imex_gmres(self.h, op, ptr(b), ptr(x), precond, ptr(pf), mbig, C.byref(info)), "gmres")
+        return x, struct_dict(info), rc
+
+    def pressure_newton(self, b, p0):
+        p = np.array(p0, dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        info = SolveInfo()
+        rc = self._chk(self.L.imex_pressure_newton(self.h, ptr(b), ptr(p), C.byref(info)), "pressure_newton")
+        return p, struct_dict(info), rc

@@ -1,0 +1,623 @@
+// kernels_stage.hip — stencil and pointwise kernels of one IMEX stage (FP64 form).
+//
+// Every kernel works on SoA FP64 planes with the reference's ghost frame.  Kernels
+// that the reference runs on the interior followed by a ghost fill are either
+// written that way (interior kernel + k_bc_fill) or, where cheaper, evaluated at
+// every padded cell through the boundary index map, which produces the same
+// values as the reference's copy-type ghost fill.
+#include "kernels.hpp"
+
+// ================================================================ ghost fill
+// bound_cond_u / bound_cond_array (boundary.cpp:5-175, timestep.cpp:166-222) and
+// bound_cond_staggered (constrained_transport.cpp:11-64): ghost (i,j) <- (mx(i), my(j)).
+struct FillMap { int lox, hix, loy, hiy; };   // identity range of the map
+
+__device__ __forceinline__ int fill_mx(const Dims& d, int kind, int i) {
+  if (d.perx) return map_x(d, i);
+  int lo = (kind == 1) ? GC - 1 : GC;           // staggered x-normal keeps face g-1
+  if (i < lo) return lo;
+  if (i > d.nx + GC - 1) return d.nx + GC - 1;
+  return i;
+}
+__device__ __forceinline__ int fill_my(const Dims& d, int kind, int j) {
+  if (d.pery) return map_y(d, j);
+  int lo = (kind == 2) ? GC - 1 : GC;           // staggered y-normal keeps face g-1
+  if (j < lo) return lo;
+  if (j > d.ny + GC - 1) return d.ny + GC - 1;
+  return j;
+}
+
+__global__ void k_bc_fill(Dims d, PlaneSet ps, int kind, FillMap fm, int nbot, int ntop, int nside_l, int nside_r, int total) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int i, j;
+  int rows_mid = fm.hiy - fm.loy + 1;
+  if (t < nbot) { j = t / d.NX; i = t % d.NX; }
+  else if (t < nbot + ntop) { int u = t - nbot; j = fm.hiy + 1 + u / d.NX; i = u % d.NX; }
+  else {
+    int u = t - nbot - ntop;
+    int per_row = nside_l + nside_r;
+    int r = u / per_row, c = u % per_row;
+    j = fm.loy + r;
+    i = (c < nside_l) ? c : (fm.hix + 1 + (c - nside_l));
+    if (r >= rows_mid) return;
+  }
+  int si = fill_mx(d, kind, i), sj = fill_my(d, kind, j);
+  long long dst = pidx(d, i, j), src = pidx(d, si, sj);
+  for (int p = 0; p < ps.n; p++) ps.p[p][dst] = ps.p[p][src];
+}
+
+void launch_bc_fill(const Dims& d, const PlaneSet& ps, int kind, hipStream_t st) {
+  FillMap fm;
+  fm.lox = (!d.perx && kind == 1) ? GC - 1 : GC;
+  fm.hix = d.nx + GC - 1;
+  fm.loy = (!d.pery && kind == 2) ? GC - 1 : GC;
+  fm.hiy = d.ny + GC - 1;
+  int nbot = fm.loy * d.NX;
+  int ntop = (d.NY - 1 - fm.hiy) * d.NX;
+  int nl = fm.lox, nr = d.NX - 1 - fm.hix;
+  int rows_mid = fm.hiy - fm.loy + 1;
+  int total = nbot + ntop + rows_mid * (nl + nr);
+  int bs = 256;
+  k_bc_fill<<<(total + bs - 1) / bs, bs, 0, st>>>(d, ps, kind, fm, nbot, ntop, nl, nr, total);
+}
+
+// ================================================================ start of step
+// calch + caldt + MagneticEigenvalues (timestep.cpp:5-67, 89-120, 254-273) fused with
+// the con -> pri -> con round trip of main.cpp:356/561: u_n = pri2con(pri).
+// mode 0: `uin` holds primitives; mode 1: `uin` holds conservatives (pri = con2pri).
+__global__ void __launch_bounds__(256) k_step_start(Dims d, Phys P, int mode, const double* __restrict__ uin,
+                                                    double* __restrict__ un, double* partial, unsigned* counter,
+                                                    double* S, int* eflag) {
+  __shared__ double sh[16];
+  double r[8] = {1e-8, 1e-8, 1e-8, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const long long total = (long long)d.NX * d.NY;
+  int ef = 0;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < total; c += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(c % d.NX), j = (int)(c / d.NX);
+    double q[NVAR], pri[NVAR];
+    for (int k = 0; k < NVAR; k++) q[k] = uin[k * d.PS + c];
+    if (mode == 0) { for (int k = 0; k < NVAR; k++) pri[k] = q[k]; }
+    else con2pri(P, pri, q, &ef);
+    double rho = pri[0], p = pri[4], Bx = pri[5], By = pri[6], Bz = pri[7];
+    double B2 = Bx * Bx + By * By + Bz * Bz;
+    double B = sqrt(B2);
+    double v = sqrt(pri[1] * pri[1] + pri[2] * pri[2] + pri[3] * pri[3]);
+    double cfx = eos_fast(P, rho, p, Bx, B2, &ef), cfy = eos_fast(P, rho, p, By, B2, &ef), cf = eos_fast(P, rho, p, B, B2, &ef);
+    r[0] = fmax(r[0], pri[1] + cfx);
+    r[1] = fmax(r[1], pri[2] + cfy);
+    r[2] = fmax(r[2], v + cf);
+    double bsq = pri[5] * pri[5] + pri[6] * pri[6] + pri[7] * pri[7];
+    double bx = 0.5 * (fabs(pri[1]) + sqrt(pri[1] * pri[1] + 4 * bsq / pri[0]));
+    double by = 0.5 * (fabs(pri[2]) + sqrt(pri[2] * pri[2] + 4 * bsq / pri[0]));
+    r[3] = fmax(r[3], bx);
+    r[4] = fmax(r[4], by);
+    if (i >= GC && i < d.nx + GC && j >= GC && j < d.ny + GC) {
+      r[5] = fmax(r[5], fabs(pri[1]));
+      r[6] = fmax(r[6], fabs(pri[2]));
+      r[7] = fmax(r[7], v);
+    }
+    double con[NVAR];
+    pri2con(P, con, pri, &ef);
+    for (int k = 0; k < NVAR; k++) un[k * d.PS + c] = con[k];
+  }
+  if (ef) atomicOr(eflag, 1);
+  // block max of the 8 values
+  for (int k = 0; k < 8; k++) {
+    double v = wave_max(r[k]);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = sh[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
+      partial[blockIdx.x * 8 + k] = m;
+    }
+  }
+  if (!last_block_arrive(counter)) return;
+  if (threadIdx.x == 0) {
+    double m[8];
+    for (int k = 0; k < 8; k++) m[k] = partial[k];
+    for (int b = 1; b < (int)gridDim.x; b++)
+      for (int k = 0; k < 8; k++) m[k] = fmax(m[k], partial[b * 8 + k]);
+    *counter = 0;
+    int ts = (int)S[S_TIMESTEP] + 1;
+    double ch = m[2], v_max = m[7];
+    S[S_CHX] = m[0]; S[S_CHY] = m[1]; S[S_CH] = ch;
+    S[S_VXMAX] = m[5]; S[S_VYMAX] = m[6]; S[S_VMAX] = v_max;
+    S[S_BMAXX] = m[3]; S[S_BMAXY] = m[4];
+    double a_max = (ts < 2) ? ch : v_max;
+    double dt = P.cfl * 1 / (2 * a_max / fmin(d.dx, d.dy));
+    double prev = S[S_PREVDT];
+    if (ts > 1 && prev > 0.0) dt = fmin(dt, 2.0 * prev);
+    S[S_PREVDT] = dt;
+    if (ts > 1) a_max = v_max;
+    S[S_AMAX] = a_max;
+    dt = fmin(dt, P.time_stop - S[S_T]);
+    S[S_DT] = dt;
+    S[S_T] = S[S_T] + dt;
+    S[S_TIMESTEP] = (double)ts;
+    const double bb = 1.0 - 1.0 / sqrt(2.0);
+    S[S_ADT] = bb * dt;
+  }
+}
+
+// ================================================================ stage combine (main.cpp:368-414)
+// Stage 1 of IMEX-RK2: Hs0 = (u1 - u_n)/adt0, Qe = u_n + Ae10 dt Hs0, Qs = u_n + Ai10 dt Hs0.
+__global__ void k_stage_combine(Dims d, const double* __restrict__ un, const double* __restrict__ u1,
+                                double* __restrict__ Qe, double* __restrict__ Qs,
+                                const double* __restrict__ bxn, const double* __restrict__ bx1,
+                                const double* __restrict__ byn, const double* __restrict__ by1,
+                                double* __restrict__ bxqs, double* __restrict__ byqs, const double* S) {
+  long long total = (long long)d.NX * d.NY;
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  const double bb = 1.0 - 1.0 / sqrt(2.0);
+  const double dt = S[S_DT], adt0 = bb * dt;
+  const double ce = (1.0 / (2.0 * bb)) * dt, ci = (1 - bb) * dt;
+  for (int k = 0; k < NVAR; k++) {
+    double n = un[k * d.PS + c];
+    double hs = (u1[k * d.PS + c] - n) / adt0;
+    Qe[k * d.PS + c] = n + ce * hs;
+    Qs[k * d.PS + c] = n + ci * hs;
+  }
+  double hbx = (bx1[c] - bxn[c]) / adt0, hby = (by1[c] - byn[c]) / adt0;
+  bxqs[c] = bxn[c] + ci * hbx;
+  byqs[c] = byn[c] + ci * hby;
+}
+
+// ================================================================ per-stage Qe preparation
+// p(Qe) at every padded cell (Qe_pri(.,.,4)) and the fast speed sqrt(a^2 + |B|^2/rho)
+// used by ShockDetector (flux.cpp:22-37).
+__global__ void k_qe_prep(Dims d, Phys P, const double* __restrict__ Qe, double* __restrict__ pe,
+                          double* __restrict__ cf, int* eflag) {
+  long long total = (long long)d.NX * d.NY;
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  double q[NVAR];
+  for (int k = 0; k < NVAR; k++) q[k] = Qe[k * d.PS + c];
+  int ef = 0;
+  double p = pressure_of(P, q, &ef);
+  pe[c] = p;
+  double rho = q[0];
+  double a = eos_a(P, rho, p, &ef);
+  double a2 = a * a;
+  double cA2 = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) / rho;
+  cf[c] = sqrt(a2 + cA2);
+  if (ef) atomicOr(eflag, 1);
+}
+
+// ShockDetector step 1 (flux.cpp:14-53): chi~ on the interior.
+__global__ void k_shock_chi(Dims d, Phys P, const double* __restrict__ Qe, const double* __restrict__ cf,
+                            double* __restrict__ chi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  double cmin = 1.0e300;
+  for (int di = -1; di <= 1; di++)
+    for (int dj = -1; dj <= 1; dj++) {
+      double v = cf[pidx(d, i + di, j + dj)];
+      if (v < cmin) cmin = v;
+    }
+  const double* r0 = Qe; const double* m1 = Qe + d.PS; const double* m2 = Qe + 2 * d.PS;
+  double vxp = m1[pidx(d, i + 1, j)] / r0[pidx(d, i + 1, j)], vxm = m1[pidx(d, i - 1, j)] / r0[pidx(d, i - 1, j)];
+  double vyp = m2[pidx(d, i, j + 1)] / r0[pidx(d, i, j + 1)], vym = m2[pidx(d, i, j - 1)] / r0[pidx(d, i, j - 1)];
+  double div_u = (vxp - vxm) / (2.0 * d.dx) + (vyp - vym) / (2.0 * d.dy);
+  double h = fmin(d.dx, d.dy);
+  double num = h * div_u + P.kappa1 * cmin;
+  double den = P.kappa1 * cmin;
+  chi[pidx(d, i, j)] = fmin(1.0, fmax(0.0, -num / den));
+}
+
+// ShockDetector step 2 (flux.cpp:60-75): eta = 3x3 max of chi~.
+__global__ void k_shock_eta(Dims d, const double* __restrict__ chi, double* __restrict__ eta) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  double m = 0.0;
+  for (int di = -1; di <= 1; di++)
+    for (int dj = -1; dj <= 1; dj++) {
+      double v = chi[pidx(d, i + di, j + dj)];
+      if (v > m) m = v;
+    }
+  eta[pidx(d, i, j)] = m;
+}
+
+// ================================================================ reconstruction + explicit fluxes
+// TVDReconstruction (reconstruction.cpp:5-287) at one cell from the 5-point stencil a[0..4].
+__device__ __forceinline__ void recon5(int recon, int central, const double* a, double& wL, double& wR) {
+  const double um2 = a[0], um1 = a[1], u0 = a[2], up1 = a[3], up2 = a[4];
+  if (recon == IMEX_RECON_MINMOD) {
+    double sR = up1 - u0, sL = u0 - um1, c;
+    if (sL * sR <= 0) c = 0;
+    else if (fabs(sR) <= fabs(sL)) c = sR;
+    else c = sL;
+    if (central) c = 0.5 * (sL + sR);
+    wL = u0 - 0.5 * c; wR = u0 + 0.5 * c;
+  } else if (recon == IMEX_RECON_VANLEER) {
+    double sR = up1 - u0, sL = u0 - um1, c;
+    if (fabs(sR) < 1e-16) c = 0;
+    else { double r = sL / sR; c = (r < 0) ? 0 : fmin(2 / (1 + r), (2 * r) / (1 + r)); }
+    wL = u0 - 0.5 * c * sL; wR = u0 + 0.5 * c * sR;
+  } else {
+    double t0 = um2 - 2 * um1 + u0, t1 = um2 - 4 * um1 + 3 * u0;
+    double beta0 = 13.0 / 12.0 * (t0 * t0) + 1.0 / 4.0 * (t1 * t1);
+    double t2 = um1 - 2 * u0 + up1, t3 = um1 - up1;
+    double beta1 = 13.0 / 12.0 * (t2 * t2) + 1.0 / 4.0 * (t3 * t3);
+    double t4 = u0 - 2 * up1 + up2, t5 = 3 * u0 - 4 * up1 + up2;
+    double beta2 = 13.0 / 12.0 * (t4 * t4) + 1.0 / 4.0 * (t5 * t5);
+    double e0 = 1e-6 + beta0, e1 = 1e-6 + beta1, e2 = 1e-6 + beta2;
+    e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
+    double q0 = -1.0 / 6.0 * um2 + 5.0 / 6.0 * um1 + 1.0 / 3.0 * u0;
+    double q1 = 1.0 / 3.0 * um1 + 5.0 / 6.0 * u0 - 1.0 / 6.0 * up1;
+    double q2 = 11.0 / 6.0 * u0 - 7.0 / 6.0 * up1 + 1.0 / 3.0 * up2;
+    double a0 = 3.0 / 10.0 / e0, a1 = 6.0 / 10.0 / e1, a2 = 1.0 / 10.0 / e2;
+    double s = a0 + a1 + a2;
+    wL = a0 / s * q0 + a1 / s * q1 + a2 / s * q2;
+    q0 = 1.0 / 3.0 * um2 - 7.0 / 6.0 * um1 + 11.0 / 6.0 * u0;
+    q1 = -1.0 / 6.0 * um1 + 5.0 / 6.0 * u0 + 1.0 / 3.0 * up1;
+    q2 = 1.0 / 3.0 * u0 + 5.0 / 6.0 * up1 - 1.0 / 6.0 * up2;
+    a0 = 1.0 / 10.0 / e0; a1 = 6.0 / 10.0 / e1; a2 = 3.0 / 10.0 / e2;
+    s = a0 + a1 + a2;
+    wR = a0 / s * q0 + a1 / s * q1 + a2 / s * q2;
+  }
+}
+
+// Rusanov-type flux with GLF dissipation (flux.cpp:202-243, 329-367; state.cpp:27-44, 86-106, 298-322).
+// kind 0: convective only; kind 1: convective + magnetic pressure-formula terms.
+__device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, const double* QR, double s_max, double* F) {
+  double fl[5], fr[5];
+  const double* Qs2[2] = {QL, QR};
+  double* fo[2] = {fl, fr};
+  for (int side = 0; side < 2; side++) {
+    const double* Q = Qs2[side];
+    double* f = fo[side];
+    double rho = Q[0], vx = Q[1] / Q[0], vy = Q[2] / Q[0], vz = Q[3] / Q[0];
+    double kin = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    double vn = dir == 0 ? vx : vy;
+    f[0] = rho * vn;
+    f[1] = dir == 0 ? rho * vx * vx : rho * vx * vy;
+    f[2] = dir == 0 ? rho * vy * vx : rho * vy * vy;
+    f[3] = rho * vz * vn;
+    f[4] = kin * vn;
+    if (kind == 1) {
+      double B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
+      double vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) / Q[0];
+      if (dir == 0) {
+        f[1] = f[1] + (B2 * 0.5 - Q[5] * Q[5]);
+        f[2] = f[2] + (-Q[5] * Q[6]);
+        f[3] = f[3] + (-Q[5] * Q[7]);
+        f[4] = f[4] + (Q[1] / Q[0] * B2 - Q[5] * vB);
+      } else {
+        f[1] = f[1] + (-Q[5] * Q[6]);
+        f[2] = f[2] + (B2 * 0.5 - Q[6] * Q[6]);
+        f[3] = f[3] + (-Q[6] * Q[7]);
+        f[4] = f[4] + (Q[2] / Q[0] * B2 - Q[6] * vB);
+      }
+    }
+  }
+  for (int k = 0; k < 5; k++) F[k] = 0.5 * (fl[k] + fr[k]) - 0.5 * s_max * (QR[k] - QL[k]);
+}
+
+// RusanovC_Update (kind 0, flux.cpp:409-455) and Pressure_flux_Update (kind 1, flux.cpp:290-326)
+// fused with TVDReconstruction.  Face states of a cell c are the reconstruction at
+// the ghost-map image of c, as the reference's bound_cond_u on w_* produces.
+// kind 0 writes u[0..3] (the energy slot of the convective predictor is never read);
+// kind 1 writes u[1..4] (its density equals the convective one bit for bit).
+__global__ void __launch_bounds__(256) k_explicit(Dims d, Phys P, int kind, const double* __restrict__ Qe,
+                                                  const double* __restrict__ Qs, double* __restrict__ u, const double* S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const double h = S[S_ADT], s_max = S[S_AMAX];
+  double F[2][2][5];   // [dir][minus/plus face][var]
+  for (int dir = 0; dir < 2; dir++) {
+    // face states: L/R of the three cells c-1, c, c+1 along dir
+    double wl[3][NVAR], wr[3][NVAR];
+    for (int m = 0; m < 3; m++) {
+      int ci = dir == 0 ? map_x(d, i - 1 + m) : i;
+      int cj = dir == 0 ? j : map_y(d, j - 1 + m);
+      for (int k = 0; k < 5; k++) {
+        const double* q = Qe + k * d.PS;
+        double a[5];
+        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? q[pidx(d, ci - 2 + o, cj)] : q[pidx(d, ci, cj - 2 + o)];
+        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl[m][k], wr[m][k]);
+      }
+      if (kind == 1) {
+        int bi = dir == 0 ? i - 1 + m : i, bj = dir == 0 ? j : j - 1 + m;
+        for (int k = 5; k < 8; k++) { double b = u[k * d.PS + pidx(d, bi, bj)]; wl[m][k] = b; wr[m][k] = b; }
+      } else {
+        for (int k = 5; k < 8; k++) { wl[m][k] = 0.0; wr[m][k] = 0.0; }
+      }
+    }
+    gl_flux(dir, kind, wr[0], wl[1], s_max, F[dir][0]);
+    gl_flux(dir, kind, wr[1], wl[2], s_max, F[dir][1]);
+  }
+  long long c = pidx(d, i, j);
+  int k0 = kind == 0 ? 0 : 1, k1 = kind == 0 ? 4 : 5;
+  for (int k = k0; k < k1; k++)
+    u[k * d.PS + c] = Qs[k * d.PS + c] - h / d.dx * (F[0][1][k] - F[0][0][k]) - h / d.dy * (F[1][1][k] - F[1][0][k]);
+}
+
+// ================================================================ magnetic subsystem
+// RHS of MagSubsystem_UpdateB (imex_subsystems.cpp:549-613), initial guess x0 = B(Qe),
+// and the enthalpy ratio hhat = calenthalpy(Qe) rho^e / rho (used by the pressure subsystem).
+__global__ void __launch_bounds__(256) k_mag_rhs(Dims d, Phys P, const double* __restrict__ u, const double* __restrict__ Qe,
+                                                 const double* __restrict__ Qs, const double* __restrict__ pe,
+                                                 double* __restrict__ b, double* __restrict__ x0, double* __restrict__ hhat,
+                                                 const double* S, int* eflag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const double h = S[S_ADT], dx = d.dx, dy = d.dy;
+  auto U = [&](int a, int bb, int k) { return u[k * d.PS + pidx(d, a, bb)]; };
+  auto Q = [&](int a, int bb, int k) { return Qe[k * d.PS + pidx(d, a, bb)]; };
+  auto PE = [&](int a, int bb) { return pe[pidx(d, a, bb)]; };
+  double rho_p0 = 0.5 * (U(i, j, 0) + U(i + 1, j, 0)), rho_m0 = 0.5 * (U(i, j, 0) + U(i - 1, j, 0));
+  double rho_0p = 0.5 * (U(i, j, 0) + U(i, j + 1, 0)), rho_0m = 0.5 * (U(i, j, 0) + U(i, j - 1, 0));
+  double m[3][4], n[3][4];
+  for (int c = 0; c < 3; c++) {
+    m[c][0] = 0.5 * (U(i, j, 1 + c) + U(i + 1, j, 1 + c)); m[c][1] = 0.5 * (U(i, j, 1 + c) + U(i - 1, j, 1 + c));
+    m[c][2] = 0.5 * (U(i, j, 1 + c) + U(i, j + 1, 1 + c)); m[c][3] = 0.5 * (U(i, j, 1 + c) + U(i, j - 1, 1 + c));
+    n[c][0] = 0.5 * (Q(i, j, 5 + c) + Q(i + 1, j, 5 + c)); n[c][1] = 0.5 * (Q(i, j, 5 + c) + Q(i - 1, j, 5 + c));
+    n[c][2] = 0.5 * (Q(i, j, 5 + c) + Q(i, j + 1, 5 + c)); n[c][3] = 0.5 * (Q(i, j, 5 + c) + Q(i, j - 1, 5 + c));
+  }
+  double p00 = PE(i, j), pp0 = PE(i + 1, j), pm0 = PE(i - 1, j), p0p = PE(i, j + 1), p0m = PE(i, j - 1);
+  double p_pp = 0.25 * (p00 + pp0 + p0p + PE(i + 1, j + 1));
+  double p_pm = 0.25 * (p00 + pp0 + p0m + PE(i + 1, j - 1));
+  double p_mp = 0.25 * (p00 + pm0 + p0p + PE(i - 1, j + 1));
+  double p_mm = 0.25 * (p00 + pm0 + p0m + PE(i - 1, j - 1));
+  long long c0 = pidx(d, i, j);
+  double Bx_star = Qs[5 * d.PS + c0] - (h / dy) * ((n[0][2] / rho_0p * (m[1][2] - (h / dy) * (p0p - p00)) - n[1][2] / rho_0p * (m[0][2] - h / dx * (p_pp - p_mp)))
+                                                 - (n[0][3] / rho_0m * (m[1][3] - (h / dy) * (p00 - p0m)) - n[1][3] / rho_0m * (m[0][3] - h / dx * (p_pm - p_mm))));
+  double By_star = Qs[6 * d.PS + c0] - (h / dx) * ((n[1][0] / rho_p0 * (m[0][0] - (h / dx) * (pp0 - p00)) - n[0][0] / rho_p0 * (m[1][0] - h / dy * (p_pp - p_pm)))
+                                                 - (n[1][1] / rho_m0 * (m[0][1] - (h / dx) * (p00 - pm0)) - n[0][1] / rho_m0 * (m[1][1] - h / dy * (p_mp - p_mm))));
+  double Bz_1 = ((n[2][0] / rho_p0 * (m[0][0] - h / dx * (pp0 - p00)) - n[0][0] / rho_p0 * (m[2][0]))
+               - (n[2][1] / rho_m0 * (m[0][1] - h / dx * (p00 - pm0)) - n[0][1] / rho_m0 * (m[2][1])));
+  double Bz_2 = ((n[2][2] / rho_0p * (m[1][2] - h / dy * (p0p - p00)) - n[1][2] / rho_0p * (m[2][2]))
+               - (n[2][3] / rho_0m * (m[1][3] - h / dy * (p00 - p0m)) - n[1][3] / rho_0m * (m[2][3])));
+  double Bz_star = Qs[7 * d.PS + c0] - (h / dx) * Bz_1 - (h / dy) * Bz_2;
+  int idx = (i - GC) + d.nx * (j - GC);
+  b[idx] = Bx_star; b[idx + d.N] = By_star; b[idx + 2 * d.N] = Bz_star;
+  x0[idx] = Q(i, j, 5); x0[idx + d.N] = Q(i, j, 6); x0[idx + 2 * d.N] = Q(i, j, 7);
+  double q[NVAR];
+  for (int k = 0; k < NVAR; k++) q[k] = Qe[k * d.PS + c0];
+  int ef = 0;
+  hhat[c0] = enthalpy_of(P, q, &ef) * q[0] / U(i, j, 0);
+  if (ef) atomicOr(eflag, 1);
+}
+
+// u_con[5..7] <- B~ at every padded cell through the M_B map (imex_subsystems.cpp:618-625).
+__global__ void k_mag_writeback(Dims d, const double* __restrict__ x, double* __restrict__ u) {
+  long long total = (long long)d.NX * d.NY;
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  int i = (int)(c % d.NX), j = (int)(c / d.NX);
+  int id = kidx(d, i, j);
+  u[5 * d.PS + c] = x[id];
+  u[6 * d.PS + c] = x[id + d.N];
+  u[7 * d.PS + c] = x[id + 2 * d.N];
+}
+
+// CollectMagFluxes (ct_flux.cpp:5-130): Fx at x faces (i+1/2, j), Fy at y faces (i, j+1/2),
+// evaluated at the ghost-map image of every padded index (= compute on [1, n+2g-1) then bound_cond_array).
+__global__ void __launch_bounds__(256) k_emf_flux(Dims d, const double* __restrict__ u, const double* __restrict__ Qe,
+                                                  const double* __restrict__ pe, double* __restrict__ Fx,
+                                                  double* __restrict__ Fy, const double* S) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  const int i = map_x(d, ii), j = map_y(d, jj);
+  const double h = S[S_ADT], dx = d.dx, dy = d.dy;
+  auto U = [&](int a, int b, int k) { return u[k * d.PS + pidx(d, a, b)]; };
+  auto Q = [&](int a, int b, int k) { return Qe[k * d.PS + pidx(d, a, b)]; };
+  auto PE = [&](int a, int b) { return pe[pidx(d, a, b)]; };
+  double rho_p0 = 0.5 * (U(i, j, 0) + U(i + 1, j, 0)), rho_0p = 0.5 * (U(i, j, 0) + U(i, j + 1, 0));
+  double rvx_p0 = 0.5 * (U(i, j, 1) + U(i + 1, j, 1)), rvy_p0 = 0.5 * (U(i, j, 2) + U(i + 1, j, 2));
+  double rvx_0p = 0.5 * (U(i, j, 1) + U(i, j + 1, 1)), rvy_0p = 0.5 * (U(i, j, 2) + U(i, j + 1, 2));
+  double p00 = PE(i, j);
+  double p_pp = 0.25 * (p00 + PE(i + 1, j) + PE(i, j + 1) + PE(i + 1, j + 1));
+  double p_pm = 0.25 * (p00 + PE(i + 1, j) + PE(i, j - 1) + PE(i + 1, j - 1));
+  double p_mp = 0.25 * (p00 + PE(i - 1, j) + PE(i, j + 1) + PE(i - 1, j + 1));
+  double Bxn_p0 = 0.5 * (Q(i, j, 5) + Q(i + 1, j, 5)), Byn_p0 = 0.5 * (Q(i, j, 6) + Q(i + 1, j, 6));
+  double Bxn_0p = 0.5 * (Q(i, j, 5) + Q(i, j + 1, 5)), Byn_0p = 0.5 * (Q(i, j, 6) + Q(i, j + 1, 6));
+  double n_pp[3], n_pm[3], n_mp[3], b_pp[3], b_pm[3], b_mp[3];
+  for (int c = 0; c < 3; c++) {
+    int k = 5 + c;
+    double q00 = Q(i, j, k), qp0 = Q(i + 1, j, k), qm0 = Q(i - 1, j, k), q0p = Q(i, j + 1, k), q0m = Q(i, j - 1, k);
+    n_pp[c] = 0.25 * (q00 + qp0 + q0p + Q(i + 1, j + 1, k));
+    n_pm[c] = 0.25 * (q00 + qp0 + q0m + Q(i + 1, j - 1, k));
+    n_mp[c] = 0.25 * (q00 + qm0 + q0p + Q(i - 1, j + 1, k));
+    double u00 = U(i, j, k), up0 = U(i + 1, j, k), um0 = U(i - 1, j, k), u0p = U(i, j + 1, k), u0m = U(i, j - 1, k);
+    b_pp[c] = 0.25 * (u00 + up0 + u0p + U(i + 1, j + 1, k));
+    b_pm[c] = 0.25 * (u00 + up0 + u0m + U(i + 1, j - 1, k));
+    b_mp[c] = 0.25 * (u00 + um0 + u0p + U(i - 1, j + 1, k));
+  }
+  auto PM = [&](int a, int b) { return 0.5 * (Q(a, b, 5) * U(a, b, 5) + Q(a, b, 6) * U(a, b, 6) + Q(a, b, 7) * U(a, b, 7)); };
+  auto PC = [&](const double* nn, const double* bb, int mm) { return 0.5 * (nn[0] * bb[0] + nn[1] * bb[1] + nn[2] * bb[2]) - nn[mm] * bb[mm]; };
+  double Fn_x = Byn_p0 / rho_p0 * (rvx_p0 - (h / dx) * (PE(i + 1, j) - p00)) - Bxn_p0 / rho_p0 * (rvy_p0 - (h / dy) * (p_pp - p_pm));
+  double By_1 = Byn_p0 / rho_p0 * (-1.0 / dx * ((PM(i + 1, j) - Q(i + 1, j, 5) * U(i + 1, j, 5)) - (PM(i, j) - Q(i, j, 5) * U(i, j, 5))) + 1.0 / dy * (n_pp[0] * b_pp[1] - n_pm[0] * b_pm[1]));
+  double By_2 = Bxn_p0 / rho_p0 * (-1.0 / dx * (Q(i + 1, j, 5) * U(i + 1, j, 6) - Q(i, j, 5) * U(i, j, 6)) + 1.0 / dy * (PC(n_pp, b_pp, 1) - PC(n_pm, b_pm, 1)));
+  double nu_x = S[S_BMAXX] * dx;
+  double Fdiff_x = -nu_x * (U(i + 1, j, 6) - U(i, j, 6)) / dx;
+  Fx[pidx(d, ii, jj)] = Fn_x + h * (By_1 + By_2) + Fdiff_x;
+  double Fn_y = Bxn_0p / rho_0p * (rvy_0p - (h / dy) * (PE(i, j + 1) - p00)) - Byn_0p / rho_0p * (rvx_0p - (h / dx) * (p_pp - p_mp));
+  double Bx_1 = Bxn_0p / rho_0p * (1.0 / dx * (n_pp[0] * b_pp[1] - n_mp[0] * b_mp[1]) - 1.0 / dy * ((PM(i, j + 1) - Q(i, j + 1, 6) * U(i, j + 1, 6)) - (PM(i, j) - Q(i, j, 6) * U(i, j, 6))));
+  double Bx_2 = Byn_0p / rho_0p * (1.0 / dx * (PC(n_pp, b_pp, 0) - PC(n_mp, b_mp, 0)) - 1.0 / dy * (Q(i, j + 1, 6) * U(i, j + 1, 5) - Q(i, j, 6) * U(i, j, 5)));
+  double nu_y = S[S_BMAXY] * dy;
+  double Fdiff_y = -nu_y * (U(i, j + 1, 5) - U(i, j, 5)) / dy;
+  Fy[pidx(d, ii, jj)] = Fn_y + h * (Bx_1 + Bx_2) + Fdiff_y;
+}
+
+// ================================================================ pressure subsystem pointwise/5-point kernels
+// RHS of PreSubsystem_Pressure (imex_subsystems.cpp:123-168 first pass, :226-270 second pass).
+// pass 0 also stores rho v* (mxs, mys), the initial pressure p0 = p(Qe) and the Jacobi diagonal
+// part Tdiag (implicit_operators.cpp:631-649).
+__global__ void __launch_bounds__(256) k_p_rhs(Dims d, int pass, const double* __restrict__ u, const double* __restrict__ hhat,
+                                               double* __restrict__ mxs, double* __restrict__ mys,
+                                               const double* __restrict__ pe, double* __restrict__ b,
+                                               double* __restrict__ p0, double* __restrict__ Tdiag, const double* S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const double h = S[S_ADT], dx = d.dx, dy = d.dy;
+  long long c = pidx(d, i, j);
+  auto H = [&](int a, int bb) { return hhat[pidx(d, a, bb)]; };
+  const double* MX = pass == 0 ? u + 1 * d.PS : mxs;
+  const double* MY = pass == 0 ? u + 2 * d.PS : mys;
+  auto mx = [&](int a, int bb) { return MX[pidx(d, a, bb)]; };
+  auto my = [&](int a, int bb) { return MY[pidx(d, a, bb)]; };
+  double u5 = u[5 * d.PS + c], u6 = u[6 * d.PS + c], u7 = u[7 * d.PS + c], rho = u[c];
+  double B2 = u5 * u5 + u6 * u6 + u7 * u7;
+  double h00 = H(i, j), hp0 = H(i + 1, j), hm0 = H(i - 1, j), h0p = H(i, j + 1), h0m = H(i, j - 1);
+  double p_star = u[4 * d.PS + c] - 0.5 * B2;
+  double u1 = u[1 * d.PS + c], u2 = u[2 * d.PS + c], u3 = u[3 * d.PS + c];
+  double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3) / rho;   // pass 0: rho v* == u; pass 1: updated momentum
+  double RHS = p_star - ke - h * (1 / dx * (0.5 * (hp0 + h00) * 0.5 * (mx(i + 1, j) + mx(i, j)) - 0.5 * (hm0 + h00) * 0.5 * (mx(i - 1, j) + mx(i, j)))
+                                 + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (my(i, j + 1) + my(i, j)) - 0.5 * (h0m + h00) * 0.5 * (my(i, j - 1) + my(i, j)))));
+  int idx = (i - GC) + d.nx * (j - GC);
+  b[idx] = RHS;
+  if (pass == 0) {
+    mxs[c] = u1; mys[c] = u2;
+    p0[idx] = pe[c];
+    Tdiag[idx] = h * h * ((0.5 * (hp0 + h00) + 0.5 * (hm0 + h00)) / (dx * dx) + (0.5 * (h0p + h00) + 0.5 * (h0m + h00)) / (dy * dy));
+  }
+}
+
+// Momentum update (imex_subsystems.cpp:186-215).
+__global__ void k_p_momentum(Dims d, const double* __restrict__ p, const double* __restrict__ mxs,
+                             const double* __restrict__ mys, double* __restrict__ u, const double* S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const double h = S[S_ADT];
+  double pc = p[kidx(d, i, j)];
+  double fxm = 0.5 * (p[kidx(d, i - 1, j)] + pc), fxp = 0.5 * (pc + p[kidx(d, i + 1, j)]);
+  double fym = 0.5 * (p[kidx(d, i, j - 1)] + pc), fyp = 0.5 * (pc + p[kidx(d, i, j + 1)]);
+  long long c = pidx(d, i, j);
+  u[1 * d.PS + c] = mxs[c] - h / d.dx * (fxp - fxm);
+  u[2 * d.PS + c] = mys[c] - h / d.dy * (fyp - fym);
+}
+
+// Energy from the definition (imex_subsystems.cpp:286-302).
+__global__ void k_p_energy(Dims d, Phys P, const double* __restrict__ p, double* __restrict__ u, int* eflag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  long long c = pidx(d, i, j);
+  double q[NVAR];
+  for (int k = 0; k < NVAR; k++) q[k] = u[k * d.PS + c];
+  int ef = 0;
+  u[4 * d.PS + c] = Vbase(P, p[kidx(d, i, j)], q[0], &ef) + 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0]
+                  + 0.5 * (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]);
+  if (ef) atomicOr(eflag, 1);
+}
+
+// ================================================================ constrained transport
+// CT_CornerAssemble (constrained_transport.cpp:147-164) with Ez_p0 = -Fx, Ez_0p = +Fy and the
+// cell-centre Ez~ = v_y B_x - v_x B_y of the post-pressure state (ct_flux.cpp:155-171).
+// Corners (i,j) on [g-2, n+g) — the range the face update reads.
+__device__ __forceinline__ double ez_tilde(const Dims& d, const double* u, int i, int j) {
+  long long c = pidx(d, i, j);
+  double r = u[c];
+  return u[2 * d.PS + c] / r * u[5 * d.PS + c] - u[1 * d.PS + c] / r * u[6 * d.PS + c];
+}
+__global__ void k_ct_corner(Dims d, const double* __restrict__ ezx, const double* __restrict__ ezy, double sx, double sy,
+                            const double* __restrict__ u, double* __restrict__ Ezpp) {
+  // Ez_p0 = sx * ezx, Ez_0p = sy * ezy (sx = -1, sy = +1 for the EMF-consistent form)
+  int i = blockIdx.x * blockDim.x + threadIdx.x + GC - 2, j = blockIdx.y * blockDim.y + threadIdx.y + GC - 2;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  auto EX = [&](int a, int b) { return sx * ezx[pidx(d, a, b)]; };
+  auto EY = [&](int a, int b) { return sy * ezy[pidx(d, a, b)]; };
+  int I = map_x(d, i), J = map_y(d, j);
+  double bar = 0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J));
+  const double dx = d.dx, dy = d.dy;
+  double e00 = ez_tilde(d, u, i, j), e10 = ez_tilde(d, u, i + 1, j), e01 = ez_tilde(d, u, i, j + 1), e11 = ez_tilde(d, u, i + 1, j + 1);
+  double t1 = 0.5 * ((EX(i, j) - e00) / (dx / 2.0) + (EX(i, j + 1) - e01) / (dx / 2.0));
+  double t2 = 0.5 * ((e10 - EX(i, j)) / (dx / 2.0) + (e11 - EX(i, j + 1)) / (dx / 2.0));
+  double t3 = 0.5 * ((EY(i, j) - e00) / (dy / 2.0) + (EY(i + 1, j) - e10) / (dy / 2.0));
+  double t4 = 0.5 * ((e01 - EY(i, j)) / (dy / 2.0) + (e11 - EY(i + 1, j)) / (dy / 2.0));
+  Ezpp[pidx(d, i, j)] = bar + dx / 8.0 * (t1 - t2) + dy / 8.0 * (t3 - t4);
+}
+
+// Face update (ct_flux.cpp:174-181) + bound_cond_staggered (:180-181) + cell B = face average
+// (:184-190) + bound_cond_u, all at every padded index through the maps.
+__device__ __forceinline__ int smap_x(const Dims& d, int axis, int i) {
+  if (d.perx) return map_x(d, i);
+  int lo = axis == 0 ? GC - 1 : GC;
+  return i < lo ? lo : (i > d.nx + GC - 1 ? d.nx + GC - 1 : i);
+}
+__device__ __forceinline__ int smap_y(const Dims& d, int axis, int j) {
+  if (d.pery) return map_y(d, j);
+  int lo = axis == 1 ? GC - 1 : GC;
+  return j < lo ? lo : (j > d.ny + GC - 1 ? d.ny + GC - 1 : j);
+}
+__global__ void k_ct_faces(Dims d, const double* __restrict__ bxqs, const double* __restrict__ byqs,
+                           const double* __restrict__ Ezpp, double* __restrict__ bx, double* __restrict__ by,
+                           double* __restrict__ u, const double* S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.NX || j >= d.NY) return;
+  const double h = S[S_ADT];
+  auto BX = [&](int a, int b) { int I = smap_x(d, 0, a), J = smap_y(d, 0, b); return bxqs[pidx(d, I, J)] - h / d.dy * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I, J - 1)]); };
+  auto BY = [&](int a, int b) { int I = smap_x(d, 1, a), J = smap_y(d, 1, b); return byqs[pidx(d, I, J)] + h / d.dx * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I - 1, J)]); };
+  long long c = pidx(d, i, j);
+  bx[c] = BX(i, j);
+  by[c] = BY(i, j);
+  int I = map_x(d, i), J = map_y(d, j);
+  u[5 * d.PS + c] = 0.5 * (BX(I - 1, J) + BX(I, J));
+  u[6 * d.PS + c] = 0.5 * (BY(I, J - 1) + BY(I, J));
+}
+
+// Plain Staggered_CT face EMFs (constrained_transport.cpp:73-100): central averages of
+// f_full_con[6] / g_full_con[5] over the reconstructed states of u_con.
+__global__ void __launch_bounds__(256) k_ct_plain_emf(Dims d, Phys P, const double* __restrict__ u,
+                                                      double* __restrict__ ezx, double* __restrict__ ezy) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  int i = map_x(d, ii), j = map_y(d, jj);
+  double w[2][2][NVAR];   // [dir][0: right state of (i,j), 1: left state of neighbour]
+  for (int dir = 0; dir < 2; dir++)
+    for (int side = 0; side < 2; side++) {
+      int ci = dir == 0 ? map_x(d, i + side) : i, cj = dir == 0 ? j : map_y(d, j + side);
+      for (int k = 0; k < NVAR; k++) {
+        const double* q = u + k * d.PS;
+        double a[5];
+        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? q[pidx(d, ci - 2 + o, cj)] : q[pidx(d, ci, cj - 2 + o)];
+        double wl, wr;
+        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl, wr);
+        w[dir][side][k] = side == 0 ? wr : wl;
+      }
+    }
+  auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[6] * vx - vy * Q[5]; };
+  auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[5] * vy - vx * Q[6]; };
+  ezx[pidx(d, ii, jj)] = 0.5 * (fx6(w[0][0]) + fx6(w[0][1]));   // Ez_p0 = -ezx
+  ezy[pidx(d, ii, jj)] = 0.5 * (gy5(w[1][0]) + gy5(w[1][1]));   // Ez_0p = +ezy
+}
+
+// ================================================================ state conversion, diagnostics
+__global__ void k_con2pri(Dims d, Phys P, const double* __restrict__ con, double* __restrict__ pri, int* eflag) {
+  long long total = (long long)d.NX * d.NY;
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  double q[NVAR], r[NVAR];
+  for (int k = 0; k < NVAR; k++) q[k] = con[k * d.PS + c];
+  int ef = 0;
+  con2pri(P, r, q, &ef);
+  for (int k = 0; k < NVAR; k++) pri[k * d.PS + c] = r[k];
+  if (ef) atomicOr(eflag, 1);
+}
+__global__ void k_pri2con(Dims d, Phys P, const double* __restrict__ pri, double* __restrict__ con, int* eflag) {
+  long long total = (long long)d.NX * d.NY;
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  double q[NVAR], r[NVAR];
+  for (int k = 0; k < NVAR; k++) q[k] = pri[k * d.PS + c];
+  int ef = 0;
+  pri2con(P, r, q, &ef);
+  for (int k = 0; k < NVAR; k++) con[k * d.PS + c] = r[k];
+  if (ef) atomicOr(eflag, 1);
+}
+// cal_divergence_error, StaggerCT branch (imex_subsystems.cpp:630-660): per-block partial sums.
+__global__ void k_divb(Dims d, const double* __restrict__ bx, const double* __restrict__ by, double* partial) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < d.N; t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(t % d.nx) + GC, j = (int)(t / d.nx) + GC;
+    double v = (bx[pidx(d, i, j)] - bx[pidx(d, i - 1, j)]) / d.dx + (by[pidx(d, i, j)] - by[pidx(d, i, j - 1)]) / d.dy;
+    acc += d.dx * d.dy * v * v;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}

@@ -1,0 +1,184 @@
+/*
+ * imex.h — C ABI of the MI355X-native IMEX-MHD per-step hot path.
+ *
+ * The reference solver (ZefengChen-Kairos/IMEX) exposes its hot path as free
+ * C++ functions on AoS `Grid`/`Array` containers driven by mutable globals
+ * (include/declarations.hpp, include/config.hpp).  This header replaces that
+ * surface with an opaque context that owns every device buffer, scalar and
+ * the HIP stream.  Host arrays crossing the boundary use the reference's own
+ * layout so a caller can hand over its `Grid::data` / `Array::data` unchanged:
+ *
+ *   Grid  (include/types.hpp:37-63): double[(ny+6)*(nx+6)*8],
+ *         element (i,j,k) at 8*(i + j*(nx+6)) + k, 3 ghost layers.
+ *   Array (include/types.hpp:65-91): double[(ny+6)*(nx+6)], (i,j) at i + j*(nx+6).
+ *   Krylov vectors (M_P / M_B, src/implicit_operators.cpp:5-139):
+ *         compact, (i-3) + nx*(j-3) + c*nx*ny.
+ *
+ * Host pointers are borrowed for the duration of the call only.  Device
+ * buffers stay resident between calls; the stepping entry points never
+ * allocate on the device after imex_create.
+ *
+ * Return codes: 0 = ok; > 0 = warning bitmask (IMEX_WARN_*); < 0 = error
+ * (IMEX_ERR_*).  One context per HIP stream; contexts share no state.
+ */
+#ifndef IMEX_H
+#define IMEX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enumerations (values follow the reference enums in include/config.hpp:27-41) ---- */
+enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_PERIODIC = 2 };           /* x/y_boundary_condition */
+enum { IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_WENO5 = 4 };  /* Reconstruction */
+enum { IMEX_EOS_IDEAL = 0, IMEX_EOS_RK = 1 };                       /* EOSType (config.hpp:118) */
+enum { IMEX_CT_CONSISTENT = 0, IMEX_CT_PLAIN = 1 };                 /* Staggered_CT_Consistent / Staggered_CT */
+enum { IMEX_PREC_FP64 = 0, IMEX_PREC_MIXED = 1 };
+enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2 };
+enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2 };
+
+enum {
+  IMEX_WARN_GMRES_MAXOUTER = 1,   /* src/implicit_operators.cpp:918 */
+  IMEX_WARN_NEWTON_MAXITER = 2,   /* src/implicit_operators.cpp:720-721 */
+  IMEX_WARN_POSITIVITY     = 4,   /* src/implicit_operators.cpp:734-736 */
+  IMEX_WARN_EOS_MAXITER    = 8    /* RK temperature Newton hit its cap */
+};
+enum {
+  IMEX_ERR_ARG = -1, IMEX_ERR_HIP = -2, IMEX_ERR_NONFINITE = -3,
+  IMEX_ERR_UNSUPPORTED = -4, IMEX_ERR_NOMEM = -5
+};
+
+/* Run-time replacement of the compile-time globals of include/config.hpp:43-125. */
+typedef struct imex_params {
+  int nx, ny;                     /* interior cells (config.hpp:52-53) */
+  double x0, x1, y0, y1;          /* domain (main.cpp:9-224) */
+  double gamma;                   /* gamma_plasma (config.hpp:122) */
+  int bc_x, bc_y;                 /* IMEX_BC_* */
+  int recon;                      /* IMEX_RECON_* (config.hpp:60) */
+  int minmod_central_y;           /* 1 = the vortex test's central y-slope (reconstruction.cpp:70-72) */
+  int eos;                        /* IMEX_EOS_* */
+  double rk_alfa, rk_b, rk_R, rk_cv;  /* Redlich-Kwong constants (config.hpp:125) */
+  double cfl;                     /* CFL (config.hpp:57) */
+  int floor_on;                   /* config.hpp:100 */
+  double p_floor, theta, mbig_fac;/* config.hpp:101-103 */
+  int newton_max;                 /* Pnewton_max (config.hpp:106) */
+  double newton_tol;              /* Pnewton_tol (config.hpp:107) */
+  double gmres_tol;               /* GMREStol (config.hpp:71) */
+  int gmres_restart;              /* JMAX (implicit_operators.cpp:801) */
+  int gmres_max_outer;            /* outmax; <= 0 means dim (implicit_operators.cpp:802) */
+  double shock_kappa1;            /* config.hpp:77 */
+  double time_stop;               /* dt = min(dt, time_stop - t) (main.cpp:341) */
+  int ct_mode;                    /* IMEX_CT_* */
+  int precision;                  /* IMEX_PREC_* */
+  double mixed_inner_tol;         /* relative tolerance of the FP32 inner solve (mixed form) */
+} imex_params;
+
+/* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */
+typedef struct imex_scalars {
+  double t, dt, adt;
+  double a_max, B_max_x, B_max_y;
+  double vx_max, vy_max, v_max;
+  double ch, ch_x, ch_y;
+  double prev_dt;                 /* the function-static of caldt (timestep.cpp:58) */
+  int timestep;
+  int rks;                        /* current IMEX stage */
+} imex_scalars;
+
+/* Per-stage working set, host side, in the reference layout (Grid / Array). */
+typedef struct imex_stage_host {
+  double *u_con, *Qe, *Qs;        /* Grid */
+  double *bx, *by;                /* Bx_p0, By_0p (Array) */
+  double *bx_qs, *by_qs;          /* Bx_p0_Qs, By_0p_Qs (Array) */
+  double *fx, *fy;                /* gFluxBy_p0, gFluxBx_0p (Array) */
+} imex_stage_host;
+
+typedef struct imex_stats {
+  long long steps;
+  long long mag_solves, mag_gmres_iters, mag_gmres_restarts;
+  long long p_solves, p_gmres_iters, p_gmres_restarts;
+  long long newton_iters;
+  long long op_applies_mag, op_applies_p;
+  long long mixed_refinements;
+  int warnings;                   /* OR of IMEX_WARN_* */
+  int max_floored;                /* g_max_floored (config.hpp:109) */
+  double min_p;                   /* g_min_p_ever (config.hpp:110) */
+  double div_b;                   /* cal_divergence_error after the last step */
+} imex_stats;
+
+typedef struct imex_solve_info {
+  int iters;                      /* Arnoldi steps (GMRES) or Newton iterations */
+  int restarts;
+  double res0, res;               /* first and last residual norm */
+  int status;
+} imex_solve_info;
+
+typedef struct imex_ctx imex_ctx;
+
+/* ---- parameters and initial data ---- */
+int imex_default_params(imex_params* p);
+/* Primitive initial data + staggered faces for one of the IMEX_CASE_* problems.
+ * Replaces u_initialization (src/initial_conditions.cpp:5-732; vortex :641-669,
+ * 698-717; Orszag-Tang :426-435; blast :436-449; faces :718-725).  `amp`,
+ * `u0`, `v0` scale the vortex (amplitude, background velocity); `smooth` selects
+ * the erf-smoothed blast profile.  Outputs are host arrays (Grid / Array). */
+int imex_init_case(const imex_params* p, int case_id, double amp, double u0, double v0,
+                   int smooth, double* u_pri, double* bx, double* by);
+
+/* ---- context ---- */
+int imex_create(const imex_params* p, imex_ctx** out);
+int imex_destroy(imex_ctx* ctx);
+const char* imex_last_error(void);
+/* Device-resident footprint of the context in bytes. */
+long long imex_device_bytes(const imex_ctx* ctx);
+
+/* ---- whole-state exchange (main.cpp:231-243, 561) ---- */
+int imex_set_state(imex_ctx* ctx, const double* u_pri, const double* bx, const double* by,
+                   double t, int timestep, double prev_dt);
+int imex_get_state(imex_ctx* ctx, double* u_con, double* u_pri, double* bx, double* by,
+                   imex_scalars* sc);
+
+/* ---- the step loop body (main.cpp:334-595 without I/O and diagnostics) ---- */
+int imex_step(imex_ctx* ctx, int nsteps, imex_stats* stats);
+/* Same as imex_step but leaves the stream running; imex_sync waits for it. */
+int imex_step_async(imex_ctx* ctx, int nsteps);
+int imex_sync(imex_ctx* ctx);
+int imex_get_stats(imex_ctx* ctx, imex_stats* stats);
+int imex_reset_stats(imex_ctx* ctx);
+double imex_div_b(imex_ctx* ctx);   /* cal_divergence_error, StaggerCT branch (imex_subsystems.cpp:630-660) */
+
+/* ---- stage-level drop-ins, one per reference entry point ---- */
+int imex_set_stage(imex_ctx* ctx, const imex_stage_host* st, const imex_scalars* sc);
+int imex_get_stage(imex_ctx* ctx, imex_stage_host* st, imex_scalars* sc);
+/* calch + caldt + MagneticEigenvalues on the current u_pri (timestep.cpp:5-67, 89-120, 254-273). */
+int imex_cfl(imex_ctx* ctx, imex_scalars* sc);
+/* Qe/Qs/faces_Qs from u_n and the stored stage increments (main.cpp:368-414). */
+int imex_stage_combine(imex_ctx* ctx, int rks);
+/* TVDReconstruction + RusanovC_Update + bound_cond_u (main.cpp:417-421). */
+int imex_explicit_conv(imex_ctx* ctx);
+/* MagSubsystem_UpdateB (imex_subsystems.cpp:537-628, incl. CollectMagFluxes). */
+int imex_mag_update_b(imex_ctx* ctx);
+/* B slots <- cell B~ then Pressure_flux_Update (main.cpp:448-477). */
+int imex_explicit_press(imex_ctx* ctx);
+/* PreSubsystem_Pressure (imex_subsystems.cpp:76-302, DefinitionofU). */
+int imex_pre_subsystem_pressure(imex_ctx* ctx);
+/* Staggered_CT_Consistent (ct_flux.cpp:150-191) or Staggered_CT (constrained_transport.cpp:66-143). */
+int imex_ct_update(imex_ctx* ctx);
+
+/* ---- operator level (tests, adjoint) on the current stage working set ---- */
+/* Magnetic_Subsystem_AX / Pressure_Subsystem_AX (implicit_operators.cpp:141-353, 505-616).
+ * For IMEX_OP_P_JACOBIAN, p_freeze (N) and mbig select V_slope. Host arrays. */
+int imex_apply_op(imex_ctx* ctx, int op, const double* x, double* Ax,
+                  const double* p_freeze, double mbig);
+/* Transposed operators for the discrete adjoint. */
+int imex_apply_op_T(imex_ctx* ctx, int op, const double* x, double* ATx,
+                    const double* p_freeze, double mbig);
+/* GMRES_Solver (implicit_operators.cpp:793-919). precond: 0 none, 1 Jacobi (precP). x in/out. */
+int imex_gmres(imex_ctx* ctx, int op, const double* b, double* x, int precond,
+               const double* p_freeze, double mbig, imex_solve_info* info);
+/* Pressure_NestedNewton (implicit_operators.cpp:625-738). p in/out. */
+int imex_pressure_newton(imex_ctx* ctx, const double* b, double* p, imex_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMEX_H */

@@ -1,0 +1,213 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only as the checker.  The product package imex_amd never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from imex_amd.capi import Params, Scalars, StageHost, Stats, SolveInfo, ptr, grid_shape, array_shape, \
+    struct_dict, stage_struct, new_stage
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    dp, vp = C.POINTER(C.c_double), C.c_void_p
+    PP, SP, STP = C.POINTER(Params), C.POINTER(Scalars), C.POINTER(StageHost)
+    sig = {
+        "orc_create": (vp, [PP, C.c_int]),
+        "orc_destroy": (None, [vp]),
+        "orc_set_state": (C.c_int, [vp, dp, dp, dp, C.c_double, C.c_int, C.c_double]),
+        "orc_step": (C.c_int, [vp, C.c_int]),
+        "orc_step_capture": (C.c_int, [vp, C.c_int, STP, SP]),
+        "orc_get_state": (C.c_int, [vp, dp, dp, dp, dp, SP]),
+        "orc_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
+        "orc_div_b": (C.c_double, [PP, dp, dp]),
+        "orc_explicit_conv": (C.c_int, [PP, SP, STP]),
+        "orc_mag_update_b": (C.c_int, [PP, SP, STP, C.POINTER(Stats)]),
+        "orc_explicit_press": (C.c_int, [PP, SP, STP]),
+        "orc_pre_subsystem_pressure": (C.c_int, [PP, SP, STP, C.POINTER(Stats)]),
+        "orc_ct_update": (C.c_int, [PP, SP, STP]),
+        "orc_reconstruct": (C.c_int, [PP, dp, dp, dp, dp, dp]),
+        "orc_apply_op": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, dp, C.c_double]),
+        "orc_gmres": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, C.c_int, dp, C.c_double, C.POINTER(SolveInfo)]),
+        "orc_pressure_newton": (C.c_int, [PP, SP, STP, dp, dp, C.POINTER(SolveInfo), dp]),
+        "orc_eos": (C.c_double, [PP, C.c_int, C.c_double, C.c_double]),
+        "orc_con2pri": (C.c_int, [PP, dp, dp, C.c_int]),
+        "orc_pri2con": (C.c_int, [PP, dp, dp, C.c_int]),
+        "orc_shock": (C.c_int, [PP, dp, dp]),
+        "orc_cfl": (C.c_int, [PP, dp, SP]),
+        "orc_bc": (C.c_int, [PP, dp, C.c_int]),
+        "orc_map_index": (C.c_int, [PP, C.c_int, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _sc(d):
+    s = Scalars()
+    for k, v in (d or {}).items():
+        setattr(s, k, v)
+    return s
+
+
+class Oracle:
+    """Whole-run oracle: the reference step loop on the CPU."""
+
+    def __init__(self, params, faithful=0):
+        self.p = params
+        self.L = lib()
+        self.h = self.L.orc_create(C.byref(params), faithful)
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, u_pri, bx, by, t=0.0, timestep=0, prev_dt=0.0):
+        return self.L.orc_set_state(self.h, ptr(np.ascontiguousarray(u_pri)), ptr(np.ascontiguousarray(bx)),
+                                    ptr(np.ascontiguousarray(by)), t, timestep, prev_dt)
+
+    def step(self, n=1):
+        return self.L.orc_step(self.h, n)
+
+    def step_capture(self, point):
+        st = new_stage(self.p)
+        sc = Scalars()
+        rc = self.L.orc_step_capture(self.h, point, C.byref(stage_struct(st)), C.byref(sc))
+        return st, struct_dict(sc), rc
+
+    def get_state(self):
+        p = self.p
+        u_con, u_pri = np.zeros(grid_shape(p)), np.zeros(grid_shape(p))
+        bx, by = np.zeros(array_shape(p)), np.zeros(array_shape(p))
+        sc = Scalars()
+        self.L.orc_get_state(self.h, ptr(u_con), ptr(u_pri), ptr(bx), ptr(by), C.byref(sc))
+        return u_con, u_pri, bx, by, struct_dict(sc)
+
+    def stats(self):
+        st = Stats()
+        self.L.orc_get_stats(self.h, C.byref(st))
+        return struct_dict(st)
+
+
+# ---- stage-level helpers on copies of a stage dict ----
+def _copy(st):
+    return {k: np.array(v, copy=True) for k, v in st.items()}
+
+
+def sub(p, sc, st, name):
+    st = _copy(st)
+    s = stage_struct(st)
+    L = lib()
+    if name in ("mag_update_b", "pre_subsystem_pressure"):
+        stats = Stats()
+        rc = getattr(L, "orc_" + name)(C.byref(p), C.byref(_sc(sc)), C.byref(s), C.byref(stats))
+        return st, rc, struct_dict(stats)
+    rc = getattr(L, "orc_" + name)(C.byref(p), C.byref(_sc(sc)), C.byref(s))
+    return st, rc, None
+
+
+def apply_op(p, sc, st, op, x, p_freeze=None, mbig=0.0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
+    lib().orc_apply_op(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), op, ptr(x), ptr(y), ptr(pf), mbig)
+    return y
+
+
+def gmres(p, sc, st, op, b, x0, precond=0, p_freeze=None, mbig=0.0):
+    x = np.array(x0, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
+    info = SolveInfo()
+    rc = lib().orc_gmres(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), op, ptr(b), ptr(x), precond, ptr(pf),
+                         mbig, C.byref(info))
+    return x, struct_dict(info), rc
+
+
+def pressure_newton(p, sc, st, b, p0):
+    pv = np.array(p0, dtype=np.float64, copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    info = SolveInfo()
+    mb = np.zeros(1)
+    rc = lib().orc_pressure_newton(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), ptr(b), ptr(pv),
+                                   C.byref(info), ptr(mb))
+    return pv, struct_dict(info), rc, float(mb[0])
+
+
+def eos(p, which, a, b):
+    return lib().orc_eos(C.byref(p), which, a, b)
+
+
+def con2pri(p, con):
+    con = np.ascontiguousarray(con, dtype=np.float64)
+    pri = np.zeros_like(con)
+    lib().orc_con2pri(C.byref(p), ptr(con), ptr(pri), con.size // 8)
+    return pri
+
+
+def pri2con(p, pri):
+    pri = np.ascontiguousarray(pri, dtype=np.float64)
+    con = np.zeros_like(pri)
+    lib().orc_pri2con(C.byref(p), ptr(pri), ptr(con), pri.size // 8)
+    return con
+
+
+def reconstruct(p, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = [np.zeros_like(u) for _ in range(4)]
+    lib().orc_reconstruct(C.byref(p), ptr(u), *[ptr(o) for o in out])
+    return out
+
+
+def shock(p, Qe):
+    eta = np.zeros(array_shape(p))
+    lib().orc_shock(C.byref(p), ptr(np.ascontiguousarray(Qe)), ptr(eta))
+    return eta
+
+
+def cfl(p, u_pri, sc=None):
+    s = _sc(sc)
+    lib().orc_cfl(C.byref(p), ptr(np.ascontiguousarray(u_pri)), C.byref(s))
+    return struct_dict(s)
+
+
+def bc(p, a, kind):
+    a = np.array(a, dtype=np.float64, copy=True)
+    lib().orc_bc(C.byref(p), ptr(a), kind)
+    return a
+
+
+def div_b(p, bx, by):
+    return lib().orc_div_b(C.byref(p), ptr(np.ascontiguousarray(bx)), ptr(np.ascontiguousarray(by)))
+
+
+def map_index(p, i, j):
+    return lib().orc_map_index(C.byref(p), i, j)
