@@ -1,0 +1,133 @@
+"""Throughput of the IMEX-MHD step on one MI355X.
+
+Workload: Orszag-Tang on periodic [0,2pi]^2 at 1024x1024, gamma 5/3, WENO5,
+StaggerCT (EMF-consistent), PressureFormula, floor_on with p_floor 1e-8,
+IMEX-RK2, CFL 0.8 — synthetic initial data generated on the host.
+
+A "step" is one full IMEX-RK2 step (two stages: explicit fluxes, magnetic GMRES,
+EMF collection, two nested-Newton pressure solves, CT) with all fields resident in
+HBM.  Prints one JSON line: Mcell-steps/s, the live per-kernel HIP-event roofline
+of the dominant kernel class and the FP64 CPU oracle (faithful mode, 1 core) timed
+on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(name, steps, warmup, precision):
+    import imex_amd as im
+    p, kw = im.case_of(name)
+    p.precision = precision
+    u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
+    s = im.Solver(p)
+    s.set_state(u, bx, by)
+    log(f"[bench] {name} warmup {warmup} steps")
+    if warmup:
+        s.step(warmup)
+    state = s.get_state()
+    s.reset_stats()
+    s.profile_enable(True)
+    s.sync()
+    log(f"[bench] timing {steps} steps")
+    t0 = time.perf_counter()
+    s.step_async(steps)
+    s.sync()
+    t1 = time.perf_counter()
+    prof = s.profile_read()
+    s.profile_enable(False)
+    stats = s.stats()
+    uc, up, bxf, byf, sc = s.get_state()
+    s.close()
+    elapsed = t1 - t0
+    return dict(p=p, elapsed=elapsed, prof=prof, stats=stats, state=state, final=(uc, sc))
+
+
+def cpu_baseline(name, state, budget_steps=1):
+    """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
+    import imex_amd as im
+    import oracle as orc
+    p, kw = im.case_of(name)
+    uc, up, bx, by, sc = state
+    o = orc.Oracle(p, 1)
+    o.set_state(up, bx, by, sc["t"], sc["timestep"], sc["prev_dt"])
+    t0 = time.perf_counter()
+    o.step(budget_steps)
+    t1 = time.perf_counter()
+    return p.nx * p.ny * budget_steps / (t1 - t0) / 1e6, t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ot1024")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU domain decomposition is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import imex_amd as im
+    prec = im.PREC_FP64 if args.precision == "fp64" else im.PREC_MIXED
+    r = run_gpu(args.workload, args.steps, args.warmup, prec)
+    p = r["p"]
+    cells = p.nx * p.ny
+    value = cells * args.steps / r["elapsed"] / 1e6
+    prof = [k for k in r["prof"] if k["launches"] > 0]
+    tot_ms = sum(k["total_ms"] for k in prof)
+    work = [k for k in prof if k["name"] not in ("early_exit", "ghost_fill", "misc")]
+    dom = max(work, key=lambda k: k["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    for k in sorted(prof, key=lambda k: -k["total_ms"]):
+        gbs = k["bytes"] / (k["total_ms"] * 1e-3) / 1e9 if k["total_ms"] > 0 else 0.0
+        log(f"  {k['name']:22s} n={k['launches']:6d} total {k['total_ms']:9.3f} ms  avg {k['total_ms'] / k['launches'] * 1e3:8.1f} us"
+            f"  {gbs:8.1f} GB/s")
+    log(f"[bench] kernel time {tot_ms:.1f} ms of {r['elapsed'] * 1e3:.1f} ms wall; stats {r['stats']}")
+    out = {
+        "metric": "Mcell-steps/s at 1024^2 WENO5 Orszag-Tang (IMEX-RK2 step, one MI355X)",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == im.PREC_FP64 else "f32-operators/f64-krylov",
+        "data": "synthetic (Orszag-Tang initial data generated on the host)",
+        "config": {"workload": f"orszag_tang_{p.nx}x{p.ny}_weno5_staggerct_rk2", "nx": p.nx, "ny": p.ny,
+                   "recon": "WENO5", "p_floor": p.p_floor, "gamma": p.gamma, "precision": args.precision},
+        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
+        "gmres": {k: r["stats"][k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters")},
+    }
+    if not args.no_cpu:
+        log("[bench] CPU oracle sample")
+        v, secs = cpu_baseline(args.workload, r["state"], args.cpu_steps)
+        out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang from the post-warmup "
+                                         f"state, faithful mode, {secs:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,6 +58,10 @@ class Stats(C.Structure):
                [("warnings", C.c_int), ("max_floored", C.c_int), ("min_p", C.c_double), ("div_b", C.c_double)]
 
 
+class KernelProfile(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("launches", C.c_longlong), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("res0", C.c_double), ("res", C.c_double),
                 ("status", C.c_int)]
@@ -167,6 +171,8 @@ def lib():
         "imex_apply_op_T": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
         "imex_gmres": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, dp, C.c_double, C.POINTER(SolveInfo)]),
         "imex_pressure_newton": (C.c_int, [vp, dp, dp, C.POINTER(SolveInfo)]),
+        "imex_profile_enable": (C.c_int, [vp, C.c_int]),
+        "imex_profile_read": (C.c_int, [vp, C.POINTER(KernelProfile), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -279,6 +285,15 @@ class Solver:
 
     def div_b(self):
         return self.L.imex_div_b(self.h)
+
+    def profile_enable(self, on=True):
+        return self._chk(self.L.imex_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_read(self):
+        arr = (KernelProfile * 64)()
+        n = self._chk(self.L.imex_profile_read(self.h, arr, 64), "profile_read")
+        return [dict(name=arr[i].name.decode(), launches=arr[i].launches, total_ms=arr[i].total_ms,
+                     bytes=arr[i].bytes) for i in range(n)]
 
     def set_stage(self, st, sc):
         s = Scalars()

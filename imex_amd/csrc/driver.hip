@@ -31,17 +31,33 @@ __global__ void k_visc_rk(Dims d, Phys P, const double* pe, const double* u, dou
   visc[c] = Vbase_slope(P, pe[c], u[c], &ef);
   if (ef) atomicOr(eflag, 1);
 }
-__global__ void k_pstats(long long n, const double* p, double pfloor, double* out) {
-  // out[0] = min p, out[1] = count(p < pfloor); single block
+// Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
+// device: acc[0] = min p ever, acc[1] = max floored count, acc[2] = 1 if min p < theta p_floor - 1e-14.
+__global__ void k_pstats(long long n, const double* p, double pfloor, double thr, double* partial, unsigned* counter,
+                         double* acc) {
   __shared__ double smin[4], scnt[4];
   double mn = 1e300, cnt = 0;
-  for (long long e = threadIdx.x; e < n; e += blockDim.x) { double v = p[e]; mn = fmin(mn, v); if (v < pfloor) cnt += 1; }
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    double v = p[e];
+    mn = fmin(mn, v);
+    if (v < pfloor) cnt += 1;
+  }
   mn = wave_min(mn); cnt = wave_sum(cnt);
   if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = mn; scnt[threadIdx.x >> 6] = cnt; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mn = fmin(smin[0], smin[w]); smin[0] = mn; scnt[0] += scnt[w]; }
-    out[0] = smin[0]; out[1] = scnt[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mn = fmin(mn, smin[w]); cnt += scnt[w]; }
+    partial[2 * blockIdx.x] = smin[0] < mn ? smin[0] : mn;
+    partial[2 * blockIdx.x + 1] = cnt;
+  }
+  if (!last_block_arrive(counter)) return;
+  if (threadIdx.x == 0) {
+    *counter = 0;
+    double m = 1e300, c = 0;
+    for (int b = 0; b < (int)gridDim.x; b++) { m = fmin(m, partial[2 * b]); c += partial[2 * b + 1]; }
+    acc[0] = fmin(acc[0], m);
+    acc[1] = fmax(acc[1], c);
+    if (m < thr) acc[2] = 1.0;
   }
 }
 
@@ -50,7 +66,35 @@ struct StageBind {
   double *u, *Qe, *Qs, *bxqs, *byqs, *bxo, *byo;
 };
 
+
+// ---------------------------------------------------------------- per-kernel-class event profiler
+// When enabled, every launch on the step path is bracketed by two HIP events on the
+// context stream and tagged with its kernel class and algorithmic byte count (distinct
+// per-cell fields read once plus fields written once, times cells, times 8 bytes).
+enum KClass {
+  KC_START, KC_COMBINE, KC_QEPREP, KC_SHOCK, KC_EXPL_CONV, KC_EXPL_PRESS, KC_MAG_RHS, KC_MAG_APPLY, KC_MAG_UPDATE,
+  KC_MAG_RES, KC_MAG_XUPD, KC_P_APPLY, KC_P_UPDATE, KC_P_RES, KC_P_XUPD, KC_NEWTON_RES, KC_EMF, KC_P_RHS, KC_P_MOM,
+  KC_P_ENERGY, KC_CT, KC_BC, KC_MISC, KC_EARLY_EXIT, KC_N
+};
+static const char* kc_names[KC_N] = {
+  "step_start", "stage_combine", "qe_prep", "shock", "explicit_conv", "explicit_press", "mag_rhs", "mag_arnoldi_apply",
+  "mag_arnoldi_update", "mag_gmres_residual", "mag_gmres_xupdate", "p_arnoldi_apply", "p_arnoldi_update",
+  "p_gmres_residual", "p_gmres_xupdate", "newton_residual", "emf_flux", "p_rhs", "p_momentum", "p_energy", "ct",
+  "ghost_fill", "misc", "early_exit"};
+
+struct ProfRec { int cls; int e0, e1; double bytes; };
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  int next = 0;
+  std::vector<ProfRec> recs;
+  long long gen = 0;          // bumped whenever recs are flushed
+  double ms[KC_N] = {0}, bytes[KC_N] = {0};
+  long long n[KC_N] = {0};
+};
+
 struct imex_ctx {
+  Prof prof;
   imex_params prm;
   Dims d;
   Phys P;
@@ -72,6 +116,8 @@ struct imex_ctx {
   double* Sh = nullptr;        // pinned host copy of S
   int* eflag = nullptr;
   unsigned* cnt_start = nullptr;
+  unsigned* cnt_pstats = nullptr;
+  KrylovDev Klast;             // status of the last GMRES cycle read by the host
   hipEvent_t ev[2];
   int state_mode = 0;          // 0: u_pri holds the state; 1: U2 holds the final conservative state
   StageBind cur;
@@ -103,6 +149,39 @@ static Phys make_phys(const imex_params& p) {
   return P;
 }
 
+
+static void prof_collect(imex_ctx* c) {
+  Prof& P = c->prof;
+  if (P.recs.empty()) { P.next = 0; return; }
+  (void)hipStreamSynchronize(c->st);
+  for (const ProfRec& r : P.recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, P.ev[r.e0], P.ev[r.e1]) != hipSuccess) continue;
+    P.ms[r.cls] += ms; P.bytes[r.cls] += r.bytes; P.n[r.cls] += 1;
+  }
+  P.recs.clear();
+  P.next = 0;
+  P.gen++;
+}
+static int prof_begin(imex_ctx* c) {
+  Prof& P = c->prof;
+  if (!P.on) return -1;
+  if (P.next + 2 > 65536) prof_collect(c);
+  while ((int)P.ev.size() < P.next + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) return -1; P.ev.push_back(e); }
+  int e0 = P.next;
+  P.next += 2;
+  (void)hipEventRecord(P.ev[e0], c->st);
+  return e0;
+}
+static int prof_end(imex_ctx* c, int e0, int cls, double fields) {
+  if (e0 < 0) return -1;
+  Prof& P = c->prof;
+  (void)hipEventRecord(P.ev[e0 + 1], c->st);
+  P.recs.push_back(ProfRec{cls, e0, e0 + 1, fields * (double)c->d.N * 8.0});
+  return (int)P.recs.size() - 1;
+}
+#define LAUNCH(cls, fields, stmt) do { int e0_ = prof_begin(c); stmt; prof_end(c, e0_, (cls), (double)(fields)); } while (0)
+
 template <class T>
 static int dalloc(imex_ctx* c, T** out, size_t count) {
   void* ptr = nullptr;
@@ -120,7 +199,9 @@ static inline dim3 grid2(int nx, int ny, int bx = 64, int by = 4) { return dim3(
 static void fill_planes(imex_ctx* c, std::initializer_list<double*> planes, int kind = 0) {
   PlaneSet ps; ps.n = 0;
   for (double* p : planes) ps.p[ps.n++] = p;
+  int e0 = prof_begin(c);
   launch_bc_fill(c->d, ps, kind, c->st);
+  prof_end(c, e0, KC_BC, 0.0);
 }
 
 static MagOp mag_op(imex_ctx* c) {
@@ -138,7 +219,7 @@ static POp p_op(imex_ctx* c, int jacobian, int precond) {
 // GMRES_Solver (implicit_operators.cpp:793-919).  Arnoldi steps are queued in chunks; the
 // convergence flag of chunk k is read (async copy + event) while chunk k+1 is already queued.
 static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b, int x_is_zero,
-                     imex_solve_info* info, long long* it_acc, long long* rs_acc) {
+                     imex_solve_info* info, long long* it_acc, long long* rs_acc, const int* skip = nullptr) {
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
   const long long dim = (long long)nc * c->d.N;
@@ -152,13 +233,24 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   long long outer = 0;
   double res0 = 0, res = 0;
   for (outer = 0; outer < outmax; outer++) {
-    gmres_residual(op, mo, po, c->kb, x, b, c->K, (int)outer, (x_is_zero && outer == 0) ? 1 : 0, c->st);
+    const int xz = (x_is_zero && outer == 0) ? 1 : 0;
+    const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
+    LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(op, mo, po, c->kb, x, b, c->K, (int)outer, xz, outer == 0 ? skip : nullptr, c->st));
+    const long long gen0 = c->prof.gen;
+    int recs_a[MAXM], recs_u[MAXM];
     int j = 0, slot = 0, last = -1;
     bool stop = false;
     while (!stop) {
       if (j >= m) break;                       // whole cycle queued; the last chunk decides
       int jend = std::min(m, j + (j == 0 ? std::max(chunk, c->pred[op]) : chunk));
-      for (; j < jend; j++) arnoldi_step(op, mo, po, c->kb, c->V, j, m, c->K, c->st);
+      for (; j < jend; j++) {
+        int e0 = prof_begin(c);
+        arnoldi_apply(op, mo, po, c->kb, c->V, j, c->K, c->st);
+        recs_a[j] = prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY, op == 0 ? 13 + 3 * j : 6 + j + po.jacobian);
+        e0 = prof_begin(c);
+        arnoldi_update(nc, c->d, c->kb, c->V, j, m, c->K, c->st);
+        recs_u[j] = prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, op == 0 ? 9 + 3 * j : 3 + j);
+      }
       HIPCHK(hipMemcpyAsync(&c->Kh[slot].tol, &c->K->tol, sizeof(KrylovDev) - offsetof(KrylovDev, tol), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipEventRecord(c->ev[slot], c->st));
       int prev = last;
@@ -172,8 +264,19 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
     int pending = last;
     HIPCHK(hipEventSynchronize(c->ev[pending]));
     KrylovDev& ks = c->Kh[pending];
-    gmres_xupdate(nc, c->d, c->kb, c->V, x, c->K, c->st);
+    int rx;
+    {
+      int e0 = prof_begin(c);
+      gmres_xupdate(nc, c->d, c->kb, c->V, x, c->K, c->st);
+      rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (ks.jlast + 3));
+    }
+    if (c->prof.on && c->prof.gen == gen0) {      // launches queued past convergence exited at once
+      auto idle = [&](int r) { if (r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
+      for (int jj = ks.steps; jj < j; jj++) { idle(recs_a[jj]); idle(recs_u[jj]); }
+      if (ks.jlast < 0) idle(rx);
+    }
     total += ks.steps;
+    c->Klast = ks;
     if (outer == 0) res0 = ks.res0;
     res = ks.res;
     if (ks.converged) { converged = 1; break; }
@@ -197,35 +300,34 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
     c->stats.p_solves++;
     return rc;
   }
-  if (c->prm.floor_on) newton_mbig(d, c->P, b, c->cur.u, c->S, c->partial, c->K, c->eflag, c->st);
+  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(d, c->P, b, c->cur.u, c->S, c->partial, c->K, c->eflag, c->st));
   int it;
   double r0 = 0, r = 0;
+  const int* conv = &c->K->newton_conv;
   for (it = 0; it < c->prm.newton_max; it++) {
+    // residual, then the Jacobian solve; the solve is skipped on the device when the residual
+    // has converged, and the host learns it from the GMRES status it polls anyway
     POp carrier = p_op(c, 0, 0);
-    newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st);
-    HIPCHK(hipMemcpyAsync(&c->Kh[0].tol, &c->K->tol, sizeof(KrylovDev) - offsetof(KrylovDev, tol), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (it == 0) r0 = c->Kh[0].newton_res0;
-    r = c->Kh[0].newton_res;
-    if (c->Kh[0].newton_conv) break;
+    LAUNCH(KC_NEWTON_RES, 6, newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st));
     if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
-    k_fill<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0);
+    LAUNCH(KC_MISC, 1, (k_fill<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0)));
     POp jac = p_op(c, 1, 1);
     MagOp mo = mag_op(c);
-    rc |= run_gmres(c, 1, mo, jac, c->dp, c->ng, 1, nullptr, &c->stats.p_gmres_iters, &c->stats.p_gmres_restarts);
+    long long its = 0, rss = 0;
+    rc |= run_gmres(c, 1, mo, jac, c->dp, c->ng, 1, nullptr, &its, &rss, conv);
+    if (it == 0) r0 = c->Klast.newton_res0;
+    r = c->Klast.newton_res;
+    if (c->Klast.newton_conv) { rc &= ~IMEX_WARN_GMRES_MAXOUTER; break; }
+    c->stats.p_gmres_iters += its; c->stats.p_gmres_restarts += rss;
     c->stats.p_solves++;
-    axpy_launch(d.N, p, c->dp, c->st);
+    LAUNCH(KC_MISC, 3, axpy_launch(d.N, p, c->dp, conv, c->st));
   }
   c->stats.newton_iters += it;
   if (it == c->prm.newton_max) { rc |= IMEX_WARN_NEWTON_MAXITER; c->stats.warnings |= IMEX_WARN_NEWTON_MAXITER; }
   if (c->prm.floor_on) {
-    k_pstats<<<1, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->pst);
-    double h2[2];
-    HIPCHK(hipMemcpyAsync(h2, c->pst, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if ((int)h2[1] > c->stats.max_floored) c->stats.max_floored = (int)h2[1];
-    if (h2[0] < c->stats.min_p) c->stats.min_p = h2[0];
-    if (h2[0] < c->prm.theta * c->prm.p_floor - 1e-14) { rc |= IMEX_WARN_POSITIVITY; c->stats.warnings |= IMEX_WARN_POSITIVITY; }
+    long long g = std::min<long long>(nb(d.N), 256);
+    LAUNCH(KC_MISC, 1, (k_pstats<<<(unsigned)g, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->prm.theta * c->prm.p_floor - 1e-14,
+                                                                  c->partial, c->cnt_pstats, c->pst)));
   }
   if (info) { info->iters = it; info->res0 = r0; info->res = r; info->status = rc; info->restarts = 0; }
   return rc;
@@ -236,17 +338,17 @@ static void ensure_qe(imex_ctx* c) {
   if (c->qe_ready) return;
   const Dims& d = c->d;
   long long tot = (long long)d.NX * d.NY;
-  k_qe_prep<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->cur.Qe, c->pe, c->cf, c->eflag);
-  k_shock_chi<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.Qe, c->cf, c->chi);
+  LAUNCH(KC_QEPREP, 10, (k_qe_prep<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->cur.Qe, c->pe, c->cf, c->eflag)));
+  LAUNCH(KC_SHOCK, 5, (k_shock_chi<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.Qe, c->cf, c->chi)));
   fill_planes(c, {c->chi});
-  k_shock_eta<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->chi, c->eta);
+  LAUNCH(KC_SHOCK, 2, (k_shock_eta<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->chi, c->eta)));
   fill_planes(c, {c->eta});
   c->qe_ready = true;
 }
 
 static int sub_explicit_conv(imex_ctx* c) {
   const Dims& d = c->d;
-  k_explicit<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->S);
+  LAUNCH(KC_EXPL_CONV, 13, (k_explicit<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
   double* u = c->cur.u;
   fill_planes(c, {u, u + d.PS, u + 2 * d.PS, u + 3 * d.PS});
   c->hhat_ready = false;
@@ -256,7 +358,7 @@ static int sub_explicit_conv(imex_ctx* c) {
 static int sub_mag_update_b(imex_ctx* c) {
   const Dims& d = c->d;
   ensure_qe(c);
-  k_mag_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, c->pe, c->bk, c->xk, c->hhat, c->S, c->eflag);
+  LAUNCH(KC_MAG_RHS, 23, (k_mag_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, c->pe, c->bk, c->xk, c->hhat, c->S, c->eflag)));
   fill_planes(c, {c->hhat});
   c->hhat_ready = true;
   MagOp mo = mag_op(c);
@@ -264,14 +366,14 @@ static int sub_mag_update_b(imex_ctx* c) {
   int rc = run_gmres(c, 0, mo, po, c->xk, c->bk, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
   c->stats.mag_solves++;
   long long tot = (long long)d.NX * d.NY;
-  k_mag_writeback<<<nb(tot), 256, 0, c->st>>>(d, c->xk, c->cur.u);
-  k_emf_flux<<<grid2(d.NX, d.NY, 64, 4), dim3(64, 4), 0, c->st>>>(d, c->cur.u, c->cur.Qe, c->pe, c->fx, c->fy, c->S);
+  LAUNCH(KC_MISC, 6, (k_mag_writeback<<<nb(tot), 256, 0, c->st>>>(d, c->xk, c->cur.u)));
+  LAUNCH(KC_EMF, 12, (k_emf_flux<<<grid2(d.NX, d.NY, 64, 4), dim3(64, 4), 0, c->st>>>(d, c->cur.u, c->cur.Qe, c->pe, c->fx, c->fy, c->S)));
   return rc;
 }
 
 static int sub_explicit_press(imex_ctx* c) {
   const Dims& d = c->d;
-  k_explicit<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->S);
+  LAUNCH(KC_EXPL_PRESS, 16, (k_explicit<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
   double* u = c->cur.u;
   fill_planes(c, {u + d.PS, u + 2 * d.PS, u + 3 * d.PS, u + 4 * d.PS});
   return 0;
@@ -297,14 +399,14 @@ static int sub_pressure(imex_ctx* c) {
     long long tot = (long long)d.NX * d.NY;
     k_visc_rk<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->pe, u, c->visc, c->eflag);
   }
-  k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 0, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S);
+  LAUNCH(KC_P_RHS, 15, (k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 0, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S)));
   fill_planes(c, {c->mxs, c->mys});
   int rc = run_newton(c, c->pb, c->p, nullptr);
-  k_p_momentum<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->p, c->mxs, c->mys, u, c->S);
+  LAUNCH(KC_P_MOM, 5, (k_p_momentum<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->p, c->mxs, c->mys, u, c->S)));
   fill_planes(c, {u + d.PS, u + 2 * d.PS});
-  k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 1, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S);
+  LAUNCH(KC_P_RHS, 12, (k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 1, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S)));
   rc |= run_newton(c, c->pb, c->p, nullptr);
-  k_p_energy<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->p, u, c->eflag);
+  LAUNCH(KC_P_ENERGY, 9, (k_p_energy<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->p, u, c->eflag)));
   fill_planes(c, {u + 4 * d.PS});
   return rc;
 }
@@ -317,8 +419,8 @@ static int sub_ct(imex_ctx* c) {
     k_ct_plain_emf<<<grid2(d.NX, d.NY), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->ezx, c->ezy);
     ex = c->ezx; ey = c->ezy; sx = -1.0; sy = 1.0;
   }
-  k_ct_corner<<<grid2(d.nx + 2, d.ny + 2), dim3(64, 4), 0, c->st>>>(d, ex, ey, sx, sy, c->cur.u, c->ezpp);
-  k_ct_faces<<<grid2(d.NX, d.NY), dim3(64, 4), 0, c->st>>>(d, c->cur.bxqs, c->cur.byqs, c->ezpp, c->cur.bxo, c->cur.byo, c->cur.u, c->S);
+  LAUNCH(KC_CT, 8, (k_ct_corner<<<grid2(d.nx + 2, d.ny + 2), dim3(64, 4), 0, c->st>>>(d, ex, ey, sx, sy, c->cur.u, c->ezpp)));
+  LAUNCH(KC_CT, 7, (k_ct_faces<<<grid2(d.NX, d.NY), dim3(64, 4), 0, c->st>>>(d, c->cur.bxqs, c->cur.byqs, c->ezpp, c->cur.bxo, c->cur.byo, c->cur.u, c->S)));
   return 0;
 }
 
@@ -348,7 +450,7 @@ static int step_start(imex_ctx* c) {
   long long tot = (long long)d.NX * d.NY;
   long long g = std::min<long long>(nb(tot), c->nblk_start);
   const double* in = c->state_mode ? c->U2 : c->u_pri;
-  k_step_start<<<(unsigned)g, 256, 0, c->st>>>(d, c->P, c->state_mode, in, c->u_n, c->partial, c->cnt_start, c->S, c->eflag);
+  LAUNCH(KC_START, 16, (k_step_start<<<(unsigned)g, 256, 0, c->st>>>(d, c->P, c->state_mode, in, c->u_n, c->partial, c->cnt_start, c->S, c->eflag)));
   return 0;
 }
 
@@ -359,7 +461,7 @@ static int one_step(imex_ctx* c) {
   bind_stage0(c);
   rc |= run_stage(c);
   long long tot = (long long)d.NX * d.NY;
-  k_stage_combine<<<nb(tot), 256, 0, c->st>>>(d, c->u_n, c->U1, c->Qe, c->Qs, c->bx_n, c->bx1, c->by_n, c->by1, c->bxqs, c->byqs, c->S);
+  LAUNCH(KC_COMBINE, 38, (k_stage_combine<<<nb(tot), 256, 0, c->st>>>(d, c->u_n, c->U1, c->Qe, c->Qs, c->bx_n, c->bx1, c->by_n, c->by1, c->bxqs, c->byqs, c->S)));
   bind_stage1(c);
   rc |= run_stage(c);
   std::swap(c->bx_n, c->bx2); std::swap(c->by_n, c->by2);
@@ -591,6 +693,7 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
   if (!rc) rc = dalloc(c, &c->S, S_NSLOTS);
   if (!rc) rc = dalloc(c, &c->eflag, 1);
   if (!rc) rc = dalloc(c, &c->cnt_start, 1);
+  if (!rc) rc = dalloc(c, &c->cnt_pstats, 1);
   if (rc) { imex_destroy(c); return rc; }
   c->kb.w = c->w; c->kb.Av = c->Av; c->kb.partial = c->partial;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "stream"); }
@@ -605,10 +708,12 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
   (void)hipMemcpy(c->S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice);
   (void)hipMemset(c->eflag, 0, sizeof(int));
   (void)hipMemset(c->cnt_start, 0, sizeof(unsigned));
+  (void)hipMemset(c->cnt_pstats, 0, sizeof(unsigned));
   for (double** pp : planes) (void)hipMemset(*pp, 0, G1 * sizeof(double));
   for (double* g8 : {c->U1, c->U2, c->Qe, c->Qs, c->u_n, c->u_pri}) (void)hipMemset(g8, 0, G8 * sizeof(double));
   if (hipDeviceSynchronize() != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "init sync"); }
   c->stats.min_p = 1e300;
+  { double init[4] = {1e300, 0.0, 0.0, 0.0}; (void)hipMemcpy(c->pst, init, sizeof(init), hipMemcpyHostToDevice); }
   bind_stage0(c);
   *out = c;
   return 0;
@@ -620,6 +725,7 @@ int imex_destroy(imex_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->Kh) (void)hipHostFree(c->Kh);
   if (c->Sh) (void)hipHostFree(c->Sh);
+  for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->st) { (void)hipEventDestroy(c->ev[0]); (void)hipEventDestroy(c->ev[1]); (void)hipStreamDestroy(c->st); }
   delete c;
   return 0;
@@ -687,6 +793,11 @@ int imex_get_stats(imex_ctx* c, imex_stats* s) {
   int ef = 0;
   HIPCHK(hipMemcpy(&ef, c->eflag, sizeof(int), hipMemcpyDeviceToHost));
   if (ef) c->stats.warnings |= IMEX_WARN_EOS_MAXITER;
+  double acc[4];
+  HIPCHK(hipMemcpy(acc, c->pst, sizeof(acc), hipMemcpyDeviceToHost));
+  c->stats.min_p = acc[0];
+  c->stats.max_floored = (int)acc[1];
+  if (acc[2] != 0.0) c->stats.warnings |= IMEX_WARN_POSITIVITY;
   *s = c->stats;
   s->div_b = imex_div_b(c);
   return 0;
@@ -694,8 +805,13 @@ int imex_get_stats(imex_ctx* c, imex_stats* s) {
 
 int imex_reset_stats(imex_ctx* c) {
   if (!c) return IMEX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->st));
   c->stats = imex_stats{};
   c->stats.min_p = 1e300;
+  double init[4] = {1e300, 0.0, 0.0, 0.0};
+  HIPCHK(hipMemcpy(c->pst, init, sizeof(init), hipMemcpyHostToDevice));
+  int zero = 0;
+  HIPCHK(hipMemcpy(c->eflag, &zero, sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -867,6 +983,28 @@ int imex_pressure_newton(imex_ctx* c, const double* b, double* p, imex_solve_inf
   if (r2 < 0) return r2;
   HIPCHK(hipMemcpy(p, c->p, d.N * sizeof(double), hipMemcpyDeviceToHost));
   return rc;
+}
+
+int imex_profile_enable(imex_ctx* c, int on) {
+  if (!c) return IMEX_ERR_ARG;
+  prof_collect(c);
+  Prof& P = c->prof;
+  P.on = on != 0;
+  for (int k = 0; k < KC_N; k++) { P.ms[k] = 0; P.bytes[k] = 0; P.n[k] = 0; }
+  return 0;
+}
+
+int imex_profile_read(imex_ctx* c, imex_kernel_profile* out, int max) {
+  if (!c) return IMEX_ERR_ARG;
+  prof_collect(c);
+  int n = 0;
+  for (int k = 0; k < KC_N && n < max; k++, n++) {
+    out[n].name = kc_names[k];
+    out[n].launches = c->prof.n[k];
+    out[n].total_ms = c->prof.ms[k];
+    out[n].bytes = c->prof.bytes[k];
+  }
+  return n;
 }
 
 }  // extern "C"

@@ -75,9 +75,11 @@ struct KrylovBufs {
 
 // op = 0 magnetic (3 unknowns per cell), 1 pressure (1 unknown per cell)
 void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
-                    KrylovDev* K, int outer, int x_is_zero, hipStream_t st);
-void arnoldi_step(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j, int m,
-                  KrylovDev* K, hipStream_t st);
+                    KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st);
+void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
+                   KrylovDev* K, hipStream_t st);
+void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
+                    hipStream_t st);
 void gmres_xupdate(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, double* x, KrylovDev* K, hipStream_t st);
 void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* Ax, hipStream_t st);
 void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
@@ -86,4 +88,4 @@ void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u,
                  int* eflag, hipStream_t st);
 void vslope_launch(const Dims& d, const Phys& P, const double* pf, const double* u, const double* S, double* vs, int* eflag,
                    hipStream_t st);
-void axpy_launch(long long n, double* y, const double* x, hipStream_t st);
+void axpy_launch(long long n, double* y, const double* x, const int* skip, hipStream_t st);

@@ -181,7 +181,12 @@ __device__ __forceinline__ void final_sums(const double* partial, int stride, in
 // ---------------------------------------------------------------- GMRES residual r0 = M^-1 (b - A x), gam0 = ||r0||
 template <int OPK>
 __global__ void __launch_bounds__(256) k_gmres_residual(typename OpSel<OPK>::T op, KrylovBufs kb, const double* __restrict__ x,
-                                                        const double* __restrict__ b, KrylovDev* K, int outer, int x_is_zero) {
+                                                        const double* __restrict__ b, KrylovDev* K, int outer, int x_is_zero,
+                                                        const int* skip) {
+  if (skip && *skip) {           // the caller's outer iteration has already converged: no solve
+    if (blockIdx.x == 0 && threadIdx.x == 0) { K->done = 1; K->converged = 1; K->jlast = -1; K->steps = 0; K->outer = outer; }
+    return;
+  }
   constexpr int NC = OpSel<OPK>::NC;
   __shared__ double xs[NC * (TY + 2) * (TX + 2)];
   __shared__ double red[4];
@@ -469,7 +474,8 @@ __global__ void k_vslope(Dims d, Phys P, const double* __restrict__ pf, const do
   if (ef) atomicOr(eflag, 1);
 }
 
-__global__ void k_axpy1(long long n, double* __restrict__ y, const double* __restrict__ x) {
+__global__ void k_axpy1(long long n, double* __restrict__ y, const double* __restrict__ x, const int* skip) {
+  if (skip && *skip) return;
   long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e < n) y[e] += x[e];
 }
@@ -486,23 +492,28 @@ static VPtrs make_vptrs(const KrylovBufs&, double* const* hostV) {
 }
 
 void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
-                    KrylovDev* K, int outer, int x_is_zero, hipStream_t st) {
+                    KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st) {
   if (op == 0) {
     int g = grid_for_tiles(mo.d, kb.nblk_apply);
-    k_gmres_residual<0><<<g, 256, 0, st>>>(mo, kb, x, b, K, outer, x_is_zero);
+    k_gmres_residual<0><<<g, 256, 0, st>>>(mo, kb, x, b, K, outer, x_is_zero, skip);
   } else {
     int g = grid_for_tiles(po.d, kb.nblk_apply);
-    k_gmres_residual<1><<<g, 256, 0, st>>>(po, kb, x, b, K, outer, x_is_zero);
+    k_gmres_residual<1><<<g, 256, 0, st>>>(po, kb, x, b, K, outer, x_is_zero, skip);
   }
 }
 
-void arnoldi_step(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j, int m,
-                  KrylovDev* K, hipStream_t st) {
+void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
+                   KrylovDev* K, hipStream_t st) {
   VPtrs V = make_vptrs(kb, hostV);
   const Dims& d = op == 0 ? mo.d : po.d;
-  long long n = (long long)(op == 0 ? 3 : 1) * d.N;
   if (op == 0) k_arnoldi_apply<0><<<grid_for_tiles(d, kb.nblk_apply), 256, 0, st>>>(mo, kb, V, j, K);
   else k_arnoldi_apply<1><<<grid_for_tiles(d, kb.nblk_apply), 256, 0, st>>>(po, kb, V, j, K);
+}
+
+void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
+                    hipStream_t st) {
+  VPtrs V = make_vptrs(kb, hostV);
+  long long n = (long long)nc * d.N;
   long long nb = (n + 255) / 256;
   int g = (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
   k_arnoldi_update<<<g, 256, 0, st>>>(kb, V, j, m, n, K);
@@ -538,6 +549,6 @@ void vslope_launch(const Dims& d, const Phys& P, const double* pf, const double*
   k_vslope<<<(d.N + 255) / 256, 256, 0, st>>>(d, P, pf, u, S, vs, eflag);
 }
 
-void axpy_launch(long long n, double* y, const double* x, hipStream_t st) {
-  k_axpy1<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, y, x);
+void axpy_launch(long long n, double* y, const double* x, const int* skip, hipStream_t st) {
+  k_axpy1<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, y, x, skip);
 }

@@ -112,6 +112,14 @@ typedef struct imex_solve_info {
   int status;
 } imex_solve_info;
 
+/* Per-kernel-class timing gathered with HIP events on the context stream. */
+typedef struct imex_kernel_profile {
+  const char* name;
+  long long launches;
+  double total_ms;
+  double bytes;                   /* algorithmic bytes (distinct per-cell fields read + written) */
+} imex_kernel_profile;
+
 typedef struct imex_ctx imex_ctx;
 
 /* ---- parameters and initial data ---- */
@@ -177,6 +185,12 @@ int imex_gmres(imex_ctx* ctx, int op, const double* b, double* x, int precond,
                const double* p_freeze, double mbig, imex_solve_info* info);
 /* Pressure_NestedNewton (implicit_operators.cpp:625-738). p in/out. */
 int imex_pressure_newton(imex_ctx* ctx, const double* b, double* p, imex_solve_info* info);
+
+/* ---- profiling ---- */
+/* Enable (1) or disable (0) per-launch event timing; enabling clears the totals. */
+int imex_profile_enable(imex_ctx* ctx, int on);
+/* Fills up to `max` entries, one per kernel class; returns the number written. */
+int imex_profile_read(imex_ctx* ctx, imex_kernel_profile* out, int max);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,79 @@
+"""C-ABI library checks that need no GPU: exported symbols, defaults, host-side initial data."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import imex_amd as im
+import oracle as orc
+from helpers import G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "imex.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(imex_[a-z_0-9]+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 25
+    L = C.CDLL(im.capi.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_default_params_match_reference_config():
+    p = im.default_params()
+    # include/config.hpp:57, 71, 77, 100-107, 125; implicit_operators.cpp:801
+    assert (p.cfl, p.gmres_tol, p.gmres_restart, p.shock_kappa1) == (0.8, 1e-14, 25, 0.001)
+    assert (p.floor_on, p.p_floor, p.theta, p.mbig_fac, p.newton_max, p.newton_tol) == (1, 1e-7, 0.9, 1e6, 20, 1e-8)
+    assert (p.rk_alfa, p.rk_b, p.rk_R, p.rk_cv) == (0.05, 0.1, 0.4, 1.0)
+
+
+def test_vortex_initial_data():
+    p = im.vortex_params(32)
+    amp, u0, v0 = 1.5, 1.0, 0.5
+    u, bx, by = im.init_case(p, im.CASE_VORTEX, amp, u0, v0)
+    dx = 10.0 / 32
+    xc = -5 + (np.arange(32) + 0.5) * dx
+    X, Y = np.meshgrid(xc, xc)
+    r2 = X ** 2 + Y ** 2
+    ui = u[G:-G, G:-G]
+    assert np.allclose(ui[..., 0], 1.0)
+    assert np.allclose(ui[..., 1], u0 - amp / (2 * np.pi) * np.exp(0.5 * (1 - r2)) * Y, rtol=0, atol=1e-14)
+    assert np.allclose(ui[..., 2], v0 + amp / (2 * np.pi) * np.exp(0.5 * (1 - r2)) * X, rtol=0, atol=1e-14)
+    # dp = -amp^2 r^2 exp(1 - r^2) / (8 pi^2)
+    assert np.allclose(ui[..., 4], 1 - amp ** 2 * r2 * np.exp(1 - r2) / (8 * np.pi ** 2), rtol=0, atol=1e-14)
+    # faces come from the corner potential: discrete div B vanishes to round-off
+    assert orc.div_b(p, bx, by) < 1e-14
+    # cell B = face average (before the periodic wrap of the faces, as in the reference)
+    assert np.allclose(ui[..., 5][:, 1:], 0.5 * (bx[G:-G, G:G + 31] + bx[G:-G, G + 1:G + 32]), rtol=0, atol=1e-15)
+    # ghosts are periodic copies
+    assert np.array_equal(u[G:-G, 0], u[G:-G, 32])
+
+
+def test_orszag_tang_and_blast_initial_data():
+    p = im.orszag_tang_params(16)
+    u, bx, by = im.init_case(p, im.CASE_ORSZAG_TANG)
+    xc = (np.arange(16) + 0.5) * 2 * np.pi / 16
+    assert np.allclose(u[G:-G, G:-G, 0], 25 / 9) and np.allclose(u[G:-G, G:-G, 4], 5 / 3)
+    assert np.allclose(u[G:-G, G:-G, 6], np.sin(2 * xc)[None, :], atol=1e-15)
+    assert orc.div_b(p, bx, by) == 0.0
+    q = im.blast_params(20)
+    u, bx, by = im.init_case(q, im.CASE_BLAST, 100.0)
+    ui = u[G:-G, G:-G]
+    assert set(np.unique(ui[..., 4])) == {0.1, 1000.0}
+    assert np.allclose(ui[..., 5], 100 / np.sqrt(4 * np.pi) / np.sqrt(2))
+    assert orc.div_b(q, bx, by) < 1e-12
+
+
+def test_product_never_imports_the_oracle():
+    import subprocess
+    import sys
+    code = "import sys; import imex_amd; assert not any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules)"
+    subprocess.run([sys.executable, "-c", code], check=True, cwd=ROOT)
