@@ -33,8 +33,7 @@ __global__ void k_visc_rk(Dims d, Phys P, const double* pe, const double* u, dou
 }
 // Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
 // device: acc[0] = min p ever, acc[1] = max floored count, acc[2] = 1 if min p < theta p_floor - 1e-14.
-__global__ void k_pstats(long long n, const double* p, double pfloor, double thr, double* partial, unsigned* counter,
-                         double* acc) {
+__global__ void k_pstats(long long n, const double* p, double pfloor, double* partial) {
   __shared__ double smin[4], scnt[4];
   double mn = 1e300, cnt = 0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
@@ -50,11 +49,11 @@ __global__ void k_pstats(long long n, const double* p, double pfloor, double thr
     partial[2 * blockIdx.x] = smin[0] < mn ? smin[0] : mn;
     partial[2 * blockIdx.x + 1] = cnt;
   }
-  if (!last_block_arrive(counter)) return;
+}
+__global__ void k_pstats_fin(const double* partial, int nblk, double thr, double* acc) {
   if (threadIdx.x == 0) {
-    *counter = 0;
     double m = 1e300, c = 0;
-    for (int b = 0; b < (int)gridDim.x; b++) { m = fmin(m, partial[2 * b]); c += partial[2 * b + 1]; }
+    for (int b = 0; b < nblk; b++) { m = fmin(m, partial[2 * b]); c += partial[2 * b + 1]; }
     acc[0] = fmin(acc[0], m);
     acc[1] = fmax(acc[1], c);
     if (m < thr) acc[2] = 1.0;
@@ -123,7 +122,8 @@ struct imex_ctx {
   StageBind cur;
   bool qe_ready = false, hhat_ready = false;
   imex_stats stats{};
-  int pred[2] = {6, 6};
+  int pred[2] = {6, 6};         // Arnoldi steps of the last magnetic / pressure solve
+  int pred_newton = 1;         // Jacobian solves of the last nested-Newton solve
   int nblk_start = 1024;
   KrylovBufs kb;
 };
@@ -219,7 +219,8 @@ static POp p_op(imex_ctx* c, int jacobian, int precond) {
 // GMRES_Solver (implicit_operators.cpp:793-919).  Arnoldi steps are queued in chunks; the
 // convergence flag of chunk k is read (async copy + event) while chunk k+1 is already queued.
 static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b, int x_is_zero,
-                     imex_solve_info* info, long long* it_acc, long long* rs_acc, const int* skip = nullptr) {
+                     imex_solve_info* info, long long* it_acc, long long* rs_acc, const int* skip = nullptr,
+                     int first_chunk = 0) {
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
   const long long dim = (long long)nc * c->d.N;
@@ -227,7 +228,8 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   // additionally capped at 4000 cycles (100k Arnoldi steps at m = 25) and raises
   // IMEX_WARN_GMRES_MAXOUTER when it gives up.
   const long long outmax = std::min<long long>(c->prm.gmres_max_outer > 0 ? c->prm.gmres_max_outer : dim, 4000);
-  const int chunk = 4;
+  const int chunk = 2;
+  if (first_chunk <= 0) first_chunk = c->pred[op];
   int total = 0, rc = 0;
   int converged = 0;
   long long outer = 0;
@@ -242,7 +244,7 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
     bool stop = false;
     while (!stop) {
       if (j >= m) break;                       // whole cycle queued; the last chunk decides
-      int jend = std::min(m, j + (j == 0 ? std::max(chunk, c->pred[op]) : chunk));
+      int jend = std::min(m, j + (j == 0 ? std::max(first_chunk, 1) : chunk));
       for (; j < jend; j++) {
         int e0 = prof_begin(c);
         arnoldi_apply(op, mo, po, c->kb, c->V, j, c->K, c->st);
@@ -284,7 +286,7 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   if (!converged) { rc |= IMEX_WARN_GMRES_MAXOUTER; c->stats.warnings |= IMEX_WARN_GMRES_MAXOUTER; }
   if (it_acc) *it_acc += total;
   if (rs_acc) *rs_acc += outer;
-  c->pred[op] = std::max(2, total);
+  if (total > 0 && outer == 0) c->pred[op] = total;
   if (info) { info->iters = total; info->restarts = (int)outer; info->res0 = res0; info->res = res; info->status = rc; }
   return rc;
 }
@@ -314,7 +316,10 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
     POp jac = p_op(c, 1, 1);
     MagOp mo = mag_op(c);
     long long its = 0, rss = 0;
-    rc |= run_gmres(c, 1, mo, jac, c->dp, c->ng, 1, nullptr, &its, &rss, conv);
+    // Newton usually needs pred_newton Jacobian solves; past that the solve is expected to be
+    // skipped, so only a one-step chunk is queued speculatively
+    int first = it < c->pred_newton ? c->pred[1] : 1;
+    rc |= run_gmres(c, 1, mo, jac, c->dp, c->ng, 1, nullptr, &its, &rss, conv, first);
     if (it == 0) r0 = c->Klast.newton_res0;
     r = c->Klast.newton_res;
     if (c->Klast.newton_conv) { rc &= ~IMEX_WARN_GMRES_MAXOUTER; break; }
@@ -323,11 +328,12 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
     LAUNCH(KC_MISC, 3, axpy_launch(d.N, p, c->dp, conv, c->st));
   }
   c->stats.newton_iters += it;
+  c->pred_newton = std::max(1, it);
   if (it == c->prm.newton_max) { rc |= IMEX_WARN_NEWTON_MAXITER; c->stats.warnings |= IMEX_WARN_NEWTON_MAXITER; }
   if (c->prm.floor_on) {
     long long g = std::min<long long>(nb(d.N), 256);
-    LAUNCH(KC_MISC, 1, (k_pstats<<<(unsigned)g, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->prm.theta * c->prm.p_floor - 1e-14,
-                                                                  c->partial, c->cnt_pstats, c->pst)));
+    LAUNCH(KC_MISC, 1, (k_pstats<<<(unsigned)g, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->partial),
+                        k_pstats_fin<<<1, 64, 0, c->st>>>(c->partial, (int)g, c->prm.theta * c->prm.p_floor - 1e-14, c->pst)));
   }
   if (info) { info->iters = it; info->res0 = r0; info->res = r; info->status = rc; info->restarts = 0; }
   return rc;
@@ -348,7 +354,7 @@ static void ensure_qe(imex_ctx* c) {
 
 static int sub_explicit_conv(imex_ctx* c) {
   const Dims& d = c->d;
-  LAUNCH(KC_EXPL_CONV, 13, (k_explicit<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
+  LAUNCH(KC_EXPL_CONV, 13, (k_explicit<<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
   double* u = c->cur.u;
   fill_planes(c, {u, u + d.PS, u + 2 * d.PS, u + 3 * d.PS});
   c->hhat_ready = false;
@@ -373,7 +379,7 @@ static int sub_mag_update_b(imex_ctx* c) {
 
 static int sub_explicit_press(imex_ctx* c) {
   const Dims& d = c->d;
-  LAUNCH(KC_EXPL_PRESS, 16, (k_explicit<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
+  LAUNCH(KC_EXPL_PRESS, 16, (k_explicit<<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
   double* u = c->cur.u;
   fill_planes(c, {u + d.PS, u + 2 * d.PS, u + 3 * d.PS, u + 4 * d.PS});
   return 0;
@@ -450,7 +456,8 @@ static int step_start(imex_ctx* c) {
   long long tot = (long long)d.NX * d.NY;
   long long g = std::min<long long>(nb(tot), c->nblk_start);
   const double* in = c->state_mode ? c->U2 : c->u_pri;
-  LAUNCH(KC_START, 16, (k_step_start<<<(unsigned)g, 256, 0, c->st>>>(d, c->P, c->state_mode, in, c->u_n, c->partial, c->cnt_start, c->S, c->eflag)));
+  LAUNCH(KC_START, 16, (k_step_start<<<(unsigned)g, 256, 0, c->st>>>(d, c->P, c->state_mode, in, c->u_n, c->partial, c->eflag),
+                        k_step_finish<<<1, 256, 0, c->st>>>(d, c->P, c->partial, (int)g, c->S)));
   return 0;
 }
 

@@ -41,7 +41,8 @@ struct POp {
 // ---- launchers (kernels_stage.hip) ----
 void launch_bc_fill(const Dims& d, const PlaneSet& ps, int kind, hipStream_t st);
 
-__global__ void k_step_start(Dims d, Phys P, int mode, const double* uin, double* un, double* partial, unsigned* counter, double* S, int* eflag);
+__global__ void k_step_start(Dims d, Phys P, int mode, const double* uin, double* un, double* partial, int* eflag);
+__global__ void k_step_finish(Dims d, Phys P, const double* partial, int nblk, double* S);
 __global__ void k_stage_combine(Dims d, const double* un, const double* u1, double* Qe, double* Qs,
                                 const double* bxn, const double* bx1, const double* byn, const double* by1,
                                 double* bxqs, double* byqs, const double* S);

@@ -38,60 +38,134 @@ __device__ __forceinline__ void load_tile(double* xs, const Dims& d, const doubl
 }
 
 // ---------------------------------------------------------------- magnetic operator
-__device__ __forceinline__ void mag_apply(const MagOp& op, const double* xs, int lx, int ly, int i, int j, double* out) {
+// Magnetic_Subsystem_AX (implicit_operators.cpp:141-353, GLF branch) in face-flux form.
+// The reference's Bx_1+Bx_2 and Bz_5+Bz_6 are fluxes through the cell's upper y-face and
+// Bx_3+Bx_4, Bz_7+Bz_8 the same fluxes through its lower y-face; By_1+By_2, Bz_1+Bz_2 (and
+// By_3+By_4, Bz_3+Bz_4) likewise through its right (left) x-face.  Each face flux is formed
+// once per tile, and the four-cell corner averages once per corner (summed in the order
+// lower-left, lower-right, upper-left, upper-right).  Apart from that summation order the
+// arithmetic is the reference's expression by expression.
+struct MagSmem {
+  double x[3][TY + 2][TX + 2];     // unknown B (scaled), one-cell halo through M_B
+  double q[3][TY + 2][TX + 2];     // B of Qe (raw ghost frame)
+  double r[TY + 2][TX + 2];        // rho of u (raw ghost frame)
+  double dv[6][TY + 2][TX + 2];    // per cell: PM - Qx5 x5, PM - Qx6 x6, q6 x5, q6 x7, q5 x6, q5 x7; later face fluxes
+  double cn[6][TY + 1][TX + 1];    // per corner: n0 b1, n0 b2, n1 b0, n1 b2, PC0, PC1
+};
+
+__device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const double* __restrict__ x, double h, bool scale,
+                                         int gx0, int gy0) {
+  const Dims& d = op.d;
+  const int W = TX + 2, HH = TY + 2, NT = W * HH;
+  for (int t = threadIdx.x; t < 7 * NT; t += blockDim.x) {
+    int f = t / NT, rr = t % NT;
+    int ly = rr / W, lx = rr % W;
+    int cx = gx0 + lx - 1, cy = gy0 + ly - 1;
+    if (f < 3) {
+      int gx = cmap(cx, d.nx, d.perx), gy = cmap(cy, d.ny, d.pery);
+      double v = x[(long long)f * d.N + gx + (long long)d.nx * gy];
+      sm.x[f][ly][lx] = scale ? v / h : v;
+    } else {
+      int pi = min(cx + GC, d.NX - 1), pj = min(cy + GC, d.NY - 1);
+      long long c = pidx(d, pi, pj);
+      if (f < 6) sm.q[f - 3][ly][lx] = op.Qe[(long long)(5 + f - 3) * d.PS + c];
+      else sm.r[ly][lx] = op.u[c];
+    }
+  }
+}
+
+// All threads call this (it synchronises the block); valid threads get A x of their cell in out[3].
+__device__ __forceinline__ void mag_compute(const MagOp& op, MagSmem& sm, int tx, int ty, bool valid, double* out) {
   const Dims& d = op.d;
   const int W = TX + 2, HH = TY + 2;
   const double h = op.S[S_ADT], dx = d.dx, dy = d.dy;
-  auto X = [&](int di, int dj, int k) { return xs[((k - 5) * HH + ly + dj) * W + lx + di]; };
-  auto Q = [&](int di, int dj, int k) { return op.Qe[k * d.PS + pidx(d, i + di, j + dj)]; };
-  auto R = [&](int di, int dj) { return op.u[pidx(d, i + di, j + dj)]; };
-  double rho_p0 = 0.5 * (R(0, 0) + R(1, 0)), rho_m0 = 0.5 * (R(0, 0) + R(-1, 0));
-  double rho_0p = 0.5 * (R(0, 0) + R(0, 1)), rho_0m = 0.5 * (R(0, 0) + R(0, -1));
-  double n_p0[3], n_m0[3], n_0p[3], n_0m[3], n_pp[3], n_pm[3], n_mp[3], n_mm[3];
-  double b_pp[3], b_pm[3], b_mp[3], b_mm[3];
-  for (int c = 0; c < 3; c++) {
-    int k = 5 + c;
-    double q00 = Q(0, 0, k), qp0 = Q(1, 0, k), qm0 = Q(-1, 0, k), q0p = Q(0, 1, k), q0m = Q(0, -1, k);
-    n_p0[c] = 0.5 * (q00 + qp0); n_m0[c] = 0.5 * (q00 + qm0);
-    n_0p[c] = 0.5 * (q00 + q0p); n_0m[c] = 0.5 * (q00 + q0m);
-    n_pp[c] = 0.25 * (q00 + qp0 + q0p + Q(1, 1, k));
-    n_pm[c] = 0.25 * (q00 + qp0 + q0m + Q(1, -1, k));
-    n_mp[c] = 0.25 * (q00 + qm0 + q0p + Q(-1, 1, k));
-    n_mm[c] = 0.25 * (q00 + qm0 + q0m + Q(-1, -1, k));
-    double x00 = X(0, 0, k), xp0 = X(1, 0, k), xm0 = X(-1, 0, k), x0p = X(0, 1, k), x0m = X(0, -1, k);
-    b_pp[c] = 0.25 * (x00 + xp0 + x0p + X(1, 1, k));
-    b_pm[c] = 0.25 * (x00 + xp0 + x0m + X(1, -1, k));
-    b_mp[c] = 0.25 * (x00 + xm0 + x0p + X(-1, 1, k));
-    b_mm[c] = 0.25 * (x00 + xm0 + x0m + X(-1, -1, k));
+  const double idx = 1 / dx, idy = 1 / dy, midx = -1 / dx;
+  // per-cell products
+  for (int t = threadIdx.x; t < W * HH; t += blockDim.x) {
+    int ly = t / W, lx = t % W;
+    double q5 = sm.q[0][ly][lx], q6 = sm.q[1][ly][lx], q7 = sm.q[2][ly][lx];
+    double x5 = sm.x[0][ly][lx], x6 = sm.x[1][ly][lx], x7 = sm.x[2][ly][lx];
+    double q5x5 = q5 * x5, q6x6 = q6 * x6;
+    double PM = 0.5 * (q5x5 + q6x6 + q7 * x7);
+    sm.dv[0][ly][lx] = PM - q5x5;
+    sm.dv[1][ly][lx] = PM - q6x6;
+    sm.dv[2][ly][lx] = q6 * x5;
+    sm.dv[3][ly][lx] = q6 * x7;
+    sm.dv[4][ly][lx] = q5 * x6;
+    sm.dv[5][ly][lx] = q5 * x7;
   }
-  auto PM = [&](int di, int dj) { return 0.5 * (Q(di, dj, 5) * X(di, dj, 5) + Q(di, dj, 6) * X(di, dj, 6) + Q(di, dj, 7) * X(di, dj, 7)); };
-  auto PC = [&](const double* n, const double* b, int m) { return 0.5 * (n[0] * b[0] + n[1] * b[1] + n[2] * b[2]) - n[m] * b[m]; };
-  const double pm00 = PM(0, 0), pmp0 = PM(1, 0), pmm0 = PM(-1, 0), pm0p = PM(0, 1), pm0m = PM(0, -1);
-  double Bx_1 = n_0p[0] / rho_0p * (1 / dx * (n_pp[0] * b_pp[1] - n_mp[0] * b_mp[1]) - 1 / dy * ((pm0p - Q(0, 1, 6) * X(0, 1, 6)) - (pm00 - Q(0, 0, 6) * X(0, 0, 6))));
-  double Bx_2 = n_0p[1] / rho_0p * (1 / dx * (PC(n_pp, b_pp, 0) - PC(n_mp, b_mp, 0)) - 1 / dy * (Q(0, 1, 6) * X(0, 1, 5) - Q(0, 0, 6) * X(0, 0, 5)));
-  double Bx_3 = n_0m[0] / rho_0m * (1 / dx * (n_pm[0] * b_pm[1] - n_mm[0] * b_mm[1]) - 1 / dy * ((pm00 - Q(0, 0, 6) * X(0, 0, 6)) - (pm0m - Q(0, -1, 6) * X(0, -1, 6))));
-  double Bx_4 = n_0m[1] / rho_0m * (1 / dx * (PC(n_pm, b_pm, 0) - PC(n_mm, b_mm, 0)) - 1 / dy * (Q(0, 0, 6) * X(0, 0, 5) - Q(0, -1, 6) * X(0, -1, 5)));
-  double ax5 = X(0, 0, 5) + (h * h / dy) * ((Bx_1 + Bx_2) - (Bx_3 + Bx_4));
-  double By_1 = n_p0[1] / rho_p0 * (-1 / dx * ((pmp0 - Q(1, 0, 5) * X(1, 0, 5)) - (pm00 - Q(0, 0, 5) * X(0, 0, 5))) + 1 / dy * (n_pp[0] * b_pp[1] - n_pm[0] * b_pm[1]));
-  double By_2 = n_p0[0] / rho_p0 * (-1 / dx * (Q(1, 0, 5) * X(1, 0, 6) - Q(0, 0, 5) * X(0, 0, 6)) + 1 / dy * (PC(n_pp, b_pp, 1) - PC(n_pm, b_pm, 1)));
-  double By_3 = n_m0[1] / rho_m0 * (-1 / dx * ((pm00 - Q(0, 0, 5) * X(0, 0, 5)) - (pmm0 - Q(-1, 0, 5) * X(-1, 0, 5))) + 1 / dy * (n_mp[0] * b_mp[1] - n_mm[0] * b_mm[1]));
-  double By_4 = n_m0[0] / rho_m0 * (-1 / dx * (Q(0, 0, 5) * X(0, 0, 6) - Q(-1, 0, 5) * X(-1, 0, 6)) + 1 / dy * (PC(n_mp, b_mp, 1) - PC(n_mm, b_mm, 1)));
-  double ax6 = X(0, 0, 6) + (h * h / dx) * ((By_1 + By_2) - (By_3 + By_4));
-  double Bz_1 = n_p0[2] / rho_p0 * (-1 / dx * ((pmp0 - Q(1, 0, 5) * X(1, 0, 5)) - (pm00 - Q(0, 0, 5) * X(0, 0, 5))) + 1 / dy * (n_pp[1] * b_pp[0] - n_pm[1] * b_pm[0]));
-  double Bz_2 = n_p0[0] / rho_p0 * (-1 / dx * (Q(1, 0, 5) * X(1, 0, 7) - Q(0, 0, 5) * X(0, 0, 7)) - 1 / dy * (n_pp[1] * b_pp[2] - n_pm[1] * b_pm[2]));
-  double Bz_3 = n_m0[2] / rho_m0 * (-1 / dx * ((pm00 - Q(0, 0, 5) * X(0, 0, 5)) - (pmm0 - Q(-1, 0, 5) * X(-1, 0, 5))) + 1 / dy * (n_mp[1] * b_mp[0] - n_mm[1] * b_mm[0]));
-  double Bz_4 = n_m0[0] / rho_m0 * (-1 / dx * (Q(0, 0, 5) * X(0, 0, 7) - Q(-1, 0, 5) * X(-1, 0, 7)) - 1 / dy * (n_mp[1] * b_mp[2] - n_mm[1] * b_mm[2]));
-  double Bz_5 = n_0p[2] / rho_0p * (1 / dx * (n_pp[0] * b_pp[1] - n_mp[0] * b_mp[1]) - 1 / dy * ((pm0p - Q(0, 1, 6) * X(0, 1, 6)) - (pm00 - Q(0, 0, 6) * X(0, 0, 6))));
-  double Bz_6 = n_0p[1] / rho_0p * (-1 / dx * (n_pp[0] * b_pp[2] - n_mp[0] * b_mp[2]) - 1 / dy * (Q(0, 1, 6) * X(0, 1, 7) - Q(0, 0, 6) * X(0, 0, 7)));
-  double Bz_7 = n_0m[2] / rho_0m * (1 / dx * (n_pm[0] * b_pm[1] - n_mm[0] * b_mm[1]) - 1 / dy * ((pm00 - Q(0, 0, 6) * X(0, 0, 6)) - (pm0m - Q(0, -1, 6) * X(0, -1, 6))));
-  double Bz_8 = n_0m[1] / rho_0m * (-1 / dx * (n_pm[0] * b_pm[2] - n_mm[0] * b_mm[2]) - 1 / dy * (Q(0, 0, 6) * X(0, 0, 7) - Q(0, -1, 6) * X(0, -1, 7)));
-  double ax7 = X(0, 0, 7) + h * h / dx * ((Bz_1 + Bz_2) - (Bz_3 + Bz_4)) + h * h / dy * ((Bz_5 + Bz_6) - (Bz_7 + Bz_8));
+  // corner terms: corner (a, b) lies between LDS cells a..a+1 and b..b+1
+  for (int t = threadIdx.x; t < (TX + 1) * (TY + 1); t += blockDim.x) {
+    int b = t / (TX + 1), a = t % (TX + 1);
+    double n[3], bb[3];
+    for (int c = 0; c < 3; c++) {
+      n[c] = 0.25 * (sm.q[c][b][a] + sm.q[c][b][a + 1] + sm.q[c][b + 1][a] + sm.q[c][b + 1][a + 1]);
+      bb[c] = 0.25 * (sm.x[c][b][a] + sm.x[c][b][a + 1] + sm.x[c][b + 1][a] + sm.x[c][b + 1][a + 1]);
+    }
+    double half = 0.5 * (n[0] * bb[0] + n[1] * bb[1] + n[2] * bb[2]);
+    sm.cn[0][b][a] = n[0] * bb[1];
+    sm.cn[1][b][a] = n[0] * bb[2];
+    sm.cn[2][b][a] = n[1] * bb[0];
+    sm.cn[3][b][a] = n[1] * bb[2];
+    sm.cn[4][b][a] = half - n[0] * bb[0];
+    sm.cn[5][b][a] = half - n[1] * bb[1];
+  }
+  __syncthreads();
+  // face fluxes into registers (each thread owns at most two faces of each orientation)
+  double gxr[2], gzr[2], fyr[2], fzr[2];
+  const int NYF = (TY + 1) * TX, NXF = TY * (TX + 1);
+  for (int k = 0; k < 2; k++) {
+    int t = threadIdx.x + k * blockDim.x;
+    if (t < NYF) {   // y-face between LDS rows yb and yb+1 at LDS column a = fx+1
+      int yb = t / TX, a = t % TX + 1;
+      double rf = 0.5 * (sm.r[yb][a] + sm.r[yb + 1][a]);
+      double nf0 = 0.5 * (sm.q[0][yb][a] + sm.q[0][yb + 1][a]);
+      double nf1 = 0.5 * (sm.q[1][yb][a] + sm.q[1][yb + 1][a]);
+      double nf2 = 0.5 * (sm.q[2][yb][a] + sm.q[2][yb + 1][a]);
+      int cR = a, cL = a - 1;
+      double T1 = idx * (sm.cn[0][yb][cR] - sm.cn[0][yb][cL]) - idy * (sm.dv[1][yb + 1][a] - sm.dv[1][yb][a]);
+      gxr[k] = nf0 / rf * T1 + nf1 / rf * (idx * (sm.cn[4][yb][cR] - sm.cn[4][yb][cL]) - idy * (sm.dv[2][yb + 1][a] - sm.dv[2][yb][a]));
+      gzr[k] = nf2 / rf * T1 + nf1 / rf * (midx * (sm.cn[1][yb][cR] - sm.cn[1][yb][cL]) - idy * (sm.dv[3][yb + 1][a] - sm.dv[3][yb][a]));
+    }
+    if (t < NXF) {   // x-face between LDS columns xa and xa+1 at LDS row b = fy+1
+      int b = t / (TX + 1) + 1, xa = t % (TX + 1);
+      double rf = 0.5 * (sm.r[b][xa] + sm.r[b][xa + 1]);
+      double nf0 = 0.5 * (sm.q[0][b][xa] + sm.q[0][b][xa + 1]);
+      double nf1 = 0.5 * (sm.q[1][b][xa] + sm.q[1][b][xa + 1]);
+      double nf2 = 0.5 * (sm.q[2][b][xa] + sm.q[2][b][xa + 1]);
+      int cT = b, cB = b - 1;
+      double dpm = sm.dv[0][b][xa + 1] - sm.dv[0][b][xa];
+      double T2 = midx * dpm + idy * (sm.cn[0][cT][xa] - sm.cn[0][cB][xa]);
+      fyr[k] = nf1 / rf * T2 + nf0 / rf * (midx * (sm.dv[4][b][xa + 1] - sm.dv[4][b][xa]) + idy * (sm.cn[5][cT][xa] - sm.cn[5][cB][xa]));
+      fzr[k] = nf2 / rf * (midx * dpm + idy * (sm.cn[2][cT][xa] - sm.cn[2][cB][xa]))
+             + nf0 / rf * (midx * (sm.dv[5][b][xa + 1] - sm.dv[5][b][xa]) - idy * (sm.cn[3][cT][xa] - sm.cn[3][cB][xa]));
+    }
+  }
+  __syncthreads();
+  // park the fluxes in the dv area (no longer needed)
+  double* gx = &sm.dv[0][0][0];
+  double* gz = gx + NYF;
+  double* fy = gz + NYF;
+  double* fz = fy + NXF;
+  for (int k = 0; k < 2; k++) {
+    int t = threadIdx.x + k * blockDim.x;
+    if (t < NYF) { gx[t] = gxr[k]; gz[t] = gzr[k]; }
+    if (t < NXF) { fy[t] = fyr[k]; fz[t] = fzr[k]; }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int a = tx + 1, b = ty + 1;
+  auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
+  // upper / lower y-face of the cell: faces ty+1 / ty (row-major TX per row); right / left x-face: tx+1 / tx
+  const int yu = (ty + 1) * TX + tx, yl = ty * TX + tx;
+  const int xr = ty * (TX + 1) + tx + 1, xl = ty * (TX + 1) + tx;
+  double ax5 = X(0, 0, 0) + (h * h / dy) * (gx[yu] - gx[yl]);
+  double ax6 = X(1, 0, 0) + (h * h / dx) * (fy[xr] - fy[xl]);
+  double ax7 = X(2, 0, 0) + h * h / dx * (fz[xr] - fz[xl]) + h * h / dy * (gz[yu] - gz[yl]);
   const double nu_x = op.S[S_BMAXX] * dx, nu_y = op.S[S_BMAXY] * dy;
   double lap[3];
-  for (int c = 0; c < 3; c++) {
-    int k = 5 + c;
-    lap[c] = nu_x / (dx * dx) * (X(-1, 0, k) - 2.0 * X(0, 0, k) + X(1, 0, k)) + nu_y / (dy * dy) * (X(0, -1, k) - 2.0 * X(0, 0, k) + X(0, 1, k));
-  }
+  for (int c = 0; c < 3; c++)
+    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - 2.0 * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - 2.0 * X(c, 0, 0) + X(c, 0, 1));
   out[0] = ax5 - h * lap[0];
   out[1] = ax6 - h * lap[1];
   out[2] = ax7 - h * lap[2];
@@ -132,18 +206,35 @@ __device__ __forceinline__ void p_precond(const POp& op, int cidx, int i, int j,
   y[0] = y[0] / (Vd + op.Tdiag[cidx]);
 }
 
+// Tile interface shared by the GMRES kernels: load the input vector tile (plus whatever the
+// operator stages), then compute M^-1 A x for the block's cells.
+struct PSmem { double x[(TY + 2) * (TX + 2)]; };
 template <int OPK> struct OpSel;
 template <> struct OpSel<0> {
   static constexpr int NC = 3;
   typedef MagOp T;
-  __device__ static void apply(const MagOp& op, const double* xs, int lx, int ly, int i, int j, int, double* out) { mag_apply(op, xs, lx, ly, i, j, out); }
+  typedef MagSmem Smem;
+  __device__ static void load(const MagOp& op, Smem& sm, const double* x, double h, bool scale, int gx0, int gy0) {
+    mag_load(op, sm, x, h, scale, gx0, gy0);
+  }
+  __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int, int, int, bool valid, double* y) {
+    mag_compute(op, sm, tx, ty, valid, y);
+  }
+  __device__ static double center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
   __device__ static void precond(const MagOp&, int, int, int, double*) {}
   __device__ static const Dims& dims(const MagOp& op) { return op.d; }
 };
 template <> struct OpSel<1> {
   static constexpr int NC = 1;
   typedef POp T;
-  __device__ static void apply(const POp& op, const double* xs, int lx, int ly, int i, int j, int cidx, double* out) { p_apply(op, xs, lx, ly, i, j, cidx, out); }
+  typedef PSmem Smem;
+  __device__ static void load(const POp& op, Smem& sm, const double* x, double h, bool scale, int gx0, int gy0) {
+    load_tile<1>(sm.x, op.d, x, h, scale, gx0, gy0);
+  }
+  __device__ static void compute(const POp& op, Smem& sm, int tx, int ty, int i, int j, int cidx, bool valid, double* y) {
+    if (valid) p_apply(op, sm.x, tx + 1, ty + 1, i, j, cidx, y);
+  }
+  __device__ static double center(const Smem& sm, int, int tx, int ty) { return sm.x[(ty + 1) * (TX + 2) + tx + 1]; }
   __device__ static void precond(const POp& op, int cidx, int i, int j, double* y) { p_precond(op, cidx, i, j, y); }
   __device__ static const Dims& dims(const POp& op) { return op.d; }
 };
@@ -188,7 +279,7 @@ __global__ void __launch_bounds__(256) k_gmres_residual(typename OpSel<OPK>::T o
     return;
   }
   constexpr int NC = OpSel<OPK>::NC;
-  __shared__ double xs[NC * (TY + 2) * (TX + 2)];
+  __shared__ typename OpSel<OPK>::Smem sm;
   __shared__ double red[4];
   __shared__ double res[1];
   const Dims& d = OpSel<OPK>::dims(op);
@@ -196,15 +287,18 @@ __global__ void __launch_bounds__(256) k_gmres_residual(typename OpSel<OPK>::T o
   double acc = 0.0;
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
-    __syncthreads();
-    if (!x_is_zero) load_tile<NC>(xs, d, x, 1.0, false, gx0, gy0);
-    __syncthreads();
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
-    if (cx < d.nx && cy < d.ny) {
-      int cidx = cx + d.nx * cy;
-      double y[NC];
-      if (!x_is_zero) OpSel<OPK>::apply(op, xs, tx + 1, ty + 1, cx + GC, cy + GC, cidx, y);
+    bool valid = cx < d.nx && cy < d.ny;
+    int cidx = cx + d.nx * cy;
+    double y[NC];
+    if (!x_is_zero) {
+      __syncthreads();
+      OpSel<OPK>::load(op, sm, x, 1.0, false, gx0, gy0);
+      __syncthreads();
+      OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+    }
+    if (valid) {
       if (x_is_zero) for (int c = 0; c < NC; c++) y[c] = b[(long long)c * d.N + cidx];
       else for (int c = 0; c < NC; c++) y[c] = b[(long long)c * d.N + cidx] - y[c];
       OpSel<OPK>::precond(op, cidx, cx + GC, cy + GC, y);
@@ -213,12 +307,16 @@ __global__ void __launch_bounds__(256) k_gmres_residual(typename OpSel<OPK>::T o
   }
   block_sums<1>(&acc, 1, red, res);
   if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
-  if (!last_block_arrive(&K->cnt_r)) return;
+}
+
+// One-block finish of the residual norm: gam0, tolerance, convergence (implicit_operators.cpp:829-836).
+__global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partial, int nblk, KrylovDev* K, int outer,
+                                                            const int* skip) {
+  if (skip && *skip) return;
   __shared__ double tot[1];
-  final_sums(kb.partial, MAXM + 2, gridDim.x, 1, tot);
+  final_sums(partial, MAXM + 2, nblk, 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
-    K->cnt_r = 0;
     double g0 = sqrt(tot[0]);
     if (outer == 0) { K->tol_eff = K->tol * g0 + K->tol; K->res0 = g0; }
     K->res = g0;
@@ -232,80 +330,93 @@ __global__ void __launch_bounds__(256) k_gmres_residual(typename OpSel<OPK>::T o
 
 // ---------------------------------------------------------------- Arnoldi step j, part 1:
 // v_j = w / h (written), Av_j = M^-1 A v_j (written), H[i][j] = v_i . Av_j for i <= j.
-template <int OPK>
+// JB >= j + 1 is the number of dot accumulators kept in registers.
+template <int OPK, int JB>
 __global__ void __launch_bounds__(256) k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, KrylovDev* K) {
   if (K->done) return;
   constexpr int NC = OpSel<OPK>::NC;
-  __shared__ double xs[NC * (TY + 2) * (TX + 2)];
-  __shared__ double red[4 * (MAXM + 1)];
-  __shared__ double outv[MAXM + 1];
+  __shared__ typename OpSel<OPK>::Smem sm;
+  __shared__ double red[4 * JB];
+  __shared__ double outv[JB];
   const Dims& d = OpSel<OPK>::dims(op);
   const double h = K->hcur;
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
-  double acc[MAXM + 1];
+  double acc[JB];
 #pragma unroll
-  for (int i = 0; i <= MAXM; i++) acc[i] = 0.0;
+  for (int i = 0; i < JB; i++) acc[i] = 0.0;
   double* vj = V.v[j];
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
-    __syncthreads();
-    load_tile<NC>(xs, d, kb.w, h, true, gx0, gy0);
-    __syncthreads();
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
-    if (cx < d.nx && cy < d.ny) {
-      int cidx = cx + d.nx * cy;
-      double y[NC];
-      OpSel<OPK>::apply(op, xs, tx + 1, ty + 1, cx + GC, cy + GC, cidx, y);
+    bool valid = cx < d.nx && cy < d.ny;
+    int cidx = cx + d.nx * cy;
+    __syncthreads();
+    OpSel<OPK>::load(op, sm, kb.w, h, true, gx0, gy0);
+    __syncthreads();
+    double y[NC];
+    OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+    if (valid) {
       OpSel<OPK>::precond(op, cidx, cx + GC, cy + GC, y);
       for (int c = 0; c < NC; c++) {
         long long e = (long long)c * d.N + cidx;
-        double vc = xs[(c * (TY + 2) + ty + 1) * (TX + 2) + tx + 1];
+        double vc = OpSel<OPK>::center(sm, c, tx, ty);
         vj[e] = vc;
         kb.Av[e] = y[c];
 #pragma unroll
-        for (int i = 0; i < MAXM; i++) if (i < j) acc[i] += V.v[i][e] * y[c];
-#pragma unroll
-        for (int i = 0; i <= MAXM; i++) if (i == j) acc[i] += vc * y[c];
+        for (int i = 0; i < JB; i++) {
+          if (i < j) acc[i] += V.v[i][e] * y[c];
+          else if (i == j) acc[i] += vc * y[c];
+        }
       }
     }
   }
-  block_sums<MAXM + 1>(acc, j + 1, red, outv);
+  block_sums<JB>(acc, j + 1, red, outv);
   __syncthreads();
   if ((int)threadIdx.x <= j) kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = outv[threadIdx.x];
-  if (!last_block_arrive(&K->cnt_a)) return;
+}
+
+// One-block finish of the dots: H[i][j] for i <= j.
+__global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const double* partial, int nblk, int j, KrylovDev* K) {
+  if (K->done) return;
   __shared__ double tot[MAXM + 1];
-  final_sums(kb.partial, MAXM + 2, gridDim.x, j + 1, tot);
+  final_sums(partial, MAXM + 2, nblk, j + 1, tot);
   __syncthreads();
   if ((int)threadIdx.x <= j) K->H[threadIdx.x * (MAXM + 1) + j] = tot[threadIdx.x];
-  if (threadIdx.x == 0) K->cnt_a = 0;
 }
 
 // ---------------------------------------------------------------- Arnoldi step j, part 2:
 // w = Av_j - sum_i H[i][j] v_i, H[j+1][j] = ||w||, Givens rotations, convergence.
+template <int JB>
 __global__ void __launch_bounds__(256) k_arnoldi_update(KrylovBufs kb, VPtrs V, int j, int m, long long n, KrylovDev* K) {
   if (K->done) return;
-  __shared__ double Hc[MAXM + 1];
+  __shared__ double Hc[JB];
   __shared__ double red[4];
   __shared__ double res[1];
   if ((int)threadIdx.x <= j) Hc[threadIdx.x] = K->H[threadIdx.x * (MAXM + 1) + j];
   __syncthreads();
+  double hr[JB];
+#pragma unroll
+  for (int i = 0; i < JB; i++) hr[i] = i <= j ? Hc[i] : 0.0;
   double acc = 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
     double w = kb.Av[e];
 #pragma unroll
-    for (int i = 0; i <= MAXM; i++) if (i <= j) w -= Hc[i] * V.v[i][e];
+    for (int i = 0; i < JB; i++) if (i <= j) w -= hr[i] * V.v[i][e];
     kb.w[e] = w;
     acc += w * w;
   }
   block_sums<1>(&acc, 1, red, res);
   if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
-  if (!last_block_arrive(&K->cnt_c)) return;
+}
+
+// One-block finish of the update: H[j+1][j] = ||w||, Givens rotations, convergence (implicit_operators.cpp:857-897).
+__global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partial, int nblk, int j, int m, KrylovDev* K) {
+  if (K->done) return;
   __shared__ double tot[1];
-  final_sums(kb.partial, MAXM + 2, gridDim.x, 1, tot);
+  final_sums(partial, MAXM + 2, nblk, 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
-    K->cnt_c = 0;
     const int L = MAXM + 1;
     double* H = K->H;
     H[(j + 1) * L + j] = sqrt(tot[0]);
@@ -358,22 +469,21 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs V, double* __restri
 template <int OPK>
 __global__ void __launch_bounds__(256) k_op_apply(typename OpSel<OPK>::T op, const double* __restrict__ x, double* __restrict__ Ax) {
   constexpr int NC = OpSel<OPK>::NC;
-  __shared__ double xs[NC * (TY + 2) * (TX + 2)];
+  __shared__ typename OpSel<OPK>::Smem sm;
   const Dims& d = OpSel<OPK>::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
-    __syncthreads();
-    load_tile<NC>(xs, d, x, 1.0, false, gx0, gy0);
-    __syncthreads();
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
-    if (cx < d.nx && cy < d.ny) {
-      int cidx = cx + d.nx * cy;
-      double y[NC];
-      OpSel<OPK>::apply(op, xs, tx + 1, ty + 1, cx + GC, cy + GC, cidx, y);
-      for (int c = 0; c < NC; c++) Ax[(long long)c * d.N + cidx] = y[c];
-    }
+    bool valid = cx < d.nx && cy < d.ny;
+    int cidx = cx + d.nx * cy;
+    __syncthreads();
+    OpSel<OPK>::load(op, sm, x, 1.0, false, gx0, gy0);
+    __syncthreads();
+    double y[NC];
+    OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+    if (valid) for (int c = 0; c < NC; c++) Ax[(long long)c * d.N + cidx] = y[c];
   }
 }
 
@@ -413,12 +523,13 @@ __global__ void __launch_bounds__(256) k_newton_residual(POp op, KrylovBufs kb, 
   if (ef) atomicOr(eflag, 1);
   block_sums<1>(&acc, 1, red, res);
   if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
-  if (!last_block_arrive(&K->cnt_n)) return;
+}
+
+__global__ void __launch_bounds__(256) k_newton_fin(const double* partial, int nblk, KrylovDev* K, int it, double ntol) {
   __shared__ double tot[1];
-  final_sums(kb.partial, MAXM + 2, gridDim.x, 1, tot);
+  final_sums(partial, MAXM + 2, nblk, 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
-    K->cnt_n = 0;
     double r = sqrt(tot[0]);
     if (it == 0) K->newton_res0 = r;
     K->newton_res = r;
@@ -451,11 +562,12 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Dims d, Phys P, const doubl
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sh[w]); c = fmax(c, sh[4 + w]); }
     partial[blockIdx.x * 2] = a; partial[blockIdx.x * 2 + 1] = c;
   }
-  if (!last_block_arrive(&K->cnt_m)) return;
+}
+
+__global__ void k_newton_mbig_fin(Phys P, const double* partial, int nblk, double* S) {
   if (threadIdx.x == 0) {
-    K->cnt_m = 0;
     double a = 1e300, c = -1e300;
-    for (int bl = 0; bl < (int)gridDim.x; bl++) { a = fmin(a, partial[bl * 2]); c = fmax(c, partial[bl * 2 + 1]); }
+    for (int bl = 0; bl < nblk; bl++) { a = fmin(a, partial[bl * 2]); c = fmax(c, partial[bl * 2 + 1]); }
     double Vpe = P.p_floor / (P.gamma - 1.0);
     if (P.eos != IMEX_EOS_IDEAL && c > Vpe) Vpe = c;
     double M_cor = fmax(0.0, Vpe - a) / ((1.0 - P.theta) * P.p_floor);
@@ -496,18 +608,35 @@ void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb
   if (op == 0) {
     int g = grid_for_tiles(mo.d, kb.nblk_apply);
     k_gmres_residual<0><<<g, 256, 0, st>>>(mo, kb, x, b, K, outer, x_is_zero, skip);
+    k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, g, K, outer, skip);
   } else {
     int g = grid_for_tiles(po.d, kb.nblk_apply);
     k_gmres_residual<1><<<g, 256, 0, st>>>(po, kb, x, b, K, outer, x_is_zero, skip);
+    k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, g, K, outer, skip);
   }
+}
+
+template <int OPK, int JB>
+static void launch_apply_jb(const typename OpSel<OPK>::T& o, const KrylovBufs& kb, const VPtrs& V, int j, KrylovDev* K,
+                            int g, hipStream_t st) {
+  k_arnoldi_apply<OPK, JB><<<g, 256, 0, st>>>(o, kb, V, j, K);
 }
 
 void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
                    KrylovDev* K, hipStream_t st) {
   VPtrs V = make_vptrs(kb, hostV);
   const Dims& d = op == 0 ? mo.d : po.d;
-  if (op == 0) k_arnoldi_apply<0><<<grid_for_tiles(d, kb.nblk_apply), 256, 0, st>>>(mo, kb, V, j, K);
-  else k_arnoldi_apply<1><<<grid_for_tiles(d, kb.nblk_apply), 256, 0, st>>>(po, kb, V, j, K);
+  int g = grid_for_tiles(d, kb.nblk_apply);
+  if (op == 0) {
+    if (j < 8) launch_apply_jb<0, 8>(mo, kb, V, j, K, g, st);
+    else if (j < 16) launch_apply_jb<0, 16>(mo, kb, V, j, K, g, st);
+    else launch_apply_jb<0, 32>(mo, kb, V, j, K, g, st);
+  } else {
+    if (j < 8) launch_apply_jb<1, 8>(po, kb, V, j, K, g, st);
+    else if (j < 16) launch_apply_jb<1, 16>(po, kb, V, j, K, g, st);
+    else launch_apply_jb<1, 32>(po, kb, V, j, K, g, st);
+  }
+  k_arnoldi_dots_fin<<<1, 256, 0, st>>>(kb.partial, g, j, K);
 }
 
 void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
@@ -516,7 +645,10 @@ void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* 
   long long n = (long long)nc * d.N;
   long long nb = (n + 255) / 256;
   int g = (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
-  k_arnoldi_update<<<g, 256, 0, st>>>(kb, V, j, m, n, K);
+  if (j < 8) k_arnoldi_update<8><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
+  else if (j < 16) k_arnoldi_update<16><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
+  else k_arnoldi_update<32><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
+  k_arnoldi_givens_fin<<<1, 256, 0, st>>>(kb.partial, g, j, m, K);
 }
 
 void gmres_xupdate(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, double* x, KrylovDev* K, hipStream_t st) {
@@ -534,7 +666,9 @@ void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* A
 
 void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
                      int it, double ntol, int* eflag, hipStream_t st) {
-  k_newton_residual<<<grid_for_tiles(po.d, kb.nblk_apply), 256, 0, st>>>(po, kb, p, b, ng, K, it, ntol, eflag);
+  int g = grid_for_tiles(po.d, kb.nblk_apply);
+  k_newton_residual<<<g, 256, 0, st>>>(po, kb, p, b, ng, K, it, ntol, eflag);
+  k_newton_fin<<<1, 256, 0, st>>>(kb.partial, g, K, it, ntol);
 }
 
 void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u, double* S, double* partial, KrylovDev* K,
@@ -542,6 +676,7 @@ void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u,
   long long nb = (d.N + 255) / 256;
   int g = (int)(nb < 512 ? nb : 512);
   k_newton_mbig<<<g, 256, 0, st>>>(d, P, b, u, S, partial, K, eflag);
+  k_newton_mbig_fin<<<1, 64, 0, st>>>(P, partial, g, S);
 }
 
 void vslope_launch(const Dims& d, const Phys& P, const double* pf, const double* u, const double* S, double* vs, int* eflag,

@@ -67,8 +67,7 @@ void launch_bc_fill(const Dims& d, const PlaneSet& ps, int kind, hipStream_t st)
 // the con -> pri -> con round trip of main.cpp:356/561: u_n = pri2con(pri).
 // mode 0: `uin` holds primitives; mode 1: `uin` holds conservatives (pri = con2pri).
 __global__ void __launch_bounds__(256) k_step_start(Dims d, Phys P, int mode, const double* __restrict__ uin,
-                                                    double* __restrict__ un, double* partial, unsigned* counter,
-                                                    double* S, int* eflag) {
+                                                    double* __restrict__ un, double* partial, int* eflag) {
   __shared__ double sh[16];
   double r[8] = {1e-8, 1e-8, 1e-8, 0.0, 0.0, 0.0, 0.0, 0.0};
   const long long total = (long long)d.NX * d.NY;
@@ -114,13 +113,24 @@ __global__ void __launch_bounds__(256) k_step_start(Dims d, Phys P, int mode, co
       partial[blockIdx.x * 8 + k] = m;
     }
   }
-  if (!last_block_arrive(counter)) return;
+}
+
+// One-block finish of the start-of-step maxima and the dt rule (timestep.cpp:22-66, main.cpp:341, 353).
+__global__ void k_step_finish(Dims d, Phys P, const double* partial, int nblk, double* S) {
+  __shared__ double mm[8];
+  {   // 8 maxima over the block partials, one wave per value (max is order independent)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int k = wid; k < 8; k += nw) {
+      double v = partial[k];
+      for (int b = lane; b < nblk; b += 64) v = fmax(v, partial[b * 8 + k]);
+      v = wave_max(v);
+      if (lane == 0) mm[k] = v;
+    }
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     double m[8];
-    for (int k = 0; k < 8; k++) m[k] = partial[k];
-    for (int b = 1; b < (int)gridDim.x; b++)
-      for (int k = 0; k < 8; k++) m[k] = fmax(m[k], partial[b * 8 + k]);
-    *counter = 0;
+    for (int k = 0; k < 8; k++) m[k] = mm[k];
     int ts = (int)S[S_TIMESTEP] + 1;
     double ch = m[2], v_max = m[7];
     S[S_CHX] = m[0]; S[S_CHY] = m[1]; S[S_CH] = ch;
@@ -298,42 +308,86 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, con
 }
 
 // RusanovC_Update (kind 0, flux.cpp:409-455) and Pressure_flux_Update (kind 1, flux.cpp:290-326)
-// fused with TVDReconstruction.  Face states of a cell c are the reconstruction at
-// the ghost-map image of c, as the reference's bound_cond_u on w_* produces.
-// kind 0 writes u[0..3] (the energy slot of the convective predictor is never read);
-// kind 1 writes u[1..4] (its density equals the convective one bit for bit).
+// fused with TVDReconstruction on 32x8 cell tiles.  Per direction: every cell of the tile
+// (plus one halo cell each side) is reconstructed once into LDS, every face flux is formed
+// once into LDS, then each cell applies its flux difference.  Face states of a cell c are
+// the reconstruction at the ghost-map image of c, as the reference's bound_cond_u on w_*
+// produces.  kind 0 writes u[0..3] (the energy slot of the convective predictor is never
+// read); kind 1 writes u[1..4] (its density equals the convective one bit for bit).
+#define EX_TX 32
+#define EX_TY 8
+#define EX_NW 320            // max cells reconstructed per direction: 34x8 (x) / 32x10 (y)
+#define EX_NF 288            // max faces per direction: 33x8 (x) / 32x9 (y)
+
 __global__ void __launch_bounds__(256) k_explicit(Dims d, Phys P, int kind, const double* __restrict__ Qe,
                                                   const double* __restrict__ Qs, double* __restrict__ u, const double* S) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
-  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  __shared__ double wl[5 * EX_NW], wr[5 * EX_NW];
+  __shared__ double F[5 * EX_NF];
+  const int i0 = GC + blockIdx.x * EX_TX, j0 = GC + blockIdx.y * EX_TY;
+  const int tx = threadIdx.x % EX_TX, ty = threadIdx.x / EX_TX;
   const double h = S[S_ADT], s_max = S[S_AMAX];
-  double F[2][2][5];   // [dir][minus/plus face][var]
+  const int nrec = kind == 0 ? 4 : 5;       // kind 0 fluxes need rho and momenta only
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool own = ci < d.nx + GC && cj < d.ny + GC;
+  const int k0 = kind == 0 ? 0 : 1, k1 = kind == 0 ? 4 : 5;
+  double acc[5];
   for (int dir = 0; dir < 2; dir++) {
-    // face states: L/R of the three cells c-1, c, c+1 along dir
-    double wl[3][NVAR], wr[3][NVAR];
-    for (int m = 0; m < 3; m++) {
-      int ci = dir == 0 ? map_x(d, i - 1 + m) : i;
-      int cj = dir == 0 ? j : map_y(d, j - 1 + m);
-      for (int k = 0; k < 5; k++) {
+    const int W = dir == 0 ? EX_TX + 2 : EX_TX;          // reconstructed cells per row
+    const int nitems = dir == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
+    __syncthreads();
+    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+      int lx = it % W, ly = it / W;
+      int gi = dir == 0 ? i0 + lx - 1 : i0 + lx;
+      int gj = dir == 0 ? j0 + ly : j0 + ly - 1;
+      if (gi > d.nx + GC || gj > d.ny + GC) continue;      // beyond a partial tile: unused
+      int mi = dir == 0 ? map_x(d, gi) : gi;
+      int mj = dir == 0 ? gj : map_y(d, gj);
+      for (int k = 0; k < nrec; k++) {
         const double* q = Qe + k * d.PS;
         double a[5];
-        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? q[pidx(d, ci - 2 + o, cj)] : q[pidx(d, ci, cj - 2 + o)];
-        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl[m][k], wr[m][k]);
-      }
-      if (kind == 1) {
-        int bi = dir == 0 ? i - 1 + m : i, bj = dir == 0 ? j : j - 1 + m;
-        for (int k = 5; k < 8; k++) { double b = u[k * d.PS + pidx(d, bi, bj)]; wl[m][k] = b; wr[m][k] = b; }
-      } else {
-        for (int k = 5; k < 8; k++) { wl[m][k] = 0.0; wr[m][k] = 0.0; }
+        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? q[pidx(d, mi - 2 + o, mj)] : q[pidx(d, mi, mj - 2 + o)];
+        double L, R;
+        recon5(P.recon, dir == 1 ? P.central_y : 0, a, L, R);
+        wl[k * EX_NW + it] = L;
+        wr[k * EX_NW + it] = R;
       }
     }
-    gl_flux(dir, kind, wr[0], wl[1], s_max, F[dir][0]);
-    gl_flux(dir, kind, wr[1], wl[2], s_max, F[dir][1]);
+    __syncthreads();
+    const int nf = dir == 0 ? (EX_TX + 1) * EX_TY : EX_TX * (EX_TY + 1);
+    const int FW = dir == 0 ? EX_TX + 1 : EX_TX;          // faces per row
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+      int fx = f % FW, fy = f / FW;
+      // face between local cells (left, right) along dir; item indices into wl/wr
+      int itL, itR, gli, glj, gri, grj;
+      if (dir == 0) { itL = fy * W + fx; itR = fy * W + fx + 1; gli = i0 + fx - 1; gri = i0 + fx; glj = grj = j0 + fy; }
+      else { itL = fy * W + fx; itR = (fy + 1) * W + fx; gli = gri = i0 + fx; glj = j0 + fy - 1; grj = j0 + fy; }
+      if (gri > d.nx + GC || grj > d.ny + GC) continue;
+      double QL[NVAR], QR[NVAR];
+      for (int k = 0; k < 5; k++) {
+        QL[k] = k < nrec ? wr[k * EX_NW + itL] : 0.0;
+        QR[k] = k < nrec ? wl[k * EX_NW + itR] : 0.0;
+      }
+      if (kind == 1) {
+        for (int k = 5; k < 8; k++) { QL[k] = u[k * d.PS + pidx(d, gli, glj)]; QR[k] = u[k * d.PS + pidx(d, gri, grj)]; }
+      } else {
+        for (int k = 5; k < 8; k++) { QL[k] = 0.0; QR[k] = 0.0; }
+      }
+      double Fo[5];
+      gl_flux(dir, kind, QL, QR, s_max, Fo);
+      for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[k];
+    }
+    __syncthreads();
+    if (own) {
+      long long c = pidx(d, ci, cj);
+      if (dir == 0) {
+        int fm = ty * FW + tx, fp = fm + 1;
+        for (int k = k0; k < k1; k++) acc[k] = Qs[k * d.PS + c] - h / d.dx * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
+      } else {
+        int fm = ty * FW + tx, fp = fm + FW;
+        for (int k = k0; k < k1; k++) u[k * d.PS + c] = acc[k] - h / d.dy * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
+      }
+    }
   }
-  long long c = pidx(d, i, j);
-  int k0 = kind == 0 ? 0 : 1, k1 = kind == 0 ? 4 : 5;
-  for (int k = k0; k < k1; k++)
-    u[k * d.PS + c] = Qs[k * d.PS + c] - h / d.dx * (F[0][1][k] - F[0][0][k]) - h / d.dy * (F[1][1][k] - F[1][0][k]);
 }
 
 // ================================================================ magnetic subsystem
