@@ -26,10 +26,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(name, steps, warmup, precision):
+def run_gpu(name, steps, warmup, precision, schedule):
     import imex_amd as im
     p, kw = im.case_of(name)
     p.precision = precision
+    p.schedule = schedule
     u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
     s = im.Solver(p)
     s.set_state(u, bx, by)
@@ -38,20 +39,30 @@ def run_gpu(name, steps, warmup, precision):
         s.step(warmup)
     state = s.get_state()
     s.reset_stats()
-    s.profile_enable(True)
     s.sync()
     log(f"[bench] timing {steps} steps")
     t0 = time.perf_counter()
     s.step_async(steps)
     s.sync()
     t1 = time.perf_counter()
+    stats = s.stats()
+    # Per-kernel roofline: the same steps again, from the same post-warmup state, with every
+    # launch bracketed by HIP events on the solver's stream.  Event timing needs launches queued
+    # one by one, so this pass runs the budgeted schedule without the step graph (its first step,
+    # before any iteration budget is known, is host scheduled); its wall time is reported apart.
+    log(f"[bench] per-kernel event timing over the same {steps} steps")
+    uc, up, bxs, bys, sc = state
+    s.set_state(up, bxs, bys, sc["t"], sc["timestep"], sc["prev_dt"])
+    s.profile_enable(True)
+    s.sync()
+    t2 = time.perf_counter()
+    s.step_async(steps)
+    s.sync()
+    t3 = time.perf_counter()
     prof = s.profile_read()
     s.profile_enable(False)
-    stats = s.stats()
-    uc, up, bxf, byf, sc = s.get_state()
     s.close()
-    elapsed = t1 - t0
-    return dict(p=p, elapsed=elapsed, prof=prof, stats=stats, state=state, final=(uc, sc))
+    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state)
 
 
 def cpu_baseline(name, state, budget_steps=1):
@@ -68,23 +79,42 @@ def cpu_baseline(name, state, budget_steps=1):
     return p.nx * p.ny * budget_steps / (t1 - t0) / 1e6, t1 - t0
 
 
+def copy_calibration(mib):
+    """One read stream and one write stream of known size: the scale for rocprof's FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ot1024")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", default="graph", choices=["graph", "budget", "host"])
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU domain decomposition is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
     import imex_amd as im
     prec = im.PREC_FP64 if args.precision == "fp64" else im.PREC_MIXED
-    r = run_gpu(args.workload, args.steps, args.warmup, prec)
+    sched = {"graph": im.SCHED_GRAPH, "budget": im.SCHED_BUDGET, "host": im.SCHED_HOST}[args.schedule]
+    r = run_gpu(args.workload, args.steps, args.warmup, prec, sched)
     p = r["p"]
     cells = p.nx * p.ny
     value = cells * args.steps / r["elapsed"] / 1e6
@@ -99,7 +129,8 @@ def main():
         gbs = k["bytes"] / (k["total_ms"] * 1e-3) / 1e9 if k["total_ms"] > 0 else 0.0
         log(f"  {k['name']:22s} n={k['launches']:6d} total {k['total_ms']:9.3f} ms  avg {k['total_ms'] / k['launches'] * 1e3:8.1f} us"
             f"  {gbs:8.1f} GB/s")
-    log(f"[bench] kernel time {tot_ms:.1f} ms of {r['elapsed'] * 1e3:.1f} ms wall; stats {r['stats']}")
+    log(f"[bench] profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
+        f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
     out = {
         "metric": "Mcell-steps/s at 1024^2 WENO5 Orszag-Tang (IMEX-RK2 step, one MI355X)",
         "value": round(value, 3),
@@ -119,6 +150,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "gmres": {k: r["stats"][k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters")},
+        "schedule": {"mode": args.schedule, "budget_steps": r["stats"]["budget_steps"],
+                     "budget_reruns": r["stats"]["budget_reruns"], "graph_builds": r["stats"]["graph_builds"],
+                     "profiled_pass_ms_per_step": round(r["prof_elapsed"] / args.steps * 1e3, 3)},
     }
     if not args.no_cpu:
         log("[bench] CPU oracle sample")

@@ -18,6 +18,7 @@ CT_CONSISTENT, CT_PLAIN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN = 0, 1, 2
 CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
+SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
 WARN_GMRES_MAXOUTER, WARN_NEWTON_MAXITER, WARN_POSITIVITY, WARN_EOS_MAXITER = 1, 2, 4, 8
 
 
@@ -38,6 +39,7 @@ class Params(C.Structure):
         ("shock_kappa1", C.c_double), ("time_stop", C.c_double),
         ("ct_mode", C.c_int), ("precision", C.c_int),
         ("mixed_inner_tol", C.c_double),
+        ("schedule", C.c_int), ("budget_margin", C.c_int),
     ]
 
 
@@ -55,7 +57,8 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in ("steps", "mag_solves", "mag_gmres_iters", "mag_gmres_restarts",
                                             "p_solves", "p_gmres_iters", "p_gmres_restarts", "newton_iters",
                                             "op_applies_mag", "op_applies_p", "mixed_refinements")] + \
-               [("warnings", C.c_int), ("max_floored", C.c_int), ("min_p", C.c_double), ("div_b", C.c_double)]
+               [("warnings", C.c_int), ("max_floored", C.c_int), ("min_p", C.c_double), ("div_b", C.c_double)] + \
+               [(n, C.c_longlong) for n in ("budget_steps", "budget_reruns", "graph_builds")]
 
 
 class KernelProfile(C.Structure):

@@ -51,12 +51,51 @@ __global__ void k_pstats(long long n, const double* p, double pfloor, double* pa
   }
 }
 __global__ void k_pstats_fin(const double* partial, int nblk, double thr, double* acc) {
+  // min, and a sum of small integers held in doubles: both exact in any order, one wave reduces
+  double m = 1e300, c = 0;
+  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, partial[2 * b]); c += partial[2 * b + 1]; }
+  m = wave_min(m); c = wave_sum(c);
   if (threadIdx.x == 0) {
-    double m = 1e300, c = 0;
-    for (int b = 0; b < nblk; b++) { m = fmin(m, partial[2 * b]); c += partial[2 * b + 1]; }
     acc[0] = fmin(acc[0], m);
     acc[1] = fmax(acc[1], c);
     if (m < thr) acc[2] = 1.0;
+  }
+}
+
+__global__ void k_fill_skip(long long n, double* p, double v, const int* skip) {
+  if (skip && *skip) return;
+  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e < n) p[e] = v;
+}
+
+// ---------------------------------------------------------------- budgeted-schedule bookkeeping
+// A budgeted step runs every GMRES solve for a fixed number of Arnoldi launches and every
+// nested-Newton solve for a fixed number of Jacobian solves; kernels queued past convergence
+// exit at once.  ctl[0] collects failures (a solve that had not converged when its budget ran
+// out), ctl[1..] the iteration count of each solve, in launch order.  The scalars, positivity
+// statistics and EOS flag are saved at the start so that a failed step can be rerun exactly.
+__global__ void k_budget_begin(const double* S, const double* pst, const int* eflag, double* save, int* esave, int* ctl) {
+  const int t = threadIdx.x;
+  if (t < S_NSLOTS) save[t] = S[t];
+  if (t < 4) save[S_NSLOTS + t] = pst[t];
+  if (t == 0) { esave[0] = eflag[0]; ctl[0] = 0; }
+}
+__global__ void k_budget_restore(double* S, double* pst, int* eflag, const double* save, const int* esave) {
+  const int t = threadIdx.x;
+  if (t < S_NSLOTS) S[t] = save[t];
+  if (t < 4) pst[t] = save[S_NSLOTS + t];
+  if (t == 0) eflag[0] = esave[0];
+}
+__global__ void k_budget_gmres_check(const KrylovDev* K, int* ctl, int slot) {
+  if (threadIdx.x == 0) {
+    ctl[slot] = K->steps;
+    if (!K->converged) ctl[0] |= 1;
+  }
+}
+__global__ void k_budget_newton_check(const KrylovDev* K, int* ctl, int slot) {
+  if (threadIdx.x == 0) {
+    ctl[slot] = K->newton_conv ? K->newton_first : -1;
+    if (!K->newton_conv) ctl[0] |= 2;
   }
 }
 
@@ -73,13 +112,13 @@ struct StageBind {
 enum KClass {
   KC_START, KC_COMBINE, KC_QEPREP, KC_SHOCK, KC_EXPL_CONV, KC_EXPL_PRESS, KC_MAG_RHS, KC_MAG_APPLY, KC_MAG_UPDATE,
   KC_MAG_RES, KC_MAG_XUPD, KC_P_APPLY, KC_P_UPDATE, KC_P_RES, KC_P_XUPD, KC_NEWTON_RES, KC_EMF, KC_P_RHS, KC_P_MOM,
-  KC_P_ENERGY, KC_CT, KC_BC, KC_MISC, KC_EARLY_EXIT, KC_N
+  KC_P_ENERGY, KC_CT, KC_BC, KC_MISC, KC_KFIN, KC_EARLY_EXIT, KC_N
 };
 static const char* kc_names[KC_N] = {
   "step_start", "stage_combine", "qe_prep", "shock", "explicit_conv", "explicit_press", "mag_rhs", "mag_arnoldi_apply",
   "mag_arnoldi_update", "mag_gmres_residual", "mag_gmres_xupdate", "p_arnoldi_apply", "p_arnoldi_update",
   "p_gmres_residual", "p_gmres_xupdate", "newton_residual", "emf_flux", "p_rhs", "p_momentum", "p_energy", "ct",
-  "ghost_fill", "misc", "early_exit"};
+  "ghost_fill", "misc", "krylov_finish", "early_exit"};
 
 struct ProfRec { int cls; int e0, e1; double bytes; };
 struct Prof {
@@ -91,6 +130,24 @@ struct Prof {
   double ms[KC_N] = {0}, bytes[KC_N] = {0};
   long long n[KC_N] = {0};
 };
+
+// One budgeted GMRES (kind 0) or nested-Newton (kind 1) solve of the queued step.
+struct BudgetRec {
+  int kind, op, slot;
+  int parent = -1, it = -1;    // enclosing Newton solve (index into brecs) and its iteration
+  std::vector<int> ra, ru, rfa, rfu;      // profiler records of the Arnoldi launches and their finishes
+  int rres = -1, rresf = -1, rx = -1;
+  std::vector<int> rnres, rnresf, rfill, raxpy;   // Newton: residual (+finish) / dp fill / update per iteration
+};
+struct GraphEntry {
+  int key[5];
+  hipGraphExec_t exec = nullptr;
+  std::vector<BudgetRec> brecs;
+  int nslot = 1;
+  long long used = 0;
+};
+#define BHIST 8
+#define MAX_GRAPHS 6
 
 struct imex_ctx {
   Prof prof;
@@ -118,7 +175,7 @@ struct imex_ctx {
   unsigned* cnt_pstats = nullptr;
   KrylovDev Klast;             // status of the last GMRES cycle read by the host
   hipEvent_t ev[2];
-  int state_mode = 0;          // 0: u_pri holds the state; 1: U2 holds the final conservative state
+  int state_mode = 0;          // 0: u_pri holds the state; 1: Us holds the conservative state
   StageBind cur;
   bool qe_ready = false, hhat_ready = false;
   imex_stats stats{};
@@ -126,6 +183,22 @@ struct imex_ctx {
   int pred_newton = 1;         // Jacobian solves of the last nested-Newton solve
   int nblk_start = 1024;
   KrylovBufs kb;
+  // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo
+  double *U3 = nullptr, *Us = nullptr, *Uo = nullptr;
+  // budgeted schedule
+  bool budget = false;         // the step being queued runs on budgets
+  double* bsave = nullptr;     // S, positivity statistics at the start of the budgeted step
+  int* esave = nullptr;
+  int* ctl = nullptr;          // device: failure flags + per-solve iteration log
+  int* ctlh = nullptr;         // pinned host copy
+  int nslot = 1;
+  std::vector<BudgetRec> brecs;
+  int bud[3] = {0, 0, 0};      // Arnoldi steps per magnetic / pressure solve, Jacobian solves per Newton solve
+  int hist[3][BHIST];          // per-step maxima of the recent steps
+  int hist_n = 0, hist_pos = 0;
+  int step_mx[3] = {0, 0, 0};  // maxima of the step being run under host scheduling
+  std::vector<GraphEntry> graphs;
+  long long graph_clock = 0;
 };
 
 static Dims make_dims(const imex_params& p) {
@@ -215,12 +288,54 @@ static POp p_op(imex_ctx* c, int jacobian, int precond) {
   return o;
 }
 
+// ---------------------------------------------------------------- GMRES, budgeted schedule
+// One restart cycle of GMRES_Solver (implicit_operators.cpp:823-911) with a fixed number of
+// Arnoldi launches: the kernels are the ones the host-polled loop below uses, so a solve that
+// converges within its budget gives the same result bit for bit.  Launches past convergence
+// exit at once; a solve still unconverged at the end of its budget flags the step for a rerun.
+static int run_gmres_budget(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b,
+                            int x_is_zero, const int* skip, int parent, int it) {
+  const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
+  const int nc = op == 0 ? 3 : 1;
+  const int B = std::min(m, std::max(1, c->bud[op]));
+  BudgetRec r;
+  r.kind = 0; r.op = op; r.slot = c->nslot++; r.parent = parent; r.it = it;
+  const double res_fields = op == 0 ? (x_is_zero ? 6 : 13) : (x_is_zero ? 2 : 5) + po.precond + po.jacobian;
+  int e0 = prof_begin(c);
+  gmres_residual(op, mo, po, c->kb, x, b, c->K, 0, x_is_zero, skip, c->st);
+  r.rres = prof_end(c, e0, op == 0 ? KC_MAG_RES : KC_P_RES, res_fields);
+  e0 = prof_begin(c);
+  gmres_residual_fin(c->d, c->kb, c->K, 0, skip, c->st);
+  r.rresf = prof_end(c, e0, KC_KFIN, 0);
+  for (int j = 0; j < B; j++) {
+    e0 = prof_begin(c);
+    arnoldi_apply(op, mo, po, c->kb, c->V, j, c->K, c->st);
+    r.ra.push_back(prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY, op == 0 ? 13 + 3 * j : 6 + j + po.jacobian));
+    e0 = prof_begin(c);
+    arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
+    r.rfa.push_back(prof_end(c, e0, KC_KFIN, 0));
+    e0 = prof_begin(c);
+    arnoldi_update(nc, c->d, c->kb, c->V, j, m, c->K, c->st);
+    r.ru.push_back(prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, op == 0 ? 9 + 3 * j : 3 + j));
+    e0 = prof_begin(c);
+    arnoldi_update_fin(nc, c->d, c->kb, j, m, c->K, c->st);
+    r.rfu.push_back(prof_end(c, e0, KC_KFIN, 0));
+  }
+  e0 = prof_begin(c);
+  gmres_xupdate(nc, c->d, c->kb, c->V, x, c->K, c->st);
+  k_budget_gmres_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, r.slot);
+  r.rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (B + 2));
+  c->brecs.push_back(r);
+  return 0;
+}
+
 // ---------------------------------------------------------------- GMRES host loop
 // GMRES_Solver (implicit_operators.cpp:793-919).  Arnoldi steps are queued in chunks; the
 // convergence flag of chunk k is read (async copy + event) while chunk k+1 is already queued.
 static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b, int x_is_zero,
                      imex_solve_info* info, long long* it_acc, long long* rs_acc, const int* skip = nullptr,
                      int first_chunk = 0) {
+  if (c->budget) return run_gmres_budget(c, op, mo, po, x, b, x_is_zero, skip, -1, -1);
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
   const long long dim = (long long)nc * c->d.N;
@@ -238,8 +353,9 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
     const int xz = (x_is_zero && outer == 0) ? 1 : 0;
     const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
     LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(op, mo, po, c->kb, x, b, c->K, (int)outer, xz, outer == 0 ? skip : nullptr, c->st));
+    LAUNCH(KC_KFIN, 0, gmres_residual_fin(c->d, c->kb, c->K, (int)outer, outer == 0 ? skip : nullptr, c->st));
     const long long gen0 = c->prof.gen;
-    int recs_a[MAXM], recs_u[MAXM];
+    int recs_a[MAXM], recs_u[MAXM], recs_fa[MAXM], recs_fu[MAXM];
     int j = 0, slot = 0, last = -1;
     bool stop = false;
     while (!stop) {
@@ -250,8 +366,14 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
         arnoldi_apply(op, mo, po, c->kb, c->V, j, c->K, c->st);
         recs_a[j] = prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY, op == 0 ? 13 + 3 * j : 6 + j + po.jacobian);
         e0 = prof_begin(c);
+        arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
+        recs_fa[j] = prof_end(c, e0, KC_KFIN, 0);
+        e0 = prof_begin(c);
         arnoldi_update(nc, c->d, c->kb, c->V, j, m, c->K, c->st);
         recs_u[j] = prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, op == 0 ? 9 + 3 * j : 3 + j);
+        e0 = prof_begin(c);
+        arnoldi_update_fin(nc, c->d, c->kb, j, m, c->K, c->st);
+        recs_fu[j] = prof_end(c, e0, KC_KFIN, 0);
       }
       HIPCHK(hipMemcpyAsync(&c->Kh[slot].tol, &c->K->tol, sizeof(KrylovDev) - offsetof(KrylovDev, tol), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipEventRecord(c->ev[slot], c->st));
@@ -274,7 +396,7 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
     }
     if (c->prof.on && c->prof.gen == gen0) {      // launches queued past convergence exited at once
       auto idle = [&](int r) { if (r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
-      for (int jj = ks.steps; jj < j; jj++) { idle(recs_a[jj]); idle(recs_u[jj]); }
+      for (int jj = ks.steps; jj < j; jj++) { idle(recs_a[jj]); idle(recs_u[jj]); idle(recs_fa[jj]); idle(recs_fu[jj]); }
       if (ks.jlast < 0) idle(rx);
     }
     total += ks.steps;
@@ -287,12 +409,63 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   if (it_acc) *it_acc += total;
   if (rs_acc) *rs_acc += outer;
   if (total > 0 && outer == 0) c->pred[op] = total;
+  c->step_mx[op] = std::max(c->step_mx[op], outer == 0 ? total : MAXM + 1);
   if (info) { info->iters = total; info->restarts = (int)outer; info->res0 = res0; info->res = res; info->status = rc; }
   return rc;
 }
 
+// ---------------------------------------------------------------- nested Newton, budgeted schedule
+// Pressure_NestedNewton (implicit_operators.cpp:625-738) with a fixed number of Jacobian solves:
+// residual, solve, update per iteration, then the residual that must have converged.  Once a
+// residual has converged, the later residuals, fills, solves and updates of this Newton solve
+// skip on the device.
+static int run_newton_budget(imex_ctx* c, const double* b, double* p) {
+  const Dims& d = c->d;
+  if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL)
+    return run_gmres_budget(c, 1, mag_op(c), p_op(c, 0, 1), p, b, 0, nullptr, -1, -1);
+  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(d, c->P, b, c->cur.u, c->S, c->partial, c->K, c->eflag, c->st));
+  const int NB = std::min(c->prm.newton_max, std::max(1, c->bud[2]));
+  const int self = (int)c->brecs.size();
+  {
+    BudgetRec nr;
+    nr.kind = 1; nr.op = 1; nr.slot = c->nslot++;
+    c->brecs.push_back(nr);
+  }
+  const int* conv = &c->K->newton_conv;
+  for (int it = 0; it <= NB; it++) {
+    POp carrier = p_op(c, 0, 0);
+    int e0 = prof_begin(c);
+    newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st);
+    int rr = prof_end(c, e0, KC_NEWTON_RES, 6);
+    c->brecs[self].rnres.push_back(rr);
+    e0 = prof_begin(c);
+    newton_residual_fin(d, c->kb, c->K, it, c->prm.newton_tol, c->st);
+    rr = prof_end(c, e0, KC_KFIN, 0);
+    c->brecs[self].rnresf.push_back(rr);
+    if (it == NB) break;
+    if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
+    e0 = prof_begin(c);
+    k_fill_skip<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0, conv);
+    rr = prof_end(c, e0, KC_MISC, 1);
+    c->brecs[self].rfill.push_back(rr);
+    run_gmres_budget(c, 1, mag_op(c), p_op(c, 1, 1), c->dp, c->ng, 1, conv, self, it);
+    e0 = prof_begin(c);
+    axpy_launch(d.N, p, c->dp, conv, c->st);
+    rr = prof_end(c, e0, KC_MISC, 3);
+    c->brecs[self].raxpy.push_back(rr);
+  }
+  k_budget_newton_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, c->brecs[self].slot);
+  if (c->prm.floor_on) {
+    long long g = std::min<long long>(nb(d.N), 256);
+    LAUNCH(KC_MISC, 1, (k_pstats<<<(unsigned)g, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->partial),
+                        k_pstats_fin<<<1, 64, 0, c->st>>>(c->partial, (int)g, c->prm.theta * c->prm.p_floor - 1e-14, c->pst)));
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------- nested Newton (implicit_operators.cpp:625-738)
 static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* info) {
+  if (c->budget) return run_newton_budget(c, b, p);
   int rc = 0;
   const Dims& d = c->d;
   if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL) {
@@ -311,6 +484,7 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
     // has converged, and the host learns it from the GMRES status it polls anyway
     POp carrier = p_op(c, 0, 0);
     LAUNCH(KC_NEWTON_RES, 6, newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st));
+    LAUNCH(KC_KFIN, 0, newton_residual_fin(d, c->kb, c->K, it, c->prm.newton_tol, c->st));
     if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
     LAUNCH(KC_MISC, 1, (k_fill<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0)));
     POp jac = p_op(c, 1, 1);
@@ -329,6 +503,7 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
   }
   c->stats.newton_iters += it;
   c->pred_newton = std::max(1, it);
+  c->step_mx[2] = std::max(c->step_mx[2], it);
   if (it == c->prm.newton_max) { rc |= IMEX_WARN_NEWTON_MAXITER; c->stats.warnings |= IMEX_WARN_NEWTON_MAXITER; }
   if (c->prm.floor_on) {
     long long g = std::min<long long>(nb(d.N), 256);
@@ -370,7 +545,7 @@ static int sub_mag_update_b(imex_ctx* c) {
   MagOp mo = mag_op(c);
   POp po = p_op(c, 0, 0);
   int rc = run_gmres(c, 0, mo, po, c->xk, c->bk, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
-  c->stats.mag_solves++;
+  if (!c->budget) c->stats.mag_solves++;
   long long tot = (long long)d.NX * d.NY;
   LAUNCH(KC_MISC, 6, (k_mag_writeback<<<nb(tot), 256, 0, c->st>>>(d, c->xk, c->cur.u)));
   LAUNCH(KC_EMF, 12, (k_emf_flux<<<grid2(d.NX, d.NY, 64, 4), dim3(64, 4), 0, c->st>>>(d, c->cur.u, c->cur.Qe, c->pe, c->fx, c->fy, c->S)));
@@ -447,7 +622,7 @@ static void bind_stage0(imex_ctx* c) {
   c->cur.bxqs = c->bx_n; c->cur.byqs = c->by_n; c->cur.bxo = c->bx1; c->cur.byo = c->by1;
 }
 static void bind_stage1(imex_ctx* c) {
-  c->cur.u = c->U2; c->cur.Qe = c->Qe; c->cur.Qs = c->Qs;
+  c->cur.u = c->Uo; c->cur.Qe = c->Qe; c->cur.Qs = c->Qs;
   c->cur.bxqs = c->bxqs; c->cur.byqs = c->byqs; c->cur.bxo = c->bx2; c->cur.byo = c->by2;
 }
 
@@ -455,13 +630,15 @@ static int step_start(imex_ctx* c) {
   const Dims& d = c->d;
   long long tot = (long long)d.NX * d.NY;
   long long g = std::min<long long>(nb(tot), c->nblk_start);
-  const double* in = c->state_mode ? c->U2 : c->u_pri;
+  const double* in = c->state_mode ? c->Us : c->u_pri;
   LAUNCH(KC_START, 16, (k_step_start<<<(unsigned)g, 256, 0, c->st>>>(d, c->P, c->state_mode, in, c->u_n, c->partial, c->eflag),
                         k_step_finish<<<1, 256, 0, c->st>>>(d, c->P, c->partial, (int)g, c->S)));
   return 0;
 }
 
-static int one_step(imex_ctx* c) {
+// Everything a step queues; the step's input (Us or u_pri, bx_n, by_n) is only read, so a
+// discarded step can be queued again from the same input.
+static int step_body(imex_ctx* c) {
   int rc = 0;
   const Dims& d = c->d;
   step_start(c);
@@ -471,9 +648,134 @@ static int one_step(imex_ctx* c) {
   LAUNCH(KC_COMBINE, 38, (k_stage_combine<<<nb(tot), 256, 0, c->st>>>(d, c->u_n, c->U1, c->Qe, c->Qs, c->bx_n, c->bx1, c->by_n, c->by1, c->bxqs, c->byqs, c->S)));
   bind_stage1(c);
   rc |= run_stage(c);
-  std::swap(c->bx_n, c->bx2); std::swap(c->by_n, c->by2);
+  return rc;
+}
+
+static void step_commit(imex_ctx* c) {
+  std::swap(c->bx_n, c->bx2); std::swap(c->by_n, c->by2); std::swap(c->Us, c->Uo);
   c->state_mode = 1;
   c->stats.steps++;
+}
+
+static void hist_push(imex_ctx* c, const int mx[3]) {
+  for (int k = 0; k < 3; k++) c->hist[k][c->hist_pos] = mx[k];
+  c->hist_pos = (c->hist_pos + 1) % BHIST;
+  c->hist_n = std::min(c->hist_n + 1, BHIST);
+}
+
+// Budgets: the largest count of the recent steps, plus budget_margin Arnoldi steps per GMRES
+// solve.  Returns false (host scheduling) until there is history, or when a recent solve
+// needed a restart.
+static bool choose_budgets(imex_ctx* c) {
+  if (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0) return false;
+  const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
+  int mx[3] = {0, 0, 0};
+  for (int i = 0; i < c->hist_n; i++) for (int k = 0; k < 3; k++) mx[k] = std::max(mx[k], c->hist[k][i]);
+  if (mx[0] >= m || mx[1] >= m) return false;
+  c->bud[0] = std::min(m, std::max(1, mx[0] + c->prm.budget_margin));
+  c->bud[1] = std::min(m, std::max(1, mx[1] + c->prm.budget_margin));
+  c->bud[2] = std::max(1, mx[2]);
+  return true;
+}
+
+// Reads the step's log: returns 1 if a solve outran its budget, else books the iteration counts
+// and, with the profiler on, moves launches that found their solve converged to early_exit.
+static int budget_finish(imex_ctx* c, long long gen0) {
+  HIPCHK(hipMemcpyAsync(c->ctlh, c->ctl, (size_t)c->nslot * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipEventRecord(c->ev[0], c->st));
+  HIPCHK(hipEventSynchronize(c->ev[0]));
+  if (c->ctlh[0]) return 1;
+  const bool fix = c->prof.on && c->prof.gen == gen0;
+  auto idle = [&](int r) { if (fix && r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
+  int mx[3] = {0, 0, 0};
+  for (const BudgetRec& r : c->brecs) {
+    if (r.kind == 1) {
+      const int nf = c->ctlh[r.slot];
+      c->stats.newton_iters += nf;
+      mx[2] = std::max(mx[2], nf);
+      for (size_t i = 0; i < r.rnres.size(); i++) if ((int)i > nf) { idle(r.rnres[i]); idle(r.rnresf[i]); }
+      for (size_t i = 0; i < r.rfill.size(); i++) if ((int)i >= nf) { idle(r.rfill[i]); idle(r.raxpy[i]); }
+      continue;
+    }
+    const int steps = c->ctlh[r.slot];
+    const bool skipped = r.parent >= 0 && r.it >= c->ctlh[c->brecs[r.parent].slot];
+    if (skipped) {
+      idle(r.rres); idle(r.rresf); idle(r.rx);
+      for (size_t j = 0; j < r.ra.size(); j++) { idle(r.ra[j]); idle(r.ru[j]); idle(r.rfa[j]); idle(r.rfu[j]); }
+      continue;
+    }
+    for (size_t j = (size_t)steps; j < r.ra.size(); j++) { idle(r.ra[j]); idle(r.ru[j]); idle(r.rfa[j]); idle(r.rfu[j]); }
+    if (steps == 0) idle(r.rx);
+    mx[r.op] = std::max(mx[r.op], steps);
+    if (r.op == 0) { c->stats.mag_solves++; c->stats.mag_gmres_iters += steps; }
+    else { c->stats.p_solves++; c->stats.p_gmres_iters += steps; }
+  }
+  hist_push(c, mx);
+  return 0;
+}
+
+// Queue (or replay) the budgeted step as a captured hipGraph.  The graph depends on the budgets,
+// on which of the two state/face buffer sets is current and on the state mode.
+static int budget_graph_launch(imex_ctx* c) {
+  const int key[5] = {c->bud[0], c->bud[1], c->bud[2], c->Us == c->U2 ? 0 : 1, c->state_mode};
+  GraphEntry* g = nullptr;
+  for (GraphEntry& e : c->graphs) if (std::memcmp(e.key, key, sizeof(key)) == 0) { g = &e; break; }
+  if (!g) {
+    if ((int)c->graphs.size() >= MAX_GRAPHS) {     // drop the least recently used graph
+      size_t lru = 0;
+      for (size_t i = 1; i < c->graphs.size(); i++) if (c->graphs[i].used < c->graphs[lru].used) lru = i;
+      (void)hipGraphExecDestroy(c->graphs[lru].exec);
+      c->graphs.erase(c->graphs.begin() + (long)lru);
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int rc = step_body(c);
+    hipError_t e1 = hipStreamEndCapture(c->st, &graph);
+    if (rc < 0) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e1 != hipSuccess) return set_err(IMEX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e1));
+    GraphEntry ne;
+    std::memcpy(ne.key, key, sizeof(key));
+    hipError_t e2 = hipGraphInstantiate(&ne.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) return set_err(IMEX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    ne.brecs = c->brecs;
+    ne.nslot = c->nslot;
+    c->graphs.push_back(ne);
+    g = &c->graphs.back();
+    c->stats.graph_builds++;
+  }
+  g->used = ++c->graph_clock;
+  c->brecs = g->brecs;
+  c->nslot = g->nslot;
+  HIPCHK(hipGraphLaunch(g->exec, c->st));
+  return 0;
+}
+
+static int one_step(imex_ctx* c) {
+  int rc = 0;
+  if (choose_budgets(c)) {
+    const long long gen0 = c->prof.gen;
+    c->budget = true;
+    c->nslot = 1;
+    c->brecs.clear();
+    k_budget_begin<<<1, 64, 0, c->st>>>(c->S, c->pst, c->eflag, c->bsave, c->esave, c->ctl);
+    if (c->prm.schedule == IMEX_SCHED_GRAPH && !c->prof.on) rc = budget_graph_launch(c);
+    else rc = step_body(c);
+    c->budget = false;
+    if (rc < 0) return rc;
+    if (c->nslot > 256) return set_err(IMEX_ERR_ARG, "too many solves in one step");
+    int f = budget_finish(c, gen0);
+    if (f < 0) return f;
+    c->stats.budget_steps++;
+    if (f == 0) { step_commit(c); return rc; }
+    // a solve outran its budget: restore the step's scalars and rerun it under host scheduling
+    c->stats.budget_reruns++;
+    k_budget_restore<<<1, 64, 0, c->st>>>(c->S, c->pst, c->eflag, c->bsave, c->esave);
+  }
+  c->step_mx[0] = c->step_mx[1] = c->step_mx[2] = 0;
+  rc = step_body(c);
+  hist_push(c, c->step_mx);
+  step_commit(c);
   return rc;
 }
 
@@ -585,6 +887,8 @@ int imex_default_params(imex_params* p) {
   p->ct_mode = IMEX_CT_CONSISTENT;
   p->precision = IMEX_PREC_FP64;
   p->mixed_inner_tol = 1e-6;
+  p->schedule = IMEX_SCHED_GRAPH;
+  p->budget_margin = 1;
   return 0;
 }
 
@@ -672,6 +976,8 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
     return set_err(IMEX_ERR_UNSUPPORTED, "reconstruction must be Minmod, VanLeer or WENO5");
   if (prm->precision != IMEX_PREC_FP64 && prm->precision != IMEX_PREC_MIXED)
     return set_err(IMEX_ERR_ARG, "bad precision");
+  if (prm->schedule != IMEX_SCHED_GRAPH && prm->schedule != IMEX_SCHED_BUDGET && prm->schedule != IMEX_SCHED_HOST)
+    return set_err(IMEX_ERR_ARG, "bad schedule");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(IMEX_ERR_HIP, "no HIP device");
   imex_ctx* c = new imex_ctx();
@@ -682,7 +988,7 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
   int rc = 0;
   auto A = [&](double** p, size_t n) { if (!rc) rc = dalloc(c, p, n); };
   size_t G8 = (size_t)NVAR * d.PS, G1 = (size_t)d.PS;
-  A(&c->u_pri, G8); A(&c->u_n, G8); A(&c->U1, G8); A(&c->U2, G8); A(&c->Qe, G8); A(&c->Qs, G8);
+  A(&c->u_pri, G8); A(&c->u_n, G8); A(&c->U1, G8); A(&c->U2, G8); A(&c->U3, G8); A(&c->Qe, G8); A(&c->Qs, G8);
   double** planes[] = {&c->bx_n, &c->by_n, &c->bx1, &c->by1, &c->bx2, &c->by2, &c->bxqs, &c->byqs, &c->fx, &c->fy,
                        &c->pe, &c->cf, &c->chi, &c->eta, &c->hhat, &c->mxs, &c->mys, &c->ezpp, &c->ezx, &c->ezy, &c->visc};
   for (double** pp : planes) A(pp, G1);
@@ -701,11 +1007,16 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
   if (!rc) rc = dalloc(c, &c->eflag, 1);
   if (!rc) rc = dalloc(c, &c->cnt_start, 1);
   if (!rc) rc = dalloc(c, &c->cnt_pstats, 1);
+  if (!rc) rc = dalloc(c, &c->bsave, S_NSLOTS + 4);
+  if (!rc) rc = dalloc(c, &c->esave, 1);
+  if (!rc) rc = dalloc(c, &c->ctl, 256);
   if (rc) { imex_destroy(c); return rc; }
+  c->Us = c->U2; c->Uo = c->U3;
   c->kb.w = c->w; c->kb.Av = c->Av; c->kb.partial = c->partial;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "stream"); }
   if (hipHostMalloc((void**)&c->Kh, 2 * sizeof(KrylovDev), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
   if (hipHostMalloc((void**)&c->Sh, S_NSLOTS * sizeof(double), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
+  if (hipHostMalloc((void**)&c->ctlh, 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
   (void)hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
   (void)hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);
   // zero every buffer once (outside any captured region)
@@ -716,8 +1027,9 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
   (void)hipMemset(c->eflag, 0, sizeof(int));
   (void)hipMemset(c->cnt_start, 0, sizeof(unsigned));
   (void)hipMemset(c->cnt_pstats, 0, sizeof(unsigned));
+  (void)hipMemset(c->ctl, 0, 256 * sizeof(int));
   for (double** pp : planes) (void)hipMemset(*pp, 0, G1 * sizeof(double));
-  for (double* g8 : {c->U1, c->U2, c->Qe, c->Qs, c->u_n, c->u_pri}) (void)hipMemset(g8, 0, G8 * sizeof(double));
+  for (double* g8 : {c->U1, c->U2, c->U3, c->Qe, c->Qs, c->u_n, c->u_pri}) (void)hipMemset(g8, 0, G8 * sizeof(double));
   if (hipDeviceSynchronize() != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "init sync"); }
   c->stats.min_p = 1e300;
   { double init[4] = {1e300, 0.0, 0.0, 0.0}; (void)hipMemcpy(c->pst, init, sizeof(init), hipMemcpyHostToDevice); }
@@ -732,6 +1044,8 @@ int imex_destroy(imex_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->Kh) (void)hipHostFree(c->Kh);
   if (c->Sh) (void)hipHostFree(c->Sh);
+  if (c->ctlh) (void)hipHostFree(c->ctlh);
+  for (GraphEntry& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->st) { (void)hipEventDestroy(c->ev[0]); (void)hipEventDestroy(c->ev[1]); (void)hipStreamDestroy(c->st); }
   delete c;
@@ -752,6 +1066,7 @@ int imex_set_state(imex_ctx* c, const double* u_pri, const double* bx, const dou
   c->Sh[S_T] = t; c->Sh[S_TIMESTEP] = timestep; c->Sh[S_PREVDT] = prev_dt;
   HIPCHK(hipMemcpy(c->S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice));
   c->state_mode = 0;
+  c->hist_n = 0; c->hist_pos = 0;       // iteration budgets are relearned for the new state
   return 0;
 }
 
@@ -761,13 +1076,13 @@ int imex_get_state(imex_ctx* c, double* u_con, double* u_pri, double* bx, double
   const Dims& d = c->d;
   long long tot = (long long)d.NX * d.NY;
   if (c->state_mode == 1) {
-    k_con2pri<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->U2, c->u_pri, c->eflag);
+    k_con2pri<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->Us, c->u_pri, c->eflag);
   } else {
-    k_pri2con<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->u_pri, c->U2, c->eflag);
+    k_pri2con<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->u_pri, c->Us, c->eflag);
   }
   HIPCHK(hipStreamSynchronize(c->st));
   int rc = 0;
-  if (u_con) rc = down_grid(c, c->U2, u_con);
+  if (u_con) rc = down_grid(c, c->Us, u_con);
   if (!rc && u_pri) rc = down_grid(c, c->u_pri, u_pri);
   if (!rc && bx) rc = down_plane(c, c->bx_n, bx);
   if (!rc && by) rc = down_plane(c, c->by_n, by);

@@ -15,7 +15,7 @@ struct KrylovDev {
   double tol, tol_eff, hcur, res0, res;
   double newton_res0, newton_res;
   int outer, done, converged, jlast, steps, x_is_zero;
-  int newton_it, newton_conv;
+  int newton_it, newton_conv, newton_first;
   unsigned cnt_a, cnt_c, cnt_r, cnt_n, cnt_m;
 };
 
@@ -77,14 +77,18 @@ struct KrylovBufs {
 // op = 0 magnetic (3 unknowns per cell), 1 pressure (1 unknown per cell)
 void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
                     KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st);
+void gmres_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip, hipStream_t st);
 void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
                    KrylovDev* K, hipStream_t st);
+void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st);
+void arnoldi_update_fin(int nc, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st);
 void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
                     hipStream_t st);
 void gmres_xupdate(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, double* x, KrylovDev* K, hipStream_t st);
 void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* Ax, hipStream_t st);
 void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
                      int it, double ntol, int* eflag, hipStream_t st);
+void newton_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int it, double ntol, hipStream_t st);
 void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u, double* S, double* partial, KrylovDev* K,
                  int* eflag, hipStream_t st);
 void vslope_launch(const Dims& d, const Phys& P, const double* pf, const double* u, const double* S, double* vs, int* eflag,

@@ -15,6 +15,10 @@
 // the restart length; later kernels of the same cycle see `done` and exit at once.
 #include "kernels.hpp"
 
+#ifndef IMEX_MAG_WAVES
+#define IMEX_MAG_WAVES 3
+#endif
+
 struct VPtrs { double* v[MAXM + 1]; };
 
 // compact-coordinate boundary map for one-cell halos (M_P with i in [g-1, n+g])
@@ -41,16 +45,13 @@ __device__ __forceinline__ void load_tile(double* xs, const Dims& d, const doubl
 // Magnetic_Subsystem_AX (implicit_operators.cpp:141-353, GLF branch) in face-flux form.
 // The reference's Bx_1+Bx_2 and Bz_5+Bz_6 are fluxes through the cell's upper y-face and
 // Bx_3+Bx_4, Bz_7+Bz_8 the same fluxes through its lower y-face; By_1+By_2, Bz_1+Bz_2 (and
-// By_3+By_4, Bz_3+Bz_4) likewise through its right (left) x-face.  Each face flux is formed
-// once per tile, and the four-cell corner averages once per corner (summed in the order
-// lower-left, lower-right, upper-left, upper-right).  Apart from that summation order the
-// arithmetic is the reference's expression by expression.
+// By_3+By_4, Bz_3+Bz_4) likewise through its right (left) x-face.  The four-cell corner
+// averages are summed in the order lower-left, lower-right, upper-left, upper-right.  Apart
+// from that summation order the arithmetic is the reference's expression by expression.
 struct MagSmem {
   double x[3][TY + 2][TX + 2];     // unknown B (scaled), one-cell halo through M_B
   double q[3][TY + 2][TX + 2];     // B of Qe (raw ghost frame)
   double r[TY + 2][TX + 2];        // rho of u (raw ghost frame)
-  double dv[6][TY + 2][TX + 2];    // per cell: PM - Qx5 x5, PM - Qx6 x6, q6 x5, q6 x7, q5 x6, q5 x7; later face fluxes
-  double cn[6][TY + 1][TX + 1];    // per corner: n0 b1, n0 b2, n1 b0, n1 b2, PC0, PC1
 };
 
 __device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const double* __restrict__ x, double h, bool scale,
@@ -74,96 +75,98 @@ __device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const dou
   }
 }
 
-// All threads call this (it synchronises the block); valid threads get A x of their cell in out[3].
-__device__ __forceinline__ void mag_compute(const MagOp& op, MagSmem& sm, int tx, int ty, bool valid, double* out) {
+// Corner (a, b) lies between LDS cells a..a+1 and b..b+1.  Its six terms: n0 b1, n0 b2, n1 b0,
+// n1 b2, PC0 = |n.b|/2 - n0 b0, PC1 = |n.b|/2 - n1 b1 with n, b the four-cell averages of B^e
+// and of the unknown.
+__device__ __forceinline__ void mag_corner(const MagSmem& sm, int a, int b, double* cn) {
+  double n[3], bb[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    n[c] = 0.25 * (sm.q[c][b][a] + sm.q[c][b][a + 1] + sm.q[c][b + 1][a] + sm.q[c][b + 1][a + 1]);
+    bb[c] = 0.25 * (sm.x[c][b][a] + sm.x[c][b][a + 1] + sm.x[c][b + 1][a] + sm.x[c][b + 1][a + 1]);
+  }
+  double half = 0.5 * (n[0] * bb[0] + n[1] * bb[1] + n[2] * bb[2]);
+  cn[0] = n[0] * bb[1];
+  cn[1] = n[0] * bb[2];
+  cn[2] = n[1] * bb[0];
+  cn[3] = n[1] * bb[2];
+  cn[4] = half - n[0] * bb[0];
+  cn[5] = half - n[1] * bb[1];
+}
+// Per-cell products of LDS cell (a, b): PM - q5 x5, PM - q6 x6, q6 x5, q6 x7, q5 x6, q5 x7
+// with PM = (q . x)/2.
+__device__ __forceinline__ void mag_dv(const MagSmem& sm, int a, int b, double* dv) {
+  double q5 = sm.q[0][b][a], q6 = sm.q[1][b][a], q7 = sm.q[2][b][a];
+  double x5 = sm.x[0][b][a], x6 = sm.x[1][b][a], x7 = sm.x[2][b][a];
+  double q5x5 = q5 * x5, q6x6 = q6 * x6;
+  double PM = 0.5 * (q5x5 + q6x6 + q7 * x7);
+  dv[0] = PM - q5x5;
+  dv[1] = PM - q6x6;
+  dv[2] = q6 * x5;
+  dv[3] = q6 * x7;
+  dv[4] = q5 * x6;
+  dv[5] = q5 * x7;
+}
+// Fluxes (gx, gz) through the y-face between LDS rows yb and yb+1 at column a; cL, cR are the
+// corners (a-1, yb), (a, yb) and D, U the cell products of (a, yb), (a, yb+1).
+__device__ __forceinline__ void mag_yface(const MagSmem& sm, int a, int yb, const double* cL, const double* cR,
+                                          const double* D, const double* U, double idx, double idy, double midx,
+                                          double& gx, double& gz) {
+  double rf = 0.5 * (sm.r[yb][a] + sm.r[yb + 1][a]);
+  double nf0 = 0.5 * (sm.q[0][yb][a] + sm.q[0][yb + 1][a]);
+  double nf1 = 0.5 * (sm.q[1][yb][a] + sm.q[1][yb + 1][a]);
+  double nf2 = 0.5 * (sm.q[2][yb][a] + sm.q[2][yb + 1][a]);
+  double T1 = idx * (cR[0] - cL[0]) - idy * (U[1] - D[1]);
+  gx = nf0 / rf * T1 + nf1 / rf * (idx * (cR[4] - cL[4]) - idy * (U[2] - D[2]));
+  gz = nf2 / rf * T1 + nf1 / rf * (midx * (cR[1] - cL[1]) - idy * (U[3] - D[3]));
+}
+// Fluxes (fy, fz) through the x-face between LDS columns xa and xa+1 at row b; cB, cT are the
+// corners (xa, b-1), (xa, b) and L, R the cell products of (xa, b), (xa+1, b).
+__device__ __forceinline__ void mag_xface(const MagSmem& sm, int xa, int b, const double* cB, const double* cT,
+                                          const double* L, const double* R, double idy, double midx,
+                                          double& fy, double& fz) {
+  double rf = 0.5 * (sm.r[b][xa] + sm.r[b][xa + 1]);
+  double nf0 = 0.5 * (sm.q[0][b][xa] + sm.q[0][b][xa + 1]);
+  double nf1 = 0.5 * (sm.q[1][b][xa] + sm.q[1][b][xa + 1]);
+  double nf2 = 0.5 * (sm.q[2][b][xa] + sm.q[2][b][xa + 1]);
+  double dpm = R[0] - L[0];
+  double T2 = midx * dpm + idy * (cT[0] - cB[0]);
+  fy = nf1 / rf * T2 + nf0 / rf * (midx * (R[4] - L[4]) + idy * (cT[5] - cB[5]));
+  fz = nf2 / rf * (midx * dpm + idy * (cT[2] - cB[2])) + nf0 / rf * (midx * (R[5] - L[5]) - idy * (cT[3] - cB[3]));
+}
+
+// A x for the cell of a valid thread (no block synchronisation: each thread forms the four face
+// fluxes of its own cell from the staged tiles; a face shared by two cells is formed by both
+// with the same arithmetic).
+__device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem& sm, int tx, int ty, bool valid, double* out) {
+  if (!valid) return;
   const Dims& d = op.d;
-  const int W = TX + 2, HH = TY + 2;
   const double h = op.S[S_ADT], dx = d.dx, dy = d.dy;
   const double idx = 1 / dx, idy = 1 / dy, midx = -1 / dx;
-  // per-cell products
-  for (int t = threadIdx.x; t < W * HH; t += blockDim.x) {
-    int ly = t / W, lx = t % W;
-    double q5 = sm.q[0][ly][lx], q6 = sm.q[1][ly][lx], q7 = sm.q[2][ly][lx];
-    double x5 = sm.x[0][ly][lx], x6 = sm.x[1][ly][lx], x7 = sm.x[2][ly][lx];
-    double q5x5 = q5 * x5, q6x6 = q6 * x6;
-    double PM = 0.5 * (q5x5 + q6x6 + q7 * x7);
-    sm.dv[0][ly][lx] = PM - q5x5;
-    sm.dv[1][ly][lx] = PM - q6x6;
-    sm.dv[2][ly][lx] = q6 * x5;
-    sm.dv[3][ly][lx] = q6 * x7;
-    sm.dv[4][ly][lx] = q5 * x6;
-    sm.dv[5][ly][lx] = q5 * x7;
-  }
-  // corner terms: corner (a, b) lies between LDS cells a..a+1 and b..b+1
-  for (int t = threadIdx.x; t < (TX + 1) * (TY + 1); t += blockDim.x) {
-    int b = t / (TX + 1), a = t % (TX + 1);
-    double n[3], bb[3];
-    for (int c = 0; c < 3; c++) {
-      n[c] = 0.25 * (sm.q[c][b][a] + sm.q[c][b][a + 1] + sm.q[c][b + 1][a] + sm.q[c][b + 1][a + 1]);
-      bb[c] = 0.25 * (sm.x[c][b][a] + sm.x[c][b][a + 1] + sm.x[c][b + 1][a] + sm.x[c][b + 1][a + 1]);
-    }
-    double half = 0.5 * (n[0] * bb[0] + n[1] * bb[1] + n[2] * bb[2]);
-    sm.cn[0][b][a] = n[0] * bb[1];
-    sm.cn[1][b][a] = n[0] * bb[2];
-    sm.cn[2][b][a] = n[1] * bb[0];
-    sm.cn[3][b][a] = n[1] * bb[2];
-    sm.cn[4][b][a] = half - n[0] * bb[0];
-    sm.cn[5][b][a] = half - n[1] * bb[1];
-  }
-  __syncthreads();
-  // face fluxes into registers (each thread owns at most two faces of each orientation)
-  double gxr[2], gzr[2], fyr[2], fzr[2];
-  const int NYF = (TY + 1) * TX, NXF = TY * (TX + 1);
-  for (int k = 0; k < 2; k++) {
-    int t = threadIdx.x + k * blockDim.x;
-    if (t < NYF) {   // y-face between LDS rows yb and yb+1 at LDS column a = fx+1
-      int yb = t / TX, a = t % TX + 1;
-      double rf = 0.5 * (sm.r[yb][a] + sm.r[yb + 1][a]);
-      double nf0 = 0.5 * (sm.q[0][yb][a] + sm.q[0][yb + 1][a]);
-      double nf1 = 0.5 * (sm.q[1][yb][a] + sm.q[1][yb + 1][a]);
-      double nf2 = 0.5 * (sm.q[2][yb][a] + sm.q[2][yb + 1][a]);
-      int cR = a, cL = a - 1;
-      double T1 = idx * (sm.cn[0][yb][cR] - sm.cn[0][yb][cL]) - idy * (sm.dv[1][yb + 1][a] - sm.dv[1][yb][a]);
-      gxr[k] = nf0 / rf * T1 + nf1 / rf * (idx * (sm.cn[4][yb][cR] - sm.cn[4][yb][cL]) - idy * (sm.dv[2][yb + 1][a] - sm.dv[2][yb][a]));
-      gzr[k] = nf2 / rf * T1 + nf1 / rf * (midx * (sm.cn[1][yb][cR] - sm.cn[1][yb][cL]) - idy * (sm.dv[3][yb + 1][a] - sm.dv[3][yb][a]));
-    }
-    if (t < NXF) {   // x-face between LDS columns xa and xa+1 at LDS row b = fy+1
-      int b = t / (TX + 1) + 1, xa = t % (TX + 1);
-      double rf = 0.5 * (sm.r[b][xa] + sm.r[b][xa + 1]);
-      double nf0 = 0.5 * (sm.q[0][b][xa] + sm.q[0][b][xa + 1]);
-      double nf1 = 0.5 * (sm.q[1][b][xa] + sm.q[1][b][xa + 1]);
-      double nf2 = 0.5 * (sm.q[2][b][xa] + sm.q[2][b][xa + 1]);
-      int cT = b, cB = b - 1;
-      double dpm = sm.dv[0][b][xa + 1] - sm.dv[0][b][xa];
-      double T2 = midx * dpm + idy * (sm.cn[0][cT][xa] - sm.cn[0][cB][xa]);
-      fyr[k] = nf1 / rf * T2 + nf0 / rf * (midx * (sm.dv[4][b][xa + 1] - sm.dv[4][b][xa]) + idy * (sm.cn[5][cT][xa] - sm.cn[5][cB][xa]));
-      fzr[k] = nf2 / rf * (midx * dpm + idy * (sm.cn[2][cT][xa] - sm.cn[2][cB][xa]))
-             + nf0 / rf * (midx * (sm.dv[5][b][xa + 1] - sm.dv[5][b][xa]) - idy * (sm.cn[3][cT][xa] - sm.cn[3][cB][xa]));
-    }
-  }
-  __syncthreads();
-  // park the fluxes in the dv area (no longer needed)
-  double* gx = &sm.dv[0][0][0];
-  double* gz = gx + NYF;
-  double* fy = gz + NYF;
-  double* fz = fy + NXF;
-  for (int k = 0; k < 2; k++) {
-    int t = threadIdx.x + k * blockDim.x;
-    if (t < NYF) { gx[t] = gxr[k]; gz[t] = gzr[k]; }
-    if (t < NXF) { fy[t] = fyr[k]; fz[t] = fzr[k]; }
-  }
-  __syncthreads();
-  if (!valid) return;
   const int a = tx + 1, b = ty + 1;
+  double c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
+  mag_corner(sm, a - 1, b - 1, c00);
+  mag_corner(sm, a, b - 1, c10);
+  mag_corner(sm, a - 1, b, c01);
+  mag_corner(sm, a, b, c11);
+  double dC[6], dN[6];
+  mag_dv(sm, a, b, dC);
+  double gxu, gzu, gxl, gzl, fyr, fzr, fyl, fzl;
+  mag_dv(sm, a, b + 1, dN);
+  mag_yface(sm, a, b, c01, c11, dC, dN, idx, idy, midx, gxu, gzu);
+  mag_dv(sm, a, b - 1, dN);
+  mag_yface(sm, a, b - 1, c00, c10, dN, dC, idx, idy, midx, gxl, gzl);
+  mag_dv(sm, a + 1, b, dN);
+  mag_xface(sm, a, b, c10, c11, dC, dN, idy, midx, fyr, fzr);
+  mag_dv(sm, a - 1, b, dN);
+  mag_xface(sm, a - 1, b, c00, c01, dN, dC, idy, midx, fyl, fzl);
   auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
-  // upper / lower y-face of the cell: faces ty+1 / ty (row-major TX per row); right / left x-face: tx+1 / tx
-  const int yu = (ty + 1) * TX + tx, yl = ty * TX + tx;
-  const int xr = ty * (TX + 1) + tx + 1, xl = ty * (TX + 1) + tx;
-  double ax5 = X(0, 0, 0) + (h * h / dy) * (gx[yu] - gx[yl]);
-  double ax6 = X(1, 0, 0) + (h * h / dx) * (fy[xr] - fy[xl]);
-  double ax7 = X(2, 0, 0) + h * h / dx * (fz[xr] - fz[xl]) + h * h / dy * (gz[yu] - gz[yl]);
+  double ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
+  double ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
+  double ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
   const double nu_x = op.S[S_BMAXX] * dx, nu_y = op.S[S_BMAXY] * dy;
   double lap[3];
+#pragma unroll
   for (int c = 0; c < 3; c++)
     lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - 2.0 * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - 2.0 * X(c, 0, 0) + X(c, 0, 1));
   out[0] = ax5 - h * lap[0];
@@ -258,20 +261,40 @@ __device__ __forceinline__ void block_sums(const double* acc, int nv, double* re
     out[threadIdx.x] = s;
   }
 }
-// In the last block: out[v] = sum over blocks b of partial[b*stride + v], v < nv (fixed order per wave).
+// One block: out[v] = sum over blocks b < nblk of partial[b*stride + v], v < nv <= NV.  Thread t
+// sums rows t, t + 256, ... with all their values in flight at once; the per-thread sums are
+// then reduced wave by wave in a fixed order, so the result does not depend on timing.
+template <int NV>
 __device__ __forceinline__ void final_sums(const double* partial, int stride, int nblk, int nv, double* out) {
+  __shared__ double red[4][NV];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int v = wid; v < nv; v += nw) {
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.0;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const double* row = partial + (long long)b * stride;
+#pragma unroll
+    for (int v = 0; v < NV; v++) if (v < nv) acc[v] += row[v];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    if (v < nv) {
+      double s = wave_sum(acc[v]);
+      if (lane == 0) red[wid][v] = s;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
     double s = 0.0;
-    for (int b = lane; b < nblk; b += 64) s += partial[(long long)b * stride + v];
-    s = wave_sum(s);
-    if (lane == 0) out[v] = s;
+    for (int w = 0; w < nw; w++) s += red[w][threadIdx.x];
+    out[threadIdx.x] = s;
   }
 }
 
 // ---------------------------------------------------------------- GMRES residual r0 = M^-1 (b - A x), gam0 = ||r0||
 template <int OPK>
-__global__ void __launch_bounds__(256) k_gmres_residual(typename OpSel<OPK>::T op, KrylovBufs kb, const double* __restrict__ x,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4))) k_gmres_residual(typename OpSel<OPK>::T op, KrylovBufs kb, const double* __restrict__ x,
                                                         const double* __restrict__ b, KrylovDev* K, int outer, int x_is_zero,
                                                         const int* skip) {
   if (skip && *skip) {           // the caller's outer iteration has already converged: no solve
@@ -314,7 +337,7 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partia
                                                             const int* skip) {
   if (skip && *skip) return;
   __shared__ double tot[1];
-  final_sums(partial, MAXM + 2, nblk, 1, tot);
+  final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double g0 = sqrt(tot[0]);
@@ -332,18 +355,16 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partia
 // v_j = w / h (written), Av_j = M^-1 A v_j (written), H[i][j] = v_i . Av_j for i <= j.
 // JB >= j + 1 is the number of dot accumulators kept in registers.
 template <int OPK, int JB>
-__global__ void __launch_bounds__(256) k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, KrylovDev* K) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4))) k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, KrylovDev* K) {
   if (K->done) return;
   constexpr int NC = OpSel<OPK>::NC;
   __shared__ typename OpSel<OPK>::Smem sm;
-  __shared__ double red[4 * JB];
-  __shared__ double outv[JB];
+  __shared__ double wsum[4][JB];        // running dot sums per wave, in tile order
   const Dims& d = OpSel<OPK>::dims(op);
   const double h = K->hcur;
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
-  double acc[JB];
-#pragma unroll
-  for (int i = 0; i < JB; i++) acc[i] = 0.0;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane < JB) wsum[wid][lane] = 0.0;
   double* vj = V.v[j];
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
@@ -356,6 +377,11 @@ __global__ void __launch_bounds__(256) k_arnoldi_apply(typename OpSel<OPK>::T op
     __syncthreads();
     double y[NC];
     OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+    // the dots of this tile are formed after the operator so that their accumulators are not
+    // live during the stencil
+    double acc[JB];
+#pragma unroll
+    for (int i = 0; i < JB; i++) acc[i] = 0.0;
     if (valid) {
       OpSel<OPK>::precond(op, cidx, cx + GC, cy + GC, y);
       for (int c = 0; c < NC; c++) {
@@ -370,17 +396,29 @@ __global__ void __launch_bounds__(256) k_arnoldi_apply(typename OpSel<OPK>::T op
         }
       }
     }
+#pragma unroll
+    for (int i = 0; i < JB; i++) {
+      if (i <= j) {
+        double s = wave_sum(acc[i]);
+        if (lane == 0) wsum[wid][i] += s;
+      }
+    }
   }
-  block_sums<JB>(acc, j + 1, red, outv);
   __syncthreads();
-  if ((int)threadIdx.x <= j) kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = outv[threadIdx.x];
+  if ((int)threadIdx.x <= j) {
+    const int nw = blockDim.x >> 6;
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += wsum[w][threadIdx.x];
+    kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = s;
+  }
 }
 
 // One-block finish of the dots: H[i][j] for i <= j.
+template <int JB>
 __global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const double* partial, int nblk, int j, KrylovDev* K) {
   if (K->done) return;
   __shared__ double tot[MAXM + 1];
-  final_sums(partial, MAXM + 2, nblk, j + 1, tot);
+  final_sums<JB>(partial, MAXM + 2, nblk, j + 1, tot);
   __syncthreads();
   if ((int)threadIdx.x <= j) K->H[threadIdx.x * (MAXM + 1) + j] = tot[threadIdx.x];
 }
@@ -414,7 +452,7 @@ __global__ void __launch_bounds__(256) k_arnoldi_update(KrylovBufs kb, VPtrs V, 
 __global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partial, int nblk, int j, int m, KrylovDev* K) {
   if (K->done) return;
   __shared__ double tot[1];
-  final_sums(partial, MAXM + 2, nblk, 1, tot);
+  final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     const int L = MAXM + 1;
@@ -467,7 +505,7 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs V, double* __restri
 
 // ---------------------------------------------------------------- plain operator application (tests, adjoint)
 template <int OPK>
-__global__ void __launch_bounds__(256) k_op_apply(typename OpSel<OPK>::T op, const double* __restrict__ x, double* __restrict__ Ax) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4))) k_op_apply(typename OpSel<OPK>::T op, const double* __restrict__ x, double* __restrict__ Ax) {
   constexpr int NC = OpSel<OPK>::NC;
   __shared__ typename OpSel<OPK>::Smem sm;
   const Dims& d = OpSel<OPK>::dims(op);
@@ -491,7 +529,8 @@ __global__ void __launch_bounds__(256) k_op_apply(typename OpSel<OPK>::T op, con
 // g = L p - a p + V_mod(p, rho) - b with L the carrier operator; writes -g and ||g||.
 __global__ void __launch_bounds__(256) k_newton_residual(POp op, KrylovBufs kb, const double* __restrict__ p,
                                                          const double* __restrict__ b, double* __restrict__ ng,
-                                                         KrylovDev* K, int it, double ntol, int* eflag) {
+                                                         KrylovDev* K, int it, double ntol, int* eflag, const int* skip) {
+  if (skip && *skip) return;     // an earlier iteration of this Newton solve has converged
   __shared__ double xs[(TY + 2) * (TX + 2)];
   __shared__ double red[4];
   __shared__ double res[1];
@@ -526,15 +565,17 @@ __global__ void __launch_bounds__(256) k_newton_residual(POp op, KrylovBufs kb, 
 }
 
 __global__ void __launch_bounds__(256) k_newton_fin(const double* partial, int nblk, KrylovDev* K, int it, double ntol) {
+  if (it > 0 && K->newton_conv) return;
   __shared__ double tot[1];
-  final_sums(partial, MAXM + 2, nblk, 1, tot);
+  final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = sqrt(tot[0]);
-    if (it == 0) K->newton_res0 = r;
+    if (it == 0) { K->newton_res0 = r; K->newton_first = -1; }
     K->newton_res = r;
     K->newton_it = it;
     K->newton_conv = (r < ntol * K->newton_res0 + ntol) ? 1 : 0;
+    if (K->newton_conv) K->newton_first = it;   // Jacobian solves this Newton solve needed
   }
 }
 
@@ -565,9 +606,11 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Dims d, Phys P, const doubl
 }
 
 __global__ void k_newton_mbig_fin(Phys P, const double* partial, int nblk, double* S) {
+  // min and max are order independent: one wave reduces the block partials
+  double a = 1e300, c = -1e300;
+  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, partial[bl * 2]); c = fmax(c, partial[bl * 2 + 1]); }
+  a = wave_min(a); c = wave_max(c);
   if (threadIdx.x == 0) {
-    double a = 1e300, c = -1e300;
-    for (int bl = 0; bl < nblk; bl++) { a = fmin(a, partial[bl * 2]); c = fmax(c, partial[bl * 2 + 1]); }
     double Vpe = P.p_floor / (P.gamma - 1.0);
     if (P.eos != IMEX_EOS_IDEAL && c > Vpe) Vpe = c;
     double M_cor = fmax(0.0, Vpe - a) / ((1.0 - P.theta) * P.p_floor);
@@ -608,12 +651,13 @@ void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb
   if (op == 0) {
     int g = grid_for_tiles(mo.d, kb.nblk_apply);
     k_gmres_residual<0><<<g, 256, 0, st>>>(mo, kb, x, b, K, outer, x_is_zero, skip);
-    k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, g, K, outer, skip);
   } else {
     int g = grid_for_tiles(po.d, kb.nblk_apply);
     k_gmres_residual<1><<<g, 256, 0, st>>>(po, kb, x, b, K, outer, x_is_zero, skip);
-    k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, g, K, outer, skip);
   }
+}
+void gmres_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip, hipStream_t st) {
+  k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, outer, skip);
 }
 
 template <int OPK, int JB>
@@ -636,7 +680,12 @@ void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb,
     else if (j < 16) launch_apply_jb<1, 16>(po, kb, V, j, K, g, st);
     else launch_apply_jb<1, 32>(po, kb, V, j, K, g, st);
   }
-  k_arnoldi_dots_fin<<<1, 256, 0, st>>>(kb.partial, g, j, K);
+}
+void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st) {
+  int g = grid_for_tiles(d, kb.nblk_apply);
+  if (j < 8) k_arnoldi_dots_fin<8><<<1, 256, 0, st>>>(kb.partial, g, j, K);
+  else if (j < 16) k_arnoldi_dots_fin<16><<<1, 256, 0, st>>>(kb.partial, g, j, K);
+  else k_arnoldi_dots_fin<32><<<1, 256, 0, st>>>(kb.partial, g, j, K);
 }
 
 void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
@@ -648,6 +697,11 @@ void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* 
   if (j < 8) k_arnoldi_update<8><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
   else if (j < 16) k_arnoldi_update<16><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
   else k_arnoldi_update<32><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
+}
+void arnoldi_update_fin(int nc, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st) {
+  long long n = (long long)nc * d.N;
+  long long nb = (n + 255) / 256;
+  int g = (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
   k_arnoldi_givens_fin<<<1, 256, 0, st>>>(kb.partial, g, j, m, K);
 }
 
@@ -667,8 +721,10 @@ void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* A
 void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
                      int it, double ntol, int* eflag, hipStream_t st) {
   int g = grid_for_tiles(po.d, kb.nblk_apply);
-  k_newton_residual<<<g, 256, 0, st>>>(po, kb, p, b, ng, K, it, ntol, eflag);
-  k_newton_fin<<<1, 256, 0, st>>>(kb.partial, g, K, it, ntol);
+  k_newton_residual<<<g, 256, 0, st>>>(po, kb, p, b, ng, K, it, ntol, eflag, it > 0 ? &K->newton_conv : nullptr);
+}
+void newton_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int it, double ntol, hipStream_t st) {
+  k_newton_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, it, ntol);
 }
 
 void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u, double* S, double* partial, KrylovDev* K,

@@ -36,6 +36,13 @@ enum { IMEX_CT_CONSISTENT = 0, IMEX_CT_PLAIN = 1 };                 /* Staggered
 enum { IMEX_PREC_FP64 = 0, IMEX_PREC_MIXED = 1 };
 enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2 };
 enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2 };
+/* How a step is scheduled.  All three give identical results.
+ *   GRAPH:  every solve gets an iteration budget learned from earlier steps; the whole step is
+ *           launched as one captured hipGraph with no host synchronisation inside it; a step whose
+ *           solve outran its budget is discarded and rerun with HOST scheduling.
+ *   BUDGET: as GRAPH, launched kernel by kernel (no graph).
+ *   HOST:   the host polls GMRES/Newton convergence between launches, as the reference's loops do. */
+enum { IMEX_SCHED_GRAPH = 0, IMEX_SCHED_BUDGET = 1, IMEX_SCHED_HOST = 2 };
 
 enum {
   IMEX_WARN_GMRES_MAXOUTER = 1,   /* src/implicit_operators.cpp:918 */
@@ -71,6 +78,9 @@ typedef struct imex_params {
   int ct_mode;                    /* IMEX_CT_* */
   int precision;                  /* IMEX_PREC_* */
   double mixed_inner_tol;         /* relative tolerance of the FP32 inner solve (mixed form) */
+  int schedule;                   /* IMEX_SCHED_* */
+  int budget_margin;              /* Arnoldi steps granted per GMRES solve beyond the largest count of the
+                                     last 8 steps (budgeted schedules; default 1) */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */
@@ -103,6 +113,9 @@ typedef struct imex_stats {
   int max_floored;                /* g_max_floored (config.hpp:109) */
   double min_p;                   /* g_min_p_ever (config.hpp:110) */
   double div_b;                   /* cal_divergence_error after the last step */
+  long long budget_steps;         /* steps that ran on learned iteration budgets */
+  long long budget_reruns;        /* of those, steps rerun because a solve outran its budget */
+  long long graph_builds;         /* step graphs captured and instantiated */
 } imex_stats;
 
 typedef struct imex_solve_info {
