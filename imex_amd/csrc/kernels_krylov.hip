@@ -28,17 +28,32 @@ __device__ __forceinline__ int cmap(int g, int n, int per) {
   return g;
 }
 
+// Tiles are TX x TY cells with a one-cell frame, loaded by a block of TX*TY threads: thread t
+// loads interior cell (t % TX, t / TX) and, for t < NHALO, frame element t (top row, bottom
+// row, then the left and right columns).
+#define NHALO (2 * (TX + 2) + 2 * TY)
+__device__ __forceinline__ void halo_coord(int hidx, int& lx, int& ly) {
+  if (hidx < TX + 2) { lx = hidx; ly = 0; }
+  else if (hidx < 2 * (TX + 2)) { lx = hidx - (TX + 2); ly = TY + 1; }
+  else { int k = hidx - 2 * (TX + 2); lx = (k & 1) ? TX + 1 : 0; ly = 1 + (k >> 1); }
+}
+
 template <int NC>
 __device__ __forceinline__ void load_tile(double* xs, const Dims& d, const double* __restrict__ x, double h, bool scale,
                                           int gx0, int gy0) {
   const int W = TX + 2, HH = TY + 2;
-  for (int t = threadIdx.x; t < NC * W * HH; t += blockDim.x) {
-    int k = t / (W * HH), r = t % (W * HH);
-    int ly = r / W, lx = r % W;
+  auto put = [&](int lx, int ly) {
     int gx = cmap(gx0 + lx - 1, d.nx, d.perx), gy = cmap(gy0 + ly - 1, d.ny, d.pery);
-    double v = x[(long long)k * d.N + gx + (long long)d.nx * gy];
-    xs[t] = scale ? v / h : v;
-  }
+    long long e = gx + (long long)d.nx * gy;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      double v = x[(long long)k * d.N + e];
+      xs[k * W * HH + ly * W + lx] = scale ? v / h : v;
+    }
+  };
+  const int t = threadIdx.x;
+  put(t % TX + 1, t / TX + 1);
+  if (t < NHALO) { int lx, ly; halo_coord(t, lx, ly); put(lx, ly); }
 }
 
 // ---------------------------------------------------------------- magnetic operator
@@ -57,21 +72,32 @@ struct MagSmem {
 __device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const double* __restrict__ x, double h, bool scale,
                                          int gx0, int gy0) {
   const Dims& d = op.d;
-  const int W = TX + 2, HH = TY + 2, NT = W * HH;
-  for (int t = threadIdx.x; t < 7 * NT; t += blockDim.x) {
-    int f = t / NT, rr = t % NT;
-    int ly = rr / W, lx = rr % W;
+  auto put = [&](int lx, int ly) {
     int cx = gx0 + lx - 1, cy = gy0 + ly - 1;
-    if (f < 3) {
-      int gx = cmap(cx, d.nx, d.perx), gy = cmap(cy, d.ny, d.pery);
-      double v = x[(long long)f * d.N + gx + (long long)d.nx * gy];
+    int gx = cmap(cx, d.nx, d.perx), gy = cmap(cy, d.ny, d.pery);
+    long long e = gx + (long long)d.nx * gy;
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+      double v = x[(long long)f * d.N + e];
       sm.x[f][ly][lx] = scale ? v / h : v;
-    } else {
-      int pi = min(cx + GC, d.NX - 1), pj = min(cy + GC, d.NY - 1);
-      long long c = pidx(d, pi, pj);
-      if (f < 6) sm.q[f - 3][ly][lx] = op.Qe[(long long)(5 + f - 3) * d.PS + c];
-      else sm.r[ly][lx] = op.u[c];
     }
+    int pi = min(cx + GC, d.NX - 1), pj = min(cy + GC, d.NY - 1);
+    long long c = pidx(d, pi, pj);
+#pragma unroll
+    for (int f = 0; f < 3; f++) sm.q[f][ly][lx] = op.Qe[(long long)(5 + f) * d.PS + c];
+    sm.r[ly][lx] = op.u[c];
+  };
+  const int t = threadIdx.x;
+  // interior element, then (for t < NHALO) one frame element; kept as two passes so that the
+  // seven loads of only one element are in flight at a time
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {
+    int lx = t % TX + 1, ly = t / TX + 1;
+    if (pass == 1) {
+      if (t >= NHALO) break;
+      halo_coord(t, lx, ly);
+    }
+    put(lx, ly);
   }
 }
 

@@ -243,11 +243,14 @@ __device__ __forceinline__ void recon5(int recon, int central, const double* a, 
     if (central) c = 0.5 * (sL + sR);
     wL = u0 - 0.5 * c; wR = u0 + 0.5 * c;
   } else if (recon == IMEX_RECON_VANLEER) {
+    // c = min(2/(1+r), 2r/(1+r)) = 2 min(1, r)/(1+r) for r >= 0
     double sR = up1 - u0, sL = u0 - um1, c;
     if (fabs(sR) < 1e-16) c = 0;
-    else { double r = sL / sR; c = (r < 0) ? 0 : fmin(2 / (1 + r), (2 * r) / (1 + r)); }
+    else { double r = sL / sR; c = (r < 0) ? 0 : 2 * fmin(1.0, r) / (1 + r); }
     wL = u0 - 0.5 * c * sL; wR = u0 + 0.5 * c * sR;
   } else {
+    // Jiang-Shu weights w_k = (d_k/e_k) / sum_m (d_m/e_m), e_k = (eps + beta_k)^2, written with
+    // the common denominator e0 e1 e2 so that each face value costs one division.
     double t0 = um2 - 2 * um1 + u0, t1 = um2 - 4 * um1 + 3 * u0;
     double beta0 = 13.0 / 12.0 * (t0 * t0) + 1.0 / 4.0 * (t1 * t1);
     double t2 = um1 - 2 * u0 + up1, t3 = um1 - up1;
@@ -256,18 +259,17 @@ __device__ __forceinline__ void recon5(int recon, int central, const double* a, 
     double beta2 = 13.0 / 12.0 * (t4 * t4) + 1.0 / 4.0 * (t5 * t5);
     double e0 = 1e-6 + beta0, e1 = 1e-6 + beta1, e2 = 1e-6 + beta2;
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
+    const double P0 = e1 * e2, P1 = e0 * e2, P2 = e0 * e1;
     double q0 = -1.0 / 6.0 * um2 + 5.0 / 6.0 * um1 + 1.0 / 3.0 * u0;
     double q1 = 1.0 / 3.0 * um1 + 5.0 / 6.0 * u0 - 1.0 / 6.0 * up1;
     double q2 = 11.0 / 6.0 * u0 - 7.0 / 6.0 * up1 + 1.0 / 3.0 * up2;
-    double a0 = 3.0 / 10.0 / e0, a1 = 6.0 / 10.0 / e1, a2 = 1.0 / 10.0 / e2;
-    double s = a0 + a1 + a2;
-    wL = a0 / s * q0 + a1 / s * q1 + a2 / s * q2;
+    double a0 = 0.3 * P0, a1 = 0.6 * P1, a2 = 0.1 * P2;
+    wL = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
     q0 = 1.0 / 3.0 * um2 - 7.0 / 6.0 * um1 + 11.0 / 6.0 * u0;
     q1 = -1.0 / 6.0 * um1 + 5.0 / 6.0 * u0 + 1.0 / 3.0 * up1;
     q2 = 1.0 / 3.0 * u0 + 5.0 / 6.0 * up1 - 1.0 / 6.0 * up2;
-    a0 = 1.0 / 10.0 / e0; a1 = 6.0 / 10.0 / e1; a2 = 3.0 / 10.0 / e2;
-    s = a0 + a1 + a2;
-    wR = a0 / s * q0 + a1 / s * q1 + a2 / s * q2;
+    a0 = 0.1 * P0; a1 = 0.6 * P1; a2 = 0.3 * P2;
+    wR = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
   }
 }
 
@@ -280,7 +282,8 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, con
   for (int side = 0; side < 2; side++) {
     const double* Q = Qs2[side];
     double* f = fo[side];
-    double rho = Q[0], vx = Q[1] / Q[0], vy = Q[2] / Q[0], vz = Q[3] / Q[0];
+    const double ir = 1.0 / Q[0];
+    double rho = Q[0], vx = Q[1] * ir, vy = Q[2] * ir, vz = Q[3] * ir;
     double kin = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
     double vn = dir == 0 ? vx : vy;
     f[0] = rho * vn;
@@ -290,17 +293,17 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, con
     f[4] = kin * vn;
     if (kind == 1) {
       double B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
-      double vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) / Q[0];
+      double vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) * ir;
       if (dir == 0) {
         f[1] = f[1] + (B2 * 0.5 - Q[5] * Q[5]);
         f[2] = f[2] + (-Q[5] * Q[6]);
         f[3] = f[3] + (-Q[5] * Q[7]);
-        f[4] = f[4] + (Q[1] / Q[0] * B2 - Q[5] * vB);
+        f[4] = f[4] + (vx * B2 - Q[5] * vB);
       } else {
         f[1] = f[1] + (-Q[5] * Q[6]);
         f[2] = f[2] + (B2 * 0.5 - Q[6] * Q[6]);
         f[3] = f[3] + (-Q[6] * Q[7]);
-        f[4] = f[4] + (Q[2] / Q[0] * B2 - Q[6] * vB);
+        f[4] = f[4] + (vy * B2 - Q[6] * vB);
       }
     }
   }
@@ -319,75 +322,85 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, con
 #define EX_NW 320            // max cells reconstructed per direction: 34x8 (x) / 32x10 (y)
 #define EX_NF 288            // max faces per direction: 33x8 (x) / 32x9 (y)
 
+// One direction of the fused update: reconstruct, form the face fluxes, apply their difference.
+template <int DIR>
+__device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int kind, const double* __restrict__ Qe,
+                                             const double* __restrict__ Qs, double* __restrict__ u, double h, double s_max,
+                                             int i0, int j0, double* wl, double* wr, double* F, double* acc) {
+  constexpr int W = DIR == 0 ? EX_TX + 2 : EX_TX;                  // reconstructed cells per row
+  constexpr int NITEMS = DIR == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
+  constexpr int NF = DIR == 0 ? (EX_TX + 1) * EX_TY : EX_TX * (EX_TY + 1);
+  constexpr int FW = DIR == 0 ? EX_TX + 1 : EX_TX;                 // faces per row
+  const int nrec = kind == 0 ? 4 : 5;       // kind 0 fluxes need rho and momenta only
+  const int tx = threadIdx.x % EX_TX, ty = threadIdx.x / EX_TX;
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool own = ci < d.nx + GC && cj < d.ny + GC;
+  const int k0 = kind == 0 ? 0 : 1, k1 = kind == 0 ? 4 : 5;
+  __syncthreads();
+  for (int it = threadIdx.x; it < NITEMS; it += blockDim.x) {
+    int lx = it % W, ly = it / W;
+    int gi = DIR == 0 ? i0 + lx - 1 : i0 + lx;
+    int gj = DIR == 0 ? j0 + ly : j0 + ly - 1;
+    if (gi > d.nx + GC || gj > d.ny + GC) continue;      // beyond a partial tile: unused
+    int mi = DIR == 0 ? map_x(d, gi) : gi;
+    int mj = DIR == 0 ? gj : map_y(d, gj);
+    const long long base = pidx(d, mi, mj);
+    const long long step = DIR == 0 ? 1 : d.NX;
+    for (int k = 0; k < nrec; k++) {
+      const double* q = Qe + k * d.PS + base;
+      double a[5];
+#pragma unroll
+      for (int o = 0; o < 5; o++) a[o] = q[(o - 2) * step];
+      double L, R;
+      recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R);
+      wl[k * EX_NW + it] = L;
+      wr[k * EX_NW + it] = R;
+    }
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < NF; f += blockDim.x) {
+    int fx = f % FW, fy = f / FW;
+    // face between local cells (left, right) along DIR; item indices into wl/wr
+    int itL, itR, gli, glj, gri, grj;
+    if (DIR == 0) { itL = fy * W + fx; itR = fy * W + fx + 1; gli = i0 + fx - 1; gri = i0 + fx; glj = grj = j0 + fy; }
+    else { itL = fy * W + fx; itR = (fy + 1) * W + fx; gli = gri = i0 + fx; glj = j0 + fy - 1; grj = j0 + fy; }
+    if (gri > d.nx + GC || grj > d.ny + GC) continue;
+    double QL[NVAR], QR[NVAR];
+    for (int k = 0; k < 5; k++) {
+      QL[k] = k < nrec ? wr[k * EX_NW + itL] : 0.0;
+      QR[k] = k < nrec ? wl[k * EX_NW + itR] : 0.0;
+    }
+    if (kind == 1) {
+      for (int k = 5; k < 8; k++) { QL[k] = u[k * d.PS + pidx(d, gli, glj)]; QR[k] = u[k * d.PS + pidx(d, gri, grj)]; }
+    } else {
+      for (int k = 5; k < 8; k++) { QL[k] = 0.0; QR[k] = 0.0; }
+    }
+    double Fo[5];
+    gl_flux(DIR, kind, QL, QR, s_max, Fo);
+    for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[k];
+  }
+  __syncthreads();
+  if (own) {
+    long long c = pidx(d, ci, cj);
+    if (DIR == 0) {
+      int fm = ty * FW + tx, fp = fm + 1;
+      for (int k = k0; k < k1; k++) acc[k] = Qs[k * d.PS + c] - h / d.dx * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
+    } else {
+      int fm = ty * FW + tx, fp = fm + FW;
+      for (int k = k0; k < k1; k++) u[k * d.PS + c] = acc[k] - h / d.dy * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_explicit(Dims d, Phys P, int kind, const double* __restrict__ Qe,
                                                   const double* __restrict__ Qs, double* __restrict__ u, const double* S) {
   __shared__ double wl[5 * EX_NW], wr[5 * EX_NW];
   __shared__ double F[5 * EX_NF];
   const int i0 = GC + blockIdx.x * EX_TX, j0 = GC + blockIdx.y * EX_TY;
-  const int tx = threadIdx.x % EX_TX, ty = threadIdx.x / EX_TX;
   const double h = S[S_ADT], s_max = S[S_AMAX];
-  const int nrec = kind == 0 ? 4 : 5;       // kind 0 fluxes need rho and momenta only
-  const int ci = i0 + tx, cj = j0 + ty;
-  const bool own = ci < d.nx + GC && cj < d.ny + GC;
-  const int k0 = kind == 0 ? 0 : 1, k1 = kind == 0 ? 4 : 5;
   double acc[5];
-  for (int dir = 0; dir < 2; dir++) {
-    const int W = dir == 0 ? EX_TX + 2 : EX_TX;          // reconstructed cells per row
-    const int nitems = dir == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
-    __syncthreads();
-    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
-      int lx = it % W, ly = it / W;
-      int gi = dir == 0 ? i0 + lx - 1 : i0 + lx;
-      int gj = dir == 0 ? j0 + ly : j0 + ly - 1;
-      if (gi > d.nx + GC || gj > d.ny + GC) continue;      // beyond a partial tile: unused
-      int mi = dir == 0 ? map_x(d, gi) : gi;
-      int mj = dir == 0 ? gj : map_y(d, gj);
-      for (int k = 0; k < nrec; k++) {
-        const double* q = Qe + k * d.PS;
-        double a[5];
-        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? q[pidx(d, mi - 2 + o, mj)] : q[pidx(d, mi, mj - 2 + o)];
-        double L, R;
-        recon5(P.recon, dir == 1 ? P.central_y : 0, a, L, R);
-        wl[k * EX_NW + it] = L;
-        wr[k * EX_NW + it] = R;
-      }
-    }
-    __syncthreads();
-    const int nf = dir == 0 ? (EX_TX + 1) * EX_TY : EX_TX * (EX_TY + 1);
-    const int FW = dir == 0 ? EX_TX + 1 : EX_TX;          // faces per row
-    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-      int fx = f % FW, fy = f / FW;
-      // face between local cells (left, right) along dir; item indices into wl/wr
-      int itL, itR, gli, glj, gri, grj;
-      if (dir == 0) { itL = fy * W + fx; itR = fy * W + fx + 1; gli = i0 + fx - 1; gri = i0 + fx; glj = grj = j0 + fy; }
-      else { itL = fy * W + fx; itR = (fy + 1) * W + fx; gli = gri = i0 + fx; glj = j0 + fy - 1; grj = j0 + fy; }
-      if (gri > d.nx + GC || grj > d.ny + GC) continue;
-      double QL[NVAR], QR[NVAR];
-      for (int k = 0; k < 5; k++) {
-        QL[k] = k < nrec ? wr[k * EX_NW + itL] : 0.0;
-        QR[k] = k < nrec ? wl[k * EX_NW + itR] : 0.0;
-      }
-      if (kind == 1) {
-        for (int k = 5; k < 8; k++) { QL[k] = u[k * d.PS + pidx(d, gli, glj)]; QR[k] = u[k * d.PS + pidx(d, gri, grj)]; }
-      } else {
-        for (int k = 5; k < 8; k++) { QL[k] = 0.0; QR[k] = 0.0; }
-      }
-      double Fo[5];
-      gl_flux(dir, kind, QL, QR, s_max, Fo);
-      for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[k];
-    }
-    __syncthreads();
-    if (own) {
-      long long c = pidx(d, ci, cj);
-      if (dir == 0) {
-        int fm = ty * FW + tx, fp = fm + 1;
-        for (int k = k0; k < k1; k++) acc[k] = Qs[k * d.PS + c] - h / d.dx * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
-      } else {
-        int fm = ty * FW + tx, fp = fm + FW;
-        for (int k = k0; k < k1; k++) u[k * d.PS + c] = acc[k] - h / d.dy * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
-      }
-    }
-  }
+  explicit_dir<0>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
+  explicit_dir<1>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
 }
 
 // ================================================================ magnetic subsystem
