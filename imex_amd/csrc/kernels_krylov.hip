@@ -379,18 +379,20 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partia
 
 // ---------------------------------------------------------------- Arnoldi step j, part 1:
 // v_j = w / h (written), Av_j = M^-1 A v_j (written), H[i][j] = v_i . Av_j for i <= j.
-// JB >= j + 1 is the number of dot accumulators kept in registers.
-template <int OPK, int JB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4))) k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, KrylovDev* K) {
+// The dots of each tile are reduced over the wave right away (a uniform loop over i, so no
+// accumulator array is live across the operator) and summed per wave in LDS, in tile order.
+template <int OPK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
+k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, KrylovDev* K) {
   if (K->done) return;
   constexpr int NC = OpSel<OPK>::NC;
   __shared__ typename OpSel<OPK>::Smem sm;
-  __shared__ double wsum[4][JB];        // running dot sums per wave, in tile order
+  __shared__ double wsum[4][MAXM + 1];
   const Dims& d = OpSel<OPK>::dims(op);
   const double h = K->hcur;
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane < JB) wsum[wid][lane] = 0.0;
+  for (int i = lane; i <= j; i += 64) wsum[wid][i] = 0.0;
   double* vj = V.v[j];
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
@@ -401,33 +403,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK ==
     __syncthreads();
     OpSel<OPK>::load(op, sm, kb.w, h, true, gx0, gy0);
     __syncthreads();
-    double y[NC];
+    double y[NC], vc[NC];
     OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
-    // the dots of this tile are formed after the operator so that their accumulators are not
-    // live during the stencil
-    double acc[JB];
-#pragma unroll
-    for (int i = 0; i < JB; i++) acc[i] = 0.0;
     if (valid) {
       OpSel<OPK>::precond(op, cidx, cx + GC, cy + GC, y);
       for (int c = 0; c < NC; c++) {
         long long e = (long long)c * d.N + cidx;
-        double vc = OpSel<OPK>::center(sm, c, tx, ty);
-        vj[e] = vc;
+        vc[c] = OpSel<OPK>::center(sm, c, tx, ty);
+        vj[e] = vc[c];
         kb.Av[e] = y[c];
-#pragma unroll
-        for (int i = 0; i < JB; i++) {
-          if (i < j) acc[i] += V.v[i][e] * y[c];
-          else if (i == j) acc[i] += vc * y[c];
-        }
       }
     }
-#pragma unroll
-    for (int i = 0; i < JB; i++) {
-      if (i <= j) {
-        double s = wave_sum(acc[i]);
-        if (lane == 0) wsum[wid][i] += s;
+    for (int i = 0; i <= j; i++) {
+      double part = 0.0;
+      if (valid) {
+        const double* vi = V.v[i];
+        for (int c = 0; c < NC; c++) part += (i < j ? vi[(long long)c * d.N + cidx] : vc[c]) * y[c];
       }
+      double s = wave_sum(part);
+      if (lane == 0) wsum[wid][i] += s;
     }
   }
   __syncthreads();
@@ -686,26 +680,13 @@ void gmres_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int o
   k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, outer, skip);
 }
 
-template <int OPK, int JB>
-static void launch_apply_jb(const typename OpSel<OPK>::T& o, const KrylovBufs& kb, const VPtrs& V, int j, KrylovDev* K,
-                            int g, hipStream_t st) {
-  k_arnoldi_apply<OPK, JB><<<g, 256, 0, st>>>(o, kb, V, j, K);
-}
-
 void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
                    KrylovDev* K, hipStream_t st) {
   VPtrs V = make_vptrs(kb, hostV);
   const Dims& d = op == 0 ? mo.d : po.d;
   int g = grid_for_tiles(d, kb.nblk_apply);
-  if (op == 0) {
-    if (j < 8) launch_apply_jb<0, 8>(mo, kb, V, j, K, g, st);
-    else if (j < 16) launch_apply_jb<0, 16>(mo, kb, V, j, K, g, st);
-    else launch_apply_jb<0, 32>(mo, kb, V, j, K, g, st);
-  } else {
-    if (j < 8) launch_apply_jb<1, 8>(po, kb, V, j, K, g, st);
-    else if (j < 16) launch_apply_jb<1, 16>(po, kb, V, j, K, g, st);
-    else launch_apply_jb<1, 32>(po, kb, V, j, K, g, st);
-  }
+  if (op == 0) k_arnoldi_apply<0><<<g, 256, 0, st>>>(mo, kb, V, j, K);
+  else k_arnoldi_apply<1><<<g, 256, 0, st>>>(po, kb, V, j, K);
 }
 void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st) {
   int g = grid_for_tiles(d, kb.nblk_apply);
