@@ -26,11 +26,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(name, steps, warmup, precision, schedule):
+def run_gpu(name, steps, warmup, precision, schedule, tols=None):
     import imex_amd as im
     p, kw = im.case_of(name)
     p.precision = precision
     p.schedule = schedule
+    if tols:
+        p.mixed_inner_tol, p.mixed_outer_tol = tols
     u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
     s = im.Solver(p)
     s.set_state(u, bx, by)
@@ -46,6 +48,7 @@ def run_gpu(name, steps, warmup, precision, schedule):
     s.sync()
     t1 = time.perf_counter()
     stats = s.stats()
+    final = s.get_state()
     # Per-kernel roofline: the same steps again, from the same post-warmup state, with every
     # launch bracketed by HIP events on the solver's stream.  Event timing needs launches queued
     # one by one, so this pass runs the budgeted schedule without the step graph (its first step,
@@ -62,7 +65,7 @@ def run_gpu(name, steps, warmup, precision, schedule):
     prof = s.profile_read()
     s.profile_enable(False)
     s.close()
-    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state)
+    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final)
 
 
 def cpu_baseline(name, state, budget_steps=1):
@@ -91,16 +94,51 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}), flush=True)
 
 
+def summarize(r, steps, label):
+    """Throughput, per-class table (stderr) and the roofline of the dominant kernel class."""
+    p = r["p"]
+    value = p.nx * p.ny * steps / r["elapsed"] / 1e6
+    prof = [k for k in r["prof"] if k["launches"] > 0]
+    tot_ms = sum(k["total_ms"] for k in prof)
+    work = [k for k in prof if k["name"] not in ("early_exit", "ghost_fill", "misc", "krylov_finish")]
+    dom = max(work, key=lambda k: k["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"[bench] {label}: {value:.1f} Mcell-steps/s")
+    for k in sorted(prof, key=lambda k: -k["total_ms"]):
+        gbs = k["bytes"] / (k["total_ms"] * 1e-3) / 1e9 if k["total_ms"] > 0 else 0.0
+        log(f"  {k['name']:22s} n={k['launches']:6d} total {k['total_ms']:9.3f} ms  avg {k['total_ms'] / k['launches'] * 1e3:8.1f} us"
+            f"  {gbs:8.1f} GB/s")
+    log(f"[bench] {label} profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
+        f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
+    st = r["stats"]
+    return {
+        "value": round(value, 3),
+        "ms_per_step": round(r["elapsed"] / steps * 1e3, 3),
+        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
+        "gmres": {k: st[k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters",
+                                     "mixed_refinements")},
+        "schedule": {"budget_steps": st["budget_steps"], "budget_reruns": st["budget_reruns"],
+                     "graph_builds": st["graph_builds"],
+                     "profiled_pass_ms_per_step": round(r["prof_elapsed"] / steps * 1e3, 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ot1024")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", default="graph", choices=["graph", "budget", "host"])
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--mixed-tols", type=float, nargs=2, default=None, metavar=("INNER", "OUTER"),
+                    help="override the mixed form's inner / outer refinement tolerances")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -111,52 +149,52 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU domain decomposition is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import imex_amd as im
-    prec = im.PREC_FP64 if args.precision == "fp64" else im.PREC_MIXED
     sched = {"graph": im.SCHED_GRAPH, "budget": im.SCHED_BUDGET, "host": im.SCHED_HOST}[args.schedule]
-    r = run_gpu(args.workload, args.steps, args.warmup, prec, sched)
-    p = r["p"]
-    cells = p.nx * p.ny
-    value = cells * args.steps / r["elapsed"] / 1e6
-    prof = [k for k in r["prof"] if k["launches"] > 0]
-    tot_ms = sum(k["total_ms"] for k in prof)
-    work = [k for k in prof if k["name"] not in ("early_exit", "ghost_fill", "misc")]
-    dom = max(work, key=lambda k: k["total_ms"])
-    avg_ms = dom["total_ms"] / dom["launches"]
-    bytes_per_launch = dom["bytes"] / dom["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    for k in sorted(prof, key=lambda k: -k["total_ms"]):
-        gbs = k["bytes"] / (k["total_ms"] * 1e-3) / 1e9 if k["total_ms"] > 0 else 0.0
-        log(f"  {k['name']:22s} n={k['launches']:6d} total {k['total_ms']:9.3f} ms  avg {k['total_ms'] / k['launches'] * 1e3:8.1f} us"
-            f"  {gbs:8.1f} GB/s")
-    log(f"[bench] profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
-        f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
+    forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {}
+    for f in forms:
+        prec = im.PREC_FP64 if f == "fp64" else im.PREC_MIXED
+        runs[f] = run_gpu(args.workload, args.steps, args.warmup, prec, sched, args.mixed_tols if f == "mixed" else None)
+    res = {f: summarize(runs[f], args.steps, f) for f in forms}
+    head = "mixed" if "mixed" in res else "fp64"
+    p = runs[head]["p"]
     out = {
-        "metric": "Mcell-steps/s at 1024^2 WENO5 Orszag-Tang (IMEX-RK2 step, one MI355X)",
-        "value": round(value, 3),
+        "metric": "Mcell-steps/s at 1024^2 WENO5 Orszag-Tang, mixed vs FP64 GPU; % HBM peak",
+        "value": res[head]["value"],
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 3),
+        "ms_per_step": res[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == im.PREC_FP64 else "f32-operators/f64-krylov",
+        "dtype": "f64" if head == "fp64" else "f32-operators/f64-krylov",
         "data": "synthetic (Orszag-Tang initial data generated on the host)",
         "config": {"workload": f"orszag_tang_{p.nx}x{p.ny}_weno5_staggerct_rk2", "nx": p.nx, "ny": p.ny,
-                   "recon": "WENO5", "p_floor": p.p_floor, "gamma": p.gamma, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
-        "gmres": {k: r["stats"][k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters")},
-        "schedule": {"mode": args.schedule, "budget_steps": r["stats"]["budget_steps"],
-                     "budget_reruns": r["stats"]["budget_reruns"], "graph_builds": r["stats"]["graph_builds"],
-                     "profiled_pass_ms_per_step": round(r["prof_elapsed"] / args.steps * 1e3, 3)},
+                   "recon": "WENO5", "p_floor": p.p_floor, "gamma": p.gamma, "form": head,
+                   "schedule": args.schedule},
+        "roofline": res[head]["roofline"],
+        "gmres": res[head]["gmres"],
+        "schedule": res[head]["schedule"],
     }
+    if head == "mixed":
+        out["config"]["mixed_inner_tol"] = runs["mixed"]["p"].mixed_inner_tol
+        out["config"]["mixed_outer_tol"] = runs["mixed"]["p"].mixed_outer_tol
+    if "fp64" in res and head != "fp64":
+        out["fp64"] = res["fp64"]
+        a, b = runs["mixed"]["final"][0], runs["fp64"]["final"][0]
+        g = im.GC
+        errs = [float(np.linalg.norm(a[g:-g, g:-g, k] - b[g:-g, g:-g, k]) / max(np.linalg.norm(b[g:-g, g:-g, k]), 1e-300))
+                for k in range(b.shape[-1])]
+        out["mixed_accuracy"] = {"rel_l2_vs_fp64_max": max(errs), "rel_l2_vs_fp64_per_field": [float(f"{e:.3e}") for e in errs],
+                                 "steps_compared": args.warmup + args.steps}
+        out["speedup_mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
     if not args.no_cpu:
         log("[bench] CPU oracle sample")
-        v, secs = cpu_baseline(args.workload, r["state"], args.cpu_steps)
+        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps)
         out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang from the post-warmup "
                                          f"state, faithful mode, {secs:.1f} s"}

@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("gmres_tol", C.c_double), ("gmres_restart", C.c_int), ("gmres_max_outer", C.c_int),
         ("shock_kappa1", C.c_double), ("time_stop", C.c_double),
         ("ct_mode", C.c_int), ("precision", C.c_int),
-        ("mixed_inner_tol", C.c_double),
+        ("mixed_inner_tol", C.c_double), ("mixed_outer_tol", C.c_double),
         ("schedule", C.c_int), ("budget_margin", C.c_int),
     ]
 

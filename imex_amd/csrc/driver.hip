@@ -86,11 +86,23 @@ __global__ void k_budget_restore(double* S, double* pst, int* eflag, const doubl
   if (t < 4) pst[t] = save[S_NSLOTS + t];
   if (t == 0) eflag[0] = esave[0];
 }
-__global__ void k_budget_gmres_check(const KrylovDev* K, int* ctl, int slot) {
-  if (threadIdx.x == 0) {
-    ctl[slot] = K->steps;
-    if (!K->converged) ctl[0] |= 1;
+// GMRES log: ctl[slot] = Arnoldi steps over all sweeps, ctl[slot+1] = sweeps, then the steps of
+// each of the nsw budgeted sweeps.  A FP64-form solve has one sweep and converges inside it; a
+// mixed-form solve converges at the FP64 residual that opens sweep `sweeps`.
+__global__ void k_budget_gmres_check(const KrylovDev* K, int* ctl, int slot, int mixed, int nsw) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    if (mixed) {
+      ctl[slot] = K->steps_total;
+      ctl[slot + 1] = K->outer;
+      if (!K->outer_conv) ctl[0] |= 1;
+    } else {
+      ctl[slot] = K->steps;
+      ctl[slot + 1] = 1;
+      if (!K->converged) ctl[0] |= 1;
+    }
   }
+  if (t < nsw) ctl[slot + 2 + t] = mixed ? (t < K->outer ? K->sweep_steps[t] : 0) : K->steps;
 }
 __global__ void k_budget_newton_check(const KrylovDev* K, int* ctl, int slot) {
   if (threadIdx.x == 0) {
@@ -131,23 +143,46 @@ struct Prof {
   long long n[KC_N] = {0};
 };
 
+// Solve positions within a step: GMRES solve key = (stage*3 + sub)*NIT + Newton iteration with sub 0
+// the magnetic solve, 1/2 the two pressure solves; Newton key = stage*3 + sub.
+#define NIT 20
+#define NGKEY (6 * NIT)
+#define NNKEY 6
+// Budget table of a step: Arnoldi steps per sweep and sweeps per GMRES key, Jacobian solves per
+// Newton key.  It is also the identity of the step's graph.
+struct BudTab { int B[NGKEY], S[NGKEY], N[NNKEY]; };
+
 // One budgeted GMRES (kind 0) or nested-Newton (kind 1) solve of the queued step.
 struct BudgetRec {
-  int kind, op, slot;
+  int kind, op, slot, key = 0;
   int parent = -1, it = -1;    // enclosing Newton solve (index into brecs) and its iteration
-  std::vector<int> ra, ru, rfa, rfu;      // profiler records of the Arnoldi launches and their finishes
-  int rres = -1, rresf = -1, rx = -1;
+  int nsw = 1, B = 0;                     // budgeted sweeps and Arnoldi steps per sweep
+  std::vector<int> ra, ru, rfa, rfu;      // profiler records of the Arnoldi launches and finishes, sweep-major
+  std::vector<int> rres, rresf, rx;       // per sweep (+ the closing residual of a mixed-form solve)
   std::vector<int> rnres, rnresf, rfill, raxpy;   // Newton: residual (+finish) / dp fill / update per iteration
 };
+// Launches of a captured step whose execution depends on the budgets: Arnoldi step j of sweep
+// sw of GMRES key `key` (4 kernels), the residual (+finish) opening sweep sw, the x update of
+// sweep sw, the log kernel of a GMRES solve, and, per Newton key and iteration j, the
+// iteration's work (vslope, dp fill, update) and its residual (+finish).
+enum { TG_ARN, TG_RES, TG_XUPD, TG_GCHK, TG_NIT, TG_NRES };
+struct NodeTag { hipGraphNode_t node; unsigned char kind; short key, sw, j; };
+// A step graph is captured with a capacity budget table; each replay enables the nodes that
+// the current budgets use and disables the rest (disabled nodes are skipped, not launched).
 struct GraphEntry {
-  int key[5];
+  BudTab cap, applied;
+  int parity, state_mode;
+  hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<NodeTag> tags;
   std::vector<BudgetRec> brecs;
   int nslot = 1;
   long long used = 0;
 };
 #define BHIST 8
 #define MAX_GRAPHS 6
+#define NBK 5
+
 
 struct imex_ctx {
   Prof prof;
@@ -193,12 +228,25 @@ struct imex_ctx {
   int* ctlh = nullptr;         // pinned host copy
   int nslot = 1;
   std::vector<BudgetRec> brecs;
-  int bud[3] = {0, 0, 0};      // Arnoldi steps per magnetic / pressure solve, Jacobian solves per Newton solve
-  int hist[3][BHIST];          // per-step maxima of the recent steps
+  // budget history, per kind: 0/1 Arnoldi steps per sweep of a magnetic / pressure
+  // solve, 2 Jacobian solves per Newton solve, 3/4 sweeps per magnetic / pressure solve
+  int hist[NBK][BHIST];        // per-step maxima of the recent steps
   int hist_n = 0, hist_pos = 0;
-  int step_mx[3] = {0, 0, 0};  // maxima of the step being run under host scheduling
+  int step_mx[NBK] = {0, 0, 0, 0, 0};   // maxima of the step being run under host scheduling
+  // the same per solve position (-1: no such solve in that step)
+  int khist[BHIST][NGKEY][2], nhist[BHIST][NNKEY];
+  int kcur[NGKEY][2], ncur[NNKEY];
+  BudTab tab;
+  int stage_idx = 0, sub_idx = 0, newton_it = 0;
+  // mixed form
+  int mixed = 0;
+  float* V32[MAXM + 1];
   std::vector<GraphEntry> graphs;
   long long graph_clock = 0;
+  bool capturing = false;      // launches are being captured into a step graph
+  bool tag_err = false;
+  bool no_toggle = false;      // node enabling failed once: graphs are captured per exact budget table
+  std::vector<NodeTag> tags;
 };
 
 static Dims make_dims(const imex_params& p) {
@@ -288,43 +336,91 @@ static POp p_op(imex_ctx* c, int jacobian, int precond) {
   return o;
 }
 
+static inline int gkey(const imex_ctx* c) {
+  return (c->stage_idx * 3 + c->sub_idx) * NIT + std::min(std::max(c->newton_it, 0), NIT - 1);
+}
+static inline int nkey(const imex_ctx* c) { return c->stage_idx * 3 + c->sub_idx; }
+// Records the node that the launch just captured (single-stream capture: it is the only
+// dependency of the next launch).
+static void tag_last(imex_ctx* c, int kind, int key, int sw, int j) {
+  if (!c->capturing) return;
+  hipStreamCaptureStatus stt;
+  const hipGraphNode_t* deps = nullptr;
+  size_t nd = 0;
+  if (hipStreamGetCaptureInfo_v2(c->st, &stt, nullptr, nullptr, &deps, &nd) != hipSuccess || nd != 1 || !deps) {
+    c->tag_err = true;
+    return;
+  }
+  c->tags.push_back(NodeTag{deps[0], (unsigned char)kind, (short)key, (short)sw, (short)j});
+}
+
+static void cur_reset(imex_ctx* c) {
+  for (int k = 0; k < NGKEY; k++) c->kcur[k][0] = c->kcur[k][1] = -1;
+  for (int k = 0; k < NNKEY; k++) c->ncur[k] = -1;
+}
+static void cur_gmres(imex_ctx* c, int key, int steps, int sweeps) {
+  c->kcur[key][0] = std::max(c->kcur[key][0], steps);
+  c->kcur[key][1] = std::max(c->kcur[key][1], sweeps);
+}
+
 // ---------------------------------------------------------------- GMRES, budgeted schedule
-// One restart cycle of GMRES_Solver (implicit_operators.cpp:823-911) with a fixed number of
-// Arnoldi launches: the kernels are the ones the host-polled loop below uses, so a solve that
-// converges within its budget gives the same result bit for bit.  Launches past convergence
-// exit at once; a solve still unconverged at the end of its budget flags the step for a rerun.
+// GMRES_Solver (implicit_operators.cpp:823-911) with a fixed number of launches: the FP64 form
+// runs one restart cycle of B Arnoldi steps; the mixed form runs nsw refinement sweeps (FP64
+// residual, B FP32 Arnoldi steps, x update) and the FP64 residual that must have converged.
+// The kernels are the ones the host-polled loop below uses, so a solve that converges within
+// its budget gives the same result bit for bit.  Launches past convergence exit at once; a solve
+// still unconverged at the end of its budget flags the step for a rerun.
 static int run_gmres_budget(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b,
                             int x_is_zero, const int* skip, int parent, int it) {
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
-  const int B = std::min(m, std::max(1, c->bud[op]));
+  const int prec = c->mixed;
   BudgetRec r;
-  r.kind = 0; r.op = op; r.slot = c->nslot++; r.parent = parent; r.it = it;
-  const double res_fields = op == 0 ? (x_is_zero ? 6 : 13) : (x_is_zero ? 2 : 5) + po.precond + po.jacobian;
-  int e0 = prof_begin(c);
-  gmres_residual(op, mo, po, c->kb, x, b, c->K, 0, x_is_zero, skip, c->st);
-  r.rres = prof_end(c, e0, op == 0 ? KC_MAG_RES : KC_P_RES, res_fields);
-  e0 = prof_begin(c);
-  gmres_residual_fin(c->d, c->kb, c->K, 0, skip, c->st);
-  r.rresf = prof_end(c, e0, KC_KFIN, 0);
-  for (int j = 0; j < B; j++) {
+  r.kind = 0; r.op = op; r.parent = parent; r.it = it; r.key = gkey(c);
+  r.B = std::min(m, std::max(1, c->tab.B[r.key]));
+  r.nsw = prec ? std::min(MAXSWEEP - 1, std::max(1, c->tab.S[r.key])) : 1;
+  r.slot = c->nslot; c->nslot += 2 + r.nsw;
+  for (int sw = 0; sw <= r.nsw; sw++) {
+    if (sw == r.nsw && !prec) break;
+    const int xz = (x_is_zero && sw == 0) ? 1 : 0;
+    const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
+    const int k = r.key;
+    int e0 = prof_begin(c);
+    gmres_residual(op, prec, mo, po, c->kb, x, b, c->K, sw, xz, skip, c->st);
+    r.rres.push_back(prof_end(c, e0, op == 0 ? KC_MAG_RES : KC_P_RES, res_fields));
+    tag_last(c, TG_RES, k, sw, 0);
     e0 = prof_begin(c);
-    arnoldi_apply(op, mo, po, c->kb, c->V, j, c->K, c->st);
-    r.ra.push_back(prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY, op == 0 ? 13 + 3 * j : 6 + j + po.jacobian));
+    gmres_residual_fin(c->d, prec, c->kb, c->K, sw, skip, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol, c->st);
+    r.rresf.push_back(prof_end(c, e0, KC_KFIN, 0));
+    tag_last(c, TG_RES, k, sw, 0);
+    if (sw == r.nsw) break;
+    const double vb = prec ? 0.5 : 1.0;       // bytes of a basis field relative to an FP64 field
+    for (int j = 0; j < r.B; j++) {
+      e0 = prof_begin(c);
+      arnoldi_apply(op, prec, mo, po, c->kb, c->V, c->V32, j, c->K, c->st);
+      r.ra.push_back(prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY,
+                              op == 0 ? 4 + vb * (9 + 3 * j) : 5 + po.jacobian + vb * (1 + j)));
+      tag_last(c, TG_ARN, k, sw, j);
+      e0 = prof_begin(c);
+      arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
+      r.rfa.push_back(prof_end(c, e0, KC_KFIN, 0));
+      tag_last(c, TG_ARN, k, sw, j);
+      e0 = prof_begin(c);
+      arnoldi_update(nc, prec, c->d, c->kb, c->V, c->V32, j, m, c->K, c->st);
+      r.ru.push_back(prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, vb * (op == 0 ? 9 + 3 * j : 3 + j)));
+      tag_last(c, TG_ARN, k, sw, j);
+      e0 = prof_begin(c);
+      arnoldi_update_fin(nc, prec, c->d, c->kb, j, m, c->K, c->st);
+      r.rfu.push_back(prof_end(c, e0, KC_KFIN, 0));
+      tag_last(c, TG_ARN, k, sw, j);
+    }
     e0 = prof_begin(c);
-    arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
-    r.rfa.push_back(prof_end(c, e0, KC_KFIN, 0));
-    e0 = prof_begin(c);
-    arnoldi_update(nc, c->d, c->kb, c->V, j, m, c->K, c->st);
-    r.ru.push_back(prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, op == 0 ? 9 + 3 * j : 3 + j));
-    e0 = prof_begin(c);
-    arnoldi_update_fin(nc, c->d, c->kb, j, m, c->K, c->st);
-    r.rfu.push_back(prof_end(c, e0, KC_KFIN, 0));
+    gmres_xupdate(nc, prec, c->d, c->kb, c->V, c->V32, x, c->K, c->st);
+    r.rx.push_back(prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (2 + vb * r.B)));
+    tag_last(c, TG_XUPD, k, sw, 0);
   }
-  e0 = prof_begin(c);
-  gmres_xupdate(nc, c->d, c->kb, c->V, x, c->K, c->st);
-  k_budget_gmres_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, r.slot);
-  r.rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (B + 2));
+  k_budget_gmres_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, r.slot, prec, r.nsw);
+  tag_last(c, TG_GCHK, r.key, 0, 0);
   c->brecs.push_back(r);
   return 0;
 }
@@ -344,16 +440,19 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   // IMEX_WARN_GMRES_MAXOUTER when it gives up.
   const long long outmax = std::min<long long>(c->prm.gmres_max_outer > 0 ? c->prm.gmres_max_outer : dim, 4000);
   const int chunk = 2;
+  const int prec = c->mixed;
+  const double vb = prec ? 0.5 : 1.0;       // bytes of a basis field relative to an FP64 field
   if (first_chunk <= 0) first_chunk = c->pred[op];
-  int total = 0, rc = 0;
+  int total = 0, rc = 0, maxsweep = 0;
   int converged = 0;
   long long outer = 0;
   double res0 = 0, res = 0;
   for (outer = 0; outer < outmax; outer++) {
     const int xz = (x_is_zero && outer == 0) ? 1 : 0;
     const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
-    LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(op, mo, po, c->kb, x, b, c->K, (int)outer, xz, outer == 0 ? skip : nullptr, c->st));
-    LAUNCH(KC_KFIN, 0, gmres_residual_fin(c->d, c->kb, c->K, (int)outer, outer == 0 ? skip : nullptr, c->st));
+    const int* sk = (outer == 0 || prec) ? skip : nullptr;
+    LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(op, prec, mo, po, c->kb, x, b, c->K, (int)outer, xz, sk, c->st));
+    LAUNCH(KC_KFIN, 0, gmres_residual_fin(c->d, prec, c->kb, c->K, (int)outer, sk, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol, c->st));
     const long long gen0 = c->prof.gen;
     int recs_a[MAXM], recs_u[MAXM], recs_fa[MAXM], recs_fu[MAXM];
     int j = 0, slot = 0, last = -1;
@@ -363,16 +462,17 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
       int jend = std::min(m, j + (j == 0 ? std::max(first_chunk, 1) : chunk));
       for (; j < jend; j++) {
         int e0 = prof_begin(c);
-        arnoldi_apply(op, mo, po, c->kb, c->V, j, c->K, c->st);
-        recs_a[j] = prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY, op == 0 ? 13 + 3 * j : 6 + j + po.jacobian);
+        arnoldi_apply(op, prec, mo, po, c->kb, c->V, c->V32, j, c->K, c->st);
+        recs_a[j] = prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY,
+                             op == 0 ? 4 + vb * (9 + 3 * j) : 5 + po.jacobian + vb * (1 + j));
         e0 = prof_begin(c);
         arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
         recs_fa[j] = prof_end(c, e0, KC_KFIN, 0);
         e0 = prof_begin(c);
-        arnoldi_update(nc, c->d, c->kb, c->V, j, m, c->K, c->st);
-        recs_u[j] = prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, op == 0 ? 9 + 3 * j : 3 + j);
+        arnoldi_update(nc, prec, c->d, c->kb, c->V, c->V32, j, m, c->K, c->st);
+        recs_u[j] = prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, vb * (op == 0 ? 9 + 3 * j : 3 + j));
         e0 = prof_begin(c);
-        arnoldi_update_fin(nc, c->d, c->kb, j, m, c->K, c->st);
+        arnoldi_update_fin(nc, prec, c->d, c->kb, j, m, c->K, c->st);
         recs_fu[j] = prof_end(c, e0, KC_KFIN, 0);
       }
       HIPCHK(hipMemcpyAsync(&c->Kh[slot].tol, &c->K->tol, sizeof(KrylovDev) - offsetof(KrylovDev, tol), hipMemcpyDeviceToHost, c->st));
@@ -391,8 +491,8 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
     int rx;
     {
       int e0 = prof_begin(c);
-      gmres_xupdate(nc, c->d, c->kb, c->V, x, c->K, c->st);
-      rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (ks.jlast + 3));
+      gmres_xupdate(nc, prec, c->d, c->kb, c->V, c->V32, x, c->K, c->st);
+      rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (2 + vb * (ks.jlast + 1)));
     }
     if (c->prof.on && c->prof.gen == gen0) {      // launches queued past convergence exited at once
       auto idle = [&](int r) { if (r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
@@ -400,6 +500,7 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
       if (ks.jlast < 0) idle(rx);
     }
     total += ks.steps;
+    maxsweep = std::max(maxsweep, ks.steps);
     c->Klast = ks;
     if (outer == 0) res0 = ks.res0;
     res = ks.res;
@@ -407,9 +508,18 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   }
   if (!converged) { rc |= IMEX_WARN_GMRES_MAXOUTER; c->stats.warnings |= IMEX_WARN_GMRES_MAXOUTER; }
   if (it_acc) *it_acc += total;
-  if (rs_acc) *rs_acc += outer;
+  if (prec) c->stats.mixed_refinements += outer;
+  else if (rs_acc) *rs_acc += outer;
   if (total > 0 && outer == 0) c->pred[op] = total;
-  c->step_mx[op] = std::max(c->step_mx[op], outer == 0 ? total : MAXM + 1);
+  if (prec) {
+    c->step_mx[op] = std::max(c->step_mx[op], maxsweep);
+    c->step_mx[3 + op] = std::max(c->step_mx[3 + op], (int)outer);
+    cur_gmres(c, gkey(c), maxsweep, (int)outer);
+  } else {
+    c->step_mx[op] = std::max(c->step_mx[op], outer == 0 ? total : MAXM + 1);
+    c->step_mx[3 + op] = std::max(c->step_mx[3 + op], 1);
+    cur_gmres(c, gkey(c), outer == 0 ? total : MAXM + 1, 1);
+  }
   if (info) { info->iters = total; info->restarts = (int)outer; info->res0 = res0; info->res = res; info->status = rc; }
   return rc;
 }
@@ -424,35 +534,45 @@ static int run_newton_budget(imex_ctx* c, const double* b, double* p) {
   if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL)
     return run_gmres_budget(c, 1, mag_op(c), p_op(c, 0, 1), p, b, 0, nullptr, -1, -1);
   if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(d, c->P, b, c->cur.u, c->S, c->partial, c->K, c->eflag, c->st));
-  const int NB = std::min(c->prm.newton_max, std::max(1, c->bud[2]));
+  const int NB = std::min(c->prm.newton_max, std::max(1, c->tab.N[nkey(c)]));
   const int self = (int)c->brecs.size();
   {
     BudgetRec nr;
-    nr.kind = 1; nr.op = 1; nr.slot = c->nslot++;
+    nr.kind = 1; nr.op = 1; nr.slot = c->nslot++; nr.key = nkey(c);
     c->brecs.push_back(nr);
   }
   const int* conv = &c->K->newton_conv;
   for (int it = 0; it <= NB; it++) {
+    c->newton_it = it;
     POp carrier = p_op(c, 0, 0);
+    const int nk = c->brecs[self].key;
     int e0 = prof_begin(c);
     newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st);
     int rr = prof_end(c, e0, KC_NEWTON_RES, 6);
     c->brecs[self].rnres.push_back(rr);
+    tag_last(c, TG_NRES, nk, 0, it);
     e0 = prof_begin(c);
     newton_residual_fin(d, c->kb, c->K, it, c->prm.newton_tol, c->st);
     rr = prof_end(c, e0, KC_KFIN, 0);
     c->brecs[self].rnresf.push_back(rr);
+    tag_last(c, TG_NRES, nk, 0, it);
     if (it == NB) break;
-    if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
+    if (c->P.eos != IMEX_EOS_IDEAL) {
+      vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
+      tag_last(c, TG_NIT, nk, 0, it);
+    }
     e0 = prof_begin(c);
     k_fill_skip<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0, conv);
     rr = prof_end(c, e0, KC_MISC, 1);
     c->brecs[self].rfill.push_back(rr);
+    tag_last(c, TG_NIT, nk, 0, it);
     run_gmres_budget(c, 1, mag_op(c), p_op(c, 1, 1), c->dp, c->ng, 1, conv, self, it);
+    c->newton_it = 0;
     e0 = prof_begin(c);
     axpy_launch(d.N, p, c->dp, conv, c->st);
     rr = prof_end(c, e0, KC_MISC, 3);
     c->brecs[self].raxpy.push_back(rr);
+    tag_last(c, TG_NIT, nk, 0, it);
   }
   k_budget_newton_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, c->brecs[self].slot);
   if (c->prm.floor_on) {
@@ -480,6 +600,7 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
   double r0 = 0, r = 0;
   const int* conv = &c->K->newton_conv;
   for (it = 0; it < c->prm.newton_max; it++) {
+    c->newton_it = it;
     // residual, then the Jacobian solve; the solve is skipped on the device when the residual
     // has converged, and the host learns it from the GMRES status it polls anyway
     POp carrier = p_op(c, 0, 0);
@@ -501,7 +622,9 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
     c->stats.p_solves++;
     LAUNCH(KC_MISC, 3, axpy_launch(d.N, p, c->dp, conv, c->st));
   }
+  c->newton_it = 0;
   c->stats.newton_iters += it;
+  c->ncur[nkey(c)] = std::max(c->ncur[nkey(c)], it);
   c->pred_newton = std::max(1, it);
   c->step_mx[2] = std::max(c->step_mx[2], it);
   if (it == c->prm.newton_max) { rc |= IMEX_WARN_NEWTON_MAXITER; c->stats.warnings |= IMEX_WARN_NEWTON_MAXITER; }
@@ -527,9 +650,17 @@ static void ensure_qe(imex_ctx* c) {
   c->qe_ready = true;
 }
 
+// TVDReconstruction + RusanovC_Update (kind 0) / Pressure_flux_Update (kind 1); the mixed form
+// reconstructs and forms the fluxes in FP32.
+static void launch_explicit(imex_ctx* c, int kind) {
+  const Dims& d = c->d;
+  if (c->mixed) k_explicit<float><<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, kind, c->cur.Qe, c->cur.Qs, c->cur.u, c->S);
+  else k_explicit<double><<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, kind, c->cur.Qe, c->cur.Qs, c->cur.u, c->S);
+}
+
 static int sub_explicit_conv(imex_ctx* c) {
   const Dims& d = c->d;
-  LAUNCH(KC_EXPL_CONV, 13, (k_explicit<<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
+  LAUNCH(KC_EXPL_CONV, 13, launch_explicit(c, 0));
   double* u = c->cur.u;
   fill_planes(c, {u, u + d.PS, u + 2 * d.PS, u + 3 * d.PS});
   c->hhat_ready = false;
@@ -538,6 +669,7 @@ static int sub_explicit_conv(imex_ctx* c) {
 
 static int sub_mag_update_b(imex_ctx* c) {
   const Dims& d = c->d;
+  c->sub_idx = 0;
   ensure_qe(c);
   LAUNCH(KC_MAG_RHS, 23, (k_mag_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, c->pe, c->bk, c->xk, c->hhat, c->S, c->eflag)));
   fill_planes(c, {c->hhat});
@@ -554,7 +686,7 @@ static int sub_mag_update_b(imex_ctx* c) {
 
 static int sub_explicit_press(imex_ctx* c) {
   const Dims& d = c->d;
-  LAUNCH(KC_EXPL_PRESS, 16, (k_explicit<<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->S)));
+  LAUNCH(KC_EXPL_PRESS, 16, launch_explicit(c, 1));
   double* u = c->cur.u;
   fill_planes(c, {u + d.PS, u + 2 * d.PS, u + 3 * d.PS, u + 4 * d.PS});
   return 0;
@@ -582,10 +714,12 @@ static int sub_pressure(imex_ctx* c) {
   }
   LAUNCH(KC_P_RHS, 15, (k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 0, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S)));
   fill_planes(c, {c->mxs, c->mys});
+  c->sub_idx = 1;
   int rc = run_newton(c, c->pb, c->p, nullptr);
   LAUNCH(KC_P_MOM, 5, (k_p_momentum<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->p, c->mxs, c->mys, u, c->S)));
   fill_planes(c, {u + d.PS, u + 2 * d.PS});
   LAUNCH(KC_P_RHS, 12, (k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 1, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S)));
+  c->sub_idx = 2;
   rc |= run_newton(c, c->pb, c->p, nullptr);
   LAUNCH(KC_P_ENERGY, 9, (k_p_energy<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->p, u, c->eflag)));
   fill_planes(c, {u + 4 * d.PS});
@@ -618,10 +752,12 @@ static int run_stage(imex_ctx* c) {
 }
 
 static void bind_stage0(imex_ctx* c) {
+  c->stage_idx = 0;
   c->cur.u = c->U1; c->cur.Qe = c->u_n; c->cur.Qs = c->u_n;
   c->cur.bxqs = c->bx_n; c->cur.byqs = c->by_n; c->cur.bxo = c->bx1; c->cur.byo = c->by1;
 }
 static void bind_stage1(imex_ctx* c) {
+  c->stage_idx = 1;
   c->cur.u = c->Uo; c->cur.Qe = c->Qe; c->cur.Qs = c->Qs;
   c->cur.bxqs = c->bxqs; c->cur.byqs = c->byqs; c->cur.bxo = c->bx2; c->cur.byo = c->by2;
 }
@@ -657,24 +793,38 @@ static void step_commit(imex_ctx* c) {
   c->stats.steps++;
 }
 
-static void hist_push(imex_ctx* c, const int mx[3]) {
-  for (int k = 0; k < 3; k++) c->hist[k][c->hist_pos] = mx[k];
+static void hist_push(imex_ctx* c, const int mx[NBK]) {
+  for (int k = 0; k < NBK; k++) c->hist[k][c->hist_pos] = mx[k];
+  std::memcpy(c->khist[c->hist_pos], c->kcur, sizeof(c->kcur));
+  std::memcpy(c->nhist[c->hist_pos], c->ncur, sizeof(c->ncur));
   c->hist_pos = (c->hist_pos + 1) % BHIST;
   c->hist_n = std::min(c->hist_n + 1, BHIST);
 }
 
-// Budgets: the largest count of the recent steps, plus budget_margin Arnoldi steps per GMRES
-// solve.  Returns false (host scheduling) until there is history, or when a recent solve
-// needed a restart.
+// Budgets, per solve position: the largest count that position needed in the recent steps
+// (or, for a position not seen lately, the largest of its kind), plus budget_margin Arnoldi
+// steps per GMRES sweep.  Returns false (host scheduling) until there is history, or when a
+// recent solve needed a restart.
 static bool choose_budgets(imex_ctx* c) {
   if (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0) return false;
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
-  int mx[3] = {0, 0, 0};
-  for (int i = 0; i < c->hist_n; i++) for (int k = 0; k < 3; k++) mx[k] = std::max(mx[k], c->hist[k][i]);
-  if (mx[0] >= m || mx[1] >= m) return false;
-  c->bud[0] = std::min(m, std::max(1, mx[0] + c->prm.budget_margin));
-  c->bud[1] = std::min(m, std::max(1, mx[1] + c->prm.budget_margin));
-  c->bud[2] = std::max(1, mx[2]);
+  int mx[NBK] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < c->hist_n; i++) for (int k = 0; k < NBK; k++) mx[k] = std::max(mx[k], c->hist[k][i]);
+  if (mx[0] >= m || mx[1] >= m) return false;              // a recent solve needed a restart
+  if (mx[3] >= MAXSWEEP - 1 || mx[4] >= MAXSWEEP - 1) return false;
+  for (int k = 0; k < NGKEY; k++) {
+    const int op = (k / NIT) % 3 == 0 ? 0 : 1;
+    int st = -1, sw = -1;
+    for (int i = 0; i < c->hist_n; i++) { st = std::max(st, c->khist[i][k][0]); sw = std::max(sw, c->khist[i][k][1]); }
+    if (st < 0) { st = mx[op]; sw = mx[3 + op]; }
+    c->tab.B[k] = std::min(m, std::max(1, st + c->prm.budget_margin));
+    c->tab.S[k] = std::max(1, sw);
+  }
+  for (int k = 0; k < NNKEY; k++) {
+    int nb = -1;
+    for (int i = 0; i < c->hist_n; i++) nb = std::max(nb, c->nhist[i][k]);
+    c->tab.N[k] = std::max(1, nb < 0 ? mx[2] : nb);
+  }
   return true;
 }
 
@@ -687,26 +837,36 @@ static int budget_finish(imex_ctx* c, long long gen0) {
   if (c->ctlh[0]) return 1;
   const bool fix = c->prof.on && c->prof.gen == gen0;
   auto idle = [&](int r) { if (fix && r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
-  int mx[3] = {0, 0, 0};
+  int mx[NBK] = {0, 0, 0, 0, 0};
   for (const BudgetRec& r : c->brecs) {
     if (r.kind == 1) {
       const int nf = c->ctlh[r.slot];
       c->stats.newton_iters += nf;
       mx[2] = std::max(mx[2], nf);
+      c->ncur[r.key] = std::max(c->ncur[r.key], nf);
       for (size_t i = 0; i < r.rnres.size(); i++) if ((int)i > nf) { idle(r.rnres[i]); idle(r.rnresf[i]); }
       for (size_t i = 0; i < r.rfill.size(); i++) if ((int)i >= nf) { idle(r.rfill[i]); idle(r.raxpy[i]); }
       continue;
     }
-    const int steps = c->ctlh[r.slot];
+    const int steps = c->ctlh[r.slot], sweeps = c->ctlh[r.slot + 1];
     const bool skipped = r.parent >= 0 && r.it >= c->ctlh[c->brecs[r.parent].slot];
-    if (skipped) {
-      idle(r.rres); idle(r.rresf); idle(r.rx);
-      for (size_t j = 0; j < r.ra.size(); j++) { idle(r.ra[j]); idle(r.ru[j]); idle(r.rfa[j]); idle(r.rfu[j]); }
-      continue;
+    for (int sw = 0; sw < (int)r.rres.size(); sw++) {
+      const bool live = !skipped && sw <= sweeps;                       // residuals that ran
+      if (!live) { idle(r.rres[sw]); idle(r.rresf[sw]); }
     }
-    for (size_t j = (size_t)steps; j < r.ra.size(); j++) { idle(r.ra[j]); idle(r.ru[j]); idle(r.rfa[j]); idle(r.rfu[j]); }
-    if (steps == 0) idle(r.rx);
-    mx[r.op] = std::max(mx[r.op], steps);
+    for (int sw = 0; sw < r.nsw; sw++) {
+      const int ss = (!skipped && sw < sweeps) ? c->ctlh[r.slot + 2 + sw] : 0;   // Arnoldi steps of this sweep
+      for (int j = ss; j < r.B; j++) {
+        const size_t q = (size_t)sw * r.B + j;
+        idle(r.ra[q]); idle(r.ru[q]); idle(r.rfa[q]); idle(r.rfu[q]);
+      }
+      if (ss == 0) idle(r.rx[sw]);
+      if (!skipped && sw < sweeps) { mx[r.op] = std::max(mx[r.op], ss); cur_gmres(c, r.key, ss, 0); }
+    }
+    if (skipped) continue;
+    mx[3 + r.op] = std::max(mx[3 + r.op], sweeps);
+    cur_gmres(c, r.key, c->mixed ? 0 : steps, sweeps);
+    if (c->mixed) c->stats.mixed_refinements += sweeps;
     if (r.op == 0) { c->stats.mag_solves++; c->stats.mag_gmres_iters += steps; }
     else { c->stats.p_solves++; c->stats.p_gmres_iters += steps; }
   }
@@ -714,35 +874,102 @@ static int budget_finish(imex_ctx* c, long long gen0) {
   return 0;
 }
 
-// Queue (or replay) the budgeted step as a captured hipGraph.  The graph depends on the budgets,
-// on which of the two state/face buffer sets is current and on the state mode.
+// Whether a tagged node runs under budget table tb.
+static bool tag_enabled(const NodeTag& t, const BudTab& tb, int mixed) {
+  if (t.kind == TG_NIT) return t.j < tb.N[t.key];
+  if (t.kind == TG_NRES) return t.j <= tb.N[t.key];
+  const int sub = (t.key / NIT) % 3, it = t.key % NIT;
+  const bool live = sub == 0 || it < tb.N[t.key / NIT];      // the Newton iteration holding the solve runs
+  if (!live) return false;
+  const int S = mixed ? tb.S[t.key] : 1;
+  switch (t.kind) {
+    case TG_ARN: return t.sw < S && t.j < tb.B[t.key];
+    case TG_RES: return !mixed || t.sw <= S;
+    case TG_XUPD: return t.sw < S;
+    default: return true;                                     // TG_GCHK
+  }
+}
+
+static bool tab_covers(const BudTab& cap, const BudTab& t, int mixed) {
+  for (int k = 0; k < NGKEY; k++) {
+    if (t.B[k] > cap.B[k]) return false;
+    if (mixed && t.S[k] > cap.S[k]) return false;
+  }
+  for (int k = 0; k < NNKEY; k++) if (t.N[k] > cap.N[k]) return false;
+  return true;
+}
+
+// Queue (or replay) the budgeted step as a captured hipGraph.  A graph is captured for a
+// capacity table a little above the current budgets (and for the current state/face buffer set
+// and state mode); replays toggle the budget-dependent nodes to the current table.
 static int budget_graph_launch(imex_ctx* c) {
-  const int key[5] = {c->bud[0], c->bud[1], c->bud[2], c->Us == c->U2 ? 0 : 1, c->state_mode};
+  const int parity = c->Us == c->U2 ? 0 : 1;
+  const BudTab cur = c->tab;
   GraphEntry* g = nullptr;
-  for (GraphEntry& e : c->graphs) if (std::memcmp(e.key, key, sizeof(key)) == 0) { g = &e; break; }
+  for (GraphEntry& e : c->graphs) {
+    if (e.parity != parity || e.state_mode != c->state_mode) continue;
+    if (c->no_toggle ? std::memcmp(&e.cap, &cur, sizeof(BudTab)) == 0 : tab_covers(e.cap, cur, c->mixed)) { g = &e; break; }
+  }
   if (!g) {
     if ((int)c->graphs.size() >= MAX_GRAPHS) {     // drop the least recently used graph
       size_t lru = 0;
       for (size_t i = 1; i < c->graphs.size(); i++) if (c->graphs[i].used < c->graphs[lru].used) lru = i;
       (void)hipGraphExecDestroy(c->graphs[lru].exec);
+      (void)hipGraphDestroy(c->graphs[lru].graph);
       c->graphs.erase(c->graphs.begin() + (long)lru);
     }
+    BudTab cap = cur;
+    if (!c->no_toggle) {
+      const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
+      for (int k = 0; k < NGKEY; k++) {
+        cap.B[k] = std::min(m, cur.B[k] + 2);
+        cap.S[k] = c->mixed ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : 1;
+      }
+      for (int k = 0; k < NNKEY; k++) cap.N[k] = std::min(c->prm.newton_max, cur.N[k] + 1);
+    }
+    c->tab = cap;
+    c->tags.clear();
+    c->tag_err = false;
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     int rc = step_body(c);
+    c->capturing = false;
     hipError_t e1 = hipStreamEndCapture(c->st, &graph);
+    c->tab = cur;
     if (rc < 0) { if (graph) (void)hipGraphDestroy(graph); return rc; }
     if (e1 != hipSuccess) return set_err(IMEX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e1));
     GraphEntry ne;
-    std::memcpy(ne.key, key, sizeof(key));
+    ne.cap = cap; ne.applied = cap; ne.parity = parity; ne.state_mode = c->state_mode;
     hipError_t e2 = hipGraphInstantiate(&ne.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e2 != hipSuccess) return set_err(IMEX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    if (e2 != hipSuccess) { (void)hipGraphDestroy(graph); return set_err(IMEX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2)); }
+    ne.graph = graph;
+    if (!c->tag_err) ne.tags = c->tags;
     ne.brecs = c->brecs;
     ne.nslot = c->nslot;
     c->graphs.push_back(ne);
     g = &c->graphs.back();
     c->stats.graph_builds++;
+    if (c->tag_err && !c->no_toggle) {           // nodes cannot be identified: fall back to exact tables
+      c->no_toggle = true;
+      return budget_graph_launch(c);
+    }
+  }
+  if (!c->no_toggle && std::memcmp(&g->applied, &cur, sizeof(BudTab)) != 0) {
+    for (const NodeTag& t : g->tags) {
+      const bool en = tag_enabled(t, cur, c->mixed);
+      if (en == tag_enabled(t, g->applied, c->mixed)) continue;
+      if (hipGraphNodeSetEnabled(g->exec, t.node, en ? 1 : 0) != hipSuccess) {
+        // enabling is not available here: drop the graph and capture per exact table from now on
+        c->no_toggle = true;
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(g->exec);
+        (void)hipGraphDestroy(g->graph);
+        c->graphs.erase(c->graphs.begin() + (g - c->graphs.data()));
+        return budget_graph_launch(c);
+      }
+    }
+    g->applied = cur;
   }
   g->used = ++c->graph_clock;
   c->brecs = g->brecs;
@@ -753,6 +980,7 @@ static int budget_graph_launch(imex_ctx* c) {
 
 static int one_step(imex_ctx* c) {
   int rc = 0;
+  cur_reset(c);
   if (choose_budgets(c)) {
     const long long gen0 = c->prof.gen;
     c->budget = true;
@@ -771,8 +999,9 @@ static int one_step(imex_ctx* c) {
     // a solve outran its budget: restore the step's scalars and rerun it under host scheduling
     c->stats.budget_reruns++;
     k_budget_restore<<<1, 64, 0, c->st>>>(c->S, c->pst, c->eflag, c->bsave, c->esave);
+    cur_reset(c);
   }
-  c->step_mx[0] = c->step_mx[1] = c->step_mx[2] = 0;
+  for (int k = 0; k < NBK; k++) c->step_mx[k] = 0;
   rc = step_body(c);
   hist_push(c, c->step_mx);
   step_commit(c);
@@ -887,6 +1116,7 @@ int imex_default_params(imex_params* p) {
   p->ct_mode = IMEX_CT_CONSISTENT;
   p->precision = IMEX_PREC_FP64;
   p->mixed_inner_tol = 1e-6;
+  p->mixed_outer_tol = 1e-9;
   p->schedule = IMEX_SCHED_GRAPH;
   p->budget_margin = 1;
   return 0;
@@ -998,6 +1228,15 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
     if (i <= m) A(&c->V[i], n3); else c->V[i] = c->V[0];
   }
   A(&c->w, n3); A(&c->Av, n3); A(&c->xk, n3); A(&c->bk, n3); A(&c->tmpN, 6 * (size_t)d.N);
+  c->mixed = prm->precision == IMEX_PREC_MIXED ? 1 : 0;
+  c->kb.w32 = nullptr; c->kb.Av32 = nullptr;
+  for (int i = 0; i <= MAXM; i++) c->V32[i] = nullptr;
+  if (c->mixed) {       // FP32 basis of the inner solves
+    auto AF = [&](float** pp, size_t n) { if (!rc) rc = dalloc(c, pp, n); };
+    for (int i = 0; i <= m; i++) AF(&c->V32[i], n3);
+    for (int i = m + 1; i <= MAXM; i++) c->V32[i] = c->V32[0];
+    AF(&c->kb.w32, n3); AF(&c->kb.Av32, n3);
+  }
   c->nblk_start = 1024;
   c->kb.nblk_apply = 2048; c->kb.nblk_vec = 2048;
   A(&c->partial, (size_t)4096 * (MAXM + 2));
@@ -1012,7 +1251,7 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
   if (!rc) rc = dalloc(c, &c->ctl, 256);
   if (rc) { imex_destroy(c); return rc; }
   c->Us = c->U2; c->Uo = c->U3;
-  c->kb.w = c->w; c->kb.Av = c->Av; c->kb.partial = c->partial;
+  c->kb.w = c->w; c->kb.Av = c->Av; c->kb.partial = c->partial;     // w32 / Av32 set above
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "stream"); }
   if (hipHostMalloc((void**)&c->Kh, 2 * sizeof(KrylovDev), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
   if (hipHostMalloc((void**)&c->Sh, S_NSLOTS * sizeof(double), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
@@ -1045,7 +1284,7 @@ int imex_destroy(imex_ctx* c) {
   if (c->Kh) (void)hipHostFree(c->Kh);
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ctlh) (void)hipHostFree(c->ctlh);
-  for (GraphEntry& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  for (GraphEntry& g : c->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
   for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->st) { (void)hipEventDestroy(c->ev[0]); (void)hipEventDestroy(c->ev[1]); (void)hipStreamDestroy(c->st); }
   delete c;

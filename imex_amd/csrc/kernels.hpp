@@ -5,6 +5,7 @@
 #define MAXM 32            // largest GMRES restart length supported
 #define TX 64              // operator tile (cells) in x
 #define TY 4               // operator tile (cells) in y
+#define MAXSWEEP 16        // refinement sweeps recorded per mixed-form solve
 
 struct PlaneSet { double* p[8]; int n; };
 
@@ -14,8 +15,11 @@ struct KrylovDev {
   double c[MAXM + 2], s[MAXM + 2], gam[MAXM + 2];
   double tol, tol_eff, hcur, res0, res;
   double newton_res0, newton_res;
+  double tol_outer;                    // mixed form: FP64 residual target of the refinement
   int outer, done, converged, jlast, steps, x_is_zero;
   int newton_it, newton_conv, newton_first;
+  int steps_total, outer_conv;         // Arnoldi steps over all sweeps; refinement converged
+  int sweep_steps[MAXSWEEP];           // Arnoldi steps of each refinement sweep
   unsigned cnt_a, cnt_c, cnt_r, cnt_n, cnt_m;
 };
 
@@ -49,6 +53,7 @@ __global__ void k_stage_combine(Dims d, const double* un, const double* u1, doub
 __global__ void k_qe_prep(Dims d, Phys P, const double* Qe, double* pe, double* cf, int* eflag);
 __global__ void k_shock_chi(Dims d, Phys P, const double* Qe, const double* cf, double* chi);
 __global__ void k_shock_eta(Dims d, const double* chi, double* eta);
+template <class T>
 __global__ void k_explicit(Dims d, Phys P, int kind, const double* Qe, const double* Qs, double* u, const double* S);
 __global__ void k_mag_rhs(Dims d, Phys P, const double* u, const double* Qe, const double* Qs, const double* pe,
                           double* b, double* x0, double* hhat, const double* S, int* eflag);
@@ -68,23 +73,29 @@ __global__ void k_divb(Dims d, const double* bx, const double* by, double* parti
 
 // ---- Krylov (kernels_krylov.hip) ----
 struct KrylovBufs {
-  double* w;           // unnormalised next basis vector / residual
-  double* Av;          // preconditioned operator image of the current basis vector
+  double* w;           // unnormalised next basis vector / residual (FP64 form)
+  double* Av;          // preconditioned operator image of the current basis vector (FP64 form)
+  float* w32;          // the same two vectors for the FP32-stored inner solves of the mixed form
+  float* Av32;
   double* partial;     // per-block partial sums, MAXM+2 per block
   int nblk_apply, nblk_vec;
 };
 
-// op = 0 magnetic (3 unknowns per cell), 1 pressure (1 unknown per cell)
-void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
+// op = 0 magnetic (3 unknowns per cell), 1 pressure (1 unknown per cell); prec = 0 FP64 basis and
+// operator, 1 FP32 basis and operator (inner solves of the mixed form; dots, norms, Givens and
+// the residual of the refinement stay FP64)
+void gmres_residual(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
                     KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st);
-void gmres_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip, hipStream_t st);
-void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
-                   KrylovDev* K, hipStream_t st);
+void gmres_residual_fin(const Dims& d, int prec, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip,
+                        double outer_tol, double inner_tol, hipStream_t st);
+void arnoldi_apply(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* V64,
+                   float* const* V32, int j, KrylovDev* K, hipStream_t st);
 void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st);
-void arnoldi_update_fin(int nc, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st);
-void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
-                    hipStream_t st);
-void gmres_xupdate(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, double* x, KrylovDev* K, hipStream_t st);
+void arnoldi_update(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, int j,
+                    int m, KrylovDev* K, hipStream_t st);
+void arnoldi_update_fin(int nc, int prec, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st);
+void gmres_xupdate(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, double* x,
+                   KrylovDev* K, hipStream_t st);
 void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* Ax, hipStream_t st);
 void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
                      int it, double ntol, int* eflag, hipStream_t st);

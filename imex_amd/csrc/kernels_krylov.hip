@@ -19,7 +19,9 @@
 #define IMEX_MAG_WAVES 3
 #endif
 
-struct VPtrs { double* v[MAXM + 1]; };
+// Krylov basis pointers of one storage type (double for the FP64 form, float for the inner
+// solves of the mixed form).
+template <class VT> struct VPtrs { VT* v[MAXM + 1]; };
 
 // compact-coordinate boundary map for one-cell halos (M_P with i in [g-1, n+g])
 __device__ __forceinline__ int cmap(int g, int n, int per) {
@@ -38,8 +40,8 @@ __device__ __forceinline__ void halo_coord(int hidx, int& lx, int& ly) {
   else { int k = hidx - 2 * (TX + 2); lx = (k & 1) ? TX + 1 : 0; ly = 1 + (k >> 1); }
 }
 
-template <int NC>
-__device__ __forceinline__ void load_tile(double* xs, const Dims& d, const double* __restrict__ x, double h, bool scale,
+template <int NC, class T, class VT>
+__device__ __forceinline__ void load_tile(T* xs, const Dims& d, const VT* __restrict__ x, double h, bool scale,
                                           int gx0, int gy0) {
   const int W = TX + 2, HH = TY + 2;
   auto put = [&](int lx, int ly) {
@@ -47,8 +49,8 @@ __device__ __forceinline__ void load_tile(double* xs, const Dims& d, const doubl
     long long e = gx + (long long)d.nx * gy;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-      double v = x[(long long)k * d.N + e];
-      xs[k * W * HH + ly * W + lx] = scale ? v / h : v;
+      double v = (double)x[(long long)k * d.N + e];
+      xs[k * W * HH + ly * W + lx] = (T)(scale ? v / h : v);
     }
   };
   const int t = threadIdx.x;
@@ -63,13 +65,14 @@ __device__ __forceinline__ void load_tile(double* xs, const Dims& d, const doubl
 // By_3+By_4, Bz_3+Bz_4) likewise through its right (left) x-face.  The four-cell corner
 // averages are summed in the order lower-left, lower-right, upper-left, upper-right.  Apart
 // from that summation order the arithmetic is the reference's expression by expression.
-struct MagSmem {
-  double x[3][TY + 2][TX + 2];     // unknown B (scaled), one-cell halo through M_B
-  double q[3][TY + 2][TX + 2];     // B of Qe (raw ghost frame)
-  double r[TY + 2][TX + 2];        // rho of u (raw ghost frame)
+template <class T> struct MagSmem {
+  T x[3][TY + 2][TX + 2];     // unknown B (scaled), one-cell halo through M_B
+  T q[3][TY + 2][TX + 2];     // B of Qe (raw ghost frame)
+  T r[TY + 2][TX + 2];        // rho of u (raw ghost frame)
 };
 
-__device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const double* __restrict__ x, double h, bool scale,
+template <class T, class VT>
+__device__ __forceinline__ void mag_load(const MagOp& op, MagSmem<T>& sm, const VT* __restrict__ x, double h, bool scale,
                                          int gx0, int gy0) {
   const Dims& d = op.d;
   auto put = [&](int lx, int ly) {
@@ -78,14 +81,14 @@ __device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const dou
     long long e = gx + (long long)d.nx * gy;
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-      double v = x[(long long)f * d.N + e];
-      sm.x[f][ly][lx] = scale ? v / h : v;
+      double v = (double)x[(long long)f * d.N + e];
+      sm.x[f][ly][lx] = (T)(scale ? v / h : v);
     }
     int pi = min(cx + GC, d.NX - 1), pj = min(cy + GC, d.NY - 1);
     long long c = pidx(d, pi, pj);
 #pragma unroll
-    for (int f = 0; f < 3; f++) sm.q[f][ly][lx] = op.Qe[(long long)(5 + f) * d.PS + c];
-    sm.r[ly][lx] = op.u[c];
+    for (int f = 0; f < 3; f++) sm.q[f][ly][lx] = (T)op.Qe[(long long)(5 + f) * d.PS + c];
+    sm.r[ly][lx] = (T)op.u[c];
   };
   const int t = threadIdx.x;
   // interior element, then (for t < NHALO) one frame element; kept as two passes so that the
@@ -104,14 +107,15 @@ __device__ __forceinline__ void mag_load(const MagOp& op, MagSmem& sm, const dou
 // Corner (a, b) lies between LDS cells a..a+1 and b..b+1.  Its six terms: n0 b1, n0 b2, n1 b0,
 // n1 b2, PC0 = |n.b|/2 - n0 b0, PC1 = |n.b|/2 - n1 b1 with n, b the four-cell averages of B^e
 // and of the unknown.
-__device__ __forceinline__ void mag_corner(const MagSmem& sm, int a, int b, double* cn) {
-  double n[3], bb[3];
+template <class T>
+__device__ __forceinline__ void mag_corner(const MagSmem<T>& sm, int a, int b, T* cn) {
+  T n[3], bb[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    n[c] = 0.25 * (sm.q[c][b][a] + sm.q[c][b][a + 1] + sm.q[c][b + 1][a] + sm.q[c][b + 1][a + 1]);
-    bb[c] = 0.25 * (sm.x[c][b][a] + sm.x[c][b][a + 1] + sm.x[c][b + 1][a] + sm.x[c][b + 1][a + 1]);
+    n[c] = T(0.25) * (sm.q[c][b][a] + sm.q[c][b][a + 1] + sm.q[c][b + 1][a] + sm.q[c][b + 1][a + 1]);
+    bb[c] = T(0.25) * (sm.x[c][b][a] + sm.x[c][b][a + 1] + sm.x[c][b + 1][a] + sm.x[c][b + 1][a + 1]);
   }
-  double half = 0.5 * (n[0] * bb[0] + n[1] * bb[1] + n[2] * bb[2]);
+  T half = T(0.5) * (n[0] * bb[0] + n[1] * bb[1] + n[2] * bb[2]);
   cn[0] = n[0] * bb[1];
   cn[1] = n[0] * bb[2];
   cn[2] = n[1] * bb[0];
@@ -121,11 +125,12 @@ __device__ __forceinline__ void mag_corner(const MagSmem& sm, int a, int b, doub
 }
 // Per-cell products of LDS cell (a, b): PM - q5 x5, PM - q6 x6, q6 x5, q6 x7, q5 x6, q5 x7
 // with PM = (q . x)/2.
-__device__ __forceinline__ void mag_dv(const MagSmem& sm, int a, int b, double* dv) {
-  double q5 = sm.q[0][b][a], q6 = sm.q[1][b][a], q7 = sm.q[2][b][a];
-  double x5 = sm.x[0][b][a], x6 = sm.x[1][b][a], x7 = sm.x[2][b][a];
-  double q5x5 = q5 * x5, q6x6 = q6 * x6;
-  double PM = 0.5 * (q5x5 + q6x6 + q7 * x7);
+template <class T>
+__device__ __forceinline__ void mag_dv(const MagSmem<T>& sm, int a, int b, T* dv) {
+  T q5 = sm.q[0][b][a], q6 = sm.q[1][b][a], q7 = sm.q[2][b][a];
+  T x5 = sm.x[0][b][a], x6 = sm.x[1][b][a], x7 = sm.x[2][b][a];
+  T q5x5 = q5 * x5, q6x6 = q6 * x6;
+  T PM = T(0.5) * (q5x5 + q6x6 + q7 * x7);
   dv[0] = PM - q5x5;
   dv[1] = PM - q6x6;
   dv[2] = q6 * x5;
@@ -135,49 +140,50 @@ __device__ __forceinline__ void mag_dv(const MagSmem& sm, int a, int b, double* 
 }
 // Fluxes (gx, gz) through the y-face between LDS rows yb and yb+1 at column a; cL, cR are the
 // corners (a-1, yb), (a, yb) and D, U the cell products of (a, yb), (a, yb+1).
-__device__ __forceinline__ void mag_yface(const MagSmem& sm, int a, int yb, const double* cL, const double* cR,
-                                          const double* D, const double* U, double idx, double idy, double midx,
-                                          double& gx, double& gz) {
-  double rf = 0.5 * (sm.r[yb][a] + sm.r[yb + 1][a]);
-  double nf0 = 0.5 * (sm.q[0][yb][a] + sm.q[0][yb + 1][a]);
-  double nf1 = 0.5 * (sm.q[1][yb][a] + sm.q[1][yb + 1][a]);
-  double nf2 = 0.5 * (sm.q[2][yb][a] + sm.q[2][yb + 1][a]);
-  double T1 = idx * (cR[0] - cL[0]) - idy * (U[1] - D[1]);
+template <class T>
+__device__ __forceinline__ void mag_yface(const MagSmem<T>& sm, int a, int yb, const T* cL, const T* cR,
+                                          const T* D, const T* U, T idx, T idy, T midx, T& gx, T& gz) {
+  T rf = T(0.5) * (sm.r[yb][a] + sm.r[yb + 1][a]);
+  T nf0 = T(0.5) * (sm.q[0][yb][a] + sm.q[0][yb + 1][a]);
+  T nf1 = T(0.5) * (sm.q[1][yb][a] + sm.q[1][yb + 1][a]);
+  T nf2 = T(0.5) * (sm.q[2][yb][a] + sm.q[2][yb + 1][a]);
+  T T1 = idx * (cR[0] - cL[0]) - idy * (U[1] - D[1]);
   gx = nf0 / rf * T1 + nf1 / rf * (idx * (cR[4] - cL[4]) - idy * (U[2] - D[2]));
   gz = nf2 / rf * T1 + nf1 / rf * (midx * (cR[1] - cL[1]) - idy * (U[3] - D[3]));
 }
 // Fluxes (fy, fz) through the x-face between LDS columns xa and xa+1 at row b; cB, cT are the
 // corners (xa, b-1), (xa, b) and L, R the cell products of (xa, b), (xa+1, b).
-__device__ __forceinline__ void mag_xface(const MagSmem& sm, int xa, int b, const double* cB, const double* cT,
-                                          const double* L, const double* R, double idy, double midx,
-                                          double& fy, double& fz) {
-  double rf = 0.5 * (sm.r[b][xa] + sm.r[b][xa + 1]);
-  double nf0 = 0.5 * (sm.q[0][b][xa] + sm.q[0][b][xa + 1]);
-  double nf1 = 0.5 * (sm.q[1][b][xa] + sm.q[1][b][xa + 1]);
-  double nf2 = 0.5 * (sm.q[2][b][xa] + sm.q[2][b][xa + 1]);
-  double dpm = R[0] - L[0];
-  double T2 = midx * dpm + idy * (cT[0] - cB[0]);
+template <class T>
+__device__ __forceinline__ void mag_xface(const MagSmem<T>& sm, int xa, int b, const T* cB, const T* cT,
+                                          const T* L, const T* R, T idy, T midx, T& fy, T& fz) {
+  T rf = T(0.5) * (sm.r[b][xa] + sm.r[b][xa + 1]);
+  T nf0 = T(0.5) * (sm.q[0][b][xa] + sm.q[0][b][xa + 1]);
+  T nf1 = T(0.5) * (sm.q[1][b][xa] + sm.q[1][b][xa + 1]);
+  T nf2 = T(0.5) * (sm.q[2][b][xa] + sm.q[2][b][xa + 1]);
+  T dpm = R[0] - L[0];
+  T T2 = midx * dpm + idy * (cT[0] - cB[0]);
   fy = nf1 / rf * T2 + nf0 / rf * (midx * (R[4] - L[4]) + idy * (cT[5] - cB[5]));
   fz = nf2 / rf * (midx * dpm + idy * (cT[2] - cB[2])) + nf0 / rf * (midx * (R[5] - L[5]) - idy * (cT[3] - cB[3]));
 }
 
 // A x for the cell of a valid thread (no block synchronisation: each thread forms the four face
 // fluxes of its own cell from the staged tiles; a face shared by two cells is formed by both
-// with the same arithmetic).
-__device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem& sm, int tx, int ty, bool valid, double* out) {
+// with the same arithmetic).  T is the arithmetic type (double, or float in the mixed form).
+template <class T>
+__device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem<T>& sm, int tx, int ty, bool valid, T* out) {
   if (!valid) return;
   const Dims& d = op.d;
-  const double h = op.S[S_ADT], dx = d.dx, dy = d.dy;
-  const double idx = 1 / dx, idy = 1 / dy, midx = -1 / dx;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
+  const T idx = T(1) / dx, idy = T(1) / dy, midx = T(-1) / dx;
   const int a = tx + 1, b = ty + 1;
-  double c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
+  T c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
   mag_corner(sm, a - 1, b - 1, c00);
   mag_corner(sm, a, b - 1, c10);
   mag_corner(sm, a - 1, b, c01);
   mag_corner(sm, a, b, c11);
-  double dC[6], dN[6];
+  T dC[6], dN[6];
   mag_dv(sm, a, b, dC);
-  double gxu, gzu, gxl, gzl, fyr, fzr, fyl, fzl;
+  T gxu, gzu, gxl, gzl, fyr, fzr, fyl, fzl;
   mag_dv(sm, a, b + 1, dN);
   mag_yface(sm, a, b, c01, c11, dC, dN, idx, idy, midx, gxu, gzu);
   mag_dv(sm, a, b - 1, dN);
@@ -187,14 +193,14 @@ __device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem& sm, 
   mag_dv(sm, a - 1, b, dN);
   mag_xface(sm, a - 1, b, c00, c01, dN, dC, idy, midx, fyl, fzl);
   auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
-  double ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
-  double ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
-  double ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
-  const double nu_x = op.S[S_BMAXX] * dx, nu_y = op.S[S_BMAXY] * dy;
-  double lap[3];
+  T ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
+  T ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
+  T ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
+  const T nu_x = (T)op.S[S_BMAXX] * dx, nu_y = (T)op.S[S_BMAXY] * dy;
+  T lap[3];
 #pragma unroll
   for (int c = 0; c < 3; c++)
-    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - 2.0 * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - 2.0 * X(c, 0, 0) + X(c, 0, 1));
+    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - T(2) * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - T(2) * X(c, 0, 0) + X(c, 0, 1));
   out[0] = ax5 - h * lap[0];
   out[1] = ax6 - h * lap[1];
   out[2] = ax7 - h * lap[2];
@@ -208,63 +214,67 @@ __device__ __forceinline__ double p_vslope(const POp& op, int cidx, double rho) 
   if (!P.floor_on) return 1.0 / (P.gamma - 1.0);
   return (p >= P.p_floor) ? 1.0 / (P.gamma - 1.0) : op.S[S_MBIG];
 }
-__device__ __forceinline__ void p_apply(const POp& op, const double* xs, int lx, int ly, int i, int j, int cidx, double* out) {
+template <class T>
+__device__ __forceinline__ void p_apply(const POp& op, const T* xs, int lx, int ly, int i, int j, int cidx, T* out) {
   const Dims& d = op.d;
   const int W = TX + 2;
-  const double h = op.S[S_ADT], dx = d.dx, dy = d.dy;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
   auto X = [&](int di, int dj) { return xs[(ly + dj) * W + lx + di]; };
-  auto H = [&](int di, int dj) { return op.hhat[pidx(d, i + di, j + dj)]; };
-  auto E = [&](int di, int dj) { return op.eta[pidx(d, i + di, j + dj)]; };
-  double h00 = H(0, 0), hp0 = H(1, 0), hm0 = H(-1, 0), h0p = H(0, 1), h0m = H(0, -1);
-  double pii = X(0, 0), xp0 = X(1, 0), xm0 = X(-1, 0), x0p = X(0, 1), x0m = X(0, -1);
+  auto H = [&](int di, int dj) { return (T)op.hhat[pidx(d, i + di, j + dj)]; };
+  auto E = [&](int di, int dj) { return (T)op.eta[pidx(d, i + di, j + dj)]; };
+  T h00 = H(0, 0), hp0 = H(1, 0), hm0 = H(-1, 0), h0p = H(0, 1), h0m = H(0, -1);
+  T pii = X(0, 0), xp0 = X(1, 0), xm0 = X(-1, 0), x0p = X(0, 1), x0m = X(0, -1);
   double rho = op.u[pidx(d, i, j)];
-  double Vdiag = op.jacobian ? p_vslope(op, cidx, rho) : (1.0 / (op.P.gamma - 1.0));
-  double ax = Vdiag * pii - h * h * (1 / (dx * dx) * (0.5 * (hp0 + h00) * (xp0 - pii) - 0.5 * (h00 + hm0) * (pii - xm0))
-                                   + 1 / (dy * dy) * (0.5 * (h0p + h00) * (x0p - pii) - 0.5 * (h0m + h00) * (pii - x0m)));
-  double e00 = E(0, 0);
-  double nu_x = op.S[S_VXMAX] * dx, nu_y = op.S[S_VYMAX] * dy;
-  double lap = nu_x / (dx * dx) * (0.5 * (E(1, 0) + e00) * (xp0 - pii) - 0.5 * (e00 + E(-1, 0)) * (pii - xm0))
-             + nu_y / (dy * dy) * (0.5 * (E(0, 1) + e00) * (x0p - pii) - 0.5 * (e00 + E(0, -1)) * (pii - x0m));
-  double visc = op.visc ? op.visc[pidx(d, i, j)] : 1.0 / (op.P.gamma - 1.0);
+  T Vdiag = (T)(op.jacobian ? p_vslope(op, cidx, rho) : (1.0 / (op.P.gamma - 1.0)));
+  T ax = Vdiag * pii - h * h * (T(1) / (dx * dx) * (T(0.5) * (hp0 + h00) * (xp0 - pii) - T(0.5) * (h00 + hm0) * (pii - xm0))
+                              + T(1) / (dy * dy) * (T(0.5) * (h0p + h00) * (x0p - pii) - T(0.5) * (h0m + h00) * (pii - x0m)));
+  T e00 = E(0, 0);
+  T nu_x = (T)op.S[S_VXMAX] * dx, nu_y = (T)op.S[S_VYMAX] * dy;
+  T lap = nu_x / (dx * dx) * (T(0.5) * (E(1, 0) + e00) * (xp0 - pii) - T(0.5) * (e00 + E(-1, 0)) * (pii - xm0))
+        + nu_y / (dy * dy) * (T(0.5) * (E(0, 1) + e00) * (x0p - pii) - T(0.5) * (e00 + E(0, -1)) * (pii - x0m));
+  T visc = (T)(op.visc ? op.visc[pidx(d, i, j)] : 1.0 / (op.P.gamma - 1.0));
   out[0] = ax - visc * h * lap;
 }
-__device__ __forceinline__ void p_precond(const POp& op, int cidx, int i, int j, double* y) {
+template <class T>
+__device__ __forceinline__ void p_precond(const POp& op, int cidx, int i, int j, T* y) {
   if (!op.precond) return;
   double rho = op.u[pidx(op.d, i, j)];
   double Vd = op.jacobian ? p_vslope(op, cidx, rho) : 1.0 / (op.P.gamma - 1.0);
-  y[0] = y[0] / (Vd + op.Tdiag[cidx]);
+  y[0] = y[0] / (T)(Vd + op.Tdiag[cidx]);
 }
 
 // Tile interface shared by the GMRES kernels: load the input vector tile (plus whatever the
-// operator stages), then compute M^-1 A x for the block's cells.
-struct PSmem { double x[(TY + 2) * (TX + 2)]; };
-template <int OPK> struct OpSel;
-template <> struct OpSel<0> {
+// operator stages), then compute M^-1 A x for the block's cells in arithmetic type T.
+template <class T> struct PSmem { T x[(TY + 2) * (TX + 2)]; };
+template <int OPK, class T> struct OpSel;
+template <class T> struct OpSel<0, T> {
   static constexpr int NC = 3;
-  typedef MagOp T;
-  typedef MagSmem Smem;
-  __device__ static void load(const MagOp& op, Smem& sm, const double* x, double h, bool scale, int gx0, int gy0) {
+  typedef MagOp Op;
+  typedef MagSmem<T> Smem;
+  template <class VT>
+  __device__ static void load(const MagOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     mag_load(op, sm, x, h, scale, gx0, gy0);
   }
-  __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int, int, int, bool valid, double* y) {
+  __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int, int, int, bool valid, T* y) {
     mag_compute(op, sm, tx, ty, valid, y);
   }
-  __device__ static double center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
-  __device__ static void precond(const MagOp&, int, int, int, double*) {}
+  __device__ static T center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
+  __device__ static void precond(const MagOp&, int, int, int, T*) {}
   __device__ static const Dims& dims(const MagOp& op) { return op.d; }
 };
-template <> struct OpSel<1> {
+template <class T> struct OpSel<1, T> {
   static constexpr int NC = 1;
-  typedef POp T;
-  typedef PSmem Smem;
-  __device__ static void load(const POp& op, Smem& sm, const double* x, double h, bool scale, int gx0, int gy0) {
+  typedef POp Op;
+  typedef PSmem<T> Smem;
+  template <class VT>
+  __device__ static void load(const POp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     load_tile<1>(sm.x, op.d, x, h, scale, gx0, gy0);
   }
-  __device__ static void compute(const POp& op, Smem& sm, int tx, int ty, int i, int j, int cidx, bool valid, double* y) {
+  __device__ static void compute(const POp& op, Smem& sm, int tx, int ty, int i, int j, int cidx, bool valid, T* y) {
     if (valid) p_apply(op, sm.x, tx + 1, ty + 1, i, j, cidx, y);
   }
-  __device__ static double center(const Smem& sm, int, int tx, int ty) { return sm.x[(ty + 1) * (TX + 2) + tx + 1]; }
-  __device__ static void precond(const POp& op, int cidx, int i, int j, double* y) { p_precond(op, cidx, i, j, y); }
+  __device__ static T center(const Smem& sm, int, int tx, int ty) { return sm.x[(ty + 1) * (TX + 2) + tx + 1]; }
+  __device__ static void precond(const POp& op, int cidx, int i, int j, T* y) { p_precond(op, cidx, i, j, y); }
   __device__ static const Dims& dims(const POp& op) { return op.d; }
 };
 
@@ -319,21 +329,28 @@ __device__ __forceinline__ void final_sums(const double* partial, int stride, in
 }
 
 // ---------------------------------------------------------------- GMRES residual r0 = M^-1 (b - A x), gam0 = ||r0||
-template <int OPK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4))) k_gmres_residual(typename OpSel<OPK>::T op, KrylovBufs kb, const double* __restrict__ x,
-                                                        const double* __restrict__ b, KrylovDev* K, int outer, int x_is_zero,
-                                                        const int* skip) {
+// Always in FP64 (it is the outer residual of the mixed form's iterative refinement); r0 is
+// stored in the basis type VT, its norm is taken in FP64 before rounding.
+template <int OPK, class VT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
+k_gmres_residual(typename OpSel<OPK, double>::Op op, KrylovBufs kb, VT* __restrict__ w, const double* __restrict__ x,
+                 const double* __restrict__ b, KrylovDev* K, int outer, int x_is_zero, const int* skip, int mixed) {
   if (skip && *skip) {           // the caller's outer iteration has already converged: no solve
-    if (blockIdx.x == 0 && threadIdx.x == 0) { K->done = 1; K->converged = 1; K->jlast = -1; K->steps = 0; K->outer = outer; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      K->done = 1; K->converged = 1; K->jlast = -1; K->steps = 0; K->outer = outer; K->outer_conv = 1;
+      if (outer == 0) K->steps_total = 0;
+    }
     return;
   }
-  constexpr int NC = OpSel<OPK>::NC;
-  __shared__ typename OpSel<OPK>::Smem sm;
-  __shared__ double red[4];
-  __shared__ double res[1];
-  const Dims& d = OpSel<OPK>::dims(op);
+  if (mixed && outer > 0 && K->outer_conv) return;    // refinement finished in an earlier sweep
+  typedef OpSel<OPK, double> S;
+  constexpr int NC = S::NC;
+  __shared__ typename S::Smem sm;
+  __shared__ double red[8];
+  __shared__ double res[2];
+  const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
-  double acc = 0.0;
+  double acc[2] = {0.0, 0.0};     // ||M^-1 r||^2, and ||M^-1 b||^2 on the first sweep of the mixed form
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -343,57 +360,78 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK ==
     double y[NC];
     if (!x_is_zero) {
       __syncthreads();
-      OpSel<OPK>::load(op, sm, x, 1.0, false, gx0, gy0);
+      S::load(op, sm, x, 1.0, false, gx0, gy0);
       __syncthreads();
-      OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+      S::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
     }
     if (valid) {
+      if (mixed && outer == 0) {
+        double bb[NC];
+        for (int c = 0; c < NC; c++) bb[c] = b[(long long)c * d.N + cidx];
+        S::precond(op, cidx, cx + GC, cy + GC, bb);
+        for (int c = 0; c < NC; c++) acc[1] += bb[c] * bb[c];
+      }
       if (x_is_zero) for (int c = 0; c < NC; c++) y[c] = b[(long long)c * d.N + cidx];
       else for (int c = 0; c < NC; c++) y[c] = b[(long long)c * d.N + cidx] - y[c];
-      OpSel<OPK>::precond(op, cidx, cx + GC, cy + GC, y);
-      for (int c = 0; c < NC; c++) { kb.w[(long long)c * d.N + cidx] = y[c]; acc += y[c] * y[c]; }
+      S::precond(op, cidx, cx + GC, cy + GC, y);
+      for (int c = 0; c < NC; c++) { w[(long long)c * d.N + cidx] = (VT)y[c]; acc[0] += y[c] * y[c]; }
     }
   }
-  block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
+  block_sums<2>(acc, 2, red, res);
+  __syncthreads();
+  if (threadIdx.x < 2) kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = res[threadIdx.x];
 }
 
 // One-block finish of the residual norm: gam0, tolerance, convergence (implicit_operators.cpp:829-836).
+// Mixed form: the solve is converged when the FP64 residual is below outer_tol ||M^-1 b|| +
+// gmres_tol; otherwise the inner solve of this sweep runs to inner_tol relative to this residual.
 __global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partial, int nblk, KrylovDev* K, int outer,
-                                                            const int* skip) {
+                                                            const int* skip, int mixed, double outer_tol, double inner_tol) {
   if (skip && *skip) return;
-  __shared__ double tot[1];
-  final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
+  if (mixed && outer > 0 && K->outer_conv) return;
+  __shared__ double tot[2];
+  final_sums<2>(partial, MAXM + 2, nblk, 2, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double g0 = sqrt(tot[0]);
-    if (outer == 0) { K->tol_eff = K->tol * g0 + K->tol; K->res0 = g0; }
+    if (outer == 0) {
+      K->tol_eff = K->tol * g0 + K->tol; K->res0 = g0; K->steps_total = 0; K->outer_conv = 0;
+      K->tol_outer = outer_tol * sqrt(tot[1]) + K->tol;
+    }
     K->res = g0;
     K->gam[0] = g0;
     K->outer = outer;
     K->steps = 0;
-    if (g0 <= K->tol_eff) { K->done = 1; K->converged = 1; K->jlast = -1; }
-    else { K->done = 0; K->converged = 0; K->jlast = -1; K->hcur = g0; }
+    if (outer < MAXSWEEP) K->sweep_steps[outer] = 0;
+    const double tol = mixed ? K->tol_outer : K->tol_eff;
+    if (g0 <= tol) { K->done = 1; K->converged = 1; K->jlast = -1; K->outer_conv = 1; }
+    else {
+      K->done = 0; K->converged = 0; K->jlast = -1; K->hcur = g0;
+      if (mixed) K->tol_eff = inner_tol * g0;
+    }
   }
 }
 
 // ---------------------------------------------------------------- Arnoldi step j, part 1:
 // v_j = w / h (written), Av_j = M^-1 A v_j (written), H[i][j] = v_i . Av_j for i <= j.
+// T is the operator's arithmetic type, VT the basis storage type; dots are summed in FP64.
 // The dots of each tile are reduced over the wave right away (a uniform loop over i, so no
 // accumulator array is live across the operator) and summed per wave in LDS, in tile order.
-template <int OPK>
+template <int OPK, class T, class VT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
-k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, KrylovDev* K) {
+k_arnoldi_apply(typename OpSel<OPK, T>::Op op, KrylovBufs kb, VT* __restrict__ w, VT* __restrict__ Av, VPtrs<VT> V,
+                int j, KrylovDev* K) {
   if (K->done) return;
-  constexpr int NC = OpSel<OPK>::NC;
-  __shared__ typename OpSel<OPK>::Smem sm;
+  typedef OpSel<OPK, T> S;
+  constexpr int NC = S::NC;
+  __shared__ typename S::Smem sm;
   __shared__ double wsum[4][MAXM + 1];
-  const Dims& d = OpSel<OPK>::dims(op);
+  const Dims& d = S::dims(op);
   const double h = K->hcur;
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lane; i <= j; i += 64) wsum[wid][i] = 0.0;
-  double* vj = V.v[j];
+  VT* vj = V.v[j];
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -401,24 +439,28 @@ k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, Krylov
     bool valid = cx < d.nx && cy < d.ny;
     int cidx = cx + d.nx * cy;
     __syncthreads();
-    OpSel<OPK>::load(op, sm, kb.w, h, true, gx0, gy0);
+    S::load(op, sm, w, h, true, gx0, gy0);
     __syncthreads();
-    double y[NC], vc[NC];
-    OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+    T y[NC];
+    double vc[NC];
+    S::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
     if (valid) {
-      OpSel<OPK>::precond(op, cidx, cx + GC, cy + GC, y);
+      S::precond(op, cidx, cx + GC, cy + GC, y);
       for (int c = 0; c < NC; c++) {
         long long e = (long long)c * d.N + cidx;
-        vc[c] = OpSel<OPK>::center(sm, c, tx, ty);
-        vj[e] = vc[c];
-        kb.Av[e] = y[c];
+        VT vs = (VT)S::center(sm, c, tx, ty);
+        VT as = (VT)y[c];
+        vj[e] = vs;
+        Av[e] = as;
+        vc[c] = (double)vs;
+        y[c] = (T)as;
       }
     }
     for (int i = 0; i <= j; i++) {
       double part = 0.0;
       if (valid) {
-        const double* vi = V.v[i];
-        for (int c = 0; c < NC; c++) part += (i < j ? vi[(long long)c * d.N + cidx] : vc[c]) * y[c];
+        const VT* vi = V.v[i];
+        for (int c = 0; c < NC; c++) part += (i < j ? (double)vi[(long long)c * d.N + cidx] : vc[c]) * (double)y[c];
       }
       double s = wave_sum(part);
       if (lane == 0) wsum[wid][i] += s;
@@ -428,7 +470,7 @@ k_arnoldi_apply(typename OpSel<OPK>::T op, KrylovBufs kb, VPtrs V, int j, Krylov
   if ((int)threadIdx.x <= j) {
     const int nw = blockDim.x >> 6;
     double s = 0.0;
-    for (int w = 0; w < nw; w++) s += wsum[w][threadIdx.x];
+    for (int ww = 0; ww < nw; ww++) s += wsum[ww][threadIdx.x];
     kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = s;
   }
 }
@@ -444,9 +486,10 @@ __global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const double* partial,
 }
 
 // ---------------------------------------------------------------- Arnoldi step j, part 2:
-// w = Av_j - sum_i H[i][j] v_i, H[j+1][j] = ||w||, Givens rotations, convergence.
-template <int JB>
-__global__ void __launch_bounds__(256) k_arnoldi_update(KrylovBufs kb, VPtrs V, int j, int m, long long n, KrylovDev* K) {
+// w = Av_j - sum_i H[i][j] v_i (FP64 arithmetic, stored as VT), ||w|| of the stored w.
+template <int JB, class VT>
+__global__ void __launch_bounds__(256) k_arnoldi_update(KrylovBufs kb, VT* __restrict__ w, const VT* __restrict__ Av,
+                                                        VPtrs<VT> V, int j, int m, long long n, KrylovDev* K) {
   if (K->done) return;
   __shared__ double Hc[JB];
   __shared__ double red[4];
@@ -458,18 +501,22 @@ __global__ void __launch_bounds__(256) k_arnoldi_update(KrylovBufs kb, VPtrs V, 
   for (int i = 0; i < JB; i++) hr[i] = i <= j ? Hc[i] : 0.0;
   double acc = 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
-    double w = kb.Av[e];
+    double wv = (double)Av[e];
 #pragma unroll
-    for (int i = 0; i < JB; i++) if (i <= j) w -= hr[i] * V.v[i][e];
-    kb.w[e] = w;
-    acc += w * w;
+    for (int i = 0; i < JB; i++) if (i <= j) wv -= hr[i] * (double)V.v[i][e];
+    VT ws = (VT)wv;
+    w[e] = ws;
+    acc += (double)ws * (double)ws;
   }
   block_sums<1>(&acc, 1, red, res);
   if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
 }
 
 // One-block finish of the update: H[j+1][j] = ||w||, Givens rotations, convergence (implicit_operators.cpp:857-897).
-__global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partial, int nblk, int j, int m, KrylovDev* K) {
+// In the mixed form an inner solve that reaches its tolerance ends the sweep but not the solve:
+// convergence is decided by the next FP64 residual.
+__global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partial, int nblk, int j, int m, KrylovDev* K,
+                                                            int mixed) {
   if (K->done) return;
   __shared__ double tot[1];
   final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
@@ -492,17 +539,20 @@ __global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partia
     K->gam[j] = K->c[j + 1] * K->gam[j];
     K->res = fabs(K->gam[j + 1]);
     K->steps = j + 1;
+    K->steps_total += 1;
+    if (K->outer < MAXSWEEP) K->sweep_steps[K->outer] = j + 1;
     if (K->gam[j + 1] > K->tol_eff) {
       K->hcur = H[(j + 1) * L + j];
       if (j + 1 == m) { K->done = 1; K->converged = 0; K->jlast = j; }
     } else {
-      K->done = 1; K->converged = 1; K->jlast = j;
+      K->done = 1; K->converged = mixed ? 0 : 1; K->jlast = j;
     }
   }
 }
 
 // ---------------------------------------------------------------- x += sum alpha_i v_i (back substitution per block)
-__global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs V, double* __restrict__ x, long long n, KrylovDev* K) {
+template <class VT>
+__global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs<VT> V, double* __restrict__ x, long long n, KrylovDev* K) {
   __shared__ double alpha[MAXM + 1];
   const int jl = K->jlast;
   if (jl < 0) return;
@@ -518,17 +568,19 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs V, double* __restri
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
     double xv = x[e];
 #pragma unroll
-    for (int i = 0; i <= MAXM; i++) if (i <= jl) xv += alpha[i] * V.v[i][e];
+    for (int i = 0; i <= MAXM; i++) if (i <= jl) xv += alpha[i] * (double)V.v[i][e];
     x[e] = xv;
   }
 }
 
 // ---------------------------------------------------------------- plain operator application (tests, adjoint)
 template <int OPK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4))) k_op_apply(typename OpSel<OPK>::T op, const double* __restrict__ x, double* __restrict__ Ax) {
-  constexpr int NC = OpSel<OPK>::NC;
-  __shared__ typename OpSel<OPK>::Smem sm;
-  const Dims& d = OpSel<OPK>::dims(op);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
+k_op_apply(typename OpSel<OPK, double>::Op op, const double* __restrict__ x, double* __restrict__ Ax) {
+  typedef OpSel<OPK, double> S;
+  constexpr int NC = S::NC;
+  __shared__ typename S::Smem sm;
+  const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
@@ -537,10 +589,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK ==
     bool valid = cx < d.nx && cy < d.ny;
     int cidx = cx + d.nx * cy;
     __syncthreads();
-    OpSel<OPK>::load(op, sm, x, 1.0, false, gx0, gy0);
+    S::load(op, sm, x, 1.0, false, gx0, gy0);
     __syncthreads();
     double y[NC];
-    OpSel<OPK>::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
+    S::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
     if (valid) for (int c = 0; c < NC; c++) Ax[(long long)c * d.N + cidx] = y[c];
   }
 }
@@ -564,7 +616,7 @@ __global__ void __launch_bounds__(256) k_newton_residual(POp op, KrylovBufs kb, 
   for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     __syncthreads();
-    load_tile<1>(xs, d, p, 1.0, false, gx0, gy0);
+    load_tile<1, double, double>(xs, d, p, 1.0, false, gx0, gy0);
     __syncthreads();
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
@@ -660,33 +712,43 @@ static inline int grid_for_tiles(const Dims& d, int cap) {
   int nt = ((d.nx + TX - 1) / TX) * ((d.ny + TY - 1) / TY);
   return nt < cap ? nt : cap;
 }
-static VPtrs make_vptrs(const KrylovBufs&, double* const* hostV) {
-  VPtrs V;
+template <class VT>
+static VPtrs<VT> make_vptrs(VT* const* hostV) {
+  VPtrs<VT> V;
   for (int i = 0; i <= MAXM; i++) V.v[i] = hostV[i];
   return V;
 }
 
-void gmres_residual(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
+void gmres_residual(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
                     KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st) {
-  if (op == 0) {
-    int g = grid_for_tiles(mo.d, kb.nblk_apply);
-    k_gmres_residual<0><<<g, 256, 0, st>>>(mo, kb, x, b, K, outer, x_is_zero, skip);
-  } else {
-    int g = grid_for_tiles(po.d, kb.nblk_apply);
-    k_gmres_residual<1><<<g, 256, 0, st>>>(po, kb, x, b, K, outer, x_is_zero, skip);
-  }
-}
-void gmres_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip, hipStream_t st) {
-  k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, outer, skip);
-}
-
-void arnoldi_apply(int op, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* hostV, int j,
-                   KrylovDev* K, hipStream_t st) {
-  VPtrs V = make_vptrs(kb, hostV);
   const Dims& d = op == 0 ? mo.d : po.d;
   int g = grid_for_tiles(d, kb.nblk_apply);
-  if (op == 0) k_arnoldi_apply<0><<<g, 256, 0, st>>>(mo, kb, V, j, K);
-  else k_arnoldi_apply<1><<<g, 256, 0, st>>>(po, kb, V, j, K);
+  if (prec == 0) {
+    if (op == 0) k_gmres_residual<0, double><<<g, 256, 0, st>>>(mo, kb, kb.w, x, b, K, outer, x_is_zero, skip, 0);
+    else k_gmres_residual<1, double><<<g, 256, 0, st>>>(po, kb, kb.w, x, b, K, outer, x_is_zero, skip, 0);
+  } else {
+    if (op == 0) k_gmres_residual<0, float><<<g, 256, 0, st>>>(mo, kb, kb.w32, x, b, K, outer, x_is_zero, skip, 1);
+    else k_gmres_residual<1, float><<<g, 256, 0, st>>>(po, kb, kb.w32, x, b, K, outer, x_is_zero, skip, 1);
+  }
+}
+void gmres_residual_fin(const Dims& d, int prec, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip,
+                        double outer_tol, double inner_tol, hipStream_t st) {
+  k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, outer, skip, prec, outer_tol, inner_tol);
+}
+
+void arnoldi_apply(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* V64,
+                   float* const* V32, int j, KrylovDev* K, hipStream_t st) {
+  const Dims& d = op == 0 ? mo.d : po.d;
+  int g = grid_for_tiles(d, kb.nblk_apply);
+  if (prec == 0) {
+    VPtrs<double> V = make_vptrs(V64);
+    if (op == 0) k_arnoldi_apply<0, double, double><<<g, 256, 0, st>>>(mo, kb, kb.w, kb.Av, V, j, K);
+    else k_arnoldi_apply<1, double, double><<<g, 256, 0, st>>>(po, kb, kb.w, kb.Av, V, j, K);
+  } else {
+    VPtrs<float> V = make_vptrs(V32);
+    if (op == 0) k_arnoldi_apply<0, float, float><<<g, 256, 0, st>>>(mo, kb, kb.w32, kb.Av32, V, j, K);
+    else k_arnoldi_apply<1, float, float><<<g, 256, 0, st>>>(po, kb, kb.w32, kb.Av32, V, j, K);
+  }
 }
 void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st) {
   int g = grid_for_tiles(d, kb.nblk_apply);
@@ -695,29 +757,35 @@ void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K,
   else k_arnoldi_dots_fin<32><<<1, 256, 0, st>>>(kb.partial, g, j, K);
 }
 
-void arnoldi_update(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, int j, int m, KrylovDev* K,
-                    hipStream_t st) {
-  VPtrs V = make_vptrs(kb, hostV);
+static inline int vec_grid(int nc, const Dims& d, const KrylovBufs& kb) {
   long long n = (long long)nc * d.N;
   long long nb = (n + 255) / 256;
-  int g = (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
-  if (j < 8) k_arnoldi_update<8><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
-  else if (j < 16) k_arnoldi_update<16><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
-  else k_arnoldi_update<32><<<g, 256, 0, st>>>(kb, V, j, m, n, K);
+  return (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
 }
-void arnoldi_update_fin(int nc, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st) {
+template <class VT>
+static void launch_update(const KrylovBufs& kb, VT* w, const VT* Av, const VPtrs<VT>& V, int j, int m, long long n,
+                          KrylovDev* K, int g, hipStream_t st) {
+  if (j < 8) k_arnoldi_update<8, VT><<<g, 256, 0, st>>>(kb, w, Av, V, j, m, n, K);
+  else if (j < 16) k_arnoldi_update<16, VT><<<g, 256, 0, st>>>(kb, w, Av, V, j, m, n, K);
+  else k_arnoldi_update<32, VT><<<g, 256, 0, st>>>(kb, w, Av, V, j, m, n, K);
+}
+void arnoldi_update(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, int j,
+                    int m, KrylovDev* K, hipStream_t st) {
   long long n = (long long)nc * d.N;
-  long long nb = (n + 255) / 256;
-  int g = (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
-  k_arnoldi_givens_fin<<<1, 256, 0, st>>>(kb.partial, g, j, m, K);
+  int g = vec_grid(nc, d, kb);
+  if (prec == 0) launch_update(kb, kb.w, (const double*)kb.Av, make_vptrs(V64), j, m, n, K, g, st);
+  else launch_update(kb, kb.w32, (const float*)kb.Av32, make_vptrs(V32), j, m, n, K, g, st);
+}
+void arnoldi_update_fin(int nc, int prec, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st) {
+  k_arnoldi_givens_fin<<<1, 256, 0, st>>>(kb.partial, vec_grid(nc, d, kb), j, m, K, prec);
 }
 
-void gmres_xupdate(int nc, const Dims& d, const KrylovBufs& kb, double* const* hostV, double* x, KrylovDev* K, hipStream_t st) {
-  VPtrs V = make_vptrs(kb, hostV);
+void gmres_xupdate(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, double* x,
+                   KrylovDev* K, hipStream_t st) {
   long long n = (long long)nc * d.N;
-  long long nb = (n + 255) / 256;
-  int g = (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
-  k_gmres_xupdate<<<g, 256, 0, st>>>(V, x, n, K);
+  int g = vec_grid(nc, d, kb);
+  if (prec == 0) k_gmres_xupdate<double><<<g, 256, 0, st>>>(make_vptrs(V64), x, n, K);
+  else k_gmres_xupdate<float><<<g, 256, 0, st>>>(make_vptrs(V32), x, n, K);
 }
 
 void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* Ax, hipStream_t st) {

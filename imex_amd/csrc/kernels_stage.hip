@@ -6,6 +6,7 @@
 // every padded cell through the boundary index map, which produces the same
 // values as the reference's copy-type ghost fill.
 #include "kernels.hpp"
+#include <type_traits>
 
 // ================================================================ ghost fill
 // bound_cond_u / bound_cond_array (boundary.cpp:5-175, timestep.cpp:166-222) and
@@ -233,81 +234,96 @@ __global__ void k_shock_eta(Dims d, const double* __restrict__ chi, double* __re
 
 // ================================================================ reconstruction + explicit fluxes
 // TVDReconstruction (reconstruction.cpp:5-287) at one cell from the 5-point stencil a[0..4].
-__device__ __forceinline__ void recon5(int recon, int central, const double* a, double& wL, double& wR) {
-  const double um2 = a[0], um1 = a[1], u0 = a[2], up1 = a[3], up2 = a[4];
+// FP32 reciprocal for the mixed form's flux kernels (v_rcp_f32, about 1 ulp)
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+
+template <class T>
+__device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL, T& wR) {
+  const T um2 = a[0], um1 = a[1], u0 = a[2], up1 = a[3], up2 = a[4];
   if (recon == IMEX_RECON_MINMOD) {
-    double sR = up1 - u0, sL = u0 - um1, c;
-    if (sL * sR <= 0) c = 0;
+    T sR = up1 - u0, sL = u0 - um1, c;
+    if (sL * sR <= T(0)) c = T(0);
     else if (fabs(sR) <= fabs(sL)) c = sR;
     else c = sL;
-    if (central) c = 0.5 * (sL + sR);
-    wL = u0 - 0.5 * c; wR = u0 + 0.5 * c;
+    if (central) c = T(0.5) * (sL + sR);
+    wL = u0 - T(0.5) * c; wR = u0 + T(0.5) * c;
   } else if (recon == IMEX_RECON_VANLEER) {
     // c = min(2/(1+r), 2r/(1+r)) = 2 min(1, r)/(1+r) for r >= 0
-    double sR = up1 - u0, sL = u0 - um1, c;
-    if (fabs(sR) < 1e-16) c = 0;
-    else { double r = sL / sR; c = (r < 0) ? 0 : 2 * fmin(1.0, r) / (1 + r); }
-    wL = u0 - 0.5 * c * sL; wR = u0 + 0.5 * c * sR;
+    T sR = up1 - u0, sL = u0 - um1, c;
+    if (fabs(sR) < T(1e-16)) c = T(0);
+    else {
+      T r = std::is_same<T, float>::value ? sL * rcp_t(sR) : sL / sR;
+      c = (r < T(0)) ? T(0) : (std::is_same<T, float>::value ? T(2) * fmin(T(1), r) * rcp_t(T(1) + r)
+                                                               : T(2) * fmin(T(1), r) / (T(1) + r));
+    }
+    wL = u0 - T(0.5) * c * sL; wR = u0 + T(0.5) * c * sR;
   } else {
-    // Jiang-Shu weights w_k = (d_k/e_k) / sum_m (d_m/e_m), e_k = (eps + beta_k)^2, written with
-    // the common denominator e0 e1 e2 so that each face value costs one division.
-    double t0 = um2 - 2 * um1 + u0, t1 = um2 - 4 * um1 + 3 * u0;
-    double beta0 = 13.0 / 12.0 * (t0 * t0) + 1.0 / 4.0 * (t1 * t1);
-    double t2 = um1 - 2 * u0 + up1, t3 = um1 - up1;
-    double beta1 = 13.0 / 12.0 * (t2 * t2) + 1.0 / 4.0 * (t3 * t3);
-    double t4 = u0 - 2 * up1 + up2, t5 = 3 * u0 - 4 * up1 + up2;
-    double beta2 = 13.0 / 12.0 * (t4 * t4) + 1.0 / 4.0 * (t5 * t5);
-    double e0 = 1e-6 + beta0, e1 = 1e-6 + beta1, e2 = 1e-6 + beta2;
+    // Jiang-Shu weights w_k = (d_k/e_k) / sum_m (d_m/e_m), e_k = (eps + beta_k)^2.  FP64: written
+    // with the common denominator e0 e1 e2 so that each face value costs one division.  FP32:
+    // hardware reciprocals (the products e_i e_j could overflow the FP32 range).
+    T t0 = um2 - T(2) * um1 + u0, t1 = um2 - T(4) * um1 + T(3) * u0;
+    T beta0 = T(13.0 / 12.0) * (t0 * t0) + T(1.0 / 4.0) * (t1 * t1);
+    T t2 = um1 - T(2) * u0 + up1, t3 = um1 - up1;
+    T beta1 = T(13.0 / 12.0) * (t2 * t2) + T(1.0 / 4.0) * (t3 * t3);
+    T t4 = u0 - T(2) * up1 + up2, t5 = T(3) * u0 - T(4) * up1 + up2;
+    T beta2 = T(13.0 / 12.0) * (t4 * t4) + T(1.0 / 4.0) * (t5 * t5);
+    T e0 = T(1e-6) + beta0, e1 = T(1e-6) + beta1, e2 = T(1e-6) + beta2;
     e0 = e0 * e0; e1 = e1 * e1; e2 = e2 * e2;
-    const double P0 = e1 * e2, P1 = e0 * e2, P2 = e0 * e1;
-    double q0 = -1.0 / 6.0 * um2 + 5.0 / 6.0 * um1 + 1.0 / 3.0 * u0;
-    double q1 = 1.0 / 3.0 * um1 + 5.0 / 6.0 * u0 - 1.0 / 6.0 * up1;
-    double q2 = 11.0 / 6.0 * u0 - 7.0 / 6.0 * up1 + 1.0 / 3.0 * up2;
-    double a0 = 0.3 * P0, a1 = 0.6 * P1, a2 = 0.1 * P2;
-    wL = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
-    q0 = 1.0 / 3.0 * um2 - 7.0 / 6.0 * um1 + 11.0 / 6.0 * u0;
-    q1 = -1.0 / 6.0 * um1 + 5.0 / 6.0 * u0 + 1.0 / 3.0 * up1;
-    q2 = 1.0 / 3.0 * u0 + 5.0 / 6.0 * up1 - 1.0 / 6.0 * up2;
-    a0 = 0.1 * P0; a1 = 0.6 * P1; a2 = 0.3 * P2;
-    wR = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+    T P0, P1, P2;
+    if (std::is_same<T, float>::value) { P0 = rcp_t(e0); P1 = rcp_t(e1); P2 = rcp_t(e2); }
+    else { P0 = e1 * e2; P1 = e0 * e2; P2 = e0 * e1; }
+    T q0 = T(-1.0 / 6.0) * um2 + T(5.0 / 6.0) * um1 + T(1.0 / 3.0) * u0;
+    T q1 = T(1.0 / 3.0) * um1 + T(5.0 / 6.0) * u0 - T(1.0 / 6.0) * up1;
+    T q2 = T(11.0 / 6.0) * u0 - T(7.0 / 6.0) * up1 + T(1.0 / 3.0) * up2;
+    T a0 = T(0.3) * P0, a1 = T(0.6) * P1, a2 = T(0.1) * P2;
+    wL = std::is_same<T, float>::value ? (a0 * q0 + a1 * q1 + a2 * q2) * rcp_t(a0 + a1 + a2)
+                                       : (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+    q0 = T(1.0 / 3.0) * um2 - T(7.0 / 6.0) * um1 + T(11.0 / 6.0) * u0;
+    q1 = T(-1.0 / 6.0) * um1 + T(5.0 / 6.0) * u0 + T(1.0 / 3.0) * up1;
+    q2 = T(1.0 / 3.0) * u0 + T(5.0 / 6.0) * up1 - T(1.0 / 6.0) * up2;
+    a0 = T(0.1) * P0; a1 = T(0.6) * P1; a2 = T(0.3) * P2;
+    wR = std::is_same<T, float>::value ? (a0 * q0 + a1 * q1 + a2 * q2) * rcp_t(a0 + a1 + a2)
+                                       : (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
   }
 }
 
 // Rusanov-type flux with GLF dissipation (flux.cpp:202-243, 329-367; state.cpp:27-44, 86-106, 298-322).
 // kind 0: convective only; kind 1: convective + magnetic pressure-formula terms.
-__device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, const double* QR, double s_max, double* F) {
-  double fl[5], fr[5];
-  const double* Qs2[2] = {QL, QR};
-  double* fo[2] = {fl, fr};
+template <class T>
+__device__ __forceinline__ void gl_flux(int dir, int kind, const T* QL, const T* QR, T s_max, T* F) {
+  T fl[5], fr[5];
+  const T* Qs2[2] = {QL, QR};
+  T* fo[2] = {fl, fr};
   for (int side = 0; side < 2; side++) {
-    const double* Q = Qs2[side];
-    double* f = fo[side];
-    const double ir = 1.0 / Q[0];
-    double rho = Q[0], vx = Q[1] * ir, vy = Q[2] * ir, vz = Q[3] * ir;
-    double kin = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-    double vn = dir == 0 ? vx : vy;
+    const T* Q = Qs2[side];
+    T* f = fo[side];
+    const T ir = rcp_t(Q[0]);
+    T rho = Q[0], vx = Q[1] * ir, vy = Q[2] * ir, vz = Q[3] * ir;
+    T kin = T(0.5) * rho * (vx * vx + vy * vy + vz * vz);
+    T vn = dir == 0 ? vx : vy;
     f[0] = rho * vn;
     f[1] = dir == 0 ? rho * vx * vx : rho * vx * vy;
     f[2] = dir == 0 ? rho * vy * vx : rho * vy * vy;
     f[3] = rho * vz * vn;
     f[4] = kin * vn;
     if (kind == 1) {
-      double B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
-      double vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) * ir;
+      T B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
+      T vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) * ir;
       if (dir == 0) {
-        f[1] = f[1] + (B2 * 0.5 - Q[5] * Q[5]);
+        f[1] = f[1] + (B2 * T(0.5) - Q[5] * Q[5]);
         f[2] = f[2] + (-Q[5] * Q[6]);
         f[3] = f[3] + (-Q[5] * Q[7]);
         f[4] = f[4] + (vx * B2 - Q[5] * vB);
       } else {
         f[1] = f[1] + (-Q[5] * Q[6]);
-        f[2] = f[2] + (B2 * 0.5 - Q[6] * Q[6]);
+        f[2] = f[2] + (B2 * T(0.5) - Q[6] * Q[6]);
         f[3] = f[3] + (-Q[6] * Q[7]);
         f[4] = f[4] + (vy * B2 - Q[6] * vB);
       }
     }
   }
-  for (int k = 0; k < 5; k++) F[k] = 0.5 * (fl[k] + fr[k]) - 0.5 * s_max * (QR[k] - QL[k]);
+  for (int k = 0; k < 5; k++) F[k] = T(0.5) * (fl[k] + fr[k]) - T(0.5) * s_max * (QR[k] - QL[k]);
 }
 
 // RusanovC_Update (kind 0, flux.cpp:409-455) and Pressure_flux_Update (kind 1, flux.cpp:290-326)
@@ -323,10 +339,12 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const double* QL, con
 #define EX_NF 288            // max faces per direction: 33x8 (x) / 32x9 (y)
 
 // One direction of the fused update: reconstruct, form the face fluxes, apply their difference.
-template <int DIR>
+// T is the arithmetic type of the reconstruction and the fluxes (float in the mixed form); the
+// conservative update u = Qs - adt div F is always formed in FP64.
+template <int DIR, class T>
 __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int kind, const double* __restrict__ Qe,
-                                             const double* __restrict__ Qs, double* __restrict__ u, double h, double s_max,
-                                             int i0, int j0, double* wl, double* wr, double* F, double* acc) {
+                                             const double* __restrict__ Qs, double* __restrict__ u, double h, T s_max,
+                                             int i0, int j0, T* wl, T* wr, T* F, double* acc) {
   constexpr int W = DIR == 0 ? EX_TX + 2 : EX_TX;                  // reconstructed cells per row
   constexpr int NITEMS = DIR == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
   constexpr int NF = DIR == 0 ? (EX_TX + 1) * EX_TY : EX_TX * (EX_TY + 1);
@@ -348,10 +366,10 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     const long long step = DIR == 0 ? 1 : d.NX;
     for (int k = 0; k < nrec; k++) {
       const double* q = Qe + k * d.PS + base;
-      double a[5];
+      T a[5];
 #pragma unroll
-      for (int o = 0; o < 5; o++) a[o] = q[(o - 2) * step];
-      double L, R;
+      for (int o = 0; o < 5; o++) a[o] = (T)q[(o - 2) * step];
+      T L, R;
       recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R);
       wl[k * EX_NW + it] = L;
       wr[k * EX_NW + it] = R;
@@ -365,17 +383,17 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     if (DIR == 0) { itL = fy * W + fx; itR = fy * W + fx + 1; gli = i0 + fx - 1; gri = i0 + fx; glj = grj = j0 + fy; }
     else { itL = fy * W + fx; itR = (fy + 1) * W + fx; gli = gri = i0 + fx; glj = j0 + fy - 1; grj = j0 + fy; }
     if (gri > d.nx + GC || grj > d.ny + GC) continue;
-    double QL[NVAR], QR[NVAR];
+    T QL[NVAR], QR[NVAR];
     for (int k = 0; k < 5; k++) {
-      QL[k] = k < nrec ? wr[k * EX_NW + itL] : 0.0;
-      QR[k] = k < nrec ? wl[k * EX_NW + itR] : 0.0;
+      QL[k] = k < nrec ? wr[k * EX_NW + itL] : T(0);
+      QR[k] = k < nrec ? wl[k * EX_NW + itR] : T(0);
     }
     if (kind == 1) {
-      for (int k = 5; k < 8; k++) { QL[k] = u[k * d.PS + pidx(d, gli, glj)]; QR[k] = u[k * d.PS + pidx(d, gri, grj)]; }
+      for (int k = 5; k < 8; k++) { QL[k] = (T)u[k * d.PS + pidx(d, gli, glj)]; QR[k] = (T)u[k * d.PS + pidx(d, gri, grj)]; }
     } else {
-      for (int k = 5; k < 8; k++) { QL[k] = 0.0; QR[k] = 0.0; }
+      for (int k = 5; k < 8; k++) { QL[k] = T(0); QR[k] = T(0); }
     }
-    double Fo[5];
+    T Fo[5];
     gl_flux(DIR, kind, QL, QR, s_max, Fo);
     for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[k];
   }
@@ -384,24 +402,28 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     long long c = pidx(d, ci, cj);
     if (DIR == 0) {
       int fm = ty * FW + tx, fp = fm + 1;
-      for (int k = k0; k < k1; k++) acc[k] = Qs[k * d.PS + c] - h / d.dx * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
+      for (int k = k0; k < k1; k++) acc[k] = Qs[k * d.PS + c] - h / d.dx * ((double)F[k * EX_NF + fp] - (double)F[k * EX_NF + fm]);
     } else {
       int fm = ty * FW + tx, fp = fm + FW;
-      for (int k = k0; k < k1; k++) u[k * d.PS + c] = acc[k] - h / d.dy * (F[k * EX_NF + fp] - F[k * EX_NF + fm]);
+      for (int k = k0; k < k1; k++) u[k * d.PS + c] = acc[k] - h / d.dy * ((double)F[k * EX_NF + fp] - (double)F[k * EX_NF + fm]);
     }
   }
 }
 
+template <class T>
 __global__ void __launch_bounds__(256) k_explicit(Dims d, Phys P, int kind, const double* __restrict__ Qe,
                                                   const double* __restrict__ Qs, double* __restrict__ u, const double* S) {
-  __shared__ double wl[5 * EX_NW], wr[5 * EX_NW];
-  __shared__ double F[5 * EX_NF];
+  __shared__ T wl[5 * EX_NW], wr[5 * EX_NW];
+  __shared__ T F[5 * EX_NF];
   const int i0 = GC + blockIdx.x * EX_TX, j0 = GC + blockIdx.y * EX_TY;
-  const double h = S[S_ADT], s_max = S[S_AMAX];
+  const double h = S[S_ADT];
+  const T s_max = (T)S[S_AMAX];
   double acc[5];
-  explicit_dir<0>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
-  explicit_dir<1>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
+  explicit_dir<0, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
+  explicit_dir<1, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
 }
+template __global__ void k_explicit<double>(Dims, Phys, int, const double*, const double*, double*, const double*);
+template __global__ void k_explicit<float>(Dims, Phys, int, const double*, const double*, double*, const double*);
 
 // ================================================================ magnetic subsystem
 // RHS of MagSubsystem_UpdateB (imex_subsystems.cpp:549-613), initial guess x0 = B(Qe),

@@ -77,7 +77,9 @@ typedef struct imex_params {
   double time_stop;               /* dt = min(dt, time_stop - t) (main.cpp:341) */
   int ct_mode;                    /* IMEX_CT_* */
   int precision;                  /* IMEX_PREC_* */
-  double mixed_inner_tol;         /* relative tolerance of the FP32 inner solve (mixed form) */
+  double mixed_inner_tol;         /* mixed form: each FP32 inner solve reduces its FP64 residual by this factor */
+  double mixed_outer_tol;         /* mixed form: refinement stops when the FP64 residual is below this
+                                     factor of the (preconditioned) right-hand side norm, plus gmres_tol */
   int schedule;                   /* IMEX_SCHED_* */
   int budget_margin;              /* Arnoldi steps granted per GMRES solve beyond the largest count of the
                                      last 8 steps (budgeted schedules; default 1) */
