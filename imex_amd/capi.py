@@ -19,6 +19,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN = 0, 1, 2
 CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
 SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
+TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
 WARN_GMRES_MAXOUTER, WARN_NEWTON_MAXITER, WARN_POSITIVITY, WARN_EOS_MAXITER = 1, 2, 4, 8
 
 
@@ -63,6 +64,11 @@ class Stats(C.Structure):
 
 class KernelProfile(C.Structure):
     _fields_ = [("name", C.c_char_p), ("launches", C.c_longlong), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
+class TuneInfo(C.Structure):
+    _fields_ = [("explicit_tile_x", C.c_int), ("explicit_tile_y", C.c_int), ("krylov_blocks", C.c_int),
+                ("explicit_us", C.c_double * 3), ("krylov_us", C.c_double * 3)]
 
 
 class SolveInfo(C.Structure):
@@ -176,6 +182,12 @@ def lib():
         "imex_pressure_newton": (C.c_int, [vp, dp, dp, C.POINTER(SolveInfo)]),
         "imex_profile_enable": (C.c_int, [vp, C.c_int]),
         "imex_profile_read": (C.c_int, [vp, C.POINTER(KernelProfile), C.c_int]),
+        "imex_batch_create": (C.c_int, [C.POINTER(Params), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(vp)]),
+        "imex_batch_size": (C.c_int, [vp]),
+        "imex_batch_set_state": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_double]),
+        "imex_batch_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, C.POINTER(Scalars)]),
+        "imex_batch_div_b": (C.c_double, [vp, C.c_int]),
+        "imex_autotune": (C.c_int, [vp, C.c_int, C.POINTER(TuneInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -289,6 +301,13 @@ class Solver:
     def div_b(self):
         return self.L.imex_div_b(self.h)
 
+    def autotune(self, flags=TUNE_EXPLICIT | TUNE_KRYLOV):
+        """Pick launch shapes by timing on the current state; returns the choice and the timings (us)."""
+        ti = TuneInfo()
+        self._chk(self.L.imex_autotune(self.h, flags, C.byref(ti)), "autotune")
+        return dict(explicit_tile=(ti.explicit_tile_x, ti.explicit_tile_y), krylov_blocks=ti.krylov_blocks,
+                    explicit_us=list(ti.explicit_us), krylov_us=list(ti.krylov_us))
+
     def profile_enable(self, on=True):
         return self._chk(self.L.imex_profile_enable(self.h, 1 if on else 0), "profile_enable")
 
@@ -334,3 +353,43 @@ class Solver:
         info = SolveInfo()
         rc = self._chk(self.L.imex_pressure_newton(self.h, ptr(b), ptr(p), C.byref(info)), "pressure_newton")
         return p, struct_dict(info), rc
+
+
+class Batch(Solver):
+    """A batch context: independent instances sharing `params` except the grid size."""
+
+    def __init__(self, params, sizes):
+        self.p = params
+        self.L = lib()
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        n = len(self.sizes)
+        nx = (C.c_int * n)(*[a for a, _ in self.sizes])
+        ny = (C.c_int * n)(*[b for _, b in self.sizes])
+        h = C.c_void_p()
+        rc = self.L.imex_batch_create(C.byref(params), n, nx, ny, C.byref(h))
+        if rc < 0:
+            raise ImexError(f"imex_batch_create: {self.L.imex_last_error().decode()}")
+        self.h = h
+
+    def __len__(self):
+        return self.L.imex_batch_size(self.h)
+
+    def params_of(self, i):
+        q = Params.from_buffer_copy(self.p)
+        q.nx, q.ny = self.sizes[i]
+        return q
+
+    def set_instance(self, i, u_pri, bx, by, t=0.0, timestep=0, prev_dt=0.0):
+        return self._chk(self.L.imex_batch_set_state(self.h, i, ptr(np.ascontiguousarray(u_pri)), ptr(np.ascontiguousarray(bx)),
+                                                     ptr(np.ascontiguousarray(by)), t, timestep, prev_dt), "batch_set_state")
+
+    def get_instance(self, i):
+        p = self.params_of(i)
+        u_con, u_pri = np.zeros(grid_shape(p)), np.zeros(grid_shape(p))
+        bx, by = np.zeros(array_shape(p)), np.zeros(array_shape(p))
+        sc = Scalars()
+        self._chk(self.L.imex_batch_get_state(self.h, i, ptr(u_con), ptr(u_pri), ptr(bx), ptr(by), C.byref(sc)), "batch_get_state")
+        return u_con, u_pri, bx, by, struct_dict(sc)
+
+    def div_b_of(self, i):
+        return self.L.imex_batch_div_b(self.h, i)

@@ -1,0 +1,87 @@
+// batch.hpp — instance table and block map shared by every kernel.
+//
+// A context holds one or more independent instances (grids of their own size, with their own
+// scalars, Krylov control block and buffers).  Kernels receive the device table of instances
+// and small buffer ids instead of raw pointers; one launch covers every instance: workgroup b
+// belongs to the instance i with pre[i] <= b < pre[i+1] and is local block b - pre[i] of that
+// instance's grid (gx[i] blocks per row).  A single-instance context is a batch of one.
+#pragma once
+
+#define MAXM 32            // largest GMRES restart length supported
+#define MAXSWEEP 16        // refinement sweeps recorded per mixed-form solve
+
+// FP64 buffers of an instance.  Grids (B_UPRI..B_QS) are 8 planes; the others one plane or one
+// compact Krylov vector (3 components for the magnetic system).
+enum BufId {
+  B_UPRI, B_UN, B_U1, B_U2, B_U3, B_QE, B_QS,
+  B_BX0, B_BY0, B_BX1, B_BY1, B_BX2, B_BY2, B_BXQS, B_BYQS, B_FX, B_FY,
+  B_PE, B_CF, B_CHI, B_ETA, B_HHAT, B_MXS, B_MYS, B_EZPP, B_EZX, B_EZY, B_VISC,
+  B_W, B_AV, B_XK, B_BK, B_TMPN, B_P, B_PB, B_NG, B_DP, B_TDIAG, B_VSLOPE,
+  B_V0, B_VEND = B_V0 + MAXM + 1,
+  NBUF = B_VEND
+};
+// FP32 buffers (inner solves of the mixed form)
+enum FBufId { F_W32, F_AV32, F_V0, F_VEND = F_V0 + MAXM + 1, NFBUF = F_VEND };
+
+struct KrylovDev;
+
+struct Inst {
+  Dims d;
+  double* buf[NBUF];
+  float* fbuf[NFBUF];
+  double* S;             // step scalars (ScalarSlot)
+  double* partial;       // per-block partial sums
+  double* pst;           // positivity statistics (min p, max floored, violation)
+  int* eflag;            // EOS iteration cap reached
+  KrylovDev* K;          // GMRES / Newton control block
+  int* ctl;              // budgeted-schedule failure flags and iteration log
+  double* bsave;         // scalar snapshot of the budgeted step
+  int* esave;
+};
+
+struct BMap {
+  const int* pre;        // ni + 1 prefix sums of the per-instance block counts
+  const int* gx;         // blocks per grid row of each instance
+  int ni;
+};
+
+__device__ __forceinline__ void bmap_locate(const BMap& m, int b, int& inst, int& lb) {
+  int lo = 0, hi = m.ni;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (m.pre[mid] <= b) lo = mid; else hi = mid;
+  }
+  inst = lo;
+  lb = b - m.pre[lo];
+}
+
+// Opens a kernel body: inst_ is the instance, bx_/by_ the local block indices, gx_/gy_ the local
+// grid (for 1D grids gy_ = 1 and gx_ is the local block count).
+#define BATCH_BLOCK(bm)                                                     \
+  int inst_, lb_;                                                           \
+  bmap_locate(bm, blockIdx.x, inst_, lb_);                                  \
+  const int gx_ = (bm).gx[inst_];                                           \
+  const int gy_ = ((bm).pre[inst_ + 1] - (bm).pre[inst_]) / gx_;            \
+  const int bx_ = lb_ % gx_, by_ = lb_ / gx_;                               \
+  (void)gy_; (void)bx_; (void)by_
+
+// launch shapes (per-instance grids, computed once per context)
+enum Shape {
+  SH_INT64x4,     // (nx, ny) in 64x4 tiles
+  SH_EXPL,        // (nx, ny) in explicit tiles (16x16, 32x8 or 64x4, autotuned)
+  SH_PAD64x4,     // (nx+6, ny+6) in 64x4 tiles
+  SH_INT2_64x4,   // (nx+2, ny+2) in 64x4 tiles
+  SH_LIN_TOT,     // (nx+6)(ny+6) cells, 256 per block
+  SH_LIN_N,       // nx ny cells, 256 per block
+  SH_START,       // start-of-step reduction grid
+  SH_TILES,       // operator tiles (64x4), capped
+  SH_TILES_ALL,   // operator tiles, one block each up to 4096
+  SH_VEC3,        // 3 nx ny vector elements, capped
+  SH_VEC1,        // nx ny vector elements, capped
+  SH_MBIG,        // Mbig reduction grid
+  SH_PSTATS,      // positivity statistics grid
+  SH_DIVB,        // div B reduction grid
+  SH_ONE,         // one block per instance
+  SH_FILL0, SH_FILL1, SH_FILL2,   // ghost-frame fills (cell / x-normal faces / y-normal faces)
+  NSHAPE
+};

@@ -5,6 +5,10 @@
 // device for the lifetime of the context, in SoA planes with the 3-cell ghost frame.
 // The step loop body of src/main.cpp:334-595 is reproduced kernel by kernel; host code
 // only sequences launches and reads the GMRES / Newton convergence flags.
+//
+// A context holds one instance (imex_create) or a batch of independent instances of
+// uneven grid size (imex_batch_create).  Every launch covers all instances: kernels find
+// their instance and local block through a per-shape block map (batch.hpp).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,77 +23,39 @@ static int set_err(int code, const std::string& m) { g_err = m; return code; }
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return set_err(IMEX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } while (0)
 
-// ---------------------------------------------------------------- small kernels local to the driver
-__global__ void k_fill(long long n, double* p, double v) {
-  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e < n) p[e] = v;
-}
-__global__ void k_visc_rk(Dims d, Phys P, const double* pe, const double* u, double* visc, int* eflag) {
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= (long long)d.NX * d.NY) return;
-  int ef = 0;
-  visc[c] = Vbase_slope(P, pe[c], u[c], &ef);
-  if (ef) atomicOr(eflag, 1);
-}
-// Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
-// device: acc[0] = min p ever, acc[1] = max floored count, acc[2] = 1 if min p < theta p_floor - 1e-14.
-__global__ void k_pstats(long long n, const double* p, double pfloor, double* partial) {
-  __shared__ double smin[4], scnt[4];
-  double mn = 1e300, cnt = 0;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
-    double v = p[e];
-    mn = fmin(mn, v);
-    if (v < pfloor) cnt += 1;
-  }
-  mn = wave_min(mn); cnt = wave_sum(cnt);
-  if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = mn; scnt[threadIdx.x >> 6] = cnt; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mn = fmin(mn, smin[w]); cnt += scnt[w]; }
-    partial[2 * blockIdx.x] = smin[0] < mn ? smin[0] : mn;
-    partial[2 * blockIdx.x + 1] = cnt;
-  }
-}
-__global__ void k_pstats_fin(const double* partial, int nblk, double thr, double* acc) {
-  // min, and a sum of small integers held in doubles: both exact in any order, one wave reduces
-  double m = 1e300, c = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, partial[2 * b]); c += partial[2 * b + 1]; }
-  m = wave_min(m); c = wave_sum(c);
-  if (threadIdx.x == 0) {
-    acc[0] = fmin(acc[0], m);
-    acc[1] = fmax(acc[1], c);
-    if (m < thr) acc[2] = 1.0;
-  }
-}
-
-__global__ void k_fill_skip(long long n, double* p, double v, const int* skip) {
-  if (skip && *skip) return;
-  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e < n) p[e] = v;
-}
+#define CTLN 2048          // ints of budget log per instance (bounds the solves of one step)
 
 // ---------------------------------------------------------------- budgeted-schedule bookkeeping
 // A budgeted step runs every GMRES solve for a fixed number of Arnoldi launches and every
 // nested-Newton solve for a fixed number of Jacobian solves; kernels queued past convergence
-// exit at once.  ctl[0] collects failures (a solve that had not converged when its budget ran
-// out), ctl[1..] the iteration count of each solve, in launch order.  The scalars, positivity
-// statistics and EOS flag are saved at the start so that a failed step can be rerun exactly.
-__global__ void k_budget_begin(const double* S, const double* pst, const int* eflag, double* save, int* esave, int* ctl) {
+// exit at once.  ctl[0] of each instance collects failures (a solve that had not converged when
+// its budget ran out), ctl[1..] the iteration count of each solve, in launch order.  The
+// scalars, positivity statistics and EOS flag are saved at the start so that a failed step can
+// be rerun exactly.
+__global__ void k_budget_begin(const Inst* __restrict__ IT, BMap bm) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
   const int t = threadIdx.x;
-  if (t < S_NSLOTS) save[t] = S[t];
-  if (t < 4) save[S_NSLOTS + t] = pst[t];
-  if (t == 0) { esave[0] = eflag[0]; ctl[0] = 0; }
+  if (t < S_NSLOTS) I.bsave[t] = I.S[t];
+  if (t < 4) I.bsave[S_NSLOTS + t] = I.pst[t];
+  if (t == 0) { I.esave[0] = I.eflag[0]; I.ctl[0] = 0; }
 }
-__global__ void k_budget_restore(double* S, double* pst, int* eflag, const double* save, const int* esave) {
+__global__ void k_budget_restore(const Inst* __restrict__ IT, BMap bm) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
   const int t = threadIdx.x;
-  if (t < S_NSLOTS) S[t] = save[t];
-  if (t < 4) pst[t] = save[S_NSLOTS + t];
-  if (t == 0) eflag[0] = esave[0];
+  if (t < S_NSLOTS) I.S[t] = I.bsave[t];
+  if (t < 4) I.pst[t] = I.bsave[S_NSLOTS + t];
+  if (t == 0) I.eflag[0] = I.esave[0];
 }
 // GMRES log: ctl[slot] = Arnoldi steps over all sweeps, ctl[slot+1] = sweeps, then the steps of
 // each of the nsw budgeted sweeps.  A FP64-form solve has one sweep and converges inside it; a
 // mixed-form solve converges at the FP64 residual that opens sweep `sweeps`.
-__global__ void k_budget_gmres_check(const KrylovDev* K, int* ctl, int slot, int mixed, int nsw) {
+__global__ void k_budget_gmres_check(const Inst* __restrict__ IT, BMap bm, int slot, int mixed, int nsw) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  const KrylovDev* K = I.K;
+  int* ctl = I.ctl;
   const int t = threadIdx.x;
   if (t == 0) {
     if (mixed) {
@@ -104,18 +70,14 @@ __global__ void k_budget_gmres_check(const KrylovDev* K, int* ctl, int slot, int
   }
   if (t < nsw) ctl[slot + 2 + t] = mixed ? (t < K->outer ? K->sweep_steps[t] : 0) : K->steps;
 }
-__global__ void k_budget_newton_check(const KrylovDev* K, int* ctl, int slot) {
+__global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int slot) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
   if (threadIdx.x == 0) {
-    ctl[slot] = K->newton_conv ? K->newton_first : -1;
-    if (!K->newton_conv) ctl[0] |= 2;
+    I.ctl[slot] = I.K->newton_conv ? I.K->newton_first : -1;
+    if (!I.K->newton_conv) I.ctl[0] |= 2;
   }
 }
-
-// ---------------------------------------------------------------- context
-struct StageBind {
-  double *u, *Qe, *Qs, *bxqs, *byqs, *bxo, *byo;
-};
-
 
 // ---------------------------------------------------------------- per-kernel-class event profiler
 // When enabled, every launch on the step path is bracketed by two HIP events on the
@@ -183,32 +145,35 @@ struct GraphEntry {
 #define MAX_GRAPHS 6
 #define NBK 5
 
+// buffer ids of the stage being queued
+struct StageBind { int u, Qe, Qs, bxqs, byqs, bxo, byo; };
 
 struct imex_ctx {
   Prof prof;
-  imex_params prm;
-  Dims d;
+  imex_params prm;             // shared parameters (nx, ny of instance 0)
   Phys P;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
   long long bytes = 0;
-  // grids (8 planes)
-  double *u_pri = nullptr, *u_n = nullptr, *U1 = nullptr, *U2 = nullptr, *Qe = nullptr, *Qs = nullptr;
-  // planes
-  double *bx_n, *by_n, *bx1, *by1, *bx2, *by2, *bxqs, *byqs, *fx, *fy;
-  double *pe, *cf, *chi, *eta, *hhat, *mxs, *mys, *ezpp, *ezx, *ezy, *visc;
-  // krylov
-  double* V[MAXM + 1];
-  double *w, *Av, *xk, *bk, *partial;
-  double *p, *pb, *ng, *dp, *Tdiag, *vslope, *tmpN, *pst;
-  KrylovDev* K = nullptr;
-  KrylovDev* Kh = nullptr;     // pinned host mirrors (2 slots)
-  double* S = nullptr;
-  double* Sh = nullptr;        // pinned host copy of S
-  int* eflag = nullptr;
-  unsigned* cnt_start = nullptr;
-  unsigned* cnt_pstats = nullptr;
+  // instances
+  int ni = 1;
+  std::vector<imex_params> iprm;     // per-instance parameters (grid size)
+  std::vector<Inst> hI;              // host copy of the instance table
+  Inst* IT = nullptr;                // device instance table
+  int* bmap_dev = nullptr;           // block maps of all shapes
+  int* bmap1_dev = nullptr;          // block maps of a one-instance view (set-up work)
+  Launch L;
+  long long cells = 0;               // sum of nx ny over instances
+  int cap_tiles = 2048, cap_vec = 2048;   // grid caps of the Krylov kernels (autotuned)
+  int ex_tile = 32;                       // explicit-kernel tile width (16, 32 or 64; autotuned)
+  // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo;
+  // faces likewise (bxn/byn read, bx2/by2 written)
+  int iUs = B_U2, iUo = B_U3, iBxn = B_BX0, iByn = B_BY0, iBx2 = B_BX2, iBy2 = B_BY2;
+  KrylovDev* Kh = nullptr;     // pinned host mirrors of instance 0's control block (2 slots)
   KrylovDev Klast;             // status of the last GMRES cycle read by the host
+  double* Sh = nullptr;        // pinned host copy of one instance's scalars
+  int* ctlh = nullptr;         // pinned host copy of every instance's budget log
+  int* ctl_all = nullptr;      // device: ni * CTLN
   hipEvent_t ev[2];
   int state_mode = 0;          // 0: u_pri holds the state; 1: Us holds the conservative state
   StageBind cur;
@@ -216,16 +181,8 @@ struct imex_ctx {
   imex_stats stats{};
   int pred[2] = {6, 6};         // Arnoldi steps of the last magnetic / pressure solve
   int pred_newton = 1;         // Jacobian solves of the last nested-Newton solve
-  int nblk_start = 1024;
-  KrylovBufs kb;
-  // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo
-  double *U3 = nullptr, *Us = nullptr, *Uo = nullptr;
   // budgeted schedule
   bool budget = false;         // the step being queued runs on budgets
-  double* bsave = nullptr;     // S, positivity statistics at the start of the budgeted step
-  int* esave = nullptr;
-  int* ctl = nullptr;          // device: failure flags + per-solve iteration log
-  int* ctlh = nullptr;         // pinned host copy
   int nslot = 1;
   std::vector<BudgetRec> brecs;
   // budget history, per kind: 0/1 Arnoldi steps per sweep of a magnetic / pressure
@@ -240,13 +197,13 @@ struct imex_ctx {
   int stage_idx = 0, sub_idx = 0, newton_it = 0;
   // mixed form
   int mixed = 0;
-  float* V32[MAXM + 1];
   std::vector<GraphEntry> graphs;
   long long graph_clock = 0;
   bool capturing = false;      // launches are being captured into a step graph
   bool tag_err = false;
   bool no_toggle = false;      // node enabling failed once: graphs are captured per exact budget table
   std::vector<NodeTag> tags;
+  const Dims& d0() const { return hI[0].d; }
 };
 
 static Dims make_dims(const imex_params& p) {
@@ -269,7 +226,6 @@ static Phys make_phys(const imex_params& p) {
   P.recon = p.recon; P.central_y = p.minmod_central_y;
   return P;
 }
-
 
 static void prof_collect(imex_ctx* c) {
   Prof& P = c->prof;
@@ -298,7 +254,7 @@ static int prof_end(imex_ctx* c, int e0, int cls, double fields) {
   if (e0 < 0) return -1;
   Prof& P = c->prof;
   (void)hipEventRecord(P.ev[e0 + 1], c->st);
-  P.recs.push_back(ProfRec{cls, e0, e0 + 1, fields * (double)c->d.N * 8.0});
+  P.recs.push_back(ProfRec{cls, e0, e0 + 1, fields * (double)c->cells * 8.0});
   return (int)P.recs.size() - 1;
 }
 #define LAUNCH(cls, fields, stmt) do { int e0_ = prof_begin(c); stmt; prof_end(c, e0_, (cls), (double)(fields)); } while (0)
@@ -307,6 +263,7 @@ template <class T>
 static int dalloc(imex_ctx* c, T** out, size_t count) {
   void* ptr = nullptr;
   size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
   if (hipMalloc(&ptr, bytes) != hipSuccess) return set_err(IMEX_ERR_NOMEM, "hipMalloc failed");
   c->allocs.push_back(ptr);
   c->bytes += (long long)bytes;
@@ -314,32 +271,85 @@ static int dalloc(imex_ctx* c, T** out, size_t count) {
   return 0;
 }
 
-static inline unsigned nb(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
-static inline dim3 grid2(int nx, int ny, int bx = 64, int by = 4) { return dim3((nx + bx - 1) / bx, (ny + by - 1) / by); }
+static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
-static void fill_planes(imex_ctx* c, std::initializer_list<double*> planes, int kind = 0) {
-  PlaneSet ps; ps.n = 0;
-  for (double* p : planes) ps.p[ps.n++] = p;
+// Per-instance grid of a launch shape: (blocks per row, blocks).
+static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& nb) {
+  long long tiles = cdiv(d.nx, TX) * cdiv(d.ny, TY);
+  long long tot = (long long)d.NX * d.NY;
+  gx = 1;
+  switch (sh) {
+    case SH_INT64x4: gx = (int)cdiv(d.nx, 64); nb = gx * (int)cdiv(d.ny, 4); break;
+    case SH_EXPL: {
+      const int tx = c->ex_tile, ty = 256 / tx;
+      gx = (int)cdiv(d.nx, tx); nb = gx * (int)cdiv(d.ny, ty); break;
+    }
+    case SH_PAD64x4: gx = (int)cdiv(d.NX, 64); nb = gx * (int)cdiv(d.NY, 4); break;
+    case SH_INT2_64x4: gx = (int)cdiv(d.nx + 2, 64); nb = gx * (int)cdiv(d.ny + 2, 4); break;
+    case SH_LIN_TOT: nb = (int)cdiv(tot, 256); gx = nb; break;
+    case SH_LIN_N: nb = (int)cdiv(d.N, 256); gx = nb; break;
+    case SH_START: nb = (int)std::min<long long>(cdiv(tot, 256), 1024); gx = nb; break;
+    case SH_TILES: nb = (int)std::min<long long>(tiles, c->cap_tiles); gx = nb; break;
+    case SH_TILES_ALL: nb = (int)std::min<long long>(tiles, 4096); gx = nb; break;
+    case SH_VEC3: nb = (int)std::min<long long>(cdiv(3LL * d.N, 256), c->cap_vec); gx = nb; break;
+    case SH_VEC1: nb = (int)std::min<long long>(cdiv(d.N, 256), c->cap_vec); gx = nb; break;
+    case SH_MBIG: nb = (int)std::min<long long>(cdiv(d.N, 256), 512); gx = nb; break;
+    case SH_PSTATS: nb = (int)std::min<long long>(cdiv(d.N, 256), 256); gx = nb; break;
+    case SH_DIVB: nb = (int)std::min<long long>(cdiv(d.N, 256), 256); gx = nb; break;
+    case SH_ONE: nb = 1; gx = 1; break;
+    default: {   // ghost fills
+      const int kind = sh - SH_FILL0;
+      int lox = (!d.perx && kind == 1) ? GC - 1 : GC, hix = d.nx + GC - 1;
+      int loy = (!d.pery && kind == 2) ? GC - 1 : GC, hiy = d.ny + GC - 1;
+      long long total = (long long)loy * d.NX + (long long)(d.NY - 1 - hiy) * d.NX + (long long)(hiy - loy + 1) * (lox + d.NX - 1 - hix);
+      nb = (int)std::max<long long>(1, cdiv(total, 256)); gx = nb; break;
+    }
+  }
+  if (nb < 1) { nb = 1; gx = 1; }
+}
+
+// Block maps of every shape for the current instance sizes and caps (device table of prefix sums
+// and row widths).
+static int build_maps(imex_ctx* c) {
+  const int ni = c->ni;
+  HIPCHK(hipStreamSynchronize(c->st));      // no launch may be reading the maps being replaced
+  std::vector<int> h((size_t)NSHAPE * (2 * ni + 1));
+  for (int sh = 0; sh < NSHAPE; sh++) {
+    int* pre = &h[(size_t)sh * (2 * ni + 1)];
+    int* gx = pre + ni + 1;
+    pre[0] = 0;
+    for (int i = 0; i < ni; i++) {
+      int g, nb;
+      shape_grid(c, sh, c->hI[i].d, g, nb);
+      pre[i + 1] = pre[i] + nb;
+      gx[i] = g;
+    }
+    c->L.nblocks[sh] = pre[ni];
+    c->L.bm[sh].pre = c->bmap_dev + (size_t)sh * (2 * ni + 1);
+    c->L.bm[sh].gx = c->L.bm[sh].pre + ni + 1;
+    c->L.bm[sh].ni = ni;
+  }
+  HIPCHK(hipMemcpy(c->bmap_dev, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+
+static inline MagIds mag_ids(const imex_ctx* c) { MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; return m; }
+static inline PIds p_ids(const imex_ctx* c, int jacobian, int precond) { PIds q; q.u = c->cur.u; q.jacobian = jacobian; q.precond = precond; return q; }
+
+static void fill_planes(imex_ctx* c, std::initializer_list<std::pair<int, int>> planes, int kind = 0) {
+  int b[8], k[8], n = 0;
+  for (auto& pk : planes) { b[n] = pk.first; k[n] = pk.second; n++; }
   int e0 = prof_begin(c);
-  launch_bc_fill(c->d, ps, kind, c->st);
+  launch_bc_fill(c->L, n, b, k, kind);
   prof_end(c, e0, KC_BC, 0.0);
-}
-
-static MagOp mag_op(imex_ctx* c) {
-  MagOp m; m.d = c->d; m.Qe = c->cur.Qe; m.u = c->cur.u; m.S = c->S; return m;
-}
-static POp p_op(imex_ctx* c, int jacobian, int precond) {
-  POp o; o.d = c->d; o.P = c->P; o.hhat = c->hhat; o.eta = c->eta; o.u = c->cur.u;
-  o.visc = c->P.eos == IMEX_EOS_IDEAL ? nullptr : c->visc;
-  o.pfreeze = c->p; o.vslope = c->P.eos == IMEX_EOS_IDEAL ? nullptr : c->vslope;
-  o.Tdiag = c->Tdiag; o.S = c->S; o.jacobian = jacobian; o.precond = precond;
-  return o;
 }
 
 static inline int gkey(const imex_ctx* c) {
   return (c->stage_idx * 3 + c->sub_idx) * NIT + std::min(std::max(c->newton_it, 0), NIT - 1);
 }
 static inline int nkey(const imex_ctx* c) { return c->stage_idx * 3 + c->sub_idx; }
+
 // Records the node that the launch just captured (single-stream capture: it is the only
 // dependency of the next launch).
 static void tag_last(imex_ctx* c, int kind, int key, int sw, int j) {
@@ -370,11 +380,12 @@ static void cur_gmres(imex_ctx* c, int key, int steps, int sweeps) {
 // The kernels are the ones the host-polled loop below uses, so a solve that converges within
 // its budget gives the same result bit for bit.  Launches past convergence exit at once; a solve
 // still unconverged at the end of its budget flags the step for a rerun.
-static int run_gmres_budget(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b,
-                            int x_is_zero, const int* skip, int parent, int it) {
+static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int ix, int ib,
+                            int x_is_zero, int skip, int parent, int it) {
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
   const int prec = c->mixed;
+  const Launch& L = c->L;
   BudgetRec r;
   r.kind = 0; r.op = op; r.parent = parent; r.it = it; r.key = gkey(c);
   r.B = std::min(m, std::max(1, c->tab.B[r.key]));
@@ -386,40 +397,40 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagOp& mo, const POp& po,
     const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
     const int k = r.key;
     int e0 = prof_begin(c);
-    gmres_residual(op, prec, mo, po, c->kb, x, b, c->K, sw, xz, skip, c->st);
+    gmres_residual(L, op, prec, mo, po, c->P, ix, ib, sw, xz, skip);
     r.rres.push_back(prof_end(c, e0, op == 0 ? KC_MAG_RES : KC_P_RES, res_fields));
     tag_last(c, TG_RES, k, sw, 0);
     e0 = prof_begin(c);
-    gmres_residual_fin(c->d, prec, c->kb, c->K, sw, skip, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol, c->st);
+    gmres_residual_fin(L, op, prec, sw, skip, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
     r.rresf.push_back(prof_end(c, e0, KC_KFIN, 0));
     tag_last(c, TG_RES, k, sw, 0);
     if (sw == r.nsw) break;
     const double vb = prec ? 0.5 : 1.0;       // bytes of a basis field relative to an FP64 field
     for (int j = 0; j < r.B; j++) {
       e0 = prof_begin(c);
-      arnoldi_apply(op, prec, mo, po, c->kb, c->V, c->V32, j, c->K, c->st);
+      arnoldi_apply(L, op, prec, mo, po, c->P, j);
       r.ra.push_back(prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY,
                               op == 0 ? 4 + vb * (9 + 3 * j) : 5 + po.jacobian + vb * (1 + j)));
       tag_last(c, TG_ARN, k, sw, j);
       e0 = prof_begin(c);
-      arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
+      arnoldi_apply_fin(L, op, j);
       r.rfa.push_back(prof_end(c, e0, KC_KFIN, 0));
       tag_last(c, TG_ARN, k, sw, j);
       e0 = prof_begin(c);
-      arnoldi_update(nc, prec, c->d, c->kb, c->V, c->V32, j, m, c->K, c->st);
+      arnoldi_update(L, op, prec, j, m);
       r.ru.push_back(prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, vb * (op == 0 ? 9 + 3 * j : 3 + j)));
       tag_last(c, TG_ARN, k, sw, j);
       e0 = prof_begin(c);
-      arnoldi_update_fin(nc, prec, c->d, c->kb, j, m, c->K, c->st);
+      arnoldi_update_fin(L, op, prec, j, m);
       r.rfu.push_back(prof_end(c, e0, KC_KFIN, 0));
       tag_last(c, TG_ARN, k, sw, j);
     }
     e0 = prof_begin(c);
-    gmres_xupdate(nc, prec, c->d, c->kb, c->V, c->V32, x, c->K, c->st);
+    gmres_xupdate(L, op, prec, ix);
     r.rx.push_back(prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (2 + vb * r.B)));
     tag_last(c, TG_XUPD, k, sw, 0);
   }
-  k_budget_gmres_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, r.slot, prec, r.nsw);
+  k_budget_gmres_check<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], r.slot, prec, r.nsw);
   tag_last(c, TG_GCHK, r.key, 0, 0);
   c->brecs.push_back(r);
   return 0;
@@ -428,13 +439,16 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagOp& mo, const POp& po,
 // ---------------------------------------------------------------- GMRES host loop
 // GMRES_Solver (implicit_operators.cpp:793-919).  Arnoldi steps are queued in chunks; the
 // convergence flag of chunk k is read (async copy + event) while chunk k+1 is already queued.
-static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double* x, const double* b, int x_is_zero,
-                     imex_solve_info* info, long long* it_acc, long long* rs_acc, const int* skip = nullptr,
+// Host polling reads the control block of the context's only instance.
+static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int ix, int ib, int x_is_zero,
+                     imex_solve_info* info, long long* it_acc, long long* rs_acc, int skip = SKIP_NONE,
                      int first_chunk = 0) {
-  if (c->budget) return run_gmres_budget(c, op, mo, po, x, b, x_is_zero, skip, -1, -1);
+  if (c->budget) return run_gmres_budget(c, op, mo, po, ix, ib, x_is_zero, skip, -1, -1);
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
-  const long long dim = (long long)nc * c->d.N;
+  const long long dim = (long long)nc * c->d0().N;
+  const Launch& L = c->L;
+  KrylovDev* Kd = c->hI[0].K;
   // The reference allows dim restart cycles (implicit_operators.cpp:802); the host loop is
   // additionally capped at 4000 cycles (100k Arnoldi steps at m = 25) and raises
   // IMEX_WARN_GMRES_MAXOUTER when it gives up.
@@ -450,9 +464,9 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
   for (outer = 0; outer < outmax; outer++) {
     const int xz = (x_is_zero && outer == 0) ? 1 : 0;
     const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
-    const int* sk = (outer == 0 || prec) ? skip : nullptr;
-    LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(op, prec, mo, po, c->kb, x, b, c->K, (int)outer, xz, sk, c->st));
-    LAUNCH(KC_KFIN, 0, gmres_residual_fin(c->d, prec, c->kb, c->K, (int)outer, sk, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol, c->st));
+    const int sk = (outer == 0 || prec) ? skip : SKIP_NONE;
+    LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(L, op, prec, mo, po, c->P, ix, ib, (int)outer, xz, sk));
+    LAUNCH(KC_KFIN, 0, gmres_residual_fin(L, op, prec, (int)outer, sk, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol));
     const long long gen0 = c->prof.gen;
     int recs_a[MAXM], recs_u[MAXM], recs_fa[MAXM], recs_fu[MAXM];
     int j = 0, slot = 0, last = -1;
@@ -462,20 +476,20 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
       int jend = std::min(m, j + (j == 0 ? std::max(first_chunk, 1) : chunk));
       for (; j < jend; j++) {
         int e0 = prof_begin(c);
-        arnoldi_apply(op, prec, mo, po, c->kb, c->V, c->V32, j, c->K, c->st);
+        arnoldi_apply(L, op, prec, mo, po, c->P, j);
         recs_a[j] = prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY,
                              op == 0 ? 4 + vb * (9 + 3 * j) : 5 + po.jacobian + vb * (1 + j));
         e0 = prof_begin(c);
-        arnoldi_apply_fin(c->d, c->kb, j, c->K, c->st);
+        arnoldi_apply_fin(L, op, j);
         recs_fa[j] = prof_end(c, e0, KC_KFIN, 0);
         e0 = prof_begin(c);
-        arnoldi_update(nc, prec, c->d, c->kb, c->V, c->V32, j, m, c->K, c->st);
+        arnoldi_update(L, op, prec, j, m);
         recs_u[j] = prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, vb * (op == 0 ? 9 + 3 * j : 3 + j));
         e0 = prof_begin(c);
-        arnoldi_update_fin(nc, prec, c->d, c->kb, j, m, c->K, c->st);
+        arnoldi_update_fin(L, op, prec, j, m);
         recs_fu[j] = prof_end(c, e0, KC_KFIN, 0);
       }
-      HIPCHK(hipMemcpyAsync(&c->Kh[slot].tol, &c->K->tol, sizeof(KrylovDev) - offsetof(KrylovDev, tol), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(&c->Kh[slot].tol, &Kd->tol, sizeof(KrylovDev) - offsetof(KrylovDev, tol), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipEventRecord(c->ev[slot], c->st));
       int prev = last;
       last = slot;
@@ -491,7 +505,7 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
     int rx;
     {
       int e0 = prof_begin(c);
-      gmres_xupdate(nc, prec, c->d, c->kb, c->V, c->V32, x, c->K, c->st);
+      gmres_xupdate(L, op, prec, ix);
       rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (2 + vb * (ks.jlast + 1)));
     }
     if (c->prof.on && c->prof.gen == gen0) {      // launches queued past convergence exited at once
@@ -529,11 +543,11 @@ static int run_gmres(imex_ctx* c, int op, const MagOp& mo, const POp& po, double
 // residual, solve, update per iteration, then the residual that must have converged.  Once a
 // residual has converged, the later residuals, fills, solves and updates of this Newton solve
 // skip on the device.
-static int run_newton_budget(imex_ctx* c, const double* b, double* p) {
-  const Dims& d = c->d;
+static int run_newton_budget(imex_ctx* c, int ib, int ip) {
+  const Launch& L = c->L;
   if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL)
-    return run_gmres_budget(c, 1, mag_op(c), p_op(c, 0, 1), p, b, 0, nullptr, -1, -1);
-  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(d, c->P, b, c->cur.u, c->S, c->partial, c->K, c->eflag, c->st));
+    return run_gmres_budget(c, 1, mag_ids(c), p_ids(c, 0, 1), ip, ib, 0, SKIP_NONE, -1, -1);
+  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(L, c->P, ib, c->cur.u));
   const int NB = std::min(c->prm.newton_max, std::max(1, c->tab.N[nkey(c)]));
   const int self = (int)c->brecs.size();
   {
@@ -541,86 +555,75 @@ static int run_newton_budget(imex_ctx* c, const double* b, double* p) {
     nr.kind = 1; nr.op = 1; nr.slot = c->nslot++; nr.key = nkey(c);
     c->brecs.push_back(nr);
   }
-  const int* conv = &c->K->newton_conv;
   for (int it = 0; it <= NB; it++) {
     c->newton_it = it;
-    POp carrier = p_op(c, 0, 0);
     const int nk = c->brecs[self].key;
     int e0 = prof_begin(c);
-    newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st);
+    newton_residual(L, p_ids(c, 0, 0), c->P, ip, ib, it, c->prm.newton_tol);
     int rr = prof_end(c, e0, KC_NEWTON_RES, 6);
     c->brecs[self].rnres.push_back(rr);
     tag_last(c, TG_NRES, nk, 0, it);
     e0 = prof_begin(c);
-    newton_residual_fin(d, c->kb, c->K, it, c->prm.newton_tol, c->st);
+    newton_residual_fin(L, it, c->prm.newton_tol);
     rr = prof_end(c, e0, KC_KFIN, 0);
     c->brecs[self].rnresf.push_back(rr);
     tag_last(c, TG_NRES, nk, 0, it);
     if (it == NB) break;
     if (c->P.eos != IMEX_EOS_IDEAL) {
-      vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
+      vslope_launch(L, c->P, ip, c->cur.u);
       tag_last(c, TG_NIT, nk, 0, it);
     }
     e0 = prof_begin(c);
-    k_fill_skip<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0, conv);
+    fill_launch(L, B_DP, 0.0, SKIP_NEWTON);
     rr = prof_end(c, e0, KC_MISC, 1);
     c->brecs[self].rfill.push_back(rr);
     tag_last(c, TG_NIT, nk, 0, it);
-    run_gmres_budget(c, 1, mag_op(c), p_op(c, 1, 1), c->dp, c->ng, 1, conv, self, it);
+    run_gmres_budget(c, 1, mag_ids(c), p_ids(c, 1, 1), B_DP, B_NG, 1, SKIP_NEWTON, self, it);
     c->newton_it = 0;
     e0 = prof_begin(c);
-    axpy_launch(d.N, p, c->dp, conv, c->st);
+    axpy_launch(L, ip, B_DP, SKIP_NEWTON);
     rr = prof_end(c, e0, KC_MISC, 3);
     c->brecs[self].raxpy.push_back(rr);
     tag_last(c, TG_NIT, nk, 0, it);
   }
-  k_budget_newton_check<<<1, 64, 0, c->st>>>(c->K, c->ctl, c->brecs[self].slot);
-  if (c->prm.floor_on) {
-    long long g = std::min<long long>(nb(d.N), 256);
-    LAUNCH(KC_MISC, 1, (k_pstats<<<(unsigned)g, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->partial),
-                        k_pstats_fin<<<1, 64, 0, c->st>>>(c->partial, (int)g, c->prm.theta * c->prm.p_floor - 1e-14, c->pst)));
-  }
+  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot);
+  if (c->prm.floor_on)
+    LAUNCH(KC_MISC, 1, pstats_launch(L, ip, c->prm.p_floor, c->prm.theta * c->prm.p_floor - 1e-14));
   return 0;
 }
 
 // ---------------------------------------------------------------- nested Newton (implicit_operators.cpp:625-738)
-static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* info) {
-  if (c->budget) return run_newton_budget(c, b, p);
+static int run_newton(imex_ctx* c, int ib, int ip, imex_solve_info* info) {
+  if (c->budget) return run_newton_budget(c, ib, ip);
   int rc = 0;
-  const Dims& d = c->d;
+  const Launch& L = c->L;
   if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL) {
-    POp po = p_op(c, 0, 1);
-    MagOp mo = mag_op(c);
-    rc |= run_gmres(c, 1, mo, po, p, b, 0, info, &c->stats.p_gmres_iters, &c->stats.p_gmres_restarts);
+    rc |= run_gmres(c, 1, mag_ids(c), p_ids(c, 0, 1), ip, ib, 0, info, &c->stats.p_gmres_iters, &c->stats.p_gmres_restarts);
     c->stats.p_solves++;
     return rc;
   }
-  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(d, c->P, b, c->cur.u, c->S, c->partial, c->K, c->eflag, c->st));
+  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(L, c->P, ib, c->cur.u));
   int it;
   double r0 = 0, r = 0;
-  const int* conv = &c->K->newton_conv;
   for (it = 0; it < c->prm.newton_max; it++) {
     c->newton_it = it;
     // residual, then the Jacobian solve; the solve is skipped on the device when the residual
     // has converged, and the host learns it from the GMRES status it polls anyway
-    POp carrier = p_op(c, 0, 0);
-    LAUNCH(KC_NEWTON_RES, 6, newton_residual(carrier, c->kb, p, b, c->ng, c->K, it, c->prm.newton_tol, c->eflag, c->st));
-    LAUNCH(KC_KFIN, 0, newton_residual_fin(d, c->kb, c->K, it, c->prm.newton_tol, c->st));
-    if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(d, c->P, p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
-    LAUNCH(KC_MISC, 1, (k_fill<<<nb(d.N), 256, 0, c->st>>>(d.N, c->dp, 0.0)));
-    POp jac = p_op(c, 1, 1);
-    MagOp mo = mag_op(c);
+    LAUNCH(KC_NEWTON_RES, 6, newton_residual(L, p_ids(c, 0, 0), c->P, ip, ib, it, c->prm.newton_tol));
+    LAUNCH(KC_KFIN, 0, newton_residual_fin(L, it, c->prm.newton_tol));
+    if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(L, c->P, ip, c->cur.u);
+    LAUNCH(KC_MISC, 1, fill_launch(L, B_DP, 0.0, SKIP_NONE));
     long long its = 0, rss = 0;
     // Newton usually needs pred_newton Jacobian solves; past that the solve is expected to be
     // skipped, so only a one-step chunk is queued speculatively
     int first = it < c->pred_newton ? c->pred[1] : 1;
-    rc |= run_gmres(c, 1, mo, jac, c->dp, c->ng, 1, nullptr, &its, &rss, conv, first);
+    rc |= run_gmres(c, 1, mag_ids(c), p_ids(c, 1, 1), B_DP, B_NG, 1, nullptr, &its, &rss, SKIP_NEWTON, first);
     if (it == 0) r0 = c->Klast.newton_res0;
     r = c->Klast.newton_res;
     if (c->Klast.newton_conv) { rc &= ~IMEX_WARN_GMRES_MAXOUTER; break; }
     c->stats.p_gmres_iters += its; c->stats.p_gmres_restarts += rss;
     c->stats.p_solves++;
-    LAUNCH(KC_MISC, 3, axpy_launch(d.N, p, c->dp, conv, c->st));
+    LAUNCH(KC_MISC, 3, axpy_launch(L, ip, B_DP, SKIP_NEWTON));
   }
   c->newton_it = 0;
   c->stats.newton_iters += it;
@@ -628,11 +631,8 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
   c->pred_newton = std::max(1, it);
   c->step_mx[2] = std::max(c->step_mx[2], it);
   if (it == c->prm.newton_max) { rc |= IMEX_WARN_NEWTON_MAXITER; c->stats.warnings |= IMEX_WARN_NEWTON_MAXITER; }
-  if (c->prm.floor_on) {
-    long long g = std::min<long long>(nb(d.N), 256);
-    LAUNCH(KC_MISC, 1, (k_pstats<<<(unsigned)g, 256, 0, c->st>>>(d.N, p, c->prm.p_floor, c->partial),
-                        k_pstats_fin<<<1, 64, 0, c->st>>>(c->partial, (int)g, c->prm.theta * c->prm.p_floor - 1e-14, c->pst)));
-  }
+  if (c->prm.floor_on)
+    LAUNCH(KC_MISC, 1, pstats_launch(L, ip, c->prm.p_floor, c->prm.theta * c->prm.p_floor - 1e-14));
   if (info) { info->iters = it; info->res0 = r0; info->res = r; info->status = rc; info->restarts = 0; }
   return rc;
 }
@@ -640,102 +640,70 @@ static int run_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* 
 // ---------------------------------------------------------------- stage pieces
 static void ensure_qe(imex_ctx* c) {
   if (c->qe_ready) return;
-  const Dims& d = c->d;
-  long long tot = (long long)d.NX * d.NY;
-  LAUNCH(KC_QEPREP, 10, (k_qe_prep<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->cur.Qe, c->pe, c->cf, c->eflag)));
-  LAUNCH(KC_SHOCK, 5, (k_shock_chi<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.Qe, c->cf, c->chi)));
-  fill_planes(c, {c->chi});
-  LAUNCH(KC_SHOCK, 2, (k_shock_eta<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->chi, c->eta)));
-  fill_planes(c, {c->eta});
+  LAUNCH(KC_QEPREP, 10, launch_qe_prep(c->L, c->P, c->cur.Qe));
+  LAUNCH(KC_SHOCK, 7, launch_shock(c->L, c->P, c->cur.Qe));
   c->qe_ready = true;
 }
 
-// TVDReconstruction + RusanovC_Update (kind 0) / Pressure_flux_Update (kind 1); the mixed form
-// reconstructs and forms the fluxes in FP32.
-static void launch_explicit(imex_ctx* c, int kind) {
-  const Dims& d = c->d;
-  if (c->mixed) k_explicit<float><<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, kind, c->cur.Qe, c->cur.Qs, c->cur.u, c->S);
-  else k_explicit<double><<<grid2(d.nx, d.ny, 32, 8), dim3(256), 0, c->st>>>(d, c->P, kind, c->cur.Qe, c->cur.Qs, c->cur.u, c->S);
-}
-
 static int sub_explicit_conv(imex_ctx* c) {
-  const Dims& d = c->d;
-  LAUNCH(KC_EXPL_CONV, 13, launch_explicit(c, 0));
-  double* u = c->cur.u;
-  fill_planes(c, {u, u + d.PS, u + 2 * d.PS, u + 3 * d.PS});
+  LAUNCH(KC_EXPL_CONV, 13, launch_explicit(c->L, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed, c->ex_tile));
+  const int u = c->cur.u;
+  fill_planes(c, {{u, 0}, {u, 1}, {u, 2}, {u, 3}});
   c->hhat_ready = false;
   return 0;
 }
 
 static int sub_mag_update_b(imex_ctx* c) {
-  const Dims& d = c->d;
   c->sub_idx = 0;
   ensure_qe(c);
-  LAUNCH(KC_MAG_RHS, 23, (k_mag_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, c->pe, c->bk, c->xk, c->hhat, c->S, c->eflag)));
-  fill_planes(c, {c->hhat});
+  LAUNCH(KC_MAG_RHS, 23, launch_mag_rhs(c->L, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, B_BK, B_XK));
+  fill_planes(c, {{B_HHAT, 0}});
   c->hhat_ready = true;
-  MagOp mo = mag_op(c);
-  POp po = p_op(c, 0, 0);
-  int rc = run_gmres(c, 0, mo, po, c->xk, c->bk, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
+  int rc = run_gmres(c, 0, mag_ids(c), p_ids(c, 0, 0), B_XK, B_BK, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
   if (!c->budget) c->stats.mag_solves++;
-  long long tot = (long long)d.NX * d.NY;
-  LAUNCH(KC_MISC, 6, (k_mag_writeback<<<nb(tot), 256, 0, c->st>>>(d, c->xk, c->cur.u)));
-  LAUNCH(KC_EMF, 12, (k_emf_flux<<<grid2(d.NX, d.NY, 64, 4), dim3(64, 4), 0, c->st>>>(d, c->cur.u, c->cur.Qe, c->pe, c->fx, c->fy, c->S)));
+  LAUNCH(KC_MISC, 6, launch_mag_writeback(c->L, c->cur.u));
+  LAUNCH(KC_EMF, 12, launch_emf_flux(c->L, c->cur.u, c->cur.Qe));
   return rc;
 }
 
 static int sub_explicit_press(imex_ctx* c) {
-  const Dims& d = c->d;
-  LAUNCH(KC_EXPL_PRESS, 16, launch_explicit(c, 1));
-  double* u = c->cur.u;
-  fill_planes(c, {u + d.PS, u + 2 * d.PS, u + 3 * d.PS, u + 4 * d.PS});
+  LAUNCH(KC_EXPL_PRESS, 16, launch_explicit(c->L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed, c->ex_tile));
+  const int u = c->cur.u;
+  fill_planes(c, {{u, 1}, {u, 2}, {u, 3}, {u, 4}});
   return 0;
 }
 
 static void ensure_hhat(imex_ctx* c) {
   if (c->hhat_ready) return;
   // hhat is produced by the magnetic RHS kernel; run it for its side output when the
-  // pressure subsystem is called on its own.
-  const Dims& d = c->d;
+  // pressure subsystem is called on its own (its b and x0 go to scratch)
   ensure_qe(c);
-  k_mag_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, c->pe, c->tmpN, c->tmpN + 3 * (long long)d.N, c->hhat, c->S, c->eflag);
-  fill_planes(c, {c->hhat});
+  launch_mag_rhs(c->L, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, B_TMPN, B_W);
+  fill_planes(c, {{B_HHAT, 0}});
   c->hhat_ready = true;
 }
 
 static int sub_pressure(imex_ctx* c) {
-  const Dims& d = c->d;
   ensure_qe(c);
   ensure_hhat(c);
-  double* u = c->cur.u;
-  if (c->P.eos != IMEX_EOS_IDEAL) {
-    long long tot = (long long)d.NX * d.NY;
-    k_visc_rk<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->pe, u, c->visc, c->eflag);
-  }
-  LAUNCH(KC_P_RHS, 15, (k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 0, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S)));
-  fill_planes(c, {c->mxs, c->mys});
+  const int u = c->cur.u;
+  if (c->P.eos != IMEX_EOS_IDEAL) launch_visc_rk(c->L, c->P, u);
+  LAUNCH(KC_P_RHS, 15, launch_p_rhs(c->L, 0, u));
+  fill_planes(c, {{B_MXS, 0}, {B_MYS, 0}});
   c->sub_idx = 1;
-  int rc = run_newton(c, c->pb, c->p, nullptr);
-  LAUNCH(KC_P_MOM, 5, (k_p_momentum<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->p, c->mxs, c->mys, u, c->S)));
-  fill_planes(c, {u + d.PS, u + 2 * d.PS});
-  LAUNCH(KC_P_RHS, 12, (k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 1, u, c->hhat, c->mxs, c->mys, c->pe, c->pb, c->p, c->Tdiag, c->S)));
+  int rc = run_newton(c, B_PB, B_P, nullptr);
+  LAUNCH(KC_P_MOM, 5, launch_p_momentum(c->L, u));
+  fill_planes(c, {{u, 1}, {u, 2}});
+  LAUNCH(KC_P_RHS, 12, launch_p_rhs(c->L, 1, u));
   c->sub_idx = 2;
-  rc |= run_newton(c, c->pb, c->p, nullptr);
-  LAUNCH(KC_P_ENERGY, 9, (k_p_energy<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, c->P, c->p, u, c->eflag)));
-  fill_planes(c, {u + 4 * d.PS});
+  rc |= run_newton(c, B_PB, B_P, nullptr);
+  LAUNCH(KC_P_ENERGY, 9, launch_p_energy(c->L, c->P, u));
+  fill_planes(c, {{u, 4}});
   return rc;
 }
 
 static int sub_ct(imex_ctx* c) {
-  const Dims& d = c->d;
-  const double* ex = c->fx; const double* ey = c->fy;
-  double sx = -1.0, sy = 1.0;
-  if (c->prm.ct_mode == IMEX_CT_PLAIN) {
-    k_ct_plain_emf<<<grid2(d.NX, d.NY), dim3(64, 4), 0, c->st>>>(d, c->P, c->cur.u, c->ezx, c->ezy);
-    ex = c->ezx; ey = c->ezy; sx = -1.0; sy = 1.0;
-  }
-  LAUNCH(KC_CT, 8, (k_ct_corner<<<grid2(d.nx + 2, d.ny + 2), dim3(64, 4), 0, c->st>>>(d, ex, ey, sx, sy, c->cur.u, c->ezpp)));
-  LAUNCH(KC_CT, 7, (k_ct_faces<<<grid2(d.NX, d.NY), dim3(64, 4), 0, c->st>>>(d, c->cur.bxqs, c->cur.byqs, c->ezpp, c->cur.bxo, c->cur.byo, c->cur.u, c->S)));
+  LAUNCH(KC_CT, 15, launch_ct(c->L, c->P, c->prm.ct_mode == IMEX_CT_PLAIN, c->cur.u, c->cur.bxqs, c->cur.byqs, c->cur.bxo, c->cur.byo));
   return 0;
 }
 
@@ -753,45 +721,40 @@ static int run_stage(imex_ctx* c) {
 
 static void bind_stage0(imex_ctx* c) {
   c->stage_idx = 0;
-  c->cur.u = c->U1; c->cur.Qe = c->u_n; c->cur.Qs = c->u_n;
-  c->cur.bxqs = c->bx_n; c->cur.byqs = c->by_n; c->cur.bxo = c->bx1; c->cur.byo = c->by1;
+  c->cur.u = B_U1; c->cur.Qe = B_UN; c->cur.Qs = B_UN;
+  c->cur.bxqs = c->iBxn; c->cur.byqs = c->iByn; c->cur.bxo = B_BX1; c->cur.byo = B_BY1;
 }
 static void bind_stage1(imex_ctx* c) {
   c->stage_idx = 1;
-  c->cur.u = c->Uo; c->cur.Qe = c->Qe; c->cur.Qs = c->Qs;
-  c->cur.bxqs = c->bxqs; c->cur.byqs = c->byqs; c->cur.bxo = c->bx2; c->cur.byo = c->by2;
+  c->cur.u = c->iUo; c->cur.Qe = B_QE; c->cur.Qs = B_QS;
+  c->cur.bxqs = B_BXQS; c->cur.byqs = B_BYQS; c->cur.bxo = c->iBx2; c->cur.byo = c->iBy2;
 }
 
 static int step_start(imex_ctx* c) {
-  const Dims& d = c->d;
-  long long tot = (long long)d.NX * d.NY;
-  long long g = std::min<long long>(nb(tot), c->nblk_start);
-  const double* in = c->state_mode ? c->Us : c->u_pri;
-  LAUNCH(KC_START, 16, (k_step_start<<<(unsigned)g, 256, 0, c->st>>>(d, c->P, c->state_mode, in, c->u_n, c->partial, c->eflag),
-                        k_step_finish<<<1, 256, 0, c->st>>>(d, c->P, c->partial, (int)g, c->S)));
+  const int in = c->state_mode ? c->iUs : B_UPRI;
+  LAUNCH(KC_START, 16, launch_step_start(c->L, c->P, c->state_mode, in));
   return 0;
 }
 
-// Everything a step queues; the step's input (Us or u_pri, bx_n, by_n) is only read, so a
+// Everything a step queues; the step's input (Us or u_pri, bxn, byn) is only read, so a
 // discarded step can be queued again from the same input.
 static int step_body(imex_ctx* c) {
   int rc = 0;
-  const Dims& d = c->d;
   step_start(c);
   bind_stage0(c);
   rc |= run_stage(c);
-  long long tot = (long long)d.NX * d.NY;
-  LAUNCH(KC_COMBINE, 38, (k_stage_combine<<<nb(tot), 256, 0, c->st>>>(d, c->u_n, c->U1, c->Qe, c->Qs, c->bx_n, c->bx1, c->by_n, c->by1, c->bxqs, c->byqs, c->S)));
+  LAUNCH(KC_COMBINE, 38, launch_stage_combine(c->L, c->iBxn, c->iByn));
   bind_stage1(c);
   rc |= run_stage(c);
   return rc;
 }
 
 static void step_commit(imex_ctx* c) {
-  std::swap(c->bx_n, c->bx2); std::swap(c->by_n, c->by2); std::swap(c->Us, c->Uo);
+  std::swap(c->iBxn, c->iBx2); std::swap(c->iByn, c->iBy2); std::swap(c->iUs, c->iUo);
   c->state_mode = 1;
   c->stats.steps++;
 }
+
 
 static void hist_push(imex_ctx* c, const int mx[NBK]) {
   for (int k = 0; k < NBK; k++) c->hist[k][c->hist_pos] = mx[k];
@@ -804,14 +767,16 @@ static void hist_push(imex_ctx* c, const int mx[NBK]) {
 // Budgets, per solve position: the largest count that position needed in the recent steps
 // (or, for a position not seen lately, the largest of its kind), plus budget_margin Arnoldi
 // steps per GMRES sweep.  Returns false (host scheduling) until there is history, or when a
-// recent solve needed a restart.
+// recent solve needed a restart.  A batch has no host scheduling: it starts from generous
+// budgets and grows them when a step outruns them.
 static bool choose_budgets(imex_ctx* c) {
-  if (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0) return false;
+  if (c->ni == 1 && (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0)) return false;
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   int mx[NBK] = {0, 0, 0, 0, 0};
+  if (c->hist_n == 0) { mx[0] = mx[1] = std::min(m - 1, 11); mx[2] = 2; mx[3] = mx[4] = 3; }
   for (int i = 0; i < c->hist_n; i++) for (int k = 0; k < NBK; k++) mx[k] = std::max(mx[k], c->hist[k][i]);
-  if (mx[0] >= m || mx[1] >= m) return false;              // a recent solve needed a restart
-  if (mx[3] >= MAXSWEEP - 1 || mx[4] >= MAXSWEEP - 1) return false;
+  if (c->ni == 1 && (mx[0] >= m || mx[1] >= m)) return false;              // a recent solve needed a restart
+  if (c->ni == 1 && (mx[3] >= MAXSWEEP - 1 || mx[4] >= MAXSWEEP - 1)) return false;
   for (int k = 0; k < NGKEY; k++) {
     const int op = (k / NIT) % 3 == 0 ? 0 : 1;
     int st = -1, sw = -1;
@@ -823,52 +788,56 @@ static bool choose_budgets(imex_ctx* c) {
   for (int k = 0; k < NNKEY; k++) {
     int nb = -1;
     for (int i = 0; i < c->hist_n; i++) nb = std::max(nb, c->nhist[i][k]);
-    c->tab.N[k] = std::max(1, nb < 0 ? mx[2] : nb);
+    c->tab.N[k] = std::min(NIT - 1, std::max(1, nb < 0 ? mx[2] : nb));
   }
   return true;
 }
 
-// Reads the step's log: returns 1 if a solve outran its budget, else books the iteration counts
-// and, with the profiler on, moves launches that found their solve converged to early_exit.
+// Reads the step's logs (all instances): returns 1 if a solve outran its budget, else books the
+// iteration counts (sums over instances, maxima for the budgets) and, with the profiler on,
+// moves launches that found their solve converged to early_exit.
 static int budget_finish(imex_ctx* c, long long gen0) {
-  HIPCHK(hipMemcpyAsync(c->ctlh, c->ctl, (size_t)c->nslot * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->ctlh, c->ctl_all, (size_t)c->ni * CTLN * sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipEventRecord(c->ev[0], c->st));
   HIPCHK(hipEventSynchronize(c->ev[0]));
-  if (c->ctlh[0]) return 1;
-  const bool fix = c->prof.on && c->prof.gen == gen0;
+  for (int i = 0; i < c->ni; i++) if (c->ctlh[(size_t)i * CTLN]) return 1;
+  const bool fix = c->prof.on && c->prof.gen == gen0 && c->ni == 1;
   auto idle = [&](int r) { if (fix && r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
   int mx[NBK] = {0, 0, 0, 0, 0};
-  for (const BudgetRec& r : c->brecs) {
-    if (r.kind == 1) {
-      const int nf = c->ctlh[r.slot];
-      c->stats.newton_iters += nf;
-      mx[2] = std::max(mx[2], nf);
-      c->ncur[r.key] = std::max(c->ncur[r.key], nf);
-      for (size_t i = 0; i < r.rnres.size(); i++) if ((int)i > nf) { idle(r.rnres[i]); idle(r.rnresf[i]); }
-      for (size_t i = 0; i < r.rfill.size(); i++) if ((int)i >= nf) { idle(r.rfill[i]); idle(r.raxpy[i]); }
-      continue;
-    }
-    const int steps = c->ctlh[r.slot], sweeps = c->ctlh[r.slot + 1];
-    const bool skipped = r.parent >= 0 && r.it >= c->ctlh[c->brecs[r.parent].slot];
-    for (int sw = 0; sw < (int)r.rres.size(); sw++) {
-      const bool live = !skipped && sw <= sweeps;                       // residuals that ran
-      if (!live) { idle(r.rres[sw]); idle(r.rresf[sw]); }
-    }
-    for (int sw = 0; sw < r.nsw; sw++) {
-      const int ss = (!skipped && sw < sweeps) ? c->ctlh[r.slot + 2 + sw] : 0;   // Arnoldi steps of this sweep
-      for (int j = ss; j < r.B; j++) {
-        const size_t q = (size_t)sw * r.B + j;
-        idle(r.ra[q]); idle(r.ru[q]); idle(r.rfa[q]); idle(r.rfu[q]);
+  for (int i = 0; i < c->ni; i++) {
+    const int* ctl = c->ctlh + (size_t)i * CTLN;
+    for (const BudgetRec& r : c->brecs) {
+      if (r.kind == 1) {
+        const int nf = ctl[r.slot];
+        c->stats.newton_iters += nf;
+        mx[2] = std::max(mx[2], nf);
+        c->ncur[r.key] = std::max(c->ncur[r.key], nf);
+        for (size_t q = 0; q < r.rnres.size(); q++) if ((int)q > nf) { idle(r.rnres[q]); idle(r.rnresf[q]); }
+        for (size_t q = 0; q < r.rfill.size(); q++) if ((int)q >= nf) { idle(r.rfill[q]); idle(r.raxpy[q]); }
+        continue;
       }
-      if (ss == 0) idle(r.rx[sw]);
-      if (!skipped && sw < sweeps) { mx[r.op] = std::max(mx[r.op], ss); cur_gmres(c, r.key, ss, 0); }
+      const int steps = ctl[r.slot], sweeps = ctl[r.slot + 1];
+      const bool skipped = r.parent >= 0 && r.it >= ctl[c->brecs[r.parent].slot];
+      for (int sw = 0; sw < (int)r.rres.size(); sw++) {
+        const bool live = !skipped && sw <= sweeps;                       // residuals that ran
+        if (!live) { idle(r.rres[sw]); idle(r.rresf[sw]); }
+      }
+      for (int sw = 0; sw < r.nsw; sw++) {
+        const int ss = (!skipped && sw < sweeps) ? ctl[r.slot + 2 + sw] : 0;   // Arnoldi steps of this sweep
+        for (int j = ss; j < r.B; j++) {
+          const size_t q = (size_t)sw * r.B + j;
+          if (q < r.ra.size()) { idle(r.ra[q]); idle(r.ru[q]); idle(r.rfa[q]); idle(r.rfu[q]); }
+        }
+        if (ss == 0 && sw < (int)r.rx.size()) idle(r.rx[sw]);
+        if (!skipped && sw < sweeps) { mx[r.op] = std::max(mx[r.op], ss); cur_gmres(c, r.key, ss, 0); }
+      }
+      if (skipped) continue;
+      mx[3 + r.op] = std::max(mx[3 + r.op], sweeps);
+      cur_gmres(c, r.key, c->mixed ? 0 : steps, sweeps);
+      if (c->mixed) c->stats.mixed_refinements += sweeps;
+      if (r.op == 0) { c->stats.mag_solves++; c->stats.mag_gmres_iters += steps; }
+      else { c->stats.p_solves++; c->stats.p_gmres_iters += steps; }
     }
-    if (skipped) continue;
-    mx[3 + r.op] = std::max(mx[3 + r.op], sweeps);
-    cur_gmres(c, r.key, c->mixed ? 0 : steps, sweeps);
-    if (c->mixed) c->stats.mixed_refinements += sweeps;
-    if (r.op == 0) { c->stats.mag_solves++; c->stats.mag_gmres_iters += steps; }
-    else { c->stats.p_solves++; c->stats.p_gmres_iters += steps; }
   }
   hist_push(c, mx);
   return 0;
@@ -899,11 +868,16 @@ static bool tab_covers(const BudTab& cap, const BudTab& t, int mixed) {
   return true;
 }
 
+static void drop_graphs(imex_ctx* c) {
+  for (GraphEntry& g : c->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
+  c->graphs.clear();
+}
+
 // Queue (or replay) the budgeted step as a captured hipGraph.  A graph is captured for a
 // capacity table a little above the current budgets (and for the current state/face buffer set
 // and state mode); replays toggle the budget-dependent nodes to the current table.
 static int budget_graph_launch(imex_ctx* c) {
-  const int parity = c->Us == c->U2 ? 0 : 1;
+  const int parity = c->iUs == B_U2 ? 0 : 1;
   const BudTab cur = c->tab;
   GraphEntry* g = nullptr;
   for (GraphEntry& e : c->graphs) {
@@ -925,7 +899,7 @@ static int budget_graph_launch(imex_ctx* c) {
         cap.B[k] = std::min(m, cur.B[k] + 2);
         cap.S[k] = c->mixed ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : 1;
       }
-      for (int k = 0; k < NNKEY; k++) cap.N[k] = std::min(c->prm.newton_max, cur.N[k] + 1);
+      for (int k = 0; k < NNKEY; k++) cap.N[k] = std::min(std::min(c->prm.newton_max, NIT - 1), cur.N[k] + 1);
     }
     c->tab = cap;
     c->tags.clear();
@@ -980,27 +954,37 @@ static int budget_graph_launch(imex_ctx* c) {
 
 static int one_step(imex_ctx* c) {
   int rc = 0;
-  cur_reset(c);
-  if (choose_budgets(c)) {
+  const Launch& L = c->L;
+  for (int attempt = 0;; attempt++) {
+    cur_reset(c);
+    if (!choose_budgets(c)) break;
+    if (c->ni > 1 && attempt > 0) {               // a batch grows the budgets of a failed step
+      const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
+      for (int k = 0; k < NGKEY; k++) { c->tab.B[k] = std::min(m, c->tab.B[k] + 4 * attempt); c->tab.S[k] = std::min(MAXSWEEP - 1, c->tab.S[k] + attempt); }
+      for (int k = 0; k < NNKEY; k++) c->tab.N[k] = std::min(std::min(c->prm.newton_max, NIT - 1), c->tab.N[k] + attempt);
+    }
     const long long gen0 = c->prof.gen;
     c->budget = true;
     c->nslot = 1;
     c->brecs.clear();
-    k_budget_begin<<<1, 64, 0, c->st>>>(c->S, c->pst, c->eflag, c->bsave, c->esave, c->ctl);
-    if (c->prm.schedule == IMEX_SCHED_GRAPH && !c->prof.on) rc = budget_graph_launch(c);
+    k_budget_begin<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE]);
+    if (c->prm.schedule != IMEX_SCHED_BUDGET && !c->prof.on) rc = budget_graph_launch(c);
     else rc = step_body(c);
     c->budget = false;
     if (rc < 0) return rc;
-    if (c->nslot > 256) return set_err(IMEX_ERR_ARG, "too many solves in one step");
+    if (c->nslot > CTLN) return set_err(IMEX_ERR_ARG, "too many solves in one step");
     int f = budget_finish(c, gen0);
     if (f < 0) return f;
     c->stats.budget_steps++;
     if (f == 0) { step_commit(c); return rc; }
-    // a solve outran its budget: restore the step's scalars and rerun it under host scheduling
+    // a solve outran its budget: restore the step's scalars and rerun it (host scheduling for a
+    // single instance, larger budgets for a batch)
     c->stats.budget_reruns++;
-    k_budget_restore<<<1, 64, 0, c->st>>>(c->S, c->pst, c->eflag, c->bsave, c->esave);
-    cur_reset(c);
+    k_budget_restore<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE]);
+    if (c->ni == 1) break;
+    if (attempt >= 3) return set_err(IMEX_ERR_UNSUPPORTED, "batch step did not converge within its budgets");
   }
+  cur_reset(c);
   for (int k = 0; k < NBK; k++) c->step_mx[k] = 0;
   rc = step_body(c);
   hist_push(c, c->step_mx);
@@ -1018,23 +1002,27 @@ static void soa_to_aos(const Dims& d, const std::vector<double>& soa, double* ao
   for (long long c = 0; c < (long long)d.NX * d.NY; c++)
     for (int k = 0; k < NVAR; k++) aos[NVAR * c + k] = soa[(size_t)k * d.PS + c];
 }
-static int up_grid(imex_ctx* c, double* dst, const double* aos) {
-  std::vector<double> s; aos_to_soa(c->d, aos, s);
-  HIPCHK(hipMemcpy(dst, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
+static int up_grid(imex_ctx* c, int i, int b, const double* aos) {
+  const Dims& d = c->hI[i].d;
+  std::vector<double> s; aos_to_soa(d, aos, s);
+  HIPCHK(hipMemcpy(c->hI[i].buf[b], s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
-static int down_grid(imex_ctx* c, const double* src, double* aos) {
-  std::vector<double> s((size_t)NVAR * c->d.PS);
-  HIPCHK(hipMemcpy(s.data(), src, s.size() * sizeof(double), hipMemcpyDeviceToHost));
-  soa_to_aos(c->d, s, aos);
+static int down_grid(imex_ctx* c, int i, int b, double* aos) {
+  const Dims& d = c->hI[i].d;
+  std::vector<double> s((size_t)NVAR * d.PS);
+  HIPCHK(hipMemcpy(s.data(), c->hI[i].buf[b], s.size() * sizeof(double), hipMemcpyDeviceToHost));
+  soa_to_aos(d, s, aos);
   return 0;
 }
-static int up_plane(imex_ctx* c, double* dst, const double* a) {
-  HIPCHK(hipMemcpy(dst, a, (size_t)c->d.NX * c->d.NY * sizeof(double), hipMemcpyHostToDevice));
+static int up_plane(imex_ctx* c, int i, int b, const double* a) {
+  const Dims& d = c->hI[i].d;
+  HIPCHK(hipMemcpy(c->hI[i].buf[b], a, (size_t)d.NX * d.NY * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
-static int down_plane(imex_ctx* c, const double* src, double* a) {
-  HIPCHK(hipMemcpy(a, src, (size_t)c->d.NX * c->d.NY * sizeof(double), hipMemcpyDeviceToHost));
+static int down_plane(imex_ctx* c, int i, int b, double* a) {
+  const Dims& d = c->hI[i].d;
+  HIPCHK(hipMemcpy(a, c->hI[i].buf[b], (size_t)d.NX * d.NY * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -1050,8 +1038,8 @@ static void scalars_to_dev(const imex_scalars* sc, double* Sh) {
   Sh[S_VMAX] = sc->v_max; Sh[S_CH] = sc->ch; Sh[S_CHX] = sc->ch_x; Sh[S_CHY] = sc->ch_y; Sh[S_PREVDT] = sc->prev_dt;
   Sh[S_TIMESTEP] = sc->timestep; Sh[S_RKS] = sc->rks;
 }
-static int pull_scalars(imex_ctx* c) {
-  HIPCHK(hipMemcpyAsync(c->Sh, c->S, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, c->st));
+static int pull_scalars(imex_ctx* c, int i) {
+  HIPCHK(hipMemcpyAsync(c->Sh, c->hI[i].S, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
@@ -1195,9 +1183,8 @@ int imex_init_case(const imex_params* p, int case_id, double amp, double u0, dou
   return 0;
 }
 
-int imex_create(const imex_params* prm, imex_ctx** out) {
-  if (!prm || !out) return set_err(IMEX_ERR_ARG, "null argument");
-  if (prm->nx < 4 || prm->ny < 4) return set_err(IMEX_ERR_ARG, "grid too small (need nx, ny >= 4)");
+
+static int check_params(const imex_params* prm) {
   if (prm->gmres_restart > MAXM) return set_err(IMEX_ERR_ARG, "gmres_restart exceeds 32");
   if ((prm->bc_x != IMEX_BC_PERIODIC && prm->bc_x != IMEX_BC_TRANSMISSIVE) ||
       (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE))
@@ -1208,83 +1195,152 @@ int imex_create(const imex_params* prm, imex_ctx** out) {
     return set_err(IMEX_ERR_ARG, "bad precision");
   if (prm->schedule != IMEX_SCHED_GRAPH && prm->schedule != IMEX_SCHED_BUDGET && prm->schedule != IMEX_SCHED_HOST)
     return set_err(IMEX_ERR_ARG, "bad schedule");
+  return 0;
+}
+
+// Device memory of instance i, carved from one allocation (256-byte aligned pieces).
+static int alloc_instance(imex_ctx* c, int i) {
+  Inst& I = c->hI[i];
+  const Dims& d = I.d;
+  const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
+  const size_t G8 = (size_t)NVAR * d.PS, G1 = (size_t)d.PS, N = (size_t)d.N, n3 = 3 * N;
+  // partial rows: the largest local block count of any kernel that writes per-block partials
+  const long long tiles = cdiv(d.nx, TX) * cdiv(d.ny, TY);
+  const long long rows = std::min<long long>(4096, std::max({tiles, cdiv(3LL * d.N, 256), cdiv((long long)d.NX * d.NY, 256)}));
+  std::vector<std::pair<void**, size_t>> parts;
+  auto P8 = [&](double** p, size_t n) { parts.push_back({(void**)p, n * sizeof(double)}); };
+  for (int b = B_UPRI; b <= B_QS; b++) P8(&I.buf[b], G8);
+  for (int b = B_BX0; b <= B_VISC; b++) P8(&I.buf[b], G1);
+  for (int b : {B_W, B_AV, B_XK, B_BK}) P8(&I.buf[b], n3);
+  P8(&I.buf[B_TMPN], 6 * N);
+  for (int b = B_P; b <= B_VSLOPE; b++) P8(&I.buf[b], N);
+  for (int k = 0; k <= m; k++) P8(&I.buf[B_V0 + k], n3);
+  for (int k = 0; k < NFBUF; k++) I.fbuf[k] = nullptr;
+  if (c->mixed) {
+    for (int b : {F_W32, F_AV32}) parts.push_back({(void**)&I.fbuf[b], n3 * sizeof(float)});
+    for (int k = 0; k <= m; k++) parts.push_back({(void**)&I.fbuf[F_V0 + k], n3 * sizeof(float)});
+  }
+  P8(&I.S, S_NSLOTS);
+  P8(&I.partial, (size_t)rows * (MAXM + 2));
+  P8(&I.pst, 4);
+  P8(&I.bsave, S_NSLOTS + 4);
+  parts.push_back({(void**)&I.eflag, sizeof(int)});
+  parts.push_back({(void**)&I.esave, sizeof(int)});
+  parts.push_back({(void**)&I.K, sizeof(KrylovDev)});
+  size_t off = 0;
+  std::vector<size_t> offs;
+  for (auto& pt : parts) { off = (off + 255) & ~(size_t)255; offs.push_back(off); off += pt.second; }
+  char* base = nullptr;
+  int rc = dalloc(c, &base, off);
+  if (rc) return rc;
+  HIPCHK(hipMemset(base, 0, off));
+  for (size_t q = 0; q < parts.size(); q++) *parts[q].first = base + offs[q];
+  for (int k = m + 1; k <= MAXM; k++) I.buf[B_V0 + k] = I.buf[B_V0];
+  if (c->mixed) for (int k = m + 1; k <= MAXM; k++) I.fbuf[F_V0 + k] = I.fbuf[F_V0];
+  I.ctl = c->ctl_all + (size_t)i * CTLN;
+  KrylovDev kz; std::memset(&kz, 0, sizeof(kz)); kz.tol = c->prm.gmres_tol;
+  HIPCHK(hipMemcpy(I.K, &kz, sizeof(kz), hipMemcpyHostToDevice));
+  const double init[4] = {1e300, 0.0, 0.0, 0.0};
+  HIPCHK(hipMemcpy(I.pst, init, sizeof(init), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Launch descriptor covering instance i alone (host set-up work on one instance of a batch).
+static int one_launch(imex_ctx* c, int i, Launch& L1) {
+  if (c->ni == 1) { L1 = c->L; return 0; }
+  std::vector<int> h((size_t)NSHAPE * 3);
+  L1.IT = c->IT + i; L1.st = c->st;
+  for (int sh = 0; sh < NSHAPE; sh++) {
+    int g, nb;
+    shape_grid(c, sh, c->hI[i].d, g, nb);
+    h[(size_t)sh * 3] = 0; h[(size_t)sh * 3 + 1] = nb; h[(size_t)sh * 3 + 2] = g;
+    L1.nblocks[sh] = nb;
+    L1.bm[sh].pre = c->bmap1_dev + (size_t)sh * 3;
+    L1.bm[sh].gx = L1.bm[sh].pre + 2;
+    L1.bm[sh].ni = 1;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(c->bmap1_dev, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ||div B||_2 of every instance
+static int div_b_all(imex_ctx* c, std::vector<double>& out) {
+  launch_divb(c->L, c->iBxn, c->iByn);
+  HIPCHK(hipStreamSynchronize(c->st));
+  out.assign(c->ni, 0.0);
+  std::vector<double> h;
+  for (int i = 0; i < c->ni; i++) {
+    int g, nb;
+    shape_grid(c, SH_DIVB, c->hI[i].d, g, nb);
+    h.resize(nb);
+    HIPCHK(hipMemcpy(h.data(), c->hI[i].partial, nb * sizeof(double), hipMemcpyDeviceToHost));
+    double s = 0; for (double v : h) s += v;
+    out[i] = std::sqrt(s);
+  }
+  return 0;
+}
+
+int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* ny, imex_ctx** out) {
+  if (!prm || !out || n < 1 || !nx || !ny) return set_err(IMEX_ERR_ARG, "null argument or empty batch");
+  int rc = check_params(prm);
+  if (rc) return rc;
+  for (int i = 0; i < n; i++)
+    if (nx[i] < 4 || ny[i] < 4) return set_err(IMEX_ERR_ARG, "grid too small (need nx, ny >= 4)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(IMEX_ERR_HIP, "no HIP device");
   imex_ctx* c = new imex_ctx();
   c->prm = *prm;
-  c->d = make_dims(*prm);
+  c->prm.nx = nx[0]; c->prm.ny = ny[0];
   c->P = make_phys(*prm);
-  const Dims& d = c->d;
-  int rc = 0;
-  auto A = [&](double** p, size_t n) { if (!rc) rc = dalloc(c, p, n); };
-  size_t G8 = (size_t)NVAR * d.PS, G1 = (size_t)d.PS;
-  A(&c->u_pri, G8); A(&c->u_n, G8); A(&c->U1, G8); A(&c->U2, G8); A(&c->U3, G8); A(&c->Qe, G8); A(&c->Qs, G8);
-  double** planes[] = {&c->bx_n, &c->by_n, &c->bx1, &c->by1, &c->bx2, &c->by2, &c->bxqs, &c->byqs, &c->fx, &c->fy,
-                       &c->pe, &c->cf, &c->chi, &c->eta, &c->hhat, &c->mxs, &c->mys, &c->ezpp, &c->ezx, &c->ezy, &c->visc};
-  for (double** pp : planes) A(pp, G1);
-  size_t n3 = (size_t)3 * d.N;
-  int m = std::min(std::max(prm->gmres_restart, 1), MAXM);
-  for (int i = 0; i <= MAXM; i++) {
-    if (i <= m) A(&c->V[i], n3); else c->V[i] = c->V[0];
-  }
-  A(&c->w, n3); A(&c->Av, n3); A(&c->xk, n3); A(&c->bk, n3); A(&c->tmpN, 6 * (size_t)d.N);
   c->mixed = prm->precision == IMEX_PREC_MIXED ? 1 : 0;
-  c->kb.w32 = nullptr; c->kb.Av32 = nullptr;
-  for (int i = 0; i <= MAXM; i++) c->V32[i] = nullptr;
-  if (c->mixed) {       // FP32 basis of the inner solves
-    auto AF = [&](float** pp, size_t n) { if (!rc) rc = dalloc(c, pp, n); };
-    for (int i = 0; i <= m; i++) AF(&c->V32[i], n3);
-    for (int i = m + 1; i <= MAXM; i++) c->V32[i] = c->V32[0];
-    AF(&c->kb.w32, n3); AF(&c->kb.Av32, n3);
+  c->ni = n;
+  c->iprm.assign(n, c->prm);
+  c->hI.resize(n);
+  for (int i = 0; i < n; i++) {
+    c->iprm[i].nx = nx[i]; c->iprm[i].ny = ny[i];
+    std::memset(&c->hI[i], 0, sizeof(Inst));
+    c->hI[i].d = make_dims(c->iprm[i]);
+    c->cells += (long long)nx[i] * ny[i];
   }
-  c->nblk_start = 1024;
-  c->kb.nblk_apply = 2048; c->kb.nblk_vec = 2048;
-  A(&c->partial, (size_t)4096 * (MAXM + 2));
-  A(&c->p, d.N); A(&c->pb, d.N); A(&c->ng, d.N); A(&c->dp, d.N); A(&c->Tdiag, d.N); A(&c->vslope, d.N); A(&c->pst, 4);
-  if (!rc) rc = dalloc(c, &c->K, 1);
-  if (!rc) rc = dalloc(c, &c->S, S_NSLOTS);
-  if (!rc) rc = dalloc(c, &c->eflag, 1);
-  if (!rc) rc = dalloc(c, &c->cnt_start, 1);
-  if (!rc) rc = dalloc(c, &c->cnt_pstats, 1);
-  if (!rc) rc = dalloc(c, &c->bsave, S_NSLOTS + 4);
-  if (!rc) rc = dalloc(c, &c->esave, 1);
-  if (!rc) rc = dalloc(c, &c->ctl, 256);
-  if (rc) { imex_destroy(c); return rc; }
-  c->Us = c->U2; c->Uo = c->U3;
-  c->kb.w = c->w; c->kb.Av = c->Av; c->kb.partial = c->partial;     // w32 / Av32 set above
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "stream"); }
-  if (hipHostMalloc((void**)&c->Kh, 2 * sizeof(KrylovDev), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
-  if (hipHostMalloc((void**)&c->Sh, S_NSLOTS * sizeof(double), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
-  if (hipHostMalloc((void**)&c->ctlh, 256 * sizeof(int), hipHostMallocDefault) != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "pinned"); }
+  auto fail = [&](int r) { imex_destroy(c); return r; };
+  if ((rc = dalloc(c, &c->ctl_all, (size_t)n * CTLN))) return fail(rc);
+  if (hipMemset(c->ctl_all, 0, (size_t)n * CTLN * sizeof(int)) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "memset"));
+  for (int i = 0; i < n; i++) if ((rc = alloc_instance(c, i))) return fail(rc);
+  if ((rc = dalloc(c, &c->IT, (size_t)n))) return fail(rc);
+  if (hipMemcpy(c->IT, c->hI.data(), (size_t)n * sizeof(Inst), hipMemcpyHostToDevice) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "table"));
+  if ((rc = dalloc(c, &c->bmap_dev, (size_t)NSHAPE * (2 * n + 1)))) return fail(rc);
+  if ((rc = dalloc(c, &c->bmap1_dev, (size_t)NSHAPE * 3))) return fail(rc);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { c->st = nullptr; return fail(set_err(IMEX_ERR_HIP, "stream")); }
   (void)hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
   (void)hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);
-  // zero every buffer once (outside any captured region)
-  KrylovDev kz; std::memset(&kz, 0, sizeof(kz)); kz.tol = prm->gmres_tol;
-  (void)hipMemcpy(c->K, &kz, sizeof(kz), hipMemcpyHostToDevice);
-  std::memset(c->Sh, 0, S_NSLOTS * sizeof(double));
-  (void)hipMemcpy(c->S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice);
-  (void)hipMemset(c->eflag, 0, sizeof(int));
-  (void)hipMemset(c->cnt_start, 0, sizeof(unsigned));
-  (void)hipMemset(c->cnt_pstats, 0, sizeof(unsigned));
-  (void)hipMemset(c->ctl, 0, 256 * sizeof(int));
-  for (double** pp : planes) (void)hipMemset(*pp, 0, G1 * sizeof(double));
-  for (double* g8 : {c->U1, c->U2, c->U3, c->Qe, c->Qs, c->u_n, c->u_pri}) (void)hipMemset(g8, 0, G8 * sizeof(double));
-  if (hipDeviceSynchronize() != hipSuccess) { imex_destroy(c); return set_err(IMEX_ERR_HIP, "init sync"); }
+  c->L.IT = c->IT; c->L.st = c->st;
+  if ((rc = build_maps(c))) return fail(rc);
+  if (hipHostMalloc((void**)&c->Kh, 2 * sizeof(KrylovDev), hipHostMallocDefault) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "pinned"));
+  if (hipHostMalloc((void**)&c->Sh, S_NSLOTS * sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "pinned"));
+  if (hipHostMalloc((void**)&c->ctlh, (size_t)n * CTLN * sizeof(int), hipHostMallocDefault) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "pinned"));
+  if (hipDeviceSynchronize() != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "init sync"));
   c->stats.min_p = 1e300;
-  { double init[4] = {1e300, 0.0, 0.0, 0.0}; (void)hipMemcpy(c->pst, init, sizeof(init), hipMemcpyHostToDevice); }
   bind_stage0(c);
   *out = c;
   return 0;
 }
 
+int imex_create(const imex_params* prm, imex_ctx** out) {
+  if (!prm || !out) return set_err(IMEX_ERR_ARG, "null argument");
+  return imex_batch_create(prm, 1, &prm->nx, &prm->ny, out);
+}
+
+int imex_batch_size(const imex_ctx* c) { return c ? c->ni : 0; }
+
 int imex_destroy(imex_ctx* c) {
   if (!c) return 0;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  drop_graphs(c);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->Kh) (void)hipHostFree(c->Kh);
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ctlh) (void)hipHostFree(c->ctlh);
-  for (GraphEntry& g : c->graphs) { (void)hipGraphExecDestroy(g.exec); (void)hipGraphDestroy(g.graph); }
   for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
   if (c->st) { (void)hipEventDestroy(c->ev[0]); (void)hipEventDestroy(c->ev[1]); (void)hipStreamDestroy(c->st); }
   delete c;
@@ -1293,41 +1349,57 @@ int imex_destroy(imex_ctx* c) {
 
 long long imex_device_bytes(const imex_ctx* c) { return c ? c->bytes : 0; }
 
-int imex_set_state(imex_ctx* c, const double* u_pri, const double* bx, const double* by, double t, int timestep, double prev_dt) {
+int imex_batch_set_state(imex_ctx* c, int i, const double* u_pri, const double* bx, const double* by,
+                         double t, int timestep, double prev_dt) {
   if (!c || !u_pri || !bx || !by) return set_err(IMEX_ERR_ARG, "null argument");
-  std::vector<double> tmp(u_pri, u_pri + (size_t)NVAR * c->d.NX * c->d.NY);
-  hfill(c->prm, tmp.data(), NVAR, 0);           // bound_cond_u(u_pri) at the top of the step (main.cpp:336)
-  int rc = up_grid(c, c->u_pri, tmp.data());
-  if (!rc) rc = up_plane(c, c->bx_n, bx);
-  if (!rc) rc = up_plane(c, c->by_n, by);
+  if (i < 0 || i >= c->ni) return set_err(IMEX_ERR_ARG, "instance index out of range");
+  HIPCHK(hipStreamSynchronize(c->st));
+  const Dims& d = c->hI[i].d;
+  std::vector<double> tmp(u_pri, u_pri + (size_t)NVAR * d.NX * d.NY);
+  hfill(c->iprm[i], tmp.data(), NVAR, 0);       // bound_cond_u(u_pri) at the top of the step (main.cpp:336)
+  int rc = up_grid(c, i, B_UPRI, tmp.data());
+  if (!rc) rc = up_plane(c, i, c->iBxn, bx);
+  if (!rc) rc = up_plane(c, i, c->iByn, by);
   if (rc) return rc;
   std::memset(c->Sh, 0, S_NSLOTS * sizeof(double));
   c->Sh[S_T] = t; c->Sh[S_TIMESTEP] = timestep; c->Sh[S_PREVDT] = prev_dt;
-  HIPCHK(hipMemcpy(c->S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice));
-  c->state_mode = 0;
+  HIPCHK(hipMemcpy(c->hI[i].S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice));
+  if (c->ni == 1) c->state_mode = 0;
+  else if (c->state_mode == 1) {        // the batch already holds conservative states: convert this one
+    Launch L1;
+    if ((rc = one_launch(c, i, L1))) return rc;
+    launch_pri2con(L1, c->P, B_UPRI, c->iUs);
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
   c->hist_n = 0; c->hist_pos = 0;       // iteration budgets are relearned for the new state
   return 0;
 }
 
-int imex_get_state(imex_ctx* c, double* u_con, double* u_pri, double* bx, double* by, imex_scalars* sc) {
+int imex_set_state(imex_ctx* c, const double* u_pri, const double* bx, const double* by, double t, int timestep, double prev_dt) {
+  return imex_batch_set_state(c, 0, u_pri, bx, by, t, timestep, prev_dt);
+}
+
+int imex_batch_get_state(imex_ctx* c, int i, double* u_con, double* u_pri, double* bx, double* by, imex_scalars* sc) {
   if (!c) return set_err(IMEX_ERR_ARG, "null ctx");
+  if (i < 0 || i >= c->ni) return set_err(IMEX_ERR_ARG, "instance index out of range");
   HIPCHK(hipStreamSynchronize(c->st));
-  const Dims& d = c->d;
-  long long tot = (long long)d.NX * d.NY;
-  if (c->state_mode == 1) {
-    k_con2pri<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->Us, c->u_pri, c->eflag);
-  } else {
-    k_pri2con<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->u_pri, c->Us, c->eflag);
-  }
-  HIPCHK(hipStreamSynchronize(c->st));
-  int rc = 0;
-  if (u_con) rc = down_grid(c, c->Us, u_con);
-  if (!rc && u_pri) rc = down_grid(c, c->u_pri, u_pri);
-  if (!rc && bx) rc = down_plane(c, c->bx_n, bx);
-  if (!rc && by) rc = down_plane(c, c->by_n, by);
+  Launch L1;
+  int rc = one_launch(c, i, L1);
   if (rc) return rc;
-  if (sc) { if ((rc = pull_scalars(c))) return rc; scalars_from_dev(c->Sh, sc); }
+  if (c->state_mode == 1) launch_con2pri(L1, c->P, c->iUs, B_UPRI);
+  else launch_pri2con(L1, c->P, B_UPRI, c->iUs);
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (u_con) rc = down_grid(c, i, c->iUs, u_con);
+  if (!rc && u_pri) rc = down_grid(c, i, B_UPRI, u_pri);
+  if (!rc && bx) rc = down_plane(c, i, c->iBxn, bx);
+  if (!rc && by) rc = down_plane(c, i, c->iByn, by);
+  if (rc) return rc;
+  if (sc) { if ((rc = pull_scalars(c, i))) return rc; scalars_from_dev(c->Sh, sc); }
   return 0;
+}
+
+int imex_get_state(imex_ctx* c, double* u_con, double* u_pri, double* bx, double* by, imex_scalars* sc) {
+  return imex_batch_get_state(c, 0, u_con, u_pri, bx, by, sc);
 }
 
 int imex_step_async(imex_ctx* c, int nsteps) {
@@ -1351,14 +1423,21 @@ int imex_sync(imex_ctx* c) {
 int imex_get_stats(imex_ctx* c, imex_stats* s) {
   if (!c || !s) return IMEX_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->st));
-  int ef = 0;
-  HIPCHK(hipMemcpy(&ef, c->eflag, sizeof(int), hipMemcpyDeviceToHost));
-  if (ef) c->stats.warnings |= IMEX_WARN_EOS_MAXITER;
-  double acc[4];
-  HIPCHK(hipMemcpy(acc, c->pst, sizeof(acc), hipMemcpyDeviceToHost));
-  c->stats.min_p = acc[0];
-  c->stats.max_floored = (int)acc[1];
-  if (acc[2] != 0.0) c->stats.warnings |= IMEX_WARN_POSITIVITY;
+  double mn = 1e300; int mf = 0; bool viol = false, ef_any = false;
+  for (int i = 0; i < c->ni; i++) {
+    int ef = 0;
+    HIPCHK(hipMemcpy(&ef, c->hI[i].eflag, sizeof(int), hipMemcpyDeviceToHost));
+    double acc[4];
+    HIPCHK(hipMemcpy(acc, c->hI[i].pst, sizeof(acc), hipMemcpyDeviceToHost));
+    ef_any |= ef != 0;
+    mn = std::min(mn, acc[0]);
+    mf = std::max(mf, (int)acc[1]);
+    viol |= acc[2] != 0.0;
+  }
+  if (ef_any) c->stats.warnings |= IMEX_WARN_EOS_MAXITER;
+  if (viol) c->stats.warnings |= IMEX_WARN_POSITIVITY;
+  c->stats.min_p = mn;
+  c->stats.max_floored = mf;
   *s = c->stats;
   s->div_b = imex_div_b(c);
   return 0;
@@ -1369,10 +1448,12 @@ int imex_reset_stats(imex_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->st));
   c->stats = imex_stats{};
   c->stats.min_p = 1e300;
-  double init[4] = {1e300, 0.0, 0.0, 0.0};
-  HIPCHK(hipMemcpy(c->pst, init, sizeof(init), hipMemcpyHostToDevice));
-  int zero = 0;
-  HIPCHK(hipMemcpy(c->eflag, &zero, sizeof(int), hipMemcpyHostToDevice));
+  const double init[4] = {1e300, 0.0, 0.0, 0.0};
+  const int zero = 0;
+  for (int i = 0; i < c->ni; i++) {
+    HIPCHK(hipMemcpy(c->hI[i].pst, init, sizeof(init), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->hI[i].eflag, &zero, sizeof(int), hipMemcpyHostToDevice));
+  }
   return 0;
 }
 
@@ -1385,55 +1466,61 @@ int imex_step(imex_ctx* c, int nsteps, imex_stats* stats) {
   return rc | c->stats.warnings;
 }
 
-double imex_div_b(imex_ctx* c) {
-  if (!c) return NAN;
-  const Dims& d = c->d;
-  int g = 256;
-  k_divb<<<g, 256, 0, c->st>>>(d, c->bx_n, c->by_n, c->partial);
-  std::vector<double> h(g);
-  if (hipMemcpyAsync(h.data(), c->partial, g * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return NAN;
-  if (hipStreamSynchronize(c->st) != hipSuccess) return NAN;
-  double s = 0; for (double v : h) s += v;
-  return sqrt(s);
+double imex_batch_div_b(imex_ctx* c, int i) {
+  if (!c || i < 0 || i >= c->ni) return NAN;
+  std::vector<double> v;
+  if (div_b_all(c, v)) return NAN;
+  return v[i];
 }
 
-// ---- stage-level drop-ins ----
+double imex_div_b(imex_ctx* c) {
+  if (!c) return NAN;
+  std::vector<double> v;
+  if (div_b_all(c, v)) return NAN;
+  return *std::max_element(v.begin(), v.end());
+}
+
+// ---- stage-level drop-ins (single-instance contexts) ----
+#define SINGLE(c) do { if (!(c)) return IMEX_ERR_ARG; if ((c)->ni != 1) return set_err(IMEX_ERR_UNSUPPORTED, "stage- and operator-level calls take a single-instance context"); } while (0)
+
 int imex_set_stage(imex_ctx* c, const imex_stage_host* st, const imex_scalars* sc) {
-  if (!c || !st || !sc) return set_err(IMEX_ERR_ARG, "null argument");
-  c->cur.u = c->U1; c->cur.Qe = c->Qe; c->cur.Qs = c->Qs;
-  c->cur.bxqs = c->bxqs; c->cur.byqs = c->byqs; c->cur.bxo = c->bx1; c->cur.byo = c->by1;
-  int rc = up_grid(c, c->U1, st->u_con);
-  if (!rc) rc = up_grid(c, c->Qe, st->Qe);
-  if (!rc) rc = up_grid(c, c->Qs, st->Qs);
-  if (!rc && st->bx) rc = up_plane(c, c->bx1, st->bx);
-  if (!rc && st->by) rc = up_plane(c, c->by1, st->by);
-  if (!rc && st->bx_qs) rc = up_plane(c, c->bxqs, st->bx_qs);
-  if (!rc && st->by_qs) rc = up_plane(c, c->byqs, st->by_qs);
-  if (!rc && st->fx) rc = up_plane(c, c->fx, st->fx);
-  if (!rc && st->fy) rc = up_plane(c, c->fy, st->fy);
+  SINGLE(c);
+  if (!st || !sc) return set_err(IMEX_ERR_ARG, "null argument");
+  c->cur.u = B_U1; c->cur.Qe = B_QE; c->cur.Qs = B_QS;
+  c->cur.bxqs = B_BXQS; c->cur.byqs = B_BYQS; c->cur.bxo = B_BX1; c->cur.byo = B_BY1;
+  int rc = up_grid(c, 0, B_U1, st->u_con);
+  if (!rc) rc = up_grid(c, 0, B_QE, st->Qe);
+  if (!rc) rc = up_grid(c, 0, B_QS, st->Qs);
+  if (!rc && st->bx) rc = up_plane(c, 0, B_BX1, st->bx);
+  if (!rc && st->by) rc = up_plane(c, 0, B_BY1, st->by);
+  if (!rc && st->bx_qs) rc = up_plane(c, 0, B_BXQS, st->bx_qs);
+  if (!rc && st->by_qs) rc = up_plane(c, 0, B_BYQS, st->by_qs);
+  if (!rc && st->fx) rc = up_plane(c, 0, B_FX, st->fx);
+  if (!rc && st->fy) rc = up_plane(c, 0, B_FY, st->fy);
   if (rc) return rc;
   std::memset(c->Sh, 0, S_NSLOTS * sizeof(double));
   scalars_to_dev(sc, c->Sh);
-  HIPCHK(hipMemcpy(c->S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->hI[0].S, c->Sh, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice));
   c->qe_ready = false; c->hhat_ready = false;
   return 0;
 }
 
 int imex_get_stage(imex_ctx* c, imex_stage_host* st, imex_scalars* sc) {
-  if (!c || !st) return set_err(IMEX_ERR_ARG, "null argument");
+  SINGLE(c);
+  if (!st) return set_err(IMEX_ERR_ARG, "null argument");
   HIPCHK(hipStreamSynchronize(c->st));
   int rc = 0;
-  if (st->u_con) rc = down_grid(c, c->cur.u, st->u_con);
-  if (!rc && st->Qe) rc = down_grid(c, c->cur.Qe, st->Qe);
-  if (!rc && st->Qs) rc = down_grid(c, c->cur.Qs, st->Qs);
-  if (!rc && st->bx) rc = down_plane(c, c->cur.bxo, st->bx);
-  if (!rc && st->by) rc = down_plane(c, c->cur.byo, st->by);
-  if (!rc && st->bx_qs) rc = down_plane(c, c->cur.bxqs, st->bx_qs);
-  if (!rc && st->by_qs) rc = down_plane(c, c->cur.byqs, st->by_qs);
-  if (!rc && st->fx) rc = down_plane(c, c->fx, st->fx);
-  if (!rc && st->fy) rc = down_plane(c, c->fy, st->fy);
+  if (st->u_con) rc = down_grid(c, 0, c->cur.u, st->u_con);
+  if (!rc && st->Qe) rc = down_grid(c, 0, c->cur.Qe, st->Qe);
+  if (!rc && st->Qs) rc = down_grid(c, 0, c->cur.Qs, st->Qs);
+  if (!rc && st->bx) rc = down_plane(c, 0, c->cur.bxo, st->bx);
+  if (!rc && st->by) rc = down_plane(c, 0, c->cur.byo, st->by);
+  if (!rc && st->bx_qs) rc = down_plane(c, 0, c->cur.bxqs, st->bx_qs);
+  if (!rc && st->by_qs) rc = down_plane(c, 0, c->cur.byqs, st->by_qs);
+  if (!rc && st->fx) rc = down_plane(c, 0, B_FX, st->fx);
+  if (!rc && st->fy) rc = down_plane(c, 0, B_FY, st->fy);
   if (rc) return rc;
-  if (sc) { if ((rc = pull_scalars(c))) return rc; scalars_from_dev(c->Sh, sc); }
+  if (sc) { if ((rc = pull_scalars(c, 0))) return rc; scalars_from_dev(c->Sh, sc); }
   return 0;
 }
 
@@ -1446,65 +1533,61 @@ static int finish(imex_ctx* c, int rc) {
 }
 
 int imex_cfl(imex_ctx* c, imex_scalars* sc) {
-  if (!c) return IMEX_ERR_ARG;
+  SINGLE(c);
   step_start(c);
   int rc = finish(c, 0);
   if (rc) return rc;
-  if (sc) { if ((rc = pull_scalars(c))) return rc; scalars_from_dev(c->Sh, sc); }
+  if (sc) { if ((rc = pull_scalars(c, 0))) return rc; scalars_from_dev(c->Sh, sc); }
   return 0;
 }
 
 int imex_stage_combine(imex_ctx* c, int rks) {
-  if (!c) return IMEX_ERR_ARG;
+  SINGLE(c);
   if (rks == 0) { bind_stage0(c); c->qe_ready = false; return 0; }
-  const Dims& d = c->d;
-  long long tot = (long long)d.NX * d.NY;
-  k_stage_combine<<<nb(tot), 256, 0, c->st>>>(d, c->u_n, c->U1, c->Qe, c->Qs, c->bx_n, c->bx1, c->by_n, c->by1, c->bxqs, c->byqs, c->S);
+  launch_stage_combine(c->L, c->iBxn, c->iByn);
   bind_stage1(c);
   c->qe_ready = false;
   return finish(c, 0);
 }
 
-int imex_explicit_conv(imex_ctx* c) { if (!c) return IMEX_ERR_ARG; ensure_qe(c); return finish(c, sub_explicit_conv(c)); }
-int imex_mag_update_b(imex_ctx* c) { if (!c) return IMEX_ERR_ARG; return finish(c, sub_mag_update_b(c)); }
-int imex_explicit_press(imex_ctx* c) { if (!c) return IMEX_ERR_ARG; return finish(c, sub_explicit_press(c)); }
-int imex_pre_subsystem_pressure(imex_ctx* c) { if (!c) return IMEX_ERR_ARG; return finish(c, sub_pressure(c)); }
-int imex_ct_update(imex_ctx* c) { if (!c) return IMEX_ERR_ARG; return finish(c, sub_ct(c)); }
+int imex_explicit_conv(imex_ctx* c) { SINGLE(c); ensure_qe(c); return finish(c, sub_explicit_conv(c)); }
+int imex_mag_update_b(imex_ctx* c) { SINGLE(c); return finish(c, sub_mag_update_b(c)); }
+int imex_explicit_press(imex_ctx* c) { SINGLE(c); return finish(c, sub_explicit_press(c)); }
+int imex_pre_subsystem_pressure(imex_ctx* c) { SINGLE(c); return finish(c, sub_pressure(c)); }
+int imex_ct_update(imex_ctx* c) { SINGLE(c); return finish(c, sub_ct(c)); }
 
 // ---- operator level ----
 static int prep_pressure_operator(imex_ctx* c, int op, const double* p_freeze, double mbig) {
   ensure_qe(c);
   ensure_hhat(c);
-  const Dims& d = c->d;
-  if (c->P.eos != IMEX_EOS_IDEAL) {
-    long long tot = (long long)d.NX * d.NY;
-    k_visc_rk<<<nb(tot), 256, 0, c->st>>>(d, c->P, c->pe, c->cur.u, c->visc, c->eflag);
-  }
-  // Tdiag via the first-pass RHS kernel (its b / p outputs go to scratch)
-  k_p_rhs<<<grid2(d.nx, d.ny), dim3(64, 4), 0, c->st>>>(d, 0, c->cur.u, c->hhat, c->mxs, c->mys, c->pe, c->tmpN, c->tmpN + d.N, c->Tdiag, c->S);
+  const Dims& d = c->d0();
+  if (c->P.eos != IMEX_EOS_IDEAL) launch_visc_rk(c->L, c->P, c->cur.u);
+  // Tdiag via the first-pass RHS kernel (its b and p outputs are overwritten by the caller)
+  launch_p_rhs(c->L, 0, c->cur.u);
   if (op == IMEX_OP_P_JACOBIAN) {
     if (!p_freeze) return set_err(IMEX_ERR_ARG, "p_freeze required");
-    HIPCHK(hipMemcpy(c->p, p_freeze, d.N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(c->hI[0].buf[B_P], p_freeze, d.N * sizeof(double), hipMemcpyHostToDevice));
     double Sm = mbig;
-    HIPCHK(hipMemcpy(c->S + S_MBIG, &Sm, sizeof(double), hipMemcpyHostToDevice));
-    if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(d, c->P, c->p, c->cur.u, c->S, c->vslope, c->eflag, c->st);
+    HIPCHK(hipMemcpy(c->hI[0].S + S_MBIG, &Sm, sizeof(double), hipMemcpyHostToDevice));
+    if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(c->L, c->P, B_P, c->cur.u);
   }
   return 0;
 }
 
 int imex_apply_op(imex_ctx* c, int op, const double* x, double* Ax, const double* p_freeze, double mbig) {
-  if (!c || !x || !Ax) return set_err(IMEX_ERR_ARG, "null argument");
-  const Dims& d = c->d;
-  int nc = op == IMEX_OP_MAG ? 3 : 1;
-  size_t n = (size_t)nc * d.N;
+  SINGLE(c);
+  if (!x || !Ax) return set_err(IMEX_ERR_ARG, "null argument");
+  const Dims& d = c->d0();
+  const int nc = op == IMEX_OP_MAG ? 3 : 1;
+  const size_t n = (size_t)nc * d.N;
   int rc = 0;
   if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
-  HIPCHK(hipMemcpy(c->tmpN + 3 * (size_t)d.N, x, n * sizeof(double), hipMemcpyHostToDevice));
-  MagOp mo = mag_op(c);
-  POp po = p_op(c, op == IMEX_OP_P_JACOBIAN, 0);
-  op_apply(op == IMEX_OP_MAG ? 0 : 1, mo, po, c->tmpN + 3 * (size_t)d.N, c->tmpN, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(c->hI[0].buf[B_XK], x, n * sizeof(double), hipMemcpyHostToDevice));
+  op_apply(c->L, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), p_ids(c, op == IMEX_OP_P_JACOBIAN, 0), c->P, B_XK, B_AV);
   if ((rc = finish(c, 0))) return rc;
-  HIPCHK(hipMemcpy(Ax, c->tmpN, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(Ax, c->hI[0].buf[B_AV], n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -1515,34 +1598,37 @@ int imex_apply_op_T(imex_ctx* c, int op, const double* x, double* ATx, const dou
 
 int imex_gmres(imex_ctx* c, int op, const double* b, double* x, int precond, const double* p_freeze, double mbig,
                imex_solve_info* info) {
-  if (!c || !b || !x) return set_err(IMEX_ERR_ARG, "null argument");
-  const Dims& d = c->d;
-  int nc = op == IMEX_OP_MAG ? 3 : 1;
-  size_t n = (size_t)nc * d.N;
+  SINGLE(c);
+  if (!b || !x) return set_err(IMEX_ERR_ARG, "null argument");
+  const Dims& d = c->d0();
+  const int nc = op == IMEX_OP_MAG ? 3 : 1;
+  const size_t n = (size_t)nc * d.N;
   int rc = 0;
   if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
-  HIPCHK(hipMemcpy(c->bk, b, n * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->xk, x, n * sizeof(double), hipMemcpyHostToDevice));
-  MagOp mo = mag_op(c);
-  POp po = p_op(c, op == IMEX_OP_P_JACOBIAN, precond && op != IMEX_OP_MAG);
-  rc = run_gmres(c, op == IMEX_OP_MAG ? 0 : 1, mo, po, c->xk, c->bk, 0, info, nullptr, nullptr);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(c->hI[0].buf[B_BK], b, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->hI[0].buf[B_XK], x, n * sizeof(double), hipMemcpyHostToDevice));
+  rc = run_gmres(c, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), p_ids(c, op == IMEX_OP_P_JACOBIAN, precond && op != IMEX_OP_MAG),
+                 B_XK, B_BK, 0, info, nullptr, nullptr);
   int r2 = finish(c, rc);
   if (r2 < 0) return r2;
-  HIPCHK(hipMemcpy(x, c->xk, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(x, c->hI[0].buf[B_XK], n * sizeof(double), hipMemcpyDeviceToHost));
   return rc;
 }
 
 int imex_pressure_newton(imex_ctx* c, const double* b, double* p, imex_solve_info* info) {
-  if (!c || !b || !p) return set_err(IMEX_ERR_ARG, "null argument");
-  const Dims& d = c->d;
+  SINGLE(c);
+  if (!b || !p) return set_err(IMEX_ERR_ARG, "null argument");
+  const Dims& d = c->d0();
   int rc = prep_pressure_operator(c, IMEX_OP_P_CARRIER, nullptr, 0.0);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(c->pb, b, d.N * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->p, p, d.N * sizeof(double), hipMemcpyHostToDevice));
-  rc = run_newton(c, c->pb, c->p, info);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(c->hI[0].buf[B_PB], b, d.N * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->hI[0].buf[B_P], p, d.N * sizeof(double), hipMemcpyHostToDevice));
+  rc = run_newton(c, B_PB, B_P, info);
   int r2 = finish(c, rc);
   if (r2 < 0) return r2;
-  HIPCHK(hipMemcpy(p, c->p, d.N * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(p, c->hI[0].buf[B_P], d.N * sizeof(double), hipMemcpyDeviceToHost));
   return rc;
 }
 
@@ -1566,6 +1652,87 @@ int imex_profile_read(imex_ctx* c, imex_kernel_profile* out, int max) {
     out[n].bytes = c->prof.bytes[k];
   }
   return n;
+}
+
+// ---- autotuner ----
+// Times the candidates of each tunable on the context's current state and keeps the fastest:
+// the explicit tile (16x16, 32x8, 64x4 cells; the same results bit for bit) and the grid cap of
+// the Krylov kernels (1024, 2048, 4096 blocks per instance; a different cap changes the order of
+// the dot-product sums, so results move by rounding).  The step's scalars and statistics are
+// snapshotted and restored; the state itself is only read.
+int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
+  if (!c) return IMEX_ERR_ARG;
+  if (c->capturing) return set_err(IMEX_ERR_ARG, "autotune inside a capture");
+  HIPCHK(hipStreamSynchronize(c->st));
+  imex_tune_info info;
+  std::memset(&info, 0, sizeof(info));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(IMEX_ERR_HIP, "event"); }
+  const Launch& L = c->L;
+  auto timed = [&](int reps, auto&& body) -> double {
+    body();                                   // warm-up
+    (void)hipEventRecord(e0, c->st);
+    for (int r = 0; r < reps; r++) body();
+    (void)hipEventRecord(e1, c->st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return 1000.0 * ms / reps;
+  };
+  int rc = 0;
+  k_budget_begin<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE]);
+  step_start(c);
+  bind_stage0(c);
+  c->qe_ready = false;
+  ensure_qe(c);
+  const int tiles[3] = {16, 32, 64};
+  const int caps[3] = {1024, 2048, 4096};
+  if (flags & IMEX_TUNE_EXPLICIT) {
+    int best = 1;
+    for (int q = 0; q < 3 && !rc; q++) {
+      c->ex_tile = tiles[q];
+      if ((rc = build_maps(c))) break;
+      info.explicit_us[q] = timed(5, [&] {
+        launch_explicit(L, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed, c->ex_tile);
+        launch_explicit(L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed, c->ex_tile);
+      });
+      if (info.explicit_us[q] < info.explicit_us[best]) best = q;
+    }
+    c->ex_tile = tiles[best];
+  }
+  if (!rc && (flags & IMEX_TUNE_KRYLOV)) {
+    const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
+    const int j = std::min(5, m - 1);
+    int best = 1;
+    launch_mag_rhs(L, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, B_BK, B_XK);
+    for (int q = 0; q < 3 && !rc; q++) {
+      c->cap_tiles = c->cap_vec = caps[q];
+      if ((rc = build_maps(c))) break;
+      gmres_residual(L, 0, c->mixed, mag_ids(c), p_ids(c, 0, 0), c->P, B_XK, B_BK, 0, 0, SKIP_NONE);
+      gmres_residual_fin(L, 0, c->mixed, 0, SKIP_NONE, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
+      info.krylov_us[q] = timed(10, [&] {
+        arnoldi_apply(L, 0, c->mixed, mag_ids(c), p_ids(c, 0, 0), c->P, j);
+        arnoldi_update(L, 0, c->mixed, j, m);
+      });
+      if (info.krylov_us[q] < info.krylov_us[best]) best = q;
+    }
+    c->cap_tiles = c->cap_vec = caps[best];
+  }
+  if (!rc) rc = build_maps(c);
+  k_budget_restore<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE]);
+  (void)hipStreamSynchronize(c->st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  drop_graphs(c);                 // captured launches carry the old grids
+  c->qe_ready = false; c->hhat_ready = false;
+  if (rc) return rc;
+  HIPCHK(hipGetLastError());
+  info.explicit_tile_x = c->ex_tile;
+  info.explicit_tile_y = 256 / c->ex_tile;
+  info.krylov_blocks = c->cap_tiles;
+  if (out) *out = info;
+  return 0;
 }
 
 }  // extern "C"

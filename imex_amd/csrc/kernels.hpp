@@ -1,13 +1,13 @@
 // kernels.hpp — internal declarations shared by the kernel files and the host driver.
+//
+// Every kernel takes the device instance table (`const Inst*`) and a block map (`BMap`) and
+// addresses its buffers by id (batch.hpp), so one launch covers every instance of a context.
 #pragma once
 #include "device_common.hpp"
+#include "batch.hpp"
 
-#define MAXM 32            // largest GMRES restart length supported
 #define TX 64              // operator tile (cells) in x
 #define TY 4               // operator tile (cells) in y
-#define MAXSWEEP 16        // refinement sweeps recorded per mixed-form solve
-
-struct PlaneSet { double* p[8]; int n; };
 
 // Device-resident GMRES / Newton control block.
 struct KrylovDev {
@@ -23,7 +23,8 @@ struct KrylovDev {
   unsigned cnt_a, cnt_c, cnt_r, cnt_n, cnt_m;
 };
 
-// Operator descriptors.  The magnetic operator has 3 unknowns per cell, the pressure operator 1.
+// Operator descriptors, built per instance inside the kernels.  The magnetic operator has 3
+// unknowns per cell, the pressure operator 1.
 struct MagOp {
   Dims d;
   const double* Qe;   // padded planes (B in 5..7)
@@ -41,67 +42,72 @@ struct POp {
   const double* S;
   int jacobian, precond;
 };
+// What the host passes: buffer ids and flags.
+struct MagIds { int Qe, u; };
+struct PIds { int u, jacobian, precond; };
 
-// ---- launchers (kernels_stage.hip) ----
-void launch_bc_fill(const Dims& d, const PlaneSet& ps, int kind, hipStream_t st);
+__device__ __forceinline__ MagOp make_mag_op(const Inst& I, const MagIds& m) {
+  MagOp o; o.d = I.d; o.Qe = I.buf[m.Qe]; o.u = I.buf[m.u]; o.S = I.S; return o;
+}
+__device__ __forceinline__ POp make_p_op(const Inst& I, const Phys& P, const PIds& q) {
+  POp o; o.d = I.d; o.P = P; o.hhat = I.buf[B_HHAT]; o.eta = I.buf[B_ETA]; o.u = I.buf[q.u];
+  const bool rk = P.eos != IMEX_EOS_IDEAL;
+  o.visc = rk ? I.buf[B_VISC] : nullptr; o.pfreeze = I.buf[B_P]; o.vslope = rk ? I.buf[B_VSLOPE] : nullptr;
+  o.Tdiag = I.buf[B_TDIAG]; o.S = I.S; o.jacobian = q.jacobian; o.precond = q.precond;
+  return o;
+}
 
-__global__ void k_step_start(Dims d, Phys P, int mode, const double* uin, double* un, double* partial, int* eflag);
-__global__ void k_step_finish(Dims d, Phys P, const double* partial, int nblk, double* S);
-__global__ void k_stage_combine(Dims d, const double* un, const double* u1, double* Qe, double* Qs,
-                                const double* bxn, const double* bx1, const double* byn, const double* by1,
-                                double* bxqs, double* byqs, const double* S);
-__global__ void k_qe_prep(Dims d, Phys P, const double* Qe, double* pe, double* cf, int* eflag);
-__global__ void k_shock_chi(Dims d, Phys P, const double* Qe, const double* cf, double* chi);
-__global__ void k_shock_eta(Dims d, const double* chi, double* eta);
-template <class T>
-__global__ void k_explicit(Dims d, Phys P, int kind, const double* Qe, const double* Qs, double* u, const double* S);
-__global__ void k_mag_rhs(Dims d, Phys P, const double* u, const double* Qe, const double* Qs, const double* pe,
-                          double* b, double* x0, double* hhat, const double* S, int* eflag);
-__global__ void k_mag_writeback(Dims d, const double* x, double* u);
-__global__ void k_emf_flux(Dims d, const double* u, const double* Qe, const double* pe, double* Fx, double* Fy, const double* S);
-__global__ void k_p_rhs(Dims d, int pass, const double* u, const double* hhat, double* mxs, double* mys,
-                        const double* pe, double* b, double* p0, double* Tdiag, const double* S);
-__global__ void k_p_momentum(Dims d, const double* p, const double* mxs, const double* mys, double* u, const double* S);
-__global__ void k_p_energy(Dims d, Phys P, const double* p, double* u, int* eflag);
-__global__ void k_ct_corner(Dims d, const double* ezx, const double* ezy, double sx, double sy, const double* u, double* Ezpp);
-__global__ void k_ct_faces(Dims d, const double* bxqs, const double* byqs, const double* Ezpp, double* bx, double* by,
-                           double* u, const double* S);
-__global__ void k_ct_plain_emf(Dims d, Phys P, const double* u, double* ezx, double* ezy);
-__global__ void k_con2pri(Dims d, Phys P, const double* con, double* pri, int* eflag);
-__global__ void k_pri2con(Dims d, Phys P, const double* pri, double* con, int* eflag);
-__global__ void k_divb(Dims d, const double* bx, const double* by, double* partial);
+// skip kinds: launches of a Jacobian solve whose Newton iteration has already converged
+enum { SKIP_NONE = 0, SKIP_NEWTON = 1 };
+__device__ __forceinline__ bool skipped(const Inst& I, int kind) { return kind == SKIP_NEWTON && I.K->newton_conv; }
 
-// ---- Krylov (kernels_krylov.hip) ----
-struct KrylovBufs {
-  double* w;           // unnormalised next basis vector / residual (FP64 form)
-  double* Av;          // preconditioned operator image of the current basis vector (FP64 form)
-  float* w32;          // the same two vectors for the FP32-stored inner solves of the mixed form
-  float* Av32;
-  double* partial;     // per-block partial sums, MAXM+2 per block
-  int nblk_apply, nblk_vec;
+// The host side of a launch: the instance table, the block map of the launch's shape and the
+// number of instances (grid = total blocks), and the stream.
+struct Launch {
+  const Inst* IT;
+  BMap bm[NSHAPE];
+  int nblocks[NSHAPE];
+  hipStream_t st;
 };
 
+// ---- stage kernels (kernels_stage.hip) ----
+// ghost-frame fill of n planes: plane k[p] of buffer b[p]; kind 0 cell, 1 x-normal faces, 2 y-normal faces
+void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind);
+void launch_step_start(const Launch& L, const Phys& P, int mode, int iin);
+void launch_stage_combine(const Launch& L, int ibxn, int ibyn);
+void launch_qe_prep(const Launch& L, const Phys& P, int iQe);
+void launch_shock(const Launch& L, const Phys& P, int iQe);
+// tile: width of the explicit tile (16, 32 or 64 cells; 256 cells per tile); SH_EXPL is mapped for it
+void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile);
+void launch_mag_rhs(const Launch& L, const Phys& P, int iu, int iQe, int iQs, int ib, int ix0);
+void launch_mag_writeback(const Launch& L, int iu);
+void launch_emf_flux(const Launch& L, int iu, int iQe);
+void launch_p_rhs(const Launch& L, int pass, int iu);
+void launch_p_momentum(const Launch& L, int iu);
+void launch_p_energy(const Launch& L, const Phys& P, int iu);
+void launch_ct(const Launch& L, const Phys& P, int plain, int iu, int ibxqs, int ibyqs, int ibx, int iby);
+void launch_con2pri(const Launch& L, const Phys& P, int icon, int ipri);
+void launch_pri2con(const Launch& L, const Phys& P, int ipri, int icon);
+void launch_divb(const Launch& L, int ibx, int iby);
+void launch_visc_rk(const Launch& L, const Phys& P, int iu);
+
+// ---- Krylov (kernels_krylov.hip) ----
 // op = 0 magnetic (3 unknowns per cell), 1 pressure (1 unknown per cell); prec = 0 FP64 basis and
 // operator, 1 FP32 basis and operator (inner solves of the mixed form; dots, norms, Givens and
 // the residual of the refinement stay FP64)
-void gmres_residual(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
-                    KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st);
-void gmres_residual_fin(const Dims& d, int prec, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip,
-                        double outer_tol, double inner_tol, hipStream_t st);
-void arnoldi_apply(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* V64,
-                   float* const* V32, int j, KrylovDev* K, hipStream_t st);
-void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st);
-void arnoldi_update(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, int j,
-                    int m, KrylovDev* K, hipStream_t st);
-void arnoldi_update_fin(int nc, int prec, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st);
-void gmres_xupdate(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, double* x,
-                   KrylovDev* K, hipStream_t st);
-void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* Ax, hipStream_t st);
-void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
-                     int it, double ntol, int* eflag, hipStream_t st);
-void newton_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int it, double ntol, hipStream_t st);
-void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u, double* S, double* partial, KrylovDev* K,
-                 int* eflag, hipStream_t st);
-void vslope_launch(const Dims& d, const Phys& P, const double* pf, const double* u, const double* S, double* vs, int* eflag,
-                   hipStream_t st);
-void axpy_launch(long long n, double* y, const double* x, const int* skip, hipStream_t st);
+void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int ix, int ib,
+                    int outer, int x_is_zero, int skip);
+void gmres_residual_fin(const Launch& L, int op, int prec, int outer, int skip, double outer_tol, double inner_tol);
+void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int j);
+void arnoldi_apply_fin(const Launch& L, int op, int j);
+void arnoldi_update(const Launch& L, int op, int prec, int j, int m);
+void arnoldi_update_fin(const Launch& L, int op, int prec, int j, int m);
+void gmres_xupdate(const Launch& L, int op, int prec, int ix);
+void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy);
+void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int ib, int it, double ntol);
+void newton_residual_fin(const Launch& L, int it, double ntol);
+void newton_mbig(const Launch& L, const Phys& P, int ib, int iu);
+void vslope_launch(const Launch& L, const Phys& P, int ipf, int iu);
+void axpy_launch(const Launch& L, int iy, int ix, int skip);
+void fill_launch(const Launch& L, int ibuf, double v, int skip);
+void pstats_launch(const Launch& L, int ip, double pfloor, double thr);

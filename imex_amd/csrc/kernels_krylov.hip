@@ -19,9 +19,19 @@
 #define IMEX_MAG_WAVES 3
 #endif
 
-// Krylov basis pointers of one storage type (double for the FP64 form, float for the inner
-// solves of the mixed form).
-template <class VT> struct VPtrs { VT* v[MAXM + 1]; };
+// Krylov vectors of one storage type (double for the FP64 form, float for the inner solves of
+// the mixed form): the basis V_i, the unnormalised next vector w and the operator image Av.
+template <class VT> struct KV;
+template <> struct KV<double> {
+  __device__ static double* V(const Inst& I, int i) { return I.buf[B_V0 + i]; }
+  __device__ static double* w(const Inst& I) { return I.buf[B_W]; }
+  __device__ static double* Av(const Inst& I) { return I.buf[B_AV]; }
+};
+template <> struct KV<float> {
+  __device__ static float* V(const Inst& I, int i) { return I.fbuf[F_V0 + i]; }
+  __device__ static float* w(const Inst& I) { return I.fbuf[F_W32]; }
+  __device__ static float* Av(const Inst& I) { return I.fbuf[F_AV32]; }
+};
 
 // compact-coordinate boundary map for one-cell halos (M_P with i in [g-1, n+g])
 __device__ __forceinline__ int cmap(int g, int n, int per) {
@@ -250,6 +260,8 @@ template <int OPK, class T> struct OpSel;
 template <class T> struct OpSel<0, T> {
   static constexpr int NC = 3;
   typedef MagOp Op;
+  typedef MagIds Ids;
+  __device__ static Op make(const Inst& I, const Phys&, const Ids& ids) { return make_mag_op(I, ids); }
   typedef MagSmem<T> Smem;
   template <class VT>
   __device__ static void load(const MagOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
@@ -265,6 +277,8 @@ template <class T> struct OpSel<0, T> {
 template <class T> struct OpSel<1, T> {
   static constexpr int NC = 1;
   typedef POp Op;
+  typedef PIds Ids;
+  __device__ static Op make(const Inst& I, const Phys& P, const Ids& ids) { return make_p_op(I, P, ids); }
   typedef PSmem<T> Smem;
   template <class VT>
   __device__ static void load(const POp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
@@ -333,10 +347,13 @@ __device__ __forceinline__ void final_sums(const double* partial, int stride, in
 // stored in the basis type VT, its norm is taken in FP64 before rounding.
 template <int OPK, class VT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
-k_gmres_residual(typename OpSel<OPK, double>::Op op, KrylovBufs kb, VT* __restrict__ w, const double* __restrict__ x,
-                 const double* __restrict__ b, KrylovDev* K, int outer, int x_is_zero, const int* skip, int mixed) {
-  if (skip && *skip) {           // the caller's outer iteration has already converged: no solve
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int ib,
+                 int outer, int x_is_zero, int skip, int mixed) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
+  if (skipped(I, skip)) {        // the caller's outer iteration has already converged: no solve
+    if (bx_ == 0 && threadIdx.x == 0) {
       K->done = 1; K->converged = 1; K->jlast = -1; K->steps = 0; K->outer = outer; K->outer_conv = 1;
       if (outer == 0) K->steps_total = 0;
     }
@@ -345,13 +362,17 @@ k_gmres_residual(typename OpSel<OPK, double>::Op op, KrylovBufs kb, VT* __restri
   if (mixed && outer > 0 && K->outer_conv) return;    // refinement finished in an earlier sweep
   typedef OpSel<OPK, double> S;
   constexpr int NC = S::NC;
+  const typename S::Op op = S::make(I, P, ids);
+  const double* __restrict__ x = I.buf[ix];
+  const double* __restrict__ b = I.buf[ib];
+  VT* __restrict__ w = KV<VT>::w(I);
   __shared__ typename S::Smem sm;
   __shared__ double red[8];
   __shared__ double res[2];
   const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   double acc[2] = {0.0, 0.0};     // ||M^-1 r||^2, and ||M^-1 b||^2 on the first sweep of the mixed form
-  for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+  for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
@@ -379,18 +400,24 @@ k_gmres_residual(typename OpSel<OPK, double>::Op op, KrylovBufs kb, VT* __restri
   }
   block_sums<2>(acc, 2, red, res);
   __syncthreads();
-  if (threadIdx.x < 2) kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = res[threadIdx.x];
+  if (threadIdx.x < 2) I.partial[bx_ * (MAXM + 2) + threadIdx.x] = res[threadIdx.x];
 }
+
+// local block count of instance i in the map of the kernel whose partials a finish reduces
+#define SRC_BLOCKS(src) ((src).pre[inst_ + 1] - (src).pre[inst_])
 
 // One-block finish of the residual norm: gam0, tolerance, convergence (implicit_operators.cpp:829-836).
 // Mixed form: the solve is converged when the FP64 residual is below outer_tol ||M^-1 b|| +
 // gmres_tol; otherwise the inner solve of this sweep runs to inner_tol relative to this residual.
-__global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partial, int nblk, KrylovDev* K, int outer,
-                                                            const int* skip, int mixed, double outer_tol, double inner_tol) {
-  if (skip && *skip) return;
+__global__ void __launch_bounds__(256) k_gmres_residual_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int outer,
+                                                            int skip, int mixed, double outer_tol, double inner_tol) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
+  if (skipped(I, skip)) return;
   if (mixed && outer > 0 && K->outer_conv) return;
   __shared__ double tot[2];
-  final_sums<2>(partial, MAXM + 2, nblk, 2, tot);
+  final_sums<2>(I.partial, MAXM + 2, SRC_BLOCKS(src), 2, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double g0 = sqrt(tot[0]);
@@ -419,11 +446,16 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const double* partia
 // accumulator array is live across the operator) and summed per wave in LDS, in tile order.
 template <int OPK, class T, class VT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
-k_arnoldi_apply(typename OpSel<OPK, T>::Op op, KrylovBufs kb, VT* __restrict__ w, VT* __restrict__ Av, VPtrs<VT> V,
-                int j, KrylovDev* K) {
+k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int j) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
   if (K->done) return;
   typedef OpSel<OPK, T> S;
   constexpr int NC = S::NC;
+  const typename S::Op op = S::make(I, P, ids);
+  const VT* __restrict__ w = KV<VT>::w(I);
+  VT* __restrict__ Av = KV<VT>::Av(I);
   __shared__ typename S::Smem sm;
   __shared__ double wsum[4][MAXM + 1];
   const Dims& d = S::dims(op);
@@ -431,8 +463,8 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Op op, KrylovBufs kb, VT* __restrict__ w
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lane; i <= j; i += 64) wsum[wid][i] = 0.0;
-  VT* vj = V.v[j];
-  for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+  VT* vj = KV<VT>::V(I, j);
+  for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
@@ -459,7 +491,7 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Op op, KrylovBufs kb, VT* __restrict__ w
     for (int i = 0; i <= j; i++) {
       double part = 0.0;
       if (valid) {
-        const VT* vi = V.v[i];
+        const VT* vi = KV<VT>::V(I, i);
         for (int c = 0; c < NC; c++) part += (i < j ? (double)vi[(long long)c * d.N + cidx] : vc[c]) * (double)y[c];
       }
       double s = wave_sum(part);
@@ -471,16 +503,19 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Op op, KrylovBufs kb, VT* __restrict__ w
     const int nw = blockDim.x >> 6;
     double s = 0.0;
     for (int ww = 0; ww < nw; ww++) s += wsum[ww][threadIdx.x];
-    kb.partial[blockIdx.x * (MAXM + 2) + threadIdx.x] = s;
+    I.partial[bx_ * (MAXM + 2) + threadIdx.x] = s;
   }
 }
 
 // One-block finish of the dots: H[i][j] for i <= j.
 template <int JB>
-__global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const double* partial, int nblk, int j, KrylovDev* K) {
+__global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
   if (K->done) return;
   __shared__ double tot[MAXM + 1];
-  final_sums<JB>(partial, MAXM + 2, nblk, j + 1, tot);
+  final_sums<JB>(I.partial, MAXM + 2, SRC_BLOCKS(src), j + 1, tot);
   __syncthreads();
   if ((int)threadIdx.x <= j) K->H[threadIdx.x * (MAXM + 1) + j] = tot[threadIdx.x];
 }
@@ -488,38 +523,47 @@ __global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const double* partial,
 // ---------------------------------------------------------------- Arnoldi step j, part 2:
 // w = Av_j - sum_i H[i][j] v_i (FP64 arithmetic, stored as VT), ||w|| of the stored w.
 template <int JB, class VT>
-__global__ void __launch_bounds__(256) k_arnoldi_update(KrylovBufs kb, VT* __restrict__ w, const VT* __restrict__ Av,
-                                                        VPtrs<VT> V, int j, int m, long long n, KrylovDev* K) {
+__global__ void __launch_bounds__(256) k_arnoldi_update(const Inst* __restrict__ IT, BMap bm, int j, int m, int nc) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
   if (K->done) return;
+  const long long n = (long long)nc * I.d.N;
+  VT* __restrict__ w = KV<VT>::w(I);
+  const VT* __restrict__ Av = KV<VT>::Av(I);
   __shared__ double Hc[JB];
   __shared__ double red[4];
   __shared__ double res[1];
   if ((int)threadIdx.x <= j) Hc[threadIdx.x] = K->H[threadIdx.x * (MAXM + 1) + j];
   __syncthreads();
   double hr[JB];
+  const VT* vp[JB];
 #pragma unroll
-  for (int i = 0; i < JB; i++) hr[i] = i <= j ? Hc[i] : 0.0;
+  for (int i = 0; i < JB; i++) { hr[i] = i <= j ? Hc[i] : 0.0; vp[i] = KV<VT>::V(I, i <= j ? i : 0); }
   double acc = 0.0;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+  for (long long e = bx_ * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gx_ * blockDim.x) {
     double wv = (double)Av[e];
 #pragma unroll
-    for (int i = 0; i < JB; i++) if (i <= j) wv -= hr[i] * (double)V.v[i][e];
+    for (int i = 0; i < JB; i++) if (i <= j) wv -= hr[i] * (double)vp[i][e];
     VT ws = (VT)wv;
     w[e] = ws;
     acc += (double)ws * (double)ws;
   }
   block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
+  if (threadIdx.x == 0) I.partial[bx_ * (MAXM + 2)] = res[0];
 }
 
 // One-block finish of the update: H[j+1][j] = ||w||, Givens rotations, convergence (implicit_operators.cpp:857-897).
 // In the mixed form an inner solve that reaches its tolerance ends the sweep but not the solve:
 // convergence is decided by the next FP64 residual.
-__global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partial, int nblk, int j, int m, KrylovDev* K,
+__global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j, int m,
                                                             int mixed) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
   if (K->done) return;
   __shared__ double tot[1];
-  final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
+  final_sums<1>(I.partial, MAXM + 2, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     const int L = MAXM + 1;
@@ -552,7 +596,10 @@ __global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const double* partia
 
 // ---------------------------------------------------------------- x += sum alpha_i v_i (back substitution per block)
 template <class VT>
-__global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs<VT> V, double* __restrict__ x, long long n, KrylovDev* K) {
+__global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ IT, BMap bm, int ix, int nc) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
   __shared__ double alpha[MAXM + 1];
   const int jl = K->jlast;
   if (jl < 0) return;
@@ -565,10 +612,11 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs<VT> V, double* __re
     }
   }
   __syncthreads();
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+  double* __restrict__ x = I.buf[ix];
+  const long long n = (long long)nc * I.d.N;
+  for (long long e = bx_ * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gx_ * blockDim.x) {
     double xv = x[e];
-#pragma unroll
-    for (int i = 0; i <= MAXM; i++) if (i <= jl) xv += alpha[i] * (double)V.v[i][e];
+    for (int i = 0; i <= jl; i++) xv += alpha[i] * (double)KV<VT>::V(I, i)[e];
     x[e] = xv;
   }
 }
@@ -576,13 +624,18 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(VPtrs<VT> V, double* __re
 // ---------------------------------------------------------------- plain operator application (tests, adjoint)
 template <int OPK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
-k_op_apply(typename OpSel<OPK, double>::Op op, const double* __restrict__ x, double* __restrict__ Ax) {
+k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int iy) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
   typedef OpSel<OPK, double> S;
   constexpr int NC = S::NC;
+  const typename S::Op op = S::make(I, P, ids);
+  const double* __restrict__ x = I.buf[ix];
+  double* __restrict__ Ax = I.buf[iy];
   __shared__ typename S::Smem sm;
   const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
-  for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+  for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     int cx = gx0 + tx, cy = gy0 + ty;
@@ -599,21 +652,25 @@ k_op_apply(typename OpSel<OPK, double>::Op op, const double* __restrict__ x, dou
 
 // ---------------------------------------------------------------- Newton residual (implicit_operators.cpp:690-710)
 // g = L p - a p + V_mod(p, rho) - b with L the carrier operator; writes -g and ||g||.
-__global__ void __launch_bounds__(256) k_newton_residual(POp op, KrylovBufs kb, const double* __restrict__ p,
-                                                         const double* __restrict__ b, double* __restrict__ ng,
-                                                         KrylovDev* K, int it, double ntol, int* eflag, const int* skip) {
-  if (skip && *skip) return;     // an earlier iteration of this Newton solve has converged
+__global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ip,
+                                                         int ib, int it, double ntol) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  if (it > 0 && I.K->newton_conv) return;     // an earlier iteration of this Newton solve has converged
+  const POp op = make_p_op(I, P, ids);
+  const double* __restrict__ p = I.buf[ip];
+  const double* __restrict__ b = I.buf[ib];
+  double* __restrict__ ng = I.buf[B_NG];
   __shared__ double xs[(TY + 2) * (TX + 2)];
   __shared__ double red[4];
   __shared__ double res[1];
   const Dims& d = op.d;
-  const Phys& P = op.P;
   const double a = 1.0 / (P.gamma - 1.0);
   const double Mbig = op.S[S_MBIG];
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   double acc = 0.0;
   int ef = 0;
-  for (int tile = blockIdx.x; tile < ntx * nty; tile += gridDim.x) {
+  for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     __syncthreads();
     load_tile<1, double, double>(xs, d, p, 1.0, false, gx0, gy0);
@@ -631,15 +688,18 @@ __global__ void __launch_bounds__(256) k_newton_residual(POp op, KrylovBufs kb, 
       acc += g * g;
     }
   }
-  if (ef) atomicOr(eflag, 1);
+  if (ef) atomicOr(I.eflag, 1);
   block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) kb.partial[blockIdx.x * (MAXM + 2)] = res[0];
+  if (threadIdx.x == 0) I.partial[bx_ * (MAXM + 2)] = res[0];
 }
 
-__global__ void __launch_bounds__(256) k_newton_fin(const double* partial, int nblk, KrylovDev* K, int it, double ntol) {
+__global__ void __launch_bounds__(256) k_newton_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int it, double ntol) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  KrylovDev* K = I.K;
   if (it > 0 && K->newton_conv) return;
   __shared__ double tot[1];
-  final_sums<1>(partial, MAXM + 2, nblk, 1, tot);
+  final_sums<1>(I.partial, MAXM + 2, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = sqrt(tot[0]);
@@ -652,12 +712,16 @@ __global__ void __launch_bounds__(256) k_newton_fin(const double* partial, int n
 }
 
 // Mbig from Cor. 1 (implicit_operators.cpp:670-686): min(b) and, for RK, max rho e(rho, p_floor).
-__global__ void __launch_bounds__(256) k_newton_mbig(Dims d, Phys P, const double* __restrict__ b, const double* __restrict__ u,
-                                                     double* S, double* partial, KrylovDev* K, int* eflag) {
+__global__ void __launch_bounds__(256) k_newton_mbig(Phys P, const Inst* __restrict__ IT, BMap bm, int ib, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  const Dims& d = I.d;
+  const double* __restrict__ b = I.buf[ib];
+  const double* __restrict__ u = I.buf[iu];
   __shared__ double sh[8];
   double bmin = 1e300, vmax = -1e300;
   int ef = 0;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < d.N; e += (long long)gridDim.x * blockDim.x) {
+  for (long long e = bx_ * (long long)blockDim.x + threadIdx.x; e < d.N; e += (long long)gx_ * blockDim.x) {
     bmin = fmin(bmin, b[e]);
     if (P.eos != IMEX_EOS_IDEAL) {
       int i = (int)(e % d.nx) + GC, j = (int)(e / d.nx) + GC;
@@ -665,7 +729,7 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Dims d, Phys P, const doubl
       if (isfinite(v)) vmax = fmax(vmax, v);
     }
   }
-  if (ef) atomicOr(eflag, 1);
+  if (ef) atomicOr(I.eflag, 1);
   bmin = wave_min(bmin); vmax = wave_max(vmax);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) { sh[threadIdx.x >> 6] = bmin; sh[4 + (threadIdx.x >> 6)] = vmax; }
@@ -673,148 +737,185 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Dims d, Phys P, const doubl
   if (threadIdx.x == 0) {
     double a = sh[0], c = sh[4];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sh[w]); c = fmax(c, sh[4 + w]); }
-    partial[blockIdx.x * 2] = a; partial[blockIdx.x * 2 + 1] = c;
+    I.partial[bx_ * 2] = a; I.partial[bx_ * 2 + 1] = c;
   }
 }
 
-__global__ void k_newton_mbig_fin(Phys P, const double* partial, int nblk, double* S) {
+__global__ void k_newton_mbig_fin(Phys P, const Inst* __restrict__ IT, BMap bm, BMap src) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  const int nblk = SRC_BLOCKS(src);
   // min and max are order independent: one wave reduces the block partials
   double a = 1e300, c = -1e300;
-  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, partial[bl * 2]); c = fmax(c, partial[bl * 2 + 1]); }
+  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, I.partial[bl * 2]); c = fmax(c, I.partial[bl * 2 + 1]); }
   a = wave_min(a); c = wave_max(c);
   if (threadIdx.x == 0) {
     double Vpe = P.p_floor / (P.gamma - 1.0);
     if (P.eos != IMEX_EOS_IDEAL && c > Vpe) Vpe = c;
     double M_cor = fmax(0.0, Vpe - a) / ((1.0 - P.theta) * P.p_floor);
     double M_base = P.mbig_fac / (P.gamma - 1.0);
-    S[S_MBIG] = fmax(M_base, M_cor);
+    I.S[S_MBIG] = fmax(M_base, M_cor);
   }
 }
 
-__global__ void k_vslope(Dims d, Phys P, const double* __restrict__ pf, const double* __restrict__ u, const double* S,
-                         double* __restrict__ vs, int* eflag) {
-  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__global__ void k_vslope(Phys P, const Inst* __restrict__ IT, BMap bm, int ipf, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  const Dims& d = I.d;
+  long long e = bx_ * (long long)blockDim.x + threadIdx.x;
   if (e >= d.N) return;
   int i = (int)(e % d.nx) + GC, j = (int)(e / d.nx) + GC;
   int ef = 0;
-  vs[e] = V_slope(P, pf[e], u[pidx(d, i, j)], S[S_MBIG], &ef);
-  if (ef) atomicOr(eflag, 1);
+  I.buf[B_VSLOPE][e] = V_slope(P, I.buf[ipf][e], I.buf[iu][pidx(d, i, j)], I.S[S_MBIG], &ef);
+  if (ef) atomicOr(I.eflag, 1);
 }
 
-__global__ void k_axpy1(long long n, double* __restrict__ y, const double* __restrict__ x, const int* skip) {
-  if (skip && *skip) return;
-  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e < n) y[e] += x[e];
+__global__ void k_axpy1(const Inst* __restrict__ IT, BMap bm, int iy, int ix, int skip) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  if (skipped(I, skip)) return;
+  long long e = bx_ * (long long)blockDim.x + threadIdx.x;
+  if (e < I.d.N) I.buf[iy][e] += I.buf[ix][e];
+}
+
+__global__ void k_fill_n(const Inst* __restrict__ IT, BMap bm, int ib, double v, int skip) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  if (skipped(I, skip)) return;
+  long long e = bx_ * (long long)blockDim.x + threadIdx.x;
+  if (e < I.d.N) I.buf[ib][e] = v;
+}
+
+// Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
+// device: pst[0] = min p ever, pst[1] = max floored count, pst[2] = 1 if min p < theta p_floor - 1e-14.
+__global__ void k_pstats(const Inst* __restrict__ IT, BMap bm, int ip, double pfloor) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  const double* __restrict__ p = I.buf[ip];
+  const long long n = I.d.N;
+  __shared__ double smin[4], scnt[4];
+  double mn = 1e300, cnt = 0;
+  for (long long e = bx_ * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gx_ * blockDim.x) {
+    double v = p[e];
+    mn = fmin(mn, v);
+    if (v < pfloor) cnt += 1;
+  }
+  mn = wave_min(mn); cnt = wave_sum(cnt);
+  if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = mn; scnt[threadIdx.x >> 6] = cnt; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mn = fmin(mn, smin[w]); cnt += scnt[w]; }
+    I.partial[2 * bx_] = smin[0] < mn ? smin[0] : mn;
+    I.partial[2 * bx_ + 1] = cnt;
+  }
+}
+__global__ void k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, double thr) {
+  BATCH_BLOCK(bm);
+  const Inst& I = IT[inst_];
+  const int nblk = SRC_BLOCKS(src);
+  // min, and a sum of small integers held in doubles: both exact in any order, one wave reduces
+  double m = 1e300, c = 0;
+  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, I.partial[2 * b]); c += I.partial[2 * b + 1]; }
+  m = wave_min(m); c = wave_sum(c);
+  if (threadIdx.x == 0) {
+    double* acc = I.pst;
+    acc[0] = fmin(acc[0], m);
+    acc[1] = fmax(acc[1], c);
+    if (m < thr) acc[2] = 1.0;
+  }
 }
 
 // ================================================================ launchers
-static inline int grid_for_tiles(const Dims& d, int cap) {
-  int nt = ((d.nx + TX - 1) / TX) * ((d.ny + TY - 1) / TY);
-  return nt < cap ? nt : cap;
-}
-template <class VT>
-static VPtrs<VT> make_vptrs(VT* const* hostV) {
-  VPtrs<VT> V;
-  for (int i = 0; i <= MAXM; i++) V.v[i] = hostV[i];
-  return V;
-}
+#define GRID(L, sh) dim3((unsigned)(L).nblocks[sh])
+static inline int tiles_shape(int) { return SH_TILES; }
+static inline int vec_shape(int op) { return op == 0 ? SH_VEC3 : SH_VEC1; }
 
-void gmres_residual(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, const double* x, const double* b,
-                    KrylovDev* K, int outer, int x_is_zero, const int* skip, hipStream_t st) {
-  const Dims& d = op == 0 ? mo.d : po.d;
-  int g = grid_for_tiles(d, kb.nblk_apply);
+void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int ix, int ib,
+                    int outer, int x_is_zero, int skip) {
+  const BMap& bm = L.bm[SH_TILES];
   if (prec == 0) {
-    if (op == 0) k_gmres_residual<0, double><<<g, 256, 0, st>>>(mo, kb, kb.w, x, b, K, outer, x_is_zero, skip, 0);
-    else k_gmres_residual<1, double><<<g, 256, 0, st>>>(po, kb, kb.w, x, b, K, outer, x_is_zero, skip, 0);
+    if (op == 0) k_gmres_residual<0, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    else k_gmres_residual<1, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
   } else {
-    if (op == 0) k_gmres_residual<0, float><<<g, 256, 0, st>>>(mo, kb, kb.w32, x, b, K, outer, x_is_zero, skip, 1);
-    else k_gmres_residual<1, float><<<g, 256, 0, st>>>(po, kb, kb.w32, x, b, K, outer, x_is_zero, skip, 1);
+    if (op == 0) k_gmres_residual<0, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    else k_gmres_residual<1, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
   }
 }
-void gmres_residual_fin(const Dims& d, int prec, const KrylovBufs& kb, KrylovDev* K, int outer, const int* skip,
-                        double outer_tol, double inner_tol, hipStream_t st) {
-  k_gmres_residual_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, outer, skip, prec, outer_tol, inner_tol);
+void gmres_residual_fin(const Launch& L, int, int prec, int outer, int skip, double outer_tol, double inner_tol) {
+  k_gmres_residual_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], outer, skip, prec, outer_tol, inner_tol);
 }
 
-void arnoldi_apply(int op, int prec, const MagOp& mo, const POp& po, const KrylovBufs& kb, double* const* V64,
-                   float* const* V32, int j, KrylovDev* K, hipStream_t st) {
-  const Dims& d = op == 0 ? mo.d : po.d;
-  int g = grid_for_tiles(d, kb.nblk_apply);
+void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int j) {
+  const BMap& bm = L.bm[SH_TILES];
   if (prec == 0) {
-    VPtrs<double> V = make_vptrs(V64);
-    if (op == 0) k_arnoldi_apply<0, double, double><<<g, 256, 0, st>>>(mo, kb, kb.w, kb.Av, V, j, K);
-    else k_arnoldi_apply<1, double, double><<<g, 256, 0, st>>>(po, kb, kb.w, kb.Av, V, j, K);
+    if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else k_arnoldi_apply<1, double, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   } else {
-    VPtrs<float> V = make_vptrs(V32);
-    if (op == 0) k_arnoldi_apply<0, float, float><<<g, 256, 0, st>>>(mo, kb, kb.w32, kb.Av32, V, j, K);
-    else k_arnoldi_apply<1, float, float><<<g, 256, 0, st>>>(po, kb, kb.w32, kb.Av32, V, j, K);
+    if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else k_arnoldi_apply<1, float, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   }
 }
-void arnoldi_apply_fin(const Dims& d, const KrylovBufs& kb, int j, KrylovDev* K, hipStream_t st) {
-  int g = grid_for_tiles(d, kb.nblk_apply);
-  if (j < 8) k_arnoldi_dots_fin<8><<<1, 256, 0, st>>>(kb.partial, g, j, K);
-  else if (j < 16) k_arnoldi_dots_fin<16><<<1, 256, 0, st>>>(kb.partial, g, j, K);
-  else k_arnoldi_dots_fin<32><<<1, 256, 0, st>>>(kb.partial, g, j, K);
+void arnoldi_apply_fin(const Launch& L, int, int j) {
+  const BMap& one = L.bm[SH_ONE];
+  const BMap& src = L.bm[SH_TILES];
+  if (j < 8) k_arnoldi_dots_fin<8><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
+  else if (j < 16) k_arnoldi_dots_fin<16><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
+  else k_arnoldi_dots_fin<32><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
 }
 
-static inline int vec_grid(int nc, const Dims& d, const KrylovBufs& kb) {
-  long long n = (long long)nc * d.N;
-  long long nb = (n + 255) / 256;
-  return (int)(nb < kb.nblk_vec ? nb : kb.nblk_vec);
-}
 template <class VT>
-static void launch_update(const KrylovBufs& kb, VT* w, const VT* Av, const VPtrs<VT>& V, int j, int m, long long n,
-                          KrylovDev* K, int g, hipStream_t st) {
-  if (j < 8) k_arnoldi_update<8, VT><<<g, 256, 0, st>>>(kb, w, Av, V, j, m, n, K);
-  else if (j < 16) k_arnoldi_update<16, VT><<<g, 256, 0, st>>>(kb, w, Av, V, j, m, n, K);
-  else k_arnoldi_update<32, VT><<<g, 256, 0, st>>>(kb, w, Av, V, j, m, n, K);
+static void launch_update(const Launch& L, int sh, int j, int m, int nc) {
+  const BMap& bm = L.bm[sh];
+  if (j < 8) k_arnoldi_update<8, VT><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, bm, j, m, nc);
+  else if (j < 16) k_arnoldi_update<16, VT><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, bm, j, m, nc);
+  else k_arnoldi_update<32, VT><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, bm, j, m, nc);
 }
-void arnoldi_update(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, int j,
-                    int m, KrylovDev* K, hipStream_t st) {
-  long long n = (long long)nc * d.N;
-  int g = vec_grid(nc, d, kb);
-  if (prec == 0) launch_update(kb, kb.w, (const double*)kb.Av, make_vptrs(V64), j, m, n, K, g, st);
-  else launch_update(kb, kb.w32, (const float*)kb.Av32, make_vptrs(V32), j, m, n, K, g, st);
+void arnoldi_update(const Launch& L, int op, int prec, int j, int m) {
+  const int nc = op == 0 ? 3 : 1;
+  if (prec == 0) launch_update<double>(L, vec_shape(op), j, m, nc);
+  else launch_update<float>(L, vec_shape(op), j, m, nc);
 }
-void arnoldi_update_fin(int nc, int prec, const Dims& d, const KrylovBufs& kb, int j, int m, KrylovDev* K, hipStream_t st) {
-  k_arnoldi_givens_fin<<<1, 256, 0, st>>>(kb.partial, vec_grid(nc, d, kb), j, m, K, prec);
+void arnoldi_update_fin(const Launch& L, int op, int prec, int j, int m) {
+  k_arnoldi_givens_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[vec_shape(op)], j, m, prec);
 }
 
-void gmres_xupdate(int nc, int prec, const Dims& d, const KrylovBufs& kb, double* const* V64, float* const* V32, double* x,
-                   KrylovDev* K, hipStream_t st) {
-  long long n = (long long)nc * d.N;
-  int g = vec_grid(nc, d, kb);
-  if (prec == 0) k_gmres_xupdate<double><<<g, 256, 0, st>>>(make_vptrs(V64), x, n, K);
-  else k_gmres_xupdate<float><<<g, 256, 0, st>>>(make_vptrs(V32), x, n, K);
+void gmres_xupdate(const Launch& L, int op, int prec, int ix) {
+  const int nc = op == 0 ? 3 : 1;
+  const int sh = vec_shape(op);
+  if (prec == 0) k_gmres_xupdate<double><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], ix, nc);
+  else k_gmres_xupdate<float><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], ix, nc);
 }
 
-void op_apply(int op, const MagOp& mo, const POp& po, const double* x, double* Ax, hipStream_t st) {
-  if (op == 0) k_op_apply<0><<<grid_for_tiles(mo.d, 4096), 256, 0, st>>>(mo, x, Ax);
-  else k_op_apply<1><<<grid_for_tiles(po.d, 4096), 256, 0, st>>>(po, x, Ax);
+void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy) {
+  if (op == 0) k_op_apply<0><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
+  else k_op_apply<1><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
 }
 
-void newton_residual(const POp& po, const KrylovBufs& kb, const double* p, const double* b, double* ng, KrylovDev* K,
-                     int it, double ntol, int* eflag, hipStream_t st) {
-  int g = grid_for_tiles(po.d, kb.nblk_apply);
-  k_newton_residual<<<g, 256, 0, st>>>(po, kb, p, b, ng, K, it, ntol, eflag, it > 0 ? &K->newton_conv : nullptr);
+void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int ib, int it, double ntol) {
+  k_newton_residual<<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
 }
-void newton_residual_fin(const Dims& d, const KrylovBufs& kb, KrylovDev* K, int it, double ntol, hipStream_t st) {
-  k_newton_fin<<<1, 256, 0, st>>>(kb.partial, grid_for_tiles(d, kb.nblk_apply), K, it, ntol);
+void newton_residual_fin(const Launch& L, int it, double ntol) {
+  k_newton_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], it, ntol);
 }
 
-void newton_mbig(const Dims& d, const Phys& P, const double* b, const double* u, double* S, double* partial, KrylovDev* K,
-                 int* eflag, hipStream_t st) {
-  long long nb = (d.N + 255) / 256;
-  int g = (int)(nb < 512 ? nb : 512);
-  k_newton_mbig<<<g, 256, 0, st>>>(d, P, b, u, S, partial, K, eflag);
-  k_newton_mbig_fin<<<1, 64, 0, st>>>(P, partial, g, S);
+void newton_mbig(const Launch& L, const Phys& P, int ib, int iu) {
+  k_newton_mbig<<<GRID(L, SH_MBIG), 256, 0, L.st>>>(P, L.IT, L.bm[SH_MBIG], ib, iu);
+  k_newton_mbig_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_MBIG]);
 }
 
-void vslope_launch(const Dims& d, const Phys& P, const double* pf, const double* u, const double* S, double* vs, int* eflag,
-                   hipStream_t st) {
-  k_vslope<<<(d.N + 255) / 256, 256, 0, st>>>(d, P, pf, u, S, vs, eflag);
+void vslope_launch(const Launch& L, const Phys& P, int ipf, int iu) {
+  k_vslope<<<GRID(L, SH_LIN_N), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_N], ipf, iu);
 }
 
-void axpy_launch(long long n, double* y, const double* x, const int* skip, hipStream_t st) {
-  k_axpy1<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, y, x, skip);
+void axpy_launch(const Launch& L, int iy, int ix, int skip) {
+  k_axpy1<<<GRID(L, SH_LIN_N), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_N], iy, ix, skip);
+}
+
+void fill_launch(const Launch& L, int ib, double v, int skip) {
+  k_fill_n<<<GRID(L, SH_LIN_N), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_N], ib, v, skip);
+}
+
+void pstats_launch(const Launch& L, int ip, double pfloor, double thr) {
+  k_pstats<<<GRID(L, SH_PSTATS), 256, 0, L.st>>>(L.IT, L.bm[SH_PSTATS], ip, pfloor);
+  k_pstats_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_PSTATS], thr);
 }

@@ -12,6 +12,7 @@
 // bound_cond_u / bound_cond_array (boundary.cpp:5-175, timestep.cpp:166-222) and
 // bound_cond_staggered (constrained_transport.cpp:11-64): ghost (i,j) <- (mx(i), my(j)).
 struct FillMap { int lox, hix, loy, hiy; };   // identity range of the map
+struct FillPlanes { int b[8], k[8], n; };      // planes: buffer id b[p], plane k[p] of that buffer
 
 __device__ __forceinline__ int fill_mx(const Dims& d, int kind, int i) {
   if (d.perx) return map_x(d, i);
@@ -28,8 +29,21 @@ __device__ __forceinline__ int fill_my(const Dims& d, int kind, int j) {
   return j;
 }
 
-__global__ void k_bc_fill(Dims d, PlaneSet ps, int kind, FillMap fm, int nbot, int ntop, int nside_l, int nside_r, int total) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_bc_fill(const Inst* __restrict__ IT, BMap bm, FillPlanes fp, int kind) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  FillMap fm;
+  fm.lox = (!d.perx && kind == 1) ? GC - 1 : GC;
+  fm.hix = d.nx + GC - 1;
+  fm.loy = (!d.pery && kind == 2) ? GC - 1 : GC;
+  fm.hiy = d.ny + GC - 1;
+  const int nbot = fm.loy * d.NX;
+  const int ntop = (d.NY - 1 - fm.hiy) * d.NX;
+  const int nside_l = fm.lox, nside_r = d.NX - 1 - fm.hix;
+  const int total = nbot + ntop + (fm.hiy - fm.loy + 1) * (nside_l + nside_r);
+
+  int t = bx_ * blockDim.x + threadIdx.x;
   if (t >= total) return;
   int i, j;
   int rows_mid = fm.hiy - fm.loy + 1;
@@ -45,35 +59,32 @@ __global__ void k_bc_fill(Dims d, PlaneSet ps, int kind, FillMap fm, int nbot, i
   }
   int si = fill_mx(d, kind, i), sj = fill_my(d, kind, j);
   long long dst = pidx(d, i, j), src = pidx(d, si, sj);
-  for (int p = 0; p < ps.n; p++) ps.p[p][dst] = ps.p[p][src];
+  for (int p = 0; p < fp.n; p++) {
+    double* a = I_.buf[fp.b[p]] + fp.k[p] * d.PS;
+    a[dst] = a[src];
+  }
 }
 
-void launch_bc_fill(const Dims& d, const PlaneSet& ps, int kind, hipStream_t st) {
-  FillMap fm;
-  fm.lox = (!d.perx && kind == 1) ? GC - 1 : GC;
-  fm.hix = d.nx + GC - 1;
-  fm.loy = (!d.pery && kind == 2) ? GC - 1 : GC;
-  fm.hiy = d.ny + GC - 1;
-  int nbot = fm.loy * d.NX;
-  int ntop = (d.NY - 1 - fm.hiy) * d.NX;
-  int nl = fm.lox, nr = d.NX - 1 - fm.hix;
-  int rows_mid = fm.hiy - fm.loy + 1;
-  int total = nbot + ntop + rows_mid * (nl + nr);
-  int bs = 256;
-  k_bc_fill<<<(total + bs - 1) / bs, bs, 0, st>>>(d, ps, kind, fm, nbot, ntop, nl, nr, total);
-}
+
 
 // ================================================================ start of step
 // calch + caldt + MagneticEigenvalues (timestep.cpp:5-67, 89-120, 254-273) fused with
 // the con -> pri -> con round trip of main.cpp:356/561: u_n = pri2con(pri).
 // mode 0: `uin` holds primitives; mode 1: `uin` holds conservatives (pri = con2pri).
-__global__ void __launch_bounds__(256) k_step_start(Dims d, Phys P, int mode, const double* __restrict__ uin,
-                                                    double* __restrict__ un, double* partial, int* eflag) {
+__global__ void __launch_bounds__(256) k_step_start(Phys P, const Inst* __restrict__ IT, BMap bm, int mode, int iin) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ uin = I_.buf[iin];
+  double* __restrict__ un = I_.buf[B_UN];
+  double* partial = I_.partial;
+  int* eflag = I_.eflag;
+
   __shared__ double sh[16];
   double r[8] = {1e-8, 1e-8, 1e-8, 0.0, 0.0, 0.0, 0.0, 0.0};
   const long long total = (long long)d.NX * d.NY;
   int ef = 0;
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < total; c += (long long)gridDim.x * blockDim.x) {
+  for (long long c = bx_ * (long long)blockDim.x + threadIdx.x; c < total; c += (long long)gx_ * blockDim.x) {
     int i = (int)(c % d.NX), j = (int)(c / d.NX);
     double q[NVAR], pri[NVAR];
     for (int k = 0; k < NVAR; k++) q[k] = uin[k * d.PS + c];
@@ -111,13 +122,20 @@ __global__ void __launch_bounds__(256) k_step_start(Dims d, Phys P, int mode, co
     if (threadIdx.x == 0) {
       double m = sh[0];
       for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
-      partial[blockIdx.x * 8 + k] = m;
+      partial[bx_ * 8 + k] = m;
     }
   }
 }
 
 // One-block finish of the start-of-step maxima and the dt rule (timestep.cpp:22-66, main.cpp:341, 353).
-__global__ void k_step_finish(Dims d, Phys P, const double* partial, int nblk, double* S) {
+__global__ void k_step_finish(Phys P, const Inst* __restrict__ IT, BMap bm, BMap src) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* partial = I_.partial;
+  const int nblk = src.pre[inst_ + 1] - src.pre[inst_];
+  double* S = I_.S;
+
   __shared__ double mm[8];
   {   // 8 maxima over the block partials, one wave per value (max is order independent)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -155,13 +173,24 @@ __global__ void k_step_finish(Dims d, Phys P, const double* partial, int nblk, d
 
 // ================================================================ stage combine (main.cpp:368-414)
 // Stage 1 of IMEX-RK2: Hs0 = (u1 - u_n)/adt0, Qe = u_n + Ae10 dt Hs0, Qs = u_n + Ai10 dt Hs0.
-__global__ void k_stage_combine(Dims d, const double* __restrict__ un, const double* __restrict__ u1,
-                                double* __restrict__ Qe, double* __restrict__ Qs,
-                                const double* __restrict__ bxn, const double* __restrict__ bx1,
-                                const double* __restrict__ byn, const double* __restrict__ by1,
-                                double* __restrict__ bxqs, double* __restrict__ byqs, const double* S) {
+__global__ void k_stage_combine(const Inst* __restrict__ IT, BMap bm, int ibxn, int ibyn) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ un = I_.buf[B_UN];
+  const double* __restrict__ u1 = I_.buf[B_U1];
+  double* __restrict__ Qe = I_.buf[B_QE];
+  double* __restrict__ Qs = I_.buf[B_QS];
+  const double* __restrict__ bxn = I_.buf[ibxn];
+  const double* __restrict__ byn = I_.buf[ibyn];
+  const double* __restrict__ bx1 = I_.buf[B_BX1];
+  const double* __restrict__ by1 = I_.buf[B_BY1];
+  double* __restrict__ bxqs = I_.buf[B_BXQS];
+  double* __restrict__ byqs = I_.buf[B_BYQS];
+  const double* S = I_.S;
+
   long long total = (long long)d.NX * d.NY;
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= total) return;
   const double bb = 1.0 - 1.0 / sqrt(2.0);
   const double dt = S[S_DT], adt0 = bb * dt;
@@ -180,10 +209,17 @@ __global__ void k_stage_combine(Dims d, const double* __restrict__ un, const dou
 // ================================================================ per-stage Qe preparation
 // p(Qe) at every padded cell (Qe_pri(.,.,4)) and the fast speed sqrt(a^2 + |B|^2/rho)
 // used by ShockDetector (flux.cpp:22-37).
-__global__ void k_qe_prep(Dims d, Phys P, const double* __restrict__ Qe, double* __restrict__ pe,
-                          double* __restrict__ cf, int* eflag) {
+__global__ void k_qe_prep(Phys P, const Inst* __restrict__ IT, BMap bm, int iQe) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ Qe = I_.buf[iQe];
+  double* __restrict__ pe = I_.buf[B_PE];
+  double* __restrict__ cf = I_.buf[B_CF];
+  int* eflag = I_.eflag;
+
   long long total = (long long)d.NX * d.NY;
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= total) return;
   double q[NVAR];
   for (int k = 0; k < NVAR; k++) q[k] = Qe[k * d.PS + c];
@@ -199,9 +235,15 @@ __global__ void k_qe_prep(Dims d, Phys P, const double* __restrict__ Qe, double*
 }
 
 // ShockDetector step 1 (flux.cpp:14-53): chi~ on the interior.
-__global__ void k_shock_chi(Dims d, Phys P, const double* __restrict__ Qe, const double* __restrict__ cf,
-                            double* __restrict__ chi) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+__global__ void k_shock_chi(Phys P, const Inst* __restrict__ IT, BMap bm, int iQe) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ Qe = I_.buf[iQe];
+  const double* __restrict__ cf = I_.buf[B_CF];
+  double* __restrict__ chi = I_.buf[B_CHI];
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   double cmin = 1.0e300;
   for (int di = -1; di <= 1; di++)
@@ -220,8 +262,14 @@ __global__ void k_shock_chi(Dims d, Phys P, const double* __restrict__ Qe, const
 }
 
 // ShockDetector step 2 (flux.cpp:60-75): eta = 3x3 max of chi~.
-__global__ void k_shock_eta(Dims d, const double* __restrict__ chi, double* __restrict__ eta) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+__global__ void k_shock_eta(const Inst* __restrict__ IT, BMap bm) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ chi = I_.buf[B_CHI];
+  double* __restrict__ eta = I_.buf[B_ETA];
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   double m = 0.0;
   for (int di = -1; di <= 1; di++)
@@ -333,18 +381,24 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const T* QL, const T*
 // the reconstruction at the ghost-map image of c, as the reference's bound_cond_u on w_*
 // produces.  kind 0 writes u[0..3] (the energy slot of the convective predictor is never
 // read); kind 1 writes u[1..4] (its density equals the convective one bit for bit).
-#define EX_TX 32
-#define EX_TY 8
-#define EX_NW 320            // max cells reconstructed per direction: 34x8 (x) / 32x10 (y)
-#define EX_NF 288            // max faces per direction: 33x8 (x) / 32x9 (y)
+// The tile is EX_TX x (256 / EX_TX) cells (EX_TX = 16, 32 or 64, chosen by the autotuner); the
+// tile shape changes which lane forms which face value, not the value.
+template <int EX_TX> struct ExTile {
+  static constexpr int ROWS = 256 / EX_TX;
+  static constexpr int NWX = (EX_TX + 2) * ROWS, NWY = EX_TX * (ROWS + 2);
+  static constexpr int NFX = (EX_TX + 1) * ROWS, NFY = EX_TX * (ROWS + 1);
+  static constexpr int NW = NWX > NWY ? NWX : NWY;     // cells reconstructed per direction, at most
+  static constexpr int NF = NFX > NFY ? NFX : NFY;     // faces per direction, at most
+};
 
 // One direction of the fused update: reconstruct, form the face fluxes, apply their difference.
 // T is the arithmetic type of the reconstruction and the fluxes (float in the mixed form); the
 // conservative update u = Qs - adt div F is always formed in FP64.
-template <int DIR, class T>
+template <int DIR, int EX_TX, class T>
 __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int kind, const double* __restrict__ Qe,
                                              const double* __restrict__ Qs, double* __restrict__ u, double h, T s_max,
                                              int i0, int j0, T* wl, T* wr, T* F, double* acc) {
+  constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
   constexpr int W = DIR == 0 ? EX_TX + 2 : EX_TX;                  // reconstructed cells per row
   constexpr int NITEMS = DIR == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
   constexpr int NF = DIR == 0 ? (EX_TX + 1) * EX_TY : EX_TX * (EX_TY + 1);
@@ -410,29 +464,46 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_explicit(Dims d, Phys P, int kind, const double* __restrict__ Qe,
-                                                  const double* __restrict__ Qs, double* __restrict__ u, const double* S) {
+template <int EX_TX, class T>
+__global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict__ IT, BMap bm, int kind, int iQe, int iQs, int iu) {
+  constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims d = I_.d;
+  const double* __restrict__ Qe = I_.buf[iQe];
+  const double* __restrict__ Qs = I_.buf[iQs];
+  double* __restrict__ u = I_.buf[iu];
+  const double* S = I_.S;
+
   __shared__ T wl[5 * EX_NW], wr[5 * EX_NW];
   __shared__ T F[5 * EX_NF];
-  const int i0 = GC + blockIdx.x * EX_TX, j0 = GC + blockIdx.y * EX_TY;
+  const int i0 = GC + bx_ * EX_TX, j0 = GC + by_ * EX_TY;
   const double h = S[S_ADT];
   const T s_max = (T)S[S_AMAX];
   double acc[5];
-  explicit_dir<0, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
-  explicit_dir<1, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
+  explicit_dir<0, EX_TX, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
+  explicit_dir<1, EX_TX, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
 }
-template __global__ void k_explicit<double>(Dims, Phys, int, const double*, const double*, double*, const double*);
-template __global__ void k_explicit<float>(Dims, Phys, int, const double*, const double*, double*, const double*);
 
 // ================================================================ magnetic subsystem
 // RHS of MagSubsystem_UpdateB (imex_subsystems.cpp:549-613), initial guess x0 = B(Qe),
 // and the enthalpy ratio hhat = calenthalpy(Qe) rho^e / rho (used by the pressure subsystem).
-__global__ void __launch_bounds__(256) k_mag_rhs(Dims d, Phys P, const double* __restrict__ u, const double* __restrict__ Qe,
-                                                 const double* __restrict__ Qs, const double* __restrict__ pe,
-                                                 double* __restrict__ b, double* __restrict__ x0, double* __restrict__ hhat,
-                                                 const double* S, int* eflag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+__global__ void __launch_bounds__(256) k_mag_rhs(Phys P, const Inst* __restrict__ IT, BMap bm, int iu, int iQe, int iQs,
+                                                 int ib, int ix0) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = I_.buf[iu];
+  const double* __restrict__ Qe = I_.buf[iQe];
+  const double* __restrict__ Qs = I_.buf[iQs];
+  const double* __restrict__ pe = I_.buf[B_PE];
+  double* __restrict__ b = I_.buf[ib];
+  double* __restrict__ x0 = I_.buf[ix0];
+  double* __restrict__ hhat = I_.buf[B_HHAT];
+  const double* S = I_.S;
+  int* eflag = I_.eflag;
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   const double h = S[S_ADT], dx = d.dx, dy = d.dy;
   auto U = [&](int a, int bb, int k) { return u[k * d.PS + pidx(d, a, bb)]; };
@@ -473,9 +544,15 @@ __global__ void __launch_bounds__(256) k_mag_rhs(Dims d, Phys P, const double* _
 }
 
 // u_con[5..7] <- B~ at every padded cell through the M_B map (imex_subsystems.cpp:618-625).
-__global__ void k_mag_writeback(Dims d, const double* __restrict__ x, double* __restrict__ u) {
+__global__ void k_mag_writeback(const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ x = I_.buf[B_XK];
+  double* __restrict__ u = I_.buf[iu];
+
   long long total = (long long)d.NX * d.NY;
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= total) return;
   int i = (int)(c % d.NX), j = (int)(c / d.NX);
   int id = kidx(d, i, j);
@@ -486,10 +563,18 @@ __global__ void k_mag_writeback(Dims d, const double* __restrict__ x, double* __
 
 // CollectMagFluxes (ct_flux.cpp:5-130): Fx at x faces (i+1/2, j), Fy at y faces (i, j+1/2),
 // evaluated at the ghost-map image of every padded index (= compute on [1, n+2g-1) then bound_cond_array).
-__global__ void __launch_bounds__(256) k_emf_flux(Dims d, const double* __restrict__ u, const double* __restrict__ Qe,
-                                                  const double* __restrict__ pe, double* __restrict__ Fx,
-                                                  double* __restrict__ Fy, const double* S) {
-  int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y * blockDim.y + threadIdx.y;
+__global__ void __launch_bounds__(256) k_emf_flux(const Inst* __restrict__ IT, BMap bm, int iu, int iQe) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = I_.buf[iu];
+  const double* __restrict__ Qe = I_.buf[iQe];
+  const double* __restrict__ pe = I_.buf[B_PE];
+  double* __restrict__ Fx = I_.buf[B_FX];
+  double* __restrict__ Fy = I_.buf[B_FY];
+  const double* S = I_.S;
+
+  int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
   const int i = map_x(d, ii), j = map_y(d, jj);
   const double h = S[S_ADT], dx = d.dx, dy = d.dy;
@@ -537,11 +622,21 @@ __global__ void __launch_bounds__(256) k_emf_flux(Dims d, const double* __restri
 // RHS of PreSubsystem_Pressure (imex_subsystems.cpp:123-168 first pass, :226-270 second pass).
 // pass 0 also stores rho v* (mxs, mys), the initial pressure p0 = p(Qe) and the Jacobi diagonal
 // part Tdiag (implicit_operators.cpp:631-649).
-__global__ void __launch_bounds__(256) k_p_rhs(Dims d, int pass, const double* __restrict__ u, const double* __restrict__ hhat,
-                                               double* __restrict__ mxs, double* __restrict__ mys,
-                                               const double* __restrict__ pe, double* __restrict__ b,
-                                               double* __restrict__ p0, double* __restrict__ Tdiag, const double* S) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+__global__ void __launch_bounds__(256) k_p_rhs(const Inst* __restrict__ IT, BMap bm, int pass, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = I_.buf[iu];
+  const double* __restrict__ hhat = I_.buf[B_HHAT];
+  double* __restrict__ mxs = I_.buf[B_MXS];
+  double* __restrict__ mys = I_.buf[B_MYS];
+  const double* __restrict__ pe = I_.buf[B_PE];
+  double* __restrict__ b = I_.buf[B_PB];
+  double* __restrict__ p0 = I_.buf[B_P];
+  double* __restrict__ Tdiag = I_.buf[B_TDIAG];
+  const double* S = I_.S;
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   const double h = S[S_ADT], dx = d.dx, dy = d.dy;
   long long c = pidx(d, i, j);
@@ -568,9 +663,17 @@ __global__ void __launch_bounds__(256) k_p_rhs(Dims d, int pass, const double* _
 }
 
 // Momentum update (imex_subsystems.cpp:186-215).
-__global__ void k_p_momentum(Dims d, const double* __restrict__ p, const double* __restrict__ mxs,
-                             const double* __restrict__ mys, double* __restrict__ u, const double* S) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+__global__ void k_p_momentum(const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ p = I_.buf[B_P];
+  const double* __restrict__ mxs = I_.buf[B_MXS];
+  const double* __restrict__ mys = I_.buf[B_MYS];
+  double* __restrict__ u = I_.buf[iu];
+  const double* S = I_.S;
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   const double h = S[S_ADT];
   double pc = p[kidx(d, i, j)];
@@ -582,8 +685,15 @@ __global__ void k_p_momentum(Dims d, const double* __restrict__ p, const double*
 }
 
 // Energy from the definition (imex_subsystems.cpp:286-302).
-__global__ void k_p_energy(Dims d, Phys P, const double* __restrict__ p, double* __restrict__ u, int* eflag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC, j = blockIdx.y * blockDim.y + threadIdx.y + GC;
+__global__ void k_p_energy(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ p = I_.buf[B_P];
+  double* __restrict__ u = I_.buf[iu];
+  int* eflag = I_.eflag;
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   long long c = pidx(d, i, j);
   double q[NVAR];
@@ -603,10 +713,17 @@ __device__ __forceinline__ double ez_tilde(const Dims& d, const double* u, int i
   double r = u[c];
   return u[2 * d.PS + c] / r * u[5 * d.PS + c] - u[1 * d.PS + c] / r * u[6 * d.PS + c];
 }
-__global__ void k_ct_corner(Dims d, const double* __restrict__ ezx, const double* __restrict__ ezy, double sx, double sy,
-                            const double* __restrict__ u, double* __restrict__ Ezpp) {
+__global__ void k_ct_corner(const Inst* __restrict__ IT, BMap bm, int iex, int iey, double sx, double sy, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ ezx = I_.buf[iex];
+  const double* __restrict__ ezy = I_.buf[iey];
+  const double* __restrict__ u = I_.buf[iu];
+  double* __restrict__ Ezpp = I_.buf[B_EZPP];
+
   // Ez_p0 = sx * ezx, Ez_0p = sy * ezy (sx = -1, sy = +1 for the EMF-consistent form)
-  int i = blockIdx.x * blockDim.x + threadIdx.x + GC - 2, j = blockIdx.y * blockDim.y + threadIdx.y + GC - 2;
+  int i = bx_ * blockDim.x + threadIdx.x + GC - 2, j = by_ * blockDim.y + threadIdx.y + GC - 2;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   auto EX = [&](int a, int b) { return sx * ezx[pidx(d, a, b)]; };
   auto EY = [&](int a, int b) { return sy * ezy[pidx(d, a, b)]; };
@@ -633,10 +750,19 @@ __device__ __forceinline__ int smap_y(const Dims& d, int axis, int j) {
   int lo = axis == 1 ? GC - 1 : GC;
   return j < lo ? lo : (j > d.ny + GC - 1 ? d.ny + GC - 1 : j);
 }
-__global__ void k_ct_faces(Dims d, const double* __restrict__ bxqs, const double* __restrict__ byqs,
-                           const double* __restrict__ Ezpp, double* __restrict__ bx, double* __restrict__ by,
-                           double* __restrict__ u, const double* S) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+__global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int ibyqs, int ibx, int iby, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ bxqs = I_.buf[ibxqs];
+  const double* __restrict__ byqs = I_.buf[ibyqs];
+  const double* __restrict__ Ezpp = I_.buf[B_EZPP];
+  double* __restrict__ bx = I_.buf[ibx];
+  double* __restrict__ by = I_.buf[iby];
+  double* __restrict__ u = I_.buf[iu];
+  const double* S = I_.S;
+
+  int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
   if (i >= d.NX || j >= d.NY) return;
   const double h = S[S_ADT];
   auto BX = [&](int a, int b) { int I = smap_x(d, 0, a), J = smap_y(d, 0, b); return bxqs[pidx(d, I, J)] - h / d.dy * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I, J - 1)]); };
@@ -651,9 +777,15 @@ __global__ void k_ct_faces(Dims d, const double* __restrict__ bxqs, const double
 
 // Plain Staggered_CT face EMFs (constrained_transport.cpp:73-100): central averages of
 // f_full_con[6] / g_full_con[5] over the reconstructed states of u_con.
-__global__ void __launch_bounds__(256) k_ct_plain_emf(Dims d, Phys P, const double* __restrict__ u,
-                                                      double* __restrict__ ezx, double* __restrict__ ezy) {
-  int ii = blockIdx.x * blockDim.x + threadIdx.x, jj = blockIdx.y * blockDim.y + threadIdx.y;
+__global__ void __launch_bounds__(256) k_ct_plain_emf(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = I_.buf[iu];
+  double* __restrict__ ezx = I_.buf[B_EZX];
+  double* __restrict__ ezy = I_.buf[B_EZY];
+
+  int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
   int i = map_x(d, ii), j = map_y(d, jj);
   double w[2][2][NVAR];   // [dir][0: right state of (i,j), 1: left state of neighbour]
@@ -676,9 +808,16 @@ __global__ void __launch_bounds__(256) k_ct_plain_emf(Dims d, Phys P, const doub
 }
 
 // ================================================================ state conversion, diagnostics
-__global__ void k_con2pri(Dims d, Phys P, const double* __restrict__ con, double* __restrict__ pri, int* eflag) {
+__global__ void k_con2pri(Phys P, const Inst* __restrict__ IT, BMap bm, int icon, int ipri) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ con = I_.buf[icon];
+  double* __restrict__ pri = I_.buf[ipri];
+  int* eflag = I_.eflag;
+
   long long total = (long long)d.NX * d.NY;
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= total) return;
   double q[NVAR], r[NVAR];
   for (int k = 0; k < NVAR; k++) q[k] = con[k * d.PS + c];
@@ -687,9 +826,16 @@ __global__ void k_con2pri(Dims d, Phys P, const double* __restrict__ con, double
   for (int k = 0; k < NVAR; k++) pri[k * d.PS + c] = r[k];
   if (ef) atomicOr(eflag, 1);
 }
-__global__ void k_pri2con(Dims d, Phys P, const double* __restrict__ pri, double* __restrict__ con, int* eflag) {
+__global__ void k_pri2con(Phys P, const Inst* __restrict__ IT, BMap bm, int ipri, int icon) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ pri = I_.buf[ipri];
+  double* __restrict__ con = I_.buf[icon];
+  int* eflag = I_.eflag;
+
   long long total = (long long)d.NX * d.NY;
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= total) return;
   double q[NVAR], r[NVAR];
   for (int k = 0; k < NVAR; k++) q[k] = pri[k * d.PS + c];
@@ -699,14 +845,114 @@ __global__ void k_pri2con(Dims d, Phys P, const double* __restrict__ pri, double
   if (ef) atomicOr(eflag, 1);
 }
 // cal_divergence_error, StaggerCT branch (imex_subsystems.cpp:630-660): per-block partial sums.
-__global__ void k_divb(Dims d, const double* __restrict__ bx, const double* __restrict__ by, double* partial) {
+__global__ void k_divb(const Inst* __restrict__ IT, BMap bm, int ibx, int iby) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ bx = I_.buf[ibx];
+  const double* __restrict__ by = I_.buf[iby];
+  double* partial = I_.partial;
+
   __shared__ double sh[16];
   double acc = 0.0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < d.N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = bx_ * (long long)blockDim.x + threadIdx.x; t < d.N; t += (long long)gx_ * blockDim.x) {
     int i = (int)(t % d.nx) + GC, j = (int)(t / d.nx) + GC;
     double v = (bx[pidx(d, i, j)] - bx[pidx(d, i - 1, j)]) / d.dx + (by[pidx(d, i, j)] - by[pidx(d, i, j - 1)]) / d.dy;
     acc += d.dx * d.dy * v * v;
   }
   double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial[bx_] = s;
+}
+
+// Vbase_slope(p^e, rho) of the RK equation of state at every padded cell (the viscosity factor of
+// the pressure operator, implicit_operators.cpp:584-600).
+__global__ void k_visc_rk(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  long long c = bx_ * (long long)blockDim.x + threadIdx.x;
+  if (c >= (long long)d.NX * d.NY) return;
+  int ef = 0;
+  I_.buf[B_VISC][c] = Vbase_slope(P, I_.buf[B_PE][c], I_.buf[iu][c], &ef);
+  if (ef) atomicOr(I_.eflag, 1);
+}
+
+// ================================================================ launchers
+#define GRID(L, sh) dim3((unsigned)(L).nblocks[sh])
+static const dim3 B64x4(64, 4);
+
+void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind) {
+  FillPlanes fp;
+  fp.n = n;
+  for (int p = 0; p < n; p++) { fp.b[p] = b[p]; fp.k[p] = k[p]; }
+  const int sh = kind == 0 ? SH_FILL0 : (kind == 1 ? SH_FILL1 : SH_FILL2);
+  k_bc_fill<<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], fp, kind);
+}
+void launch_step_start(const Launch& L, const Phys& P, int mode, int iin) {
+  k_step_start<<<GRID(L, SH_START), 256, 0, L.st>>>(P, L.IT, L.bm[SH_START], mode, iin);
+  k_step_finish<<<GRID(L, SH_ONE), 256, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_START]);
+}
+void launch_stage_combine(const Launch& L, int ibxn, int ibyn) {
+  k_stage_combine<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_TOT], ibxn, ibyn);
+}
+void launch_qe_prep(const Launch& L, const Phys& P, int iQe) {
+  k_qe_prep<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_TOT], iQe);
+}
+void launch_shock(const Launch& L, const Phys& P, int iQe) {
+  k_shock_chi<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iQe);
+  const int chi[1] = {B_CHI}, k0[1] = {0};
+  launch_bc_fill(L, 1, chi, k0, 0);
+  k_shock_eta<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT64x4]);
+  const int eta[1] = {B_ETA};
+  launch_bc_fill(L, 1, eta, k0, 0);
+}
+template <int TXE>
+static void launch_explicit_t(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed) {
+  if (mixed) k_explicit<TXE, float><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  else k_explicit<TXE, double><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+}
+// tile: the explicit tile width (16, 32 or 64 cells); the SH_EXPL block map must be built for it
+void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile) {
+  if (tile == 16) launch_explicit_t<16>(L, P, kind, iQe, iQs, iu, mixed);
+  else if (tile == 64) launch_explicit_t<64>(L, P, kind, iQe, iQs, iu, mixed);
+  else launch_explicit_t<32>(L, P, kind, iQe, iQs, iu, mixed);
+}
+void launch_mag_rhs(const Launch& L, const Phys& P, int iu, int iQe, int iQs, int ib, int ix0) {
+  k_mag_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu, iQe, iQs, ib, ix0);
+}
+void launch_mag_writeback(const Launch& L, int iu) {
+  k_mag_writeback<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_TOT], iu);
+}
+void launch_emf_flux(const Launch& L, int iu, int iQe) {
+  k_emf_flux<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x4], iu, iQe);
+}
+void launch_p_rhs(const Launch& L, int pass, int iu) {
+  k_p_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT64x4], pass, iu);
+}
+void launch_p_momentum(const Launch& L, int iu) {
+  k_p_momentum<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT64x4], iu);
+}
+void launch_p_energy(const Launch& L, const Phys& P, int iu) {
+  k_p_energy<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu);
+}
+void launch_ct(const Launch& L, const Phys& P, int plain, int iu, int ibxqs, int ibyqs, int ibx, int iby) {
+  int iex = B_FX, iey = B_FY;
+  if (plain) {
+    k_ct_plain_emf<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_PAD64x4], iu);
+    iex = B_EZX; iey = B_EZY;
+  }
+  k_ct_corner<<<GRID(L, SH_INT2_64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT2_64x4], iex, iey, -1.0, 1.0, iu);
+  k_ct_faces<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x4], ibxqs, ibyqs, ibx, iby, iu);
+}
+void launch_con2pri(const Launch& L, const Phys& P, int icon, int ipri) {
+  k_con2pri<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_TOT], icon, ipri);
+}
+void launch_pri2con(const Launch& L, const Phys& P, int ipri, int icon) {
+  k_pri2con<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_TOT], ipri, icon);
+}
+void launch_divb(const Launch& L, int ibx, int iby) {
+  k_divb<<<GRID(L, SH_DIVB), 256, 0, L.st>>>(L.IT, L.bm[SH_DIVB], ibx, iby);
+}
+void launch_visc_rk(const Launch& L, const Phys& P, int iu) {
+  k_visc_rk<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_TOT], iu);
 }

@@ -167,7 +167,39 @@ int imex_step_async(imex_ctx* ctx, int nsteps);
 int imex_sync(imex_ctx* ctx);
 int imex_get_stats(imex_ctx* ctx, imex_stats* stats);
 int imex_reset_stats(imex_ctx* ctx);
-double imex_div_b(imex_ctx* ctx);   /* cal_divergence_error, StaggerCT branch (imex_subsystems.cpp:630-660) */
+double imex_div_b(imex_ctx* ctx);   /* cal_divergence_error, StaggerCT branch (imex_subsystems.cpp:630-660); max over a batch */
+
+/* ---- batches: independent instances of uneven grid size stepped together ----
+ * The reference runs one grid per process (src/main.cpp:231-250, one Grid built from
+ * config.hpp's nx, ny); a batch holds n such runs, sharing every parameter of `p` except
+ * the grid size (nx[i], ny[i]), each with its own state, time step and solver history.
+ * Every launch of a step covers all instances; the step runs on iteration budgets (the
+ * largest any instance needs) and is replayed as a captured graph.  imex_create is a
+ * batch of one.  Stage- and operator-level calls take a single-instance context.
+ * Stats sum iteration counts over instances (min_p is the minimum, max_floored the
+ * maximum).  Instance indices run from 0 to n-1. */
+int imex_batch_create(const imex_params* p, int n, const int* nx, const int* ny, imex_ctx** out);
+int imex_batch_size(const imex_ctx* ctx);
+int imex_batch_set_state(imex_ctx* ctx, int i, const double* u_pri, const double* bx, const double* by,
+                         double t, int timestep, double prev_dt);
+int imex_batch_get_state(imex_ctx* ctx, int i, double* u_con, double* u_pri, double* bx, double* by,
+                         imex_scalars* sc);
+double imex_batch_div_b(imex_ctx* ctx, int i);
+
+/* ---- autotuner (no counterpart in the reference, whose loops have no launch shape) ----
+ * Times candidate launch shapes on the context's current state (set it first) and keeps the
+ * fastest: IMEX_TUNE_EXPLICIT picks the explicit-flux tile among 16x16, 32x8 and 64x4 cells
+ * (results identical bit for bit); IMEX_TUNE_KRYLOV picks the grid cap of the Krylov kernels
+ * among 1024, 2048 and 4096 blocks per instance (a different cap reorders the dot-product sums,
+ * so results change by rounding).  The state, time and statistics are left as they were. */
+enum { IMEX_TUNE_EXPLICIT = 1, IMEX_TUNE_KRYLOV = 2 };
+typedef struct imex_tune_info {
+  int explicit_tile_x, explicit_tile_y;   /* chosen explicit tile (cells) */
+  int krylov_blocks;                      /* chosen Krylov grid cap (blocks per instance) */
+  double explicit_us[3];                  /* both explicit kernels, per candidate 16x16, 32x8, 64x4 */
+  double krylov_us[3];                    /* one Arnoldi step (apply + update), per cap 1024, 2048, 4096 */
+} imex_tune_info;
+int imex_autotune(imex_ctx* ctx, int flags, imex_tune_info* out);
 
 /* ---- stage-level drop-ins, one per reference entry point ---- */
 int imex_set_stage(imex_ctx* ctx, const imex_stage_host* st, const imex_scalars* sc);
