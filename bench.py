@@ -68,6 +68,95 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None):
     return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final)
 
 
+def run_pair(name, steps, warmup, schedule):
+    """FP64 and mixed forms of one workload (no per-kernel pass): throughput and the mixed form's error."""
+    import numpy as np
+    import imex_amd as im
+    out = {}
+    finals = {}
+    for f, prec in (("fp64", im.PREC_FP64), ("mixed", im.PREC_MIXED)):
+        p, kw = im.case_of(name)
+        p.precision = prec
+        p.schedule = schedule
+        u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
+        s = im.Solver(p)
+        try:
+            s.set_state(u, bx, by)
+            log(f"[bench] {name} {f}: warmup {warmup}, timing {steps} steps")
+            if warmup:
+                s.step(warmup)
+            s.reset_stats()
+            s.sync()
+            t0 = time.perf_counter()
+            s.step_async(steps)
+            s.sync()
+            t1 = time.perf_counter()
+            st = s.stats()
+            finals[f] = s.get_state()[0]
+        finally:
+            s.close()
+        out[f] = {"value": round(p.nx * p.ny * steps / (t1 - t0) / 1e6, 3), "ms_per_step": round((t1 - t0) / steps * 1e3, 3),
+                  "gmres": {k: st[k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters",
+                                               "mixed_refinements")},
+                  "min_p": st["min_p"], "div_b": st["div_b"]}
+        log(f"[bench] {name} {f}: {out[f]['value']} Mcell-steps/s")
+    a, b = finals["mixed"], finals["fp64"]
+    g = im.GC
+    errs = [float(np.linalg.norm(a[g:-g, g:-g, k] - b[g:-g, g:-g, k]) / max(np.linalg.norm(b[g:-g, g:-g, k]), 1e-300))
+            for k in range(b.shape[-1])]
+    out["mixed_rel_l2_vs_fp64_max"] = float(f"{max(errs):.3e}")
+    out["steps_compared"] = warmup + steps
+    out["speedup_mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
+BATCH_SIZES = (64, 96, 128, 192, 256, 384, 512)
+
+
+def run_batch(n_inst, steps, warmup, eos, seed, schedule):
+    """Independent MHD vortices of uneven size in one batch context; aggregate throughput."""
+    import numpy as np
+    import imex_amd as im
+    rng = np.random.default_rng(seed)
+    sizes = [(int(rng.choice(BATCH_SIZES)), int(rng.choice(BATCH_SIZES))) for _ in range(n_inst)]
+    amps = rng.uniform(0.5, 2.0, size=(n_inst, 3))
+    p = im.vortex_params(64, eos=eos)
+    p.schedule = schedule
+    cells = sum(a * b for a, b in sizes)
+    log(f"[bench] batch: {n_inst} vortices, eos {'RK' if eos == im.EOS_RK else 'ideal'}, {cells / 1e6:.2f} Mcells")
+    t_setup = time.perf_counter()
+    b = im.Batch(p, sizes)
+    try:
+        for i, (nx, ny) in enumerate(sizes):
+            q = b.params_of(i)
+            u, bx, by = im.init_case(q, im.CASE_VORTEX, amps[i, 0], amps[i, 1], amps[i, 2], 0)
+            b.set_instance(i, u, bx, by)
+        tune = b.autotune(im.TUNE_EXPLICIT | im.TUNE_KRYLOV)
+        log(f"[bench] batch autotune: {tune}")
+        if warmup:
+            b.step(warmup)
+        b.reset_stats()
+        b.sync()
+        t_setup = time.perf_counter() - t_setup
+        t0 = time.perf_counter()
+        b.step_async(steps)
+        b.sync()
+        t1 = time.perf_counter()
+        st = b.stats()
+        dev_gb = b.L.imex_device_bytes(b.h) / 1e9
+    finally:
+        b.close()
+    value = cells * steps / (t1 - t0) / 1e6
+    log(f"[bench] batch eos {eos}: {value:.1f} Mcell-steps/s aggregate; {st}")
+    return {"value": round(value, 3), "ms_per_step": round((t1 - t0) / steps * 1e3, 3), "instances": n_inst,
+            "cells": cells, "seed": seed, "eos": "redlich_kwong" if eos == im.EOS_RK else "ideal",
+            "autotuned": {"explicit_tile": list(tune["explicit_tile"]), "krylov_blocks": tune["krylov_blocks"],
+                          "explicit_us": [round(x, 1) for x in tune["explicit_us"]],
+                          "krylov_us": [round(x, 1) for x in tune["krylov_us"]]},
+            "budget_steps": st["budget_steps"], "budget_reruns": st["budget_reruns"], "graph_builds": st["graph_builds"],
+            "max_div_b": st["div_b"], "device_gb": round(dev_gb, 2), "setup_s": round(t_setup, 1)}
+
+
 def cpu_baseline(name, state, budget_steps=1):
     """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
     import imex_amd as im
@@ -139,6 +228,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--mixed-tols", type=float, nargs=2, default=None, metavar=("INNER", "OUTER"),
                     help="override the mixed form's inner / outer refinement tolerances")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary workloads (blast 1024^2 FP64 vs mixed; the 256-vortex batches)")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--blast-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -192,6 +286,14 @@ def main():
         out["mixed_accuracy"] = {"rel_l2_vs_fp64_max": max(errs), "rel_l2_vs_fp64_per_field": [float(f"{e:.3e}") for e in errs],
                                  "steps_compared": args.warmup + args.steps}
         out["speedup_mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    if not args.no_extra:
+        out["blast"] = dict(workload="magnetic blast 1024x1024 VanLeer, transmissive, nested-Newton p_floor 1e-10",
+                            steps=args.blast_steps, **run_pair("blast1024", args.blast_steps, 0, sched))
+        out["batch"] = {"workload": f"{args.batch_instances} MHD vortices, sizes from {list(BATCH_SIZES)}^2, "
+                                    "amplitude and background velocity U[0.5,2], Minmod, StaggerCT, FP64",
+                        "steps": args.batch_steps, "warmup": 1,
+                        "ideal": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_IDEAL, 2024, sched),
+                        "redlich_kwong": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_RK, 2024, sched)}
     if not args.no_cpu:
         log("[bench] CPU oracle sample")
         v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps)

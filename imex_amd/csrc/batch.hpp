@@ -4,7 +4,8 @@
 // scalars, Krylov control block and buffers).  Kernels receive the device table of instances
 // and small buffer ids instead of raw pointers; one launch covers every instance: workgroup b
 // belongs to the instance i with pre[i] <= b < pre[i+1] and is local block b - pre[i] of that
-// instance's grid (gx[i] blocks per row).  A single-instance context is a batch of one.
+// instance's grid (gx[i] blocks per row); a per-workgroup table holds (i, b - pre[i]).  A
+// single-instance context is a batch of one.
 #pragma once
 
 #define MAXM 32            // largest GMRES restart length supported
@@ -39,29 +40,39 @@ struct Inst {
   int* esave;
 };
 
+// Buffers of every instance live in device global memory, but a pointer read from the instance
+// table is generic to the compiler, which would then address it with flat instructions (these
+// count against both the memory and the LDS wait counters).  gmem() states the address space:
+// going through an integer keeps the compiler from folding the cast away.
+template <class V>
+__device__ __forceinline__ V* gmem(V* p) {
+  return (V*)(__attribute__((address_space(1))) V*)(unsigned long long)p;
+}
+
 struct BMap {
+  const int2* blk;       // per workgroup: (instance, local block)
   const int* pre;        // ni + 1 prefix sums of the per-instance block counts
   const int* gx;         // blocks per grid row of each instance
   int ni;
+  int gx0, nb0;          // row width and block count when ni == 1 (no table reads)
 };
 
-__device__ __forceinline__ void bmap_locate(const BMap& m, int b, int& inst, int& lb) {
-  int lo = 0, hi = m.ni;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (m.pre[mid] <= b) lo = mid; else hi = mid;
-  }
-  inst = lo;
-  lb = b - m.pre[lo];
+// Local block count of instance i.
+__device__ __forceinline__ int bmap_count(const BMap& m, int i) {
+  return m.ni == 1 ? m.nb0 : m.pre[i + 1] - m.pre[i];
 }
 
 // Opens a kernel body: inst_ is the instance, bx_/by_ the local block indices, gx_/gy_ the local
-// grid (for 1D grids gy_ = 1 and gx_ is the local block count).
+// grid (for 1D grids gy_ = 1 and gx_ is the local block count).  A single-instance map reads no
+// table, so the kernel's first loads do not wait on a lookup.
 #define BATCH_BLOCK(bm)                                                     \
-  int inst_, lb_;                                                           \
-  bmap_locate(bm, blockIdx.x, inst_, lb_);                                  \
-  const int gx_ = (bm).gx[inst_];                                           \
-  const int gy_ = ((bm).pre[inst_ + 1] - (bm).pre[inst_]) / gx_;            \
+  int inst_, lb_, gx_;                                                      \
+  if ((bm).ni == 1) { inst_ = 0; lb_ = blockIdx.x; gx_ = (bm).gx0; }        \
+  else {                                                                    \
+    const int2 e_ = (bm).blk[blockIdx.x];                                   \
+    inst_ = e_.x; lb_ = e_.y; gx_ = (bm).gx[inst_];                         \
+  }                                                                         \
+  const int gy_ = bmap_count(bm, inst_) / gx_;                              \
   const int bx_ = lb_ % gx_, by_ = lb_ / gx_;                               \
   (void)gy_; (void)bx_; (void)by_
 

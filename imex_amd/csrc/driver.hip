@@ -36,17 +36,17 @@ __global__ void k_budget_begin(const Inst* __restrict__ IT, BMap bm) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   const int t = threadIdx.x;
-  if (t < S_NSLOTS) I.bsave[t] = I.S[t];
-  if (t < 4) I.bsave[S_NSLOTS + t] = I.pst[t];
-  if (t == 0) { I.esave[0] = I.eflag[0]; I.ctl[0] = 0; }
+  if (t < S_NSLOTS) gmem(I.bsave)[t] = gmem(I.S)[t];
+  if (t < 4) gmem(I.bsave)[S_NSLOTS + t] = gmem(I.pst)[t];
+  if (t == 0) { gmem(I.esave)[0] = gmem(I.eflag)[0]; gmem(I.ctl)[0] = 0; }
 }
 __global__ void k_budget_restore(const Inst* __restrict__ IT, BMap bm) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   const int t = threadIdx.x;
-  if (t < S_NSLOTS) I.S[t] = I.bsave[t];
-  if (t < 4) I.pst[t] = I.bsave[S_NSLOTS + t];
-  if (t == 0) I.eflag[0] = I.esave[0];
+  if (t < S_NSLOTS) gmem(I.S)[t] = gmem(I.bsave)[t];
+  if (t < 4) gmem(I.pst)[t] = gmem(I.bsave)[S_NSLOTS + t];
+  if (t == 0) gmem(I.eflag)[0] = gmem(I.esave)[0];
 }
 // GMRES log: ctl[slot] = Arnoldi steps over all sweeps, ctl[slot+1] = sweeps, then the steps of
 // each of the nsw budgeted sweeps.  A FP64-form solve has one sweep and converges inside it; a
@@ -54,8 +54,8 @@ __global__ void k_budget_restore(const Inst* __restrict__ IT, BMap bm) {
 __global__ void k_budget_gmres_check(const Inst* __restrict__ IT, BMap bm, int slot, int mixed, int nsw) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  const KrylovDev* K = I.K;
-  int* ctl = I.ctl;
+  const KrylovDev* K = gmem(I.K);
+  int* ctl = gmem(I.ctl);
   const int t = threadIdx.x;
   if (t == 0) {
     if (mixed) {
@@ -74,8 +74,8 @@ __global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int 
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   if (threadIdx.x == 0) {
-    I.ctl[slot] = I.K->newton_conv ? I.K->newton_first : -1;
-    if (!I.K->newton_conv) I.ctl[0] |= 2;
+    gmem(I.ctl)[slot] = gmem(I.K)->newton_conv ? gmem(I.K)->newton_first : -1;
+    if (!gmem(I.K)->newton_conv) gmem(I.ctl)[0] |= 2;
   }
 }
 
@@ -161,7 +161,8 @@ struct imex_ctx {
   std::vector<Inst> hI;              // host copy of the instance table
   Inst* IT = nullptr;                // device instance table
   int* bmap_dev = nullptr;           // block maps of all shapes
-  int* bmap1_dev = nullptr;          // block maps of a one-instance view (set-up work)
+  int2* blk_dev = nullptr;           // per-workgroup table of a batch (all shapes)
+  long long blk_cap = 0;             // its capacity (the largest caps the autotuner may choose)
   Launch L;
   long long cells = 0;               // sum of nx ny over instances
   int cap_tiles = 2048, cap_vec = 2048;   // grid caps of the Krylov kernels (autotuned)
@@ -189,6 +190,10 @@ struct imex_ctx {
   // solve, 2 Jacobian solves per Newton solve, 3/4 sweeps per magnetic / pressure solve
   int hist[NBK][BHIST];        // per-step maxima of the recent steps
   int hist_n = 0, hist_pos = 0;
+  // extra budgets of a batch after an overrun: Arnoldi steps (+2 per GMRES overrun), refinement
+  // sweeps and Newton iterations (+1 per overrun of their kind); each decays by 1 every 16 steps
+  // that fit
+  int margin_extra = 0, sweep_extra = 0, newton_extra = 0, fit_streak = 0;
   int step_mx[NBK] = {0, 0, 0, 0, 0};   // maxima of the step being run under host scheduling
   // the same per solve position (-1: no such solve in that step)
   int khist[BHIST][NGKEY][2], nhist[BHIST][NNKEY];
@@ -308,28 +313,47 @@ static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& n
   if (nb < 1) { nb = 1; gx = 1; }
 }
 
-// Block maps of every shape for the current instance sizes and caps (device table of prefix sums
-// and row widths).
+// Block maps of every shape for the current instance sizes and caps: per-instance prefix sums and
+// row widths, and for a batch the per-workgroup (instance, local block) table.
+static long long map_blocks(imex_ctx* c) {
+  long long tot = 0;
+  for (int sh = 0; sh < NSHAPE; sh++)
+    for (int i = 0; i < c->ni; i++) { int g, nb; shape_grid(c, sh, c->hI[i].d, g, nb); tot += nb; }
+  return tot;
+}
 static int build_maps(imex_ctx* c) {
   const int ni = c->ni;
   HIPCHK(hipStreamSynchronize(c->st));      // no launch may be reading the maps being replaced
   std::vector<int> h((size_t)NSHAPE * (2 * ni + 1));
+  std::vector<int2> blk;
+  std::vector<size_t> boff(NSHAPE);
   for (int sh = 0; sh < NSHAPE; sh++) {
     int* pre = &h[(size_t)sh * (2 * ni + 1)];
     int* gx = pre + ni + 1;
     pre[0] = 0;
+    boff[sh] = blk.size();
     for (int i = 0; i < ni; i++) {
       int g, nb;
       shape_grid(c, sh, c->hI[i].d, g, nb);
       pre[i + 1] = pre[i] + nb;
       gx[i] = g;
+      if (ni > 1) for (int b = 0; b < nb; b++) blk.push_back(make_int2(i, b));
     }
+    BMap& m = c->L.bm[sh];
     c->L.nblocks[sh] = pre[ni];
-    c->L.bm[sh].pre = c->bmap_dev + (size_t)sh * (2 * ni + 1);
-    c->L.bm[sh].gx = c->L.bm[sh].pre + ni + 1;
-    c->L.bm[sh].ni = ni;
+    m.pre = c->bmap_dev + (size_t)sh * (2 * ni + 1);
+    m.gx = m.pre + ni + 1;
+    m.ni = ni;
+    m.gx0 = gx[0];
+    m.nb0 = pre[1];
+    m.blk = nullptr;
   }
   HIPCHK(hipMemcpy(c->bmap_dev, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (ni > 1) {
+    if ((long long)blk.size() > c->blk_cap) return set_err(IMEX_ERR_ARG, "block table capacity exceeded");
+    for (int sh = 0; sh < NSHAPE; sh++) c->L.bm[sh].blk = c->blk_dev + boff[sh];
+    HIPCHK(hipMemcpy(c->blk_dev, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
   return 0;
 }
 
@@ -766,7 +790,7 @@ static void hist_push(imex_ctx* c, const int mx[NBK]) {
 
 // Budgets, per solve position: the largest count that position needed in the recent steps
 // (or, for a position not seen lately, the largest of its kind), plus budget_margin Arnoldi
-// steps per GMRES sweep.  Returns false (host scheduling) until there is history, or when a
+// steps per GMRES sweep, plus the extra margins that recent overruns have raised.  Returns false (host scheduling) until there is history, or when a
 // recent solve needed a restart.  A batch has no host scheduling: it starts from generous
 // budgets and grows them when a step outruns them.
 static bool choose_budgets(imex_ctx* c) {
@@ -782,25 +806,28 @@ static bool choose_budgets(imex_ctx* c) {
     int st = -1, sw = -1;
     for (int i = 0; i < c->hist_n; i++) { st = std::max(st, c->khist[i][k][0]); sw = std::max(sw, c->khist[i][k][1]); }
     if (st < 0) { st = mx[op]; sw = mx[3 + op]; }
-    c->tab.B[k] = std::min(m, std::max(1, st + c->prm.budget_margin));
-    c->tab.S[k] = std::max(1, sw);
+    c->tab.B[k] = std::min(m, std::max(1, st + c->prm.budget_margin + c->margin_extra));
+    c->tab.S[k] = std::min(MAXSWEEP - 1, std::max(1, sw + c->sweep_extra));
   }
   for (int k = 0; k < NNKEY; k++) {
     int nb = -1;
     for (int i = 0; i < c->hist_n; i++) nb = std::max(nb, c->nhist[i][k]);
-    c->tab.N[k] = std::min(NIT - 1, std::max(1, nb < 0 ? mx[2] : nb));
+    c->tab.N[k] = std::min(std::min(c->prm.newton_max, NIT - 1), std::max(1, (nb < 0 ? mx[2] : nb) + c->newton_extra));
   }
   return true;
 }
 
-// Reads the step's logs (all instances): returns 1 if a solve outran its budget, else books the
-// iteration counts (sums over instances, maxima for the budgets) and, with the profiler on,
-// moves launches that found their solve converged to early_exit.
+// Reads the step's logs (all instances): returns the failure bits (1 GMRES, 2 Newton) if a solve
+// outran its budget, else 0 after booking the iteration counts (sums over instances, maxima for
+// the budgets) and, with the profiler on, moving launches that found their solve converged to
+// early_exit.
 static int budget_finish(imex_ctx* c, long long gen0) {
   HIPCHK(hipMemcpyAsync(c->ctlh, c->ctl_all, (size_t)c->ni * CTLN * sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipEventRecord(c->ev[0], c->st));
   HIPCHK(hipEventSynchronize(c->ev[0]));
-  for (int i = 0; i < c->ni; i++) if (c->ctlh[(size_t)i * CTLN]) return 1;
+  int fail = 0;
+  for (int i = 0; i < c->ni; i++) fail |= c->ctlh[(size_t)i * CTLN];
+  if (fail) return fail;
   const bool fix = c->prof.on && c->prof.gen == gen0 && c->ni == 1;
   auto idle = [&](int r) { if (fix && r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
   int mx[NBK] = {0, 0, 0, 0, 0};
@@ -976,7 +1003,22 @@ static int one_step(imex_ctx* c) {
     int f = budget_finish(c, gen0);
     if (f < 0) return f;
     c->stats.budget_steps++;
-    if (f == 0) { step_commit(c); return rc; }
+    if (f == 0) {
+      if (++c->fit_streak >= 16) {
+        c->fit_streak = 0;
+        c->margin_extra = std::max(0, c->margin_extra - 1);
+        c->sweep_extra = std::max(0, c->sweep_extra - 1);
+        c->newton_extra = std::max(0, c->newton_extra - 1);
+      }
+      step_commit(c);
+      return rc;
+    }
+    c->fit_streak = 0;
+    if (c->ni > 1 && (f & 1)) {     // a single instance reruns the step host-scheduled instead
+      c->margin_extra = std::min(8, c->margin_extra + 2);
+      if (c->mixed) c->sweep_extra = std::min(4, c->sweep_extra + 1);
+    }
+    if (c->ni > 1 && (f & 2)) c->newton_extra = std::min(4, c->newton_extra + 1);
     // a solve outran its budget: restore the step's scalars and rerun it (host scheduling for a
     // single instance, larger budgets for a batch)
     c->stats.budget_reruns++;
@@ -1248,19 +1290,15 @@ static int alloc_instance(imex_ctx* c, int i) {
 // Launch descriptor covering instance i alone (host set-up work on one instance of a batch).
 static int one_launch(imex_ctx* c, int i, Launch& L1) {
   if (c->ni == 1) { L1 = c->L; return 0; }
-  std::vector<int> h((size_t)NSHAPE * 3);
   L1.IT = c->IT + i; L1.st = c->st;
   for (int sh = 0; sh < NSHAPE; sh++) {
     int g, nb;
     shape_grid(c, sh, c->hI[i].d, g, nb);
-    h[(size_t)sh * 3] = 0; h[(size_t)sh * 3 + 1] = nb; h[(size_t)sh * 3 + 2] = g;
     L1.nblocks[sh] = nb;
-    L1.bm[sh].pre = c->bmap1_dev + (size_t)sh * 3;
-    L1.bm[sh].gx = L1.bm[sh].pre + 2;
-    L1.bm[sh].ni = 1;
+    BMap& m = L1.bm[sh];
+    m.blk = nullptr; m.pre = nullptr; m.gx = nullptr;
+    m.ni = 1; m.gx0 = g; m.nb0 = nb;
   }
-  HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(c->bmap1_dev, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1310,7 +1348,13 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   if ((rc = dalloc(c, &c->IT, (size_t)n))) return fail(rc);
   if (hipMemcpy(c->IT, c->hI.data(), (size_t)n * sizeof(Inst), hipMemcpyHostToDevice) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "table"));
   if ((rc = dalloc(c, &c->bmap_dev, (size_t)NSHAPE * (2 * n + 1)))) return fail(rc);
-  if ((rc = dalloc(c, &c->bmap1_dev, (size_t)NSHAPE * 3))) return fail(rc);
+  if (n > 1) {          // block table sized for every launch shape the autotuner may choose
+    const int ct = c->cap_tiles, cv = c->cap_vec, et = c->ex_tile;
+    c->cap_tiles = c->cap_vec = 4096;
+    for (int t : {16, 32, 64}) { c->ex_tile = t; c->blk_cap = std::max(c->blk_cap, map_blocks(c)); }
+    c->cap_tiles = ct; c->cap_vec = cv; c->ex_tile = et;
+    if ((rc = dalloc(c, &c->blk_dev, (size_t)c->blk_cap))) return fail(rc);
+  }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { c->st = nullptr; return fail(set_err(IMEX_ERR_HIP, "stream")); }
   (void)hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming);
   (void)hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming);
@@ -1372,6 +1416,7 @@ int imex_batch_set_state(imex_ctx* c, int i, const double* u_pri, const double* 
     HIPCHK(hipStreamSynchronize(c->st));
   }
   c->hist_n = 0; c->hist_pos = 0;       // iteration budgets are relearned for the new state
+  c->margin_extra = c->sweep_extra = c->newton_extra = c->fit_streak = 0;
   return 0;
 }
 

@@ -47,19 +47,19 @@ struct MagIds { int Qe, u; };
 struct PIds { int u, jacobian, precond; };
 
 __device__ __forceinline__ MagOp make_mag_op(const Inst& I, const MagIds& m) {
-  MagOp o; o.d = I.d; o.Qe = I.buf[m.Qe]; o.u = I.buf[m.u]; o.S = I.S; return o;
+  MagOp o; o.d = I.d; o.Qe = gmem(I.buf[m.Qe]); o.u = gmem(I.buf[m.u]); o.S = gmem(I.S); return o;
 }
 __device__ __forceinline__ POp make_p_op(const Inst& I, const Phys& P, const PIds& q) {
-  POp o; o.d = I.d; o.P = P; o.hhat = I.buf[B_HHAT]; o.eta = I.buf[B_ETA]; o.u = I.buf[q.u];
+  POp o; o.d = I.d; o.P = P; o.hhat = gmem(I.buf[B_HHAT]); o.eta = gmem(I.buf[B_ETA]); o.u = gmem(I.buf[q.u]);
   const bool rk = P.eos != IMEX_EOS_IDEAL;
-  o.visc = rk ? I.buf[B_VISC] : nullptr; o.pfreeze = I.buf[B_P]; o.vslope = rk ? I.buf[B_VSLOPE] : nullptr;
-  o.Tdiag = I.buf[B_TDIAG]; o.S = I.S; o.jacobian = q.jacobian; o.precond = q.precond;
+  o.visc = rk ? gmem(I.buf[B_VISC]) : nullptr; o.pfreeze = gmem(I.buf[B_P]); o.vslope = rk ? gmem(I.buf[B_VSLOPE]) : nullptr;
+  o.Tdiag = gmem(I.buf[B_TDIAG]); o.S = gmem(I.S); o.jacobian = q.jacobian; o.precond = q.precond;
   return o;
 }
 
 // skip kinds: launches of a Jacobian solve whose Newton iteration has already converged
 enum { SKIP_NONE = 0, SKIP_NEWTON = 1 };
-__device__ __forceinline__ bool skipped(const Inst& I, int kind) { return kind == SKIP_NEWTON && I.K->newton_conv; }
+__device__ __forceinline__ bool skipped(const Inst& I, int kind) { return kind == SKIP_NEWTON && gmem(I.K)->newton_conv; }
 
 // The host side of a launch: the instance table, the block map of the launch's shape and the
 // number of instances (grid = total blocks), and the stream.

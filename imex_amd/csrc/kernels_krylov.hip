@@ -23,14 +23,14 @@
 // the mixed form): the basis V_i, the unnormalised next vector w and the operator image Av.
 template <class VT> struct KV;
 template <> struct KV<double> {
-  __device__ static double* V(const Inst& I, int i) { return I.buf[B_V0 + i]; }
-  __device__ static double* w(const Inst& I) { return I.buf[B_W]; }
-  __device__ static double* Av(const Inst& I) { return I.buf[B_AV]; }
+  __device__ static double* V(const Inst& I, int i) { return gmem(I.buf[B_V0 + i]); }
+  __device__ static double* w(const Inst& I) { return gmem(I.buf[B_W]); }
+  __device__ static double* Av(const Inst& I) { return gmem(I.buf[B_AV]); }
 };
 template <> struct KV<float> {
-  __device__ static float* V(const Inst& I, int i) { return I.fbuf[F_V0 + i]; }
-  __device__ static float* w(const Inst& I) { return I.fbuf[F_W32]; }
-  __device__ static float* Av(const Inst& I) { return I.fbuf[F_AV32]; }
+  __device__ static float* V(const Inst& I, int i) { return gmem(I.fbuf[F_V0 + i]); }
+  __device__ static float* w(const Inst& I) { return gmem(I.fbuf[F_W32]); }
+  __device__ static float* Av(const Inst& I) { return gmem(I.fbuf[F_AV32]); }
 };
 
 // compact-coordinate boundary map for one-cell halos (M_P with i in [g-1, n+g])
@@ -351,7 +351,7 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
                  int outer, int x_is_zero, int skip, int mixed) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (skipped(I, skip)) {        // the caller's outer iteration has already converged: no solve
     if (bx_ == 0 && threadIdx.x == 0) {
       K->done = 1; K->converged = 1; K->jlast = -1; K->steps = 0; K->outer = outer; K->outer_conv = 1;
@@ -363,8 +363,8 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
   typedef OpSel<OPK, double> S;
   constexpr int NC = S::NC;
   const typename S::Op op = S::make(I, P, ids);
-  const double* __restrict__ x = I.buf[ix];
-  const double* __restrict__ b = I.buf[ib];
+  const double* __restrict__ x = gmem(I.buf[ix]);
+  const double* __restrict__ b = gmem(I.buf[ib]);
   VT* __restrict__ w = KV<VT>::w(I);
   __shared__ typename S::Smem sm;
   __shared__ double red[8];
@@ -400,11 +400,11 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
   }
   block_sums<2>(acc, 2, red, res);
   __syncthreads();
-  if (threadIdx.x < 2) I.partial[bx_ * (MAXM + 2) + threadIdx.x] = res[threadIdx.x];
+  if (threadIdx.x < 2) gmem(I.partial)[bx_ * (MAXM + 2) + threadIdx.x] = res[threadIdx.x];
 }
 
 // local block count of instance i in the map of the kernel whose partials a finish reduces
-#define SRC_BLOCKS(src) ((src).pre[inst_ + 1] - (src).pre[inst_])
+#define SRC_BLOCKS(src) bmap_count(src, inst_)
 
 // One-block finish of the residual norm: gam0, tolerance, convergence (implicit_operators.cpp:829-836).
 // Mixed form: the solve is converged when the FP64 residual is below outer_tol ||M^-1 b|| +
@@ -413,11 +413,11 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const Inst* __restri
                                                             int skip, int mixed, double outer_tol, double inner_tol) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (skipped(I, skip)) return;
   if (mixed && outer > 0 && K->outer_conv) return;
   __shared__ double tot[2];
-  final_sums<2>(I.partial, MAXM + 2, SRC_BLOCKS(src), 2, tot);
+  final_sums<2>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 2, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double g0 = sqrt(tot[0]);
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK ==
 k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (K->done) return;
   typedef OpSel<OPK, T> S;
   constexpr int NC = S::NC;
@@ -503,7 +503,7 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict_
     const int nw = blockDim.x >> 6;
     double s = 0.0;
     for (int ww = 0; ww < nw; ww++) s += wsum[ww][threadIdx.x];
-    I.partial[bx_ * (MAXM + 2) + threadIdx.x] = s;
+    gmem(I.partial)[bx_ * (MAXM + 2) + threadIdx.x] = s;
   }
 }
 
@@ -512,10 +512,10 @@ template <int JB>
 __global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (K->done) return;
   __shared__ double tot[MAXM + 1];
-  final_sums<JB>(I.partial, MAXM + 2, SRC_BLOCKS(src), j + 1, tot);
+  final_sums<JB>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), j + 1, tot);
   __syncthreads();
   if ((int)threadIdx.x <= j) K->H[threadIdx.x * (MAXM + 1) + j] = tot[threadIdx.x];
 }
@@ -526,7 +526,7 @@ template <int JB, class VT>
 __global__ void __launch_bounds__(256) k_arnoldi_update(const Inst* __restrict__ IT, BMap bm, int j, int m, int nc) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (K->done) return;
   const long long n = (long long)nc * I.d.N;
   VT* __restrict__ w = KV<VT>::w(I);
@@ -550,7 +550,7 @@ __global__ void __launch_bounds__(256) k_arnoldi_update(const Inst* __restrict__
     acc += (double)ws * (double)ws;
   }
   block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) I.partial[bx_ * (MAXM + 2)] = res[0];
+  if (threadIdx.x == 0) gmem(I.partial)[bx_ * (MAXM + 2)] = res[0];
 }
 
 // One-block finish of the update: H[j+1][j] = ||w||, Givens rotations, convergence (implicit_operators.cpp:857-897).
@@ -560,10 +560,10 @@ __global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const Inst* __restri
                                                             int mixed) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (K->done) return;
   __shared__ double tot[1];
-  final_sums<1>(I.partial, MAXM + 2, SRC_BLOCKS(src), 1, tot);
+  final_sums<1>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     const int L = MAXM + 1;
@@ -599,7 +599,7 @@ template <class VT>
 __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ IT, BMap bm, int ix, int nc) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   __shared__ double alpha[MAXM + 1];
   const int jl = K->jlast;
   if (jl < 0) return;
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ 
     }
   }
   __syncthreads();
-  double* __restrict__ x = I.buf[ix];
+  double* __restrict__ x = gmem(I.buf[ix]);
   const long long n = (long long)nc * I.d.N;
   for (long long e = bx_ * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gx_ * blockDim.x) {
     double xv = x[e];
@@ -630,8 +630,8 @@ k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict_
   typedef OpSel<OPK, double> S;
   constexpr int NC = S::NC;
   const typename S::Op op = S::make(I, P, ids);
-  const double* __restrict__ x = I.buf[ix];
-  double* __restrict__ Ax = I.buf[iy];
+  const double* __restrict__ x = gmem(I.buf[ix]);
+  double* __restrict__ Ax = gmem(I.buf[iy]);
   __shared__ typename S::Smem sm;
   const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
@@ -656,11 +656,11 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
                                                          int ib, int it, double ntol) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  if (it > 0 && I.K->newton_conv) return;     // an earlier iteration of this Newton solve has converged
+  if (it > 0 && gmem(I.K)->newton_conv) return;     // an earlier iteration of this Newton solve has converged
   const POp op = make_p_op(I, P, ids);
-  const double* __restrict__ p = I.buf[ip];
-  const double* __restrict__ b = I.buf[ib];
-  double* __restrict__ ng = I.buf[B_NG];
+  const double* __restrict__ p = gmem(I.buf[ip]);
+  const double* __restrict__ b = gmem(I.buf[ib]);
+  double* __restrict__ ng = gmem(I.buf[B_NG]);
   __shared__ double xs[(TY + 2) * (TX + 2)];
   __shared__ double red[4];
   __shared__ double res[1];
@@ -688,18 +688,18 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
       acc += g * g;
     }
   }
-  if (ef) atomicOr(I.eflag, 1);
+  if (ef) atomicOr(gmem(I.eflag), 1);
   block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) I.partial[bx_ * (MAXM + 2)] = res[0];
+  if (threadIdx.x == 0) gmem(I.partial)[bx_ * (MAXM + 2)] = res[0];
 }
 
 __global__ void __launch_bounds__(256) k_newton_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int it, double ntol) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  KrylovDev* K = I.K;
+  KrylovDev* K = gmem(I.K);
   if (it > 0 && K->newton_conv) return;
   __shared__ double tot[1];
-  final_sums<1>(I.partial, MAXM + 2, SRC_BLOCKS(src), 1, tot);
+  final_sums<1>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = sqrt(tot[0]);
@@ -716,8 +716,8 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Phys P, const Inst* __restr
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   const Dims& d = I.d;
-  const double* __restrict__ b = I.buf[ib];
-  const double* __restrict__ u = I.buf[iu];
+  const double* __restrict__ b = gmem(I.buf[ib]);
+  const double* __restrict__ u = gmem(I.buf[iu]);
   __shared__ double sh[8];
   double bmin = 1e300, vmax = -1e300;
   int ef = 0;
@@ -729,7 +729,7 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Phys P, const Inst* __restr
       if (isfinite(v)) vmax = fmax(vmax, v);
     }
   }
-  if (ef) atomicOr(I.eflag, 1);
+  if (ef) atomicOr(gmem(I.eflag), 1);
   bmin = wave_min(bmin); vmax = wave_max(vmax);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) { sh[threadIdx.x >> 6] = bmin; sh[4 + (threadIdx.x >> 6)] = vmax; }
@@ -737,7 +737,7 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Phys P, const Inst* __restr
   if (threadIdx.x == 0) {
     double a = sh[0], c = sh[4];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sh[w]); c = fmax(c, sh[4 + w]); }
-    I.partial[bx_ * 2] = a; I.partial[bx_ * 2 + 1] = c;
+    gmem(I.partial)[bx_ * 2] = a; gmem(I.partial)[bx_ * 2 + 1] = c;
   }
 }
 
@@ -747,14 +747,14 @@ __global__ void k_newton_mbig_fin(Phys P, const Inst* __restrict__ IT, BMap bm, 
   const int nblk = SRC_BLOCKS(src);
   // min and max are order independent: one wave reduces the block partials
   double a = 1e300, c = -1e300;
-  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, I.partial[bl * 2]); c = fmax(c, I.partial[bl * 2 + 1]); }
+  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, gmem(I.partial)[bl * 2]); c = fmax(c, gmem(I.partial)[bl * 2 + 1]); }
   a = wave_min(a); c = wave_max(c);
   if (threadIdx.x == 0) {
     double Vpe = P.p_floor / (P.gamma - 1.0);
     if (P.eos != IMEX_EOS_IDEAL && c > Vpe) Vpe = c;
     double M_cor = fmax(0.0, Vpe - a) / ((1.0 - P.theta) * P.p_floor);
     double M_base = P.mbig_fac / (P.gamma - 1.0);
-    I.S[S_MBIG] = fmax(M_base, M_cor);
+    gmem(I.S)[S_MBIG] = fmax(M_base, M_cor);
   }
 }
 
@@ -766,8 +766,8 @@ __global__ void k_vslope(Phys P, const Inst* __restrict__ IT, BMap bm, int ipf, 
   if (e >= d.N) return;
   int i = (int)(e % d.nx) + GC, j = (int)(e / d.nx) + GC;
   int ef = 0;
-  I.buf[B_VSLOPE][e] = V_slope(P, I.buf[ipf][e], I.buf[iu][pidx(d, i, j)], I.S[S_MBIG], &ef);
-  if (ef) atomicOr(I.eflag, 1);
+  gmem(I.buf[B_VSLOPE])[e] = V_slope(P, gmem(I.buf[ipf])[e], gmem(I.buf[iu])[pidx(d, i, j)], gmem(I.S)[S_MBIG], &ef);
+  if (ef) atomicOr(gmem(I.eflag), 1);
 }
 
 __global__ void k_axpy1(const Inst* __restrict__ IT, BMap bm, int iy, int ix, int skip) {
@@ -775,7 +775,7 @@ __global__ void k_axpy1(const Inst* __restrict__ IT, BMap bm, int iy, int ix, in
   const Inst& I = IT[inst_];
   if (skipped(I, skip)) return;
   long long e = bx_ * (long long)blockDim.x + threadIdx.x;
-  if (e < I.d.N) I.buf[iy][e] += I.buf[ix][e];
+  if (e < I.d.N) gmem(I.buf[iy])[e] += gmem(I.buf[ix])[e];
 }
 
 __global__ void k_fill_n(const Inst* __restrict__ IT, BMap bm, int ib, double v, int skip) {
@@ -783,7 +783,7 @@ __global__ void k_fill_n(const Inst* __restrict__ IT, BMap bm, int ib, double v,
   const Inst& I = IT[inst_];
   if (skipped(I, skip)) return;
   long long e = bx_ * (long long)blockDim.x + threadIdx.x;
-  if (e < I.d.N) I.buf[ib][e] = v;
+  if (e < I.d.N) gmem(I.buf[ib])[e] = v;
 }
 
 // Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
@@ -791,7 +791,7 @@ __global__ void k_fill_n(const Inst* __restrict__ IT, BMap bm, int ib, double v,
 __global__ void k_pstats(const Inst* __restrict__ IT, BMap bm, int ip, double pfloor) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  const double* __restrict__ p = I.buf[ip];
+  const double* __restrict__ p = gmem(I.buf[ip]);
   const long long n = I.d.N;
   __shared__ double smin[4], scnt[4];
   double mn = 1e300, cnt = 0;
@@ -805,8 +805,8 @@ __global__ void k_pstats(const Inst* __restrict__ IT, BMap bm, int ip, double pf
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mn = fmin(mn, smin[w]); cnt += scnt[w]; }
-    I.partial[2 * bx_] = smin[0] < mn ? smin[0] : mn;
-    I.partial[2 * bx_ + 1] = cnt;
+    gmem(I.partial)[2 * bx_] = smin[0] < mn ? smin[0] : mn;
+    gmem(I.partial)[2 * bx_ + 1] = cnt;
   }
 }
 __global__ void k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, double thr) {
@@ -815,10 +815,10 @@ __global__ void k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, dou
   const int nblk = SRC_BLOCKS(src);
   // min, and a sum of small integers held in doubles: both exact in any order, one wave reduces
   double m = 1e300, c = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, I.partial[2 * b]); c += I.partial[2 * b + 1]; }
+  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, gmem(I.partial)[2 * b]); c += gmem(I.partial)[2 * b + 1]; }
   m = wave_min(m); c = wave_sum(c);
   if (threadIdx.x == 0) {
-    double* acc = I.pst;
+    double* acc = gmem(I.pst);
     acc[0] = fmin(acc[0], m);
     acc[1] = fmax(acc[1], c);
     if (m < thr) acc[2] = 1.0;

@@ -60,7 +60,7 @@ __global__ void k_bc_fill(const Inst* __restrict__ IT, BMap bm, FillPlanes fp, i
   int si = fill_mx(d, kind, i), sj = fill_my(d, kind, j);
   long long dst = pidx(d, i, j), src = pidx(d, si, sj);
   for (int p = 0; p < fp.n; p++) {
-    double* a = I_.buf[fp.b[p]] + fp.k[p] * d.PS;
+    double* a = gmem(I_.buf[fp.b[p]]) + fp.k[p] * d.PS;
     a[dst] = a[src];
   }
 }
@@ -75,10 +75,10 @@ __global__ void __launch_bounds__(256) k_step_start(Phys P, const Inst* __restri
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ uin = I_.buf[iin];
-  double* __restrict__ un = I_.buf[B_UN];
-  double* partial = I_.partial;
-  int* eflag = I_.eflag;
+  const double* __restrict__ uin = gmem(I_.buf[iin]);
+  double* __restrict__ un = gmem(I_.buf[B_UN]);
+  double* partial = gmem(I_.partial);
+  int* eflag = gmem(I_.eflag);
 
   __shared__ double sh[16];
   double r[8] = {1e-8, 1e-8, 1e-8, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -132,9 +132,9 @@ __global__ void k_step_finish(Phys P, const Inst* __restrict__ IT, BMap bm, BMap
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* partial = I_.partial;
-  const int nblk = src.pre[inst_ + 1] - src.pre[inst_];
-  double* S = I_.S;
+  const double* partial = gmem(I_.partial);
+  const int nblk = bmap_count(src, inst_);
+  double* S = gmem(I_.S);
 
   __shared__ double mm[8];
   {   // 8 maxima over the block partials, one wave per value (max is order independent)
@@ -177,17 +177,17 @@ __global__ void k_stage_combine(const Inst* __restrict__ IT, BMap bm, int ibxn, 
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ un = I_.buf[B_UN];
-  const double* __restrict__ u1 = I_.buf[B_U1];
-  double* __restrict__ Qe = I_.buf[B_QE];
-  double* __restrict__ Qs = I_.buf[B_QS];
-  const double* __restrict__ bxn = I_.buf[ibxn];
-  const double* __restrict__ byn = I_.buf[ibyn];
-  const double* __restrict__ bx1 = I_.buf[B_BX1];
-  const double* __restrict__ by1 = I_.buf[B_BY1];
-  double* __restrict__ bxqs = I_.buf[B_BXQS];
-  double* __restrict__ byqs = I_.buf[B_BYQS];
-  const double* S = I_.S;
+  const double* __restrict__ un = gmem(I_.buf[B_UN]);
+  const double* __restrict__ u1 = gmem(I_.buf[B_U1]);
+  double* __restrict__ Qe = gmem(I_.buf[B_QE]);
+  double* __restrict__ Qs = gmem(I_.buf[B_QS]);
+  const double* __restrict__ bxn = gmem(I_.buf[ibxn]);
+  const double* __restrict__ byn = gmem(I_.buf[ibyn]);
+  const double* __restrict__ bx1 = gmem(I_.buf[B_BX1]);
+  const double* __restrict__ by1 = gmem(I_.buf[B_BY1]);
+  double* __restrict__ bxqs = gmem(I_.buf[B_BXQS]);
+  double* __restrict__ byqs = gmem(I_.buf[B_BYQS]);
+  const double* S = gmem(I_.S);
 
   long long total = (long long)d.NX * d.NY;
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
@@ -213,10 +213,10 @@ __global__ void k_qe_prep(Phys P, const Inst* __restrict__ IT, BMap bm, int iQe)
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ Qe = I_.buf[iQe];
-  double* __restrict__ pe = I_.buf[B_PE];
-  double* __restrict__ cf = I_.buf[B_CF];
-  int* eflag = I_.eflag;
+  const double* __restrict__ Qe = gmem(I_.buf[iQe]);
+  double* __restrict__ pe = gmem(I_.buf[B_PE]);
+  double* __restrict__ cf = gmem(I_.buf[B_CF]);
+  int* eflag = gmem(I_.eflag);
 
   long long total = (long long)d.NX * d.NY;
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
@@ -239,9 +239,9 @@ __global__ void k_shock_chi(Phys P, const Inst* __restrict__ IT, BMap bm, int iQ
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ Qe = I_.buf[iQe];
-  const double* __restrict__ cf = I_.buf[B_CF];
-  double* __restrict__ chi = I_.buf[B_CHI];
+  const double* __restrict__ Qe = gmem(I_.buf[iQe]);
+  const double* __restrict__ cf = gmem(I_.buf[B_CF]);
+  double* __restrict__ chi = gmem(I_.buf[B_CHI]);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
@@ -266,8 +266,8 @@ __global__ void k_shock_eta(const Inst* __restrict__ IT, BMap bm) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ chi = I_.buf[B_CHI];
-  double* __restrict__ eta = I_.buf[B_ETA];
+  const double* __restrict__ chi = gmem(I_.buf[B_CHI]);
+  double* __restrict__ eta = gmem(I_.buf[B_ETA]);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
@@ -470,10 +470,10 @@ __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims d = I_.d;
-  const double* __restrict__ Qe = I_.buf[iQe];
-  const double* __restrict__ Qs = I_.buf[iQs];
-  double* __restrict__ u = I_.buf[iu];
-  const double* S = I_.S;
+  const double* __restrict__ Qe = gmem(I_.buf[iQe]);
+  const double* __restrict__ Qs = gmem(I_.buf[iQs]);
+  double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* S = gmem(I_.S);
 
   __shared__ T wl[5 * EX_NW], wr[5 * EX_NW];
   __shared__ T F[5 * EX_NF];
@@ -493,15 +493,15 @@ __global__ void __launch_bounds__(256) k_mag_rhs(Phys P, const Inst* __restrict_
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ u = I_.buf[iu];
-  const double* __restrict__ Qe = I_.buf[iQe];
-  const double* __restrict__ Qs = I_.buf[iQs];
-  const double* __restrict__ pe = I_.buf[B_PE];
-  double* __restrict__ b = I_.buf[ib];
-  double* __restrict__ x0 = I_.buf[ix0];
-  double* __restrict__ hhat = I_.buf[B_HHAT];
-  const double* S = I_.S;
-  int* eflag = I_.eflag;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* __restrict__ Qe = gmem(I_.buf[iQe]);
+  const double* __restrict__ Qs = gmem(I_.buf[iQs]);
+  const double* __restrict__ pe = gmem(I_.buf[B_PE]);
+  double* __restrict__ b = gmem(I_.buf[ib]);
+  double* __restrict__ x0 = gmem(I_.buf[ix0]);
+  double* __restrict__ hhat = gmem(I_.buf[B_HHAT]);
+  const double* S = gmem(I_.S);
+  int* eflag = gmem(I_.eflag);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
@@ -548,8 +548,8 @@ __global__ void k_mag_writeback(const Inst* __restrict__ IT, BMap bm, int iu) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ x = I_.buf[B_XK];
-  double* __restrict__ u = I_.buf[iu];
+  const double* __restrict__ x = gmem(I_.buf[B_XK]);
+  double* __restrict__ u = gmem(I_.buf[iu]);
 
   long long total = (long long)d.NX * d.NY;
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
@@ -567,12 +567,12 @@ __global__ void __launch_bounds__(256) k_emf_flux(const Inst* __restrict__ IT, B
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ u = I_.buf[iu];
-  const double* __restrict__ Qe = I_.buf[iQe];
-  const double* __restrict__ pe = I_.buf[B_PE];
-  double* __restrict__ Fx = I_.buf[B_FX];
-  double* __restrict__ Fy = I_.buf[B_FY];
-  const double* S = I_.S;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* __restrict__ Qe = gmem(I_.buf[iQe]);
+  const double* __restrict__ pe = gmem(I_.buf[B_PE]);
+  double* __restrict__ Fx = gmem(I_.buf[B_FX]);
+  double* __restrict__ Fy = gmem(I_.buf[B_FY]);
+  const double* S = gmem(I_.S);
 
   int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
@@ -626,15 +626,15 @@ __global__ void __launch_bounds__(256) k_p_rhs(const Inst* __restrict__ IT, BMap
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ u = I_.buf[iu];
-  const double* __restrict__ hhat = I_.buf[B_HHAT];
-  double* __restrict__ mxs = I_.buf[B_MXS];
-  double* __restrict__ mys = I_.buf[B_MYS];
-  const double* __restrict__ pe = I_.buf[B_PE];
-  double* __restrict__ b = I_.buf[B_PB];
-  double* __restrict__ p0 = I_.buf[B_P];
-  double* __restrict__ Tdiag = I_.buf[B_TDIAG];
-  const double* S = I_.S;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* __restrict__ hhat = gmem(I_.buf[B_HHAT]);
+  double* __restrict__ mxs = gmem(I_.buf[B_MXS]);
+  double* __restrict__ mys = gmem(I_.buf[B_MYS]);
+  const double* __restrict__ pe = gmem(I_.buf[B_PE]);
+  double* __restrict__ b = gmem(I_.buf[B_PB]);
+  double* __restrict__ p0 = gmem(I_.buf[B_P]);
+  double* __restrict__ Tdiag = gmem(I_.buf[B_TDIAG]);
+  const double* S = gmem(I_.S);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
@@ -667,11 +667,11 @@ __global__ void k_p_momentum(const Inst* __restrict__ IT, BMap bm, int iu) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ p = I_.buf[B_P];
-  const double* __restrict__ mxs = I_.buf[B_MXS];
-  const double* __restrict__ mys = I_.buf[B_MYS];
-  double* __restrict__ u = I_.buf[iu];
-  const double* S = I_.S;
+  const double* __restrict__ p = gmem(I_.buf[B_P]);
+  const double* __restrict__ mxs = gmem(I_.buf[B_MXS]);
+  const double* __restrict__ mys = gmem(I_.buf[B_MYS]);
+  double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* S = gmem(I_.S);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
@@ -689,9 +689,9 @@ __global__ void k_p_energy(Phys P, const Inst* __restrict__ IT, BMap bm, int iu)
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ p = I_.buf[B_P];
-  double* __restrict__ u = I_.buf[iu];
-  int* eflag = I_.eflag;
+  const double* __restrict__ p = gmem(I_.buf[B_P]);
+  double* __restrict__ u = gmem(I_.buf[iu]);
+  int* eflag = gmem(I_.eflag);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
@@ -717,10 +717,10 @@ __global__ void k_ct_corner(const Inst* __restrict__ IT, BMap bm, int iex, int i
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ ezx = I_.buf[iex];
-  const double* __restrict__ ezy = I_.buf[iey];
-  const double* __restrict__ u = I_.buf[iu];
-  double* __restrict__ Ezpp = I_.buf[B_EZPP];
+  const double* __restrict__ ezx = gmem(I_.buf[iex]);
+  const double* __restrict__ ezy = gmem(I_.buf[iey]);
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  double* __restrict__ Ezpp = gmem(I_.buf[B_EZPP]);
 
   // Ez_p0 = sx * ezx, Ez_0p = sy * ezy (sx = -1, sy = +1 for the EMF-consistent form)
   int i = bx_ * blockDim.x + threadIdx.x + GC - 2, j = by_ * blockDim.y + threadIdx.y + GC - 2;
@@ -754,13 +754,13 @@ __global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int 
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ bxqs = I_.buf[ibxqs];
-  const double* __restrict__ byqs = I_.buf[ibyqs];
-  const double* __restrict__ Ezpp = I_.buf[B_EZPP];
-  double* __restrict__ bx = I_.buf[ibx];
-  double* __restrict__ by = I_.buf[iby];
-  double* __restrict__ u = I_.buf[iu];
-  const double* S = I_.S;
+  const double* __restrict__ bxqs = gmem(I_.buf[ibxqs]);
+  const double* __restrict__ byqs = gmem(I_.buf[ibyqs]);
+  const double* __restrict__ Ezpp = gmem(I_.buf[B_EZPP]);
+  double* __restrict__ bx = gmem(I_.buf[ibx]);
+  double* __restrict__ by = gmem(I_.buf[iby]);
+  double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* S = gmem(I_.S);
 
   int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
   if (i >= d.NX || j >= d.NY) return;
@@ -781,9 +781,9 @@ __global__ void __launch_bounds__(256) k_ct_plain_emf(Phys P, const Inst* __rest
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ u = I_.buf[iu];
-  double* __restrict__ ezx = I_.buf[B_EZX];
-  double* __restrict__ ezy = I_.buf[B_EZY];
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  double* __restrict__ ezx = gmem(I_.buf[B_EZX]);
+  double* __restrict__ ezy = gmem(I_.buf[B_EZY]);
 
   int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
@@ -812,9 +812,9 @@ __global__ void k_con2pri(Phys P, const Inst* __restrict__ IT, BMap bm, int icon
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ con = I_.buf[icon];
-  double* __restrict__ pri = I_.buf[ipri];
-  int* eflag = I_.eflag;
+  const double* __restrict__ con = gmem(I_.buf[icon]);
+  double* __restrict__ pri = gmem(I_.buf[ipri]);
+  int* eflag = gmem(I_.eflag);
 
   long long total = (long long)d.NX * d.NY;
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
@@ -830,9 +830,9 @@ __global__ void k_pri2con(Phys P, const Inst* __restrict__ IT, BMap bm, int ipri
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ pri = I_.buf[ipri];
-  double* __restrict__ con = I_.buf[icon];
-  int* eflag = I_.eflag;
+  const double* __restrict__ pri = gmem(I_.buf[ipri]);
+  double* __restrict__ con = gmem(I_.buf[icon]);
+  int* eflag = gmem(I_.eflag);
 
   long long total = (long long)d.NX * d.NY;
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
@@ -849,9 +849,9 @@ __global__ void k_divb(const Inst* __restrict__ IT, BMap bm, int ibx, int iby) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
-  const double* __restrict__ bx = I_.buf[ibx];
-  const double* __restrict__ by = I_.buf[iby];
-  double* partial = I_.partial;
+  const double* __restrict__ bx = gmem(I_.buf[ibx]);
+  const double* __restrict__ by = gmem(I_.buf[iby]);
+  double* partial = gmem(I_.partial);
 
   __shared__ double sh[16];
   double acc = 0.0;
@@ -873,8 +873,8 @@ __global__ void k_visc_rk(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) 
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= (long long)d.NX * d.NY) return;
   int ef = 0;
-  I_.buf[B_VISC][c] = Vbase_slope(P, I_.buf[B_PE][c], I_.buf[iu][c], &ef);
-  if (ef) atomicOr(I_.eflag, 1);
+  gmem(I_.buf[B_VISC])[c] = Vbase_slope(P, gmem(I_.buf[B_PE])[c], gmem(I_.buf[iu])[c], &ef);
+  if (ef) atomicOr(gmem(I_.eflag), 1);
 }
 
 // ================================================================ launchers
