@@ -230,6 +230,8 @@ def main():
                     help="override the mixed form's inner / outer refinement tolerances")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary workloads (blast 1024^2 FP64 vs mixed; the 256-vortex batches)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the 256-vortex batches and print their results (for profiling)")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--blast-steps", type=int, default=20)
@@ -246,6 +248,11 @@ def main():
     import numpy as np
     import imex_amd as im
     sched = {"graph": im.SCHED_GRAPH, "budget": im.SCHED_BUDGET, "host": im.SCHED_HOST}[args.schedule]
+    if args.batch_only:
+        res = {e: run_batch(args.batch_instances, args.batch_steps, 1, eos, 2024, sched)
+               for e, eos in (("ideal", im.EOS_IDEAL), ("redlich_kwong", im.EOS_RK))}
+        print(json.dumps({"batch": res}), flush=True)
+        return
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {}
     for f in forms:
