@@ -157,6 +157,81 @@ def run_batch(n_inst, steps, warmup, eos, seed, schedule):
             "max_div_b": st["div_b"], "device_gb": round(dev_gb, 2), "setup_s": round(t_setup, 1)}
 
 
+def run_adjoint(n, steps, every, ndir, seed, eps=1e-6, noise=1e-3):
+    """Discrete adjoint of `steps` FP64 steps of the n^2 MHD vortex, J = total kinetic energy at the
+    final step, against central differences along `ndir` seeded directions."""
+    import numpy as np
+    import imex_amd as im
+    G = im.GC
+    p, kw = im.case_of(f"vortex{n}")
+    u, bx, by = im.init_case(p, kw["case"], kw["amp"], kw["u0"], kw["v0"])
+    r = np.random.default_rng(seed + 1000)
+    u[G:-G, G:-G, :] *= 1 + noise * r.standard_normal(u[G:-G, G:-G, :].shape)
+    bx[G:-G, G:-G] *= 1 + noise * r.standard_normal(bx[G:-G, G:-G].shape)
+    by[G:-G, G:-G] *= 1 + noise * r.standard_normal(by[G:-G, G:-G].shape)
+    dx, dy = (p.x1 - p.x0) / p.nx, (p.y1 - p.y0) / p.ny
+
+    def kinetic(uc):
+        ui = uc[G:-G, G:-G]
+        rho, m = ui[..., 0], ui[..., 1:4]
+        m2 = (m * m).sum(-1)
+        g = np.zeros_like(uc)
+        g[G:-G, G:-G, 0] = -0.5 * m2 / rho ** 2 * dx * dy
+        g[G:-G, G:-G, 1:4] = m / rho[..., None] * dx * dy
+        return float((0.5 * m2 / rho).sum() * dx * dy), g
+
+    s = im.Solver(p)
+    try:
+        def J_of(uu, bbx, bby):
+            s.set_state(uu, bbx, bby)
+            s.step(steps)
+            return kinetic(s.get_state()[0])[0]
+        J_of(u, bx, by)                          # warm-up (graph capture)
+        s.set_state(u, bx, by)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step_async(steps)
+        s.sync()
+        t_fwd = time.perf_counter() - t0
+        s.set_state(u, bx, by)
+        s.sync()
+        t0 = time.perf_counter()
+        s.adjoint_forward(steps, every)
+        s.sync()
+        t1 = time.perf_counter()
+        J, g = kinetic(s.get_state()[0])
+        t2 = time.perf_counter()
+        au, abx, aby, info = s.adjoint_backward(g)
+        t3 = time.perf_counter()
+        log(f"[bench] adjoint {n}^2, {steps} steps: forward {t_fwd * 1e3:.1f} ms, checkpointed forward "
+            f"{(t1 - t0) * 1e3:.1f} ms, backward {(t3 - t2) * 1e3:.1f} ms; {info}")
+        rng = np.random.default_rng(seed)
+        ad, fd = [], []
+        for k in range(ndir):
+            du = np.zeros_like(u); dbx = np.zeros_like(bx); dby = np.zeros_like(by)
+            du[G:-G, G:-G, :] = rng.standard_normal(du[G:-G, G:-G, :].shape) * 1e-2
+            dbx[G:-G, G:-G] = rng.standard_normal(dbx[G:-G, G:-G].shape) * 1e-2
+            dby[G:-G, G:-G] = rng.standard_normal(dby[G:-G, G:-G].shape) * 1e-2
+            f = (J_of(u + eps * du, bx + eps * dbx, by + eps * dby) - J_of(u - eps * du, bx - eps * dbx, by - eps * dby)) / (2 * eps)
+            a = float((au * du).sum() + (abx * dbx).sum() + (aby * dby).sum())
+            ad.append(a); fd.append(f)
+            log(f"[bench] adjoint direction {k}: {a:.12e} fd {f:.12e} rel {abs(a - f) / abs(f):.2e}")
+    finally:
+        s.close()
+    ad, fd = np.array(ad), np.array(fd)
+    rel = np.abs(ad - fd) / np.abs(fd)
+    return {"workload": f"MHD vortex {n}x{n} (seeded 1e-3 relative perturbation), FP64, {steps} steps, "
+                        f"J = total kinetic energy at the final step, checkpoint every {every} steps",
+            "directions": ndir, "fd_eps": eps, "direction_seed": seed,
+            "rel_error_median": float(f"{np.median(rel):.3e}"), "rel_error_max": float(f"{rel.max():.3e}"),
+            "rel_error_all_directions": float(f"{np.linalg.norm(ad - fd) / np.linalg.norm(fd):.3e}"),
+            "forward_ms": round(t_fwd * 1e3, 2), "checkpointed_forward_ms": round((t1 - t0) * 1e3, 2),
+            "backward_ms": round((t3 - t2) * 1e3, 2),
+            "adjoint_over_forward": round(((t1 - t0) + (t3 - t2)) / t_fwd, 2),
+            "transposed_solves": info["transposed_solves"], "transposed_gmres_iters": info["mag_iters"] + info["p_iters"],
+            "worst_solve_residual": info["worst_solve_residual"]}
+
+
 def cpu_baseline(name, state, budget_steps=1):
     """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
     import imex_amd as im
@@ -229,7 +304,9 @@ def main():
     ap.add_argument("--mixed-tols", type=float, nargs=2, default=None, metavar=("INNER", "OUTER"),
                     help="override the mixed form's inner / outer refinement tolerances")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the secondary workloads (blast 1024^2 FP64 vs mixed; the 256-vortex batches)")
+                    help="skip the secondary workloads (blast 1024^2 FP64 vs mixed; the 256-vortex batches; the 256^2 adjoint)")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the 256^2 adjoint check and print it")
+    ap.add_argument("--adjoint-dirs", type=int, default=32)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 256-vortex batches and print their results (for profiling)")
     ap.add_argument("--batch-instances", type=int, default=256)
@@ -248,6 +325,9 @@ def main():
     import numpy as np
     import imex_amd as im
     sched = {"graph": im.SCHED_GRAPH, "budget": im.SCHED_BUDGET, "host": im.SCHED_HOST}[args.schedule]
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint(256, 20, 5, args.adjoint_dirs, 1)}), flush=True)
+        return
     if args.batch_only:
         res = {e: run_batch(args.batch_instances, args.batch_steps, 1, eos, 2024, sched)
                for e, eos in (("ideal", im.EOS_IDEAL), ("redlich_kwong", im.EOS_RK))}
@@ -301,6 +381,7 @@ def main():
                         "steps": args.batch_steps, "warmup": 1,
                         "ideal": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_IDEAL, 2024, sched),
                         "redlich_kwong": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_RK, 2024, sched)}
+        out["adjoint"] = run_adjoint(256, 20, 5, args.adjoint_dirs, 1)
     if not args.no_cpu:
         log("[bench] CPU oracle sample")
         v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps)

@@ -71,6 +71,12 @@ class TuneInfo(C.Structure):
                 ("explicit_us", C.c_double * 3), ("krylov_us", C.c_double * 3)]
 
 
+class AdjointInfo(C.Structure):
+    _fields_ = [("steps", C.c_int), ("checkpoints", C.c_int), ("recomputed_steps", C.c_int),
+                ("transposed_solves", C.c_longlong), ("mag_iters", C.c_longlong), ("p_iters", C.c_longlong),
+                ("worst_solve_residual", C.c_double), ("backward_ms", C.c_double)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("res0", C.c_double), ("res", C.c_double),
                 ("status", C.c_int)]
@@ -188,6 +194,8 @@ def lib():
         "imex_batch_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, C.POINTER(Scalars)]),
         "imex_batch_div_b": (C.c_double, [vp, C.c_int]),
         "imex_autotune": (C.c_int, [vp, C.c_int, C.POINTER(TuneInfo)]),
+        "imex_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int]),
+        "imex_adjoint_backward": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, C.POINTER(AdjointInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -346,6 +354,27 @@ class Solver:
         info = SolveInfo()
         rc = self._chk(self.L.imex_gmres(self.h, op, ptr(b), ptr(x), precond, ptr(pf), mbig, C.byref(info)), "gmres")
         return x, struct_dict(info), rc
+
+    def apply_op_T(self, op, x, p_freeze=None, mbig=0.0):
+        out = np.zeros_like(x)
+        self._chk(self.L.imex_apply_op_T(self.h, op, ptr(np.ascontiguousarray(x)), ptr(out), ptr(p_freeze), mbig), "apply_op_T")
+        return out
+
+    def adjoint_forward(self, nsteps, every):
+        """Run nsteps from the state given to set_state, keeping a checkpoint every `every` steps."""
+        return self._chk(self.L.imex_adjoint_forward(self.h, nsteps, every), "adjoint_forward")
+
+    def adjoint_backward(self, gu_con, gbx=None, gby=None):
+        """Cotangent of the final (conservative state, faces) -> gradient w.r.t. the initial primitive state and faces."""
+        p = self.p
+        gu = np.ascontiguousarray(gu_con, dtype=np.float64)
+        gbx = None if gbx is None else np.ascontiguousarray(gbx, dtype=np.float64)
+        gby = None if gby is None else np.ascontiguousarray(gby, dtype=np.float64)
+        au, abx, aby = np.zeros(grid_shape(p)), np.zeros(array_shape(p)), np.zeros(array_shape(p))
+        info = AdjointInfo()
+        self._chk(self.L.imex_adjoint_backward(self.h, ptr(gu), ptr(gbx), ptr(gby), ptr(au), ptr(abx), ptr(aby),
+                                               C.byref(info)), "adjoint_backward")
+        return au, abx, aby, struct_dict(info)
 
     def pressure_newton(self, b, p0):
         p = np.array(p0, dtype=np.float64, copy=True)

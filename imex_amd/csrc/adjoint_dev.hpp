@@ -1,0 +1,241 @@
+// adjoint_dev.hpp — device side of the discrete adjoint of the IMEX-RK2 step.
+//
+// Scope: periodic grids, ideal gas, Minmod or VanLeer reconstruction, EMF-consistent CT, FP64.
+// On a periodic grid every ghost value is a copy of an interior value, so each field is treated
+// as a field on the torus of interior cells, read through a wrapped index (0-based interior
+// coordinates).
+//
+// Each nonlinear piece of the stage is written once more as an element function templated on
+// its scalar type: with T = double it evaluates the same formula as the forward kernels (and the
+// reference lines they cite); with T = DN<K> (a forward-mode dual number carrying K partial
+// derivatives) it yields the derivatives of the element's outputs with respect to the K inputs of
+// one seeded cell.  A vector-Jacobian product then loops, per element, over the cells of the
+// element's stencil, seeds one cell at a time and adds (output cotangent . derivative) into that
+// cell's input cotangent with atomics.  Scalars (h, a_max, B_max, v_max) get their cotangents
+// from a second pass that seeds them instead, reduced over the grid.
+#pragma once
+#include "kernels.hpp"
+
+// ---------------------------------------------------------------- dual numbers
+template <int K>
+struct DN {
+  double v, g[K];
+  __device__ DN() {}
+  __device__ DN(double x) : v(x) {
+#pragma unroll
+    for (int k = 0; k < K; k++) g[k] = 0.0;
+  }
+};
+template <class T> struct IsDual { static constexpr bool value = false; };
+template <int K> struct IsDual<DN<K>> { static constexpr bool value = true; };
+
+__device__ __forceinline__ double val(double x) { return x; }
+template <int K> __device__ __forceinline__ double val(const DN<K>& x) { return x.v; }
+
+template <int K> __device__ __forceinline__ DN<K> operator+(const DN<K>& a, const DN<K>& b) {
+  DN<K> r; r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] + b.g[k];
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator-(const DN<K>& a, const DN<K>& b) {
+  DN<K> r; r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] - b.g[k];
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator-(const DN<K>& a) {
+  DN<K> r; r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = -a.g[k];
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator*(const DN<K>& a, const DN<K>& b) {
+  DN<K> r; r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] * b.v + a.v * b.g[k];
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator/(const DN<K>& a, const DN<K>& b) {
+  DN<K> r; r.v = a.v / b.v;
+  const double ib = 1.0 / b.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = (a.g[k] - r.v * b.g[k]) * ib;
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator+(const DN<K>& a, double b) { DN<K> r = a; r.v += b; return r; }
+template <int K> __device__ __forceinline__ DN<K> operator+(double a, const DN<K>& b) { DN<K> r = b; r.v += a; return r; }
+template <int K> __device__ __forceinline__ DN<K> operator-(const DN<K>& a, double b) { DN<K> r = a; r.v -= b; return r; }
+template <int K> __device__ __forceinline__ DN<K> operator-(double a, const DN<K>& b) {
+  DN<K> r; r.v = a - b.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = -b.g[k];
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator*(const DN<K>& a, double b) {
+  DN<K> r; r.v = a.v * b;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] * b;
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator*(double a, const DN<K>& b) { return b * a; }
+template <int K> __device__ __forceinline__ DN<K> operator/(const DN<K>& a, double b) {
+  DN<K> r; r.v = a.v / b;
+  const double ib = 1.0 / b;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] * ib;
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> operator/(double a, const DN<K>& b) {
+  DN<K> r; r.v = a / b.v;
+  const double s = -r.v / b.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = b.g[k] * s;
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> sqrt(const DN<K>& a) {
+  DN<K> r; r.v = ::sqrt(a.v);
+  const double s = r.v > 0.0 ? 0.5 / r.v : 0.0;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] * s;
+  return r;
+}
+template <int K> __device__ __forceinline__ DN<K> fabs(const DN<K>& a) { return a.v < 0.0 ? -a : a; }
+
+// branch helpers that work for both scalar types
+template <class T> __device__ __forceinline__ T tmax(const T& a, const T& b) { return val(a) >= val(b) ? a : b; }
+template <class T> __device__ __forceinline__ T tmin(const T& a, const T& b) { return val(a) <= val(b) ? a : b; }
+
+// ---------------------------------------------------------------- torus geometry and fields
+struct AD {
+  int nx, ny, NX;
+  long long PS;
+  double dx, dy;
+};
+__device__ __forceinline__ int wrapi(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+__device__ __forceinline__ long long tix(const AD& a, int i, int j) {
+  return (long long)(wrapi(i, a.nx) + GC) + (long long)(wrapi(j, a.ny) + GC) * a.NX;
+}
+
+#define ADJ_NF 6
+// Input fields of an element function (plane 0 of each; component k at + k*PS) and the
+// cotangent buffers they accumulate into (same layout; null: no cotangent wanted).
+struct AFields {
+  const double* f[ADJ_NF];
+  double* bar[ADJ_NF];
+};
+
+// Step scalars as seen by the element functions.
+enum { SC_H, SC_AMAX, SC_BMX, SC_BMY, SC_VXM, SC_VYM, SC_N };
+template <class T>
+struct Sc { T s[SC_N]; };
+
+// Reader of field values for element functions: seeds the inputs of one cell (si, sj) when T is a
+// dual number; Fn::slot(fid, k) gives the derivative slot of component k of field fid (-1: none).
+template <class T, class Fn>
+struct Rd {
+  const AD* a;
+  const AFields* F;
+  int si, sj;
+  __device__ __forceinline__ T operator()(int fid, int k, int i, int j) const {
+    const int wi = wrapi(i, a->nx), wj = wrapi(j, a->ny);
+    T r(F->f[fid][k * a->PS + (long long)(wi + GC) + (long long)(wj + GC) * a->NX]);
+    if constexpr (IsDual<T>::value) {
+      if (wi == si && wj == sj) {
+        const int s = Fn::slot(fid, k);
+        if (s >= 0) r.g[s] = 1.0;
+      }
+    }
+    return r;
+  }
+};
+
+// ---------------------------------------------------------------- generic kernels
+// Input cotangents: for every element e (interior cell (i, j)) with output cotangents obar[o]
+// (plane o of `obar`), and every cell c of its stencil, bar[c] += sum_o obar[o] d out_o / d in_c.
+template <class Fn>
+__global__ void __launch_bounds__(128) k_vjp(Fn fn, AD a, AFields F, const double* __restrict__ obar, Sc<double> sc) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.nx * a.ny) return;
+  const int i = n % a.nx, j = n / a.nx;
+  const long long e = tix(a, i, j);
+  double ob[Fn::NOUT];
+  bool any = false;
+#pragma unroll
+  for (int o = 0; o < Fn::NOUT; o++) { ob[o] = obar[o * a.PS + e]; any |= ob[o] != 0.0; }
+  if (!any) return;
+  typedef DN<Fn::K> T;
+  Sc<T> st;
+#pragma unroll
+  for (int q = 0; q < SC_N; q++) st.s[q] = T(sc.s[q]);
+  for (int s = 0; s < Fn::NS; s++) {
+    int oi, oj;
+    Fn::off(s, oi, oj);
+    Rd<T, Fn> rd{&a, &F, wrapi(i + oi, a.nx), wrapi(j + oj, a.ny)};
+    T out[Fn::NOUT];
+    fn.eval(rd, st, i, j, out);
+    const long long c = tix(a, i + oi, j + oj);
+    for (int q = 0; q < Fn::K; q++) {
+      double g = 0.0;
+#pragma unroll
+      for (int o = 0; o < Fn::NOUT; o++) g += ob[o] * out[o].g[q];
+      if (g != 0.0) {
+        int fid, k;
+        Fn::which(q, fid, k);
+        double* b = F.bar[fid];
+        if (b) atomicAdd(b + k * a.PS + c, g);
+      }
+    }
+  }
+}
+
+// Scalar cotangents: sbar[q] += sum_e sum_o obar[o] d out_o / d scalar_q.
+template <class Fn>
+__global__ void __launch_bounds__(128) k_vjp_scalars(Fn fn, AD a, AFields F, const double* __restrict__ obar, Sc<double> sc,
+                                                     double* sbar) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[SC_N];
+#pragma unroll
+  for (int q = 0; q < SC_N; q++) acc[q] = 0.0;
+  if (n < a.nx * a.ny) {
+    const int i = n % a.nx, j = n / a.nx;
+    const long long e = tix(a, i, j);
+    typedef DN<SC_N> T;
+    Sc<T> st;
+#pragma unroll
+    for (int q = 0; q < SC_N; q++) { st.s[q] = T(sc.s[q]); st.s[q].g[q] = 1.0; }
+    Rd<T, Fn> rd{&a, &F, -1, -1};
+    T out[Fn::NOUT];
+    fn.eval(rd, st, i, j, out);
+#pragma unroll
+    for (int o = 0; o < Fn::NOUT; o++) {
+      const double ob = obar[o * a.PS + e];
+#pragma unroll
+      for (int q = 0; q < SC_N; q++) acc[q] += ob * out[o].g[q];
+    }
+  }
+  __shared__ double sh[SC_N][2];
+  for (int q = 0; q < SC_N; q++) {
+    double v = wave_sum(acc[q]);
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < SC_N) {
+    double v = sh[threadIdx.x][0] + sh[threadIdx.x][1];
+    if (v != 0.0) atomicAdd(sbar + threadIdx.x, v);
+  }
+}
+
+// Plain evaluation (T = double) into NOUT planes of `out`.
+template <class Fn>
+__global__ void __launch_bounds__(128) k_eval(Fn fn, AD a, AFields F, double* __restrict__ out, Sc<double> sc) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.nx * a.ny) return;
+  const int i = n % a.nx, j = n / a.nx;
+  Rd<double, Fn> rd{&a, &F, -1, -1};
+  double o[Fn::NOUT];
+  fn.eval(rd, sc, i, j, o);
+  const long long e = tix(a, i, j);
+#pragma unroll
+  for (int q = 0; q < Fn::NOUT; q++) out[q * a.PS + e] = o[q];
+}

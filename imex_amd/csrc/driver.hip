@@ -17,6 +17,7 @@
 #include <vector>
 #include <algorithm>
 #include "kernels.hpp"
+#include "adjoint.hpp"
 
 static thread_local std::string g_err;
 static int set_err(int code, const std::string& m) { g_err = m; return code; }
@@ -148,6 +149,13 @@ struct GraphEntry {
 // buffer ids of the stage being queued
 struct StageBind { int u, Qe, Qs, bxqs, byqs, bxo, byo; };
 
+// Points of a host-scheduled step at which a recording hook (the adjoint's tape) is called.
+enum HookPoint { HK_START, HK_IN, HK_CONV, HK_MAG, HK_PRESS, HK_P0, HK_MOM, HK_P1, HK_PR, HK_CT };
+struct imex_ctx;
+typedef void (*StepHook)(imex_ctx*, int);
+struct AdjState;
+static void adj_free(imex_ctx* c);
+
 struct imex_ctx {
   Prof prof;
   imex_params prm;             // shared parameters (nx, ny of instance 0)
@@ -208,6 +216,9 @@ struct imex_ctx {
   bool tag_err = false;
   bool no_toggle = false;      // node enabling failed once: graphs are captured per exact budget table
   std::vector<NodeTag> tags;
+  StepHook hook = nullptr;     // called at HookPoint's while a step is queued host-scheduled
+  bool taping = false;         // steps are host-scheduled (the hook sees every point)
+  AdjState* adj = nullptr;     // discrete-adjoint checkpoints, tape and work buffers
   const Dims& d0() const { return hI[0].d; }
 };
 
@@ -716,13 +727,17 @@ static int sub_pressure(imex_ctx* c) {
   fill_planes(c, {{B_MXS, 0}, {B_MYS, 0}});
   c->sub_idx = 1;
   int rc = run_newton(c, B_PB, B_P, nullptr);
+  if (c->hook) c->hook(c, HK_P0);
   LAUNCH(KC_P_MOM, 5, launch_p_momentum(c->L, u));
   fill_planes(c, {{u, 1}, {u, 2}});
+  if (c->hook) c->hook(c, HK_MOM);
   LAUNCH(KC_P_RHS, 12, launch_p_rhs(c->L, 1, u));
   c->sub_idx = 2;
   rc |= run_newton(c, B_PB, B_P, nullptr);
+  if (c->hook) c->hook(c, HK_P1);
   LAUNCH(KC_P_ENERGY, 9, launch_p_energy(c->L, c->P, u));
   fill_planes(c, {{u, 4}});
+  if (c->hook) c->hook(c, HK_PR);
   return rc;
 }
 
@@ -735,11 +750,16 @@ static int run_stage(imex_ctx* c) {
   int rc = 0;
   c->qe_ready = false; c->hhat_ready = false;
   ensure_qe(c);
+  if (c->hook) c->hook(c, HK_IN);
   rc |= sub_explicit_conv(c);
+  if (c->hook) c->hook(c, HK_CONV);
   rc |= sub_mag_update_b(c);
+  if (c->hook) c->hook(c, HK_MAG);
   rc |= sub_explicit_press(c);
+  if (c->hook) c->hook(c, HK_PRESS);
   rc |= sub_pressure(c);
   rc |= sub_ct(c);
+  if (c->hook) c->hook(c, HK_CT);
   return rc;
 }
 
@@ -765,6 +785,7 @@ static int step_start(imex_ctx* c) {
 static int step_body(imex_ctx* c) {
   int rc = 0;
   step_start(c);
+  if (c->hook) c->hook(c, HK_START);
   bind_stage0(c);
   rc |= run_stage(c);
   LAUNCH(KC_COMBINE, 38, launch_stage_combine(c->L, c->iBxn, c->iByn));
@@ -794,6 +815,7 @@ static void hist_push(imex_ctx* c, const int mx[NBK]) {
 // recent solve needed a restart.  A batch has no host scheduling: it starts from generous
 // budgets and grows them when a step outruns them.
 static bool choose_budgets(imex_ctx* c) {
+  if (c->taping) return false;
   if (c->ni == 1 && (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0)) return false;
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   int mx[NBK] = {0, 0, 0, 0, 0};
@@ -1380,6 +1402,7 @@ int imex_batch_size(const imex_ctx* c) { return c ? c->ni : 0; }
 int imex_destroy(imex_ctx* c) {
   if (!c) return 0;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  adj_free(c);
   drop_graphs(c);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->Kh) (void)hipHostFree(c->Kh);
@@ -1636,10 +1659,6 @@ int imex_apply_op(imex_ctx* c, int op, const double* x, double* Ax, const double
   return 0;
 }
 
-int imex_apply_op_T(imex_ctx* c, int op, const double* x, double* ATx, const double* p_freeze, double mbig) {
-  (void)c; (void)op; (void)x; (void)ATx; (void)p_freeze; (void)mbig;
-  return set_err(IMEX_ERR_UNSUPPORTED, "transposed operators are not built yet");
-}
 
 int imex_gmres(imex_ctx* c, int op, const double* b, double* x, int precond, const double* p_freeze, double mbig,
                imex_solve_info* info) {
@@ -1781,3 +1800,5 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
 }
 
 }  // extern "C"
+
+#include "adjoint_host.inc"

@@ -224,7 +224,9 @@ int imex_ct_update(imex_ctx* ctx);
  * For IMEX_OP_P_JACOBIAN, p_freeze (N) and mbig select V_slope. Host arrays. */
 int imex_apply_op(imex_ctx* ctx, int op, const double* x, double* Ax,
                   const double* p_freeze, double mbig);
-/* Transposed operators for the discrete adjoint. */
+/* Transposed operators for the discrete adjoint: ATx = A^T x for the operator imex_apply_op
+ * applies (periodic grids, ideal gas; the pressure Jacobian only where p_freeze is above the
+ * floor).  Same arguments as imex_apply_op. */
 int imex_apply_op_T(imex_ctx* ctx, int op, const double* x, double* ATx,
                     const double* p_freeze, double mbig);
 /* GMRES_Solver (implicit_operators.cpp:793-919). precond: 0 none, 1 Jacobi (precP). x in/out. */
@@ -232,6 +234,28 @@ int imex_gmres(imex_ctx* ctx, int op, const double* b, double* x, int precond,
                const double* p_freeze, double mbig, imex_solve_info* info);
 /* Pressure_NestedNewton (implicit_operators.cpp:625-738). p in/out. */
 int imex_pressure_newton(imex_ctx* ctx, const double* b, double* p, imex_solve_info* info);
+
+/* ---- discrete adjoint of the step (no counterpart in the reference, which has no derivative) ----
+ * Supported: single-instance contexts, periodic grids, ideal gas, Minmod or VanLeer, EMF-consistent
+ * CT, FP64 form (other settings return IMEX_ERR_UNSUPPORTED).  Usage: imex_set_state (the initial
+ * state x0), imex_adjoint_forward (runs nsteps, keeping a checkpoint every `every` steps; the
+ * context then holds the final state), read the final state and form the cotangent of the
+ * objective J with respect to it (gu_con: host Grid of conservative variables; gbx, gby: face
+ * Arrays, may be null), imex_adjoint_backward: returns dJ/dx0 with respect to the primitive state
+ * and the faces given to imex_set_state (interior cells; ghosts are zero).  The backward pass
+ * recomputes each checkpoint segment once, taping its intermediate states, and solves the
+ * transposed magnetic and pressure systems with GMRES.  The context is left at the final state.
+ * Exact up to the solver tolerances where every step is differentiable: the CFL maxima are
+ * differentiated through the cells that attain them, limiters through their active branch. */
+typedef struct imex_adjoint_info {
+  int steps, checkpoints, recomputed_steps;
+  long long transposed_solves, mag_iters, p_iters;   /* GMRES iterations of the transposed solves */
+  double worst_solve_residual;                       /* largest relative residual left by a transposed solve */
+  double backward_ms;                                /* device time of the backward pass */
+} imex_adjoint_info;
+int imex_adjoint_forward(imex_ctx* ctx, int nsteps, int every);
+int imex_adjoint_backward(imex_ctx* ctx, const double* gu_con, const double* gbx, const double* gby,
+                          double* adj_u_pri, double* adj_bx, double* adj_by, imex_adjoint_info* info);
 
 /* ---- profiling ---- */
 /* Enable (1) or disable (0) per-launch event timing; enabling clears the totals. */
