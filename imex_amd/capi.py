@@ -68,7 +68,8 @@ class KernelProfile(C.Structure):
 
 class TuneInfo(C.Structure):
     _fields_ = [("explicit_tile_x", C.c_int), ("explicit_tile_y", C.c_int), ("krylov_blocks", C.c_int),
-                ("explicit_us", C.c_double * 3), ("krylov_us", C.c_double * 3)]
+                ("explicit_us", C.c_double * 3), ("krylov_us", C.c_double * 3),
+                ("krylov_p_blocks", C.c_int), ("krylov_p_us", C.c_double * 4)]
 
 
 class AdjointInfo(C.Structure):
@@ -314,7 +315,8 @@ class Solver:
         ti = TuneInfo()
         self._chk(self.L.imex_autotune(self.h, flags, C.byref(ti)), "autotune")
         return dict(explicit_tile=(ti.explicit_tile_x, ti.explicit_tile_y), krylov_blocks=ti.krylov_blocks,
-                    explicit_us=list(ti.explicit_us), krylov_us=list(ti.krylov_us))
+                    explicit_us=list(ti.explicit_us), krylov_us=list(ti.krylov_us),
+                    krylov_p_blocks=ti.krylov_p_blocks, krylov_p_us=list(ti.krylov_p_us))
 
     def profile_enable(self, on=True):
         return self._chk(self.L.imex_profile_enable(self.h, 1 if on else 0), "profile_enable")

@@ -173,7 +173,10 @@ struct imex_ctx {
   long long blk_cap = 0;             // its capacity (the largest caps the autotuner may choose)
   Launch L;
   long long cells = 0;               // sum of nx ny over instances
-  int cap_tiles = 2048, cap_vec = 2048;   // grid caps of the Krylov kernels (autotuned)
+  // grid caps of the Krylov kernels (autotuned): a cap that is a multiple of the blocks a kernel
+  // keeps resident avoids a partly filled last round (magnetic FP64 apply: 3 waves/SIMD, 768
+  // blocks; pressure FP64: 6, 1536; pressure FP32 apply: 8, 2048)
+  int cap_tiles = 2048, cap_tiles_p = 2048, cap_vec = 2048;
   int ex_tile = 32;                       // explicit-kernel tile width (16, 32 or 64; autotuned)
   // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo;
   // faces likewise (bxn/byn read, bx2/by2 written)
@@ -306,6 +309,7 @@ static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& n
     case SH_LIN_N: nb = (int)cdiv(d.N, 256); gx = nb; break;
     case SH_START: nb = (int)std::min<long long>(cdiv(tot, 256), 1024); gx = nb; break;
     case SH_TILES: nb = (int)std::min<long long>(tiles, c->cap_tiles); gx = nb; break;
+    case SH_TILES_P: nb = (int)std::min<long long>(tiles, c->cap_tiles_p); gx = nb; break;
     case SH_TILES_ALL: nb = (int)std::min<long long>(tiles, 4096); gx = nb; break;
     case SH_VEC3: nb = (int)std::min<long long>(cdiv(3LL * d.N, 256), c->cap_vec); gx = nb; break;
     case SH_VEC1: nb = (int)std::min<long long>(cdiv(d.N, 256), c->cap_vec); gx = nb; break;
@@ -1354,6 +1358,7 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   c->prm.nx = nx[0]; c->prm.ny = ny[0];
   c->P = make_phys(*prm);
   c->mixed = prm->precision == IMEX_PREC_MIXED ? 1 : 0;
+  c->cap_tiles_p = c->mixed ? 2048 : 1536;
   c->ni = n;
   c->iprm.assign(n, c->prm);
   c->hI.resize(n);
@@ -1371,10 +1376,10 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   if (hipMemcpy(c->IT, c->hI.data(), (size_t)n * sizeof(Inst), hipMemcpyHostToDevice) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "table"));
   if ((rc = dalloc(c, &c->bmap_dev, (size_t)NSHAPE * (2 * n + 1)))) return fail(rc);
   if (n > 1) {          // block table sized for every launch shape the autotuner may choose
-    const int ct = c->cap_tiles, cv = c->cap_vec, et = c->ex_tile;
-    c->cap_tiles = c->cap_vec = 4096;
+    const int ct = c->cap_tiles, ctp = c->cap_tiles_p, cv = c->cap_vec, et = c->ex_tile;
+    c->cap_tiles = c->cap_tiles_p = c->cap_vec = 4096;
     for (int t : {16, 32, 64}) { c->ex_tile = t; c->blk_cap = std::max(c->blk_cap, map_blocks(c)); }
-    c->cap_tiles = ct; c->cap_vec = cv; c->ex_tile = et;
+    c->cap_tiles = ct; c->cap_tiles_p = ctp; c->cap_vec = cv; c->ex_tile = et;
     if ((rc = dalloc(c, &c->blk_dev, (size_t)c->blk_cap))) return fail(rc);
   }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { c->st = nullptr; return fail(set_err(IMEX_ERR_HIP, "stream")); }
@@ -1782,6 +1787,20 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
       if (info.krylov_us[q] < info.krylov_us[best]) best = q;
     }
     c->cap_tiles = c->cap_vec = caps[best];
+    // pressure operator: its own cap
+    const int pcaps[4] = {1024, 1536, 2048, 4096};
+    int pbest = 0;
+    for (int q = 0; q < 4 && !rc; q++) {
+      c->cap_tiles_p = pcaps[q];
+      if ((rc = build_maps(c))) break;
+      gmres_residual(L, 1, c->mixed, mag_ids(c), p_ids(c, 0, 1), c->P, B_XK, B_BK, 0, 1, SKIP_NONE);
+      gmres_residual_fin(L, 1, c->mixed, 0, SKIP_NONE, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
+      info.krylov_p_us[q] = timed(10, [&] {
+        arnoldi_apply(L, 1, c->mixed, mag_ids(c), p_ids(c, 0, 1), c->P, j);
+      });
+      if (info.krylov_p_us[q] < info.krylov_p_us[pbest]) pbest = q;
+    }
+    c->cap_tiles_p = pcaps[pbest];
   }
   if (!rc) rc = build_maps(c);
   k_budget_restore<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE]);
@@ -1795,6 +1814,7 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
   info.explicit_tile_x = c->ex_tile;
   info.explicit_tile_y = 256 / c->ex_tile;
   info.krylov_blocks = c->cap_tiles;
+  info.krylov_p_blocks = c->cap_tiles_p;
   if (out) *out = info;
   return 0;
 }

@@ -445,7 +445,7 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const Inst* __restri
 // The dots of each tile are reduced over the wave right away (a uniform loop over i, so no
 // accumulator array is live across the operator) and summed per wave in LDS, in tile order.
 template <int OPK, class T, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : (sizeof(T) == 4 ? 8 : 4))))
 k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -827,37 +827,41 @@ __global__ void k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, dou
 
 // ================================================================ launchers
 #define GRID(L, sh) dim3((unsigned)(L).nblocks[sh])
-static inline int tiles_shape(int) { return SH_TILES; }
+// magnetic-operator kernels and the Newton residual on SH_TILES, pressure-operator kernels on
+// SH_TILES_P: their occupancies differ, so their grid caps are set apart
+static inline int tiles_shape(int op) { return op == 0 ? SH_TILES : SH_TILES_P; }
 static inline int vec_shape(int op) { return op == 0 ? SH_VEC3 : SH_VEC1; }
 
 void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int ix, int ib,
                     int outer, int x_is_zero, int skip) {
-  const BMap& bm = L.bm[SH_TILES];
+  const int sh = tiles_shape(op);
+  const BMap& bm = L.bm[sh];
   if (prec == 0) {
-    if (op == 0) k_gmres_residual<0, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
-    else k_gmres_residual<1, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    if (op == 0) k_gmres_residual<0, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    else k_gmres_residual<1, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
   } else {
-    if (op == 0) k_gmres_residual<0, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
-    else k_gmres_residual<1, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    if (op == 0) k_gmres_residual<0, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    else k_gmres_residual<1, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
   }
 }
-void gmres_residual_fin(const Launch& L, int, int prec, int outer, int skip, double outer_tol, double inner_tol) {
-  k_gmres_residual_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], outer, skip, prec, outer_tol, inner_tol);
+void gmres_residual_fin(const Launch& L, int op, int prec, int outer, int skip, double outer_tol, double inner_tol) {
+  k_gmres_residual_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[tiles_shape(op)], outer, skip, prec, outer_tol, inner_tol);
 }
 
 void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int j) {
-  const BMap& bm = L.bm[SH_TILES];
+  const int sh = tiles_shape(op);
+  const BMap& bm = L.bm[sh];
   if (prec == 0) {
-    if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
-    else k_arnoldi_apply<1, double, double><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, j);
+    if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else k_arnoldi_apply<1, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   } else {
-    if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
-    else k_arnoldi_apply<1, float, float><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, bm, j);
+    if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else k_arnoldi_apply<1, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   }
 }
-void arnoldi_apply_fin(const Launch& L, int, int j) {
+void arnoldi_apply_fin(const Launch& L, int op, int j) {
   const BMap& one = L.bm[SH_ONE];
-  const BMap& src = L.bm[SH_TILES];
+  const BMap& src = L.bm[tiles_shape(op)];
   if (j < 8) k_arnoldi_dots_fin<8><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
   else if (j < 16) k_arnoldi_dots_fin<16><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
   else k_arnoldi_dots_fin<32><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);

@@ -189,15 +189,18 @@ double imex_batch_div_b(imex_ctx* ctx, int i);
 /* ---- autotuner (no counterpart in the reference, whose loops have no launch shape) ----
  * Times candidate launch shapes on the context's current state (set it first) and keeps the
  * fastest: IMEX_TUNE_EXPLICIT picks the explicit-flux tile among 16x16, 32x8 and 64x4 cells
- * (results identical bit for bit); IMEX_TUNE_KRYLOV picks the grid cap of the Krylov kernels
- * among 1024, 2048 and 4096 blocks per instance (a different cap reorders the dot-product sums,
- * so results change by rounding).  The state, time and statistics are left as they were. */
+ * (results identical bit for bit); IMEX_TUNE_KRYLOV picks the grid cap of the magnetic Krylov
+ * kernels among 1024, 2048 and 4096 blocks per instance and that of the pressure-operator kernels
+ * among 1024, 1536, 2048 and 4096 (a different cap reorders the dot-product sums, so results
+ * change by rounding).  The state, time and statistics are left as they were. */
 enum { IMEX_TUNE_EXPLICIT = 1, IMEX_TUNE_KRYLOV = 2 };
 typedef struct imex_tune_info {
   int explicit_tile_x, explicit_tile_y;   /* chosen explicit tile (cells) */
   int krylov_blocks;                      /* chosen Krylov grid cap (blocks per instance) */
   double explicit_us[3];                  /* both explicit kernels, per candidate 16x16, 32x8, 64x4 */
   double krylov_us[3];                    /* one Arnoldi step (apply + update), per cap 1024, 2048, 4096 */
+  int krylov_p_blocks;                    /* chosen grid cap of the pressure-operator kernels */
+  double krylov_p_us[4];                  /* pressure Arnoldi apply, per cap 1024, 1536, 2048, 4096 */
 } imex_tune_info;
 int imex_autotune(imex_ctx* ctx, int flags, imex_tune_info* out);
 
