@@ -40,7 +40,7 @@ class Params(C.Structure):
         ("shock_kappa1", C.c_double), ("time_stop", C.c_double),
         ("ct_mode", C.c_int), ("precision", C.c_int),
         ("mixed_inner_tol", C.c_double), ("mixed_outer_tol", C.c_double),
-        ("schedule", C.c_int), ("budget_margin", C.c_int),
+        ("schedule", C.c_int), ("budget_margin", C.c_int), ("mixed_pressure", C.c_int),
     ]
 
 

@@ -211,8 +211,8 @@ struct imex_ctx {
   int kcur[NGKEY][2], ncur[NNKEY];
   BudTab tab;
   int stage_idx = 0, sub_idx = 0, newton_it = 0;
-  // mixed form
-  int mixed = 0;
+  // mixed form: FP32 fluxes; per operator (bit 0 magnetic, bit 1 pressure) FP32 inner solves
+  int mixed = 0, mixed_ops = 0;
   std::vector<GraphEntry> graphs;
   long long graph_clock = 0;
   bool capturing = false;      // launches are being captured into a step graph
@@ -373,6 +373,10 @@ static int build_maps(imex_ctx* c) {
 }
 
 
+// precision of the inner GMRES of operator op (0 FP64, 1 FP32 with FP64 refinement), and of the
+// solve at budget key k
+static inline int op_prec(const imex_ctx* c, int op) { return (c->mixed_ops >> op) & 1; }
+static inline int key_prec(const imex_ctx* c, int k) { return op_prec(c, (k / NIT) % 3 == 0 ? 0 : 1); }
 static inline MagIds mag_ids(const imex_ctx* c) { MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; return m; }
 static inline PIds p_ids(const imex_ctx* c, int jacobian, int precond) { PIds q; q.u = c->cur.u; q.jacobian = jacobian; q.precond = precond; return q; }
 
@@ -423,7 +427,7 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
                             int x_is_zero, int skip, int parent, int it) {
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   const int nc = op == 0 ? 3 : 1;
-  const int prec = c->mixed;
+  const int prec = op_prec(c, op);
   const Launch& L = c->L;
   BudgetRec r;
   r.kind = 0; r.op = op; r.parent = parent; r.it = it; r.key = gkey(c);
@@ -493,7 +497,7 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
   // IMEX_WARN_GMRES_MAXOUTER when it gives up.
   const long long outmax = std::min<long long>(c->prm.gmres_max_outer > 0 ? c->prm.gmres_max_outer : dim, 4000);
   const int chunk = 2;
-  const int prec = c->mixed;
+  const int prec = op_prec(c, op);
   const double vb = prec ? 0.5 : 1.0;       // bytes of a basis field relative to an FP64 field
   if (first_chunk <= 0) first_chunk = c->pred[op];
   int total = 0, rc = 0, maxsweep = 0;
@@ -886,8 +890,9 @@ static int budget_finish(imex_ctx* c, long long gen0) {
       }
       if (skipped) continue;
       mx[3 + r.op] = std::max(mx[3 + r.op], sweeps);
-      cur_gmres(c, r.key, c->mixed ? 0 : steps, sweeps);
-      if (c->mixed) c->stats.mixed_refinements += sweeps;
+      const int rp = op_prec(c, r.op);
+      cur_gmres(c, r.key, rp ? 0 : steps, sweeps);
+      if (rp) c->stats.mixed_refinements += sweeps;
       if (r.op == 0) { c->stats.mag_solves++; c->stats.mag_gmres_iters += steps; }
       else { c->stats.p_solves++; c->stats.p_gmres_iters += steps; }
     }
@@ -897,12 +902,13 @@ static int budget_finish(imex_ctx* c, long long gen0) {
 }
 
 // Whether a tagged node runs under budget table tb.
-static bool tag_enabled(const NodeTag& t, const BudTab& tb, int mixed) {
+static bool tag_enabled(const imex_ctx* c, const NodeTag& t, const BudTab& tb) {
   if (t.kind == TG_NIT) return t.j < tb.N[t.key];
   if (t.kind == TG_NRES) return t.j <= tb.N[t.key];
   const int sub = (t.key / NIT) % 3, it = t.key % NIT;
   const bool live = sub == 0 || it < tb.N[t.key / NIT];      // the Newton iteration holding the solve runs
   if (!live) return false;
+  const int mixed = key_prec(c, t.key);
   const int S = mixed ? tb.S[t.key] : 1;
   switch (t.kind) {
     case TG_ARN: return t.sw < S && t.j < tb.B[t.key];
@@ -912,10 +918,10 @@ static bool tag_enabled(const NodeTag& t, const BudTab& tb, int mixed) {
   }
 }
 
-static bool tab_covers(const BudTab& cap, const BudTab& t, int mixed) {
+static bool tab_covers(const imex_ctx* c, const BudTab& cap, const BudTab& t) {
   for (int k = 0; k < NGKEY; k++) {
     if (t.B[k] > cap.B[k]) return false;
-    if (mixed && t.S[k] > cap.S[k]) return false;
+    if (key_prec(c, k) && t.S[k] > cap.S[k]) return false;
   }
   for (int k = 0; k < NNKEY; k++) if (t.N[k] > cap.N[k]) return false;
   return true;
@@ -935,7 +941,7 @@ static int budget_graph_launch(imex_ctx* c) {
   GraphEntry* g = nullptr;
   for (GraphEntry& e : c->graphs) {
     if (e.parity != parity || e.state_mode != c->state_mode) continue;
-    if (c->no_toggle ? std::memcmp(&e.cap, &cur, sizeof(BudTab)) == 0 : tab_covers(e.cap, cur, c->mixed)) { g = &e; break; }
+    if (c->no_toggle ? std::memcmp(&e.cap, &cur, sizeof(BudTab)) == 0 : tab_covers(c, e.cap, cur)) { g = &e; break; }
   }
   if (!g) {
     if ((int)c->graphs.size() >= MAX_GRAPHS) {     // drop the least recently used graph
@@ -950,7 +956,7 @@ static int budget_graph_launch(imex_ctx* c) {
       const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
       for (int k = 0; k < NGKEY; k++) {
         cap.B[k] = std::min(m, cur.B[k] + 2);
-        cap.S[k] = c->mixed ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : 1;
+        cap.S[k] = key_prec(c, k) ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : 1;
       }
       for (int k = 0; k < NNKEY; k++) cap.N[k] = std::min(std::min(c->prm.newton_max, NIT - 1), cur.N[k] + 1);
     }
@@ -984,8 +990,8 @@ static int budget_graph_launch(imex_ctx* c) {
   }
   if (!c->no_toggle && std::memcmp(&g->applied, &cur, sizeof(BudTab)) != 0) {
     for (const NodeTag& t : g->tags) {
-      const bool en = tag_enabled(t, cur, c->mixed);
-      if (en == tag_enabled(t, g->applied, c->mixed)) continue;
+      const bool en = tag_enabled(c, t, cur);
+      if (en == tag_enabled(c, t, g->applied)) continue;
       if (hipGraphNodeSetEnabled(g->exec, t.node, en ? 1 : 0) != hipSuccess) {
         // enabling is not available here: drop the graph and capture per exact table from now on
         c->no_toggle = true;
@@ -1042,7 +1048,7 @@ static int one_step(imex_ctx* c) {
     c->fit_streak = 0;
     if (c->ni > 1 && (f & 1)) {     // a single instance reruns the step host-scheduled instead
       c->margin_extra = std::min(8, c->margin_extra + 2);
-      if (c->mixed) c->sweep_extra = std::min(4, c->sweep_extra + 1);
+      if (c->mixed_ops) c->sweep_extra = std::min(4, c->sweep_extra + 1);
     }
     if (c->ni > 1 && (f & 2)) c->newton_extra = std::min(4, c->newton_extra + 1);
     // a solve outran its budget: restore the step's scalars and rerun it (host scheduling for a
@@ -1358,7 +1364,8 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   c->prm.nx = nx[0]; c->prm.ny = ny[0];
   c->P = make_phys(*prm);
   c->mixed = prm->precision == IMEX_PREC_MIXED ? 1 : 0;
-  c->cap_tiles_p = c->mixed ? 2048 : 1536;
+  c->mixed_ops = c->mixed ? (1 | (prm->mixed_pressure ? 2 : 0)) : 0;
+  c->cap_tiles_p = op_prec(c, 1) ? 2048 : 1536;
   c->ni = n;
   c->iprm.assign(n, c->prm);
   c->hI.resize(n);
@@ -1778,11 +1785,11 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
     for (int q = 0; q < 3 && !rc; q++) {
       c->cap_tiles = c->cap_vec = caps[q];
       if ((rc = build_maps(c))) break;
-      gmres_residual(L, 0, c->mixed, mag_ids(c), p_ids(c, 0, 0), c->P, B_XK, B_BK, 0, 0, SKIP_NONE);
-      gmres_residual_fin(L, 0, c->mixed, 0, SKIP_NONE, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
+      gmres_residual(L, 0, op_prec(c, 0), mag_ids(c), p_ids(c, 0, 0), c->P, B_XK, B_BK, 0, 0, SKIP_NONE);
+      gmres_residual_fin(L, 0, op_prec(c, 0), 0, SKIP_NONE, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
       info.krylov_us[q] = timed(10, [&] {
-        arnoldi_apply(L, 0, c->mixed, mag_ids(c), p_ids(c, 0, 0), c->P, j);
-        arnoldi_update(L, 0, c->mixed, j, m);
+        arnoldi_apply(L, 0, op_prec(c, 0), mag_ids(c), p_ids(c, 0, 0), c->P, j);
+        arnoldi_update(L, 0, op_prec(c, 0), j, m);
       });
       if (info.krylov_us[q] < info.krylov_us[best]) best = q;
     }
@@ -1793,10 +1800,10 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
     for (int q = 0; q < 4 && !rc; q++) {
       c->cap_tiles_p = pcaps[q];
       if ((rc = build_maps(c))) break;
-      gmres_residual(L, 1, c->mixed, mag_ids(c), p_ids(c, 0, 1), c->P, B_XK, B_BK, 0, 1, SKIP_NONE);
-      gmres_residual_fin(L, 1, c->mixed, 0, SKIP_NONE, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
+      gmres_residual(L, 1, op_prec(c, 1), mag_ids(c), p_ids(c, 0, 1), c->P, B_XK, B_BK, 0, 1, SKIP_NONE);
+      gmres_residual_fin(L, 1, op_prec(c, 1), 0, SKIP_NONE, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
       info.krylov_p_us[q] = timed(10, [&] {
-        arnoldi_apply(L, 1, c->mixed, mag_ids(c), p_ids(c, 0, 1), c->P, j);
+        arnoldi_apply(L, 1, op_prec(c, 1), mag_ids(c), p_ids(c, 0, 1), c->P, j);
       });
       if (info.krylov_p_us[q] < info.krylov_p_us[pbest]) pbest = q;
     }

@@ -83,6 +83,9 @@ typedef struct imex_params {
   int schedule;                   /* IMEX_SCHED_* */
   int budget_margin;              /* Arnoldi steps granted per GMRES solve beyond the largest count of the
                                      last 8 steps (budgeted schedules; default 1) */
+  int mixed_pressure;             /* mixed form: 1 = the pressure solves also run FP32 inner solves; 0 (default)
+                                     = they stay FP64 (the pressure system converges in a few FP64 steps, fewer
+                                     than the refinement needs) while the magnetic solves and fluxes run FP32 */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */

@@ -1,8 +1,8 @@
 """Mixed-precision form against the FP64 CPU oracle.
 
 The mixed form reconstructs and forms the explicit fluxes in FP32 (the conservative
-update stays FP64) and applies the implicit operators in FP32 inside GMRES solves whose
-basis is stored in FP32; dot products, norms, the Hessenberg/Givens solve and the outer
+update stays FP64) and applies the magnetic operator (and, with mixed_pressure = 1, the
+pressure operator) in FP32 inside GMRES solves whose basis is stored in FP32; dot products, norms, the Hessenberg/Givens solve and the outer
 residual of the iterative refinement are FP64, and so are CT and the Newton residual.
 Bound: relative L2 at most 1e-6 per conserved field after N steps, div B at most 1e-12
 (CT is unchanged).  The accuracy reached is printed for each case.
@@ -47,6 +47,24 @@ def test_mixed_form_within_1e6_of_oracle(case, n, m, steps):
     assert rel_l2(interior(bxg), interior(bx_r)) <= 1e-6
     assert stats["div_b"] <= 1e-12, stats["div_b"]
     assert stats["mixed_refinements"] > 0
+
+
+@pytest.mark.parametrize("case,n,m,steps", [("ot", 64, 64, 8), ("blast", 48, 40, 3)])
+def test_mixed_fp32_pressure_solves_within_1e6(case, n, m, steps):
+    """The opt-in FP32 inner solves of the pressure system (mixed_pressure = 1) keep the same bound."""
+    print(f"[mixed-pressure] {case} {n}x{m} steps={steps}", flush=True)
+    p, u, bx, by = make_case(case, n, m)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(steps)
+    uc_r = o.get_state()[0]
+    p.precision = im.PREC_MIXED
+    p.mixed_pressure = 1
+    (uc, up, bxg, byg, sc), stats = run_gpu(p, u, bx, by, steps)
+    errs = field_errors(uc, uc_r)
+    print("  rel L2 per field:", " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) <= 1e-6, errs
+    assert stats["div_b"] <= 1e-12
 
 
 @pytest.mark.parametrize("case,n,m,steps", [("vortex", 64, 64, 12), ("ot", 64, 64, 8)])
