@@ -36,6 +36,8 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None):
     u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
     s = im.Solver(p)
     s.set_state(u, bx, by)
+    tune = s.autotune(im.TUNE_EXPLICIT | im.TUNE_KRYLOV)       # launch shapes for this grid
+    log(f"[bench] {name} autotune: {tune}")
     log(f"[bench] {name} warmup {warmup} steps")
     if warmup:
         s.step(warmup)
@@ -65,7 +67,7 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None):
     prof = s.profile_read()
     s.profile_enable(False)
     s.close()
-    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final)
+    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final, tune=tune)
 
 
 def run_pair(name, steps, warmup, schedule):
@@ -82,6 +84,7 @@ def run_pair(name, steps, warmup, schedule):
         s = im.Solver(p)
         try:
             s.set_state(u, bx, by)
+            s.autotune(im.TUNE_EXPLICIT | im.TUNE_KRYLOV)
             log(f"[bench] {name} {f}: warmup {warmup}, timing {steps} steps")
             if warmup:
                 s.step(warmup)
@@ -258,6 +261,32 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}), flush=True)
 
 
+KERNEL_OF_CLASS = {   # (class, form) -> kernel symbol in the committed rocprof summary
+    ("mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<0, double, double>",
+    ("mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<0, float, float>",
+    ("p_arnoldi_apply", "fp64"): "k_arnoldi_apply<1, double, double>",
+    ("p_arnoldi_apply", "mixed"): "k_arnoldi_apply<1, double, double>",
+}
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "imex_step_ot1024_kernels.md")
+
+
+def measured_traffic(cls, form):
+    """HBM bytes per active dispatch of the kernel behind `cls`, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a plain copy); None if it is not listed there."""
+    import re
+    name = KERNEL_OF_CLASS.get((cls, form))
+    if not name or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    in_traffic = False
+    for line in open(TRAFFIC_SUMMARY):
+        if line.startswith("## HBM traffic"):
+            in_traffic = True
+        m = re.match(r"\| `(.+?)` \| (\d+) \| ([\d.]+) \| ([\d.]+) \|", line)
+        if in_traffic and m and m.group(1) == name:
+            return round((float(m.group(3)) + float(m.group(4))) * (1 << 20))
+    return None
+
+
 def summarize(r, steps, label):
     """Throughput, per-class table (stderr) and the roofline of the dominant kernel class."""
     p = r["p"]
@@ -277,14 +306,20 @@ def summarize(r, steps, label):
     log(f"[bench] {label} profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
         f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
     st = r["stats"]
+    traffic = measured_traffic(dom["name"], label)
     return {
         "value": round(value, 3),
         "ms_per_step": round(r["elapsed"] / steps * 1e3, 3),
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per active dispatch of "
+                                        f"{KERNEL_OF_CLASS.get((dom['name'], label))}, calibrated, "
+                                        "profiles/imex_step_ot1024_kernels.md") if traffic else None,
                      "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "gmres": {k: st[k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters",
                                      "mixed_refinements")},
+        "autotuned": {"explicit_tile": list(r["tune"]["explicit_tile"]), "krylov_blocks": r["tune"]["krylov_blocks"],
+                      "pressure_krylov_blocks": r["tune"]["krylov_p_blocks"]},
         "schedule": {"budget_steps": st["budget_steps"], "budget_reruns": st["budget_reruns"],
                      "graph_builds": st["graph_builds"],
                      "profiled_pass_ms_per_step": round(r["prof_elapsed"] / steps * 1e3, 3)},
@@ -360,6 +395,7 @@ def main():
         "roofline": res[head]["roofline"],
         "gmres": res[head]["gmres"],
         "schedule": res[head]["schedule"],
+        "autotuned": res[head]["autotuned"],
     }
     if head == "mixed":
         out["config"]["mixed_inner_tol"] = runs["mixed"]["p"].mixed_inner_tol

@@ -78,3 +78,25 @@ def test_product_never_imports_the_oracle():
     import sys
     code = "import sys; import imex_amd; assert not any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules)"
     subprocess.run([sys.executable, "-c", code], check=True, cwd=ROOT)
+
+
+def test_batch_create_rejects_bad_arguments_before_touching_a_device():
+    """Argument checks of imex_batch_create run before any device call (no GPU needed)."""
+    L = im.lib()
+    p = im.vortex_params(32)
+    h = C.c_void_p()
+    nx = (C.c_int * 2)(32, 3)
+    ny = (C.c_int * 2)(32, 32)
+    assert L.imex_batch_create(C.byref(p), 0, nx, ny, C.byref(h)) == -1          # empty batch
+    assert L.imex_batch_create(C.byref(p), 2, nx, ny, C.byref(h)) == -1          # a grid below 4 cells
+    assert b"too small" in L.imex_last_error()
+    q = im.vortex_params(32)
+    q.recon = 3                                                                   # no such reconstruction
+    nx[1] = 32
+    assert L.imex_batch_create(C.byref(q), 2, nx, ny, C.byref(h)) == -4
+    assert L.imex_batch_size(None) == 0
+
+
+def test_mixed_pressure_defaults_off():
+    p = im.default_params()
+    assert p.mixed_pressure == 0 and (p.mixed_inner_tol, p.mixed_outer_tol) == (1e-6, 1e-9)
