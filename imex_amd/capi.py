@@ -16,7 +16,8 @@ RECON_MINMOD, RECON_VANLEER, RECON_WENO5 = 1, 2, 4
 EOS_IDEAL, EOS_RK = 0, 1
 CT_CONSISTENT, CT_PLAIN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN = 0, 1, 2
+OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN, OP_ENERGY = 0, 1, 2, 3
+FORMULA_ENERGY, FORMULA_PRESSURE = 0, 1
 CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
 SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
 TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
@@ -41,6 +42,7 @@ class Params(C.Structure):
         ("ct_mode", C.c_int), ("precision", C.c_int),
         ("mixed_inner_tol", C.c_double), ("mixed_outer_tol", C.c_double),
         ("schedule", C.c_int), ("budget_margin", C.c_int), ("mixed_pressure", C.c_int),
+        ("formula", C.c_int),
     ]
 
 
@@ -182,6 +184,8 @@ def lib():
         "imex_mag_update_b": (C.c_int, [vp]),
         "imex_explicit_press": (C.c_int, [vp]),
         "imex_pre_subsystem_pressure": (C.c_int, [vp]),
+        "imex_explicit_energy": (C.c_int, [vp]),
+        "imex_pre_subsystem_energy": (C.c_int, [vp]),
         "imex_ct_update": (C.c_int, [vp]),
         "imex_apply_op": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
         "imex_apply_op_T": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),

@@ -87,13 +87,15 @@ __global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int 
 enum KClass {
   KC_START, KC_COMBINE, KC_QEPREP, KC_SHOCK, KC_EXPL_CONV, KC_EXPL_PRESS, KC_MAG_RHS, KC_MAG_APPLY, KC_MAG_UPDATE,
   KC_MAG_RES, KC_MAG_XUPD, KC_P_APPLY, KC_P_UPDATE, KC_P_RES, KC_P_XUPD, KC_NEWTON_RES, KC_EMF, KC_P_RHS, KC_P_MOM,
-  KC_P_ENERGY, KC_CT, KC_BC, KC_MISC, KC_KFIN, KC_EARLY_EXIT, KC_N
+  KC_P_ENERGY, KC_CT, KC_BC, KC_MISC, KC_KFIN, KC_EARLY_EXIT, KC_EXPL_ENERGY, KC_E_RHS, KC_E_APPLY, KC_E_UPDATE,
+  KC_E_RES, KC_E_XUPD, KC_E_MOM, KC_N
 };
 static const char* kc_names[KC_N] = {
   "step_start", "stage_combine", "qe_prep", "shock", "explicit_conv", "explicit_press", "mag_rhs", "mag_arnoldi_apply",
   "mag_arnoldi_update", "mag_gmres_residual", "mag_gmres_xupdate", "p_arnoldi_apply", "p_arnoldi_update",
   "p_gmres_residual", "p_gmres_xupdate", "newton_residual", "emf_flux", "p_rhs", "p_momentum", "p_energy", "ct",
-  "ghost_fill", "misc", "krylov_finish", "early_exit"};
+  "ghost_fill", "misc", "krylov_finish", "early_exit", "explicit_energy", "e_rhs", "e_arnoldi_apply",
+  "e_arnoldi_update", "e_gmres_residual", "e_gmres_xupdate", "e_momentum"};
 
 struct ProfRec { int cls; int e0, e1; double bytes; };
 struct Prof {
@@ -378,7 +380,19 @@ static int build_maps(imex_ctx* c) {
 static inline int op_prec(const imex_ctx* c, int op) { return (c->mixed_ops >> op) & 1; }
 static inline int key_prec(const imex_ctx* c, int k) { return op_prec(c, (k / NIT) % 3 == 0 ? 0 : 1); }
 static inline MagIds mag_ids(const imex_ctx* c) { MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; return m; }
-static inline PIds p_ids(const imex_ctx* c, int jacobian, int precond) { PIds q; q.u = c->cur.u; q.jacobian = jacobian; q.precond = precond; return q; }
+static inline PIds p_ids(const imex_ctx* c, int jacobian, int precond) {
+  PIds q; q.u = c->cur.u; q.jacobian = jacobian; q.precond = precond; q.energy = 0; return q;
+}
+static inline PIds e_ids(const imex_ctx* c, int precond) { PIds q = p_ids(c, 0, precond); q.energy = 1; return q; }
+// profiler class of a Krylov launch: magnetic, pressure or energy operator
+static inline int kcls(int op, const PIds& po, int mag, int p, int e) { return op == 0 ? mag : (po.energy ? e : p); }
+// algorithmic fields of one residual / Arnoldi apply of the single-unknown operators (bench bytes)
+static inline double p_res_fields(const PIds& po, int xz) {
+  return po.energy ? (xz ? 2 : 4) + po.precond : (xz ? 2 : 5) + po.precond + po.jacobian;
+}
+static inline double p_apply_fields(const PIds& po, double vb, int j) {
+  return po.energy ? 1 + po.precond + vb * (3 + j) : 5 + po.jacobian + vb * (1 + j);
+}
 
 static void fill_planes(imex_ctx* c, std::initializer_list<std::pair<int, int>> planes, int kind = 0) {
   int b[8], k[8], n = 0;
@@ -437,11 +451,11 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
   for (int sw = 0; sw <= r.nsw; sw++) {
     if (sw == r.nsw && !prec) break;
     const int xz = (x_is_zero && sw == 0) ? 1 : 0;
-    const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
+    const double res_fields = op == 0 ? (xz ? 6 : 13) : p_res_fields(po, xz);
     const int k = r.key;
     int e0 = prof_begin(c);
     gmres_residual(L, op, prec, mo, po, c->P, ix, ib, sw, xz, skip);
-    r.rres.push_back(prof_end(c, e0, op == 0 ? KC_MAG_RES : KC_P_RES, res_fields));
+    r.rres.push_back(prof_end(c, e0, kcls(op, po, KC_MAG_RES, KC_P_RES, KC_E_RES), res_fields));
     tag_last(c, TG_RES, k, sw, 0);
     e0 = prof_begin(c);
     gmres_residual_fin(L, op, prec, sw, skip, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol);
@@ -452,8 +466,8 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
     for (int j = 0; j < r.B; j++) {
       e0 = prof_begin(c);
       arnoldi_apply(L, op, prec, mo, po, c->P, j);
-      r.ra.push_back(prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY,
-                              op == 0 ? 4 + vb * (9 + 3 * j) : 5 + po.jacobian + vb * (1 + j)));
+      r.ra.push_back(prof_end(c, e0, kcls(op, po, KC_MAG_APPLY, KC_P_APPLY, KC_E_APPLY),
+                              op == 0 ? 4 + vb * (9 + 3 * j) : p_apply_fields(po, vb, j)));
       tag_last(c, TG_ARN, k, sw, j);
       e0 = prof_begin(c);
       arnoldi_apply_fin(L, op, j);
@@ -461,7 +475,7 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
       tag_last(c, TG_ARN, k, sw, j);
       e0 = prof_begin(c);
       arnoldi_update(L, op, prec, j, m);
-      r.ru.push_back(prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, vb * (op == 0 ? 9 + 3 * j : 3 + j)));
+      r.ru.push_back(prof_end(c, e0, kcls(op, po, KC_MAG_UPDATE, KC_P_UPDATE, KC_E_UPDATE), vb * (op == 0 ? 9 + 3 * j : 3 + j)));
       tag_last(c, TG_ARN, k, sw, j);
       e0 = prof_begin(c);
       arnoldi_update_fin(L, op, prec, j, m);
@@ -470,7 +484,7 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
     }
     e0 = prof_begin(c);
     gmres_xupdate(L, op, prec, ix);
-    r.rx.push_back(prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (2 + vb * r.B)));
+    r.rx.push_back(prof_end(c, e0, kcls(op, po, KC_MAG_XUPD, KC_P_XUPD, KC_E_XUPD), (double)nc * (2 + vb * r.B)));
     tag_last(c, TG_XUPD, k, sw, 0);
   }
   k_budget_gmres_check<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], r.slot, prec, r.nsw);
@@ -506,9 +520,9 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
   double res0 = 0, res = 0;
   for (outer = 0; outer < outmax; outer++) {
     const int xz = (x_is_zero && outer == 0) ? 1 : 0;
-    const double res_fields = op == 0 ? (xz ? 6 : 13) : (xz ? 2 : 5) + po.precond + po.jacobian;
+    const double res_fields = op == 0 ? (xz ? 6 : 13) : p_res_fields(po, xz);
     const int sk = (outer == 0 || prec) ? skip : SKIP_NONE;
-    LAUNCH(op == 0 ? KC_MAG_RES : KC_P_RES, res_fields, gmres_residual(L, op, prec, mo, po, c->P, ix, ib, (int)outer, xz, sk));
+    LAUNCH(kcls(op, po, KC_MAG_RES, KC_P_RES, KC_E_RES), res_fields, gmres_residual(L, op, prec, mo, po, c->P, ix, ib, (int)outer, xz, sk));
     LAUNCH(KC_KFIN, 0, gmres_residual_fin(L, op, prec, (int)outer, sk, c->prm.mixed_outer_tol, c->prm.mixed_inner_tol));
     const long long gen0 = c->prof.gen;
     int recs_a[MAXM], recs_u[MAXM], recs_fa[MAXM], recs_fu[MAXM];
@@ -520,14 +534,14 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
       for (; j < jend; j++) {
         int e0 = prof_begin(c);
         arnoldi_apply(L, op, prec, mo, po, c->P, j);
-        recs_a[j] = prof_end(c, e0, op == 0 ? KC_MAG_APPLY : KC_P_APPLY,
-                             op == 0 ? 4 + vb * (9 + 3 * j) : 5 + po.jacobian + vb * (1 + j));
+        recs_a[j] = prof_end(c, e0, kcls(op, po, KC_MAG_APPLY, KC_P_APPLY, KC_E_APPLY),
+                             op == 0 ? 4 + vb * (9 + 3 * j) : p_apply_fields(po, vb, j));
         e0 = prof_begin(c);
         arnoldi_apply_fin(L, op, j);
         recs_fa[j] = prof_end(c, e0, KC_KFIN, 0);
         e0 = prof_begin(c);
         arnoldi_update(L, op, prec, j, m);
-        recs_u[j] = prof_end(c, e0, op == 0 ? KC_MAG_UPDATE : KC_P_UPDATE, vb * (op == 0 ? 9 + 3 * j : 3 + j));
+        recs_u[j] = prof_end(c, e0, kcls(op, po, KC_MAG_UPDATE, KC_P_UPDATE, KC_E_UPDATE), vb * (op == 0 ? 9 + 3 * j : 3 + j));
         e0 = prof_begin(c);
         arnoldi_update_fin(L, op, prec, j, m);
         recs_fu[j] = prof_end(c, e0, KC_KFIN, 0);
@@ -549,7 +563,7 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
     {
       int e0 = prof_begin(c);
       gmres_xupdate(L, op, prec, ix);
-      rx = prof_end(c, e0, op == 0 ? KC_MAG_XUPD : KC_P_XUPD, (double)nc * (2 + vb * (ks.jlast + 1)));
+      rx = prof_end(c, e0, kcls(op, po, KC_MAG_XUPD, KC_P_XUPD, KC_E_XUPD), (double)nc * (2 + vb * (ks.jlast + 1)));
     }
     if (c->prof.on && c->prof.gen == gen0) {      // launches queued past convergence exited at once
       auto idle = [&](int r) { if (r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
@@ -749,6 +763,35 @@ static int sub_pressure(imex_ctx* c) {
   return rc;
 }
 
+// B slots <- cell B~, then Energy_flux_Update (main.cpp:448-474, flux.cpp:165-200)
+static int sub_explicit_energy(imex_ctx* c) {
+  LAUNCH(KC_EXPL_ENERGY, 16, launch_explicit(c->L, c->P, 2, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed, c->ex_tile));
+  const int u = c->cur.u;
+  fill_planes(c, {{u, 1}, {u, 2}, {u, 3}, {u, 4}});
+  return 0;
+}
+
+// PreSubsystem_Energy (imex_subsystems.cpp:415-534): coefficient plane, RHS, initial guess and precE
+// divisor, one preconditioned GMRES solve (the pressure solve's slot: budgets, precision), then E
+// and the momentum update.  The coefficient overwrites the pressure enthalpy plane.
+static int prep_energy(imex_ctx* c) {
+  ensure_qe(c);
+  LAUNCH(KC_E_RHS, 14, launch_e_rhs(c->L, c->P, c->cur.u, c->cur.Qe));
+  fill_planes(c, {{B_HHAT, 0}});
+  c->hhat_ready = false;
+  return 0;
+}
+static int sub_energy(imex_ctx* c) {
+  prep_energy(c);
+  c->sub_idx = 1;
+  int rc = run_gmres(c, 1, mag_ids(c), e_ids(c, 1), B_P, B_PB, 0, nullptr, &c->stats.p_gmres_iters, &c->stats.p_gmres_restarts);
+  if (!c->budget) c->stats.p_solves++;
+  const int u = c->cur.u;
+  LAUNCH(KC_E_MOM, 6, launch_e_update(c->L, c->P, u));
+  fill_planes(c, {{u, 1}, {u, 2}, {u, 4}});
+  return rc;
+}
+
 static int sub_ct(imex_ctx* c) {
   LAUNCH(KC_CT, 15, launch_ct(c->L, c->P, c->prm.ct_mode == IMEX_CT_PLAIN, c->cur.u, c->cur.bxqs, c->cur.byqs, c->cur.bxo, c->cur.byo));
   return 0;
@@ -763,9 +806,14 @@ static int run_stage(imex_ctx* c) {
   if (c->hook) c->hook(c, HK_CONV);
   rc |= sub_mag_update_b(c);
   if (c->hook) c->hook(c, HK_MAG);
-  rc |= sub_explicit_press(c);
-  if (c->hook) c->hook(c, HK_PRESS);
-  rc |= sub_pressure(c);
+  if (c->prm.formula == IMEX_FORMULA_ENERGY) {     // main.cpp:470-486
+    rc |= sub_explicit_energy(c);
+    rc |= sub_energy(c);
+  } else {
+    rc |= sub_explicit_press(c);
+    if (c->hook) c->hook(c, HK_PRESS);
+    rc |= sub_pressure(c);
+  }
   rc |= sub_ct(c);
   if (c->hook) c->hook(c, HK_CT);
   return rc;
@@ -1181,6 +1229,7 @@ int imex_default_params(imex_params* p) {
   p->mixed_outer_tol = 1e-9;
   p->schedule = IMEX_SCHED_GRAPH;
   p->budget_margin = 1;
+  p->formula = IMEX_FORMULA_PRESSURE;                                    // config.hpp:44
   return 0;
 }
 
@@ -1269,6 +1318,8 @@ static int check_params(const imex_params* prm) {
     return set_err(IMEX_ERR_ARG, "bad precision");
   if (prm->schedule != IMEX_SCHED_GRAPH && prm->schedule != IMEX_SCHED_BUDGET && prm->schedule != IMEX_SCHED_HOST)
     return set_err(IMEX_ERR_ARG, "bad schedule");
+  if (prm->formula != IMEX_FORMULA_PRESSURE && prm->formula != IMEX_FORMULA_ENERGY)
+    return set_err(IMEX_ERR_ARG, "bad formula");
   return 0;
 }
 
@@ -1635,6 +1686,8 @@ int imex_mag_update_b(imex_ctx* c) { SINGLE(c); return finish(c, sub_mag_update_
 int imex_explicit_press(imex_ctx* c) { SINGLE(c); return finish(c, sub_explicit_press(c)); }
 int imex_pre_subsystem_pressure(imex_ctx* c) { SINGLE(c); return finish(c, sub_pressure(c)); }
 int imex_ct_update(imex_ctx* c) { SINGLE(c); return finish(c, sub_ct(c)); }
+int imex_explicit_energy(imex_ctx* c) { SINGLE(c); return finish(c, sub_explicit_energy(c)); }
+int imex_pre_subsystem_energy(imex_ctx* c) { SINGLE(c); return finish(c, sub_energy(c)); }
 
 // ---- operator level ----
 static int prep_pressure_operator(imex_ctx* c, int op, const double* p_freeze, double mbig) {
@@ -1662,10 +1715,13 @@ int imex_apply_op(imex_ctx* c, int op, const double* x, double* Ax, const double
   const int nc = op == IMEX_OP_MAG ? 3 : 1;
   const size_t n = (size_t)nc * d.N;
   int rc = 0;
-  if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
+  if (op < IMEX_OP_MAG || op > IMEX_OP_ENERGY) return set_err(IMEX_ERR_ARG, "bad operator");
+  if (op == IMEX_OP_ENERGY) prep_energy(c);
+  else if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_XK], x, n * sizeof(double), hipMemcpyHostToDevice));
-  op_apply(c->L, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), p_ids(c, op == IMEX_OP_P_JACOBIAN, 0), c->P, B_XK, B_AV);
+  const PIds po = op == IMEX_OP_ENERGY ? e_ids(c, 0) : p_ids(c, op == IMEX_OP_P_JACOBIAN, 0);
+  op_apply(c->L, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), po, c->P, B_XK, B_AV);
   if ((rc = finish(c, 0))) return rc;
   HIPCHK(hipMemcpy(Ax, c->hI[0].buf[B_AV], n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
@@ -1680,12 +1736,15 @@ int imex_gmres(imex_ctx* c, int op, const double* b, double* x, int precond, con
   const int nc = op == IMEX_OP_MAG ? 3 : 1;
   const size_t n = (size_t)nc * d.N;
   int rc = 0;
-  if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
+  if (op < IMEX_OP_MAG || op > IMEX_OP_ENERGY) return set_err(IMEX_ERR_ARG, "bad operator");
+  if (op == IMEX_OP_ENERGY) prep_energy(c);
+  else if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_BK], b, n * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_XK], x, n * sizeof(double), hipMemcpyHostToDevice));
-  rc = run_gmres(c, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), p_ids(c, op == IMEX_OP_P_JACOBIAN, precond && op != IMEX_OP_MAG),
-                 B_XK, B_BK, 0, info, nullptr, nullptr);
+  const int pc = precond && op != IMEX_OP_MAG;
+  const PIds po = op == IMEX_OP_ENERGY ? e_ids(c, pc) : p_ids(c, op == IMEX_OP_P_JACOBIAN, pc);
+  rc = run_gmres(c, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), po, B_XK, B_BK, 0, info, nullptr, nullptr);
   int r2 = finish(c, rc);
   if (r2 < 0) return r2;
   HIPCHK(hipMemcpy(x, c->hI[0].buf[B_XK], n * sizeof(double), hipMemcpyDeviceToHost));

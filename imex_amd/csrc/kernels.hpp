@@ -24,7 +24,8 @@ struct KrylovDev {
 };
 
 // Operator descriptors, built per instance inside the kernels.  The magnetic operator has 3
-// unknowns per cell, the pressure operator 1.
+// unknowns per cell, the pressure and energy operators 1 (the energy operator reads its
+// coefficient from `hhat` and its precE divisor from `Tdiag`).
 struct MagOp {
   Dims d;
   const double* Qe;   // padded planes (B in 5..7)
@@ -44,7 +45,7 @@ struct POp {
 };
 // What the host passes: buffer ids and flags.
 struct MagIds { int Qe, u; };
-struct PIds { int u, jacobian, precond; };
+struct PIds { int u, jacobian, precond, energy; };   // energy: Energy_Subsystem_AX on the same slots
 
 __device__ __forceinline__ MagOp make_mag_op(const Inst& I, const MagIds& m) {
   MagOp o; o.d = I.d; o.Qe = gmem(I.buf[m.Qe]); o.u = gmem(I.buf[m.u]); o.S = gmem(I.S); return o;
@@ -85,6 +86,9 @@ void launch_emf_flux(const Launch& L, int iu, int iQe);
 void launch_p_rhs(const Launch& L, int pass, int iu);
 void launch_p_momentum(const Launch& L, int iu);
 void launch_p_energy(const Launch& L, const Phys& P, int iu);
+// energy formula: RHS, coefficient plane, initial guess and precE divisor; then E and momentum update
+void launch_e_rhs(const Launch& L, const Phys& P, int iu, int iQe);
+void launch_e_update(const Launch& L, const Phys& P, int iu);
 void launch_ct(const Launch& L, const Phys& P, int plain, int iu, int ibxqs, int ibyqs, int ibx, int iby);
 void launch_con2pri(const Launch& L, const Phys& P, int icon, int ipri);
 void launch_pri2con(const Launch& L, const Phys& P, int ipri, int icon);
@@ -92,7 +96,8 @@ void launch_divb(const Launch& L, int ibx, int iby);
 void launch_visc_rk(const Launch& L, const Phys& P, int iu);
 
 // ---- Krylov (kernels_krylov.hip) ----
-// op = 0 magnetic (3 unknowns per cell), 1 pressure (1 unknown per cell); prec = 0 FP64 basis and
+// op = 0 magnetic (3 unknowns per cell), 1 pressure or, with po.energy, energy (1 unknown per
+// cell); prec = 0 FP64 basis and
 // operator, 1 FP32 basis and operator (inner solves of the mixed form; dots, norms, Givens and
 // the residual of the refinement stay FP64)
 void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int ix, int ib,

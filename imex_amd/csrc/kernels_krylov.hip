@@ -1,10 +1,10 @@
 // kernels_krylov.hip — matrix-free implicit operators and the device-resident GMRES / Newton pieces.
 //
-// Operators (Magnetic_Subsystem_AX, Pressure_Subsystem_AX; src/implicit_operators.cpp:141-353,
-// 505-616, GLF / MaterialSpeedp branches) are evaluated on 64x4 cell tiles whose input
-// vector, with a one-cell halo mapped through M_P / M_B, is staged in LDS.  In Arnoldi
-// mode the staged values are the unnormalised vector divided by its norm, so the
-// normalisation v = w / ||w|| (implicit_operators.cpp:876) happens once per element
+// Operators (Magnetic_Subsystem_AX, Pressure_Subsystem_AX, Energy_Subsystem_AX;
+// src/implicit_operators.cpp:141-353, 505-616, 741-790, GLF / MaterialSpeedp branches) are
+// evaluated on 64x4 cell tiles whose input vector, with a one-cell halo mapped through M_P / M_B,
+// is staged in LDS.  In Arnoldi mode the staged values are the unnormalised vector divided by its
+// norm, so the normalisation v = w / ||w|| (implicit_operators.cpp:876) happens once per element
 // inside the operator kernel and v is written back in the same pass.
 //
 // GMRES (implicit_operators.cpp:793-919): classical Gram-Schmidt as in the reference —
@@ -253,6 +253,29 @@ __device__ __forceinline__ void p_precond(const POp& op, int cidx, int i, int j,
   y[0] = y[0] / (T)(Vd + op.Tdiag[cidx]);
 }
 
+// ---------------------------------------------------------------- energy operator
+// Energy_Subsystem_AX (implicit_operators.cpp:741-790): (E - (gamma-1) h^2 div(hh grad E)) dx dy with
+// face coefficients hh the means of the cell coefficients; precE divides by diag dx dy.
+template <class T>
+__device__ __forceinline__ void e_apply(const POp& op, const T* xs, int lx, int ly, int i, int j, T* out) {
+  const Dims& d = op.d;
+  const int W = TX + 2;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy, gm1 = (T)(op.P.gamma - 1.0);
+  auto X = [&](int di, int dj) { return xs[(ly + dj) * W + lx + di]; };
+  auto H = [&](int di, int dj) { return (T)op.hhat[pidx(d, i + di, j + dj)]; };
+  T h00 = H(0, 0);
+  T hhp_x = T(0.5) * (h00 + H(1, 0)), hhm_x = T(0.5) * (h00 + H(-1, 0));
+  T hhp_y = T(0.5) * (h00 + H(0, 1)), hhm_y = T(0.5) * (h00 + H(0, -1));
+  T Ei = X(0, 0);
+  T ax = Ei - gm1 * h * h * (T(1) / (dx * dx) * (hhp_x * (X(1, 0) - Ei) - hhm_x * (Ei - X(-1, 0)))
+                           + T(1) / (dy * dy) * (hhp_y * (X(0, 1) - Ei) - hhm_y * (Ei - X(0, -1))));
+  out[0] = ax * dx * dy;
+}
+template <class T>
+__device__ __forceinline__ void e_precond(const POp& op, int cidx, T* y) {
+  if (op.precond) y[0] = y[0] / (T)op.Tdiag[cidx];
+}
+
 // Tile interface shared by the GMRES kernels: load the input vector tile (plus whatever the
 // operator stages), then compute M^-1 A x for the block's cells in arithmetic type T.
 template <class T> struct PSmem { T x[(TY + 2) * (TX + 2)]; };
@@ -290,6 +313,14 @@ template <class T> struct OpSel<1, T> {
   __device__ static T center(const Smem& sm, int, int tx, int ty) { return sm.x[(ty + 1) * (TX + 2) + tx + 1]; }
   __device__ static void precond(const POp& op, int cidx, int i, int j, T* y) { p_precond(op, cidx, i, j, y); }
   __device__ static const Dims& dims(const POp& op) { return op.d; }
+};
+
+template <class T> struct OpSel<2, T> : OpSel<1, T> {
+  typedef PSmem<T> Smem;
+  __device__ static void compute(const POp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
+    if (valid) e_apply(op, sm.x, tx + 1, ty + 1, i, j, y);
+  }
+  __device__ static void precond(const POp& op, int cidx, int, int, T* y) { e_precond(op, cidx, y); }
 };
 
 // ---------------------------------------------------------------- reductions
@@ -838,9 +869,11 @@ void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const P
   const BMap& bm = L.bm[sh];
   if (prec == 0) {
     if (op == 0) k_gmres_residual<0, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    else if (po.energy) k_gmres_residual<2, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
     else k_gmres_residual<1, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
   } else {
     if (op == 0) k_gmres_residual<0, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    else if (po.energy) k_gmres_residual<2, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
     else k_gmres_residual<1, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
   }
 }
@@ -853,9 +886,11 @@ void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PI
   const BMap& bm = L.bm[sh];
   if (prec == 0) {
     if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else if (po.energy) k_arnoldi_apply<2, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
     else k_arnoldi_apply<1, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   } else {
     if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else if (po.energy) k_arnoldi_apply<2, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
     else k_arnoldi_apply<1, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   }
 }
@@ -892,6 +927,7 @@ void gmres_xupdate(const Launch& L, int op, int prec, int ix) {
 
 void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy) {
   if (op == 0) k_op_apply<0><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
+  else if (po.energy) k_op_apply<2><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
   else k_op_apply<1><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
 }
 

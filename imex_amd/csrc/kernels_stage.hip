@@ -336,10 +336,12 @@ __device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL
   }
 }
 
-// Rusanov-type flux with GLF dissipation (flux.cpp:202-243, 329-367; state.cpp:27-44, 86-106, 298-322).
-// kind 0: convective only; kind 1: convective + magnetic pressure-formula terms.
+// Rusanov-type flux with GLF dissipation (flux.cpp:78-163, 202-243, 329-367; state.cpp:27-44, 86-106,
+// 272-346).  kind 0: convective only; kind 1: convective + magnetic pressure-formula terms; kind 2:
+// energy formula — convective without its energy part, magnetic terms with the (2 - gamma) B^2/2
+// normal-momentum and v_n B^2/2 energy parts, and -(gamma - 1) |m|^2 / (2 rho) on the normal momentum.
 template <class T>
-__device__ __forceinline__ void gl_flux(int dir, int kind, const T* QL, const T* QR, T s_max, T* F) {
+__device__ __forceinline__ void gl_flux(int dir, int kind, T gam, const T* QL, const T* QR, T s_max, T* F) {
   T fl[5], fr[5];
   const T* Qs2[2] = {QL, QR};
   T* fo[2] = {fl, fr};
@@ -354,7 +356,23 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const T* QL, const T*
     f[1] = dir == 0 ? rho * vx * vx : rho * vx * vy;
     f[2] = dir == 0 ? rho * vy * vx : rho * vy * vy;
     f[3] = rho * vz * vn;
-    f[4] = kin * vn;
+    f[4] = kind == 2 ? T(0) : kin * vn;
+    if (kind == 2) {
+      T B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
+      T vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) * ir;
+      T fk = -(gam - T(1)) * T(0.5) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * ir;
+      if (dir == 0) {
+        f[1] = f[1] + ((T(2) - gam) * B2 * T(0.5) - Q[5] * Q[5]) + fk;
+        f[2] = f[2] + (-Q[5] * Q[6]);
+        f[3] = f[3] + (-Q[5] * Q[7]);
+        f[4] = f[4] + (vx * B2 * T(0.5) - Q[5] * vB);
+      } else {
+        f[1] = f[1] + (-Q[5] * Q[6]);
+        f[2] = f[2] + ((T(2) - gam) * B2 * T(0.5) - Q[6] * Q[6]) + fk;
+        f[3] = f[3] + (-Q[6] * Q[7]);
+        f[4] = f[4] + (vy * B2 * T(0.5) - Q[6] * vB);
+      }
+    }
     if (kind == 1) {
       T B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
       T vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) * ir;
@@ -374,13 +392,13 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, const T* QL, const T*
   for (int k = 0; k < 5; k++) F[k] = T(0.5) * (fl[k] + fr[k]) - T(0.5) * s_max * (QR[k] - QL[k]);
 }
 
-// RusanovC_Update (kind 0, flux.cpp:409-455) and Pressure_flux_Update (kind 1, flux.cpp:290-326)
-// fused with TVDReconstruction on 32x8 cell tiles.  Per direction: every cell of the tile
+// RusanovC_Update (kind 0, flux.cpp:409-455), Pressure_flux_Update (kind 1, flux.cpp:290-326) and
+// Energy_flux_Update (kind 2, flux.cpp:165-200) fused with TVDReconstruction on 32x8 cell tiles.  Per direction: every cell of the tile
 // (plus one halo cell each side) is reconstructed once into LDS, every face flux is formed
 // once into LDS, then each cell applies its flux difference.  Face states of a cell c are
 // the reconstruction at the ghost-map image of c, as the reference's bound_cond_u on w_*
 // produces.  kind 0 writes u[0..3] (the energy slot of the convective predictor is never
-// read); kind 1 writes u[1..4] (its density equals the convective one bit for bit).
+// read); kinds 1 and 2 write u[1..4] (their density equals the convective one bit for bit).
 // The tile is EX_TX x (256 / EX_TX) cells (EX_TX = 16, 32 or 64, chosen by the autotuner); the
 // tile shape changes which lane forms which face value, not the value.
 template <int EX_TX> struct ExTile {
@@ -442,13 +460,13 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
       QL[k] = k < nrec ? wr[k * EX_NW + itL] : T(0);
       QR[k] = k < nrec ? wl[k * EX_NW + itR] : T(0);
     }
-    if (kind == 1) {
+    if (kind != 0) {
       for (int k = 5; k < 8; k++) { QL[k] = (T)u[k * d.PS + pidx(d, gli, glj)]; QR[k] = (T)u[k * d.PS + pidx(d, gri, grj)]; }
     } else {
       for (int k = 5; k < 8; k++) { QL[k] = T(0); QR[k] = T(0); }
     }
     T Fo[5];
-    gl_flux(DIR, kind, QL, QR, s_max, Fo);
+    gl_flux(DIR, kind, (T)P.gamma, QL, QR, s_max, Fo);
     for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[k];
   }
   __syncthreads();
@@ -704,6 +722,68 @@ __global__ void k_p_energy(Phys P, const Inst* __restrict__ IT, BMap bm, int iu)
   if (ef) atomicOr(eflag, 1);
 }
 
+// ================================================================ energy subsystem (energy formula)
+// RHS of PreSubsystem_Energy (imex_subsystems.cpp:427-472) with the coefficient
+// h = caltotalenthalpy(Qe) rho^e / rho (timestep.cpp:232-237) formed at the cell and its four
+// neighbours from Qe, p(Qe) and rho of u (ghost cells hold boundary copies of all three, so the
+// neighbour values equal what the reference computes there).  Writes the cell's h (the operator's
+// coefficient plane, B_HHAT), b = U* dx dy, the initial guess E (B_P) and the precE divisor
+// diag dx dy (B_TDIAG, :479-500).
+__global__ void __launch_bounds__(256) k_e_rhs(Phys P, const Inst* __restrict__ IT, BMap bm, int iu, int iQe) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* __restrict__ Qe = gmem(I_.buf[iQe]);
+  const double* __restrict__ pe = gmem(I_.buf[B_PE]);
+  double* __restrict__ hE = gmem(I_.buf[B_HHAT]);
+  double* __restrict__ b = gmem(I_.buf[B_PB]);
+  double* __restrict__ x0 = gmem(I_.buf[B_P]);
+  double* __restrict__ D = gmem(I_.buf[B_TDIAG]);
+  const double* S = gmem(I_.S);
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const double h = S[S_ADT], dx = d.dx, dy = d.dy, g = P.gamma;
+  auto H = [&](int a, int bb) {
+    long long c = pidx(d, a, bb);
+    double re = Qe[c];
+    return (Qe[4 * d.PS + c] + pe[c]) / re * re / u[c];
+  };
+  double h00 = H(i, j), hp0 = H(i + 1, j), hm0 = H(i - 1, j), h0p = H(i, j + 1), h0m = H(i, j - 1);
+  long long c = pidx(d, i, j);
+  auto U = [&](int a, int bb, int k) { return u[k * d.PS + pidx(d, a, bb)]; };
+  double e = u[4 * d.PS + c];
+  double U_star = e - 0.5 * h / dx * (hp0 * U(i + 1, j, 1) - hm0 * U(i - 1, j, 1))
+                - 0.5 * h / dy * (h0p * U(i, j + 1, 2) - h0m * U(i, j - 1, 2));
+  int idx = (i - GC) + d.nx * (j - GC);
+  b[idx] = U_star * dx * dy;
+  x0[idx] = e;
+  hE[c] = h00;
+  double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0), hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
+  double diag = 1.0 + (g - 1.0) * h * h * ((hhp_x + hhm_x) / (dx * dx) + (hhp_y + hhm_y) / (dy * dy));
+  D[idx] = diag * dx * dy;
+}
+
+// After the solve (imex_subsystems.cpp:508-519): E <- the solution, then
+// m_x -= h / (2 dx) (gamma - 1) (E(i+1) - E(i-1)) and likewise m_y, reading E through the ghost map.
+__global__ void k_e_update(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ E = gmem(I_.buf[B_P]);
+  double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* S = gmem(I_.S);
+
+  int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const double h = S[S_ADT], g = P.gamma;
+  long long c = pidx(d, i, j);
+  u[4 * d.PS + c] = E[kidx(d, i, j)];
+  u[1 * d.PS + c] -= h * (1 / (2 * d.dx)) * (g - 1) * (E[kidx(d, i + 1, j)] - E[kidx(d, i - 1, j)]);
+  u[2 * d.PS + c] -= h * (1 / (2 * d.dy)) * (g - 1) * (E[kidx(d, i, j + 1)] - E[kidx(d, i, j - 1)]);
+}
+
 // ================================================================ constrained transport
 // CT_CornerAssemble (constrained_transport.cpp:147-164) with Ez_p0 = -Fx, Ez_0p = +Fy and the
 // cell-centre Ez~ = v_y B_x - v_x B_y of the post-pressure state (ct_flux.cpp:155-171).
@@ -931,6 +1011,12 @@ void launch_p_rhs(const Launch& L, int pass, int iu) {
 }
 void launch_p_momentum(const Launch& L, int iu) {
   k_p_momentum<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT64x4], iu);
+}
+void launch_e_rhs(const Launch& L, const Phys& P, int iu, int iQe) {
+  k_e_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu, iQe);
+}
+void launch_e_update(const Launch& L, const Phys& P, int iu) {
+  k_e_update<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu);
 }
 void launch_p_energy(const Launch& L, const Phys& P, int iu) {
   k_p_energy<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu);

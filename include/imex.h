@@ -34,7 +34,8 @@ enum { IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_WENO5 = 4 };  /
 enum { IMEX_EOS_IDEAL = 0, IMEX_EOS_RK = 1 };                       /* EOSType (config.hpp:118) */
 enum { IMEX_CT_CONSISTENT = 0, IMEX_CT_PLAIN = 1 };                 /* Staggered_CT_Consistent / Staggered_CT */
 enum { IMEX_PREC_FP64 = 0, IMEX_PREC_MIXED = 1 };
-enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2 };
+enum { IMEX_FORMULA_ENERGY = 0, IMEX_FORMULA_PRESSURE = 1 };       /* Formula (config.hpp:27) */
+enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2, IMEX_OP_ENERGY = 3 };
 enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2 };
 /* How a step is scheduled.  All three give identical results.
  *   GRAPH:  every solve gets an iteration budget learned from earlier steps; the whole step is
@@ -86,6 +87,10 @@ typedef struct imex_params {
   int mixed_pressure;             /* mixed form: 1 = the pressure solves also run FP32 inner solves; 0 (default)
                                      = they stay FP64 (the pressure system converges in a few FP64 steps, fewer
                                      than the refinement needs) while the magnetic solves and fluxes run FP32 */
+  int formula;                    /* IMEX_FORMULA_* (config.hpp:44, default PRESSURE): which explicit flux and
+                                     which implicit subsystem a step runs after the magnetic subsystem
+                                     (main.cpp:470-486); in the mixed form the energy solve follows
+                                     mixed_pressure */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */
@@ -222,20 +227,27 @@ int imex_mag_update_b(imex_ctx* ctx);
 int imex_explicit_press(imex_ctx* ctx);
 /* PreSubsystem_Pressure (imex_subsystems.cpp:76-302, DefinitionofU). */
 int imex_pre_subsystem_pressure(imex_ctx* ctx);
+/* B slots <- cell B~ then Energy_flux_Update (main.cpp:448-474, flux.cpp:78-200). */
+int imex_explicit_energy(imex_ctx* ctx);
+/* PreSubsystem_Energy (imex_subsystems.cpp:415-534): implicit total-energy solve with the Jacobi
+ * preconditioner precE, then the momentum update by (gamma - 1) grad E. */
+int imex_pre_subsystem_energy(imex_ctx* ctx);
 /* Staggered_CT_Consistent (ct_flux.cpp:150-191) or Staggered_CT (constrained_transport.cpp:66-143). */
 int imex_ct_update(imex_ctx* ctx);
 
 /* ---- operator level (tests, adjoint) on the current stage working set ---- */
-/* Magnetic_Subsystem_AX / Pressure_Subsystem_AX (implicit_operators.cpp:141-353, 505-616).
- * For IMEX_OP_P_JACOBIAN, p_freeze (N) and mbig select V_slope. Host arrays. */
+/* Magnetic_Subsystem_AX / Pressure_Subsystem_AX / Energy_Subsystem_AX (implicit_operators.cpp:141-353,
+ * 505-616, 741-790).  For IMEX_OP_P_JACOBIAN, p_freeze (N) and mbig select V_slope.  The energy
+ * operator's coefficients come from the stage's Qe and u_con.  Host arrays. */
 int imex_apply_op(imex_ctx* ctx, int op, const double* x, double* Ax,
                   const double* p_freeze, double mbig);
-/* Transposed operators for the discrete adjoint: ATx = A^T x for the operator imex_apply_op
- * applies (periodic grids, ideal gas; the pressure Jacobian only where p_freeze is above the
- * floor).  Same arguments as imex_apply_op. */
+/* Transposed operators for the discrete adjoint: ATx = A^T x for the magnetic and pressure
+ * operators imex_apply_op applies (periodic grids, ideal gas; the pressure Jacobian only where
+ * p_freeze is above the floor).  Same arguments as imex_apply_op. */
 int imex_apply_op_T(imex_ctx* ctx, int op, const double* x, double* ATx,
                     const double* p_freeze, double mbig);
-/* GMRES_Solver (implicit_operators.cpp:793-919). precond: 0 none, 1 Jacobi (precP). x in/out. */
+/* GMRES_Solver (implicit_operators.cpp:793-919). precond: 0 none, 1 Jacobi (precP, or precE for
+ * IMEX_OP_ENERGY, imex_subsystems.cpp:479-500). x in/out. */
 int imex_gmres(imex_ctx* ctx, int op, const double* b, double* x, int precond,
                const double* p_freeze, double mbig, imex_solve_info* info);
 /* Pressure_NestedNewton (implicit_operators.cpp:625-738). p in/out. */
@@ -243,7 +255,7 @@ int imex_pressure_newton(imex_ctx* ctx, const double* b, double* p, imex_solve_i
 
 /* ---- discrete adjoint of the step (no counterpart in the reference, which has no derivative) ----
  * Supported: single-instance contexts, periodic grids, ideal gas, Minmod or VanLeer, EMF-consistent
- * CT, FP64 form (other settings return IMEX_ERR_UNSUPPORTED).  Usage: imex_set_state (the initial
+ * CT, pressure formula, FP64 form (other settings return IMEX_ERR_UNSUPPORTED).  Usage: imex_set_state (the initial
  * state x0), imex_adjoint_forward (runs nsteps, keeping a checkpoint every `every` steps; the
  * context then holds the final state), read the final state and form the cotangent of the
  * objective J with respect to it (gu_con: host Grid of conservative variables; gbx, gby: face

@@ -44,6 +44,10 @@ def lib():
         "orc_explicit_press": (C.c_int, [PP, SP, STP]),
         "orc_pre_subsystem_pressure": (C.c_int, [PP, SP, STP, C.POINTER(Stats)]),
         "orc_ct_update": (C.c_int, [PP, SP, STP]),
+        "orc_explicit_energy": (C.c_int, [PP, SP, STP]),
+        "orc_pre_subsystem_energy": (C.c_int, [PP, SP, STP, C.POINTER(Stats)]),
+        "orc_flux": (C.c_int, [PP, C.c_int, C.c_int, dp, dp, C.c_double, dp]),
+        "orc_energy_diag": (C.c_int, [PP, SP, STP, dp]),
         "orc_reconstruct": (C.c_int, [PP, dp, dp, dp, dp, dp]),
         "orc_apply_op": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, dp, C.c_double]),
         "orc_gmres": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, C.c_int, dp, C.c_double, C.POINTER(SolveInfo)]),
@@ -126,7 +130,7 @@ def sub(p, sc, st, name):
     st = _copy(st)
     s = stage_struct(st)
     L = lib()
-    if name in ("mag_update_b", "pre_subsystem_pressure"):
+    if name in ("mag_update_b", "pre_subsystem_pressure", "pre_subsystem_energy"):
         stats = Stats()
         rc = getattr(L, "orc_" + name)(C.byref(p), C.byref(_sc(sc)), C.byref(s), C.byref(stats))
         return st, rc, struct_dict(stats)
@@ -160,6 +164,21 @@ def pressure_newton(p, sc, st, b, p0):
     rc = lib().orc_pressure_newton(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), ptr(b), ptr(pv),
                                    C.byref(info), ptr(mb))
     return pv, struct_dict(info), rc, float(mb[0])
+
+
+def flux(p, kind, direction, QL, QR, s_max):
+    """One face flux: kind 0 RusanovC, 1 Pressure_flux, 2 Energy_flux; direction 0 x, 1 y."""
+    out = np.zeros(5)
+    lib().orc_flux(C.byref(p), kind, direction, ptr(np.ascontiguousarray(QL, dtype=np.float64)),
+                   ptr(np.ascontiguousarray(QR, dtype=np.float64)), s_max, ptr(out))
+    return out
+
+
+def energy_diag(p, sc, st):
+    """Divisor of the energy solve's Jacobi preconditioner per unknown (M_P order)."""
+    D = np.zeros(p.nx * p.ny)
+    lib().orc_energy_diag(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), ptr(D))
+    return D
 
 
 def eos(p, which, a, b):

@@ -9,8 +9,8 @@
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * this library, and only as the checker.  The product path never calls it.
  *
- * Scope: GLF = true, add_pvisc = MaterialSpeedp, PressureFormula,
- * DefinitionofU, all_correction, StaggerCT, ConsState, IMEXdt, RKs = 2,
+ * Scope: GLF = true, add_pvisc = MaterialSpeedp, PressureFormula (DefinitionofU,
+ * all_correction) and EnergyFormula, StaggerCT, ConsState, IMEXdt, RKs = 2,
  * periodic / transmissive boundaries (include/config.hpp:43-65).
  *
  * Parity status of each part (how it is pinned) is listed in the header of
@@ -370,6 +370,23 @@ void mag_flux_p_y(double* f, const double* Q) {
   f[0] = 0; f[1] = -Q[5] * Q[6]; f[2] = B2 * 0.5 - Q[6] * Q[6]; f[3] = -Q[6] * Q[7];
   f[4] = Q[2] / Q[0] * B2 - Q[6] * vB;
 }
+// Energy-formula pieces (state.cpp:272-296, 325-346): magnetic flux with the (2 - gamma) B^2/2
+// momentum term and v B^2/2 energy term, and the -(gamma - 1) |m|^2 / (2 rho) normal-momentum term.
+void mag_flux_e_x(double* f, const double* Q, double gam) {
+  double B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
+  double vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) / Q[0];
+  f[0] = 0; f[1] = (2 - gam) * B2 * 0.5 - Q[5] * Q[5]; f[2] = -Q[5] * Q[6]; f[3] = -Q[5] * Q[7];
+  f[4] = Q[1] / Q[0] * B2 * 0.5 - Q[5] * vB;
+}
+void mag_flux_e_y(double* f, const double* Q, double gam) {
+  double B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
+  double vB = (Q[1] * Q[5] + Q[2] * Q[6] + Q[3] * Q[7]) / Q[0];
+  f[0] = 0; f[1] = -Q[5] * Q[6]; f[2] = (2 - gam) * B2 * 0.5 - Q[6] * Q[6]; f[3] = -Q[6] * Q[7];
+  f[4] = Q[2] / Q[0] * B2 * 0.5 - Q[6] * vB;
+}
+inline double kin_flux(const double* Q, double gam) {
+  return -(gam - 1) * 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) / Q[0];
+}
 // ---------------------------------------------------------------- shock detector (flux.cpp:5-76)
 void shock_detector(Ctx& C, Vec& eta, const Vec& Qe) {
   Vec pri(Qe.size());
@@ -418,10 +435,21 @@ struct Stage {
 };
 
 // Rusanov-type face flux with GLF dissipation s_max = a_max (flux.cpp:202-243, 329-367).
-void rusanov(const imex_scalars& S, int dir, int kind, const double* QL, const double* QR, double* flux) {
+// kind 0 convective (RusanovC), 1 pressure formula (Pressure_flux), 2 energy formula (Energy_flux,
+// flux.cpp:78-163: convective with no energy part, plus magnetic, plus kinetic-energy terms).
+void rusanov(const imex_scalars& S, double gam, int dir, int kind, const double* QL, const double* QR, double* flux) {
   double fL[5], fR[5];
   if (dir == 0) { f_convective(fL, QL); f_convective(fR, QR); }
   else { g_convective(fL, QL); g_convective(fR, QR); }
+  if (kind == 2) {
+    fL[4] = 0; fR[4] = 0;
+    double bL[5], bR[5];
+    if (dir == 0) { mag_flux_e_x(bL, QL, gam); mag_flux_e_x(bR, QR, gam); }
+    else { mag_flux_e_y(bL, QL, gam); mag_flux_e_y(bR, QR, gam); }
+    double kL[5] = {0, 0, 0, 0, 0}, kR[5] = {0, 0, 0, 0, 0};
+    kL[1 + dir] = kin_flux(QL, gam); kR[1 + dir] = kin_flux(QR, gam);
+    for (int k = 0; k < 5; k++) { fL[k] = fL[k] + bL[k] + kL[k]; fR[k] = fR[k] + bR[k] + kR[k]; }
+  }
   if (kind == 1) {
     double bL[5], bR[5];
     if (dir == 0) { mag_flux_p_x(bL, QL); mag_flux_p_x(bR, QR); }
@@ -432,20 +460,21 @@ void rusanov(const imex_scalars& S, int dir, int kind, const double* QL, const d
   for (int k = 0; k < 5; k++) flux[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * s_max * (QR[k] - QL[k]);
 }
 
-// RusanovC_Update (flux.cpp:409-455) and Pressure_flux_Update (flux.cpp:290-326).
+// RusanovC_Update (flux.cpp:409-455), Pressure_flux_Update (flux.cpp:290-326) and
+// Energy_flux_Update (flux.cpp:165-200).
 void flux_update(Ctx& C, const imex_scalars& S, Vec& u_con, const Vec& Qs, const Recon& w, int kind) {
   const double h = S.adt;
   for (int j = GC; j < C.ny + GC; j++)
     for (int i = GC; i < C.nx + GC; i++) {
       double QL[NV], QR[NV], fp[5], fm[5], gp[5], gm[5];
       for (int k = 0; k < NV; k++) { QL[k] = G(w.xR, C, i, j, k); QR[k] = G(w.xL, C, i + 1, j, k); }
-      rusanov(S, 0, kind, QL, QR, fp);
+      rusanov(S, C.P.gamma, 0, kind, QL, QR, fp);
       for (int k = 0; k < NV; k++) { QL[k] = G(w.yR, C, i, j, k); QR[k] = G(w.yL, C, i, j + 1, k); }
-      rusanov(S, 1, kind, QL, QR, gp);
+      rusanov(S, C.P.gamma, 1, kind, QL, QR, gp);
       for (int k = 0; k < NV; k++) { QL[k] = G(w.xR, C, i - 1, j, k); QR[k] = G(w.xL, C, i, j, k); }
-      rusanov(S, 0, kind, QL, QR, fm);
+      rusanov(S, C.P.gamma, 0, kind, QL, QR, fm);
       for (int k = 0; k < NV; k++) { QL[k] = G(w.yR, C, i, j - 1, k); QR[k] = G(w.yL, C, i, j, k); }
-      rusanov(S, 1, kind, QL, QR, gm);
+      rusanov(S, C.P.gamma, 1, kind, QL, QR, gm);
       for (int k = 0; k < 5; k++)
         G(u_con, C, i, j, k) = G(Qs, C, i, j, k) - h / C.dx * (fp[k] - fm[k]) - h / C.dy * (gp[k] - gm[k]);
     }
@@ -887,6 +916,89 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
   return rc;
 }
 
+// ---------------------------------------------------------------- energy subsystem (imex_subsystems.cpp:415-534, implicit_operators.cpp:741-790)
+inline double caltotalenthalpy(Ctx& C, const double* con) {        // timestep.cpp:232-237
+  double pri[NV];
+  con2pri(C, pri, con);
+  return (con[4] + pri[4]) / pri[0];
+}
+// h = caltotalenthalpy(Qe) rho^e / rho on every cell (the reference evaluates it at the cell and
+// its four neighbours, ghost cells included, inside each use)
+void e_coef(Ctx& C, const Vec& u_con, const Vec& Qe, Vec& hE) {
+  hE.assign(C.NX * C.NY, 0.0);
+  for (int j = 0; j < C.NY; j++)
+    for (int i = 0; i < C.NX; i++)
+      A(hE, C, i, j) = caltotalenthalpy(C, &Qe[NV * (i + j * C.NX)]) * G(Qe, C, i, j, 0) / G(u_con, C, i, j, 0);
+}
+// Energy_Subsystem_AX (implicit_operators.cpp:741-790), periodic / transmissive getE through M_P.
+void e_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, Vec& hE, const double* x, double* Ax) {
+  if (C.faithful) e_coef(C, u_con, Qe, hE);   // the reference recomputes these inside every apply (:742-767)
+  const double h = S.adt, dx = C.dx, dy = C.dy, g = C.P.gamma;
+  auto X = [&](int a, int b) { return x[M_P(C, a, b)]; };
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double h00 = A(hE, C, i, j), hp0 = A(hE, C, i + 1, j), hm0 = A(hE, C, i - 1, j);
+      double h0p = A(hE, C, i, j + 1), h0m = A(hE, C, i, j - 1);
+      double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0);
+      double hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
+      double Ei = X(i, j);
+      double ax = Ei - (g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * (X(i + 1, j) - Ei) - hhm_x * (Ei - X(i - 1, j)))
+                                        + 1.0 / (dy * dy) * (hhp_y * (X(i, j + 1) - Ei) - hhm_y * (Ei - X(i, j - 1))));
+      Ax[M_P(C, i, j)] = ax * dx * dy;
+    }
+  C.st.op_applies_p++;
+}
+// precE (imex_subsystems.cpp:479-500): diag (dx dy) with diag = 1 + (gamma-1) h^2 (sum of face coefficients / d^2)
+void e_diag(Ctx& C, const imex_scalars& S, const Vec& hE, Vec& D) {
+  const double h = S.adt, dx = C.dx, dy = C.dy, g = C.P.gamma;
+  D.assign(C.nx * C.ny, 0.0);
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double h00 = A(hE, C, i, j), hp0 = A(hE, C, i + 1, j), hm0 = A(hE, C, i - 1, j);
+      double h0p = A(hE, C, i, j + 1), h0m = A(hE, C, i, j - 1);
+      double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0);
+      double hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
+      double diag = 1.0 + (g - 1.0) * h * h * ((hhp_x + hhm_x) / (dx * dx) + (hhp_y + hhm_y) / (dy * dy));
+      D[M_P(C, i, j)] = diag * dx * dy;
+    }
+}
+
+int pre_subsystem_energy(Ctx& C, const imex_scalars& S, Stage& W) {
+  const double h = S.adt, dx = C.dx, dy = C.dy, g = C.P.gamma;
+  const int N = C.nx * C.ny;
+  Vec hE;
+  e_coef(C, W.u_con, W.Qe, hE);
+  Vec b(N), U(N), D;
+  const Vec& u = W.u_con;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {       // :427-452, 472
+      double hp0 = A(hE, C, i + 1, j), hm0 = A(hE, C, i - 1, j), h0p = A(hE, C, i, j + 1), h0m = A(hE, C, i, j - 1);
+      double U_star = G(u, C, i, j, 4) - 0.5 * h / dx * (hp0 * G(u, C, i + 1, j, 1) - hm0 * G(u, C, i - 1, j, 1))
+                    - 0.5 * h / dy * (h0p * G(u, C, i, j + 1, 2) - h0m * G(u, C, i, j - 1, 2));
+      b[M_P(C, i, j)] = U_star * dx * dy;
+      U[M_P(C, i, j)] = G(u, C, i, j, 4);
+    }
+  e_diag(C, S, hE, D);
+  std::function<void(const double*, double*)> Aop = [&](const double* xin, double* out) { e_apply(C, S, W.u_con, W.Qe, hE, xin, out); };
+  std::function<void(double*, const double*)> precE = [&](double* out, const double* in) {
+    if (C.faithful) { Vec hf; e_coef(C, W.u_con, W.Qe, hf); }   // the reference's precE recomputes h (:481-492)
+    for (int n = 0; n < N; n++) out[n] = in[n] / D[n];
+  };
+  int rc = gmres(C, U, b, Aop, &precE, nullptr, &C.st.p_gmres_iters, &C.st.p_gmres_restarts);
+  C.st.p_solves++;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) G(W.u_con, C, i, j, 4) = U[M_P(C, i, j)];
+  bc_grid(C, W.u_con);
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {       // :514-519
+      Vec& v = W.u_con;
+      G(v, C, i, j, 1) -= h * (1 / (2 * dx)) * (g - 1) * (G(v, C, i + 1, j, 4) - G(v, C, i - 1, j, 4));
+      G(v, C, i, j, 2) -= h * (1 / (2 * dy)) * (g - 1) * (G(v, C, i, j + 1, 4) - G(v, C, i, j - 1, 4));
+    }
+  bc_grid(C, W.u_con);
+  return rc;
+}
+
 // ---------------------------------------------------------------- CT (ct_flux.cpp:150-191, constrained_transport.cpp:66-164)
 void corner_assemble(const Ctx& C, Vec& Ez_pp, const Vec& Ez_p0, const Vec& Ez_0p, const Vec& Et) {
   Vec bar(C.NX * C.NY, 0.0);
@@ -1056,10 +1168,11 @@ int run_step(Run& R) {
     maybe_capture(R, 10 * rks + 2);
     for (size_t n = 0; n < NG; n += NV)
       for (int k = 5; k < 8; k++) { w.xL[n + k] = W.u_con[n + k]; w.xR[n + k] = W.u_con[n + k]; w.yL[n + k] = W.u_con[n + k]; w.yR[n + k] = W.u_con[n + k]; }
-    flux_update(C, S, W.u_con, W.Qs, w, 1);
+    const int energy = C.P.formula == IMEX_FORMULA_ENERGY;     // main.cpp:470-486
+    flux_update(C, S, W.u_con, W.Qs, w, energy ? 2 : 1);
     bc_grid(C, W.u_con);
     maybe_capture(R, 10 * rks + 3);
-    rc |= pre_subsystem_pressure(C, S, W);
+    rc |= energy ? pre_subsystem_energy(C, S, W) : pre_subsystem_pressure(C, S, W);
     bc_grid(C, W.u_con);
     maybe_capture(R, 10 * rks + 4);
     if (C.P.ct_mode == IMEX_CT_PLAIN) ct_plain(C, S, W); else ct_consistent(C, S, W);
@@ -1187,11 +1300,13 @@ static int run_sub(const imex_params* p, const imex_scalars* sc, imex_stage_host
     Recon w; tvd_reconstruction(C, W.Qe, w); flux_update(C, S, W.u_con, W.Qs, w, 0); bc_grid(C, W.u_con);
   } else if (which == 1) {
     rc = mag_update_b(C, S, W); bc_grid(C, W.u_con);
-  } else if (which == 2) {
+  } else if (which == 2 || which == 5) {
     Recon w; tvd_reconstruction(C, W.Qe, w);
     for (size_t n = 0; n < W.u_con.size(); n += NV)
       for (int k = 5; k < 8; k++) { w.xL[n + k] = W.u_con[n + k]; w.xR[n + k] = W.u_con[n + k]; w.yL[n + k] = W.u_con[n + k]; w.yR[n + k] = W.u_con[n + k]; }
-    flux_update(C, S, W.u_con, W.Qs, w, 1); bc_grid(C, W.u_con);
+    flux_update(C, S, W.u_con, W.Qs, w, which == 2 ? 1 : 2); bc_grid(C, W.u_con);
+  } else if (which == 6) {
+    rc = pre_subsystem_energy(C, S, W); bc_grid(C, W.u_con);
   } else if (which == 3) {
     rc = pre_subsystem_pressure(C, S, W); bc_grid(C, W.u_con);
   } else if (which == 4) {
@@ -1207,6 +1322,16 @@ int orc_mag_update_b(const imex_params* p, const imex_scalars* sc, imex_stage_ho
 int orc_explicit_press(const imex_params* p, const imex_scalars* sc, imex_stage_host* st) { return run_sub(p, sc, st, 2, nullptr); }
 int orc_pre_subsystem_pressure(const imex_params* p, const imex_scalars* sc, imex_stage_host* st, imex_stats* s) { return run_sub(p, sc, st, 3, s); }
 int orc_ct_update(const imex_params* p, const imex_scalars* sc, imex_stage_host* st) { return run_sub(p, sc, st, 4, nullptr); }
+int orc_explicit_energy(const imex_params* p, const imex_scalars* sc, imex_stage_host* st) { return run_sub(p, sc, st, 5, nullptr); }
+int orc_pre_subsystem_energy(const imex_params* p, const imex_scalars* sc, imex_stage_host* st, imex_stats* s) { return run_sub(p, sc, st, 6, s); }
+
+// One face flux of the explicit updates: kind 0 RusanovC, 1 Pressure_flux, 2 Energy_flux; dir 0 x, 1 y.
+int orc_flux(const imex_params* p, int kind, int dir, const double* QL, const double* QR, double s_max, double* out) {
+  imex_scalars S{};
+  S.a_max = s_max;
+  rusanov(S, p->gamma, dir, kind, QL, QR, out);
+  return 0;
+}
 
 // TVDReconstruction on a Grid: outputs four Grids.
 int orc_reconstruct(const imex_params* p, const double* u, double* wxL, double* wxR, double* wyL, double* wyR) {
@@ -1225,6 +1350,7 @@ int orc_apply_op(const imex_params* p, const imex_scalars* sc, const imex_stage_
   Ctx C; ctx_init(C, p); C.Mbig = mbig;
   Stage W; stage_from_host(C, W, st);
   if (op == IMEX_OP_MAG) { mag_apply(C, *sc, W.u_con, W.Qe, x, Ax); return 0; }
+  if (op == IMEX_OP_ENERGY) { Vec hE; e_coef(C, W.u_con, W.Qe, hE); e_apply(C, *sc, W.u_con, W.Qe, hE, x, Ax); return 0; }
   PCoef pc; p_coef(C, W.u_con, W.Qe, pc);
   p_apply(C, *sc, W.u_con, W.Qe, pc, x, Ax, op == IMEX_OP_P_JACOBIAN, p_freeze);
   return 0;
@@ -1236,9 +1362,19 @@ int orc_gmres(const imex_params* p, const imex_scalars* sc, const imex_stage_hos
   Stage W; stage_from_host(C, W, st);
   const int N = C.nx * C.ny, dim = op == IMEX_OP_MAG ? 3 * N : N;
   Vec xv(x, x + dim), bv(b, b + dim);
+  const imex_scalars S = *sc;
+  if (op == IMEX_OP_ENERGY) {
+    Vec hE, D;
+    e_coef(C, W.u_con, W.Qe, hE);
+    e_diag(C, S, hE, D);
+    std::function<void(const double*, double*)> Ae = [&](const double* xin, double* out) { e_apply(C, S, W.u_con, W.Qe, hE, xin, out); };
+    std::function<void(double*, const double*)> precE = [&](double* out, const double* in) { for (int n = 0; n < N; n++) out[n] = in[n] / D[n]; };
+    int rc = gmres(C, xv, bv, Ae, precond ? &precE : nullptr, info, nullptr, nullptr);
+    std::memcpy(x, xv.data(), dim * sizeof(double));
+    return rc;
+  }
   PCoef pc;
   if (op != IMEX_OP_MAG) p_coef(C, W.u_con, W.Qe, pc);
-  const imex_scalars S = *sc;
   std::function<void(const double*, double*)> Aop = [&](const double* xin, double* out) {
     if (op == IMEX_OP_MAG) mag_apply(C, S, W.u_con, W.Qe, xin, out);
     else p_apply(C, S, W.u_con, W.Qe, pc, xin, out, op == IMEX_OP_P_JACOBIAN, p_freeze);
@@ -1263,6 +1399,17 @@ int orc_gmres(const imex_params* p, const imex_scalars* sc, const imex_stage_hos
   int rc = gmres(C, xv, bv, Aop, (precond && op != IMEX_OP_MAG) ? &precP : nullptr, info, nullptr, nullptr);
   std::memcpy(x, xv.data(), dim * sizeof(double));
   return rc;
+}
+
+// The energy solve's preconditioner divisor diag dx dy per unknown (M_P order).
+int orc_energy_diag(const imex_params* p, const imex_scalars* sc, const imex_stage_host* st, double* D) {
+  Ctx C; ctx_init(C, p);
+  Stage W; stage_from_host(C, W, st);
+  Vec hE, Dv;
+  e_coef(C, W.u_con, W.Qe, hE);
+  e_diag(C, *sc, hE, Dv);
+  std::memcpy(D, Dv.data(), Dv.size() * sizeof(double));
+  return 0;
 }
 
 int orc_pressure_newton(const imex_params* p, const imex_scalars* sc, const imex_stage_host* st,

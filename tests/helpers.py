@@ -23,12 +23,13 @@ def field_errors(u_test, u_ref):
     return [rel_l2(interior(u_test)[..., k], interior(u_ref)[..., k]) for k in range(u_ref.shape[-1])]
 
 
-def make_case(name, n=None, m=None, **over):
+def make_case(name, n=None, m=None, amp=None, **over):
     p, kw = im.case_of(name)
     if n is not None:
         p.nx = n
         p.ny = m or n
     for k, v in over.items():
         setattr(p, k, v)
-    u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0), kw.get("smooth", 0))
+    a = kw.get("amp", 1.0) if amp is None else amp
+    u, bx, by = im.init_case(p, kw["case"], a, kw.get("u0", 0.0), kw.get("v0", 0.0), kw.get("smooth", 0))
     return p, u, bx, by

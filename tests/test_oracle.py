@@ -12,15 +12,22 @@ term, a wrong sign or index, or a transposed operand fails a check:
   ghost fills / M_P (boundary.cpp, timestep.cpp:166-222, implicit_operators.cpp:5-139)  modular / clamped index brute force
   Minmod / VanLeer (reconstruction.cpp:46-103)   exact on linear data; the vortex central y-slope
   WENO5 (reconstruction.cpp:184-266)        exact face values for cell averages of quadratics; mirror symmetry
-  explicit fluxes (flux.cpp:202-455)        constant state unchanged; discrete conservation of every conserved slot
+  explicit fluxes (flux.cpp:78-455)         constant state unchanged; discrete conservation of every conserved slot;
+                                            explicit + implicit parts of both formulas = textbook ideal MHD flux;
+                                            GLF dissipation -s/2 (QR - QL)
   pressure operator (implicit_operators.cpp:505-616)  symmetry <y,Ax> = <x,Ay>; constant vector -> Vdiag c; linearity
   magnetic operator (implicit_operators.cpp:141-353)  h = 0 is the identity; uniform state + constant x -> x; linearity
+  energy operator (implicit_operators.cpp:741-790)  Fourier-mode eigenvalue on a uniform state; symmetry;
+                                            constant vector -> c dx dy; precE divisor = operator diagonal
+  energy subsystem (imex_subsystems.cpp:415-534)    closed-form solution and momentum update for one Fourier
+                                            mode; total energy and momentum conserved on a periodic grid
   GMRES (implicit_operators.cpp:793-919)    dense numpy.linalg.solve on assembled operators
   nested Newton (implicit_operators.cpp:625-738)  reduces to one linear solve when no cell is floored; p >= theta p_floor
   CT (ct_flux.cpp:150-191)                  discrete div B of the faces unchanged by any EMF
   div B metric (imex_subsystems.cpp:630-660)  closed form on analytic faces
   CFL (timestep.cpp:5-67, 89-120, 254-273)  brute-force numpy maxima and the dt rule
-  whole step (main.cpp:334-595)             mass conservation, div B preservation, vortex advection
+  whole step (main.cpp:334-595)             mass conservation, div B preservation, vortex advection; energy
+                                            formula: mass, momentum and total energy conservation
 
 The composed algebra of MagSubsystem_UpdateB's right-hand side, CollectMagFluxes,
 the two-pass pressure solve and the stage combination follows the reference line by
@@ -234,7 +241,7 @@ def test_explicit_constant_state_unchanged():
     con = orc.pri2con(p, np.tile(pri, (p.ny + 6, p.nx + 6, 1)))
     st = _stage_from(p, con)
     sc = dict(adt=0.01, a_max=1.7, dt=0.03, B_max_x=1.0, B_max_y=1.0, vx_max=0.4, vy_max=0.2)
-    for name, slots in (("explicit_conv", range(4)), ("explicit_press", range(1, 5))):
+    for name, slots in (("explicit_conv", range(4)), ("explicit_press", range(1, 5)), ("explicit_energy", range(1, 5))):
         out, _, _ = orc.sub(p, sc, st, name)
         for k in slots:
             assert np.allclose(out["u_con"][..., k], con[..., k], rtol=1e-14, atol=1e-15), (name, k)
@@ -248,11 +255,58 @@ def test_explicit_conservation_periodic():
     Qe = orc.bc(p, Qe, 0)
     st = _stage_from(p, con, Qe=Qe, Qs=con)
     sc = dict(adt=0.01, a_max=2.0, dt=0.03)
-    for name, slots in (("explicit_conv", range(4)), ("explicit_press", range(1, 5))):
+    for name, slots in (("explicit_conv", range(4)), ("explicit_press", range(1, 5)), ("explicit_energy", range(1, 5))):
         out, _, _ = orc.sub(p, sc, st, name)
         for k in slots:
             d = out["u_con"][G:-G, G:-G, k].sum() - con[G:-G, G:-G, k].sum()
             assert abs(d) <= 1e-11 * np.abs(con[G:-G, G:-G, k]).sum(), (name, k, d)
+
+
+def _mhd_flux(Q, g, d):
+    """Textbook ideal-MHD flux in direction d of the conservative state Q (gamma-law gas)."""
+    rho, m, E, B = Q[0], Q[1:4], Q[4], Q[5:8]
+    v = m / rho
+    B2 = B @ B
+    p = (g - 1) * (E - 0.5 * rho * (v @ v) - 0.5 * B2)
+    F = np.zeros(5)
+    F[0] = m[d]
+    F[1:4] = m[d] * v - B[d] * B
+    F[1 + d] += p + 0.5 * B2
+    F[4] = (E + p + 0.5 * B2) * v[d] - B[d] * (v @ B)
+    return F, p
+
+
+def test_flux_splits_sum_to_mhd_flux():
+    # Energy_Flux (flux.cpp:78-163) leaves (gamma - 1) grad E on the momentum and the enthalpy flux
+    # (E + p) v to PreSubsystem_Energy; Pressure_flux (flux.cpp:202-288) leaves grad p and
+    # (rho e + p) v to PreSubsystem_Pressure.  Explicit plus implicit parts are the MHD flux.
+    p = ideal()
+    g = p.gamma
+    rng = np.random.default_rng(21)
+    for _ in range(5):
+        pri = np.concatenate([[rng.uniform(0.5, 2)], rng.standard_normal(3), [rng.uniform(0.5, 2)], rng.standard_normal(3)])
+        Q = orc.pri2con(p, pri)
+        rho, v, pr, B = pri[0], pri[1:4], pri[4], pri[5:8]
+        for d in (0, 1):
+            F, p_chk = _mhd_flux(Q, g, d)
+            assert p_chk == pytest.approx(pr, rel=1e-13)
+            fe = orc.flux(p, 2, d, Q, Q, 0.7)
+            imp_e = np.zeros(5)
+            imp_e[1 + d] = (g - 1) * Q[4]
+            imp_e[4] = (Q[4] + pr) * v[d]
+            assert np.allclose(fe + imp_e, F, rtol=1e-13, atol=1e-13), (d, fe + imp_e - F)
+            fp = orc.flux(p, 1, d, Q, Q, 0.7)
+            imp_p = np.zeros(5)
+            imp_p[1 + d] = pr
+            imp_p[4] = (pr / (g - 1) + pr) * v[d]
+            assert np.allclose(fp + imp_p, F, rtol=1e-13, atol=1e-13), (d, fp + imp_p - F)
+        # GLF dissipation: F(QL, QR) = (F(QL, QL) + F(QR, QR)) / 2 - s (QR - QL) / 2
+        Q2 = orc.pri2con(p, pri * (1 + 0.1 * rng.standard_normal(8)) + np.r_[0.5, 0, 0, 0, 0.5, 0, 0, 0])
+        for kind in (0, 1, 2):
+            s = 1.9
+            lhs = orc.flux(p, kind, 0, Q, Q2, s)
+            rhs = 0.5 * (orc.flux(p, kind, 0, Q, Q, s) + orc.flux(p, kind, 0, Q2, Q2, s)) - 0.5 * s * (Q2[:5] - Q[:5])
+            assert np.allclose(lhs, rhs, rtol=1e-13, atol=1e-13), kind
 
 
 # ---------------------------------------------------------------- implicit operators
@@ -309,7 +363,90 @@ def test_magnetic_operator_identity_and_linearity():
     assert np.allclose(orc.apply_op(p, sc, st_u, im.OP_MAG, xc), xc, rtol=1e-14, atol=1e-15)
 
 
-@pytest.mark.parametrize("op,precond", [(im.OP_MAG, 0), (im.OP_P_CARRIER, 1), (im.OP_P_CARRIER, 0)])
+def _uniform_stage(p, pri):
+    con = orc.pri2con(p, np.tile(np.asarray(pri, dtype=float), (p.ny + 6, p.nx + 6, 1)))
+    return _stage_from(p, con), con
+
+
+def test_energy_operator_fourier_eigenvalue():
+    # uniform state: h = (E + p) / rho, A = dx dy (I - (gamma-1) h^2 H Laplacian_5pt), so the grid
+    # mode cos(2 pi (kx i / nx + ky j / ny)) has eigenvalue dx dy (1 + (gamma-1) h^2 H (4 sin^2(pi kx/nx)/dx^2 + ...))
+    p = ideal(12, 10)
+    p.x1, p.y1 = 1.5, 0.7
+    pri = [1.3, 0.4, -0.2, 0.1, 0.9, 0.3, -0.5, 0.2]
+    st, con = _uniform_stage(p, pri)
+    E = con[0, 0, 4]
+    H = (E + pri[4]) / pri[0]
+    dx, dy = 1.5 / p.nx, 0.7 / p.ny
+    h = 0.013
+    sc = dict(adt=h)
+    I, J = np.meshgrid(np.arange(p.nx), np.arange(p.ny))
+    for kx, ky in [(1, 0), (0, 2), (3, 1), (5, 4)]:
+        x = np.cos(2 * np.pi * (kx * I / p.nx + ky * J / p.ny)).ravel()
+        lam = dx * dy * (1 + (p.gamma - 1) * h * h * H * (4 * np.sin(np.pi * kx / p.nx) ** 2 / dx ** 2
+                                                         + 4 * np.sin(np.pi * ky / p.ny) ** 2 / dy ** 2))
+        Ax = orc.apply_op(p, sc, st, im.OP_ENERGY, x)
+        assert np.allclose(Ax, lam * x, rtol=0, atol=1e-13 * lam), (kx, ky)
+
+
+def test_energy_operator_symmetric_constant_and_precond_diagonal():
+    p, u, bx, by = make_case("ot", 10, 8, formula=im.FORMULA_ENERGY)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(1)
+    st, sc, _ = o.step_capture(3)
+    A = _assemble(p, sc, st, im.OP_ENERGY)
+    assert np.allclose(A, A.T, rtol=0, atol=1e-13 * np.abs(A).max())
+    dxdy = (2 * np.pi / p.nx) * (2 * np.pi / p.ny)
+    c = np.full(p.nx * p.ny, 2.5)
+    assert np.allclose(orc.apply_op(p, sc, st, im.OP_ENERGY, c), 2.5 * dxdy, rtol=1e-13)
+    D = orc.energy_diag(p, sc, st)
+    assert np.allclose(D, np.diag(A), rtol=1e-13)
+
+
+def test_energy_subsystem_fourier_mode_closed_form():
+    # uniform Qe and state except m_x = M sin(theta_i): U* = E0 - (h/dx) H M sin(d) cos(theta_i) with
+    # d = 2 pi / nx, the solve gives E = E0 + alpha cos(theta_i) with alpha = -(h/dx) H M sin(d) / lam,
+    # and the momentum update adds (h/dx) (gamma-1) alpha sin(d) sin(theta_i)
+    p = ideal(16, 6)
+    pri = [1.1, 0.0, 0.0, 0.0, 0.8, 0.2, 0.3, -0.1]
+    st, con = _uniform_stage(p, pri)
+    M = 0.05
+    NX = p.nx + 2 * G
+    th = 2 * np.pi * (np.arange(NX) - G) / p.nx
+    st["u_con"][..., 1] = M * np.sin(th)[None, :]
+    g, h, dx = p.gamma, 0.02, 1.0 / p.nx
+    E0 = con[0, 0, 4]
+    H = (E0 + pri[4]) / pri[0]
+    d = 2 * np.pi / p.nx
+    lam = 1 + (g - 1) * h * h * H * 4 * np.sin(np.pi / p.nx) ** 2 / dx ** 2
+    alpha = -(h / dx) * H * M * np.sin(d) / lam
+    out, rc, stats = orc.sub(p, dict(adt=h), st, "pre_subsystem_energy")
+    assert rc == 0 and stats["p_solves"] == 1
+    Ei = out["u_con"][G:-G, G:-G, 4]
+    assert np.allclose(Ei, E0 + alpha * np.cos(th[G:-G])[None, :], rtol=0, atol=1e-12 * E0)
+    mx = out["u_con"][G:-G, G:-G, 1]
+    mx_exp = M * np.sin(th[G:-G]) + (h / dx) * (g - 1) * alpha * np.sin(d) * np.sin(th[G:-G])
+    assert np.allclose(mx, mx_exp[None, :], rtol=0, atol=1e-13)
+    assert np.allclose(out["u_con"][G:-G, G:-G, 2], 0.0, atol=1e-15)
+
+
+def test_energy_subsystem_conserves_totals():
+    p, u, bx, by = make_case("ot", 16, 12, formula=im.FORMULA_ENERGY)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(1)
+    st, sc, _ = o.step_capture(13)
+    out, rc, _ = orc.sub(p, sc, st, "pre_subsystem_energy")
+    assert rc == 0
+    for k in (1, 2, 4):
+        a, b = out["u_con"][G:-G, G:-G, k].sum(), st["u_con"][G:-G, G:-G, k].sum()
+        assert abs(a - b) <= 1e-12 * np.abs(st["u_con"][G:-G, G:-G, k]).sum(), k
+    assert not np.allclose(out["u_con"][..., 4], st["u_con"][..., 4])
+
+
+@pytest.mark.parametrize("op,precond", [(im.OP_MAG, 0), (im.OP_P_CARRIER, 1), (im.OP_P_CARRIER, 0), (im.OP_ENERGY, 1),
+                                         (im.OP_ENERGY, 0)])
 def test_gmres_against_dense_solve(op, precond):
     p, st, sc = _snapshot("ot" if op else "vortex", 8, 6, 1 if op == im.OP_MAG else 3)
     A = _assemble(p, sc, st, op)
@@ -420,8 +557,26 @@ def test_run_conservation_and_divb(case):
         assert np.all(up[G:-G, G:-G, 4] > 0)
 
 
-def test_faithful_mode_is_bitwise_identical():
-    p, u, bx, by = make_case("ot", 16)
+@pytest.mark.parametrize("case", ["vortex", "ot"])
+def test_run_energy_formula_conserves_totals(case):
+    p, u, bx, by = make_case(case, 24, formula=im.FORMULA_ENERGY)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(1)
+    c0 = o.get_state()[0][G:-G, G:-G].copy()
+    o.step(3)
+    uc, up, _, _, sc = o.get_state()
+    for k in range(5):
+        d = uc[G:-G, G:-G, k].sum() - c0[..., k].sum()
+        assert abs(d) <= 1e-12 * max(np.abs(c0[..., k]).sum(), 1.0), (k, d)
+    assert o.stats()["div_b"] <= 1e-12
+    assert np.all(up[G:-G, G:-G, 4] > 0)
+    assert o.stats()["newton_iters"] == 0 and o.stats()["p_solves"] == 8
+
+
+@pytest.mark.parametrize("formula", [im.FORMULA_PRESSURE, im.FORMULA_ENERGY])
+def test_faithful_mode_is_bitwise_identical(formula):
+    p, u, bx, by = make_case("ot", 16, formula=formula)
     outs = []
     for f in (0, 1):
         o = orc.Oracle(p, f)
