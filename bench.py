@@ -1,8 +1,9 @@
 """Throughput of the IMEX-MHD step on one MI355X.
 
 Workload: Orszag-Tang on periodic [0,2pi]^2 at 1024x1024, gamma 5/3, WENO5,
-StaggerCT (EMF-consistent), PressureFormula, floor_on with p_floor 1e-8,
-IMEX-RK2, CFL 0.8 — synthetic initial data generated on the host.
+StaggerCT (EMF-consistent), PressureFormula (--formula energy: EnergyFormula),
+floor_on with p_floor 1e-8, IMEX-RK2, CFL 0.8 — synthetic initial data
+generated on the host.
 
 A "step" is one full IMEX-RK2 step (two stages: explicit fluxes, magnetic GMRES,
 EMF collection, two nested-Newton pressure solves, CT) with all fields resident in
@@ -26,11 +27,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(name, steps, warmup, precision, schedule, tols=None):
+def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None):
     import imex_amd as im
     p, kw = im.case_of(name)
     p.precision = precision
     p.schedule = schedule
+    if formula is not None:
+        p.formula = formula
     if tols:
         p.mixed_inner_tol, p.mixed_outer_tol = tols
     u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
@@ -70,7 +73,7 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None):
     return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final, tune=tune)
 
 
-def run_pair(name, steps, warmup, schedule):
+def run_pair(name, steps, warmup, schedule, formula=None):
     """FP64 and mixed forms of one workload (no per-kernel pass): throughput and the mixed form's error."""
     import numpy as np
     import imex_amd as im
@@ -80,6 +83,8 @@ def run_pair(name, steps, warmup, schedule):
         p, kw = im.case_of(name)
         p.precision = prec
         p.schedule = schedule
+        if formula is not None:
+            p.formula = formula
         u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
         s = im.Solver(p)
         try:
@@ -235,11 +240,13 @@ def run_adjoint(n, steps, every, ndir, seed, eps=1e-6, noise=1e-3):
             "worst_solve_residual": info["worst_solve_residual"]}
 
 
-def cpu_baseline(name, state, budget_steps=1):
+def cpu_baseline(name, state, budget_steps=1, formula=None):
     """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
     import imex_amd as im
     import oracle as orc
     p, kw = im.case_of(name)
+    if formula is not None:
+        p.formula = formula
     uc, up, bx, by, sc = state
     o = orc.Oracle(p, 1)
     o.set_state(up, bx, by, sc["t"], sc["timestep"], sc["prev_dt"])
@@ -266,19 +273,25 @@ KERNEL_OF_CLASS = {   # (class, form) -> kernel symbol in the committed rocprof 
     ("mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<0, float, float>",
     ("p_arnoldi_apply", "fp64"): "k_arnoldi_apply<1, double, double>",
     ("p_arnoldi_apply", "mixed"): "k_arnoldi_apply<1, double, double>",
+    ("e_arnoldi_apply", "fp64"): "k_arnoldi_apply<2, double, double>",
+    ("e_arnoldi_apply", "mixed"): "k_arnoldi_apply<2, double, double>",
+    ("explicit_energy", "fp64"): "k_explicit<32, double>",
+    ("explicit_energy", "mixed"): "k_explicit<32, float>",
 }
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "imex_step_ot1024_kernels.md")
+TRAFFIC_SUMMARY = {"pressure": os.path.join(ROOT, "profiles", "imex_step_ot1024_kernels.md"),
+                   "energy": os.path.join(ROOT, "profiles", "imex_step_ot1024_energy_kernels.md")}
 
 
-def measured_traffic(cls, form):
+def measured_traffic(cls, form, formula="pressure"):
     """HBM bytes per active dispatch of the kernel behind `cls`, from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (calibrated against a plain copy); None if it is not listed there."""
     import re
     name = KERNEL_OF_CLASS.get((cls, form))
-    if not name or not os.path.exists(TRAFFIC_SUMMARY):
+    path = TRAFFIC_SUMMARY[formula]
+    if not name or not os.path.exists(path):
         return None
     in_traffic = False
-    for line in open(TRAFFIC_SUMMARY):
+    for line in open(path):
         if line.startswith("## HBM traffic"):
             in_traffic = True
         m = re.match(r"\| `(.+?)` \| (\d+) \| ([\d.]+) \| ([\d.]+) \|", line)
@@ -287,7 +300,7 @@ def measured_traffic(cls, form):
     return None
 
 
-def summarize(r, steps, label):
+def summarize(r, steps, label, formula="pressure"):
     """Throughput, per-class table (stderr) and the roofline of the dominant kernel class."""
     p = r["p"]
     value = p.nx * p.ny * steps / r["elapsed"] / 1e6
@@ -306,7 +319,7 @@ def summarize(r, steps, label):
     log(f"[bench] {label} profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
         f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
     st = r["stats"]
-    traffic = measured_traffic(dom["name"], label)
+    traffic = measured_traffic(dom["name"], label, formula)
     return {
         "value": round(value, 3),
         "ms_per_step": round(r["elapsed"] / steps * 1e3, 3),
@@ -314,7 +327,7 @@ def summarize(r, steps, label):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per active dispatch of "
                                         f"{KERNEL_OF_CLASS.get((dom['name'], label))}, calibrated, "
-                                        "profiles/imex_step_ot1024_kernels.md") if traffic else None,
+                                        f"{os.path.relpath(TRAFFIC_SUMMARY[formula], ROOT)}") if traffic else None,
                      "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "gmres": {k: st[k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters",
                                      "mixed_refinements")},
@@ -347,6 +360,10 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--blast-steps", type=int, default=20)
+    ap.add_argument("--formula", default="pressure", choices=["pressure", "energy"],
+                    help="implicit subsystem of the headline run (PressureFormula or EnergyFormula)")
+    ap.add_argument("--energy-steps", type=int, default=40,
+                    help="timed steps of the energy-formula FP64 / mixed pair among the secondary workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -368,12 +385,14 @@ def main():
                for e, eos in (("ideal", im.EOS_IDEAL), ("redlich_kwong", im.EOS_RK))}
         print(json.dumps({"batch": res}), flush=True)
         return
+    formula = {"pressure": im.FORMULA_PRESSURE, "energy": im.FORMULA_ENERGY}[args.formula]
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {}
     for f in forms:
         prec = im.PREC_FP64 if f == "fp64" else im.PREC_MIXED
-        runs[f] = run_gpu(args.workload, args.steps, args.warmup, prec, sched, args.mixed_tols if f == "mixed" else None)
-    res = {f: summarize(runs[f], args.steps, f) for f in forms}
+        runs[f] = run_gpu(args.workload, args.steps, args.warmup, prec, sched, args.mixed_tols if f == "mixed" else None,
+                          formula)
+    res = {f: summarize(runs[f], args.steps, f, args.formula) for f in forms}
     head = "mixed" if "mixed" in res else "fp64"
     p = runs[head]["p"]
     out = {
@@ -391,7 +410,7 @@ def main():
         "data": "synthetic (Orszag-Tang initial data generated on the host)",
         "config": {"workload": f"orszag_tang_{p.nx}x{p.ny}_weno5_staggerct_rk2", "nx": p.nx, "ny": p.ny,
                    "recon": "WENO5", "p_floor": p.p_floor, "gamma": p.gamma, "form": head,
-                   "schedule": args.schedule},
+                   "formula": args.formula, "schedule": args.schedule},
         "roofline": res[head]["roofline"],
         "gmres": res[head]["gmres"],
         "schedule": res[head]["schedule"],
@@ -417,13 +436,18 @@ def main():
                         "steps": args.batch_steps, "warmup": 1,
                         "ideal": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_IDEAL, 2024, sched),
                         "redlich_kwong": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_RK, 2024, sched)}
+        if args.formula == "pressure":
+            out["energy_formula"] = dict(workload="Orszag-Tang 1024x1024 WENO5 with EnergyFormula (Energy_flux_Update + "
+                                                  "PreSubsystem_Energy in place of the pressure subsystem)",
+                                         steps=args.energy_steps,
+                                         warmup=3, **run_pair(args.workload, args.energy_steps, 3, sched, im.FORMULA_ENERGY))
         out["adjoint"] = run_adjoint(256, 20, 5, args.adjoint_dirs, 1)
     if not args.no_cpu:
         log("[bench] CPU oracle sample")
-        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps)
+        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula)
         out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang from the post-warmup "
-                                         f"state, faithful mode, {secs:.1f} s"}
+                               "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang ({args.formula} formula) "
+                                         f"from the post-warmup state, faithful mode, {secs:.1f} s"}
     print(json.dumps(out), flush=True)
 
 
