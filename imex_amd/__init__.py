@@ -8,7 +8,8 @@ from .capi import (  # noqa: F401
     default_params, vortex_params, orszag_tang_params, blast_params, case_of, grid_shape, array_shape,
     BC_PERIODIC, BC_TRANSMISSIVE, RECON_MINMOD, RECON_VANLEER, RECON_WENO5, EOS_IDEAL, EOS_RK,
     CT_CONSISTENT, CT_PLAIN, PREC_FP64, PREC_MIXED, OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN, OP_ENERGY,
-    FORMULA_ENERGY, FORMULA_PRESSURE,
+    FORMULA_ENERGY, FORMULA_PRESSURE, MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A,
+    MAGFIELD_NODIVFREE, OP_POTENTIAL, init_potential,
     CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST, GC, NVAR,
     TUNE_EXPLICIT, TUNE_KRYLOV,
 )

@@ -16,8 +16,9 @@ RECON_MINMOD, RECON_VANLEER, RECON_WENO5 = 1, 2, 4
 EOS_IDEAL, EOS_RK = 0, 1
 CT_CONSISTENT, CT_PLAIN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN, OP_ENERGY = 0, 1, 2, 3
+OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN, OP_ENERGY, OP_POTENTIAL = 0, 1, 2, 3, 4
 FORMULA_ENERGY, FORMULA_PRESSURE = 0, 1
+MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A, MAGFIELD_NODIVFREE = 0, 1, 2, 3
 CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
 SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
 TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
@@ -42,7 +43,7 @@ class Params(C.Structure):
         ("ct_mode", C.c_int), ("precision", C.c_int),
         ("mixed_inner_tol", C.c_double), ("mixed_outer_tol", C.c_double),
         ("schedule", C.c_int), ("budget_margin", C.c_int), ("mixed_pressure", C.c_int),
-        ("formula", C.c_int),
+        ("formula", C.c_int), ("magfield", C.c_int),
     ]
 
 
@@ -186,6 +187,15 @@ def lib():
         "imex_pre_subsystem_pressure": (C.c_int, [vp]),
         "imex_explicit_energy": (C.c_int, [vp]),
         "imex_pre_subsystem_energy": (C.c_int, [vp]),
+        "imex_init_potential": (C.c_int, [C.POINTER(Params), C.c_int, C.c_double, dp]),
+        "imex_set_potential": (C.c_int, [vp, dp]),
+        "imex_get_potential": (C.c_int, [vp, dp]),
+        "imex_batch_set_potential": (C.c_int, [vp, C.c_int, dp]),
+        "imex_batch_get_potential": (C.c_int, [vp, C.c_int, dp]),
+        "imex_set_stage_potential": (C.c_int, [vp, dp, dp]),
+        "imex_get_stage_potential": (C.c_int, [vp, dp, dp]),
+        "imex_mag_update_a": (C.c_int, [vp]),
+        "imex_unstaggered_ct": (C.c_int, [vp]),
         "imex_ct_update": (C.c_int, [vp]),
         "imex_apply_op": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
         "imex_apply_op_T": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
@@ -236,6 +246,15 @@ def init_case(p, case, amp=1.0, u0=0.0, v0=0.0, smooth=0):
     return u, bx, by
 
 
+def init_potential(p, case, amp=1.0):
+    """Vector potential of a built-in case (Grid layout [j][i][k], A_x, A_y, A_z in slots 0..2)."""
+    A = np.zeros(grid_shape(p))
+    rc = lib().imex_init_potential(C.byref(p), case, amp, ptr(A))
+    if rc < 0:
+        raise RuntimeError(lib().imex_last_error().decode())
+    return A
+
+
 def new_stage(p):
     g, a = grid_shape(p), array_shape(p)
     return {k: np.zeros(g) for k in ("u_con", "Qe", "Qs")} | {k: np.zeros(a) for k in ("bx", "by", "bx_qs", "by_qs", "fx", "fy")}
@@ -283,6 +302,23 @@ class Solver:
     def set_state(self, u_pri, bx, by, t=0.0, timestep=0, prev_dt=0.0):
         return self._chk(self.L.imex_set_state(self.h, ptr(np.ascontiguousarray(u_pri)), ptr(np.ascontiguousarray(bx)),
                                                ptr(np.ascontiguousarray(by)), t, timestep, prev_dt), "set_state")
+
+    def set_potential(self, A):
+        return self._chk(self.L.imex_set_potential(self.h, ptr(np.ascontiguousarray(A, dtype=np.float64))), "set_potential")
+
+    def get_potential(self):
+        A = np.zeros(grid_shape(self.p))
+        self._chk(self.L.imex_get_potential(self.h, ptr(A)), "get_potential")
+        return A
+
+    def set_stage_potential(self, A, As):
+        return self._chk(self.L.imex_set_stage_potential(self.h, ptr(np.ascontiguousarray(A, dtype=np.float64)),
+                                                         ptr(np.ascontiguousarray(As, dtype=np.float64))), "set_stage_potential")
+
+    def get_stage_potential(self):
+        A, As = np.zeros(grid_shape(self.p)), np.zeros(grid_shape(self.p))
+        self._chk(self.L.imex_get_stage_potential(self.h, ptr(A), ptr(As)), "get_stage_potential")
+        return A, As
 
     def get_state(self):
         p = self.p
@@ -346,7 +382,15 @@ class Solver:
     def sub(self, name):
         return self._chk(getattr(self.L, "imex_" + name)(self.h), name)
 
+    def _n_unknowns(self, op, *vecs):
+        n = self.p.nx * self.p.ny * (3 if op in (OP_MAG, OP_POTENTIAL) else 1)
+        for v in vecs:
+            if np.asarray(v).size != n:
+                raise ValueError(f"operator {op} takes {n} unknowns, got {np.asarray(v).size}")
+        return n
+
     def apply_op(self, op, x, p_freeze=None, mbig=0.0):
+        self._n_unknowns(op, x)
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
@@ -354,6 +398,7 @@ class Solver:
         return y
 
     def gmres(self, op, b, x0, precond=0, p_freeze=None, mbig=0.0):
+        self._n_unknowns(op, b, x0)
         x = np.array(x0, dtype=np.float64, copy=True)
         b = np.ascontiguousarray(b, dtype=np.float64)
         pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
@@ -362,6 +407,7 @@ class Solver:
         return x, struct_dict(info), rc
 
     def apply_op_T(self, op, x, p_freeze=None, mbig=0.0):
+        self._n_unknowns(op, x)
         out = np.zeros_like(x)
         self._chk(self.L.imex_apply_op_T(self.h, op, ptr(np.ascontiguousarray(x)), ptr(out), ptr(p_freeze), mbig), "apply_op_T")
         return out
@@ -425,6 +471,15 @@ class Batch(Solver):
         sc = Scalars()
         self._chk(self.L.imex_batch_get_state(self.h, i, ptr(u_con), ptr(u_pri), ptr(bx), ptr(by), C.byref(sc)), "batch_get_state")
         return u_con, u_pri, bx, by, struct_dict(sc)
+
+    def set_potential_of(self, i, A):
+        return self._chk(self.L.imex_batch_set_potential(self.h, i, ptr(np.ascontiguousarray(A, dtype=np.float64))),
+                         "batch_set_potential")
+
+    def get_potential_of(self, i):
+        A = np.zeros(grid_shape(self.params_of(i)))
+        self._chk(self.L.imex_batch_get_potential(self.h, i, ptr(A)), "batch_get_potential")
+        return A
 
     def div_b_of(self, i):
         return self.L.imex_batch_div_b(self.h, i)

@@ -11,13 +11,15 @@
 #define MAXM 32            // largest GMRES restart length supported
 #define MAXSWEEP 16        // refinement sweeps recorded per mixed-form solve
 
-// FP64 buffers of an instance.  Grids (B_UPRI..B_QS) are 8 planes; the others one plane or one
+// FP64 buffers of an instance.  Grids (B_UPRI..B_QS) are 8 planes; the vector potentials (B_A0..B_AS,
+// allocated only when the magnetic treatment evolves A) 3 planes; the others one plane or one
 // compact Krylov vector (3 components for the magnetic system).
 enum BufId {
   B_UPRI, B_UN, B_U1, B_U2, B_U3, B_QE, B_QS,
   B_BX0, B_BY0, B_BX1, B_BY1, B_BX2, B_BY2, B_BXQS, B_BYQS, B_FX, B_FY,
   B_PE, B_CF, B_CHI, B_ETA, B_HHAT, B_MXS, B_MYS, B_EZPP, B_EZX, B_EZY, B_VISC,
   B_W, B_AV, B_XK, B_BK, B_TMPN, B_P, B_PB, B_NG, B_DP, B_TDIAG, B_VSLOPE,
+  B_A0, B_A1, B_A2, B_AS,
   B_V0, B_VEND = B_V0 + MAXM + 1,
   NBUF = B_VEND
 };

@@ -16,7 +16,9 @@
 // Step-global scalars kept in device memory so that no kernel needs a host round trip.
 enum ScalarSlot {
   S_T = 0, S_DT, S_ADT, S_AMAX, S_BMAXX, S_BMAXY, S_VXMAX, S_VYMAX, S_VMAX,
-  S_CH, S_CHX, S_CHY, S_PREVDT, S_TIMESTEP, S_RKS, S_MBIG, S_ADT0, S_NSLOTS = 32
+  S_CH, S_CHX, S_CHY, S_PREVDT, S_TIMESTEP, S_RKS, S_MBIG, S_ADT0,
+  S_GBX, S_GBY,        // viscosity maxima of the vector-potential operator (magnetic eigenvalues of Qs)
+  S_NSLOTS = 32
 };
 
 struct Dims {

@@ -88,14 +88,14 @@ enum KClass {
   KC_START, KC_COMBINE, KC_QEPREP, KC_SHOCK, KC_EXPL_CONV, KC_EXPL_PRESS, KC_MAG_RHS, KC_MAG_APPLY, KC_MAG_UPDATE,
   KC_MAG_RES, KC_MAG_XUPD, KC_P_APPLY, KC_P_UPDATE, KC_P_RES, KC_P_XUPD, KC_NEWTON_RES, KC_EMF, KC_P_RHS, KC_P_MOM,
   KC_P_ENERGY, KC_CT, KC_BC, KC_MISC, KC_KFIN, KC_EARLY_EXIT, KC_EXPL_ENERGY, KC_E_RHS, KC_E_APPLY, KC_E_UPDATE,
-  KC_E_RES, KC_E_XUPD, KC_E_MOM, KC_N
+  KC_E_RES, KC_E_XUPD, KC_E_MOM, KC_A_UPDATE, KC_N
 };
 static const char* kc_names[KC_N] = {
   "step_start", "stage_combine", "qe_prep", "shock", "explicit_conv", "explicit_press", "mag_rhs", "mag_arnoldi_apply",
   "mag_arnoldi_update", "mag_gmres_residual", "mag_gmres_xupdate", "p_arnoldi_apply", "p_arnoldi_update",
   "p_gmres_residual", "p_gmres_xupdate", "newton_residual", "emf_flux", "p_rhs", "p_momentum", "p_energy", "ct",
   "ghost_fill", "misc", "krylov_finish", "early_exit", "explicit_energy", "e_rhs", "e_arnoldi_apply",
-  "e_arnoldi_update", "e_gmres_residual", "e_gmres_xupdate", "e_momentum"};
+  "e_arnoldi_update", "e_gmres_residual", "e_gmres_xupdate", "e_momentum", "a_update"};
 
 struct ProfRec { int cls; int e0, e1; double bytes; };
 struct Prof {
@@ -149,7 +149,7 @@ struct GraphEntry {
 #define NBK 5
 
 // buffer ids of the stage being queued
-struct StageBind { int u, Qe, Qs, bxqs, byqs, bxo, byo; };
+struct StageBind { int u, Qe, Qs, bxqs, byqs, bxo, byo, As, Ao; };   // As/Ao: the stage's A_s and output A
 
 // Points of a host-scheduled step at which a recording hook (the adjoint's tape) is called.
 enum HookPoint { HK_START, HK_IN, HK_CONV, HK_MAG, HK_PRESS, HK_P0, HK_MOM, HK_P1, HK_PR, HK_CT };
@@ -183,6 +183,10 @@ struct imex_ctx {
   // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo;
   // faces likewise (bxn/byn read, bx2/by2 written)
   int iUs = B_U2, iUo = B_U3, iBxn = B_BX0, iByn = B_BY0, iBx2 = B_BX2, iBy2 = B_BY2;
+  // vector potential (UNSTAGGER / POTENTIAL_A): the step reads An, stage 1 writes A1, stage 2 writes Ao
+  bool needA = false;
+  int iAn = B_A0, iAo = B_A2;
+  std::vector<char> a_set;           // per instance: imex_set_potential was called
   KrylovDev* Kh = nullptr;     // pinned host mirrors of instance 0's control block (2 slots)
   KrylovDev Klast;             // status of the last GMRES cycle read by the host
   double* Sh = nullptr;        // pinned host copy of one instance's scalars
@@ -380,7 +384,8 @@ static int build_maps(imex_ctx* c) {
 // solve at budget key k
 static inline int op_prec(const imex_ctx* c, int op) { return (c->mixed_ops >> op) & 1; }
 static inline int key_prec(const imex_ctx* c, int k) { return op_prec(c, (k / NIT) % 3 == 0 ? 0 : 1); }
-static inline MagIds mag_ids(const imex_ctx* c) { MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; return m; }
+static inline MagIds mag_ids(const imex_ctx* c) { MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; m.pot = 0; return m; }
+static inline MagIds pot_ids(const imex_ctx* c) { MagIds m = mag_ids(c); m.pot = 1; return m; }
 static inline PIds p_ids(const imex_ctx* c, int jacobian, int precond) {
   PIds q; q.u = c->cur.u; q.jacobian = jacobian; q.precond = precond; q.energy = 0; return q;
 }
@@ -711,7 +716,9 @@ static int sub_explicit_conv(imex_ctx* c) {
   return 0;
 }
 
-static int sub_mag_update_b(imex_ctx* c) {
+// emf: CollectMagFluxes (ct_flux.cpp:5-147), whose fluxes only the CT updates read; a whole step with
+// PotentialA or NoDivFree leaves it out (MagSubsystem_UpdateB still computes it when called alone)
+static int sub_mag_update_b(imex_ctx* c, bool emf = true) {
   c->sub_idx = 0;
   ensure_qe(c);
   LAUNCH(KC_MAG_RHS, 23, launch_mag_rhs(c->L, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, B_BK, B_XK));
@@ -720,8 +727,33 @@ static int sub_mag_update_b(imex_ctx* c) {
   int rc = run_gmres(c, 0, mag_ids(c), p_ids(c, 0, 0), B_XK, B_BK, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
   if (!c->budget) c->stats.mag_solves++;
   LAUNCH(KC_MISC, 6, launch_mag_writeback(c->L, c->cur.u));
-  LAUNCH(KC_EMF, 12, launch_emf_flux(c->L, c->cur.u, c->cur.Qe));
+  if (emf) LAUNCH(KC_EMF, 12, launch_emf_flux(c->L, c->cur.u, c->cur.Qe));
   return rc;
+}
+
+// MagSubsystem_UpdateA (imex_subsystems.cpp:5-74): RHS and initial guess (from `ia0`: the stage's A,
+// which a step starts as a copy of A_s), the potential operator's viscosity from Qs, one GMRES solve
+// on the magnetic slot, then A, bound_cond_A and B = curl A.
+static int sub_mag_update_a(imex_ctx* c, int ia0) {
+  c->sub_idx = 0;
+  ensure_qe(c);
+  LAUNCH(KC_MAG_RHS, 17, launch_a_rhs(c->L, c->cur.u, c->cur.Qe, c->cur.As, ia0, B_BK, B_XK));
+  LAUNCH(KC_MISC, 5, launch_pot_visc(c->L, c->cur.Qs));
+  c->hhat_ready = false;
+  int rc = run_gmres(c, 0, pot_ids(c), p_ids(c, 0, 0), B_XK, B_BK, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
+  if (!c->budget) c->stats.mag_solves++;
+  LAUNCH(KC_A_UPDATE, 12, launch_a_update(c->L, B_XK, c->cur.Ao, c->cur.u));
+  const int u = c->cur.u;
+  fill_planes(c, {{u, 5}, {u, 6}, {u, 7}});
+  return rc;
+}
+
+// Unstaggered_CT_Consistent (constrained_transport.cpp:502-540); A_x, A_y of the stage from `ia01`
+static int sub_unstaggered_ct(imex_ctx* c, int ia01) {
+  LAUNCH(KC_CT, 16, launch_unstaggered_ct(c->L, c->cur.u, c->cur.As, ia01, c->cur.Ao));
+  const int u = c->cur.u;
+  fill_planes(c, {{u, 5}, {u, 6}});
+  return 0;
 }
 
 static int sub_explicit_press(imex_ctx* c) {
@@ -805,7 +837,9 @@ static int run_stage(imex_ctx* c) {
   if (c->hook) c->hook(c, HK_IN);
   rc |= sub_explicit_conv(c);
   if (c->hook) c->hook(c, HK_CONV);
-  rc |= sub_mag_update_b(c);
+  const int mf = c->prm.magfield;                  // main.cpp:424-434
+  if (mf == IMEX_MAGFIELD_POTENTIAL_A) rc |= sub_mag_update_a(c, c->cur.As);
+  else rc |= sub_mag_update_b(c, mf == IMEX_MAGFIELD_STAGGER || mf == IMEX_MAGFIELD_UNSTAGGER);
   if (c->hook) c->hook(c, HK_MAG);
   if (c->prm.formula == IMEX_FORMULA_ENERGY) {     // main.cpp:470-486
     rc |= sub_explicit_energy(c);
@@ -815,7 +849,11 @@ static int run_stage(imex_ctx* c) {
     if (c->hook) c->hook(c, HK_PRESS);
     rc |= sub_pressure(c);
   }
-  rc |= sub_ct(c);
+  if (mf == IMEX_MAGFIELD_STAGGER) rc |= sub_ct(c);   // main.cpp:497-510
+  else {
+    if (mf == IMEX_MAGFIELD_UNSTAGGER) rc |= sub_unstaggered_ct(c, c->cur.As);
+    LAUNCH(KC_MISC, 4, launch_copy_faces(c->L, c->cur.bxqs, c->cur.byqs, c->cur.bxo, c->cur.byo));
+  }
   if (c->hook) c->hook(c, HK_CT);
   return rc;
 }
@@ -824,11 +862,13 @@ static void bind_stage0(imex_ctx* c) {
   c->stage_idx = 0;
   c->cur.u = B_U1; c->cur.Qe = B_UN; c->cur.Qs = B_UN;
   c->cur.bxqs = c->iBxn; c->cur.byqs = c->iByn; c->cur.bxo = B_BX1; c->cur.byo = B_BY1;
+  c->cur.As = c->iAn; c->cur.Ao = B_A1;
 }
 static void bind_stage1(imex_ctx* c) {
   c->stage_idx = 1;
   c->cur.u = c->iUo; c->cur.Qe = B_QE; c->cur.Qs = B_QS;
   c->cur.bxqs = B_BXQS; c->cur.byqs = B_BYQS; c->cur.bxo = c->iBx2; c->cur.byo = c->iBy2;
+  c->cur.As = B_AS; c->cur.Ao = c->iAo;
 }
 
 static int step_start(imex_ctx* c) {
@@ -846,13 +886,14 @@ static int step_body(imex_ctx* c) {
   bind_stage0(c);
   rc |= run_stage(c);
   LAUNCH(KC_COMBINE, 38, launch_stage_combine(c->L, c->iBxn, c->iByn));
+  if (c->needA) LAUNCH(KC_COMBINE, 9, launch_a_combine(c->L, c->iAn, B_A1, B_AS));
   bind_stage1(c);
   rc |= run_stage(c);
   return rc;
 }
 
 static void step_commit(imex_ctx* c) {
-  std::swap(c->iBxn, c->iBx2); std::swap(c->iByn, c->iBy2); std::swap(c->iUs, c->iUo);
+  std::swap(c->iBxn, c->iBx2); std::swap(c->iByn, c->iBy2); std::swap(c->iUs, c->iUo); std::swap(c->iAn, c->iAo);
   c->state_mode = 1;
   c->stats.steps++;
 }
@@ -1203,6 +1244,55 @@ static void hfill(const imex_params& p, double* a, int nvar, int kind) {
     }
 }
 
+// bound_cond_A (boundary.cpp:177-292) on a host Grid (slots 0..2): x over every row, then y over every
+// column; periodic copy, transmissive linear extrapolation from the two edge cells.
+static void hbc_A(const imex_params& p, double* A) {
+  const int NX = p.nx + 2 * GC, NY = p.ny + 2 * GC;
+  auto a = [&](int i, int j, int k) -> double& { return A[(size_t)NVAR * (i + (size_t)j * NX) + k]; };
+  for (int j = 0; j < NY; j++)
+    for (int k = 0; k < 3; k++) {
+      if (p.bc_x == IMEX_BC_PERIODIC) {
+        for (int g = 0; g < GC; g++) { a(g, j, k) = a(p.nx + g, j, k); a(p.nx + GC + g, j, k) = a(GC + g, j, k); }
+      } else {
+        double sL = a(GC + 1, j, k) - a(GC, j, k);
+        for (int g = 0; g < GC; g++) a(GC - 1 - g, j, k) = a(GC, j, k) - (g + 1) * sL;
+        int iR = p.nx + GC - 1;
+        double sR = a(iR, j, k) - a(iR - 1, j, k);
+        for (int g = 0; g < GC; g++) a(iR + 1 + g, j, k) = a(iR, j, k) + (g + 1) * sR;
+      }
+    }
+  for (int i = 0; i < NX; i++)
+    for (int k = 0; k < 3; k++) {
+      if (p.bc_y == IMEX_BC_PERIODIC) {
+        for (int g = 0; g < GC; g++) { a(i, g, k) = a(i, p.ny + g, k); a(i, p.ny + GC + g, k) = a(i, GC + g, k); }
+      } else {
+        double sL = a(i, GC + 1, k) - a(i, GC, k);
+        for (int g = 0; g < GC; g++) a(i, GC - 1 - g, k) = a(i, GC, k) - (g + 1) * sL;
+        int jR = p.ny + GC - 1;
+        double sR = a(i, jR, k) - a(i, jR - 1, k);
+        for (int g = 0; g < GC; g++) a(i, jR + 1 + g, k) = a(i, jR, k) + (g + 1) * sR;
+      }
+    }
+}
+
+// 3 planes of a vector-potential buffer <-> slots 0..2 of a host Grid
+static int up_pot(imex_ctx* c, int i, int b, const double* A) {
+  const Dims& d = c->hI[i].d;
+  std::vector<double> s((size_t)3 * d.PS, 0.0);
+  for (long long q = 0; q < (long long)d.NX * d.NY; q++)
+    for (int k = 0; k < 3; k++) s[(size_t)k * d.PS + q] = A[NVAR * q + k];
+  HIPCHK(hipMemcpy(c->hI[i].buf[b], s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+static int down_pot(imex_ctx* c, int i, int b, double* A) {
+  const Dims& d = c->hI[i].d;
+  std::vector<double> s((size_t)3 * d.PS);
+  HIPCHK(hipMemcpy(s.data(), c->hI[i].buf[b], s.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (long long q = 0; q < (long long)d.NX * d.NY; q++)
+    for (int k = 0; k < NVAR; k++) A[NVAR * q + k] = k < 3 ? s[(size_t)k * d.PS + q] : 0.0;
+  return 0;
+}
+
 // ================================================================ C ABI
 extern "C" {
 
@@ -1307,6 +1397,27 @@ int imex_init_case(const imex_params* p, int case_id, double amp, double u0, dou
   return 0;
 }
 
+// A_initial (constrained_transport.cpp:224-398) for the cases of imex_init_case: A_z whose curl is the
+// case's B, at every cell centre, then bound_cond_A.
+int imex_init_potential(const imex_params* p, int case_id, double amp, double* A) {
+  if (!p || !A) return set_err(IMEX_ERR_ARG, "null argument");
+  const int NX = p->nx + 2 * GC, NY = p->ny + 2 * GC;
+  const double dx = (p->x1 - p->x0) / p->nx, dy = (p->y1 - p->y0) / p->ny;
+  std::memset(A, 0, sizeof(double) * NVAR * NX * NY);
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++) {
+      double x = p->x0 + (i - GC + 0.5) * dx, y = p->y0 + (j - GC + 0.5) * dy;
+      double az;
+      if (case_id == IMEX_CASE_VORTEX) az = (amp / (2.0 * M_PI)) * exp(0.5 * (1.0 - (x * x + y * y)));   // :387-391
+      else if (case_id == IMEX_CASE_ORSZAG_TANG) az = cos(y) + cos(2.0 * x) / 2.0;   // :341-342 on [0,2pi]^2
+      else if (case_id == IMEX_CASE_BLAST) { double b0 = amp / sqrt(4.0 * M_PI) / sqrt(2.0); az = b0 * y - b0 * x; }
+      else return set_err(IMEX_ERR_ARG, "unknown case");
+      A[(size_t)NVAR * (i + (size_t)j * NX) + 2] = az;
+    }
+  hbc_A(*p, A);
+  return 0;
+}
+
 
 static int check_params(const imex_params* prm) {
   if (prm->gmres_restart > MAXM) return set_err(IMEX_ERR_ARG, "gmres_restart exceeds 32");
@@ -1321,6 +1432,8 @@ static int check_params(const imex_params* prm) {
     return set_err(IMEX_ERR_ARG, "bad schedule");
   if (prm->formula != IMEX_FORMULA_PRESSURE && prm->formula != IMEX_FORMULA_ENERGY)
     return set_err(IMEX_ERR_ARG, "bad formula");
+  if (prm->magfield < IMEX_MAGFIELD_STAGGER || prm->magfield > IMEX_MAGFIELD_NODIVFREE)
+    return set_err(IMEX_ERR_ARG, "bad magfield");
   return 0;
 }
 
@@ -1340,6 +1453,8 @@ static int alloc_instance(imex_ctx* c, int i) {
   for (int b : {B_W, B_AV, B_XK, B_BK}) P8(&I.buf[b], n3);
   P8(&I.buf[B_TMPN], 6 * N);
   for (int b = B_P; b <= B_VSLOPE; b++) P8(&I.buf[b], N);
+  for (int b = B_A0; b <= B_AS; b++) I.buf[b] = nullptr;
+  if (c->needA) for (int b = B_A0; b <= B_AS; b++) P8(&I.buf[b], 3 * G1);
   for (int k = 0; k <= m; k++) P8(&I.buf[B_V0 + k], n3);
   for (int k = 0; k < NFBUF; k++) I.fbuf[k] = nullptr;
   if (c->mixed) {
@@ -1388,7 +1503,8 @@ static int one_launch(imex_ctx* c, int i, Launch& L1) {
 
 // ||div B||_2 of every instance
 static int div_b_all(imex_ctx* c, std::vector<double>& out) {
-  launch_divb(c->L, c->iBxn, c->iByn);
+  if (c->prm.magfield == IMEX_MAGFIELD_STAGGER) launch_divb(c->L, c->iBxn, c->iByn);
+  else launch_divb_cell(c->L, c->state_mode ? c->iUs : B_UPRI);     // B slots are the same in pri and con
   HIPCHK(hipStreamSynchronize(c->st));
   out.assign(c->ni, 0.0);
   std::vector<double> h;
@@ -1421,10 +1537,12 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   // slowly growing mode (about x1.1 per step on Orszag-Tang) that FP32 flux rounding would seed
   // at 1e-7, while the pressure formula resets E from p each stage and keeps such errors flat.
   c->mixed_flux = c->mixed && prm->formula == IMEX_FORMULA_PRESSURE;
+  c->needA = prm->magfield == IMEX_MAGFIELD_UNSTAGGER || prm->magfield == IMEX_MAGFIELD_POTENTIAL_A;
   c->cap_tiles_p = op_prec(c, 1) ? 2048 : 1536;
   c->ni = n;
   c->iprm.assign(n, c->prm);
   c->hI.resize(n);
+  c->a_set.assign(n, 0);
   for (int i = 0; i < n; i++) {
     c->iprm[i].nx = nx[i]; c->iprm[i].ny = ny[i];
     std::memset(&c->hI[i], 0, sizeof(Inst));
@@ -1511,6 +1629,30 @@ int imex_batch_set_state(imex_ctx* c, int i, const double* u_pri, const double* 
   return 0;
 }
 
+int imex_batch_set_potential(imex_ctx* c, int i, const double* A) {
+  if (!c || !A) return set_err(IMEX_ERR_ARG, "null argument");
+  if (i < 0 || i >= c->ni) return set_err(IMEX_ERR_ARG, "instance index out of range");
+  if (!c->needA) return set_err(IMEX_ERR_UNSUPPORTED, "this magfield does not evolve the vector potential");
+  HIPCHK(hipStreamSynchronize(c->st));
+  int rc = up_pot(c, i, c->iAn, A);
+  if (rc) return rc;
+  Launch L1;
+  if ((rc = one_launch(c, i, L1))) return rc;
+  launch_bc_a(L1, c->iAn);
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->a_set[i] = 1;
+  return 0;
+}
+int imex_set_potential(imex_ctx* c, const double* A) { return imex_batch_set_potential(c, 0, A); }
+int imex_batch_get_potential(imex_ctx* c, int i, double* A) {
+  if (!c || !A) return set_err(IMEX_ERR_ARG, "null argument");
+  if (i < 0 || i >= c->ni) return set_err(IMEX_ERR_ARG, "instance index out of range");
+  if (!c->needA) return set_err(IMEX_ERR_UNSUPPORTED, "this magfield does not evolve the vector potential");
+  HIPCHK(hipStreamSynchronize(c->st));
+  return down_pot(c, i, c->iAn, A);
+}
+int imex_get_potential(imex_ctx* c, double* A) { return imex_batch_get_potential(c, 0, A); }
+
 int imex_set_state(imex_ctx* c, const double* u_pri, const double* bx, const double* by, double t, int timestep, double prev_dt) {
   return imex_batch_set_state(c, 0, u_pri, bx, by, t, timestep, prev_dt);
 }
@@ -1540,6 +1682,9 @@ int imex_get_state(imex_ctx* c, double* u_con, double* u_pri, double* bx, double
 
 int imex_step_async(imex_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return set_err(IMEX_ERR_ARG, "bad argument");
+  if (c->needA)
+    for (int i = 0; i < c->ni; i++)
+      if (!c->a_set[i]) return set_err(IMEX_ERR_ARG, "UNSTAGGER / POTENTIAL_A need the vector potential (imex_set_potential)");
   int rc = 0;
   for (int s = 0; s < nsteps; s++) {
     int r = one_step(c);
@@ -1624,6 +1769,7 @@ int imex_set_stage(imex_ctx* c, const imex_stage_host* st, const imex_scalars* s
   if (!st || !sc) return set_err(IMEX_ERR_ARG, "null argument");
   c->cur.u = B_U1; c->cur.Qe = B_QE; c->cur.Qs = B_QS;
   c->cur.bxqs = B_BXQS; c->cur.byqs = B_BYQS; c->cur.bxo = B_BX1; c->cur.byo = B_BY1;
+  c->cur.As = B_AS; c->cur.Ao = B_A1;
   int rc = up_grid(c, 0, B_U1, st->u_con);
   if (!rc) rc = up_grid(c, 0, B_QE, st->Qe);
   if (!rc) rc = up_grid(c, 0, B_QS, st->Qs);
@@ -1692,6 +1838,27 @@ int imex_explicit_press(imex_ctx* c) { SINGLE(c); return finish(c, sub_explicit_
 int imex_pre_subsystem_pressure(imex_ctx* c) { SINGLE(c); return finish(c, sub_pressure(c)); }
 int imex_ct_update(imex_ctx* c) { SINGLE(c); return finish(c, sub_ct(c)); }
 int imex_explicit_energy(imex_ctx* c) { SINGLE(c); return finish(c, sub_explicit_energy(c)); }
+
+#define NEEDA(c) do { if (!(c)->needA) return set_err(IMEX_ERR_UNSUPPORTED, "this magfield does not evolve the vector potential"); } while (0)
+int imex_set_stage_potential(imex_ctx* c, const double* A, const double* As) {
+  SINGLE(c); NEEDA(c);
+  if (!A || !As) return set_err(IMEX_ERR_ARG, "null argument");
+  c->cur.As = B_AS; c->cur.Ao = B_A1;
+  int rc = up_pot(c, 0, B_A1, A);
+  if (!rc) rc = up_pot(c, 0, B_AS, As);
+  return rc;
+}
+int imex_get_stage_potential(imex_ctx* c, double* A, double* As) {
+  SINGLE(c); NEEDA(c);
+  HIPCHK(hipStreamSynchronize(c->st));
+  int rc = 0;
+  if (A) rc = down_pot(c, 0, c->cur.Ao, A);
+  if (!rc && As) rc = down_pot(c, 0, c->cur.As, As);
+  return rc;
+}
+// stage level: the initial guess and A_x, A_y come from the stage's A as given
+int imex_mag_update_a(imex_ctx* c) { SINGLE(c); NEEDA(c); return finish(c, sub_mag_update_a(c, c->cur.Ao)); }
+int imex_unstaggered_ct(imex_ctx* c) { SINGLE(c); NEEDA(c); return finish(c, sub_unstaggered_ct(c, c->cur.Ao)); }
 int imex_pre_subsystem_energy(imex_ctx* c) { SINGLE(c); return finish(c, sub_energy(c)); }
 
 // ---- operator level ----
@@ -1717,16 +1884,18 @@ int imex_apply_op(imex_ctx* c, int op, const double* x, double* Ax, const double
   SINGLE(c);
   if (!x || !Ax) return set_err(IMEX_ERR_ARG, "null argument");
   const Dims& d = c->d0();
-  const int nc = op == IMEX_OP_MAG ? 3 : 1;
+  const bool three = op == IMEX_OP_MAG || op == IMEX_OP_POTENTIAL;
+  const int nc = three ? 3 : 1;
   const size_t n = (size_t)nc * d.N;
   int rc = 0;
-  if (op < IMEX_OP_MAG || op > IMEX_OP_ENERGY) return set_err(IMEX_ERR_ARG, "bad operator");
+  if (op < IMEX_OP_MAG || op > IMEX_OP_POTENTIAL) return set_err(IMEX_ERR_ARG, "bad operator");
   if (op == IMEX_OP_ENERGY) prep_energy(c);
+  else if (op == IMEX_OP_POTENTIAL) launch_pot_visc(c->L, c->cur.Qs);
   else if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_XK], x, n * sizeof(double), hipMemcpyHostToDevice));
   const PIds po = op == IMEX_OP_ENERGY ? e_ids(c, 0) : p_ids(c, op == IMEX_OP_P_JACOBIAN, 0);
-  op_apply(c->L, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), po, c->P, B_XK, B_AV);
+  op_apply(c->L, three ? 0 : 1, op == IMEX_OP_POTENTIAL ? pot_ids(c) : mag_ids(c), po, c->P, B_XK, B_AV);
   if ((rc = finish(c, 0))) return rc;
   HIPCHK(hipMemcpy(Ax, c->hI[0].buf[B_AV], n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
@@ -1738,18 +1907,20 @@ int imex_gmres(imex_ctx* c, int op, const double* b, double* x, int precond, con
   SINGLE(c);
   if (!b || !x) return set_err(IMEX_ERR_ARG, "null argument");
   const Dims& d = c->d0();
-  const int nc = op == IMEX_OP_MAG ? 3 : 1;
+  const bool three = op == IMEX_OP_MAG || op == IMEX_OP_POTENTIAL;
+  const int nc = three ? 3 : 1;
   const size_t n = (size_t)nc * d.N;
   int rc = 0;
-  if (op < IMEX_OP_MAG || op > IMEX_OP_ENERGY) return set_err(IMEX_ERR_ARG, "bad operator");
+  if (op < IMEX_OP_MAG || op > IMEX_OP_POTENTIAL) return set_err(IMEX_ERR_ARG, "bad operator");
   if (op == IMEX_OP_ENERGY) prep_energy(c);
+  else if (op == IMEX_OP_POTENTIAL) launch_pot_visc(c->L, c->cur.Qs);
   else if (op != IMEX_OP_MAG && (rc = prep_pressure_operator(c, op, p_freeze, mbig))) return rc;
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_BK], b, n * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_XK], x, n * sizeof(double), hipMemcpyHostToDevice));
-  const int pc = precond && op != IMEX_OP_MAG;
+  const int pc = precond && !three;
   const PIds po = op == IMEX_OP_ENERGY ? e_ids(c, pc) : p_ids(c, op == IMEX_OP_P_JACOBIAN, pc);
-  rc = run_gmres(c, op == IMEX_OP_MAG ? 0 : 1, mag_ids(c), po, B_XK, B_BK, 0, info, nullptr, nullptr);
+  rc = run_gmres(c, three ? 0 : 1, op == IMEX_OP_POTENTIAL ? pot_ids(c) : mag_ids(c), po, B_XK, B_BK, 0, info, nullptr, nullptr);
   int r2 = finish(c, rc);
   if (r2 < 0) return r2;
   HIPCHK(hipMemcpy(x, c->hI[0].buf[B_XK], n * sizeof(double), hipMemcpyDeviceToHost));

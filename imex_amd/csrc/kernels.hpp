@@ -43,12 +43,23 @@ struct POp {
   const double* S;
   int jacobian, precond;
 };
-// What the host passes: buffer ids and flags.
-struct MagIds { int Qe, u; };
+// Magnetic_Subsystem_potential_AX's coefficients: B of Qe, rho of u, and the viscosity maxima in S.
+struct PotOp {
+  Dims d;
+  const double* Qe;   // padded planes (B in 5..7)
+  const double* u;    // padded planes (rho in 0)
+  const double* S;
+};
+// What the host passes: buffer ids and flags (pot: the vector-potential operator on the magnetic
+// slots, 3 unknowns per cell).
+struct MagIds { int Qe, u, pot; };
 struct PIds { int u, jacobian, precond, energy; };   // energy: Energy_Subsystem_AX on the same slots
 
 __device__ __forceinline__ MagOp make_mag_op(const Inst& I, const MagIds& m) {
   MagOp o; o.d = I.d; o.Qe = gmem(I.buf[m.Qe]); o.u = gmem(I.buf[m.u]); o.S = gmem(I.S); return o;
+}
+__device__ __forceinline__ PotOp make_pot_op(const Inst& I, const MagIds& m) {
+  PotOp o; o.d = I.d; o.Qe = gmem(I.buf[m.Qe]); o.u = gmem(I.buf[m.u]); o.S = gmem(I.S); return o;
 }
 __device__ __forceinline__ POp make_p_op(const Inst& I, const Phys& P, const PIds& q) {
   POp o; o.d = I.d; o.P = P; o.hhat = gmem(I.buf[B_HHAT]); o.eta = gmem(I.buf[B_ETA]); o.u = gmem(I.buf[q.u]);
@@ -89,6 +100,19 @@ void launch_p_energy(const Launch& L, const Phys& P, int iu);
 // energy formula: RHS, coefficient plane, initial guess and precE divisor; then E and momentum update
 void launch_e_rhs(const Launch& L, const Phys& P, int iu, int iQe);
 void launch_e_update(const Launch& L, const Phys& P, int iu);
+// EMF-consistent corner assembly into B_EZPP from the magnetic subsystem's face fluxes (B_FX, B_FY)
+void launch_ct_corners(const Launch& L, int iu);
+
+// ---- vector potential (kernels_potential.hip) ----
+void launch_bc_a(const Launch& L, int ia);                       // bound_cond_A on 3 planes
+void launch_a_combine(const Launch& L, int ian, int ia1, int ias);
+// MagSubsystem_UpdateA RHS b (compact) and initial guess from A `ia0`
+void launch_a_rhs(const Launch& L, int iu, int iQe, int ias, int ia0, int ib, int ix0);
+void launch_pot_visc(const Launch& L, int iQs);                  // S_GBX, S_GBY from Qs
+void launch_a_update(const Launch& L, int ix, int ia, int iu);    // A <- x, bound_cond_A, B = curl A
+void launch_unstaggered_ct(const Launch& L, int iu, int ias, int ia01, int ia);
+void launch_copy_faces(const Launch& L, int ibx, int iby, int obx, int oby);
+void launch_divb_cell(const Launch& L, int iu);
 void launch_ct(const Launch& L, const Phys& P, int plain, int iu, int ibxqs, int ibyqs, int ibx, int iby);
 void launch_con2pri(const Launch& L, const Phys& P, int icon, int ipri);
 void launch_pri2con(const Launch& L, const Phys& P, int ipri, int icon);
@@ -96,8 +120,8 @@ void launch_divb(const Launch& L, int ibx, int iby);
 void launch_visc_rk(const Launch& L, const Phys& P, int iu);
 
 // ---- Krylov (kernels_krylov.hip) ----
-// op = 0 magnetic (3 unknowns per cell), 1 pressure or, with po.energy, energy (1 unknown per
-// cell); prec = 0 FP64 basis and
+// op = 0 magnetic or, with mo.pot, vector potential (3 unknowns per cell), 1 pressure or, with
+// po.energy, energy (1 unknown per cell); prec = 0 FP64 basis and
 // operator, 1 FP32 basis and operator (inner solves of the mixed form; dots, norms, Givens and
 // the residual of the refinement stay FP64)
 void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int ix, int ib,

@@ -1,7 +1,8 @@
 // kernels_krylov.hip — matrix-free implicit operators and the device-resident GMRES / Newton pieces.
 //
-// Operators (Magnetic_Subsystem_AX, Pressure_Subsystem_AX, Energy_Subsystem_AX;
-// src/implicit_operators.cpp:141-353, 505-616, 741-790, GLF / MaterialSpeedp branches) are
+// Operators (Magnetic_Subsystem_AX, Pressure_Subsystem_AX, Energy_Subsystem_AX,
+// Magnetic_Subsystem_potential_AX; src/implicit_operators.cpp:141-353, 505-616, 741-790, 355-503,
+// GLF / MaterialSpeedp branches) are
 // evaluated on 64x4 cell tiles whose input vector, with a one-cell halo mapped through M_P / M_B,
 // is staged in LDS.  In Arnoldi mode the staged values are the unnormalised vector divided by its
 // norm, so the normalisation v = w / ||w|| (implicit_operators.cpp:876) happens once per element
@@ -323,6 +324,118 @@ template <class T> struct OpSel<2, T> : OpSel<1, T> {
   __device__ static void precond(const POp& op, int cidx, int, int, T* y) { e_precond(op, cidx, y); }
 };
 
+// ---------------------------------------------------------------- vector-potential operator
+// Magnetic_Subsystem_potential_AX (implicit_operators.cpp:355-503), GLF branch with add_bvisc:
+// A - h B^n x (h / rho) div T(b) - h nu Laplacian(A), with b = curl A at the four face neighbours and
+// T(alpha, beta) = (B^n . b)/2 delta - B^n_beta b_alpha.  The unknown's ghost values come from
+// bound_cond_A applied to the vector (periodic copy or linear extrapolation), evaluated here in
+// the loader on a two-cell frame; in Arnoldi mode the extrapolation acts on the scaled values,
+// as the reference's acts on the stored basis vector.
+template <class T> struct PotSmem { T x[3][TY + 4][TX + 4]; };
+
+template <class VT>
+__device__ __forceinline__ double pot_ghost(const Dims& d, const VT* __restrict__ x, int f, double h, bool scale,
+                                            int cx, int cy) {
+  auto V = [&](int a, int b) {
+    double v = (double)x[(long long)f * d.N + a + (long long)d.nx * b];
+    return scale ? v / h : v;
+  };
+  auto X = [&](int a, int b) -> double {      // the x rule on row b (interior)
+    if (a >= 0 && a < d.nx) return V(a, b);
+    if (d.perx) return V(a < 0 ? a + d.nx : a - d.nx, b);
+    if (a < 0) { double v0 = V(0, b), sL = V(1, b) - v0; return v0 - (-a) * sL; }
+    double vr = V(d.nx - 1, b), sR = vr - V(d.nx - 2, b);
+    return vr + (a - (d.nx - 1)) * sR;
+  };
+  if (cy >= 0 && cy < d.ny) return X(cx, cy);
+  if (d.pery) return X(cx, cy < 0 ? cy + d.ny : cy - d.ny);
+  if (cy < 0) { double x0 = X(cx, 0), sL = X(cx, 1) - x0; return x0 - (-cy) * sL; }
+  double xr = X(cx, d.ny - 1), sR = xr - X(cx, d.ny - 2);
+  return xr + (cy - (d.ny - 1)) * sR;
+}
+
+template <class T, class VT>
+__device__ __forceinline__ void pot_load(const PotOp& op, PotSmem<T>& sm, const VT* __restrict__ x, double h, bool scale,
+                                         int gx0, int gy0) {
+  const Dims& d = op.d;
+  constexpr int W = TX + 4, H = TY + 4;
+  for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
+    int lx = e % W, ly = e / W;
+    int cx = gx0 + lx - 2, cy = gy0 + ly - 2;
+    bool used = cx <= d.nx + 1 && cy <= d.ny + 1;        // frames of a partial tile beyond it are never read
+#pragma unroll
+    for (int f = 0; f < 3; f++) sm.x[f][ly][lx] = used ? (T)pot_ghost(d, x, f, h, scale, cx, cy) : T(0);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void pot_compute(const PotOp& op, const PotSmem<T>& sm, int tx, int ty, int i, int j, bool valid, T* out) {
+  if (!valid) return;
+  const Dims& d = op.d;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
+  const int a0 = tx + 2, b0 = ty + 2;
+  auto a = [&](int di, int dj, int k) { return sm.x[k][b0 + dj][a0 + di]; };
+  auto Q = [&](int di, int dj, int k) { return (T)op.Qe[(long long)k * d.PS + pidx(d, i + di, j + dj)]; };
+  const T two_dx = T(2.0) * dx, two_dy = T(2.0) * dy;
+  T bx_P0 = (a(1, 1, 2) - a(1, -1, 2)) / two_dy;
+  T by_P0 = -(a(2, 0, 2) - a(0, 0, 2)) / two_dx;
+  T bz_P0 = (a(2, 0, 1) - a(0, 0, 1)) / two_dx - (a(1, 1, 0) - a(1, -1, 0)) / two_dy;
+  T bx_M0 = (a(-1, 1, 2) - a(-1, -1, 2)) / two_dy;
+  T by_M0 = -(a(0, 0, 2) - a(-2, 0, 2)) / two_dx;
+  T bz_M0 = (a(0, 0, 1) - a(-2, 0, 1)) / two_dx - (a(-1, 1, 0) - a(-1, -1, 0)) / two_dy;
+  T bx_0P = (a(0, 2, 2) - a(0, 0, 2)) / two_dy;
+  T by_0P = -(a(1, 1, 2) - a(-1, 1, 2)) / two_dx;
+  T bz_0P = (a(1, 1, 1) - a(-1, 1, 1)) / two_dx - (a(0, 2, 0) - a(0, 0, 0)) / two_dy;
+  T bx_0M = (a(0, 0, 2) - a(0, -2, 2)) / two_dy;
+  T by_0M = -(a(1, -1, 2) - a(-1, -1, 2)) / two_dx;
+  T bz_0M = (a(1, -1, 1) - a(-1, -1, 1)) / two_dx - (a(0, 0, 0) - a(0, -2, 0)) / two_dy;
+  T qx, qy, qz;
+  qx = Q(1, 0, 5); qy = Q(1, 0, 6); qz = Q(1, 0, 7);
+  T bd = qx * bx_P0 + qy * by_P0 + qz * bz_P0;
+  T Txx_P0 = bd / T(2.0) - qx * bx_P0, Tyx_P0 = -qx * by_P0, Tzx_P0 = -qx * bz_P0;
+  qx = Q(-1, 0, 5); qy = Q(-1, 0, 6); qz = Q(-1, 0, 7);
+  bd = qx * bx_M0 + qy * by_M0 + qz * bz_M0;
+  T Txx_M0 = bd / T(2.0) - qx * bx_M0, Tyx_M0 = -qx * by_M0, Tzx_M0 = -qx * bz_M0;
+  qx = Q(0, 1, 5); qy = Q(0, 1, 6); qz = Q(0, 1, 7);
+  bd = qx * bx_0P + qy * by_0P + qz * bz_0P;
+  T Txy_0P = -qy * bx_0P, Tyy_0P = bd / T(2.0) - qy * by_0P, Tzy_0P = -qy * bz_0P;
+  qx = Q(0, -1, 5); qy = Q(0, -1, 6); qz = Q(0, -1, 7);
+  bd = qx * bx_0M + qy * by_0M + qz * bz_0M;
+  T Txy_0M = -qy * bx_0M, Tyy_0M = bd / T(2.0) - qy * by_0M, Tzy_0M = -qy * bz_0M;
+  T divT_x = (Txx_P0 - Txx_M0) / two_dx + (Txy_0P - Txy_0M) / two_dy;
+  T divT_y = (Tyx_P0 - Tyx_M0) / two_dx + (Tyy_0P - Tyy_0M) / two_dy;
+  T divT_z = (Tzx_P0 - Tzx_M0) / two_dx + (Tzy_0P - Tzy_0M) / two_dy;
+  const T hr = h / (T)op.u[pidx(d, i, j)];
+  T Fx = hr * divT_x, Fy = hr * divT_y, Fz = hr * divT_z;
+  qx = Q(0, 0, 5); qy = Q(0, 0, 6); qz = Q(0, 0, 7);
+  T cr[3] = {qy * Fz - qz * Fy, qz * Fx - qx * Fz, qx * Fy - qy * Fx};
+  const T nu_x = (T)op.S[S_GBX] * dx, nu_y = (T)op.S[S_GBY] * dy;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T lap = nu_x / (dx * dx) * (a(-1, 0, k) - T(2.0) * a(0, 0, k) + a(1, 0, k))
+          + nu_y / (dy * dy) * (a(0, -1, k) - T(2.0) * a(0, 0, k) + a(0, 1, k));
+    out[k] = a(0, 0, k) - h * cr[k] - h * lap;
+  }
+}
+
+template <class T> struct OpSel<3, T> {
+  static constexpr int NC = 3;
+  typedef PotOp Op;
+  typedef MagIds Ids;
+  __device__ static Op make(const Inst& I, const Phys&, const Ids& ids) { return make_pot_op(I, ids); }
+  typedef PotSmem<T> Smem;
+  template <class VT>
+  __device__ static void load(const PotOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
+    pot_load(op, sm, x, h, scale, gx0, gy0);
+  }
+  __device__ static void compute(const PotOp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
+    pot_compute(op, sm, tx, ty, i, j, valid, y);
+  }
+  __device__ static T center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 2][tx + 2]; }
+  __device__ static void precond(const PotOp&, int, int, int, T*) {}
+  __device__ static const Dims& dims(const PotOp& op) { return op.d; }
+};
+
 // ---------------------------------------------------------------- reductions
 // Sum `nv` per-thread values over the block; results in thread v < nv, written to out[v].
 template <int MAXV>
@@ -377,7 +490,7 @@ __device__ __forceinline__ void final_sums(const double* partial, int stride, in
 // Always in FP64 (it is the outer residual of the mixed form's iterative refinement); r0 is
 // stored in the basis type VT, its norm is taken in FP64 before rounding.
 template <int OPK, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 ? IMEX_MAG_WAVES : 4)))
 k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int ib,
                  int outer, int x_is_zero, int skip, int mixed) {
   BATCH_BLOCK(bm);
@@ -476,7 +589,7 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const Inst* __restri
 // The dots of each tile are reduced over the wave right away (a uniform loop over i, so no
 // accumulator array is live across the operator) and summed per wave in LDS, in tile order.
 template <int OPK, class T, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : (sizeof(T) == 4 ? 8 : 4))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 ? IMEX_MAG_WAVES : (sizeof(T) == 4 ? 8 : 4))))
 k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -654,7 +767,7 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ 
 
 // ---------------------------------------------------------------- plain operator application (tests, adjoint)
 template <int OPK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 ? IMEX_MAG_WAVES : 4)))
 k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int iy) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -868,11 +981,13 @@ void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const P
   const int sh = tiles_shape(op);
   const BMap& bm = L.bm[sh];
   if (prec == 0) {
-    if (op == 0) k_gmres_residual<0, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    if (op == 0 && mo.pot) k_gmres_residual<3, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    else if (op == 0) k_gmres_residual<0, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
     else if (po.energy) k_gmres_residual<2, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
     else k_gmres_residual<1, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
   } else {
-    if (op == 0) k_gmres_residual<0, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    if (op == 0 && mo.pot) k_gmres_residual<3, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    else if (op == 0) k_gmres_residual<0, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
     else if (po.energy) k_gmres_residual<2, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
     else k_gmres_residual<1, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
   }
@@ -885,11 +1000,13 @@ void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PI
   const int sh = tiles_shape(op);
   const BMap& bm = L.bm[sh];
   if (prec == 0) {
-    if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    if (op == 0 && mo.pot) k_arnoldi_apply<3, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
     else if (po.energy) k_arnoldi_apply<2, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
     else k_arnoldi_apply<1, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   } else {
-    if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    if (op == 0 && mo.pot) k_arnoldi_apply<3, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
     else if (po.energy) k_arnoldi_apply<2, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
     else k_arnoldi_apply<1, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   }
@@ -926,7 +1043,8 @@ void gmres_xupdate(const Launch& L, int op, int prec, int ix) {
 }
 
 void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy) {
-  if (op == 0) k_op_apply<0><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
+  if (op == 0 && mo.pot) k_op_apply<3><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
+  else if (op == 0) k_op_apply<0><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
   else if (po.energy) k_op_apply<2><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
   else k_op_apply<1><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
 }

@@ -1030,6 +1030,9 @@ void launch_ct(const Launch& L, const Phys& P, int plain, int iu, int ibxqs, int
   k_ct_corner<<<GRID(L, SH_INT2_64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT2_64x4], iex, iey, -1.0, 1.0, iu);
   k_ct_faces<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x4], ibxqs, ibyqs, ibx, iby, iu);
 }
+void launch_ct_corners(const Launch& L, int iu) {
+  k_ct_corner<<<GRID(L, SH_INT2_64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT2_64x4], B_FX, B_FY, -1.0, 1.0, iu);
+}
 void launch_con2pri(const Launch& L, const Phys& P, int icon, int ipri) {
   k_con2pri<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_TOT], icon, ipri);
 }

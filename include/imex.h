@@ -35,7 +35,9 @@ enum { IMEX_EOS_IDEAL = 0, IMEX_EOS_RK = 1 };                       /* EOSType (
 enum { IMEX_CT_CONSISTENT = 0, IMEX_CT_PLAIN = 1 };                 /* Staggered_CT_Consistent / Staggered_CT */
 enum { IMEX_PREC_FP64 = 0, IMEX_PREC_MIXED = 1 };
 enum { IMEX_FORMULA_ENERGY = 0, IMEX_FORMULA_PRESSURE = 1 };       /* Formula (config.hpp:27) */
-enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2, IMEX_OP_ENERGY = 3 };
+enum { IMEX_MAGFIELD_STAGGER = 0, IMEX_MAGFIELD_UNSTAGGER = 1, IMEX_MAGFIELD_POTENTIAL_A = 2,
+       IMEX_MAGFIELD_NODIVFREE = 3 };                              /* MagField (config.hpp:28) */
+enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2, IMEX_OP_ENERGY = 3, IMEX_OP_POTENTIAL = 4 };
 enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2 };
 /* How a step is scheduled.  All three give identical results.
  *   GRAPH:  every solve gets an iteration budget learned from earlier steps; the whole step is
@@ -91,6 +93,10 @@ typedef struct imex_params {
                                      which implicit subsystem a step runs after the magnetic subsystem
                                      (main.cpp:470-486); in the mixed form the energy solve follows
                                      mixed_pressure */
+  int magfield;                   /* IMEX_MAGFIELD_* (config.hpp:45; default STAGGER): the magnetic subsystem and
+                                     divergence treatment of a stage (main.cpp:424-434, 497-510).  UNSTAGGER and
+                                     POTENTIAL_A evolve the vector potential A, which the caller sets with
+                                     imex_set_potential before stepping; with STAGGER and NODIVFREE A is not used */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */
@@ -175,7 +181,8 @@ int imex_step_async(imex_ctx* ctx, int nsteps);
 int imex_sync(imex_ctx* ctx);
 int imex_get_stats(imex_ctx* ctx, imex_stats* stats);
 int imex_reset_stats(imex_ctx* ctx);
-double imex_div_b(imex_ctx* ctx);   /* cal_divergence_error, StaggerCT branch (imex_subsystems.cpp:630-660); max over a batch */
+double imex_div_b(imex_ctx* ctx);   /* cal_divergence_error (imex_subsystems.cpp:630-660): the face divergence with
+                                      STAGGER, the cell-centred one (one-sided at the domain edges) otherwise; max over a batch */
 
 /* ---- batches: independent instances of uneven grid size stepped together ----
  * The reference runs one grid per process (src/main.cpp:231-250, one Grid built from
@@ -212,6 +219,19 @@ typedef struct imex_tune_info {
 } imex_tune_info;
 int imex_autotune(imex_ctx* ctx, int flags, imex_tune_info* out);
 
+/* ---- vector potential (UNSTAGGER and POTENTIAL_A) ----
+ * A is a Grid in the reference layout with A_x, A_y, A_z in slots 0..2 (slots 3..7 are ignored on input
+ * and zero on output), as the reference's `Grid A` (main.cpp:230). */
+/* A_initial (constrained_transport.cpp:224-398) for the built-in cases of imex_init_case (vortex
+ * A_z = amp/(2 pi) exp((1 - r^2)/2), Orszag-Tang A_z = cos y + cos(2x)/2, blast A_z = b0 (y - x)), at cell
+ * centres, then bound_cond_A (boundary.cpp:177-292). */
+int imex_init_potential(const imex_params* p, int case_id, double amp, double* A);
+/* The step-start vector potential (main.cpp:244, 347-348); ghosts are recomputed by bound_cond_A. */
+int imex_set_potential(imex_ctx* ctx, const double* A);
+int imex_get_potential(imex_ctx* ctx, double* A);
+int imex_batch_set_potential(imex_ctx* ctx, int i, const double* A);
+int imex_batch_get_potential(imex_ctx* ctx, int i, double* A);
+
 /* ---- stage-level drop-ins, one per reference entry point ---- */
 int imex_set_stage(imex_ctx* ctx, const imex_stage_host* st, const imex_scalars* sc);
 int imex_get_stage(imex_ctx* ctx, imex_stage_host* st, imex_scalars* sc);
@@ -234,11 +254,22 @@ int imex_explicit_energy(imex_ctx* ctx);
 int imex_pre_subsystem_energy(imex_ctx* ctx);
 /* Staggered_CT_Consistent (ct_flux.cpp:150-191) or Staggered_CT (constrained_transport.cpp:66-143). */
 int imex_ct_update(imex_ctx* ctx);
+/* The stage's A and A_s (main.cpp:371-412: A_s is the implicit stage combination of A, and A starts the
+ * stage as a copy of it), for the two stage-level calls below.  Grids as imex_set_potential. */
+int imex_set_stage_potential(imex_ctx* ctx, const double* A, const double* A_s);
+int imex_get_stage_potential(imex_ctx* ctx, double* A, double* A_s);
+/* MagSubsystem_UpdateA (imex_subsystems.cpp:5-74): A from the potential system (GMRES on
+ * Magnetic_Subsystem_potential_AX), then the cell B = curl A. */
+int imex_mag_update_a(imex_ctx* ctx);
+/* Unstaggered_CT_Consistent (constrained_transport.cpp:502-540): A_z from the consistent corner EMFs
+ * of the magnetic subsystem's face fluxes, then B_x, B_y = curl A_z. */
+int imex_unstaggered_ct(imex_ctx* ctx);
 
 /* ---- operator level (tests, adjoint) on the current stage working set ---- */
-/* Magnetic_Subsystem_AX / Pressure_Subsystem_AX / Energy_Subsystem_AX (implicit_operators.cpp:141-353,
- * 505-616, 741-790).  For IMEX_OP_P_JACOBIAN, p_freeze (N) and mbig select V_slope.  The energy
- * operator's coefficients come from the stage's Qe and u_con.  Host arrays. */
+/* Magnetic_Subsystem_AX / Pressure_Subsystem_AX / Energy_Subsystem_AX / Magnetic_Subsystem_potential_AX
+ * (implicit_operators.cpp:141-353, 505-616, 741-790, 355-503).  For IMEX_OP_P_JACOBIAN, p_freeze (N) and
+ * mbig select V_slope.  The energy operator's coefficients come from the stage's Qe and u_con, the
+ * potential operator's from Qe, u_con and (its viscosity) Qs.  Host arrays. */
 int imex_apply_op(imex_ctx* ctx, int op, const double* x, double* Ax,
                   const double* p_freeze, double mbig);
 /* Transposed operators for the discrete adjoint: ATx = A^T x for the magnetic and pressure
@@ -255,7 +286,7 @@ int imex_pressure_newton(imex_ctx* ctx, const double* b, double* p, imex_solve_i
 
 /* ---- discrete adjoint of the step (no counterpart in the reference, which has no derivative) ----
  * Supported: single-instance contexts, periodic grids, ideal gas, Minmod or VanLeer, EMF-consistent
- * CT, pressure formula, FP64 form (other settings return IMEX_ERR_UNSUPPORTED).  Usage: imex_set_state (the initial
+ * CT (magfield STAGGER), pressure formula, FP64 form (other settings return IMEX_ERR_UNSUPPORTED).  Usage: imex_set_state (the initial
  * state x0), imex_adjoint_forward (runs nsteps, keeping a checkpoint every `every` steps; the
  * context then holds the final state), read the final state and form the cotangent of the
  * objective J with respect to it (gu_con: host Grid of conservative variables; gbx, gby: face

@@ -48,6 +48,13 @@ def lib():
         "orc_pre_subsystem_energy": (C.c_int, [PP, SP, STP, C.POINTER(Stats)]),
         "orc_flux": (C.c_int, [PP, C.c_int, C.c_int, dp, dp, C.c_double, dp]),
         "orc_energy_diag": (C.c_int, [PP, SP, STP, dp]),
+        "orc_set_potential": (C.c_int, [vp, dp]),
+        "orc_get_potential": (C.c_int, [vp, dp]),
+        "orc_capture_potential": (C.c_int, [vp, dp, dp]),
+        "orc_mag_update_a": (C.c_int, [PP, SP, STP, dp, dp, C.POINTER(Stats)]),
+        "orc_unstaggered_ct": (C.c_int, [PP, SP, STP, dp, dp]),
+        "orc_bc_potential": (C.c_int, [PP, dp]),
+        "orc_div_b_cell": (C.c_double, [PP, dp]),
         "orc_reconstruct": (C.c_int, [PP, dp, dp, dp, dp, dp]),
         "orc_apply_op": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, dp, C.c_double]),
         "orc_gmres": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, C.c_int, dp, C.c_double, C.POINTER(SolveInfo)]),
@@ -107,6 +114,22 @@ class Oracle:
         rc = self.L.orc_step_capture(self.h, point, C.byref(stage_struct(st)), C.byref(sc))
         return st, struct_dict(sc), rc
 
+    def set_potential(self, A):
+        return self.L.orc_set_potential(self.h, ptr(np.ascontiguousarray(A, dtype=np.float64)))
+
+    def get_potential(self):
+        A = np.zeros(grid_shape(self.p))
+        self.L.orc_get_potential(self.h, ptr(A))
+        return A
+
+    def capture_potential(self):
+        """A and A_s of the last step_capture."""
+        A, As = np.zeros(grid_shape(self.p)), np.zeros(grid_shape(self.p))
+        rc = self.L.orc_capture_potential(self.h, ptr(A), ptr(As))
+        if rc != 0:
+            raise RuntimeError("no vector potential in the last capture")
+        return A, As
+
     def get_state(self):
         p = self.p
         u_con, u_pri = np.zeros(grid_shape(p)), np.zeros(grid_shape(p))
@@ -138,8 +161,40 @@ def sub(p, sc, st, name):
     return st, rc, None
 
 
+def sub_a(p, sc, st, A, As, name):
+    """mag_update_a / unstaggered_ct on copies of a stage dict and A; returns (stage, A, rc, stats)."""
+    st = _copy(st)
+    A = np.array(A, dtype=np.float64, copy=True)
+    As = np.ascontiguousarray(As, dtype=np.float64)
+    s = stage_struct(st)
+    L = lib()
+    if name == "mag_update_a":
+        stats = Stats()
+        rc = L.orc_mag_update_a(C.byref(p), C.byref(_sc(sc)), C.byref(s), ptr(A), ptr(As), C.byref(stats))
+        return st, A, rc, struct_dict(stats)
+    rc = L.orc_unstaggered_ct(C.byref(p), C.byref(_sc(sc)), C.byref(s), ptr(A), ptr(As))
+    return st, A, rc, None
+
+
+def bc_potential(p, A):
+    A = np.array(A, dtype=np.float64, copy=True)
+    lib().orc_bc_potential(C.byref(p), ptr(A))
+    return A
+
+
+def div_b_cell(p, u_con):
+    return lib().orc_div_b_cell(C.byref(p), ptr(np.ascontiguousarray(u_con, dtype=np.float64)))
+
+
+def _dim(p, op):
+    from imex_amd.capi import OP_MAG, OP_POTENTIAL
+    return p.nx * p.ny * (3 if op in (OP_MAG, OP_POTENTIAL) else 1)
+
+
 def apply_op(p, sc, st, op, x, p_freeze=None, mbig=0.0):
     x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.size != _dim(p, op):
+        raise ValueError(f"operator {op} takes {_dim(p, op)} unknowns, got {x.size}")
     y = np.zeros_like(x)
     pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
     lib().orc_apply_op(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), op, ptr(x), ptr(y), ptr(pf), mbig)
@@ -149,6 +204,8 @@ def apply_op(p, sc, st, op, x, p_freeze=None, mbig=0.0):
 def gmres(p, sc, st, op, b, x0, precond=0, p_freeze=None, mbig=0.0):
     x = np.array(x0, dtype=np.float64, copy=True)
     b = np.ascontiguousarray(b, dtype=np.float64)
+    if x.size != _dim(p, op) or b.size != _dim(p, op):
+        raise ValueError(f"operator {op} takes {_dim(p, op)} unknowns")
     pf = None if p_freeze is None else np.ascontiguousarray(p_freeze, dtype=np.float64)
     info = SolveInfo()
     rc = lib().orc_gmres(C.byref(p), C.byref(_sc(sc)), C.byref(stage_struct(st)), op, ptr(b), ptr(x), precond, ptr(pf),

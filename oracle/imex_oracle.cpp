@@ -9,8 +9,9 @@
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * this library, and only as the checker.  The product path never calls it.
  *
- * Scope: GLF = true, add_pvisc = MaterialSpeedp, PressureFormula (DefinitionofU,
- * all_correction) and EnergyFormula, StaggerCT, ConsState, IMEXdt, RKs = 2,
+ * Scope: GLF = true, add_pvisc = MaterialSpeedp, add_bvisc = true, PressureFormula
+ * (DefinitionofU, all_correction) and EnergyFormula, StaggerCT, UnStaggerCT, PotentialA
+ * and NoDivFree, ConsState, IMEXdt, RKs = 2,
  * periodic / transmissive boundaries (include/config.hpp:43-65).
  *
  * Parity status of each part (how it is pinned) is listed in the header of
@@ -432,6 +433,7 @@ namespace {
 struct Stage {
   Vec u_con, Qe, Qs;
   Vec bx, by, bx_qs, by_qs, fx, fy;
+  Vec A, As;                 // vector potential of the stage and its implicit combination (Grid, slots 0..2)
 };
 
 // Rusanov-type face flux with GLF dissipation s_max = a_max (flux.cpp:202-243, 329-367).
@@ -1072,6 +1074,173 @@ void ct_plain(Ctx& C, const imex_scalars& S, Stage& W) {
   ct_face_update(C, S, W, Ez_pp);
 }
 
+// ---------------------------------------------------------------- vector potential (UnStaggerCT, PotentialA)
+// bound_cond_A (boundary.cpp:177-292), periodic and transmissive branches: x first over every row,
+// then y over every column; transmissive ghosts are linear extrapolations of the two edge cells.
+void bc_A(const Ctx& C, Vec& Av) {
+  auto Ag = [&](int i, int j, int k) -> double& { return Av[NV * (i + j * C.NX) + k]; };
+  for (int j = 0; j < C.NY; j++)
+    for (int k = 0; k < 3; k++) {
+      if (C.P.bc_x == IMEX_BC_PERIODIC) {
+        for (int g = 0; g < GC; g++) { Ag(g, j, k) = Ag(C.nx + g, j, k); Ag(C.nx + GC + g, j, k) = Ag(GC + g, j, k); }
+      } else {
+        double sL = Ag(GC + 1, j, k) - Ag(GC, j, k);
+        for (int g = 0; g < GC; g++) Ag(GC - 1 - g, j, k) = Ag(GC, j, k) - (g + 1) * sL;
+        int iR = C.nx + GC - 1;
+        double sR = Ag(iR, j, k) - Ag(iR - 1, j, k);
+        for (int g = 0; g < GC; g++) Ag(iR + 1 + g, j, k) = Ag(iR, j, k) + (g + 1) * sR;
+      }
+    }
+  for (int i = 0; i < C.NX; i++)
+    for (int k = 0; k < 3; k++) {
+      if (C.P.bc_y == IMEX_BC_PERIODIC) {
+        for (int g = 0; g < GC; g++) { Ag(i, g, k) = Ag(i, C.ny + g, k); Ag(i, C.ny + GC + g, k) = Ag(i, GC + g, k); }
+      } else {
+        double sL = Ag(i, GC + 1, k) - Ag(i, GC, k);
+        for (int g = 0; g < GC; g++) Ag(i, GC - 1 - g, k) = Ag(i, GC, k) - (g + 1) * sL;
+        int jR = C.ny + GC - 1;
+        double sR = Ag(i, jR, k) - Ag(i, jR - 1, k);
+        for (int g = 0; g < GC; g++) Ag(i, jR + 1 + g, k) = Ag(i, jR, k) + (g + 1) * sR;
+      }
+    }
+}
+
+// GLF viscosity of the potential operator: directional maxima of the magnetic eigenvalues of Qs over
+// the interior (implicit_operators.cpp:359-374)
+void pot_visc(Ctx& C, const Vec& Qs, double& gx, double& gy) {
+  Vec pri(Qs.size());
+  grid_con2pri(C, pri, Qs);
+  gx = 0.0; gy = 0.0;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      const double* q = &pri[NV * (i + j * C.NX)];
+      double Bsq = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
+      double bxv = 0.5 * (std::fabs(q[1]) + std::sqrt(q[1] * q[1] + 4.0 * Bsq / q[0]));
+      double byv = 0.5 * (std::fabs(q[2]) + std::sqrt(q[2] * q[2] + 4.0 * Bsq / q[0]));
+      if (bxv > gx) gx = bxv;
+      if (byv > gy) gy = byv;
+    }
+}
+
+// Magnetic_Subsystem_potential_AX (implicit_operators.cpp:355-503), GLF branch with add_bvisc.
+void pot_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, const Vec& Qs, double gx, double gy,
+               const double* x, double* Ax) {
+  if (C.faithful) pot_visc(C, Qs, gx, gy);   // the reference recomputes these inside every apply (:359-374)
+  const double h = S.adt, dx = C.dx, dy = C.dy;
+  const int N = C.nx * C.ny;
+  Vec AG(NV * C.NX * C.NY, 0.0);
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++)
+      for (int k = 0; k < 3; k++) AG[NV * (i + j * C.NX) + k] = x[(i - GC) + C.nx * (j - GC) + k * N];
+  bc_A(C, AG);
+  auto a = [&](int ii, int jj, int k) { return AG[NV * (ii + jj * C.NX) + k]; };
+  auto Q = [&](int ii, int jj, int k) { return G(Qe, C, ii, jj, k); };
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double rho_00 = G(u_con, C, i, j, 0);
+      double bx_P0 = (a(i + 1, j + 1, 2) - a(i + 1, j - 1, 2)) / (2.0 * dy);
+      double by_P0 = -(a(i + 2, j, 2) - a(i, j, 2)) / (2.0 * dx);
+      double bz_P0 = (a(i + 2, j, 1) - a(i, j, 1)) / (2.0 * dx) - (a(i + 1, j + 1, 0) - a(i + 1, j - 1, 0)) / (2.0 * dy);
+      double bx_M0 = (a(i - 1, j + 1, 2) - a(i - 1, j - 1, 2)) / (2.0 * dy);
+      double by_M0 = -(a(i, j, 2) - a(i - 2, j, 2)) / (2.0 * dx);
+      double bz_M0 = (a(i, j, 1) - a(i - 2, j, 1)) / (2.0 * dx) - (a(i - 1, j + 1, 0) - a(i - 1, j - 1, 0)) / (2.0 * dy);
+      double bx_0P = (a(i, j + 2, 2) - a(i, j, 2)) / (2.0 * dy);
+      double by_0P = -(a(i + 1, j + 1, 2) - a(i - 1, j + 1, 2)) / (2.0 * dx);
+      double bz_0P = (a(i + 1, j + 1, 1) - a(i - 1, j + 1, 1)) / (2.0 * dx) - (a(i, j + 2, 0) - a(i, j, 0)) / (2.0 * dy);
+      double bx_0M = (a(i, j, 2) - a(i, j - 2, 2)) / (2.0 * dy);
+      double by_0M = -(a(i + 1, j - 1, 2) - a(i - 1, j - 1, 2)) / (2.0 * dx);
+      double bz_0M = (a(i + 1, j - 1, 1) - a(i - 1, j - 1, 1)) / (2.0 * dx) - (a(i, j, 0) - a(i, j - 2, 0)) / (2.0 * dy);
+      // magnetic stress of the linearised field: T(alpha, beta) = (B^n . b)/2 delta - B^n_beta b_alpha
+      double bd_P0 = Q(i + 1, j, 5) * bx_P0 + Q(i + 1, j, 6) * by_P0 + Q(i + 1, j, 7) * bz_P0;
+      double Txx_P0 = bd_P0 / 2.0 - Q(i + 1, j, 5) * bx_P0, Tyx_P0 = -Q(i + 1, j, 5) * by_P0, Tzx_P0 = -Q(i + 1, j, 5) * bz_P0;
+      double bd_M0 = Q(i - 1, j, 5) * bx_M0 + Q(i - 1, j, 6) * by_M0 + Q(i - 1, j, 7) * bz_M0;
+      double Txx_M0 = bd_M0 / 2.0 - Q(i - 1, j, 5) * bx_M0, Tyx_M0 = -Q(i - 1, j, 5) * by_M0, Tzx_M0 = -Q(i - 1, j, 5) * bz_M0;
+      double bd_0P = Q(i, j + 1, 5) * bx_0P + Q(i, j + 1, 6) * by_0P + Q(i, j + 1, 7) * bz_0P;
+      double Txy_0P = -Q(i, j + 1, 6) * bx_0P, Tyy_0P = bd_0P / 2.0 - Q(i, j + 1, 6) * by_0P, Tzy_0P = -Q(i, j + 1, 6) * bz_0P;
+      double bd_0M = Q(i, j - 1, 5) * bx_0M + Q(i, j - 1, 6) * by_0M + Q(i, j - 1, 7) * bz_0M;
+      double Txy_0M = -Q(i, j - 1, 6) * bx_0M, Tyy_0M = bd_0M / 2.0 - Q(i, j - 1, 6) * by_0M, Tzy_0M = -Q(i, j - 1, 6) * bz_0M;
+      double divT_x = (Txx_P0 - Txx_M0) / (2.0 * dx) + (Txy_0P - Txy_0M) / (2.0 * dy);
+      double divT_y = (Tyx_P0 - Tyx_M0) / (2.0 * dx) + (Tyy_0P - Tyy_0M) / (2.0 * dy);
+      double divT_z = (Tzx_P0 - Tzx_M0) / (2.0 * dx) + (Tzy_0P - Tzy_0M) / (2.0 * dy);
+      double Fx = h / rho_00 * divT_x, Fy = h / rho_00 * divT_y, Fz = h / rho_00 * divT_z;
+      double cross[3] = {Q(i, j, 6) * Fz - Q(i, j, 7) * Fy, Q(i, j, 7) * Fx - Q(i, j, 5) * Fz, Q(i, j, 5) * Fy - Q(i, j, 6) * Fx};
+      double nu_x = gx * dx, nu_y = gy * dy;
+      int idx = (i - GC) + C.nx * (j - GC);
+      for (int k = 0; k < 3; k++) {
+        double lap = nu_x / (dx * dx) * (a(i - 1, j, k) - 2.0 * a(i, j, k) + a(i + 1, j, k))
+                   + nu_y / (dy * dy) * (a(i, j - 1, k) - 2.0 * a(i, j, k) + a(i, j + 1, k));
+        Ax[idx + k * N] = a(i, j, k) - h * cross[k] - h * lap;
+      }
+    }
+  C.st.op_applies_mag++;
+}
+
+// MagSubsystem_UpdateA (imex_subsystems.cpp:5-74): W.A in (the initial guess, a copy of W.As) and out.
+int mag_update_a(Ctx& C, const imex_scalars& S, Stage& W) {
+  const double h = S.adt, dx = C.dx, dy = C.dy;
+  const int N = C.nx * C.ny;
+  Vec pri(W.Qe.size());
+  grid_con2pri(C, pri, W.Qe);
+  Vec b(3 * N), x(3 * N);
+  const Vec& u = W.u_con;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double Bxn = G(W.Qe, C, i, j, 5), Byn = G(W.Qe, C, i, j, 6), Bzn = G(W.Qe, C, i, j, 7);
+      double rho = G(u, C, i, j, 0);
+      double Sx = G(u, C, i, j, 1) / rho - h / rho * (G(pri, C, i + 1, j, 4) - G(pri, C, i - 1, j, 4)) / (2.0 * dx);
+      double Sy = G(u, C, i, j, 2) / rho - h / rho * (G(pri, C, i, j + 1, 4) - G(pri, C, i, j - 1, 4)) / (2.0 * dy);
+      double Sz = G(u, C, i, j, 3) / rho;
+      int idx = (i - GC) + C.nx * (j - GC);
+      b[idx] = G(W.As, C, i, j, 0) - h * (Byn * Sz - Bzn * Sy);
+      b[idx + N] = G(W.As, C, i, j, 1) - h * (Bzn * Sx - Bxn * Sz);
+      b[idx + 2 * N] = G(W.As, C, i, j, 2) - h * (Bxn * Sy - Byn * Sx);
+      for (int k = 0; k < 3; k++) x[idx + k * N] = G(W.A, C, i, j, k);
+    }
+  double gx, gy;
+  pot_visc(C, W.Qs, gx, gy);
+  std::function<void(const double*, double*)> Aop = [&](const double* xin, double* out) { pot_apply(C, S, W.u_con, W.Qe, W.Qs, gx, gy, xin, out); };
+  int rc = gmres(C, x, b, Aop, nullptr, nullptr, &C.st.mag_gmres_iters, &C.st.mag_gmres_restarts);
+  C.st.mag_solves++;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++)
+      for (int k = 0; k < 3; k++) G(W.A, C, i, j, k) = x[(i - GC) + C.nx * (j - GC) + k * N];
+  bc_A(C, W.A);
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      auto a = [&](int ii, int jj, int k) { return G(W.A, C, ii, jj, k); };
+      G(W.u_con, C, i, j, 5) = 1 / (2.0 * dy) * (a(i, j + 1, 2) - a(i, j - 1, 2));
+      G(W.u_con, C, i, j, 6) = -1 / (2.0 * dx) * (a(i + 1, j, 2) - a(i - 1, j, 2));
+      G(W.u_con, C, i, j, 7) = 1 / (2.0 * dx) * (a(i + 1, j, 1) - a(i - 1, j, 1)) - 1 / (2.0 * dy) * (a(i, j + 1, 0) - a(i, j - 1, 0));
+    }
+  bc_grid(C, W.u_con);
+  return rc;
+}
+
+// Unstaggered_CT_Consistent (constrained_transport.cpp:502-540): W.A in (components 0, 1 and the
+// initial A_z are the stage's copy of W.As) and out.
+void unstaggered_ct(Ctx& C, const imex_scalars& S, Stage& W) {
+  const double h = S.adt;
+  Vec Ez_p0(C.NX * C.NY), Ez_0p(C.NX * C.NY), Et(C.NX * C.NY), Ez_pp(C.NX * C.NY, 0.0);
+  for (int n = 0; n < C.NX * C.NY; n++) { Ez_p0[n] = -W.fx[n]; Ez_0p[n] = +W.fy[n]; }
+  for (int j = 0; j < C.NY; j++)
+    for (int i = 0; i < C.NX; i++)
+      A(Et, C, i, j) = G(W.u_con, C, i, j, 2) / G(W.u_con, C, i, j, 0) * G(W.u_con, C, i, j, 5)
+                     - G(W.u_con, C, i, j, 1) / G(W.u_con, C, i, j, 0) * G(W.u_con, C, i, j, 6);
+  corner_assemble(C, Ez_pp, Ez_p0, Ez_0p, Et);
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double Ez_cell = 0.25 * (A(Ez_pp, C, i, j) + A(Ez_pp, C, i - 1, j) + A(Ez_pp, C, i, j - 1) + A(Ez_pp, C, i - 1, j - 1));
+      G(W.A, C, i, j, 2) = G(W.As, C, i, j, 2) - h * Ez_cell;
+    }
+  bc_A(C, W.A);
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      G(W.u_con, C, i, j, 5) = (G(W.A, C, i, j + 1, 2) - G(W.A, C, i, j - 1, 2)) / (2.0 * C.dy);
+      G(W.u_con, C, i, j, 6) = -(G(W.A, C, i + 1, j, 2) - G(W.A, C, i - 1, j, 2)) / (2.0 * C.dx);
+    }
+  bc_grid(C, W.u_con);
+}
+
 // ---------------------------------------------------------------- time step (timestep.cpp:5-67, 89-120, 254-273)
 void cfl(Ctx& C, imex_scalars& S, const Vec& u_pri) {
   double ch_x = 1e-8, ch_y = 1e-8, ch = 1e-8;
@@ -1119,6 +1288,7 @@ struct Run {
   imex_scalars S{};
   Vec u_pri, u_con, u_n, Hs[2];
   Vec bx, by, bx_n, by_n, HsB[2][2];
+  Vec Apot, A_n, HsA[2];       // vector potential (Grid, slots 0..2), its step-start copy and stage increments
   Stage W;
   int cap_point = -1;          // capture: 10*rks + sub (sub: 0 before conv, 1 before mag, 2 before press-flux, 3 before pressure, 4 before CT, 5 end of stage)
   Stage cap; imex_scalars cap_S{};
@@ -1139,6 +1309,9 @@ int run_step(Run& R) {
   grid_pri2con(C, R.u_con, R.u_pri);
   R.u_n = R.u_con;
   R.bx_n = R.bx; R.by_n = R.by;
+  const int mf = C.P.magfield;
+  const bool useA = mf == IMEX_MAGFIELD_UNSTAGGER || mf == IMEX_MAGFIELD_POTENTIAL_A;
+  if (useA) R.A_n = R.Apot;                                          // main.cpp:347-348
   const double bb = 1.0 - 1.0 / std::sqrt(2.0);
   const double Ae10 = 1.0 / (2.0 * bb);
   const double Ai[2][2] = {{bb, 0.0}, {1 - bb, bb}};
@@ -1154,6 +1327,15 @@ int run_step(Run& R) {
       for (size_t n = 0; n < NG; n++) { W.Qe[n] = W.Qe[n] + ce * R.Hs[r][n]; W.Qs[n] = W.Qs[n] + ci * R.Hs[r][n]; }
       for (size_t n = 0; n < NA; n++) { W.bx_qs[n] += ci * R.HsB[r][0][n]; W.by_qs[n] += ci * R.HsB[r][1][n]; }
     }
+    if (useA) {                                                      // main.cpp:371-412
+      W.As = R.A_n;
+      for (int r = 0; r < rks; r++) {
+        double ci = Ai[rks][r] * S.dt;
+        for (size_t n = 0; n < NG; n += NV)
+          for (int k = 0; k < 3; k++) W.As[n + k] += ci * R.HsA[r][n + k];
+      }
+      W.A = W.As;
+    }
     S.adt = Ai[rks][rks] * S.dt;
     W.u_con = R.u_con; W.bx = R.bx; W.by = R.by;
     if (W.fx.size() != NA) { W.fx.assign(NA, 0.0); W.fy.assign(NA, 0.0); }
@@ -1163,7 +1345,7 @@ int run_step(Run& R) {
     flux_update(C, S, W.u_con, W.Qs, w, 0);
     bc_grid(C, W.u_con);
     maybe_capture(R, 10 * rks + 1);
-    rc |= mag_update_b(C, S, W);
+    rc |= mf == IMEX_MAGFIELD_POTENTIAL_A ? mag_update_a(C, S, W) : mag_update_b(C, S, W);   // main.cpp:424-434
     bc_grid(C, W.u_con);
     maybe_capture(R, 10 * rks + 2);
     for (size_t n = 0; n < NG; n += NV)
@@ -1175,9 +1357,19 @@ int run_step(Run& R) {
     rc |= energy ? pre_subsystem_energy(C, S, W) : pre_subsystem_pressure(C, S, W);
     bc_grid(C, W.u_con);
     maybe_capture(R, 10 * rks + 4);
-    if (C.P.ct_mode == IMEX_CT_PLAIN) ct_plain(C, S, W); else ct_consistent(C, S, W);
+    if (mf == IMEX_MAGFIELD_STAGGER) {                                 // main.cpp:497-510
+      if (C.P.ct_mode == IMEX_CT_PLAIN) ct_plain(C, S, W); else ct_consistent(C, S, W);
+    } else if (mf == IMEX_MAGFIELD_UNSTAGGER) {
+      unstaggered_ct(C, S, W);
+    }
     bc_grid(C, W.u_con);
     R.u_con = W.u_con; R.bx = W.bx; R.by = W.by;
+    if (useA) {                                                      // main.cpp:546-551
+      R.Apot = W.A;
+      if (R.HsA[rks].size() != NG) R.HsA[rks].assign(NG, 0.0);
+      for (size_t n = 0; n < NG; n += NV)
+        for (int k = 0; k < 3; k++) R.HsA[rks][n + k] = (W.A[n + k] - W.As[n + k]) / S.adt;
+    }
     grid_con2pri(C, R.u_pri, R.u_con);
     if (R.Hs[rks].size() != NG) R.Hs[rks].assign(NG, 0.0);
     for (size_t n = 0; n < NG; n++) R.Hs[rks][n] = (R.u_con[n] - W.Qs[n]) / S.adt;
@@ -1196,6 +1388,25 @@ double div_b(const Ctx& C, const Vec& bx, const Vec& by) {   // imex_subsystems.
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {
       double d = (A(bx, C, i, j) - A(bx, C, i - 1, j)) / C.dx + (A(by, C, i, j) - A(by, C, i, j - 1)) / C.dy;
+      err += C.dx * C.dy * d * d;
+    }
+  return std::sqrt(err);
+}
+
+// cal_divergence_error for UnStaggerCT / PotentialA / NoDivFree (imex_subsystems.cpp:640-655): the cell
+// B's divergence, central inside, one-sided at the domain edges.
+double div_b_cell(const Ctx& C, const Vec& u) {
+  double err = 0;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double dBxdx, dBydy;
+      if (i == GC) dBxdx = (G(u, C, i + 1, j, 5) - G(u, C, i, j, 5)) / C.dx;
+      else if (i == C.nx + GC - 1) dBxdx = (G(u, C, i, j, 5) - G(u, C, i - 1, j, 5)) / C.dx;
+      else dBxdx = (G(u, C, i + 1, j, 5) - G(u, C, i - 1, j, 5)) / (2.0 * C.dx);
+      if (j == GC) dBydy = (G(u, C, i, j + 1, 6) - G(u, C, i, j, 6)) / C.dy;
+      else if (j == C.ny + GC - 1) dBydy = (G(u, C, i, j, 6) - G(u, C, i, j - 1, 6)) / C.dy;
+      else dBydy = (G(u, C, i, j + 1, 6) - G(u, C, i, j - 1, 6)) / (2.0 * C.dy);
+      double d = dBxdx + dBydy;
       err += C.dx * C.dy * d * d;
     }
   return std::sqrt(err);
@@ -1234,7 +1445,29 @@ void* orc_create(const imex_params* p, int faithful) {
   size_t NG = (size_t)R->C.NX * R->C.NY * NV, NA = (size_t)R->C.NX * R->C.NY;
   R->u_pri.assign(NG, 0.0); R->u_con.assign(NG, 0.0);
   R->bx.assign(NA, 0.0); R->by.assign(NA, 0.0);
+  R->Apot.assign(NG, 0.0);
   return R;
+}
+
+// the step-start vector potential (Grid, slots 0..2)
+int orc_set_potential(void* h, const double* A) {
+  Run& R = *(Run*)h;
+  std::memcpy(R.Apot.data(), A, R.Apot.size() * sizeof(double));
+  return 0;
+}
+int orc_get_potential(void* h, double* A) {
+  Run& R = *(Run*)h;
+  std::memcpy(A, R.Apot.data(), R.Apot.size() * sizeof(double));
+  return 0;
+}
+// A and A_s of the last capture (orc_step_capture)
+int orc_capture_potential(void* h, double* A, double* As) {
+  Run& R = *(Run*)h;
+  size_t n = (size_t)R.C.NX * R.C.NY * NV;
+  if (R.cap.A.size() != n) return -1;
+  std::memcpy(A, R.cap.A.data(), n * sizeof(double));
+  std::memcpy(As, R.cap.As.data(), n * sizeof(double));
+  return 0;
 }
 void orc_destroy(void* h) { delete (Run*)h; }
 
@@ -1279,7 +1512,7 @@ int orc_get_state(void* h, double* u_con, double* u_pri, double* bx, double* by,
 int orc_get_stats(void* h, imex_stats* st) {
   Run& R = *(Run*)h;
   *st = R.C.st;
-  st->div_b = div_b(R.C, R.bx, R.by);
+  st->div_b = R.C.P.magfield == IMEX_MAGFIELD_STAGGER ? div_b(R.C, R.bx, R.by) : div_b_cell(R.C, R.u_con);
   return 0;
 }
 
@@ -1325,6 +1558,40 @@ int orc_ct_update(const imex_params* p, const imex_scalars* sc, imex_stage_host*
 int orc_explicit_energy(const imex_params* p, const imex_scalars* sc, imex_stage_host* st) { return run_sub(p, sc, st, 5, nullptr); }
 int orc_pre_subsystem_energy(const imex_params* p, const imex_scalars* sc, imex_stage_host* st, imex_stats* s) { return run_sub(p, sc, st, 6, s); }
 
+// MagSubsystem_UpdateA / Unstaggered_CT_Consistent on a stage snapshot; A in/out, A_s in (Grids).
+static int run_sub_a(const imex_params* p, const imex_scalars* sc, imex_stage_host* st, double* A, const double* As,
+                     int which, imex_stats* stats) {
+  Ctx C; ctx_init(C, p);
+  Stage W; stage_from_host(C, W, st);
+  size_t NG = (size_t)C.NX * C.NY * NV;
+  W.A.assign(A, A + NG); W.As.assign(As, As + NG);
+  int rc = 0;
+  if (which == 0) rc = mag_update_a(C, *sc, W); else unstaggered_ct(C, *sc, W);
+  bc_grid(C, W.u_con);
+  stage_to_host(W, st);
+  std::memcpy(A, W.A.data(), NG * sizeof(double));
+  if (stats) *stats = C.st;
+  return rc;
+}
+int orc_mag_update_a(const imex_params* p, const imex_scalars* sc, imex_stage_host* st, double* A, const double* As, imex_stats* s) {
+  return run_sub_a(p, sc, st, A, As, 0, s);
+}
+int orc_unstaggered_ct(const imex_params* p, const imex_scalars* sc, imex_stage_host* st, double* A, const double* As) {
+  return run_sub_a(p, sc, st, A, As, 1, nullptr);
+}
+int orc_bc_potential(const imex_params* p, double* A) {
+  Ctx C; ctx_init(C, p);
+  Vec v(A, A + (size_t)C.NX * C.NY * NV);
+  bc_A(C, v);
+  std::memcpy(A, v.data(), v.size() * sizeof(double));
+  return 0;
+}
+double orc_div_b_cell(const imex_params* p, const double* u_con) {
+  Ctx C; ctx_init(C, p);
+  Vec v(u_con, u_con + (size_t)C.NX * C.NY * NV);
+  return div_b_cell(C, v);
+}
+
 // One face flux of the explicit updates: kind 0 RusanovC, 1 Pressure_flux, 2 Energy_flux; dir 0 x, 1 y.
 int orc_flux(const imex_params* p, int kind, int dir, const double* QL, const double* QR, double s_max, double* out) {
   imex_scalars S{};
@@ -1351,6 +1618,7 @@ int orc_apply_op(const imex_params* p, const imex_scalars* sc, const imex_stage_
   Stage W; stage_from_host(C, W, st);
   if (op == IMEX_OP_MAG) { mag_apply(C, *sc, W.u_con, W.Qe, x, Ax); return 0; }
   if (op == IMEX_OP_ENERGY) { Vec hE; e_coef(C, W.u_con, W.Qe, hE); e_apply(C, *sc, W.u_con, W.Qe, hE, x, Ax); return 0; }
+  if (op == IMEX_OP_POTENTIAL) { double gx, gy; pot_visc(C, W.Qs, gx, gy); pot_apply(C, *sc, W.u_con, W.Qe, W.Qs, gx, gy, x, Ax); return 0; }
   PCoef pc; p_coef(C, W.u_con, W.Qe, pc);
   p_apply(C, *sc, W.u_con, W.Qe, pc, x, Ax, op == IMEX_OP_P_JACOBIAN, p_freeze);
   return 0;
@@ -1360,9 +1628,17 @@ int orc_gmres(const imex_params* p, const imex_scalars* sc, const imex_stage_hos
               const double* b, double* x, int precond, const double* p_freeze, double mbig, imex_solve_info* info) {
   Ctx C; ctx_init(C, p); C.Mbig = mbig;
   Stage W; stage_from_host(C, W, st);
-  const int N = C.nx * C.ny, dim = op == IMEX_OP_MAG ? 3 * N : N;
+  const int N = C.nx * C.ny, dim = (op == IMEX_OP_MAG || op == IMEX_OP_POTENTIAL) ? 3 * N : N;
   Vec xv(x, x + dim), bv(b, b + dim);
   const imex_scalars S = *sc;
+  if (op == IMEX_OP_POTENTIAL) {
+    double gx, gy;
+    pot_visc(C, W.Qs, gx, gy);
+    std::function<void(const double*, double*)> Ap = [&](const double* xin, double* out) { pot_apply(C, S, W.u_con, W.Qe, W.Qs, gx, gy, xin, out); };
+    int rc = gmres(C, xv, bv, Ap, nullptr, info, nullptr, nullptr);
+    std::memcpy(x, xv.data(), dim * sizeof(double));
+    return rc;
+  }
   if (op == IMEX_OP_ENERGY) {
     Vec hE, D;
     e_coef(C, W.u_con, W.Qe, hE);

@@ -33,3 +33,11 @@ def make_case(name, n=None, m=None, amp=None, **over):
     a = kw.get("amp", 1.0) if amp is None else amp
     u, bx, by = im.init_case(p, kw["case"], a, kw.get("u0", 0.0), kw.get("v0", 0.0), kw.get("smooth", 0))
     return p, u, bx, by
+
+
+def make_case_a(name, n=None, m=None, amp=None, **over):
+    """make_case plus the case's vector potential (for the UNSTAGGER / POTENTIAL_A treatments)."""
+    p, u, bx, by = make_case(name, n, m, amp, **over)
+    _, kw = im.case_of(name)
+    A = im.init_potential(p, kw["case"], kw.get("amp", 1.0) if amp is None else amp)
+    return p, u, bx, by, A

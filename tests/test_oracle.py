@@ -21,13 +21,23 @@ term, a wrong sign or index, or a transposed operand fails a check:
                                             constant vector -> c dx dy; precE divisor = operator diagonal
   energy subsystem (imex_subsystems.cpp:415-534)    closed-form solution and momentum update for one Fourier
                                             mode; total energy and momentum conserved on a periodic grid
+  bound_cond_A (boundary.cpp:177-292)       linear fields reproduced exactly by the extrapolation; periodic brute force
+  A_initial (constrained_transport.cpp:224-398)  curl A = the case's B (exact for the blast, O(dx^2) otherwise)
+  potential operator (implicit_operators.cpp:355-503)  h = 0 identity; linear A on a uniform state unchanged;
+                                            Fourier eigenvalue of the magnetic-stress and viscosity terms
+  MagSubsystem_UpdateA (imex_subsystems.cpp:5-74)  closed-form solution for a constant A_s on a uniform state
+                                            with a linear pressure; B = curl A
+  Unstaggered_CT_Consistent (constrained_transport.cpp:502-540)  uniform corner EMF shifts A_z by -h E;
+                                            central div B = 0 inside
   GMRES (implicit_operators.cpp:793-919)    dense numpy.linalg.solve on assembled operators
   nested Newton (implicit_operators.cpp:625-738)  reduces to one linear solve when no cell is floored; p >= theta p_floor
   CT (ct_flux.cpp:150-191)                  discrete div B of the faces unchanged by any EMF
   div B metric (imex_subsystems.cpp:630-660)  closed form on analytic faces
   CFL (timestep.cpp:5-67, 89-120, 254-273)  brute-force numpy maxima and the dt rule
   whole step (main.cpp:334-595)             mass conservation, div B preservation, vortex advection; energy
-                                            formula: mass, momentum and total energy conservation
+                                            formula: mass, momentum and total energy conservation; UnStaggerCT /
+                                            PotentialA: central div B = 0 inside, mass conservation; NoDivFree:
+                                            faces untouched
 
 The composed algebra of MagSubsystem_UpdateB's right-hand side, CollectMagFluxes,
 the two-pass pressure solve and the stage combination follows the reference line by
@@ -38,7 +48,7 @@ import pytest
 
 import imex_amd as im
 import oracle as orc
-from helpers import make_case, G
+from helpers import make_case, make_case_a, G
 
 
 def ideal(n=8, m=None, bc=im.BC_PERIODIC, recon=im.RECON_MINMOD):
@@ -320,7 +330,7 @@ def _snapshot(case="ot", n=12, m=10, point=3):
 
 
 def _assemble(p, sc, st, op, pf=None, mbig=0.0):
-    n = p.nx * p.ny * (3 if op == im.OP_MAG else 1)
+    n = p.nx * p.ny * (3 if op in (im.OP_MAG, im.OP_POTENTIAL) else 1)
     A = np.zeros((n, n))
     for c in range(n):
         e = np.zeros(n)
@@ -445,10 +455,195 @@ def test_energy_subsystem_conserves_totals():
     assert not np.allclose(out["u_con"][..., 4], st["u_con"][..., 4])
 
 
+# ---------------------------------------------------------------- vector potential
+@pytest.mark.parametrize("bcx,bcy", [(im.BC_TRANSMISSIVE, im.BC_TRANSMISSIVE), (im.BC_PERIODIC, im.BC_PERIODIC),
+                                     (im.BC_PERIODIC, im.BC_TRANSMISSIVE)])
+def test_bc_potential(bcx, bcy):
+    p = ideal(9, 7)
+    p.bc_x, p.bc_y = bcx, bcy
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    J, I = np.meshgrid(np.arange(NY), np.arange(NX), indexing="ij")
+    A = np.zeros((NY, NX, 8))
+    lin = [0.25 + 0.5 * I - 0.125 * J, -1.0 + 0.75 * J, 2.0 - 0.375 * I + 0.0625 * J]   # dyadic: exact arithmetic
+    rng = np.random.default_rng(31)
+    for k in range(3):
+        A[..., k] = lin[k] if bcx == bcy == im.BC_TRANSMISSIVE else rng.standard_normal((NY, NX))
+    A0 = A.copy()
+    A[:G, :] = A[-G:, :] = 7.0            # ghosts overwritten by the fill
+    A[:, :G] = A[:, -G:] = 7.0
+    out = orc.bc_potential(p, A)
+    if bcx == bcy == im.BC_TRANSMISSIVE:
+        for k in range(3):
+            assert np.array_equal(out[..., k], lin[k]), k
+    else:
+        def src(n, i, per):
+            if per:
+                return (i - G) % n + G
+            return i                           # not used below for transmissive directions
+        for j in range(NY):
+            for i in range(NX):
+                if G <= i < p.nx + G and G <= j < p.ny + G:
+                    continue
+                si = src(p.nx, i, bcx == im.BC_PERIODIC)
+                if bcy == im.BC_PERIODIC:
+                    sj = src(p.ny, j, True)
+                    assert np.array_equal(out[j, i, :3], A0[sj, si, :3]), (i, j)
+                elif G <= j < p.ny + G:
+                    assert np.array_equal(out[j, i, :3], A0[j, si, :3]), (i, j)
+                else:                          # y extrapolation of the x-filled column
+                    col = out[:, i, :3]
+                    if j < G:
+                        want = col[G] - (G - j) * (col[G + 1] - col[G])
+                    else:
+                        e = p.ny + G - 1
+                        want = col[e] + (j - e) * (col[e] - col[e - 1])
+                    assert np.array_equal(out[j, i, :3], want), (i, j)
+        assert np.array_equal(out[G:-G, G:-G], A0[G:-G, G:-G])
+
+
+@pytest.mark.parametrize("case,exact", [("blast", True), ("vortex", False), ("ot", True)])
+def test_init_potential_curl_is_the_case_field(case, exact):
+    # central curl of A_z against the case's cell B: exact for the blast (linear A_z); for Orszag-Tang
+    # the central difference of cos(k s) is -k sin(k s) sin(k d) / (k d), compared exactly; the vortex
+    # to O(dx^2)
+    p, u, bx, by, A = make_case_a(case, 64)
+    dx, dy = (p.x1 - p.x0) / p.nx, (p.y1 - p.y0) / p.ny
+    Az = A[..., 2]
+    Bx = (Az[2:, 1:-1] - Az[:-2, 1:-1]) / (2 * dy)
+    By = -(Az[1:-1, 2:] - Az[1:-1, :-2]) / (2 * dx)
+    ref_x, ref_y = u[1:-1, 1:-1, 5], u[1:-1, 1:-1, 6]
+    if case == "ot":
+        ref_x = ref_x * np.sin(dy) / dy
+        ref_y = ref_y * np.sin(2 * dx) / (2 * dx)
+    sl = (slice(G - 1, -(G - 1)), slice(G - 1, -(G - 1)))
+    scale = max(np.abs(ref_x[sl]).max(), np.abs(ref_y[sl]).max())
+    tol = 1e-12 if exact else 1e-2
+    assert np.abs(Bx[sl] - ref_x[sl]).max() <= tol * scale
+    assert np.abs(By[sl] - ref_y[sl]).max() <= tol * scale
+
+
+def _pot_stage(p, prim_e, prim_s=None, prim_u=None):
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    st = im.new_stage(p)
+    ce = orc.pri2con(p, np.tile(np.asarray(prim_e, float), (NY, NX, 1)))
+    st["Qe"][:] = ce
+    st["Qs"][:] = orc.pri2con(p, np.tile(np.asarray(prim_s if prim_s is not None else prim_e, float), (NY, NX, 1)))
+    st["u_con"][:] = orc.pri2con(p, np.tile(np.asarray(prim_u if prim_u is not None else prim_e, float), (NY, NX, 1)))
+    return st
+
+
+def test_potential_operator_identity_linear_and_linearity():
+    p = ideal(10, 8, bc=im.BC_TRANSMISSIVE)
+    st = _pot_stage(p, [1.3, 0.2, -0.1, 0.05, 0.9, 0.4, -0.6, 0.3])
+    N = p.nx * p.ny
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal(3 * N)
+    assert np.array_equal(orc.apply_op(p, dict(adt=0.0), st, im.OP_POTENTIAL, x), x)
+    sc = dict(adt=0.03)
+    I, J = np.meshgrid(np.arange(p.nx), np.arange(p.ny))
+    lin = np.concatenate([(0.3 + 0.1 * I - 0.2 * J).ravel(), (-0.5 * I + 0.25 * J).ravel(), (1.0 + 0.7 * J).ravel()])
+    assert np.allclose(orc.apply_op(p, sc, st, im.OP_POTENTIAL, lin), lin, rtol=0, atol=1e-13)
+    y = rng.standard_normal(3 * N)
+    Ax, Ay = orc.apply_op(p, sc, st, im.OP_POTENTIAL, x), orc.apply_op(p, sc, st, im.OP_POTENTIAL, y)
+    Az = orc.apply_op(p, sc, st, im.OP_POTENTIAL, 2.0 * x - 3.0 * y)
+    assert np.allclose(Az, 2 * Ax - 3 * Ay, rtol=0, atol=1e-12 * np.abs(Az).max())
+
+
+@pytest.mark.parametrize("comp", [2, 0])
+def test_potential_operator_fourier_eigenvalue(comp):
+    # uniform B^n along the mode's direction d and rho (of u): A_z(x) (comp 2, B^n = (B0,0,0)) or A_x(y)
+    # (comp 0, B^n = (0,B0,0)) is an eigenvector with eigenvalue
+    # 1 + h^2 B0^2 sin^2(theta) / (rho d^2) + 4 h g sin^2(theta/2) / d, with g the largest magnetic
+    # eigenvalue 0.5 (|v_d| + sqrt(v_d^2 + 4 |B|^2 / rho)) of Qs in direction d
+    p = ideal(16, 12)
+    p.x1, p.y1 = 1.6, 0.9
+    rho, B0 = 1.7, 0.8
+    prim = [rho, 0, 0, 0, 1.0, B0, 0, 0] if comp == 2 else [rho, 0, 0, 0, 1.0, 0, B0, 0]
+    ps = [1.2, 0.3, -0.45, 0.1, 0.8, 0.5, 0.6, -0.2]                  # Qs differs from Qe
+    st = _pot_stage(p, prim, prim_s=ps)
+    h = 0.02
+    n, dd = (p.nx, 1.6 / p.nx) if comp == 2 else (p.ny, 0.9 / p.ny)
+    I, J = np.meshgrid(np.arange(p.nx), np.arange(p.ny))
+    N = p.nx * p.ny
+    vd = ps[1] if comp == 2 else ps[2]
+    g = 0.5 * (abs(vd) + np.sqrt(vd * vd + 4 * (ps[5] ** 2 + ps[6] ** 2 + ps[7] ** 2) / ps[0]))
+    for k in (1, 3, 5):
+        th = 2 * np.pi * k / n
+        mode = np.cos(th * (I if comp == 2 else J)).ravel()
+        x = np.zeros(3 * N)
+        x[comp * N:(comp + 1) * N] = mode
+        lam = 1 + h * h * B0 * B0 * np.sin(th) ** 2 / (rho * dd * dd) + 4 * h * g * np.sin(th / 2) ** 2 / dd
+        Ax = orc.apply_op(p, dict(adt=h), st, im.OP_POTENTIAL, x)
+        assert np.allclose(Ax[comp * N:(comp + 1) * N], lam * mode, rtol=0, atol=1e-13 * lam), k
+        others = np.delete(Ax.reshape(3, N), comp, axis=0)
+        assert np.allclose(others, 0.0, atol=1e-13)
+
+
+def test_mag_update_a_closed_form():
+    # uniform state and B^n, p^e linear (transmissive grid, so the central gradient is exact), A_s
+    # constant: b = A_s - h B^n x S with S = v - (h / rho) grad p is constant, the operator is the
+    # identity on constants, so A = b and B = curl A = 0
+    p = ideal(10, 8, bc=im.BC_TRANSMISSIVE)
+    p.x1, p.y1 = 2.0, 1.0
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    dx, dy = 2.0 / p.nx, 1.0 / p.ny
+    rho, v, B = 1.4, np.array([0.3, -0.2, 0.15]), np.array([0.5, -0.4, 0.25])
+    gx, gy = 0.7, -0.35
+    prim = np.zeros((NY, NX, 8))
+    X = (np.arange(NX) - G + 0.5) * dx
+    Y = (np.arange(NY) - G + 0.5) * dy
+    prim[..., 0] = rho
+    prim[..., 1:4] = v
+    prim[..., 4] = 2.0 + gx * X[None, :] + gy * Y[:, None]
+    prim[..., 5:8] = B
+    st = im.new_stage(p)
+    st["Qe"][:] = orc.pri2con(p, prim)
+    st["Qs"][:] = st["Qe"]
+    st["u_con"][:] = st["Qe"]
+    As = np.zeros((NY, NX, 8))
+    As[..., :3] = [0.4, -1.1, 0.8]
+    h = 0.05
+    out, A, rc, stats = orc.sub_a(p, dict(adt=h), st, As, As, "mag_update_a")
+    assert rc == 0 and stats["mag_solves"] == 1
+    S = v - (h / rho) * np.array([gx, gy, 0.0])
+    b = As[0, 0, :3] - h * np.cross(B, S)
+    assert np.allclose(A[G:-G, G:-G, :3], b, rtol=0, atol=1e-13)
+    assert np.allclose(out["u_con"][G:-G, G:-G, 5:8], 0.0, atol=1e-12)
+
+
+def test_unstaggered_ct_uniform_emf_and_divergence():
+    p, u, bx, by, A0 = make_case_a("vortex", 16, 12, magfield=im.MAGFIELD_UNSTAGGER)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.set_potential(A0)
+    o.step(1)
+    st, sc, _ = o.step_capture(14)
+    A, As = o.capture_potential()
+    E0 = 0.37
+    st = {k: v.copy() for k, v in st.items()}
+    st["fx"][:] = -E0
+    st["fy"][:] = E0
+    r = st["u_con"]                        # cell Ez~ = v_y B_x - v_x B_y = E0 everywhere: B_x = E0 rho / m_y
+    r[..., 5] = E0 * r[..., 0] / r[..., 2]
+    r[..., 6] = 0.0
+    h = sc["adt"]
+    out, A1, rc, _ = orc.sub_a(p, sc, st, A, As, "unstaggered_ct")
+    assert np.allclose(A1[G:-G, G:-G, 2], As[G:-G, G:-G, 2] - h * E0, rtol=0, atol=1e-13)
+    assert np.array_equal(A1[G:-G, G:-G, :2], A[G:-G, G:-G, :2])
+    st2, _, _ = o.step_capture(14)
+    A, As = o.capture_potential()
+    out, A1, rc, _ = orc.sub_a(p, sc, st2, A, As, "unstaggered_ct")
+    Bc = out["u_con"]
+    dx, dy = 10.0 / p.nx, 10.0 / p.ny
+    div = (Bc[G:-G, G + 1:-G + 1, 5] - Bc[G:-G, G - 1:-G - 1, 5]) / (2 * dx) + (Bc[G + 1:-G + 1, G:-G, 6] - Bc[G - 1:-G - 1, G:-G, 6]) / (2 * dy)
+    assert np.abs(div[1:-1, 1:-1]).max() <= 1e-12 * np.abs(Bc[..., 5:7]).max() / dx
+
+
 @pytest.mark.parametrize("op,precond", [(im.OP_MAG, 0), (im.OP_P_CARRIER, 1), (im.OP_P_CARRIER, 0), (im.OP_ENERGY, 1),
-                                         (im.OP_ENERGY, 0)])
+                                         (im.OP_ENERGY, 0), (im.OP_POTENTIAL, 0)])
 def test_gmres_against_dense_solve(op, precond):
-    p, st, sc = _snapshot("ot" if op else "vortex", 8, 6, 1 if op == im.OP_MAG else 3)
+    p, st, sc = _snapshot("ot" if op not in (im.OP_MAG, im.OP_POTENTIAL) else "vortex", 8, 6,
+                          1 if op in (im.OP_MAG, im.OP_POTENTIAL) else 3)
     A = _assemble(p, sc, st, op)
     rng = np.random.default_rng(5)
     b = rng.standard_normal(A.shape[0])
@@ -572,6 +767,40 @@ def test_run_energy_formula_conserves_totals(case):
     assert o.stats()["div_b"] <= 1e-12
     assert np.all(up[G:-G, G:-G, 4] > 0)
     assert o.stats()["newton_iters"] == 0 and o.stats()["p_solves"] == 8
+
+
+@pytest.mark.parametrize("case,mf", [("vortex", im.MAGFIELD_UNSTAGGER), ("ot", im.MAGFIELD_POTENTIAL_A),
+                                     ("vortex", im.MAGFIELD_POTENTIAL_A), ("blast", im.MAGFIELD_UNSTAGGER)])
+def test_run_potential_treatments(case, mf):
+    p, u, bx, by, A = make_case_a(case, 24, amp=10.0 if case == "blast" else None, magfield=mf)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.set_potential(A)
+    m0 = orc.pri2con(p, u)[G:-G, G:-G, 0].sum()
+    o.step(3)
+    uc, up, bx1, by1, sc = o.get_state()
+    assert np.all(np.isfinite(uc)) and np.all(up[G:-G, G:-G, 4] > 0)
+    if case != "blast":
+        assert abs(uc[G:-G, G:-G, 0].sum() - m0) <= 1e-13 * m0
+    dx, dy = (p.x1 - p.x0) / p.nx, (p.y1 - p.y0) / p.ny
+    div = (uc[G:-G, G + 1:-G + 1, 5] - uc[G:-G, G - 1:-G - 1, 5]) / (2 * dx) + (uc[G + 1:-G + 1, G:-G, 6] - uc[G - 1:-G - 1, G:-G, 6]) / (2 * dy)
+    assert np.abs(div[1:-1, 1:-1]).max() <= 1e-11 * np.abs(uc[..., 5:7]).max() / dx
+    if case == "blast":                    # linear extrapolation at the edges: the one-sided metric vanishes too
+        assert o.stats()["div_b"] <= 1e-11
+    assert np.array_equal(bx1, bx) and np.array_equal(by1, by)          # the faces are left alone
+    Ab = o.get_potential()
+    assert not np.array_equal(Ab[..., 2], A[..., 2])
+
+
+def test_run_nodivfree_leaves_faces_and_solves_b():
+    p, u, bx, by = make_case("ot", 16, magfield=im.MAGFIELD_NODIVFREE)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(2)
+    uc, up, bx1, by1, sc = o.get_state()
+    assert np.array_equal(bx1, bx) and np.array_equal(by1, by)
+    assert o.stats()["mag_solves"] == 4
+    assert o.stats()["div_b"] == pytest.approx(orc.div_b_cell(p, uc), rel=1e-15)
 
 
 @pytest.mark.parametrize("formula", [im.FORMULA_PRESSURE, im.FORMULA_ENERGY])
