@@ -1,7 +1,8 @@
 """Throughput of the IMEX-MHD step on one MI355X.
 
 Workload: Orszag-Tang on periodic [0,2pi]^2 at 1024x1024, gamma 5/3, WENO5,
-StaggerCT (EMF-consistent), PressureFormula (--formula energy: EnergyFormula),
+StaggerCT (EMF-consistent; --magfield: UnStaggerCT, PotentialA or NoDivFree),
+PressureFormula (--formula energy: EnergyFormula),
 floor_on with p_floor 1e-8, IMEX-RK2, CFL 0.8 — synthetic initial data
 generated on the host.
 
@@ -27,24 +28,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None):
+def uses_potential(p):
+    import imex_amd as im
+    return p.magfield in (im.MAGFIELD_UNSTAGGER, im.MAGFIELD_POTENTIAL_A)
+
+
+def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None, magfield=None):
     import imex_amd as im
     p, kw = im.case_of(name)
     p.precision = precision
     p.schedule = schedule
     if formula is not None:
         p.formula = formula
+    if magfield is not None:
+        p.magfield = magfield
     if tols:
         p.mixed_inner_tol, p.mixed_outer_tol = tols
     u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
     s = im.Solver(p)
     s.set_state(u, bx, by)
+    if uses_potential(p):
+        s.set_potential(im.init_potential(p, kw["case"], kw.get("amp", 1.0)))
     tune = s.autotune(im.TUNE_EXPLICIT | im.TUNE_KRYLOV)       # launch shapes for this grid
     log(f"[bench] {name} autotune: {tune}")
     log(f"[bench] {name} warmup {warmup} steps")
     if warmup:
         s.step(warmup)
     state = s.get_state()
+    A_state = s.get_potential() if uses_potential(p) else None
     s.reset_stats()
     s.sync()
     log(f"[bench] timing {steps} steps")
@@ -61,6 +72,8 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None):
     log(f"[bench] per-kernel event timing over the same {steps} steps")
     uc, up, bxs, bys, sc = state
     s.set_state(up, bxs, bys, sc["t"], sc["timestep"], sc["prev_dt"])
+    if A_state is not None:
+        s.set_potential(A_state)
     s.profile_enable(True)
     s.sync()
     t2 = time.perf_counter()
@@ -70,25 +83,32 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None):
     prof = s.profile_read()
     s.profile_enable(False)
     s.close()
-    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final, tune=tune)
+    return dict(p=p, elapsed=t1 - t0, prof_elapsed=t3 - t2, prof=prof, stats=stats, state=state, final=final, tune=tune,
+                potential=A_state)
 
 
-def run_pair(name, steps, warmup, schedule, formula=None):
+def run_pair(name, steps, warmup, schedule, formula=None, magfield=None, forms=("fp64", "mixed")):
     """FP64 and mixed forms of one workload (no per-kernel pass): throughput and the mixed form's error."""
     import numpy as np
     import imex_amd as im
     out = {}
     finals = {}
     for f, prec in (("fp64", im.PREC_FP64), ("mixed", im.PREC_MIXED)):
+        if f not in forms:
+            continue
         p, kw = im.case_of(name)
         p.precision = prec
         p.schedule = schedule
         if formula is not None:
             p.formula = formula
+        if magfield is not None:
+            p.magfield = magfield
         u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
         s = im.Solver(p)
         try:
             s.set_state(u, bx, by)
+            if uses_potential(p):
+                s.set_potential(im.init_potential(p, kw["case"], kw.get("amp", 1.0)))
             s.autotune(im.TUNE_EXPLICIT | im.TUNE_KRYLOV)
             log(f"[bench] {name} {f}: warmup {warmup}, timing {steps} steps")
             if warmup:
@@ -108,6 +128,8 @@ def run_pair(name, steps, warmup, schedule, formula=None):
                                                "mixed_refinements")},
                   "min_p": st["min_p"], "div_b": st["div_b"]}
         log(f"[bench] {name} {f}: {out[f]['value']} Mcell-steps/s")
+    if len(finals) < 2:
+        return out
     a, b = finals["mixed"], finals["fp64"]
     g = im.GC
     errs = [float(np.linalg.norm(a[g:-g, g:-g, k] - b[g:-g, g:-g, k]) / max(np.linalg.norm(b[g:-g, g:-g, k]), 1e-300))
@@ -240,16 +262,20 @@ def run_adjoint(n, steps, every, ndir, seed, eps=1e-6, noise=1e-3):
             "worst_solve_residual": info["worst_solve_residual"]}
 
 
-def cpu_baseline(name, state, budget_steps=1, formula=None):
+def cpu_baseline(name, state, budget_steps=1, formula=None, magfield=None, potential=None):
     """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
     import imex_amd as im
     import oracle as orc
     p, kw = im.case_of(name)
     if formula is not None:
         p.formula = formula
+    if magfield is not None:
+        p.magfield = magfield
     uc, up, bx, by, sc = state
     o = orc.Oracle(p, 1)
     o.set_state(up, bx, by, sc["t"], sc["timestep"], sc["prev_dt"])
+    if potential is not None:
+        o.set_potential(potential)
     t0 = time.perf_counter()
     o.step(budget_steps)
     t1 = time.perf_counter()
@@ -269,6 +295,8 @@ def copy_calibration(mib):
 
 
 KERNEL_OF_CLASS = {   # (class, form) -> kernel symbol in the committed rocprof summary
+    ("potential_a", "mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<3, double, double>",
+    ("potential_a", "mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<3, float, float>",
     ("mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<0, double, double>",
     ("mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<0, float, float>",
     ("p_arnoldi_apply", "fp64"): "k_arnoldi_apply<1, double, double>",
@@ -279,16 +307,21 @@ KERNEL_OF_CLASS = {   # (class, form) -> kernel symbol in the committed rocprof 
     ("explicit_energy", "mixed"): "k_explicit<32, float>",
 }
 TRAFFIC_SUMMARY = {"pressure": os.path.join(ROOT, "profiles", "imex_step_ot1024_kernels.md"),
-                   "energy": os.path.join(ROOT, "profiles", "imex_step_ot1024_energy_kernels.md")}
+                   "energy": os.path.join(ROOT, "profiles", "imex_step_ot1024_energy_kernels.md"),
+                   "potential_a": os.path.join(ROOT, "profiles", "imex_step_ot1024_potential_kernels.md")}
 
 
-def measured_traffic(cls, form, formula="pressure"):
+def kernel_of(cls, form, variant):
+    return KERNEL_OF_CLASS.get((variant, cls, form), KERNEL_OF_CLASS.get((cls, form)))
+
+
+def measured_traffic(cls, form, variant="pressure"):
     """HBM bytes per active dispatch of the kernel behind `cls`, from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (calibrated against a plain copy); None if it is not listed there."""
+    WRITE_SIZE summary of the run variant (calibrated against a plain copy); None if it is not listed there."""
     import re
-    name = KERNEL_OF_CLASS.get((cls, form))
-    path = TRAFFIC_SUMMARY[formula]
-    if not name or not os.path.exists(path):
+    name = kernel_of(cls, form, variant)
+    path = TRAFFIC_SUMMARY.get(variant)
+    if not name or not path or not os.path.exists(path):
         return None
     in_traffic = False
     for line in open(path):
@@ -300,7 +333,7 @@ def measured_traffic(cls, form, formula="pressure"):
     return None
 
 
-def summarize(r, steps, label, formula="pressure"):
+def summarize(r, steps, label, variant="pressure"):
     """Throughput, per-class table (stderr) and the roofline of the dominant kernel class."""
     p = r["p"]
     value = p.nx * p.ny * steps / r["elapsed"] / 1e6
@@ -319,15 +352,15 @@ def summarize(r, steps, label, formula="pressure"):
     log(f"[bench] {label} profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
         f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
     st = r["stats"]
-    traffic = measured_traffic(dom["name"], label, formula)
+    traffic = measured_traffic(dom["name"], label, variant)
     return {
         "value": round(value, 3),
         "ms_per_step": round(r["elapsed"] / steps * 1e3, 3),
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per active dispatch of "
-                                        f"{KERNEL_OF_CLASS.get((dom['name'], label))}, calibrated, "
-                                        f"{os.path.relpath(TRAFFIC_SUMMARY[formula], ROOT)}") if traffic else None,
+                                        f"{kernel_of(dom['name'], label, variant)}, calibrated, "
+                                        f"{os.path.relpath(TRAFFIC_SUMMARY[variant], ROOT)}") if traffic else None,
                      "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "gmres": {k: st[k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters",
                                      "mixed_refinements")},
@@ -362,6 +395,10 @@ def main():
     ap.add_argument("--blast-steps", type=int, default=20)
     ap.add_argument("--formula", default="pressure", choices=["pressure", "energy"],
                     help="implicit subsystem of the headline run (PressureFormula or EnergyFormula)")
+    ap.add_argument("--magfield", default="stagger", choices=["stagger", "unstagger", "potential_a", "nodivfree"],
+                    help="magnetic treatment of the headline run (StaggerCT, UnStaggerCT, PotentialA, NoDivFree)")
+    ap.add_argument("--magfield-steps", type=int, default=20,
+                    help="timed FP64 steps of each other magnetic treatment among the secondary workloads")
     ap.add_argument("--energy-steps", type=int, default=40,
                     help="timed steps of the energy-formula FP64 / mixed pair among the secondary workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -386,13 +423,17 @@ def main():
         print(json.dumps({"batch": res}), flush=True)
         return
     formula = {"pressure": im.FORMULA_PRESSURE, "energy": im.FORMULA_ENERGY}[args.formula]
+    MAGFIELDS = {"stagger": im.MAGFIELD_STAGGER, "unstagger": im.MAGFIELD_UNSTAGGER,
+                 "potential_a": im.MAGFIELD_POTENTIAL_A, "nodivfree": im.MAGFIELD_NODIVFREE}
+    magfield = MAGFIELDS[args.magfield]
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {}
     for f in forms:
         prec = im.PREC_FP64 if f == "fp64" else im.PREC_MIXED
         runs[f] = run_gpu(args.workload, args.steps, args.warmup, prec, sched, args.mixed_tols if f == "mixed" else None,
-                          formula)
-    res = {f: summarize(runs[f], args.steps, f, args.formula) for f in forms}
+                          formula, magfield)
+    variant = args.magfield if args.magfield != "stagger" else args.formula
+    res = {f: summarize(runs[f], args.steps, f, variant) for f in forms}
     head = "mixed" if "mixed" in res else "fp64"
     p = runs[head]["p"]
     out = {
@@ -410,7 +451,7 @@ def main():
         "data": "synthetic (Orszag-Tang initial data generated on the host)",
         "config": {"workload": f"orszag_tang_{p.nx}x{p.ny}_weno5_staggerct_rk2", "nx": p.nx, "ny": p.ny,
                    "recon": "WENO5", "p_floor": p.p_floor, "gamma": p.gamma, "form": head,
-                   "formula": args.formula, "schedule": args.schedule},
+                   "formula": args.formula, "magfield": args.magfield, "schedule": args.schedule},
         "roofline": res[head]["roofline"],
         "gmres": res[head]["gmres"],
         "schedule": res[head]["schedule"],
@@ -436,6 +477,13 @@ def main():
                         "steps": args.batch_steps, "warmup": 1,
                         "ideal": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_IDEAL, 2024, sched),
                         "redlich_kwong": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_RK, 2024, sched)}
+        if args.magfield == "stagger":
+            out["magfield"] = {"workload": "Orszag-Tang 1024x1024 WENO5, FP64, with the other magnetic treatments "
+                                           "(UnStaggerCT and PotentialA evolve the vector potential A_z)",
+                               "steps": args.magfield_steps, "warmup": 3}
+            for name_mf in ("unstagger", "potential_a", "nodivfree"):
+                out["magfield"][name_mf] = run_pair(args.workload, args.magfield_steps, 3, sched, formula,
+                                                    MAGFIELDS[name_mf], forms=("fp64",))["fp64"]
         if args.formula == "pressure":
             out["energy_formula"] = dict(workload="Orszag-Tang 1024x1024 WENO5 with EnergyFormula (Energy_flux_Update + "
                                                   "PreSubsystem_Energy in place of the pressure subsystem)",
@@ -444,9 +492,10 @@ def main():
         out["adjoint"] = run_adjoint(256, 20, 5, args.adjoint_dirs, 1)
     if not args.no_cpu:
         log("[bench] CPU oracle sample")
-        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula)
+        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula, magfield, runs[head]["potential"])
         out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang ({args.formula} formula) "
+                               "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang ({args.formula} formula, "
+                                         f"{args.magfield}) "
                                          f"from the post-warmup state, faithful mode, {secs:.1f} s"}
     print(json.dumps(out), flush=True)
 
