@@ -140,6 +140,46 @@ def run_pair(name, steps, warmup, schedule, formula=None, magfield=None, forms=(
     return out
 
 
+def run_diag_cost(name, steps, warmup, schedule):
+    """Cost of the per-step diagnostics (device ring), one Diagnose_CT_EMF and the output files at this size."""
+    import tempfile
+    import imex_amd as im
+    p, kw = im.case_of(name)
+    p.schedule = schedule
+    u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
+    s = im.Solver(p)
+    try:
+        s.set_state(u, bx, by)
+        s.step(warmup)
+        uc, up, bxs, bys, sc = s.get_state()
+        times = {}
+        for label, cap in (("off", 0), ("on", 64)):     # the same steps from the same state, without and with
+            s.set_state(up, bxs, bys, sc["t"], sc["timestep"], sc["prev_dt"])
+            s.diag_enable(cap)
+            s.step(warmup)                               # budgets relearned after set_state
+            s.sync()
+            t0 = time.perf_counter()
+            s.step_async(steps)
+            s.sync()
+            times[label] = (time.perf_counter() - t0) / steps * 1e3
+        recs = s.diag_read()
+        t0 = time.perf_counter()
+        s.diagnose_ct_emf()
+        t_ct = (time.perf_counter() - t0) * 1e3
+        with tempfile.TemporaryDirectory() as tdir:
+            t0 = time.perf_counter()
+            s.write_vtk(os.path.join(tdir, "plt.vtk"), warmup + 2 * steps)
+            t_vtk = (time.perf_counter() - t0) * 1e3
+            t0 = time.perf_counter()
+            s.write_fields(tdir)
+            t_dat = (time.perf_counter() - t0) * 1e3
+    finally:
+        s.close()
+    return {"ms_per_step_without": round(times["off"], 3), "ms_per_step_with_recording": round(times["on"], 3),
+            "records": len(recs), "last_div_error": recs[-1]["div_error"] if recs else None,
+            "ct_emf_diagnose_ms": round(t_ct, 2), "write_vtk_ms": round(t_vtk, 1), "write_fields_ms": round(t_dat, 1)}
+
+
 BATCH_SIZES = (64, 96, 128, 192, 256, 384, 512)
 
 
@@ -489,6 +529,9 @@ def main():
                                                   "PreSubsystem_Energy in place of the pressure subsystem)",
                                          steps=args.energy_steps,
                                          warmup=3, **run_pair(args.workload, args.energy_steps, 3, sched, im.FORMULA_ENERGY))
+        out["diagnostics"] = dict(workload="Orszag-Tang 1024x1024, FP64: per-step diagnostics recorded on the device, "
+                                           "Diagnose_CT_EMF, VTK and RK_*.dat output", steps=20,
+                                  **run_diag_cost(args.workload, 20, 3, sched))
         out["adjoint"] = run_adjoint(256, 20, 5, args.adjoint_dirs, 1)
     if not args.no_cpu:
         log("[bench] CPU oracle sample")

@@ -81,6 +81,11 @@ class AdjointInfo(C.Structure):
                 ("worst_solve_residual", C.c_double), ("backward_ms", C.c_double)]
 
 
+class Diag(C.Structure):
+    _fields_ = [("timestep", C.c_int)] + [(n, C.c_double) for n in ("t", "dt", "div_error", "B_inf", "dt_comp", "emin",
+                                                                      "emin_x", "emin_y", "mass", "intE", "magE_max")]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("res0", C.c_double), ("res", C.c_double),
                 ("status", C.c_int)]
@@ -195,6 +200,16 @@ def lib():
         "imex_set_stage_potential": (C.c_int, [vp, dp, dp]),
         "imex_get_stage_potential": (C.c_int, [vp, dp, dp]),
         "imex_mag_update_a": (C.c_int, [vp]),
+        "imex_diag_enable": (C.c_int, [vp, C.c_int]),
+        "imex_diag_read": (C.c_int, [vp, C.c_int, C.POINTER(Diag), C.c_int]),
+        "imex_diagnose": (C.c_int, [vp, C.c_int, C.POINTER(Diag)]),
+        "imex_diagnose_ct_emf": (C.c_int, [vp, C.c_int, dp]),
+        "imex_write_vtk_host": (C.c_int, [C.POINTER(Params), dp, dp, C.c_double, C.c_int, C.c_char_p]),
+        "imex_write_vtk": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int]),
+        "imex_write_fields_host": (C.c_int, [C.POINTER(Params), dp, dp, C.c_char_p]),
+        "imex_write_fields": (C.c_int, [vp, C.c_int, C.c_char_p]),
+        "imex_write_history": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), dp]),
+        "imex_output_path": (C.c_int, [C.POINTER(Params), C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
         "imex_unstaggered_ct": (C.c_int, [vp]),
         "imex_ct_update": (C.c_int, [vp]),
         "imex_apply_op": (C.c_int, [vp, C.c_int, dp, dp, dp, C.c_double]),
@@ -244,6 +259,40 @@ def init_case(p, case, amp=1.0, u0=0.0, v0=0.0, smooth=0):
     if rc < 0:
         raise RuntimeError(lib().imex_last_error().decode())
     return u, bx, by
+
+
+def write_vtk_host(p, u_con, u_pri, t, nstep, file):
+    """printAllVariablesToVTK's file from host Grids."""
+    rc = lib().imex_write_vtk_host(C.byref(p), ptr(np.ascontiguousarray(u_con, dtype=np.float64)),
+                                   ptr(np.ascontiguousarray(u_pri, dtype=np.float64)), t, nstep, str(file).encode())
+    if rc < 0:
+        raise ImexError(lib().imex_last_error().decode())
+
+
+def write_fields_host(p, u_con, u_pri, directory):
+    """The end-of-run RK_*.dat field files from host Grids."""
+    rc = lib().imex_write_fields_host(C.byref(p), ptr(np.ascontiguousarray(u_con, dtype=np.float64)),
+                                      ptr(np.ascontiguousarray(u_pri, dtype=np.float64)), str(directory).encode())
+    if rc < 0:
+        raise ImexError(lib().imex_last_error().decode())
+
+
+def write_history(file, steps, values):
+    """A "step value" history file (RK_div_error.dat, RK_B_inf.dat, RK_dt_comp.dat)."""
+    st = np.ascontiguousarray(steps, dtype=np.int32)
+    va = np.ascontiguousarray(values, dtype=np.float64)
+    rc = lib().imex_write_history(str(file).encode(), len(st), st.ctypes.data_as(C.POINTER(C.c_int)), ptr(va))
+    if rc < 0:
+        raise ImexError(lib().imex_last_error().decode())
+
+
+def output_path(p, root, testname):
+    """The reference's output directory of a run with these parameters."""
+    buf = C.create_string_buffer(4096)
+    rc = lib().imex_output_path(C.byref(p), str(root).encode(), str(testname).encode(), buf, 4096)
+    if rc < 0:
+        raise ImexError(lib().imex_last_error().decode())
+    return buf.value.decode()
 
 
 def init_potential(p, case, amp=1.0):
@@ -302,6 +351,30 @@ class Solver:
     def set_state(self, u_pri, bx, by, t=0.0, timestep=0, prev_dt=0.0):
         return self._chk(self.L.imex_set_state(self.h, ptr(np.ascontiguousarray(u_pri)), ptr(np.ascontiguousarray(bx)),
                                                ptr(np.ascontiguousarray(by)), t, timestep, prev_dt), "set_state")
+
+    def diag_enable(self, capacity):
+        return self._chk(self.L.imex_diag_enable(self.h, capacity), "diag_enable")
+
+    def diag_read(self, i=0, max_records=100000):
+        arr = (Diag * max_records)()
+        n = self._chk(self.L.imex_diag_read(self.h, i, arr, max_records), "diag_read")
+        return [struct_dict(arr[q]) for q in range(n)]
+
+    def diagnose(self, i=0):
+        d = Diag()
+        self._chk(self.L.imex_diagnose(self.h, i, C.byref(d)), "diagnose")
+        return struct_dict(d)
+
+    def diagnose_ct_emf(self, i=0):
+        out = np.zeros((4, 4))
+        self._chk(self.L.imex_diagnose_ct_emf(self.h, i, ptr(out)), "diagnose_ct_emf")
+        return out
+
+    def write_vtk(self, file, nstep, i=0):
+        return self._chk(self.L.imex_write_vtk(self.h, i, str(file).encode(), nstep), "write_vtk")
+
+    def write_fields(self, directory, i=0):
+        return self._chk(self.L.imex_write_fields(self.h, i, str(directory).encode()), "write_fields")
 
     def set_potential(self, A):
         return self._chk(self.L.imex_set_potential(self.h, ptr(np.ascontiguousarray(A, dtype=np.float64))), "set_potential")

@@ -40,6 +40,9 @@ struct Inst {
   int* ctl;              // budgeted-schedule failure flags and iteration log
   double* bsave;         // scalar snapshot of the budgeted step
   int* esave;
+  double* diag;          // per-step diagnostics: ring of diag_cap records, then a scratch record area
+  int* diag_n;           // records written so far
+  int diag_cap;
 };
 
 // Buffers of every instance live in device global memory, but a pointer read from the instance

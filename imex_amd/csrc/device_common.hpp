@@ -27,6 +27,7 @@ struct Dims {
   int N;                  // nx*ny
   double dx, dy;
   int perx, pery;         // 1 = periodic, 0 = transmissive
+  double x0, y0;          // domain origin (cell centres at x0 + (i - g + 1/2) dx)
 };
 
 struct Phys {
@@ -60,7 +61,7 @@ __device__ __forceinline__ int kidx(const Dims& d, int i, int j) { return (map_x
 // The reference's Redlich-Kwong temperature solves loop without a cap; here
 // they stop after RK_MAXIT iterations and raise a flag in *eflag.
 #define RK_MAXIT 200
-__device__ inline double rk_T_from_rho_p(const Phys& P, double rho, double p, int* eflag) {
+__host__ __device__ inline double rk_T_from_rho_p(const Phys& P, double rho, double p, int* eflag) {
   const double mR = P.rk_R, ma = P.rk_alfa, mb = P.rk_b;
   double v = 1.0 / rho;
   double T_old, T_new = (1.0 / rho - mb) * (p + ma * rho * rho) / mR;

@@ -229,6 +229,8 @@ struct imex_ctx {
   StepHook hook = nullptr;     // called at HookPoint's while a step is queued host-scheduled
   bool taping = false;         // steps are host-scheduled (the hook sees every point)
   AdjState* adj = nullptr;     // discrete-adjoint checkpoints, tape and work buffers
+  int diag_cap = 0;            // per-step diagnostics ring capacity (0: not recording)
+  bool diag_alloc = false;     // the diagnostics buffers exist (their capacity is fixed at allocation)
   const Dims& d0() const { return hI[0].d; }
 };
 
@@ -241,6 +243,7 @@ static Dims make_dims(const imex_params& p) {
   d.dx = (p.x1 - p.x0) / ((p.nx + 1) - 1);     // main.cpp:226-227
   d.dy = (p.y1 - p.y0) / ((p.ny + 1) - 1);
   d.perx = p.bc_x == IMEX_BC_PERIODIC; d.pery = p.bc_y == IMEX_BC_PERIODIC;
+  d.x0 = p.x0; d.y0 = p.y0;
   return d;
 }
 static Phys make_phys(const imex_params& p) {
@@ -1293,6 +1296,14 @@ static int down_pot(imex_ctx* c, int i, int b, double* A) {
   return 0;
 }
 
+// per-step diagnostics of the current state (iUs after a step, u_pri before the first: the B slots and
+// rho are the same, but e is not, so a state still in u_pri is converted first)
+static void diag_launch(imex_ctx* c, const Launch& L, int append) {
+  const bool face = c->prm.magfield == IMEX_MAGFIELD_STAGGER;
+  if (c->state_mode == 0) launch_pri2con(L, c->P, B_UPRI, c->iUs);
+  launch_diag(L, c->iUs, c->iBxn, c->iByn, face ? 1 : 0, c->prm.cfl, append);
+}
+
 // ================================================================ C ABI
 extern "C" {
 
@@ -1690,6 +1701,7 @@ int imex_step_async(imex_ctx* c, int nsteps) {
     int r = one_step(c);
     if (r < 0) return r;
     rc |= r;
+    if (c->diag_cap > 0) diag_launch(c, c->L, 1);
   }
   return rc;
 }
@@ -2064,3 +2076,4 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
 }  // extern "C"
 
 #include "adjoint_host.inc"
+#include "output_host.inc"

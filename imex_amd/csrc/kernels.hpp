@@ -103,6 +103,15 @@ void launch_e_update(const Launch& L, const Phys& P, int iu);
 // EMF-consistent corner assembly into B_EZPP from the magnetic subsystem's face fluxes (B_FX, B_FY)
 void launch_ct_corners(const Launch& L, int iu);
 
+// ---- per-step diagnostics (kernels_stage.hip) ----
+#define DIAG_REC 12          // doubles per record: timestep, t, dt, div_error, B_inf, dt_comp, emin, x, y, mass, intE, magE_max
+#define DIAG_SCRATCH 32      // doubles after the ring: one record written on demand, then 16 CT-EMF statistics
+// the step's diagnostics from state iu and faces ibx/iby (face div B when face_div, else cell), into
+// ring slot diag_n % diag_cap (append) or the scratch record
+void launch_diag(const Launch& L, int iu, int ibx, int iby, int face_div, double cfl, int append);
+// Diagnose_CT_EMF on state iu: 16 statistics into the scratch area (after its record)
+void launch_ct_emf_diag(const Launch& L, const Phys& P, int iu);
+
 // ---- vector potential (kernels_potential.hip) ----
 void launch_bc_a(const Launch& L, int ia);                       // bound_cond_A on 3 planes
 void launch_a_combine(const Launch& L, int ian, int ia1, int ias);

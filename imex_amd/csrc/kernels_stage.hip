@@ -887,6 +887,208 @@ __global__ void __launch_bounds__(256) k_ct_plain_emf(Phys P, const Inst* __rest
   ezy[pidx(d, ii, jj)] = 0.5 * (gy5(w[1][0]) + gy5(w[1][1]));   // Ez_0p = +ezy
 }
 
+// ================================================================ per-step diagnostics (main.cpp:562-592)
+// Per block over the interior: sum rho, sum e, the div B sum of squares (face or cell metric), max |B|^2/2,
+// max |central div B| and the smallest e with the lowest (i-major) index among equal minima, which
+// is the cell the reference's first-strictly-smaller scan keeps.
+__global__ void k_diag(const Inst* __restrict__ IT, BMap bm, int iu, int ibx, int iby, int face_div) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  const double* __restrict__ bx = gmem(I_.buf[ibx]);
+  const double* __restrict__ by = gmem(I_.buf[iby]);
+  auto U = [&](int i, int j, int k) { return u[k * d.PS + pidx(d, i, j)]; };
+  double mass = 0, intE = 0, dsq = 0, mmax = 0, binf = 0, emin = 1e30, key = 1e300;
+  for (long long t = bx_ * (long long)blockDim.x + threadIdx.x; t < d.N; t += (long long)gx_ * blockDim.x) {
+    int i = (int)(t % d.nx) + GC, j = (int)(t / d.nx) + GC;
+    double rho = U(i, j, 0);
+    double k = 0.5 * (U(i, j, 1) * U(i, j, 1) + U(i, j, 2) * U(i, j, 2) + U(i, j, 3) * U(i, j, 3)) / rho;
+    double m = 0.5 * (U(i, j, 5) * U(i, j, 5) + U(i, j, 6) * U(i, j, 6) + U(i, j, 7) * U(i, j, 7));
+    double e = U(i, j, 4) - k - m;
+    mass += rho; intE += e; mmax = fmax(mmax, m);
+    double kk = (double)(i - GC) * d.ny + (j - GC);
+    if (e < emin || (e == emin && kk < key)) { emin = e; key = kk; }
+    double v;
+    if (face_div) v = (bx[pidx(d, i, j)] - bx[pidx(d, i - 1, j)]) / d.dx + (by[pidx(d, i, j)] - by[pidx(d, i, j - 1)]) / d.dy;
+    else {
+      double a, b;
+      if (i == GC) a = (U(i + 1, j, 5) - U(i, j, 5)) / d.dx;
+      else if (i == d.nx + GC - 1) a = (U(i, j, 5) - U(i - 1, j, 5)) / d.dx;
+      else a = (U(i + 1, j, 5) - U(i - 1, j, 5)) / (2.0 * d.dx);
+      if (j == GC) b = (U(i, j + 1, 6) - U(i, j, 6)) / d.dy;
+      else if (j == d.ny + GC - 1) b = (U(i, j, 6) - U(i, j - 1, 6)) / d.dy;
+      else b = (U(i, j + 1, 6) - U(i, j - 1, 6)) / (2.0 * d.dy);
+      v = a + b;
+    }
+    dsq += d.dx * d.dy * v * v;
+    double c = (U(i + 1, j, 5) - U(i - 1, j, 5)) / (2.0 * d.dx) + (U(i, j + 1, 6) - U(i, j - 1, 6)) / (2.0 * d.dy);
+    binf = fmax(binf, fabs(c));
+  }
+  __shared__ double sh[7][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  mass = wave_sum(mass); intE = wave_sum(intE); dsq = wave_sum(dsq); mmax = wave_max(mmax); binf = wave_max(binf);
+  for (int off = 32; off > 0; off >>= 1) {          // (emin, key) lexicographic minimum over the wave
+    double oe = __shfl_down(emin, off), ok = __shfl_down(key, off);
+    if (oe < emin || (oe == emin && ok < key)) { emin = oe; key = ok; }
+  }
+  if (lane == 0) { sh[0][w] = mass; sh[1][w] = intE; sh[2][w] = dsq; sh[3][w] = mmax; sh[4][w] = binf; sh[5][w] = emin; sh[6][w] = key; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* row = gmem(I_.partial) + bx_ * 8;
+    double r[7] = {sh[0][0], sh[1][0], sh[2][0], sh[3][0], sh[4][0], sh[5][0], sh[6][0]};
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+      r[0] += sh[0][q]; r[1] += sh[1][q]; r[2] += sh[2][q]; r[3] = fmax(r[3], sh[3][q]); r[4] = fmax(r[4], sh[4][q]);
+      if (sh[5][q] < r[5] || (sh[5][q] == r[5] && sh[6][q] < r[6])) { r[5] = sh[5][q]; r[6] = sh[6][q]; }
+    }
+    for (int q = 0; q < 7; q++) row[q] = r[q];
+  }
+}
+__global__ void k_diag_fin(const Inst* __restrict__ IT, BMap bm, BMap src, double cfl, int append) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  if (threadIdx.x != 0) return;
+  const int nblk = bmap_count(src, inst_);
+  const double* P = gmem(I_.partial);
+  double r[7] = {P[0], P[1], P[2], P[3], P[4], P[5], P[6]};
+  for (int b = 1; b < nblk; b++) {
+    const double* row = P + b * 8;
+    r[0] += row[0]; r[1] += row[1]; r[2] += row[2]; r[3] = fmax(r[3], row[3]); r[4] = fmax(r[4], row[4]);
+    if (row[5] < r[5] || (row[5] == r[5] && row[6] < r[6])) { r[5] = row[5]; r[6] = row[6]; }
+  }
+  const double* S = gmem(I_.S);
+  const int key = (int)r[6], im = key / d.ny + GC, jm = key % d.ny + GC;
+  const double dmin = fmin(d.dx, d.dy);
+  const double fully_dt = cfl * 1 / (2 * S[S_CH] / dmin), imex_dt = cfl * 1 / (2 * S[S_AMAX] / dmin);
+  double* rec;
+  if (append) { int n = *gmem(I_.diag_n); rec = gmem(I_.diag) + (long long)(n % I_.diag_cap) * DIAG_REC; *gmem(I_.diag_n) = n + 1; }
+  else rec = gmem(I_.diag) + (long long)I_.diag_cap * DIAG_REC;
+  rec[0] = S[S_TIMESTEP]; rec[1] = S[S_T]; rec[2] = S[S_DT];
+  rec[3] = sqrt(r[2]); rec[4] = r[4]; rec[5] = imex_dt / fully_dt;
+  rec[6] = r[5]; rec[7] = d.x0 + (im - GC + 0.5) * d.dx; rec[8] = d.y0 + (jm - GC + 0.5) * d.dy;
+  rec[9] = r[0] * d.dx * d.dy; rec[10] = r[1] * d.dx * d.dy; rec[11] = r[3];
+}
+
+// ---------------------------------------------------------------- Diagnose_CT_EMF (constrained_transport.cpp:181-222)
+// Face parts of the corner EMF from the reconstruction of the state: central -(f6_L + f6_R)/2 and
+// Rusanov max(|v_n|)/2 (B_t,R - B_t,L) on x faces (B_EZX, B_MXS), likewise on y faces (B_EZY, B_MYS),
+// at every padded index through the ghost maps (the reference's bound_cond_array of the face arrays).
+__global__ void __launch_bounds__(256) k_ctd_faces(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  int i = map_x(d, ii), j = map_y(d, jj);
+  const int ks[5] = {0, 1, 2, 5, 6};
+  double w[2][2][5];   // [dir][0: right state of (i,j), 1: left state of the neighbour][component]
+  for (int dir = 0; dir < 2; dir++)
+    for (int side = 0; side < 2; side++) {
+      int ci = dir == 0 ? map_x(d, i + side) : i, cj = dir == 0 ? j : map_y(d, j + side);
+      for (int q = 0; q < 5; q++) {
+        const double* src = u + ks[q] * d.PS;
+        double a[5];
+        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? src[pidx(d, ci - 2 + o, cj)] : src[pidx(d, ci, cj - 2 + o)];
+        double wl, wr;
+        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl, wr);
+        w[dir][side][q] = side == 0 ? wr : wl;
+      }
+    }
+  // components in w: 0 rho, 1 m_x, 2 m_y, 3 B_x, 4 B_y
+  auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[4] * vx - vy * Q[3]; };
+  auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[3] * vy - vx * Q[4]; };
+  const double *L = w[0][0], *R = w[0][1];
+  long long c = pidx(d, ii, jj);
+  gmem(I_.buf[B_EZX])[c] = -(0.5 * (fx6(L) + fx6(R)));
+  gmem(I_.buf[B_MXS])[c] = 0.5 * fmax(fabs(L[1] / L[0]), fabs(R[1] / R[0])) * (R[4] - L[4]);
+  L = w[1][0]; R = w[1][1];
+  gmem(I_.buf[B_EZY])[c] = 0.5 * (gy5(L) + gy5(R));
+  gmem(I_.buf[B_MYS])[c] = -0.5 * fmax(fabs(L[2] / L[0]), fabs(R[2] / R[0])) * (R[3] - L[3]);
+}
+
+// CT_CornerAssemble (constrained_transport.cpp:147-164) of the three parts: central and Rusanov faces
+// with Ez~ = 0 (B_EZPP, B_CHI), and Ez~ with zero faces (B_VISC); corners [g-2, n+g).
+__global__ void k_ctd_corners(const Inst* __restrict__ IT, BMap bm, int iu) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* __restrict__ u = gmem(I_.buf[iu]);
+  int i = bx_ * blockDim.x + threadIdx.x + GC - 2, j = by_ * blockDim.y + threadIdx.y + GC - 2;
+  if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const int I = map_x(d, i), J = map_y(d, j);
+  const double dx = d.dx, dy = d.dy;
+  auto corner = [&](const double* ex, const double* ey, bool tilde) {
+    auto EX = [&](int a, int b) { return ex ? ex[pidx(d, a, b)] : 0.0; };
+    auto EY = [&](int a, int b) { return ey ? ey[pidx(d, a, b)] : 0.0; };
+    auto ET = [&](int a, int b) { return tilde ? ez_tilde(d, u, a, b) : 0.0; };
+    double bar = 0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J));
+    double e00 = ET(i, j), e10 = ET(i + 1, j), e01 = ET(i, j + 1), e11 = ET(i + 1, j + 1);
+    double t1 = 0.5 * ((EX(i, j) - e00) / (dx / 2.0) + (EX(i, j + 1) - e01) / (dx / 2.0));
+    double t2 = 0.5 * ((e10 - EX(i, j)) / (dx / 2.0) + (e11 - EX(i, j + 1)) / (dx / 2.0));
+    double t3 = 0.5 * ((EY(i, j) - e00) / (dy / 2.0) + (EY(i + 1, j) - e10) / (dy / 2.0));
+    double t4 = 0.5 * ((e01 - EY(i, j)) / (dy / 2.0) + (e11 - EY(i + 1, j)) / (dy / 2.0));
+    return bar + dx / 8.0 * (t1 - t2) + dy / 8.0 * (t3 - t4);
+  };
+  long long c = pidx(d, i, j);
+  gmem(I_.buf[B_EZPP])[c] = corner(gmem(I_.buf[B_EZX]), gmem(I_.buf[B_EZY]), false);
+  gmem(I_.buf[B_CHI])[c] = corner(gmem(I_.buf[B_MXS]), gmem(I_.buf[B_MYS]), false);
+  gmem(I_.buf[B_VISC])[c] = corner(nullptr, nullptr, true);
+}
+
+// CT_EMF_Stats (constrained_transport.cpp:165-178) of the three parts and their sum: per block sums
+// of dx dy Ez^2 and dx dy |dB/dt|^2 and maxima of |Ez|, |dB/dt| (16 values per block row).
+__global__ void k_ctd_stats(const Inst* __restrict__ IT, BMap bm) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const Dims& d = I_.d;
+  const double* E3[3] = {gmem(I_.buf[B_EZPP]), gmem(I_.buf[B_CHI]), gmem(I_.buf[B_VISC])};
+  double acc[16];
+  for (int q = 0; q < 16; q++) acc[q] = 0.0;
+  for (long long t = bx_ * (long long)blockDim.x + threadIdx.x; t < d.N; t += (long long)gx_ * blockDim.x) {
+    int i = (int)(t % d.nx) + GC, j = (int)(t / d.nx) + GC;
+    long long c = pidx(d, i, j), cw = pidx(d, i - 1, j), cs = pidx(d, i, j - 1);
+    for (int q = 0; q < 4; q++) {
+      double e, ew, es;
+      if (q < 3) { e = E3[q][c]; ew = E3[q][cw]; es = E3[q][cs]; }
+      else {
+        e = E3[0][c] + E3[1][c] + E3[2][c];
+        ew = E3[0][cw] + E3[1][cw] + E3[2][cw];
+        es = E3[0][cs] + E3[1][cs] + E3[2][cs];
+      }
+      double dBx = -(e - es) / d.dy, dBy = (e - ew) / d.dx;
+      double m2 = dBx * dBx + dBy * dBy;
+      acc[4 * q] += d.dx * d.dy * e * e; acc[4 * q + 1] = fmax(acc[4 * q + 1], fabs(e));
+      acc[4 * q + 2] += d.dx * d.dy * m2; acc[4 * q + 3] = fmax(acc[4 * q + 3], sqrt(m2));
+    }
+  }
+  __shared__ double sh[16][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int q = 0; q < 16; q++) {
+    double v = (q & 1) ? wave_max(acc[q]) : wave_sum(acc[q]);
+    if (lane == 0) sh[q][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int q = threadIdx.x;
+    double v = sh[q][0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) v = (q & 1) ? fmax(v, sh[q][k]) : v + sh[q][k];
+    gmem(I_.partial)[bx_ * 16 + q] = v;
+  }
+}
+__global__ void k_ctd_fin(const Inst* __restrict__ IT, BMap bm, BMap src) {
+  BATCH_BLOCK(bm);
+  const Inst& I_ = IT[inst_];
+  const int q = threadIdx.x;
+  if (q >= 16) return;
+  const int nblk = bmap_count(src, inst_);
+  const double* P = gmem(I_.partial);
+  double v = P[q];
+  for (int b = 1; b < nblk; b++) v = (q & 1) ? fmax(v, P[b * 16 + q]) : v + P[b * 16 + q];
+  gmem(I_.diag)[(long long)I_.diag_cap * DIAG_REC + DIAG_REC + q] = (q & 1) ? v : sqrt(v);
+}
+
 // ================================================================ state conversion, diagnostics
 __global__ void k_con2pri(Phys P, const Inst* __restrict__ IT, BMap bm, int icon, int ipri) {
   BATCH_BLOCK(bm);
@@ -1029,6 +1231,16 @@ void launch_ct(const Launch& L, const Phys& P, int plain, int iu, int ibxqs, int
   }
   k_ct_corner<<<GRID(L, SH_INT2_64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT2_64x4], iex, iey, -1.0, 1.0, iu);
   k_ct_faces<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x4], ibxqs, ibyqs, ibx, iby, iu);
+}
+void launch_diag(const Launch& L, int iu, int ibx, int iby, int face_div, double cfl, int append) {
+  k_diag<<<GRID(L, SH_DIVB), 256, 0, L.st>>>(L.IT, L.bm[SH_DIVB], iu, ibx, iby, face_div);
+  k_diag_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_DIVB], cfl, append);
+}
+void launch_ct_emf_diag(const Launch& L, const Phys& P, int iu) {
+  k_ctd_faces<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_PAD64x4], iu);
+  k_ctd_corners<<<GRID(L, SH_INT2_64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT2_64x4], iu);
+  k_ctd_stats<<<GRID(L, SH_DIVB), 256, 0, L.st>>>(L.IT, L.bm[SH_DIVB]);
+  k_ctd_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_DIVB]);
 }
 void launch_ct_corners(const Launch& L, int iu) {
   k_ct_corner<<<GRID(L, SH_INT2_64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT2_64x4], B_FX, B_FY, -1.0, 1.0, iu);

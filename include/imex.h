@@ -306,6 +306,43 @@ int imex_adjoint_forward(imex_ctx* ctx, int nsteps, int every);
 int imex_adjoint_backward(imex_ctx* ctx, const double* gu_con, const double* gbx, const double* gby,
                           double* adj_u_pri, double* adj_bx, double* adj_by, imex_adjoint_info* info);
 
+/* ---- output files and per-step diagnostics (src/output.cpp:127-326, src/main.cpp:562-720) ---- */
+/* One step's diagnostics, as the reference prints or records them after each step. */
+typedef struct imex_diag {
+  int timestep;
+  double t, dt;
+  double div_error;               /* cal_divergence_error (main.cpp:562, 577) */
+  double B_inf;                   /* max |central div B| of the cell field over the interior (main.cpp:580-590) */
+  double dt_comp;                 /* imex_dt / fully_dt of the step (timestep.cpp:56-57, main.cpp:592) */
+  double emin, emin_x, emin_y;    /* DIAG line (main.cpp:563-575): smallest internal energy density and its cell centre */
+  double mass, intE, magE_max;    /* sum rho dx dy, sum e dx dy, max |B|^2/2 */
+} imex_diag;
+/* Record one imex_diag per instance after every committed step into a device ring of `capacity`
+ * entries (0 disables); recording adds two small launches per step and no host synchronisation. */
+int imex_diag_enable(imex_ctx* ctx, int capacity);
+/* The recorded diagnostics of instance i, oldest first (at most max); returns the count. */
+int imex_diag_read(imex_ctx* ctx, int i, imex_diag* out, int max);
+/* The diagnostics of instance i's current state, now (dt_comp from the last step's scalars). */
+int imex_diagnose(imex_ctx* ctx, int i, imex_diag* out);
+/* Diagnose_CT_EMF (constrained_transport.cpp:181-222) on instance i's current state: CT_EMF_Stats
+ * (:165-178) of the corner EMF's central, Rusanov, Ez~ and total parts, 4 rows of
+ * {L2 of Ez, max |Ez|, L2 of the face injection dB/dt, its max}. */
+int imex_diagnose_ct_emf(imex_ctx* ctx, int i, double out[4][4]);
+/* printAllVariablesToVTK's file (output.cpp:127-319), with its layout as is (the field loops start at
+ * padded index 0); from host Grids, or from instance i's current state. */
+int imex_write_vtk_host(const imex_params* p, const double* u_con, const double* u_pri, double t, int nstep,
+                        const char* file);
+int imex_write_vtk(imex_ctx* ctx, int i, const char* file, int nstep);
+/* The end-of-run field files RK_rho, RK_vx, RK_vy, RK_E, RK_p, RK_Bx, RK_By, RK_Bz, RK_v_mag (.dat), and
+ * RK_T.dat with the Redlich-Kwong EOS, into directory `dir` (main.cpp:598-720). */
+int imex_write_fields_host(const imex_params* p, const double* u_con, const double* u_pri, const char* dir);
+int imex_write_fields(imex_ctx* ctx, int i, const char* dir);
+/* A per-step history file ("step value" lines: RK_div_error.dat, RK_B_inf.dat, RK_dt_comp.dat). */
+int imex_write_history(const char* file, int n, const int* steps, const double* values);
+/* The reference's output directory for a run (output.cpp:286-288) under `root`; returns its length, or
+ * an error if it does not fit in `cap` bytes with the terminating zero. */
+int imex_output_path(const imex_params* p, const char* root, const char* testname, char* out, int cap);
+
 /* ---- profiling ---- */
 /* Enable (1) or disable (0) per-launch event timing; enabling clears the totals. */
 int imex_profile_enable(imex_ctx* ctx, int on);

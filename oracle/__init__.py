@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from imex_amd.capi import Params, Scalars, StageHost, Stats, SolveInfo, ptr, grid_shape, array_shape, \
+from imex_amd.capi import Params, Scalars, StageHost, Stats, SolveInfo, Diag, ptr, grid_shape, array_shape, \
     struct_dict, stage_struct, new_stage
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -55,6 +55,11 @@ def lib():
         "orc_unstaggered_ct": (C.c_int, [PP, SP, STP, dp, dp]),
         "orc_bc_potential": (C.c_int, [PP, dp]),
         "orc_div_b_cell": (C.c_double, [PP, dp]),
+        "orc_write_vtk": (C.c_int, [PP, dp, dp, C.c_double, C.c_int, C.c_char_p]),
+        "orc_write_fields": (C.c_int, [PP, dp, dp, C.c_char_p]),
+        "orc_output_path": (C.c_int, [PP, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
+        "orc_diagnose": (C.c_int, [PP, SP, dp, dp, dp, C.POINTER(Diag)]),
+        "orc_ct_emf_diag": (C.c_int, [PP, dp, dp]),
         "orc_reconstruct": (C.c_int, [PP, dp, dp, dp, dp, dp]),
         "orc_apply_op": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, dp, C.c_double]),
         "orc_gmres": (C.c_int, [PP, SP, STP, C.c_int, dp, dp, C.c_int, dp, C.c_double, C.POINTER(SolveInfo)]),
@@ -174,6 +179,35 @@ def sub_a(p, sc, st, A, As, name):
         return st, A, rc, struct_dict(stats)
     rc = L.orc_unstaggered_ct(C.byref(p), C.byref(_sc(sc)), C.byref(s), ptr(A), ptr(As))
     return st, A, rc, None
+
+
+def write_vtk(p, u_con, u_pri, t, nstep, file):
+    lib().orc_write_vtk(C.byref(p), ptr(np.ascontiguousarray(u_con)), ptr(np.ascontiguousarray(u_pri)), t, nstep,
+                        str(file).encode())
+
+
+def write_fields(p, u_con, u_pri, directory):
+    lib().orc_write_fields(C.byref(p), ptr(np.ascontiguousarray(u_con)), ptr(np.ascontiguousarray(u_pri)),
+                           str(directory).encode())
+
+
+def output_path(p, root, testname):
+    buf = C.create_string_buffer(4096)
+    lib().orc_output_path(C.byref(p), str(root).encode(), str(testname).encode(), buf, 4096)
+    return buf.value.decode()
+
+
+def diagnose(p, sc, u_con, bx, by):
+    d = Diag()
+    lib().orc_diagnose(C.byref(p), C.byref(_sc(sc)), ptr(np.ascontiguousarray(u_con)), ptr(np.ascontiguousarray(bx)),
+                       ptr(np.ascontiguousarray(by)), C.byref(d))
+    return struct_dict(d)
+
+
+def ct_emf_diag(p, u_con):
+    out = np.zeros((4, 4))
+    lib().orc_ct_emf_diag(C.byref(p), ptr(np.ascontiguousarray(u_con)), ptr(out))
+    return out
 
 
 def bc_potential(p, A):

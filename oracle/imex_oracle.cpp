@@ -28,6 +28,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <sstream>
+#include <string>
 #include <vector>
 #include <algorithm>
 #include <functional>
@@ -1241,6 +1246,153 @@ void unstaggered_ct(Ctx& C, const imex_scalars& S, Stage& W) {
   bc_grid(C, W.u_con);
 }
 
+double div_b(const Ctx& C, const Vec& bx, const Vec& by);
+double div_b_cell(const Ctx& C, const Vec& u);
+
+// ---------------------------------------------------------------- output files and diagnostics
+// printVTKHeader / printVTKVariable / printAllVariablesToVTK (output.cpp:127-319); the field loops run
+// over padded indices 0..n-1 as the reference's do.
+void write_vtk(const Ctx& C, const Vec& u_con, const Vec& u_pri, double t, int nstep, const std::string& file) {
+  std::ofstream o(file.c_str());
+  o << "# vtk DataFile Version 3.0" << std::endl << "vtk output" << std::endl << "ASCII" << std::endl;
+  o << "DATASET RECTILINEAR_GRID" << std::endl << "FIELD FieldData 2" << std::endl << "CYCLE 1 1 int" << std::endl;
+  o << nstep << std::endl << "TIME 1 1 double" << std::endl << t << std::endl;
+  o << "DIMENSIONS " << C.nx + 1 << " " << C.ny + 1 << " 1" << std::endl;
+  o << "X_COORDINATES " << C.nx + 1 << " double" << std::endl;
+  for (int i = GC; i < C.nx + GC; i++) o << C.P.x0 + (((double)i) - GC + 0.5) * C.dx << " ";
+  o << std::endl << "Y_COORDINATES " << C.ny + 1 << " double" << std::endl;
+  for (int j = GC; j < C.ny + GC; j++) o << C.P.y0 + (((double)j) - GC + 0.5) * C.dy << " ";
+  o << std::endl << "Z_COORDINATES 1 double" << std::endl << "0" << std::endl;
+  o << "CELL_DATA " << C.nx * C.ny << std::endl;
+  const char* names[13] = {"rho", "mom_X", "mom_Y", "mom_Z", "U", "B_X", "B_Y", "B_Z", "velX", "velY", "velZ", "p", "B"};
+  o << "FIELD FieldData " << 13 << std::endl;
+  for (int var = 0; var < 13; var++) {
+    o << names[var] << " 1 " << C.nx * C.ny << " double" << std::endl;
+    for (int j = 0; j < C.ny; j++) {
+      for (int i = 0; i < C.nx; i++) {
+        double v;
+        if (var < NV) v = G(u_con, C, i, j, var);
+        else if (var < NV + 4) v = G(u_pri, C, i, j, var - NV + 1);
+        else v = std::sqrt(G(u_con, C, i, j, 5) * G(u_con, C, i, j, 5) + G(u_con, C, i, j, 6) * G(u_con, C, i, j, 6) + G(u_con, C, i, j, 7) * G(u_con, C, i, j, 7));
+        o << std::setw(24) << std::setprecision(std::numeric_limits<double>::digits10 + 1) << v << " ";
+      }
+      o << std::endl;
+    }
+    o << std::endl;
+  }
+  o << "POINT_DATA " << (C.nx + 1) * (C.ny + 1) << std::endl;
+}
+
+// the end-of-run field files (main.cpp:598-720)
+void write_fields(Ctx& C, const Vec& u_con, const Vec& u_pri, const std::string& dir) {
+  auto one = [&](const char* name, const std::function<double(int, int)>& f) {
+    std::ofstream o((dir + "/" + name).c_str());
+    for (int i = GC; i < C.nx + GC; i++)
+      for (int j = GC; j < C.ny + GC; j++)
+        o << C.P.x0 + C.dx * (i - GC + 0.5) << " " << C.P.y0 + C.dy * (j - GC + 0.5) << " " << f(i, j) << std::endl;
+  };
+  one("RK_rho.dat", [&](int i, int j) { return G(u_con, C, i, j, 0); });
+  one("RK_vx.dat", [&](int i, int j) { return G(u_pri, C, i, j, 1); });
+  one("RK_vy.dat", [&](int i, int j) { return G(u_pri, C, i, j, 2); });
+  one("RK_E.dat", [&](int i, int j) { return G(u_con, C, i, j, 4); });
+  one("RK_p.dat", [&](int i, int j) { return G(u_pri, C, i, j, 4); });
+  one("RK_Bx.dat", [&](int i, int j) { return G(u_pri, C, i, j, 5); });
+  one("RK_By.dat", [&](int i, int j) { return G(u_pri, C, i, j, 6); });
+  one("RK_Bz.dat", [&](int i, int j) { return G(u_pri, C, i, j, 7); });
+  one("RK_v_mag.dat", [&](int i, int j) { return std::sqrt(G(u_pri, C, i, j, 1) * G(u_pri, C, i, j, 1) + G(u_pri, C, i, j, 2) * G(u_pri, C, i, j, 2)); });
+  if (C.P.eos == IMEX_EOS_RK)
+    one("RK_T.dat", [&](int i, int j) { return rk_T_from_rho_p(C, G(u_pri, C, i, j, 0), G(u_pri, C, i, j, 4)); });
+}
+
+// output directory (output.cpp:190-288): root/eos/positivity/test/formula/div/update/NXxNY_dt_RK_Order_CFL
+std::string output_path(const imex_params& P, const std::string& root, const std::string& test) {
+  std::string eos = P.eos == IMEX_EOS_IDEAL ? "Ideal Gas" : "Redlich Kwong";
+  std::string pp = P.floor_on ? "PreservingON" : "PreservingOFF";
+  std::string ft = P.formula == IMEX_FORMULA_ENERGY ? "EnergyFormula" : "PressureFormula";
+  std::string ut = P.formula == IMEX_FORMULA_ENERGY ? "" : "definition";
+  const char* dts[4] = {"StaggerCT", "UnStaggerCT", "Potential", "NoDivFree"};
+  std::string dv = (P.nx == 1 || P.ny == 1) ? "" : dts[P.magfield];
+  int order = P.recon == IMEX_RECON_WENO5 ? 4 : 2;
+  std::ostringstream ss;
+  ss << std::fixed << std::setprecision(0) << 100 * P.cfl;
+  return root + "/" + eos + "/" + pp + "/" + test + "/" + ft + "/" + dv + "/" + ut + "/" + std::to_string(P.nx) + "x" +
+         std::to_string(P.ny) + "_" + "IMEXdt" + std::to_string(1) + "_RK" + std::to_string(2) + "_Order" + std::to_string(order) +
+         "_CFL0" + ss.str();
+}
+
+// per-step diagnostics (main.cpp:562-592, timestep.cpp:56-57)
+void diagnose(Ctx& C, const imex_scalars& S, const Vec& u, const Vec& bx, const Vec& by, imex_diag& d) {
+  d.timestep = S.timestep; d.t = S.t; d.dt = S.dt;
+  d.div_error = C.P.magfield == IMEX_MAGFIELD_STAGGER ? div_b(C, bx, by) : div_b_cell(C, u);
+  double emin = 1e30, mass = 0, intE = 0, mmax = 0; int im = -1, jm = -1;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double rho = G(u, C, i, j, 0);
+      double k = 0.5 * (G(u, C, i, j, 1) * G(u, C, i, j, 1) + G(u, C, i, j, 2) * G(u, C, i, j, 2) + G(u, C, i, j, 3) * G(u, C, i, j, 3)) / rho;
+      double m = 0.5 * (G(u, C, i, j, 5) * G(u, C, i, j, 5) + G(u, C, i, j, 6) * G(u, C, i, j, 6) + G(u, C, i, j, 7) * G(u, C, i, j, 7));
+      double e = G(u, C, i, j, 4) - k - m;
+      mass += rho; intE += e; if (m > mmax) mmax = m; if (e < emin) { emin = e; im = i; jm = j; }
+    }
+  d.emin = emin; d.emin_x = C.P.x0 + (im - GC + 0.5) * C.dx; d.emin_y = C.P.y0 + (jm - GC + 0.5) * C.dy;
+  d.mass = mass * C.dx * C.dy; d.intE = intE * C.dx * C.dy; d.magE_max = mmax;
+  double binf = 0;
+  for (int i = GC; i < C.nx + GC; i++)
+    for (int j = GC; j < C.ny + GC; j++) {
+      double dv = (G(u, C, i + 1, j, 5) - G(u, C, i - 1, j, 5)) / (2.0 * C.dx) + (G(u, C, i, j + 1, 6) - G(u, C, i, j - 1, 6)) / (2.0 * C.dy);
+      if (std::fabs(dv) > binf) binf = std::fabs(dv);
+    }
+  d.B_inf = binf;
+  double dmin = std::fmin(C.dx, C.dy);
+  double fully_dt = C.P.cfl * 1 / (2 * S.ch / dmin), imex_dt = C.P.cfl * 1 / (2 * S.a_max / dmin);
+  d.dt_comp = imex_dt / fully_dt;
+}
+
+// Diagnose_CT_EMF (constrained_transport.cpp:181-222) with CT_EMF_Stats (:165-178)
+void ct_emf_diag(Ctx& C, const Vec& u, double out[16]) {
+  const size_t NA = (size_t)C.NX * C.NY;
+  Vec Cp0(NA, 0.0), C0p(NA, 0.0), Rp0(NA, 0.0), R0p(NA, 0.0), Et(NA, 0.0), Zero(NA, 0.0);
+  Vec EppC(NA, 0.0), EppR(NA, 0.0), EppT(NA, 0.0), EppTot(NA, 0.0);
+  Recon w;
+  tvd_reconstruction(C, u, w);
+  auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[6] * vx - vy * Q[5]; };   // f_full_con[6]
+  auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[5] * vy - vx * Q[6]; };   // g_full_con[5]
+  for (int i = 0; i < C.NX - 1; i++)
+    for (int j = 0; j < C.NY - 1; j++) {
+      const double* QL = &w.xR[NV * (i + j * C.NX)];
+      const double* QR = &w.xL[NV * (i + 1 + j * C.NX)];
+      A(Cp0, C, i, j) = -(0.5 * (fx6(QL) + fx6(QR)));
+      A(Rp0, C, i, j) = 0.5 * std::fmax(std::fabs(QL[1] / QL[0]), std::fabs(QR[1] / QR[0])) * (QR[6] - QL[6]);
+      const double* PL = &w.yR[NV * (i + j * C.NX)];
+      const double* PR = &w.yL[NV * (i + (j + 1) * C.NX)];
+      A(C0p, C, i, j) = 0.5 * (gy5(PL) + gy5(PR));
+      A(R0p, C, i, j) = -0.5 * std::fmax(std::fabs(PL[2] / PL[0]), std::fabs(PR[2] / PR[0])) * (PR[5] - PL[5]);
+    }
+  bc_array(C, Cp0); bc_array(C, C0p); bc_array(C, Rp0); bc_array(C, R0p);
+  for (int j = 0; j < C.NY; j++)
+    for (int i = 0; i < C.NX; i++)
+      A(Et, C, i, j) = G(u, C, i, j, 2) / G(u, C, i, j, 0) * G(u, C, i, j, 5) - G(u, C, i, j, 1) / G(u, C, i, j, 0) * G(u, C, i, j, 6);
+  corner_assemble(C, EppC, Cp0, C0p, Zero);
+  corner_assemble(C, EppR, Rp0, R0p, Zero);
+  corner_assemble(C, EppT, Zero, Zero, Et);
+  for (int j = 1; j < C.NY - 2; j++)
+    for (int i = 1; i < C.NX - 2; i++) A(EppTot, C, i, j) = A(EppC, C, i, j) + A(EppR, C, i, j) + A(EppT, C, i, j);
+  const Vec* parts[4] = {&EppC, &EppR, &EppT, &EppTot};
+  for (int q = 0; q < 4; q++) {
+    const Vec& E = *parts[q];
+    double l2E = 0, mxE = 0, l2B = 0, mxB = 0;
+    for (int j = GC; j < C.ny + GC; j++)
+      for (int i = GC; i < C.nx + GC; i++) {
+        double e = A(E, C, i, j);
+        l2E += C.dx * C.dy * e * e; if (std::fabs(e) > mxE) mxE = std::fabs(e);
+        double dBx = -(A(E, C, i, j) - A(E, C, i, j - 1)) / C.dy;
+        double dBy = (A(E, C, i, j) - A(E, C, i - 1, j)) / C.dx;
+        double m2 = dBx * dBx + dBy * dBy;
+        l2B += C.dx * C.dy * m2; if (std::sqrt(m2) > mxB) mxB = std::sqrt(m2);
+      }
+    out[4 * q] = std::sqrt(l2E); out[4 * q + 1] = mxE; out[4 * q + 2] = std::sqrt(l2B); out[4 * q + 3] = mxB;
+  }
+}
+
 // ---------------------------------------------------------------- time step (timestep.cpp:5-67, 89-120, 254-273)
 void cfl(Ctx& C, imex_scalars& S, const Vec& u_pri) {
   double ch_x = 1e-8, ch_y = 1e-8, ch = 1e-8;
@@ -1597,6 +1749,38 @@ int orc_flux(const imex_params* p, int kind, int dir, const double* QL, const do
   imex_scalars S{};
   S.a_max = s_max;
   rusanov(S, p->gamma, dir, kind, QL, QR, out);
+  return 0;
+}
+
+// ---- output files and diagnostics ----
+int orc_write_vtk(const imex_params* p, const double* u_con, const double* u_pri, double t, int nstep, const char* file) {
+  Ctx C; ctx_init(C, p);
+  size_t NG = (size_t)C.NX * C.NY * NV;
+  write_vtk(C, Vec(u_con, u_con + NG), Vec(u_pri, u_pri + NG), t, nstep, file);
+  return 0;
+}
+int orc_write_fields(const imex_params* p, const double* u_con, const double* u_pri, const char* dir) {
+  Ctx C; ctx_init(C, p);
+  size_t NG = (size_t)C.NX * C.NY * NV;
+  write_fields(C, Vec(u_con, u_con + NG), Vec(u_pri, u_pri + NG), dir);
+  return 0;
+}
+int orc_output_path(const imex_params* p, const char* root, const char* test, char* out, int cap) {
+  std::string s = output_path(*p, root, test);
+  if ((int)s.size() + 1 > cap) return -1;
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return (int)s.size();
+}
+int orc_diagnose(const imex_params* p, const imex_scalars* sc, const double* u_con, const double* bx, const double* by, imex_diag* d) {
+  Ctx C; ctx_init(C, p);
+  size_t NG = (size_t)C.NX * C.NY * NV, NA = (size_t)C.NX * C.NY;
+  diagnose(C, *sc, Vec(u_con, u_con + NG), Vec(bx, bx + NA), Vec(by, by + NA), *d);
+  return 0;
+}
+int orc_ct_emf_diag(const imex_params* p, const double* u_con, double* out16) {
+  Ctx C; ctx_init(C, p);
+  size_t NG = (size_t)C.NX * C.NY * NV;
+  ct_emf_diag(C, Vec(u_con, u_con + NG), out16);
   return 0;
 }
 
