@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+# IMEX_RECON_* values of the C ABI
+RECON_NAMES = {"none": 0, "minmod": 1, "vanleer": 2, "eno3": 3, "weno5": 4}
+RECON_LABEL = {0: "NoRecon", 1: "Minmod", 2: "VanLeer", 3: "ENO3", 4: "WENO5"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -33,15 +38,21 @@ def uses_potential(p):
     return p.magfield in (im.MAGFIELD_UNSTAGGER, im.MAGFIELD_POTENTIAL_A)
 
 
-def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None, magfield=None):
-    import imex_amd as im
-    p, kw = im.case_of(name)
-    p.precision = precision
-    p.schedule = schedule
+def apply_over(p, formula=None, magfield=None, over=None):
     if formula is not None:
         p.formula = formula
     if magfield is not None:
         p.magfield = magfield
+    for k, v in (over or {}).items():
+        setattr(p, k, v)
+
+
+def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None, magfield=None, over=None):
+    import imex_amd as im
+    p, kw = im.case_of(name)
+    p.precision = precision
+    p.schedule = schedule
+    apply_over(p, formula, magfield, over)
     if tols:
         p.mixed_inner_tol, p.mixed_outer_tol = tols
     u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
@@ -87,7 +98,7 @@ def run_gpu(name, steps, warmup, precision, schedule, tols=None, formula=None, m
                 potential=A_state)
 
 
-def run_pair(name, steps, warmup, schedule, formula=None, magfield=None, forms=("fp64", "mixed")):
+def run_pair(name, steps, warmup, schedule, formula=None, magfield=None, forms=("fp64", "mixed"), over=None):
     """FP64 and mixed forms of one workload (no per-kernel pass): throughput and the mixed form's error."""
     import numpy as np
     import imex_amd as im
@@ -99,10 +110,7 @@ def run_pair(name, steps, warmup, schedule, formula=None, magfield=None, forms=(
         p, kw = im.case_of(name)
         p.precision = prec
         p.schedule = schedule
-        if formula is not None:
-            p.formula = formula
-        if magfield is not None:
-            p.magfield = magfield
+        apply_over(p, formula, magfield, over)
         u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
         s = im.Solver(p)
         try:
@@ -302,15 +310,12 @@ def run_adjoint(n, steps, every, ndir, seed, eps=1e-6, noise=1e-3):
             "worst_solve_residual": info["worst_solve_residual"]}
 
 
-def cpu_baseline(name, state, budget_steps=1, formula=None, magfield=None, potential=None):
+def cpu_baseline(name, state, budget_steps=1, formula=None, magfield=None, potential=None, over=None):
     """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
     import imex_amd as im
     import oracle as orc
     p, kw = im.case_of(name)
-    if formula is not None:
-        p.formula = formula
-    if magfield is not None:
-        p.magfield = magfield
+    apply_over(p, formula, magfield, over)
     uc, up, bx, by, sc = state
     o = orc.Oracle(p, 1)
     o.set_state(up, bx, by, sc["t"], sc["timestep"], sc["prev_dt"])
@@ -441,6 +446,12 @@ def main():
                     help="timed FP64 steps of each other magnetic treatment among the secondary workloads")
     ap.add_argument("--energy-steps", type=int, default=40,
                     help="timed steps of the energy-formula FP64 / mixed pair among the secondary workloads")
+    ap.add_argument("--recon", default=None, choices=list(RECON_NAMES),
+                    help="reconstruction of the headline workload (default: the workload's own, WENO5)")
+    ap.add_argument("--recon-state", default="cons", choices=["cons", "prim"],
+                    help="reconstruct the conservative (default) or the primitive state")
+    ap.add_argument("--recon-steps", type=int, default=20,
+                    help="timed FP64 steps of each other reconstruction among the secondary workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -466,12 +477,15 @@ def main():
     MAGFIELDS = {"stagger": im.MAGFIELD_STAGGER, "unstagger": im.MAGFIELD_UNSTAGGER,
                  "potential_a": im.MAGFIELD_POTENTIAL_A, "nodivfree": im.MAGFIELD_NODIVFREE}
     magfield = MAGFIELDS[args.magfield]
+    over = {"recon_state": im.RECONSTATE_PRIM if args.recon_state == "prim" else im.RECONSTATE_CONS}
+    if args.recon is not None:
+        over["recon"] = RECON_NAMES[args.recon]
     forms = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {}
     for f in forms:
         prec = im.PREC_FP64 if f == "fp64" else im.PREC_MIXED
         runs[f] = run_gpu(args.workload, args.steps, args.warmup, prec, sched, args.mixed_tols if f == "mixed" else None,
-                          formula, magfield)
+                          formula, magfield, over)
     variant = args.magfield if args.magfield != "stagger" else args.formula
     res = {f: summarize(runs[f], args.steps, f, variant) for f in forms}
     head = "mixed" if "mixed" in res else "fp64"
@@ -490,7 +504,8 @@ def main():
         "dtype": "f64" if head == "fp64" else "f32-operators/f64-krylov",
         "data": "synthetic (Orszag-Tang initial data generated on the host)",
         "config": {"workload": f"orszag_tang_{p.nx}x{p.ny}_weno5_staggerct_rk2", "nx": p.nx, "ny": p.ny,
-                   "recon": "WENO5", "p_floor": p.p_floor, "gamma": p.gamma, "form": head,
+                   "recon": RECON_LABEL[p.recon], "recon_state": args.recon_state, "p_floor": p.p_floor,
+                   "gamma": p.gamma, "form": head,
                    "formula": args.formula, "magfield": args.magfield, "schedule": args.schedule},
         "roofline": res[head]["roofline"],
         "gmres": res[head]["gmres"],
@@ -529,13 +544,23 @@ def main():
                                                   "PreSubsystem_Energy in place of the pressure subsystem)",
                                          steps=args.energy_steps,
                                          warmup=3, **run_pair(args.workload, args.energy_steps, 3, sched, im.FORMULA_ENERGY))
+        if args.recon is None and args.recon_state == "cons":
+            out["reconstructions"] = {"workload": "Orszag-Tang 1024x1024, FP64, with the other reconstructions "
+                                                  "(ENO3, NoRecon) and with PrimState (primitive reconstruction)",
+                                      "steps": args.recon_steps, "warmup": 3}
+            for rn, st in (("eno3", "cons"), ("none", "cons"), ("weno5", "prim"), ("minmod", "prim")):
+                o = {"recon": RECON_NAMES[rn],
+                     "recon_state": im.RECONSTATE_PRIM if st == "prim" else im.RECONSTATE_CONS}
+                out["reconstructions"][f"{rn}_{st}"] = run_pair(args.workload, args.recon_steps, 3, sched, formula, magfield,
+                                                                forms=("fp64",), over=o)["fp64"]
         out["diagnostics"] = dict(workload="Orszag-Tang 1024x1024, FP64: per-step diagnostics recorded on the device, "
                                            "Diagnose_CT_EMF, VTK and RK_*.dat output", steps=20,
                                   **run_diag_cost(args.workload, 20, 3, sched))
         out["adjoint"] = run_adjoint(256, 20, 5, args.adjoint_dirs, 1)
     if not args.no_cpu:
         log("[bench] CPU oracle sample")
-        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula, magfield, runs[head]["potential"])
+        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula, magfield, runs[head]["potential"],
+                               over)
         out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang ({args.formula} formula, "
                                          f"{args.magfield}) "

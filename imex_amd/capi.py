@@ -19,6 +19,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN, OP_ENERGY, OP_POTENTIAL = 0, 1, 2, 3, 4
 FORMULA_ENERGY, FORMULA_PRESSURE = 0, 1
 MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A, MAGFIELD_NODIVFREE = 0, 1, 2, 3
+RECON_NONE, RECON_ENO3 = 0, 3
+RECONSTATE_PRIM, RECONSTATE_CONS = 0, 1
 CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
 SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
 TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
@@ -43,7 +45,7 @@ class Params(C.Structure):
         ("ct_mode", C.c_int), ("precision", C.c_int),
         ("mixed_inner_tol", C.c_double), ("mixed_outer_tol", C.c_double),
         ("schedule", C.c_int), ("budget_margin", C.c_int), ("mixed_pressure", C.c_int),
-        ("formula", C.c_int), ("magfield", C.c_int),
+        ("formula", C.c_int), ("recon_state", C.c_int), ("magfield", C.c_int),
     ]
 
 

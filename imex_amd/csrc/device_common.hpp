@@ -39,6 +39,7 @@ struct Phys {
   double kappa1;
   double cfl, time_stop;
   int recon, central_y;
+  int prim;               // reconstruct primitives (ReconState PrimState)
 };
 
 // ---------------------------------------------------------------- index maps

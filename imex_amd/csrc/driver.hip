@@ -253,6 +253,7 @@ static Phys make_phys(const imex_params& p) {
   P.floor_on = p.floor_on; P.p_floor = p.p_floor; P.theta = p.theta; P.mbig_fac = p.mbig_fac;
   P.kappa1 = p.shock_kappa1; P.cfl = p.cfl; P.time_stop = p.time_stop;
   P.recon = p.recon; P.central_y = p.minmod_central_y;
+  P.prim = p.recon_state == IMEX_RECONSTATE_PRIM;
   return P;
 }
 
@@ -1332,6 +1333,7 @@ int imex_default_params(imex_params* p) {
   p->schedule = IMEX_SCHED_GRAPH;
   p->budget_margin = 1;
   p->formula = IMEX_FORMULA_PRESSURE;                                    // config.hpp:44
+  p->recon_state = IMEX_RECONSTATE_CONS;                                 // config.hpp:61
   return 0;
 }
 
@@ -1435,8 +1437,10 @@ static int check_params(const imex_params* prm) {
   if ((prm->bc_x != IMEX_BC_PERIODIC && prm->bc_x != IMEX_BC_TRANSMISSIVE) ||
       (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE))
     return set_err(IMEX_ERR_UNSUPPORTED, "only periodic and transmissive boundaries are supported");
-  if (prm->recon != IMEX_RECON_MINMOD && prm->recon != IMEX_RECON_VANLEER && prm->recon != IMEX_RECON_WENO5)
-    return set_err(IMEX_ERR_UNSUPPORTED, "reconstruction must be Minmod, VanLeer or WENO5");
+  if (prm->recon < IMEX_RECON_NONE || prm->recon > IMEX_RECON_WENO5)
+    return set_err(IMEX_ERR_ARG, "bad reconstruction");
+  if (prm->recon_state != IMEX_RECONSTATE_PRIM && prm->recon_state != IMEX_RECONSTATE_CONS)
+    return set_err(IMEX_ERR_ARG, "bad recon_state");
   if (prm->precision != IMEX_PREC_FP64 && prm->precision != IMEX_PREC_MIXED)
     return set_err(IMEX_ERR_ARG, "bad precision");
   if (prm->schedule != IMEX_SCHED_GRAPH && prm->schedule != IMEX_SCHED_BUDGET && prm->schedule != IMEX_SCHED_HOST)
@@ -1547,7 +1551,7 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   // Under the energy formula the explicit fluxes stay FP64: the energy-form discretisation has a
   // slowly growing mode (about x1.1 per step on Orszag-Tang) that FP32 flux rounding would seed
   // at 1e-7, while the pressure formula resets E from p each stage and keeps such errors flat.
-  c->mixed_flux = c->mixed && prm->formula == IMEX_FORMULA_PRESSURE;
+  c->mixed_flux = c->mixed && prm->formula == IMEX_FORMULA_PRESSURE && prm->recon_state == IMEX_RECONSTATE_CONS;
   c->needA = prm->magfield == IMEX_MAGFIELD_UNSTAGGER || prm->magfield == IMEX_MAGFIELD_POTENTIAL_A;
   c->cap_tiles_p = op_prec(c, 1) ? 2048 : 1536;
   c->ni = n;

@@ -286,10 +286,43 @@ __global__ void k_shock_eta(const Inst* __restrict__ IT, BMap bm) {
 __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 
+// ENO3 stencil choice (reconstruction.cpp:104-183): 0 = cells i-2..i, 1 = i-1..i+1, 2 = i..i+2, by
+// comparing first and second differences.
+template <class U>
+__device__ __forceinline__ int eno3_stencil(const U* a) {
+  const U um2 = a[0], um1 = a[1], u0 = a[2], up1 = a[3], up2 = a[4];
+  U d1 = fabs(u0 - um1), d2 = fabs(up1 - u0), d3, d4;
+  if (d1 >= d2) {
+    d3 = fabs(fabs(up2 - up1) - fabs(up1 - u0));
+    d4 = fabs(fabs(up1 - u0) - fabs(u0 - um1));
+    return d3 >= d4 ? 1 : 2;
+  }
+  d3 = fabs(fabs(u0 - um1) - fabs(um1 - um2));
+  d4 = fabs(fabs(up1 - u0) - fabs(u0 - um1));
+  return d4 >= d3 ? 0 : 1;
+}
+
+// eno_sel >= 0 gives the ENO3 stencil chosen beforehand (in FP64 by the mixed form's flux kernel: the
+// choice is discontinuous in the data, so choosing on FP32-rounded values would flip it near ties).
 template <class T>
-__device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL, T& wR) {
+__device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL, T& wR, int eno_sel = -1) {
   const T um2 = a[0], um1 = a[1], u0 = a[2], up1 = a[3], up2 = a[4];
-  if (recon == IMEX_RECON_MINMOD) {
+  if (recon == IMEX_RECON_NONE) {
+    wL = u0; wR = u0;
+  } else if (recon == IMEX_RECON_ENO3) {
+    // the chosen third-order stencil's two face values
+    const int s = eno_sel >= 0 ? eno_sel : eno3_stencil(a);
+    if (s == 0) {
+      wL = T(-1.0 / 6.0) * um2 + T(5.0 / 6.0) * um1 + T(1.0 / 3.0) * u0;
+      wR = T(1.0 / 3.0) * um2 - T(7.0 / 6.0) * um1 + T(11.0 / 6.0) * u0;
+    } else if (s == 1) {
+      wL = T(1.0 / 3.0) * um1 + T(5.0 / 6.0) * u0 - T(1.0 / 6.0) * up1;
+      wR = T(-1.0 / 6.0) * um1 + T(5.0 / 6.0) * u0 + T(1.0 / 3.0) * up1;
+    } else {
+      wL = T(11.0 / 6.0) * u0 - T(7.0 / 6.0) * up1 + T(1.0 / 3.0) * up2;
+      wR = T(1.0 / 3.0) * u0 + T(5.0 / 6.0) * up1 - T(1.0 / 6.0) * up2;
+    }
+  } else if (recon == IMEX_RECON_MINMOD) {
     T sR = up1 - u0, sL = u0 - um1, c;
     if (sL * sR <= T(0)) c = T(0);
     else if (fabs(sR) <= fabs(sL)) c = sR;
@@ -409,13 +442,55 @@ template <int EX_TX> struct ExTile {
   static constexpr int NF = NFX > NFY ? NFX : NFY;     // faces per direction, at most
 };
 
+// Conservative face states of one cell from the primitive reconstruction (ReconState PrimState):
+// con -> pri at the five stencil cells q[(o - 2) step], reconstruct each primitive, pri -> con of both
+// face states.  NC = 4 forms rho and the momenta only (rho v needs no EOS); NC = 8 the whole state.
+// Loop bounds are compile-time so the stencil stays in registers.
+template <int NC>
+__device__ __forceinline__ void prim_faces(const Phys& P, int central, const double* __restrict__ q, long long PS,
+                                           long long step, double* Lc, double* Rc, int* ef) {
+  double s[5][NC];
+#pragma unroll
+  for (int o = 0; o < 5; o++) {
+    double c[NVAR];
+#pragma unroll
+    for (int k = 0; k < NC; k++) c[k] = q[k * PS + (o - 2) * step];
+    if constexpr (NC == 4) {
+      s[o][0] = c[0];
+#pragma unroll
+      for (int k = 1; k < 4; k++) s[o][k] = c[k] / c[0];
+    } else {
+      con2pri(P, s[o], c, ef);
+    }
+  }
+  double Lp[NC], Rp[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    double a[5];
+#pragma unroll
+    for (int o = 0; o < 5; o++) a[o] = s[o][k];
+    recon5(P.recon, central, a, Lp[k], Rp[k]);
+  }
+  if constexpr (NC == 4) {
+    Lc[0] = Lp[0]; Rc[0] = Rp[0];
+#pragma unroll
+    for (int k = 1; k < 4; k++) { Lc[k] = Lp[0] * Lp[k]; Rc[k] = Rp[0] * Rp[k]; }
+  } else {
+    pri2con(P, Lc, Lp, ef);
+    pri2con(P, Rc, Rp, ef);
+  }
+}
+
 // One direction of the fused update: reconstruct, form the face fluxes, apply their difference.
 // T is the arithmetic type of the reconstruction and the fluxes (float in the mixed form); the
-// conservative update u = Qs - adt div F is always formed in FP64.
-template <int DIR, int EX_TX, class T>
+// conservative update u = Qs - adt div F is always formed in FP64.  PRIM (FP64 only) reconstructs
+// the primitives of the five stencil cells and converts the two face states back (ReconState
+// PrimState, reconstruction.cpp:11-24, 272-281); kind 0 needs rho and v only, kinds 1 and 2 the
+// whole state, as the face energy rho e(rho, p) + kinetic + |B|^2/2 uses the reconstructed p and B.
+template <int DIR, int EX_TX, class T, bool PRIM>
 __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int kind, const double* __restrict__ Qe,
                                              const double* __restrict__ Qs, double* __restrict__ u, double h, T s_max,
-                                             int i0, int j0, T* wl, T* wr, T* F, double* acc) {
+                                             int i0, int j0, T* wl, T* wr, T* F, double* acc, int* eflag) {
   constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
   constexpr int W = DIR == 0 ? EX_TX + 2 : EX_TX;                  // reconstructed cells per row
   constexpr int NITEMS = DIR == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
@@ -436,15 +511,28 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     int mj = DIR == 0 ? gj : map_y(d, gj);
     const long long base = pidx(d, mi, mj);
     const long long step = DIR == 0 ? 1 : d.NX;
-    for (int k = 0; k < nrec; k++) {
-      const double* q = Qe + k * d.PS + base;
-      T a[5];
+    if constexpr (PRIM) {
+      static_assert(std::is_same<T, double>::value, "primitive reconstruction is FP64 only");
+      int ef = 0;
+      double Lc[NVAR], Rc[NVAR];
+      if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, Lc, Rc, &ef);
+      else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, Lc, Rc, &ef);
+      for (int k = 0; k < 5; k++)
+        if (k < nrec) { wl[k * EX_NW + it] = Lc[k]; wr[k * EX_NW + it] = Rc[k]; }
+      if (ef) atomicOr(eflag, 1);
+    } else {
+      for (int k = 0; k < nrec; k++) {
+        const double* q = Qe + k * d.PS + base;
+        double a64[5];
+        T a[5];
 #pragma unroll
-      for (int o = 0; o < 5; o++) a[o] = (T)q[(o - 2) * step];
-      T L, R;
-      recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R);
-      wl[k * EX_NW + it] = L;
-      wr[k * EX_NW + it] = R;
+        for (int o = 0; o < 5; o++) { a64[o] = q[(o - 2) * step]; a[o] = (T)a64[o]; }
+        const int sel = std::is_same<T, float>::value && P.recon == IMEX_RECON_ENO3 ? eno3_stencil(a64) : -1;
+        T L, R;
+        recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R, sel);
+        wl[k * EX_NW + it] = L;
+        wr[k * EX_NW + it] = R;
+      }
     }
   }
   __syncthreads();
@@ -482,7 +570,7 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
   }
 }
 
-template <int EX_TX, class T>
+template <int EX_TX, class T, bool PRIM>
 __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict__ IT, BMap bm, int kind, int iQe, int iQs, int iu) {
   constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
   BATCH_BLOCK(bm);
@@ -498,9 +586,10 @@ __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict
   const int i0 = GC + bx_ * EX_TX, j0 = GC + by_ * EX_TY;
   const double h = S[S_ADT];
   const T s_max = (T)S[S_AMAX];
+  int* eflag = gmem(I_.eflag);
   double acc[5];
-  explicit_dir<0, EX_TX, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
-  explicit_dir<1, EX_TX, T>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc);
+  explicit_dir<0, EX_TX, T, PRIM>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
+  explicit_dir<1, EX_TX, T, PRIM>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
 }
 
 // ================================================================ magnetic subsystem
@@ -855,6 +944,31 @@ __global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int 
   u[6 * d.PS + c] = 0.5 * (BY(I, J - 1) + BY(I, J));
 }
 
+// Reconstructed face states of rho, m_x, m_y, B_x, B_y (the EMF terms read no other component) on
+// the faces (i+1/2, j) and (i, j+1/2) of the interior cell (i, j): w[dir][0] is the right state of
+// (i, j), w[dir][1] the left state of its neighbour, taken at the neighbour's ghost-map image.  With
+// the primitive reconstruction the stencil holds rho, v_x, v_y, B_x, B_y and the face momenta are
+// rho v, as upri2ucon forms them (reconstruction.cpp:11-24, 272-281).
+__device__ __forceinline__ void face_states_5(const Dims& d, const Phys& P, const double* __restrict__ u, int i, int j,
+                                              double (*w)[2][5]) {
+  const int ks[5] = {0, 1, 2, 5, 6};
+  for (int dir = 0; dir < 2; dir++)
+    for (int side = 0; side < 2; side++) {
+      int ci = dir == 0 ? map_x(d, i + side) : i, cj = dir == 0 ? j : map_y(d, j + side);
+      long long at[5];
+      for (int o = 0; o < 5; o++) at[o] = dir == 0 ? pidx(d, ci - 2 + o, cj) : pidx(d, ci, cj - 2 + o);
+      for (int q = 0; q < 5; q++) {
+        const double* src = u + ks[q] * d.PS;
+        double a[5];
+        for (int o = 0; o < 5; o++) a[o] = (P.prim && (q == 1 || q == 2)) ? src[at[o]] / u[at[o]] : src[at[o]];
+        double wl, wr;
+        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl, wr);
+        w[dir][side][q] = side == 0 ? wr : wl;
+      }
+      if (P.prim) { w[dir][side][1] *= w[dir][side][0]; w[dir][side][2] *= w[dir][side][0]; }
+    }
+}
+
 // Plain Staggered_CT face EMFs (constrained_transport.cpp:73-100): central averages of
 // f_full_con[6] / g_full_con[5] over the reconstructed states of u_con.
 __global__ void __launch_bounds__(256) k_ct_plain_emf(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) {
@@ -868,21 +982,11 @@ __global__ void __launch_bounds__(256) k_ct_plain_emf(Phys P, const Inst* __rest
   int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
   int i = map_x(d, ii), j = map_y(d, jj);
-  double w[2][2][NVAR];   // [dir][0: right state of (i,j), 1: left state of neighbour]
-  for (int dir = 0; dir < 2; dir++)
-    for (int side = 0; side < 2; side++) {
-      int ci = dir == 0 ? map_x(d, i + side) : i, cj = dir == 0 ? j : map_y(d, j + side);
-      for (int k = 0; k < NVAR; k++) {
-        const double* q = u + k * d.PS;
-        double a[5];
-        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? q[pidx(d, ci - 2 + o, cj)] : q[pidx(d, ci, cj - 2 + o)];
-        double wl, wr;
-        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl, wr);
-        w[dir][side][k] = side == 0 ? wr : wl;
-      }
-    }
-  auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[6] * vx - vy * Q[5]; };
-  auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[5] * vy - vx * Q[6]; };
+  double w[2][2][5];
+  face_states_5(d, P, u, i, j, w);
+  // components in w: 0 rho, 1 m_x, 2 m_y, 3 B_x, 4 B_y
+  auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[4] * vx - vy * Q[3]; };
+  auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[3] * vy - vx * Q[4]; };
   ezx[pidx(d, ii, jj)] = 0.5 * (fx6(w[0][0]) + fx6(w[0][1]));   // Ez_p0 = -ezx
   ezy[pidx(d, ii, jj)] = 0.5 * (gy5(w[1][0]) + gy5(w[1][1]));   // Ez_0p = +ezy
 }
@@ -982,20 +1086,8 @@ __global__ void __launch_bounds__(256) k_ctd_faces(Phys P, const Inst* __restric
   int ii = bx_ * blockDim.x + threadIdx.x, jj = by_ * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
   int i = map_x(d, ii), j = map_y(d, jj);
-  const int ks[5] = {0, 1, 2, 5, 6};
-  double w[2][2][5];   // [dir][0: right state of (i,j), 1: left state of the neighbour][component]
-  for (int dir = 0; dir < 2; dir++)
-    for (int side = 0; side < 2; side++) {
-      int ci = dir == 0 ? map_x(d, i + side) : i, cj = dir == 0 ? j : map_y(d, j + side);
-      for (int q = 0; q < 5; q++) {
-        const double* src = u + ks[q] * d.PS;
-        double a[5];
-        for (int o = 0; o < 5; o++) a[o] = dir == 0 ? src[pidx(d, ci - 2 + o, cj)] : src[pidx(d, ci, cj - 2 + o)];
-        double wl, wr;
-        recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl, wr);
-        w[dir][side][q] = side == 0 ? wr : wl;
-      }
-    }
+  double w[2][2][5];
+  face_states_5(d, P, u, i, j, w);
   // components in w: 0 rho, 1 m_x, 2 m_y, 3 B_x, 4 B_y
   auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[4] * vx - vy * Q[3]; };
   auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[3] * vy - vx * Q[4]; };
@@ -1190,8 +1282,9 @@ void launch_shock(const Launch& L, const Phys& P, int iQe) {
 }
 template <int TXE>
 static void launch_explicit_t(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed) {
-  if (mixed) k_explicit<TXE, float><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-  else k_explicit<TXE, double><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  if (P.prim) k_explicit<TXE, double, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  else if (mixed) k_explicit<TXE, float, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  else k_explicit<TXE, double, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
 }
 // tile: the explicit tile width (16, 32 or 64 cells); the SH_EXPL block map must be built for it
 void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile) {

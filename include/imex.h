@@ -30,7 +30,9 @@ extern "C" {
 
 /* ---- enumerations (values follow the reference enums in include/config.hpp:27-41) ---- */
 enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_PERIODIC = 2 };           /* x/y_boundary_condition */
-enum { IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_WENO5 = 4 };  /* Reconstruction */
+enum { IMEX_RECON_NONE = 0, IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_ENO3 = 3,
+       IMEX_RECON_WENO5 = 4 };                                        /* Reconstruction (config.hpp:30) */
+enum { IMEX_RECONSTATE_PRIM = 0, IMEX_RECONSTATE_CONS = 1 };         /* ReconState (config.hpp:41) */
 enum { IMEX_EOS_IDEAL = 0, IMEX_EOS_RK = 1 };                       /* EOSType (config.hpp:118) */
 enum { IMEX_CT_CONSISTENT = 0, IMEX_CT_PLAIN = 1 };                 /* Staggered_CT_Consistent / Staggered_CT */
 enum { IMEX_PREC_FP64 = 0, IMEX_PREC_MIXED = 1 };
@@ -64,7 +66,9 @@ typedef struct imex_params {
   double x0, x1, y0, y1;          /* domain (main.cpp:9-224) */
   double gamma;                   /* gamma_plasma (config.hpp:122) */
   int bc_x, bc_y;                 /* IMEX_BC_* */
-  int recon;                      /* IMEX_RECON_* (config.hpp:60) */
+  int recon;                      /* IMEX_RECON_* (config.hpp:60): NoRecon, Minmod, VanLeer, ENO3 or WENO5
+                                     (reconstruction.cpp:40-266); the mixed form reconstructs in FP32,
+                                     choosing ENO3's stencil in FP64 */
   int minmod_central_y;           /* 1 = the vortex test's central y-slope (reconstruction.cpp:70-72) */
   int eos;                        /* IMEX_EOS_* */
   double rk_alfa, rk_b, rk_R, rk_cv;  /* Redlich-Kwong constants (config.hpp:125) */
@@ -93,6 +97,10 @@ typedef struct imex_params {
                                      which implicit subsystem a step runs after the magnetic subsystem
                                      (main.cpp:470-486); in the mixed form the energy solve follows
                                      mixed_pressure */
+  int recon_state;                /* IMEX_RECONSTATE_* (config.hpp:61; default CONS): reconstruct the conservative
+                                     state, or the primitive state with the face states converted back
+                                     (reconstruction.cpp:11-24, 272-281).  PRIM keeps the explicit fluxes
+                                     in FP64 in the mixed form */
   int magfield;                   /* IMEX_MAGFIELD_* (config.hpp:45; default STAGGER): the magnetic subsystem and
                                      divergence treatment of a stage (main.cpp:424-434, 497-510).  UNSTAGGER and
                                      POTENTIAL_A evolve the vector potential A, which the caller sets with
@@ -285,7 +293,8 @@ int imex_gmres(imex_ctx* ctx, int op, const double* b, double* x, int precond,
 int imex_pressure_newton(imex_ctx* ctx, const double* b, double* p, imex_solve_info* info);
 
 /* ---- discrete adjoint of the step (no counterpart in the reference, which has no derivative) ----
- * Supported: single-instance contexts, periodic grids, ideal gas, Minmod or VanLeer, EMF-consistent
+ * Supported: single-instance contexts, periodic grids, ideal gas, Minmod or VanLeer of the conservative
+ * state (recon_state CONS), EMF-consistent
  * CT (magfield STAGGER), pressure formula, FP64 form (other settings return IMEX_ERR_UNSUPPORTED).  Usage: imex_set_state (the initial
  * state x0), imex_adjoint_forward (runs nsteps, keeping a checkpoint every `every` steps; the
  * context then holds the final state), read the final state and form the cotangent of the

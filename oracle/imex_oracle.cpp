@@ -286,10 +286,14 @@ struct Recon { Vec xL, xR, yL, yR; };
 
 void tvd_reconstruction(Ctx& C, const Vec& ucon, Recon& w) {
   const int nx = C.nx, ny = C.ny;
+  const bool prim = C.P.recon_state == IMEX_RECONSTATE_PRIM;        // reconstate (:11-24, 272-281)
+  Vec upri;
+  if (prim) { upri.assign(ucon.size(), 0.0); grid_con2pri(C, upri, ucon); }
+  const Vec& src = prim ? upri : ucon;
   Vec u(ucon.size(), 0.0);
   for (int j = GC; j < ny + GC; j++)
     for (int i = GC - 1; i < nx + GC + 1; i++)
-      for (int k = 0; k < NV; k++) G(u, C, i, j, k) = G(ucon, C, i, j, k);
+      for (int k = 0; k < NV; k++) G(u, C, i, j, k) = G(src, C, i, j, k);
   bc_grid(C, u);
   w.xL.assign(ucon.size(), 0.0); w.xR.assign(ucon.size(), 0.0);
   w.yL.assign(ucon.size(), 0.0); w.yR.assign(ucon.size(), 0.0);
@@ -300,7 +304,35 @@ void tvd_reconstruction(Ctx& C, const Vec& ucon, Recon& w) {
         double c = 0.0;
         double s_xR = U(i + 1, j) - U(i, j), s_xL = U(i, j) - U(i - 1, j);
         double s_yR = U(i, j + 1) - U(i, j), s_yL = U(i, j) - U(i, j - 1);
-        if (C.P.recon == IMEX_RECON_MINMOD) {                       // :46-75
+        if (C.P.recon == IMEX_RECON_NONE) {                         // :40-46
+          G(w.xL, C, i, j, k) = U(i, j); G(w.xR, C, i, j, k) = U(i, j);
+          G(w.yL, C, i, j, k) = U(i, j); G(w.yR, C, i, j, k) = U(i, j);
+        } else if (C.P.recon == IMEX_RECON_ENO3) {                  // :104-183
+          for (int dir = 0; dir < 2; dir++) {
+            double um2, um1, u0, up1, up2;
+            if (dir == 0) { um2 = U(i - 2, j); um1 = U(i - 1, j); u0 = U(i, j); up1 = U(i + 1, j); up2 = U(i + 2, j); }
+            else { um2 = U(i, j - 2); um1 = U(i, j - 1); u0 = U(i, j); up1 = U(i, j + 1); up2 = U(i, j + 2); }
+            double q0L = -1.0 / 6.0 * um2 + 5.0 / 6.0 * um1 + 1.0 / 3.0 * u0;
+            double q1L = 1.0 / 3.0 * um1 + 5.0 / 6.0 * u0 - 1.0 / 6.0 * up1;
+            double q2L = 11.0 / 6.0 * u0 - 7.0 / 6.0 * up1 + 1.0 / 3.0 * up2;
+            double q0R = 1.0 / 3.0 * um2 - 7.0 / 6.0 * um1 + 11.0 / 6.0 * u0;
+            double q1R = -1.0 / 6.0 * um1 + 5.0 / 6.0 * u0 + 1.0 / 3.0 * up1;
+            double q2R = 1.0 / 3.0 * u0 + 5.0 / 6.0 * up1 - 1.0 / 6.0 * up2;
+            double diff1 = std::fabs(u0 - um1), diff2 = std::fabs(up1 - u0), diff3, diff4;
+            double wL, wR;
+            if (diff1 >= diff2) {                                     // the smoother side is the right one
+              diff3 = std::fabs(std::fabs(up2 - up1) - std::fabs(up1 - u0));
+              diff4 = std::fabs(std::fabs(up1 - u0) - std::fabs(u0 - um1));
+              if (diff3 >= diff4) { wL = q1L; wR = q1R; } else { wL = q2L; wR = q2R; }
+            } else {
+              diff3 = std::fabs(std::fabs(u0 - um1) - std::fabs(um1 - um2));
+              diff4 = std::fabs(std::fabs(up1 - u0) - std::fabs(u0 - um1));
+              if (diff4 >= diff3) { wL = q0L; wR = q0R; } else { wL = q1L; wR = q1R; }
+            }
+            if (dir == 0) { G(w.xL, C, i, j, k) = wL; G(w.xR, C, i, j, k) = wR; }
+            else { G(w.yL, C, i, j, k) = wL; G(w.yR, C, i, j, k) = wR; }
+          }
+        } else if (C.P.recon == IMEX_RECON_MINMOD) {                // :46-75
           if (s_xL * s_xR <= 0) c = 0;
           else if (std::fabs(s_xR) <= std::fabs(s_xL)) c = s_xR;
           else if (std::fabs(s_xL) < std::fabs(s_xR)) c = s_xL;
@@ -350,6 +382,17 @@ void tvd_reconstruction(Ctx& C, const Vec& ucon, Recon& w) {
           }
         }
       }
+  if (prim) {     // upri2ucon of the face states (:272-281); only the cells bound_cond_u keeps are converted
+    Vec* ws[4] = {&w.xL, &w.xR, &w.yL, &w.yR};
+    for (Vec* wv : ws)
+      for (int j = GC; j < ny + GC; j++)
+        for (int i = GC; i < nx + GC; i++) {
+          double pri[NV], con[NV];
+          for (int k = 0; k < NV; k++) pri[k] = G(*wv, C, i, j, k);
+          pri2con(C, con, pri);
+          for (int k = 0; k < NV; k++) G(*wv, C, i, j, k) = con[k];
+        }
+  }
   bc_grid(C, w.xL); bc_grid(C, w.xR); bc_grid(C, w.yL); bc_grid(C, w.yR);
 }
 
@@ -1312,7 +1355,7 @@ std::string output_path(const imex_params& P, const std::string& root, const std
   std::string ut = P.formula == IMEX_FORMULA_ENERGY ? "" : "definition";
   const char* dts[4] = {"StaggerCT", "UnStaggerCT", "Potential", "NoDivFree"};
   std::string dv = (P.nx == 1 || P.ny == 1) ? "" : dts[P.magfield];
-  int order = P.recon == IMEX_RECON_WENO5 ? 4 : 2;
+  int order = P.recon == IMEX_RECON_WENO5 ? 4 : (P.recon == IMEX_RECON_ENO3 ? 3 : (P.recon == IMEX_RECON_NONE ? 1 : 2));
   std::ostringstream ss;
   ss << std::fixed << std::setprecision(0) << 100 * P.cfl;
   return root + "/" + eos + "/" + pp + "/" + test + "/" + ft + "/" + dv + "/" + ut + "/" + std::to_string(P.nx) + "x" +

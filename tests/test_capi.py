@@ -91,12 +91,25 @@ def test_batch_create_rejects_bad_arguments_before_touching_a_device():
     assert L.imex_batch_create(C.byref(p), 2, nx, ny, C.byref(h)) == -1          # a grid below 4 cells
     assert b"too small" in L.imex_last_error()
     q = im.vortex_params(32)
-    q.recon = 3                                                                   # no such reconstruction
+    q.recon = 7                                                                   # no such reconstruction
     nx[1] = 32
-    assert L.imex_batch_create(C.byref(q), 2, nx, ny, C.byref(h)) == -4
+    assert L.imex_batch_create(C.byref(q), 2, nx, ny, C.byref(h)) == -1
+    q = im.vortex_params(32)
+    q.recon_state = 2                                                             # neither PrimState nor ConsState
+    assert L.imex_batch_create(C.byref(q), 2, nx, ny, C.byref(h)) == -1
     assert L.imex_batch_size(None) == 0
 
 
 def test_mixed_pressure_defaults_off():
     p = im.default_params()
     assert p.mixed_pressure == 0 and (p.mixed_inner_tol, p.mixed_outer_tol) == (1e-6, 1e-9)
+
+
+def test_bench_tables_follow_the_abi():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    assert b.RECON_NAMES == {"none": im.RECON_NONE, "minmod": im.RECON_MINMOD, "vanleer": im.RECON_VANLEER,
+                             "eno3": im.RECON_ENO3, "weno5": im.RECON_WENO5}
+    assert set(b.RECON_LABEL) == set(b.RECON_NAMES.values())

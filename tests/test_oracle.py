@@ -12,6 +12,10 @@ term, a wrong sign or index, or a transposed operand fails a check:
   ghost fills / M_P (boundary.cpp, timestep.cpp:166-222, implicit_operators.cpp:5-139)  modular / clamped index brute force
   Minmod / VanLeer (reconstruction.cpp:46-103)   exact on linear data; the vortex central y-slope
   WENO5 (reconstruction.cpp:184-266)        exact face values for cell averages of quadratics; mirror symmetry
+  ENO3 (reconstruction.cpp:104-183)         exact on quadratics; a step reproduced without overshoot; third order
+  NoRecon (reconstruction.cpp:40-46)        face states are the cell state
+  PrimState (reconstruction.cpp:11-24, 272-281)  composition con->pri, reconstruct, pri->con; a contact keeps
+                                            v, p and B exactly uniform at the faces (the conservative form does not)
   explicit fluxes (flux.cpp:78-455)         constant state unchanged; discrete conservation of every conserved slot;
                                             explicit + implicit parts of both formulas = textbook ideal MHD flux;
                                             GLF dissipation -s/2 (QR - QL)
@@ -34,7 +38,8 @@ term, a wrong sign or index, or a transposed operand fails a check:
   CT (ct_flux.cpp:150-191)                  discrete div B of the faces unchanged by any EMF
   div B metric (imex_subsystems.cpp:630-660)  closed form on analytic faces
   CFL (timestep.cpp:5-67, 89-120, 254-273)  brute-force numpy maxima and the dt rule
-  whole step (main.cpp:334-595)             mass conservation, div B preservation, vortex advection; energy
+  whole step (main.cpp:334-595)             mass conservation, div B preservation, vortex advection (every
+                                            reconstruction and ReconState; dissipation ordered NoRecon > Minmod > ENO3); energy
                                             formula: mass, momentum and total energy conservation; UnStaggerCT /
                                             PotentialA: central div B = 0 inside, mass conservation; NoDivFree:
                                             faces untouched
@@ -234,6 +239,89 @@ def test_weno5_textbook_weights_on_a_step():
     w = al / al.sum()
     q = [v[0] / 3 - 7 / 6 * v[1] + 11 / 6 * v[2], -v[1] / 6 + 5 / 6 * v[2] + v[3] / 3, v[2] / 3 + 5 / 6 * v[3] - v[4] / 6]
     assert wxR[5, 8, 0] == pytest.approx(float(np.dot(w, q)), abs=1e-14)
+
+
+def test_eno3_exact_on_quadratics_third_order_and_non_oscillatory():
+    p = ideal(16, 14, recon=im.RECON_ENO3)
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    a, b, c = 0.7, -0.3, 0.05
+    xc = np.arange(NX, dtype=float)
+    u = np.ones((NY, NX, 8))
+    u[..., 0] = (a + b * xc + c * (xc ** 2 + 1 / 12))[None, :]      # exact cell averages, h = 1
+    wxL, wxR, _, _ = orc.reconstruct(p, u)
+    f = lambda x: a + b * x + c * x ** 2
+    for i in range(G, p.nx + G):        # every candidate stencil is exact on a quadratic: pins the coefficients
+        assert wxL[5, i, 0] == pytest.approx(f(xc[i] - 0.5), abs=1e-12)
+        assert wxR[5, i, 0] == pytest.approx(f(xc[i] + 0.5), abs=1e-12)
+    # a step: the chosen stencils never cross the jump, so every face value is its own side's value
+    prof = np.zeros(NX)
+    prof[9:] = 1.0
+    u[..., 0] = prof[None, :]
+    wxL, wxR, _, _ = orc.reconstruct(p, u)
+    assert np.allclose(wxL[5, G:-G, 0], prof[G:-G], rtol=0, atol=1e-15)
+    assert np.allclose(wxR[5, G:-G, 0], prof[G:-G], rtol=0, atol=1e-15)
+    # third order: the face error of sin(k x) cell averages falls by 2^3 when k halves
+    errs = []
+    for k in (0.2, 0.1):
+        u[..., 0] = ((np.cos(k * (xc - 0.5)) - np.cos(k * (xc + 0.5))) / k)[None, :]
+        wxL, wxR, _, _ = orc.reconstruct(p, u)
+        i = np.arange(G + 2, p.nx + G - 2)
+        errs.append(max(np.abs(wxR[5, i, 0] - np.sin(k * (xc[i] + 0.5))).max(),
+                        np.abs(wxL[5, i, 0] - np.sin(k * (xc[i] - 0.5))).max()))
+    assert 6.5 < errs[0] / errs[1] < 9.5, errs
+
+
+def test_no_reconstruction_is_first_order():
+    p = ideal(12, 10, recon=im.RECON_NONE)
+    rng = np.random.default_rng(3)
+    u = orc.bc(p, rng.uniform(0.5, 2.0, (p.ny + 2 * G, p.nx + 2 * G, 8)), 0)
+    for w in orc.reconstruct(p, u):
+        assert np.array_equal(w, u)
+
+
+def _contact(p, rng):
+    """A contact discontinuity with noise in rho only: v, p and B uniform (ghosts filled)."""
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    pri = np.zeros((NY, NX, 8))
+    pri[..., 0] = np.where(np.arange(NX)[None, :] < NX // 2, 1.0, 0.125) + 0.05 * rng.random((NY, NX))
+    pri[..., 1:] = [0.7, -0.4, 0.2, 0.9, 0.3, 0.6, -0.5]
+    return orc.bc(p, orc.pri2con(p, pri), 0), pri
+
+
+@pytest.mark.parametrize("recon", [im.RECON_MINMOD, im.RECON_VANLEER, im.RECON_ENO3, im.RECON_WENO5])
+@pytest.mark.parametrize("eos", [im.EOS_IDEAL, im.EOS_RK])
+def test_prim_state_reconstruction(recon, eos):
+    """ReconState PrimState (reconstruction.cpp:11-24, 272-281): con -> pri, reconstruct, pri -> con.
+
+    Pinned by composition with the conservative path applied to the primitive grid, and by the
+    property primitive reconstruction exists for: across a contact (v, p, B uniform) the face
+    velocity and pressure stay exactly uniform, which the conservative reconstruction does not give.
+    """
+    p = ideal(12, 10, recon=recon)
+    p.eos = eos
+    p.recon_state = im.RECONSTATE_PRIM
+    u, pri = _contact(p, np.random.default_rng(5))
+    got = orc.reconstruct(p, u)
+    q = ideal(12, 10, recon=recon)
+    q.eos = eos
+    assert q.recon_state == im.RECONSTATE_CONS                       # the default
+    want = [orc.pri2con(q, w) for w in orc.reconstruct(q, orc.con2pri(q, u))]
+    inner = (slice(G, -G), slice(G, -G))
+    for g, w in zip(got, want):
+        assert np.allclose(g[inner], w[inner], rtol=1e-14, atol=0)
+    cons = orc.reconstruct(q, u)
+    dev_prim = dev_cons = 0.0
+    for g, c in zip(got, cons):
+        for w_, acc in ((g, "p"), (c, "c")):
+            wp = orc.con2pri(q, w_)[inner]
+            d = max(np.abs(wp[..., 1:8] - pri[inner][..., 1:8]).max(), 0)
+            if acc == "p":
+                dev_prim = max(dev_prim, d)
+            else:
+                dev_cons = max(dev_cons, d)
+    assert dev_prim <= 1e-11 * (1 if eos == im.EOS_IDEAL else 1e3), dev_prim
+    if eos == im.EOS_RK:      # rho e(rho, p) is nonlinear in rho: the conservative faces lose the uniform pressure
+        assert dev_cons > 1e-7 and dev_cons > 1e3 * dev_prim, dev_cons
 
 
 # ---------------------------------------------------------------- explicit fluxes
@@ -750,6 +838,44 @@ def test_run_conservation_and_divb(case):
         # (blast: E is set from the Newton pressure before CT replaces the cell B, and
         # Energy_Modify is off, so p recovered from E can go negative — reference behaviour)
         assert np.all(up[G:-G, G:-G, 4] > 0)
+
+
+RECONS = [(im.RECON_ENO3, im.RECONSTATE_CONS), (im.RECON_NONE, im.RECONSTATE_CONS),
+          (im.RECON_MINMOD, im.RECONSTATE_PRIM), (im.RECON_ENO3, im.RECONSTATE_PRIM),
+          (im.RECON_WENO5, im.RECONSTATE_PRIM)]
+
+
+@pytest.mark.parametrize("recon,state", RECONS)
+@pytest.mark.parametrize("case", ["vortex", "ot"])
+def test_run_reconstructions_conserve_mass_and_div_b(case, recon, state):
+    p, u, bx, by = make_case(case, 32, recon=recon, recon_state=state)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    m0 = orc.pri2con(p, u)[G:-G, G:-G, 0].sum()
+    o.step(3)
+    uc, up, *_ = o.get_state()
+    assert abs(uc[G:-G, G:-G, 0].sum() - m0) <= 1e-13 * m0
+    assert o.stats()["div_b"] <= 1e-12
+    assert np.all(up[G:-G, G:-G, 4] > 0)
+
+
+def test_vortex_order_of_the_reconstructions():
+    # the vortex moves rigidly with the background flow: against the exact shifted profile the
+    # velocity error after a fixed time orders NoRecon > Minmod > ENO3
+    err = {}
+    for recon in (im.RECON_NONE, im.RECON_MINMOD, im.RECON_ENO3):
+        p, u, bx, by = make_case("vortex", 32, recon=recon, minmod_central_y=0)
+        _, kw = im.case_of("vortex")
+        o = orc.Oracle(p)
+        o.set_state(u, bx, by)
+        o.step(8)
+        up, t = o.get_state()[1], o.get_state()[4]["t"]
+        q, _ = im.case_of("vortex")
+        q.nx = q.ny = 32
+        q.x0, q.x1, q.y0, q.y1 = q.x0 - kw["u0"] * t, q.x1 - kw["u0"] * t, q.y0 - kw["v0"] * t, q.y1 - kw["v0"] * t
+        ue = im.init_case(q, kw["case"], kw["amp"], kw["u0"], kw["v0"], kw.get("smooth", 0))[0]
+        err[recon] = np.abs(up[G:-G, G:-G, 1] - ue[G:-G, G:-G, 1]).mean()
+    assert err[im.RECON_NONE] > 2 * err[im.RECON_MINMOD] > 4 * err[im.RECON_ENO3], err
 
 
 @pytest.mark.parametrize("case", ["vortex", "ot"])
