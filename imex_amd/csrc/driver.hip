@@ -761,7 +761,8 @@ static int sub_unstaggered_ct(imex_ctx* c, int ia01) {
 }
 
 static int sub_explicit_press(imex_ctx* c) {
-  LAUNCH(KC_EXPL_PRESS, 16, launch_explicit(c->L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
+  // algorithmic planes: Qe 0..4 (all eight with PrimState), Qs 1..4, the cell B, u 1..4 written
+  LAUNCH(KC_EXPL_PRESS, c->P.prim ? 19 : 16, launch_explicit(c->L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
   const int u = c->cur.u;
   fill_planes(c, {{u, 1}, {u, 2}, {u, 3}, {u, 4}});
   return 0;
@@ -802,7 +803,7 @@ static int sub_pressure(imex_ctx* c) {
 
 // B slots <- cell B~, then Energy_flux_Update (main.cpp:448-474, flux.cpp:165-200)
 static int sub_explicit_energy(imex_ctx* c) {
-  LAUNCH(KC_EXPL_ENERGY, 16, launch_explicit(c->L, c->P, 2, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
+  LAUNCH(KC_EXPL_ENERGY, c->P.prim ? 19 : 16, launch_explicit(c->L, c->P, 2, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
   const int u = c->cur.u;
   fill_planes(c, {{u, 1}, {u, 2}, {u, 3}, {u, 4}});
   return 0;
