@@ -11,7 +11,7 @@ import numpy as np
 GC = 3
 NVAR = 8
 
-BC_TRANSMISSIVE, BC_PERIODIC = 0, 2
+BC_TRANSMISSIVE, BC_PERIODIC, BC_JET_INFLOW = 0, 2, 4
 RECON_MINMOD, RECON_VANLEER, RECON_WENO5 = 1, 2, 4
 EOS_IDEAL, EOS_RK = 0, 1
 CT_CONSISTENT, CT_PLAIN = 0, 1
@@ -21,7 +21,7 @@ FORMULA_ENERGY, FORMULA_PRESSURE = 0, 1
 MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A, MAGFIELD_NODIVFREE = 0, 1, 2, 3
 RECON_NONE, RECON_ENO3 = 0, 3
 RECONSTATE_PRIM, RECONSTATE_CONS = 0, 1
-CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST = 0, 1, 2
+CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, CASE_JET = 0, 1, 2, 3
 SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
 TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
 WARN_GMRES_MAXOUTER, WARN_NEWTON_MAXITER, WARN_POSITIVITY, WARN_EOS_MAXITER = 1, 2, 4, 8
@@ -46,6 +46,8 @@ class Params(C.Structure):
         ("mixed_inner_tol", C.c_double), ("mixed_outer_tol", C.c_double),
         ("schedule", C.c_int), ("budget_margin", C.c_int), ("mixed_pressure", C.c_int),
         ("formula", C.c_int), ("recon_state", C.c_int), ("magfield", C.c_int),
+        ("jet_rho", C.c_double), ("jet_v", C.c_double), ("jet_B0", C.c_double), ("jet_p", C.c_double),
+        ("jet_halfwidth", C.c_double),
     ]
 
 
@@ -141,6 +143,21 @@ def blast_params(n=64):
     return p
 
 
+def jet_params(n=64, m=None):
+    """MHD jet (testMHDJet, main.cpp:214-223) on [-0.5,0.5]x[0,1.5]: outflow on the sides and the top, the jet
+    inflow strip |x| < 0.05 on the bottom (Mach 800 ... 10000 family; here the defaults of config.hpp:257-260 with
+    jet_rho = gamma), gamma 1.4, Minmod."""
+    p = default_params()
+    p.nx, p.ny = n, (m or (3 * n) // 2)
+    p.x0, p.x1, p.y0, p.y1 = -0.5, 0.5, 0.0, 1.5
+    p.gamma = 1.4
+    p.bc_x, p.bc_y = BC_TRANSMISSIVE, BC_JET_INFLOW
+    p.recon, p.minmod_central_y = RECON_MINMOD, 0
+    p.jet_rho, p.jet_p = p.gamma, 1.0
+    p.time_stop = 0.00015
+    return p
+
+
 def case_of(name):
     """(params, init kwargs) of a named workload."""
     if name.startswith("vortex"):
@@ -152,6 +169,9 @@ def case_of(name):
     if name.startswith("blast"):
         n = int(name[5:] or 64)
         return blast_params(n), dict(case=CASE_BLAST, amp=100.0)
+    if name.startswith("jet"):
+        n = int(name[3:] or 64)
+        return jet_params(n), dict(case=CASE_JET)
     raise ValueError(name)
 
 

@@ -28,7 +28,28 @@ struct Dims {
   double dx, dy;
   int perx, pery;         // 1 = periodic, 0 = transmissive
   double x0, y0;          // domain origin (cell centres at x0 + (i - g + 1/2) dx)
+  int jet;                // 1 = bottom boundary is the MHD jet inflow (y_JetInflow)
+  double jet_hw, jet_rho, jet_v, jet_B0, jet_p, jet_E;   // strip half-width and inflow state; jet_E = jet_E_val()
 };
+
+// The jet inflow strip of y_JetInflow (config.hpp:261-270): cell centres |x_c| < half-width; x faces
+// i+1/2 with |x_f| < half-width - 1e-12 (the staggered Bx rule).
+__host__ __device__ inline bool jet_inlet(const Dims& d, int i) {
+  return d.jet && fabs(d.x0 + (i - GC + 0.5) * d.dx) < d.jet_hw;
+}
+__host__ __device__ inline bool jet_inlet_face(const Dims& d, int i) {
+  return d.jet && fabs(d.x0 + (i - GC + 1.0) * d.dx) < d.jet_hw - 1e-12;
+}
+// Conservative (prim = 0) or primitive (prim = 1) inflow state, slot k
+__host__ __device__ inline double jet_state(const Dims& d, int prim, int k) {
+  switch (k) {
+    case 0: return d.jet_rho;
+    case 2: return prim ? d.jet_v : d.jet_rho * d.jet_v;
+    case 4: return prim ? d.jet_p : d.jet_E;
+    case 6: return d.jet_B0;
+    default: return 0.0;
+  }
+}
 
 struct Phys {
   double gamma;

@@ -244,6 +244,9 @@ static Dims make_dims(const imex_params& p) {
   d.dy = (p.y1 - p.y0) / ((p.ny + 1) - 1);
   d.perx = p.bc_x == IMEX_BC_PERIODIC; d.pery = p.bc_y == IMEX_BC_PERIODIC;
   d.x0 = p.x0; d.y0 = p.y0;
+  d.jet = p.bc_y == IMEX_BC_JET_INFLOW;
+  d.jet_hw = p.jet_halfwidth; d.jet_rho = p.jet_rho; d.jet_v = p.jet_v; d.jet_B0 = p.jet_B0; d.jet_p = p.jet_p;
+  d.jet_E = p.jet_p / (p.gamma - 1.0) + 0.5 * p.jet_rho * p.jet_v * p.jet_v + 0.5 * p.jet_B0 * p.jet_B0;   // config.hpp:271-273
   return d;
 }
 static Phys make_phys(const imex_params& p) {
@@ -404,11 +407,15 @@ static inline double p_apply_fields(const PIds& po, double vb, int j) {
   return po.energy ? 1 + po.precond + vb * (3 + j) : 5 + po.jacobian + vb * (1 + j);
 }
 
-static void fill_planes(imex_ctx* c, std::initializer_list<std::pair<int, int>> planes, int kind = 0) {
+// the enthalpy-ratio plane: the reference forms a neighbour's ratio from the neighbour's own state, which in
+// the jet strip's ghost row is the inflow state (the producing kernels write that cell); elsewhere a copy
+static const signed char hhat_jv[1] = {-2};
+static void fill_planes(imex_ctx* c, std::initializer_list<std::pair<int, int>> planes, int kind = 0,
+                        const signed char* jv = nullptr) {
   int b[8], k[8], n = 0;
   for (auto& pk : planes) { b[n] = pk.first; k[n] = pk.second; n++; }
   int e0 = prof_begin(c);
-  launch_bc_fill(c->L, n, b, k, kind);
+  launch_bc_fill(c->L, n, b, k, kind, jv);
   prof_end(c, e0, KC_BC, 0.0);
 }
 
@@ -726,7 +733,7 @@ static int sub_mag_update_b(imex_ctx* c, bool emf = true) {
   c->sub_idx = 0;
   ensure_qe(c);
   LAUNCH(KC_MAG_RHS, 23, launch_mag_rhs(c->L, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, B_BK, B_XK));
-  fill_planes(c, {{B_HHAT, 0}});
+  fill_planes(c, {{B_HHAT, 0}}, 0, hhat_jv);
   c->hhat_ready = true;
   int rc = run_gmres(c, 0, mag_ids(c), p_ids(c, 0, 0), B_XK, B_BK, 0, nullptr, &c->stats.mag_gmres_iters, &c->stats.mag_gmres_restarts);
   if (!c->budget) c->stats.mag_solves++;
@@ -774,7 +781,7 @@ static void ensure_hhat(imex_ctx* c) {
   // pressure subsystem is called on its own (its b and x0 go to scratch)
   ensure_qe(c);
   launch_mag_rhs(c->L, c->P, c->cur.u, c->cur.Qe, c->cur.Qs, B_TMPN, B_W);
-  fill_planes(c, {{B_HHAT, 0}});
+  fill_planes(c, {{B_HHAT, 0}}, 0, hhat_jv);
   c->hhat_ready = true;
 }
 
@@ -784,7 +791,8 @@ static int sub_pressure(imex_ctx* c) {
   const int u = c->cur.u;
   if (c->P.eos != IMEX_EOS_IDEAL) launch_visc_rk(c->L, c->P, u);
   LAUNCH(KC_P_RHS, 15, launch_p_rhs(c->L, 0, u));
-  fill_planes(c, {{B_MXS, 0}, {B_MYS, 0}});
+  static const signed char mom_jv[2] = {1, 2};       // rhov*_star in the jet strip: (0, rho v) (imex_subsystems.cpp:107-118)
+  fill_planes(c, {{B_MXS, 0}, {B_MYS, 0}}, 0, mom_jv);
   c->sub_idx = 1;
   int rc = run_newton(c, B_PB, B_P, nullptr);
   if (c->hook) c->hook(c, HK_P0);
@@ -815,7 +823,7 @@ static int sub_explicit_energy(imex_ctx* c) {
 static int prep_energy(imex_ctx* c) {
   ensure_qe(c);
   LAUNCH(KC_E_RHS, 14, launch_e_rhs(c->L, c->P, c->cur.u, c->cur.Qe));
-  fill_planes(c, {{B_HHAT, 0}});
+  fill_planes(c, {{B_HHAT, 0}}, 0, hhat_jv);
   c->hhat_ready = false;
   return 0;
 }
@@ -1224,8 +1232,10 @@ static int hmapy(const imex_params& p, int j) {
   if (j > p.ny + GC - 1) return p.bc_y == IMEX_BC_PERIODIC ? j - p.ny : p.ny + GC - 1;
   return j;
 }
-// Ghost fill with the reference's pass order (x over all rows, then y over all columns).
-static void hfill(const imex_params& p, double* a, int nvar, int kind) {
+// Ghost fill with the reference's pass order (x over all rows, then y over all columns), then the jet
+// inflow strip: the inflow state in a Grid's bottom ghost rows (prim: the Grid holds primitives, as under
+// g_bc_prim), the staggered overrides of bound_cond_staggered (constrained_transport.cpp:50-63).
+static void hfill(const imex_params& p, double* a, int nvar, int kind, int prim = 0) {
   const int NX = p.nx + 2 * GC, NY = p.ny + 2 * GC;
   auto mx = [&](int i) {
     if (p.bc_x == IMEX_BC_PERIODIC) return hmapx(p, i);
@@ -1247,6 +1257,17 @@ static void hfill(const imex_params& p, double* a, int nvar, int kind) {
       int sj = my(j);
       if (sj != j) for (int k = 0; k < nvar; k++) a[(size_t)nvar * (i + j * NX) + k] = a[(size_t)nvar * (i + sj * NX) + k];
     }
+  if (p.bc_y != IMEX_BC_JET_INFLOW) return;
+  const Dims d = make_dims(p);
+  for (int i = 0; i < NX; i++) {
+    if (kind == 0 && nvar == NVAR && jet_inlet(d, i))
+      for (int j = 0; j < GC; j++)
+        for (int k = 0; k < NVAR; k++) a[(size_t)NVAR * (i + j * NX) + k] = jet_state(d, prim, k);
+    if (kind == 1 && i < NX - 1 && jet_inlet_face(d, i))
+      for (int j = 0; j < GC; j++) a[i + j * NX] = 0.0;
+    if (kind == 2 && jet_inlet(d, i))
+      for (int j = 0; j < GC - 1; j++) a[i + j * NX] = d.jet_B0;
+  }
 }
 
 // bound_cond_A (boundary.cpp:177-292) on a host Grid (slots 0..2): x over every row, then y over every
@@ -1278,6 +1299,13 @@ static void hbc_A(const imex_params& p, double* A) {
         for (int g = 0; g < GC; g++) a(i, jR + 1 + g, k) = a(i, jR, k) + (g + 1) * sR;
       }
     }
+  if (p.bc_y == IMEX_BC_JET_INFLOW) {                // even reflection in the inflow strip (boundary.cpp:279-291)
+    const Dims d = make_dims(p);
+    for (int i = 0; i < NX; i++)
+      if (jet_inlet(d, i))
+        for (int k = 0; k < 3; k++)
+          for (int g = 0; g < GC; g++) a(i, GC - 1 - g, k) = a(i, GC + g, k);
+  }
 }
 
 // 3 planes of a vector-potential buffer <-> slots 0..2 of a host Grid
@@ -1335,6 +1363,8 @@ int imex_default_params(imex_params* p) {
   p->budget_margin = 1;
   p->formula = IMEX_FORMULA_PRESSURE;                                    // config.hpp:44
   p->recon_state = IMEX_RECONSTATE_CONS;                                 // config.hpp:61
+  p->jet_rho = 1.4; p->jet_v = 10000.0; p->jet_B0 = sqrt(20000.0); p->jet_p = 1.0;   // config.hpp:257-260
+  p->jet_halfwidth = 0.05;                                               // config.hpp:261-269
   return 0;
 }
 
@@ -1378,11 +1408,16 @@ int imex_init_case(const imex_params* p, int case_id, double amp, double u0, dou
         else U(i, j, 4) = rad < rD ? prsL : prsR;
         double b0 = amp / sqrt(4.0 * M_PI) / sqrt(2.0);
         U(i, j, 5) = b0; U(i, j, 6) = b0; U(i, j, 7) = 0.0;
+      } else if (case_id == IMEX_CASE_JET) {           // testMHDJet ambient (initial_conditions.cpp:671-682)
+        U(i, j, 0) = 0.1 * g;
+        U(i, j, 1) = 0.0; U(i, j, 2) = 0.0; U(i, j, 3) = 0.0;
+        U(i, j, 4) = 1.0;
+        U(i, j, 5) = 0.0; U(i, j, 6) = p->jet_B0; U(i, j, 7) = 0.0;
       } else {
         return set_err(IMEX_ERR_ARG, "unknown case");
       }
     }
-  hfill(*p, u, NVAR, 0);
+  hfill(*p, u, NVAR, 0, 1);                          // primitives (initial_conditions.cpp:686-689)
   auto BX = [&](int i, int j) -> double& { return bx[i + j * NX]; };
   auto BY = [&](int i, int j) -> double& { return by[i + j * NX]; };
   if (case_id == IMEX_CASE_VORTEX) {                 // faces from the corner potential (:698-717)
@@ -1398,7 +1433,7 @@ int imex_init_case(const imex_params* p, int case_id, double amp, double u0, dou
         U(i, j, 5) = 0.5 * (BX(i - 1, j) + BX(i, j));
         U(i, j, 6) = 0.5 * (BY(i, j - 1) + BY(i, j));
       }
-    hfill(*p, u, NVAR, 0);
+    hfill(*p, u, NVAR, 0, 1);
   } else {                                          // faces = cell averages (:718-725)
     for (int i = 0; i < NX - 1; i++)
       for (int j = 0; j < NY - 1; j++) {
@@ -1425,6 +1460,7 @@ int imex_init_potential(const imex_params* p, int case_id, double amp, double* A
       if (case_id == IMEX_CASE_VORTEX) az = (amp / (2.0 * M_PI)) * exp(0.5 * (1.0 - (x * x + y * y)));   // :387-391
       else if (case_id == IMEX_CASE_ORSZAG_TANG) az = cos(y) + cos(2.0 * x) / 2.0;   // :341-342 on [0,2pi]^2
       else if (case_id == IMEX_CASE_BLAST) { double b0 = amp / sqrt(4.0 * M_PI) / sqrt(2.0); az = b0 * y - b0 * x; }
+      else if (case_id == IMEX_CASE_JET) az = -p->jet_B0 * x;                                 // :240-243
       else return set_err(IMEX_ERR_ARG, "unknown case");
       A[(size_t)NVAR * (i + (size_t)j * NX) + 2] = az;
     }
@@ -1436,8 +1472,10 @@ int imex_init_potential(const imex_params* p, int case_id, double amp, double* A
 static int check_params(const imex_params* prm) {
   if (prm->gmres_restart > MAXM) return set_err(IMEX_ERR_ARG, "gmres_restart exceeds 32");
   if ((prm->bc_x != IMEX_BC_PERIODIC && prm->bc_x != IMEX_BC_TRANSMISSIVE) ||
-      (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE))
-    return set_err(IMEX_ERR_UNSUPPORTED, "only periodic and transmissive boundaries are supported");
+      (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE && prm->bc_y != IMEX_BC_JET_INFLOW))
+    return set_err(IMEX_ERR_UNSUPPORTED, "boundaries: periodic or transmissive, and the jet inflow on y");
+  if (prm->bc_y == IMEX_BC_JET_INFLOW && !(prm->jet_rho > 0 && prm->jet_p > 0 && prm->jet_halfwidth > 0))
+    return set_err(IMEX_ERR_ARG, "jet inflow state needs jet_rho, jet_p, jet_halfwidth > 0");
   if (prm->recon < IMEX_RECON_NONE || prm->recon > IMEX_RECON_WENO5)
     return set_err(IMEX_ERR_ARG, "bad reconstruction");
   if (prm->recon_state != IMEX_RECONSTATE_PRIM && prm->recon_state != IMEX_RECONSTATE_CONS)
@@ -1625,7 +1663,7 @@ int imex_batch_set_state(imex_ctx* c, int i, const double* u_pri, const double* 
   HIPCHK(hipStreamSynchronize(c->st));
   const Dims& d = c->hI[i].d;
   std::vector<double> tmp(u_pri, u_pri + (size_t)NVAR * d.NX * d.NY);
-  hfill(c->iprm[i], tmp.data(), NVAR, 0);       // bound_cond_u(u_pri) at the top of the step (main.cpp:336)
+  hfill(c->iprm[i], tmp.data(), NVAR, 0, 1);    // bound_cond_u(u_pri) at the top of the step (main.cpp:336)
   int rc = up_grid(c, i, B_UPRI, tmp.data());
   if (!rc) rc = up_plane(c, i, c->iBxn, bx);
   if (!rc) rc = up_plane(c, i, c->iByn, by);

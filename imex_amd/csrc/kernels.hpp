@@ -84,7 +84,7 @@ struct Launch {
 
 // ---- stage kernels (kernels_stage.hip) ----
 // ghost-frame fill of n planes: plane k[p] of buffer b[p]; kind 0 cell, 1 x-normal faces, 2 y-normal faces
-void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind);
+void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind, const signed char* jv = nullptr);
 void launch_step_start(const Launch& L, const Phys& P, int mode, int iin);
 void launch_stage_combine(const Launch& L, int ibxn, int ibyn);
 void launch_qe_prep(const Launch& L, const Phys& P, int iQe);

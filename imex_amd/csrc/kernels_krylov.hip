@@ -235,10 +235,13 @@ __device__ __forceinline__ void p_apply(const POp& op, const T* xs, int lx, int 
   auto E = [&](int di, int dj) { return (T)op.eta[pidx(d, i + di, j + dj)]; };
   T h00 = H(0, 0), hp0 = H(1, 0), hm0 = H(-1, 0), h0p = H(0, 1), h0m = H(0, -1);
   T pii = X(0, 0), xp0 = X(1, 0), xm0 = X(-1, 0), x0p = X(0, 1), x0m = X(0, -1);
+  // jet inflow strip: the ghost below the first row is a homogeneous Dirichlet unknown in the enthalpy
+  // term only; its known value is lifted into the right-hand side (implicit_operators.cpp:559-561)
+  const T p_0m = (j == GC && jet_inlet(d, i)) ? T(0) : x0m;
   double rho = op.u[pidx(d, i, j)];
   T Vdiag = (T)(op.jacobian ? p_vslope(op, cidx, rho) : (1.0 / (op.P.gamma - 1.0)));
   T ax = Vdiag * pii - h * h * (T(1) / (dx * dx) * (T(0.5) * (hp0 + h00) * (xp0 - pii) - T(0.5) * (h00 + hm0) * (pii - xm0))
-                              + T(1) / (dy * dy) * (T(0.5) * (h0p + h00) * (x0p - pii) - T(0.5) * (h0m + h00) * (pii - x0m)));
+                              + T(1) / (dy * dy) * (T(0.5) * (h0p + h00) * (x0p - pii) - T(0.5) * (h0m + h00) * (pii - p_0m)));
   T e00 = E(0, 0);
   T nu_x = (T)op.S[S_VXMAX] * dx, nu_y = (T)op.S[S_VYMAX] * dy;
   T lap = nu_x / (dx * dx) * (T(0.5) * (E(1, 0) + e00) * (xp0 - pii) - T(0.5) * (e00 + E(-1, 0)) * (pii - xm0))
@@ -268,8 +271,9 @@ __device__ __forceinline__ void e_apply(const POp& op, const T* xs, int lx, int 
   T hhp_x = T(0.5) * (h00 + H(1, 0)), hhm_x = T(0.5) * (h00 + H(-1, 0));
   T hhp_y = T(0.5) * (h00 + H(0, 1)), hhm_y = T(0.5) * (h00 + H(0, -1));
   T Ei = X(0, 0);
+  const T E_0m = (j == GC && jet_inlet(d, i)) ? T(0) : X(0, -1);   // getE: the jet strip's ghost (:772-779)
   T ax = Ei - gm1 * h * h * (T(1) / (dx * dx) * (hhp_x * (X(1, 0) - Ei) - hhm_x * (Ei - X(-1, 0)))
-                           + T(1) / (dy * dy) * (hhp_y * (X(0, 1) - Ei) - hhm_y * (Ei - X(0, -1))));
+                           + T(1) / (dy * dy) * (hhp_y * (X(0, 1) - Ei) - hhm_y * (Ei - E_0m)));
   out[0] = ax * dx * dy;
 }
 template <class T>
@@ -348,6 +352,7 @@ __device__ __forceinline__ double pot_ghost(const Dims& d, const VT* __restrict_
     return vr + (a - (d.nx - 1)) * sR;
   };
   if (cy >= 0 && cy < d.ny) return X(cx, cy);
+  if (cy < 0 && jet_inlet(d, cx + GC)) return X(cx, -1 - cy);      // even reflection in the jet strip
   if (d.pery) return X(cx, cy < 0 ? cy + d.ny : cy - d.ny);
   if (cy < 0) { double x0 = X(cx, 0), sL = X(cx, 1) - x0; return x0 - (-cy) * sL; }
   double xr = X(cx, d.ny - 1), sR = xr - X(cx, d.ny - 2);

@@ -38,6 +38,7 @@ __global__ void k_bc_a(const Inst* __restrict__ IT, BMap bm, int ia) {
     };
     double v;
     if (j >= GC && j <= jR) v = X(i, j);
+    else if (j < GC && jet_inlet(d, i)) v = a[pidx(d, i, 2 * GC - 1 - j)];   // even reflection in the jet strip (:279-291)
     else if (d.pery) v = X(i, j < GC ? j + d.ny : j - d.ny);
     else if (j < GC) { double x0 = X(i, GC), sL = X(i, GC + 1) - x0; v = x0 - (GC - j) * sL; }
     else { double xr = X(i, jR), sR = xr - X(i, jR - 1); v = xr + (j - jR) * sR; }

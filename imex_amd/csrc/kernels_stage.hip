@@ -12,7 +12,10 @@
 // bound_cond_u / bound_cond_array (boundary.cpp:5-175, timestep.cpp:166-222) and
 // bound_cond_staggered (constrained_transport.cpp:11-64): ghost (i,j) <- (mx(i), my(j)).
 struct FillMap { int lox, hix, loy, hiy; };   // identity range of the map
-struct FillPlanes { int b[8], k[8], n; };      // planes: buffer id b[p], plane k[p] of that buffer
+// planes: buffer id b[p], plane k[p] of that buffer; jv[p] the plane's value in the jet inflow strip's
+// ghost rows: -1 = the copy (arrays), -2 = left as the producing kernel wrote it, 0..7 = conservative
+// slot, 8..15 = primitive slot (k - 8)
+struct FillPlanes { int b[8], k[8], n; signed char jv[8]; };
 
 __device__ __forceinline__ int fill_mx(const Dims& d, int kind, int i) {
   if (d.perx) return map_x(d, i);
@@ -59,9 +62,20 @@ __global__ void k_bc_fill(const Inst* __restrict__ IT, BMap bm, FillPlanes fp, i
   }
   int si = fill_mx(d, kind, i), sj = fill_my(d, kind, j);
   long long dst = pidx(d, i, j), src = pidx(d, si, sj);
+  // y_JetInflow overrides (boundary.cpp:148-172, constrained_transport.cpp:50-63)
+  int jet_mode = 0;
+  if (d.jet) {
+    if (kind == 0 && j < GC && jet_inlet(d, i)) jet_mode = 1;
+    else if (kind == 1 && j < GC && i < d.NX - 1 && jet_inlet_face(d, i)) jet_mode = 2;
+    else if (kind == 2 && j < GC - 1 && jet_inlet(d, i)) jet_mode = 3;
+  }
   for (int p = 0; p < fp.n; p++) {
     double* a = gmem(I_.buf[fp.b[p]]) + fp.k[p] * d.PS;
-    a[dst] = a[src];
+    if (jet_mode == 1 && fp.jv[p] == -2) continue;                    // formed from the strip's state already
+    if (jet_mode == 1 && fp.jv[p] >= 0) a[dst] = jet_state(d, fp.jv[p] >= 8, fp.jv[p] & 7);
+    else if (jet_mode == 2) a[dst] = 0.0;
+    else if (jet_mode == 3) a[dst] = d.jet_B0;
+    else a[dst] = a[src];
   }
 }
 
@@ -89,6 +103,7 @@ __global__ void __launch_bounds__(256) k_step_start(Phys P, const Inst* __restri
     double q[NVAR], pri[NVAR];
     for (int k = 0; k < NVAR; k++) q[k] = uin[k * d.PS + c];
     if (mode == 0) { for (int k = 0; k < NVAR; k++) pri[k] = q[k]; }
+    else if (j < GC && jet_inlet(d, i)) { for (int k = 0; k < NVAR; k++) pri[k] = jet_state(d, 1, k); }   // g_bc_prim fill
     else con2pri(P, pri, q, &ef);
     double rho = pri[0], p = pri[4], Bx = pri[5], By = pri[6], Bz = pri[7];
     double B2 = Bx * Bx + By * By + Bz * Bz;
@@ -446,16 +461,21 @@ template <int EX_TX> struct ExTile {
 // con -> pri at the five stencil cells q[(o - 2) step], reconstruct each primitive, pri -> con of both
 // face states.  NC = 4 forms rho and the momenta only (rho v needs no EOS); NC = 8 the whole state.
 // Loop bounds are compile-time so the stencil stays in registers.
+// Stencil cells o < raw are ghost cells of the jet inflow strip: the reference fills the primitive grid's
+// ghosts with bound_cond_u outside g_bc_prim, so they hold the conservative inflow state, taken as is.
 template <int NC>
 __device__ __forceinline__ void prim_faces(const Phys& P, int central, const double* __restrict__ q, long long PS,
-                                           long long step, double* Lc, double* Rc, int* ef) {
+                                           long long step, int raw, double* Lc, double* Rc, int* ef) {
   double s[5][NC];
 #pragma unroll
   for (int o = 0; o < 5; o++) {
     double c[NVAR];
 #pragma unroll
     for (int k = 0; k < NC; k++) c[k] = q[k * PS + (o - 2) * step];
-    if constexpr (NC == 4) {
+    if (o < raw) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) s[o][k] = c[k];
+    } else if constexpr (NC == 4) {
       s[o][0] = c[0];
 #pragma unroll
       for (int k = 1; k < 4; k++) s[o][k] = c[k] / c[0];
@@ -507,6 +527,11 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     int gi = DIR == 0 ? i0 + lx - 1 : i0 + lx;
     int gj = DIR == 0 ? j0 + ly : j0 + ly - 1;
     if (gi > d.nx + GC || gj > d.ny + GC) continue;      // beyond a partial tile: unused
+    if (DIR == 1 && gj < GC && jet_inlet(d, gi)) {        // bound_cond_u of the face states: the jet inflow
+      for (int k = 0; k < 5; k++)
+        if (k < nrec) { wl[k * EX_NW + it] = (T)jet_state(d, 0, k); wr[k * EX_NW + it] = (T)jet_state(d, 0, k); }
+      continue;
+    }
     int mi = DIR == 0 ? map_x(d, gi) : gi;
     int mj = DIR == 0 ? gj : map_y(d, gj);
     const long long base = pidx(d, mi, mj);
@@ -515,8 +540,9 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
       static_assert(std::is_same<T, double>::value, "primitive reconstruction is FP64 only");
       int ef = 0;
       double Lc[NVAR], Rc[NVAR];
-      if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, Lc, Rc, &ef);
-      else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, Lc, Rc, &ef);
+      const int raw = (DIR == 1 && jet_inlet(d, gi)) ? GC + 2 - gj : 0;   // stencil rows below GC in the strip
+      if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, Lc, Rc, &ef);
+      else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, Lc, Rc, &ef);
       for (int k = 0; k < 5; k++)
         if (k < nrec) { wl[k * EX_NW + it] = Lc[k]; wr[k * EX_NW + it] = Rc[k]; }
       if (ef) atomicOr(eflag, 1);
@@ -647,6 +673,11 @@ __global__ void __launch_bounds__(256) k_mag_rhs(Phys P, const Inst* __restrict_
   for (int k = 0; k < NVAR; k++) q[k] = Qe[k * d.PS + c0];
   int ef = 0;
   hhat[c0] = enthalpy_of(P, q, &ef) * q[0] / U(i, j, 0);
+  if (j == GC && jet_inlet(d, i)) {   // the jet strip's ghost holds the inflow state: its ratio is formed from it
+    const long long cg = pidx(d, i, j - 1);
+    for (int k = 0; k < NVAR; k++) q[k] = Qe[k * d.PS + cg];
+    hhat[cg] = enthalpy_of(P, q, &ef) * q[0] / U(i, j - 1, 0);
+  }
   if (ef) atomicOr(eflag, 1);
 }
 
@@ -662,6 +693,10 @@ __global__ void k_mag_writeback(const Inst* __restrict__ IT, BMap bm, int iu) {
   long long c = bx_ * (long long)blockDim.x + threadIdx.x;
   if (c >= total) return;
   int i = (int)(c % d.NX), j = (int)(c / d.NX);
+  if (j < GC && jet_inlet(d, i)) {                    // the bound_cond_u that follows: the jet inflow's B
+    u[5 * d.PS + c] = 0.0; u[6 * d.PS + c] = d.jet_B0; u[7 * d.PS + c] = 0.0;
+    return;
+  }
   int id = kidx(d, i, j);
   u[5 * d.PS + c] = x[id];
   u[6 * d.PS + c] = x[id + d.N];
@@ -762,6 +797,8 @@ __global__ void __launch_bounds__(256) k_p_rhs(const Inst* __restrict__ IT, BMap
                                  + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (my(i, j + 1) + my(i, j)) - 0.5 * (h0m + h00) * 0.5 * (my(i, j - 1) + my(i, j)))));
   int idx = (i - GC) + d.nx * (j - GC);
   b[idx] = RHS;
+  // jet inflow strip: the Dirichlet lifting of the ghost pressure (imex_subsystems.cpp:163-165, 266-267)
+  if (j == GC && jet_inlet(d, i)) b[idx] += h * h / (dy * dy) * 0.5 * (h00 + h0m) * d.jet_p;
   if (pass == 0) {
     mxs[c] = u1; mys[c] = u2;
     p0[idx] = pe[c];
@@ -785,7 +822,8 @@ __global__ void k_p_momentum(const Inst* __restrict__ IT, BMap bm, int iu) {
   const double h = S[S_ADT];
   double pc = p[kidx(d, i, j)];
   double fxm = 0.5 * (p[kidx(d, i - 1, j)] + pc), fxp = 0.5 * (pc + p[kidx(d, i + 1, j)]);
-  double fym = 0.5 * (p[kidx(d, i, j - 1)] + pc), fyp = 0.5 * (pc + p[kidx(d, i, j + 1)]);
+  const double gL = (j == GC && jet_inlet(d, i)) ? d.jet_p : p[kidx(d, i, j - 1)];   // jet strip ghost p (:202-203)
+  double fym = 0.5 * (gL + pc), fyp = 0.5 * (pc + p[kidx(d, i, j + 1)]);
   long long c = pidx(d, i, j);
   u[1 * d.PS + c] = mxs[c] - h / d.dx * (fxp - fxm);
   u[2 * d.PS + c] = mys[c] - h / d.dy * (fyp - fym);
@@ -849,7 +887,14 @@ __global__ void __launch_bounds__(256) k_e_rhs(Phys P, const Inst* __restrict__ 
   b[idx] = U_star * dx * dy;
   x0[idx] = e;
   hE[c] = h00;
+  if (j == GC && jet_inlet(d, i)) hE[pidx(d, i, j - 1)] = h0m;   // the jet strip's ghost: from its own state
   double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0), hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
+  if (d.jet) {   // Dirichlet lifting (imex_subsystems.cpp:453-475): the jet strip's ghost energy, zero elsewhere
+    const double ec = (j == GC && jet_inlet(d, i)) ? d.jet_E : 0.0;
+    double L_const = -(g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * 0.0 + hhm_x * 0.0) + 1.0 / (dy * dy) * (hhp_y * 0.0 + hhm_y * ec));
+    L_const *= dx * dy;
+    b[idx] -= L_const;
+  }
   double diag = 1.0 + (g - 1.0) * h * h * ((hhp_x + hhm_x) / (dx * dx) + (hhp_y + hhm_y) / (dy * dy));
   D[idx] = diag * dx * dy;
 }
@@ -870,7 +915,9 @@ __global__ void k_e_update(Phys P, const Inst* __restrict__ IT, BMap bm, int iu)
   long long c = pidx(d, i, j);
   u[4 * d.PS + c] = E[kidx(d, i, j)];
   u[1 * d.PS + c] -= h * (1 / (2 * d.dx)) * (g - 1) * (E[kidx(d, i + 1, j)] - E[kidx(d, i - 1, j)]);
-  u[2 * d.PS + c] -= h * (1 / (2 * d.dy)) * (g - 1) * (E[kidx(d, i, j + 1)] - E[kidx(d, i, j - 1)]);
+  // the ghost below the jet strip holds the inflow energy after bound_cond_u (imex_subsystems.cpp:508-519)
+  const double E_0m = (j == GC && jet_inlet(d, i)) ? d.jet_E : E[kidx(d, i, j - 1)];
+  u[2 * d.PS + c] -= h * (1 / (2 * d.dy)) * (g - 1) * (E[kidx(d, i, j + 1)] - E_0m);
 }
 
 // ================================================================ constrained transport
@@ -937,8 +984,10 @@ __global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int 
   auto BX = [&](int a, int b) { int I = smap_x(d, 0, a), J = smap_y(d, 0, b); return bxqs[pidx(d, I, J)] - h / d.dy * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I, J - 1)]); };
   auto BY = [&](int a, int b) { int I = smap_x(d, 1, a), J = smap_y(d, 1, b); return byqs[pidx(d, I, J)] + h / d.dx * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I - 1, J)]); };
   long long c = pidx(d, i, j);
-  bx[c] = BX(i, j);
-  by[c] = BY(i, j);
+  // the jet inflow strip's ghost faces and cells (constrained_transport.cpp:50-63, boundary.cpp:148-172)
+  bx[c] = (j < GC && i < d.NX - 1 && jet_inlet_face(d, i)) ? 0.0 : BX(i, j);
+  by[c] = (j < GC - 1 && jet_inlet(d, i)) ? d.jet_B0 : BY(i, j);
+  if (j < GC && jet_inlet(d, i)) { u[5 * d.PS + c] = 0.0; u[6 * d.PS + c] = d.jet_B0; return; }
   int I = map_x(d, i), J = map_y(d, j);
   u[5 * d.PS + c] = 0.5 * (BX(I - 1, J) + BX(I, J));
   u[6 * d.PS + c] = 0.5 * (BY(I, J - 1) + BY(I, J));
@@ -957,10 +1006,12 @@ __device__ __forceinline__ void face_states_5(const Dims& d, const Phys& P, cons
       int ci = dir == 0 ? map_x(d, i + side) : i, cj = dir == 0 ? j : map_y(d, j + side);
       long long at[5];
       for (int o = 0; o < 5; o++) at[o] = dir == 0 ? pidx(d, ci - 2 + o, cj) : pidx(d, ci, cj - 2 + o);
+      // jet strip ghosts of the primitive grid hold the conservative inflow state (see prim_faces)
+      const int raw = (dir == 1 && jet_inlet(d, ci)) ? GC + 2 - cj : 0;
       for (int q = 0; q < 5; q++) {
         const double* src = u + ks[q] * d.PS;
         double a[5];
-        for (int o = 0; o < 5; o++) a[o] = (P.prim && (q == 1 || q == 2)) ? src[at[o]] / u[at[o]] : src[at[o]];
+        for (int o = 0; o < 5; o++) a[o] = (P.prim && (q == 1 || q == 2) && o >= raw) ? src[at[o]] / u[at[o]] : src[at[o]];
         double wl, wr;
         recon5(P.recon, dir == 1 ? P.central_y : 0, a, wl, wr);
         w[dir][side][q] = side == 0 ? wr : wl;
@@ -1255,10 +1306,14 @@ __global__ void k_visc_rk(Phys P, const Inst* __restrict__ IT, BMap bm, int iu) 
 #define GRID(L, sh) dim3((unsigned)(L).nblocks[sh])
 static const dim3 B64x4(64, 4);
 
-void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind) {
+void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind, const signed char* jv) {
   FillPlanes fp;
   fp.n = n;
-  for (int p = 0; p < n; p++) { fp.b[p] = b[p]; fp.k[p] = k[p]; }
+  for (int p = 0; p < n; p++) {
+    fp.b[p] = b[p]; fp.k[p] = k[p];
+    // Grids take the inflow state (primitive in B_UPRI), arrays the copy, unless the caller says otherwise
+    fp.jv[p] = jv ? jv[p] : (b[p] >= B_UPRI && b[p] <= B_QS ? (signed char)(k[p] + (b[p] == B_UPRI ? 8 : 0)) : (signed char)-1);
+  }
   const int sh = kind == 0 ? SH_FILL0 : (kind == 1 ? SH_FILL1 : SH_FILL2);
   k_bc_fill<<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], fp, kind);
 }

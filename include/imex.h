@@ -29,7 +29,8 @@ extern "C" {
 #endif
 
 /* ---- enumerations (values follow the reference enums in include/config.hpp:27-41) ---- */
-enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_PERIODIC = 2 };           /* x/y_boundary_condition */
+enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_PERIODIC = 2,              /* x/y_boundary_condition (config.hpp:36-37) */
+       IMEX_BC_JET_INFLOW = 4 };                                     /* y only: y_JetInflow */
 enum { IMEX_RECON_NONE = 0, IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_ENO3 = 3,
        IMEX_RECON_WENO5 = 4 };                                        /* Reconstruction (config.hpp:30) */
 enum { IMEX_RECONSTATE_PRIM = 0, IMEX_RECONSTATE_CONS = 1 };         /* ReconState (config.hpp:41) */
@@ -40,7 +41,7 @@ enum { IMEX_FORMULA_ENERGY = 0, IMEX_FORMULA_PRESSURE = 1 };       /* Formula (c
 enum { IMEX_MAGFIELD_STAGGER = 0, IMEX_MAGFIELD_UNSTAGGER = 1, IMEX_MAGFIELD_POTENTIAL_A = 2,
        IMEX_MAGFIELD_NODIVFREE = 3 };                              /* MagField (config.hpp:28) */
 enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2, IMEX_OP_ENERGY = 3, IMEX_OP_POTENTIAL = 4 };
-enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2 };
+enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2, IMEX_CASE_JET = 3 };
 /* How a step is scheduled.  All three give identical results.
  *   GRAPH:  every solve gets an iteration budget learned from earlier steps; the whole step is
  *           launched as one captured hipGraph with no host synchronisation inside it; a step whose
@@ -105,6 +106,10 @@ typedef struct imex_params {
                                      divergence treatment of a stage (main.cpp:424-434, 497-510).  UNSTAGGER and
                                      POTENTIAL_A evolve the vector potential A, which the caller sets with
                                      imex_set_potential before stepping; with STAGGER and NODIVFREE A is not used */
+  double jet_rho, jet_v, jet_B0, jet_p;  /* bc_y = JET_INFLOW (config.hpp:253-273): the inflow state (rho, 0, v, 0,
+                                     p, 0, B0, 0) held in the bottom ghost cells of the strip |x_c| < jet_halfwidth;
+                                     outflow (transmissive) elsewhere.  Defaults: config.hpp:257-260 */
+  double jet_halfwidth;           /* 0.05 (config.hpp:261-269) */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */

@@ -44,6 +44,7 @@ constexpr int NV = 8;
 constexpr int GC = 3;   // ghost_cells (config.hpp:55)
 
 struct Ctx {
+  using P_t = imex_params;
   imex_params P;
   int nx, ny, NX, NY;       // interior, padded
   double dx, dy;
@@ -59,6 +60,22 @@ inline double G(const std::vector<double>& a, const Ctx& C, int i, int j, int k)
 inline double& A(std::vector<double>& a, const Ctx& C, int i, int j) { return a[i + j * C.NX]; }
 inline double A(const std::vector<double>& a, const Ctx& C, int i, int j) { return a[i + j * C.NX]; }
 using Vec = std::vector<double>;
+
+// ---------------------------------------------------------------- MHD jet inflow strip (config.hpp:253-273)
+inline bool jet_on(const Ctx& C) { return C.P.bc_y == IMEX_BC_JET_INFLOW; }
+inline bool jet_inlet(const Ctx& C, int i) {                         // cell centre |x_c| < 0.05 (:261-265)
+  if (!jet_on(C)) return false;
+  double xc = C.P.x0 + (i - GC + 0.5) * C.dx;
+  return std::fabs(xc) < C.P.jet_halfwidth;
+}
+inline bool jet_inlet_face(const Ctx& C, int i) {                    // x face i+1/2 (:266-270)
+  if (!jet_on(C)) return false;
+  double xf = C.P.x0 + (i - GC + 1.0) * C.dx;
+  return std::fabs(xf) < C.P.jet_halfwidth - 1e-12;
+}
+inline double jet_E_val(const Ctx& C) {                              // :271-273 (ideal-gas form, as written)
+  return C.P.jet_p / (C.P.gamma - 1.0) + 0.5 * C.P.jet_rho * C.P.jet_v * C.P.jet_v + 0.5 * C.P.jet_B0 * C.P.jet_B0;
+}
 
 // ---------------------------------------------------------------- EOS (include/config.hpp:130-239)
 // Redlich-Kwong temperature Newton solves: the reference loops without a cap
@@ -200,7 +217,8 @@ void grid_pri2con(Ctx& C, Vec& con, const Vec& pri) {               // state.cpp
 
 // ---------------------------------------------------------------- boundary conditions
 // bound_cond_u (boundary.cpp:5-175), periodic and transmissive branches: x pass over all rows, then y pass over all columns.
-void bc_grid(const Ctx& C, Vec& u) {
+// bound_cond_u (boundary.cpp:5-175); prim = g_bc_prim (the grid holds primitives: slot 4 is p)
+void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
   const int nx = C.nx, ny = C.ny;
   for (int j = 0; j < C.NY; j++)
     for (int g = 0; g < GC; g++)
@@ -224,6 +242,16 @@ void bc_grid(const Ctx& C, Vec& u) {
           G(u, C, i, ny + GC + g, k) = G(u, C, i, ny + GC - 1, k);
         }
       }
+  if (jet_on(C))                                                      // y_JetInflow (:148-172): the strip
+    for (int i = 0; i < C.NX; i++) {
+      if (!jet_inlet(C, i)) continue;
+      for (int g = 0; g < GC; g++) {
+        const Ctx::P_t& P = C.P;
+        double s[NV] = {P.jet_rho, 0.0, prim ? P.jet_v : P.jet_rho * P.jet_v, 0.0, prim ? P.jet_p : jet_E_val(C),
+                        0.0, P.jet_B0, 0.0};
+        for (int k = 0; k < NV; k++) G(u, C, i, g, k) = s[k];
+      }
+    }
 }
 // bound_cond_array (timestep.cpp:166-222)
 void bc_array(const Ctx& C, Vec& u) {
@@ -239,7 +267,7 @@ void bc_array(const Ctx& C, Vec& u) {
       else { A(u, C, i, g) = A(u, C, i, GC); A(u, C, i, ny + GC + g) = A(u, C, i, ny + GC - 1); }
     }
 }
-// bound_cond_staggered (constrained_transport.cpp:11-64), jet branch excluded.
+// bound_cond_staggered (constrained_transport.cpp:11-64)
 void bc_staggered(const Ctx& C, Vec& u, int axis) {
   const int nx = C.nx, ny = C.ny;
   if (C.P.bc_x == IMEX_BC_PERIODIC) {
@@ -265,6 +293,17 @@ void bc_staggered(const Ctx& C, Vec& u, int axis) {
   } else {
     for (int i = 0; i < C.NX; i++)
       for (int k = 0; k < GC; k++) { A(u, C, i, k) = A(u, C, i, GC); A(u, C, i, ny + GC + k) = A(u, C, i, ny + GC - 1); }
+  }
+  if (jet_on(C)) {                                                    // the inflow strip (:50-63)
+    if (axis == 1) {
+      for (int i = 0; i < C.NX; i++)
+        if (jet_inlet(C, i))
+          for (int k = 0; k < GC - 1; k++) A(u, C, i, k) = C.P.jet_B0;
+    } else {
+      for (int i = 0; i < C.NX - 1; i++)
+        if (jet_inlet_face(C, i))
+          for (int k = 0; k < GC; k++) A(u, C, i, k) = 0.0;
+    }
   }
 }
 // M_P / M_B index maps (implicit_operators.cpp:5-139), periodic and transmissive branches.
@@ -633,8 +672,9 @@ void p_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, PCo
       double pii = X(i, j);
       double rho = G(u_con, C, i, j, 0);
       double Vdiag = jacobian ? V_slope(C, p_freeze[M_P(C, i, j)], rho) : (1.0 / (C.P.gamma - 1.0));
+      double p_0m = (j == GC && jet_inlet(C, i)) ? 0.0 : X(i, j - 1);   // jet strip: homogeneous ghost (:559-561)
       double ax = Vdiag * pii - h * h * (1 / (dx * dx) * (0.5 * (hp0 + h00) * (X(i + 1, j) - X(i, j)) - 0.5 * (h00 + hm0) * (X(i, j) - X(i - 1, j)))
-                                       + 1 / (dy * dy) * (0.5 * (h0p + h00) * (X(i, j + 1) - X(i, j)) - 0.5 * (h0m + h00) * (X(i, j) - X(i, j - 1))));
+                                       + 1 / (dy * dy) * (0.5 * (h0p + h00) * (X(i, j + 1) - X(i, j)) - 0.5 * (h0m + h00) * (X(i, j) - p_0m)));
       double nu_x = S.vx_max * dx, nu_y = S.vy_max * dy;
       double lap_p = nu_x / (dx * dx) * (0.5 * (E(i + 1, j) + E(i, j)) * (X(i + 1, j) - pii) - 0.5 * (E(i, j) + E(i - 1, j)) * (pii - X(i - 1, j)))
                    + nu_y / (dy * dy) * (0.5 * (E(i, j + 1) + E(i, j)) * (X(i, j + 1) - pii) - 0.5 * (E(i, j) + E(i, j - 1)) * (pii - X(i, j - 1)));
@@ -921,6 +961,11 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
       A(mx, C, i, j) = G(W.u_con, C, i, j, 1); A(my, C, i, j) = G(W.u_con, C, i, j, 2); A(mz, C, i, j) = G(W.u_con, C, i, j, 3);
     }
   bc_array(C, mx); bc_array(C, my); bc_array(C, mz);
+  if (jet_on(C))                                                      // the inflow momentum in the strip (:107-118)
+    for (int i = 0; i < C.NX; i++) {
+      if (!jet_inlet(C, i)) continue;
+      for (int g = 0; g < GC; g++) { A(mx, C, i, g) = 0.0; A(my, C, i, g) = C.P.jet_rho * C.P.jet_v; A(mz, C, i, g) = 0.0; }
+    }
   PCoef pc;
   p_coef(C, W.u_con, W.Qe, pc);
   Vec b(N), p(N);
@@ -938,6 +983,7 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
         double RHS = p_star - ke - h * (1 / dx * (0.5 * (hp0 + h00) * 0.5 * (A(mx, C, i + 1, j) + A(mx, C, i, j)) - 0.5 * (hm0 + h00) * 0.5 * (A(mx, C, i - 1, j) + A(mx, C, i, j)))
                                         + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (A(my, C, i, j + 1) + A(my, C, i, j)) - 0.5 * (h0m + h00) * 0.5 * (A(my, C, i, j - 1) + A(my, C, i, j)))));
         b[M_P(C, i, j)] = RHS;
+        if (j == GC && jet_inlet(C, i)) b[M_P(C, i, j)] += h * h / (dy * dy) * 0.5 * (h00 + h0m) * C.P.jet_p;   // lifting (:163-165, 266-267)
         if (pass == 0) p[M_P(C, i, j)] = A(pc.pe, C, i, j);
       }
   };
@@ -948,7 +994,8 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
       double flux_xm = 0.5 * (p[M_P(C, i - 1, j)] + p[M_P(C, i, j)]);
       double flux_xp = 0.5 * (p[M_P(C, i, j)] + p[M_P(C, i + 1, j)]);
       G(W.u_con, C, i, j, 1) = A(mx, C, i, j) - h / dx * (flux_xp - flux_xm);
-      double flux_ym = 0.5 * (p[M_P(C, i, j - 1)] + p[M_P(C, i, j)]);
+      double gL = (j == GC && jet_inlet(C, i)) ? C.P.jet_p : p[M_P(C, i, j - 1)];   // jet strip ghost p (:202-203)
+      double flux_ym = 0.5 * (gL + p[M_P(C, i, j)]);
       double flux_yp = 0.5 * (p[M_P(C, i, j)] + p[M_P(C, i, j + 1)]);
       G(W.u_con, C, i, j, 2) = A(my, C, i, j) - h / dy * (flux_yp - flux_ym);
     }
@@ -992,8 +1039,9 @@ void e_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, Vec
       double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0);
       double hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
       double Ei = X(i, j);
+      double E_0m = (j == GC && jet_inlet(C, i)) ? 0.0 : X(i, j - 1);   // getE: jet strip ghost homogeneous (:772-779)
       double ax = Ei - (g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * (X(i + 1, j) - Ei) - hhm_x * (Ei - X(i - 1, j)))
-                                        + 1.0 / (dy * dy) * (hhp_y * (X(i, j + 1) - Ei) - hhm_y * (Ei - X(i, j - 1))));
+                                        + 1.0 / (dy * dy) * (hhp_y * (X(i, j + 1) - Ei) - hhm_y * (Ei - E_0m)));
       Ax[M_P(C, i, j)] = ax * dx * dy;
     }
   C.st.op_applies_p++;
@@ -1026,6 +1074,15 @@ int pre_subsystem_energy(Ctx& C, const imex_scalars& S, Stage& W) {
       double U_star = G(u, C, i, j, 4) - 0.5 * h / dx * (hp0 * G(u, C, i + 1, j, 1) - hm0 * G(u, C, i - 1, j, 1))
                     - 0.5 * h / dy * (h0p * G(u, C, i, j + 1, 2) - h0m * G(u, C, i, j - 1, 2));
       b[M_P(C, i, j)] = U_star * dx * dy;
+      if (jet_on(C)) {                                                // Dirichlet lifting (:453-475): getEc is
+        double h00 = A(hE, C, i, j);                                  // jet_E_val on the strip's ghost row only
+        double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0), hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
+        auto getEc = [&](int ii, int jj) { return (jj < GC && jet_inlet(C, ii)) ? jet_E_val(C) : 0.0; };
+        double L_const = -(g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * getEc(i + 1, j) + hhm_x * getEc(i - 1, j))
+                                           + 1.0 / (dy * dy) * (hhp_y * getEc(i, j + 1) + hhm_y * getEc(i, j - 1)));
+        L_const *= dx * dy;
+        b[M_P(C, i, j)] -= L_const;
+      }
       U[M_P(C, i, j)] = G(u, C, i, j, 4);
     }
   e_diag(C, S, hE, D);
@@ -1150,6 +1207,12 @@ void bc_A(const Ctx& C, Vec& Av) {
         double sR = Ag(i, jR, k) - Ag(i, jR - 1, k);
         for (int g = 0; g < GC; g++) Ag(i, jR + 1 + g, k) = Ag(i, jR, k) + (g + 1) * sR;
       }
+    }
+  if (jet_on(C))                                                      // even reflection in the strip (:279-291)
+    for (int i = 0; i < C.NX; i++) {
+      if (!jet_inlet(C, i)) continue;
+      for (int k = 0; k < 3; k++)
+        for (int g = 0; g < GC; g++) Ag(i, GC - 1 - g, k) = Ag(i, GC + g, k);
     }
 }
 
@@ -1498,7 +1561,7 @@ int run_step(Run& R) {
   imex_scalars& S = R.S;
   int rc = 0;
   S.timestep++;
-  bc_grid(C, R.u_pri);
+  bc_grid(C, R.u_pri, true);                                          // g_bc_prim around bound_cond_u(u_pri) (main.cpp:336)
   cfl(C, S, R.u_pri);
   S.t = S.t + S.dt;
   grid_pri2con(C, R.u_con, R.u_pri);
@@ -1964,11 +2027,12 @@ int orc_cfl(const imex_params* p, const double* u_pri, imex_scalars* sc) {
   cfl(C, *sc, u);
   return 0;
 }
-int orc_bc(const imex_params* p, double* a, int kind) {   // kind 0 grid, 1 array, 2 staggered axis 0, 3 staggered axis 1
+// kind 0 grid, 1 array, 2 staggered axis 0, 3 staggered axis 1, 4 grid of primitives
+int orc_bc(const imex_params* p, double* a, int kind) {
   Ctx C; ctx_init(C, p);
-  size_t n = (size_t)C.NX * C.NY * (kind == 0 ? NV : 1);
+  size_t n = (size_t)C.NX * C.NY * (kind == 0 || kind == 4 ? NV : 1);
   Vec v(a, a + n);
-  if (kind == 0) bc_grid(C, v); else if (kind == 1) bc_array(C, v); else bc_staggered(C, v, kind - 2);
+  if (kind == 0 || kind == 4) bc_grid(C, v, kind == 4); else if (kind == 1) bc_array(C, v); else bc_staggered(C, v, kind - 2);
   std::memcpy(a, v.data(), n * sizeof(double));
   return 0;
 }

@@ -113,3 +113,24 @@ def test_bench_tables_follow_the_abi():
     assert b.RECON_NAMES == {"none": im.RECON_NONE, "minmod": im.RECON_MINMOD, "vanleer": im.RECON_VANLEER,
                              "eno3": im.RECON_ENO3, "weno5": im.RECON_WENO5}
     assert set(b.RECON_LABEL) == set(b.RECON_NAMES.values())
+
+
+def test_jet_initial_data():
+    """testMHDJet (initial_conditions.cpp:671-689, constrained_transport.cpp:240-243): ambient
+    (0.1 gamma, 0, 0, 0, 1, 0, B0, 0) in primitives, the inflow state in the strip's bottom ghosts, faces
+    (0, B0), A_z = -B0 x."""
+    p, kw = im.case_of("jet32")
+    assert (p.nx, p.ny, p.bc_y, p.jet_rho, p.jet_halfwidth) == (32, 48, im.BC_JET_INFLOW, p.gamma, 0.05)
+    u, bx, by = im.init_case(p, im.CASE_JET, 1.0, 0.0, 0.0)
+    G = im.GC
+    amb = np.array([0.1 * p.gamma, 0, 0, 0, 1.0, 0, p.jet_B0, 0])
+    assert np.array_equal(u[G:, :], np.broadcast_to(amb, u[G:, :].shape))
+    dx = 1.0 / p.nx
+    strip = [i for i in range(p.nx + 2 * G) if abs(-0.5 + (i - G + 0.5) * dx) < 0.05]
+    jet = np.array([p.jet_rho, 0, p.jet_v, 0, p.jet_p, 0, p.jet_B0, 0])
+    for i in range(p.nx + 2 * G):
+        assert np.array_equal(u[:G, i], np.broadcast_to(jet if i in strip else amb, (G, 8))), i
+    assert np.all(bx == 0) and np.all(by == p.jet_B0)
+    A = im.init_potential(p, im.CASE_JET, 1.0)
+    xc = -0.5 + (np.arange(p.nx + 2 * G) - G + 0.5) * dx
+    assert np.allclose(A[..., 2], np.broadcast_to(-p.jet_B0 * xc, A[..., 2].shape), rtol=0, atol=1e-12)

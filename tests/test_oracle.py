@@ -955,3 +955,121 @@ def test_vortex_advects_with_background_flow():
     w = B2 / B2.sum()
     assert (w * X).sum() == pytest.approx(sc["t"], abs=0.05)
     assert (w * Y).sum() == pytest.approx(sc["t"], abs=0.05)
+
+
+# ---------------------------------------------------------------- MHD jet inflow (y_JetInflow)
+def _jet(n=32, m=48):
+    p = im.jet_params(n, m)
+    return p
+
+
+def _strip(p):
+    dx = (p.x1 - p.x0) / p.nx
+    cells = [i for i in range(p.nx + 2 * G) if abs(p.x0 + (i - G + 0.5) * dx) < p.jet_halfwidth]
+    faces = [i for i in range(p.nx + 2 * G - 1) if abs(p.x0 + (i - G + 1.0) * dx) < p.jet_halfwidth - 1e-12]
+    return cells, faces
+
+
+def test_jet_ghost_fills_brute_force():
+    """bound_cond_u / bound_cond_staggered / bound_cond_A under y_JetInflow (boundary.cpp:148-172, 279-291,
+    constrained_transport.cpp:50-63): outflow copies everywhere except the bottom strip |x_c| < 0.05."""
+    p = _jet(32, 24)
+    cells, faces = _strip(p)
+    assert cells == list(range(G + 14, G + 18)) and faces == list(range(G + 14, G + 17))
+    rng = np.random.default_rng(8)
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    a = rng.uniform(0.5, 2.0, (NY, NX, 8))
+    q = ideal(32, 24, bc=im.BC_TRANSMISSIVE)
+    q.x0, q.x1, q.y0, q.y1 = p.x0, p.x1, p.y0, p.y1
+    ref = orc.bc(q, a, 0)
+    for kind, prim in ((0, False), (4, True)):
+        got = orc.bc(p, a, kind)
+        jet = np.array([p.jet_rho, 0, p.jet_v if prim else p.jet_rho * p.jet_v, 0,
+                        p.jet_p if prim else p.jet_p / (p.gamma - 1) + 0.5 * p.jet_rho * p.jet_v ** 2 + 0.5 * p.jet_B0 ** 2,
+                        0, p.jet_B0, 0])
+        mask = np.zeros((NY, NX), bool)
+        mask[:G, cells] = True
+        assert np.array_equal(got[~mask], ref[~mask])
+        assert np.array_equal(got[mask], np.tile(jet, (mask.sum(), 1)))
+    b = rng.uniform(0.5, 2.0, (NY, NX))
+    for kind, rows, cols, val in ((2, G, faces, 0.0), (3, G - 1, cells, p.jet_B0)):
+        got, ref = orc.bc(p, b, kind), orc.bc(q, b, kind)
+        mask = np.zeros((NY, NX), bool)
+        mask[:rows, cols] = True
+        assert np.array_equal(got[~mask], ref[~mask]) and np.all(got[mask] == val)
+    A = rng.uniform(-1, 1, (NY, NX, 8))
+    A[..., 3:] = 0
+    got, ref = orc.bc_potential(p, A), orc.bc_potential(q, A)
+    mask = np.zeros((NY, NX), bool)
+    mask[:G, cells] = True
+    assert np.array_equal(got[~mask], ref[~mask])
+    for g in range(G):                                  # even reflection about the inflow face
+        assert np.array_equal(got[G - 1 - g, cells, :3], A[G + g, cells, :3])
+
+
+@pytest.mark.parametrize("op", [im.OP_P_CARRIER, im.OP_ENERGY])
+def test_jet_operators_drop_the_ghost_coupling(op):
+    """Pressure_Subsystem_AX (implicit_operators.cpp:559-561) and Energy_Subsystem_AX (:772-779): in the inflow
+    strip's first row the ghost neighbour is a homogeneous Dirichlet unknown, so the operator differs from the
+    outflow one by exactly the coupling -h^2/dy^2 hbar_0m x(i, g) that the clamp would add; nothing else moves."""
+    p, u, bx, by = make_case("jet", 32, 24)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(1)
+    st, sc, _ = o.step_capture(3)
+    q = im.jet_params(32, 24)
+    q.bc_y = im.BC_TRANSMISSIVE
+    rng = np.random.default_rng(4)
+    N = p.nx * p.ny
+    x = rng.standard_normal(N)
+    a, b = orc.apply_op(p, sc, st, op, x), orc.apply_op(q, sc, st, op, x)
+    cells, _ = _strip(p)
+    d = (a - b).reshape(p.ny, p.nx)
+    strip = [c - G for c in cells]
+    others = np.ones_like(d, bool)
+    others[0, strip] = False
+    assert np.all(d[others] == 0)
+    uc, Qe = st["u_con"], st["Qe"]
+    dx, dy = (p.x1 - p.x0) / p.nx, (p.y1 - p.y0) / p.ny
+    h = sc["adt"]
+    for i in strip:
+        I = i + G
+        if op == im.OP_P_CARRIER:     # hbar = calenthalpy(Qe) rho^e / rho (timestep.cpp:224-230)
+            hb = 0.5 * (_enthalpy_ratio(p, Qe[G, I], uc[G, I, 0]) + _enthalpy_ratio(p, Qe[G - 1, I], uc[G - 1, I, 0]))
+            want = h * h / dy ** 2 * hb * x[i]
+        else:                         # hbar = caltotalenthalpy(Qe) rho^e / rho (timestep.cpp:232-237), times dx dy
+            hb = 0.5 * (_total_enthalpy_ratio(p, Qe[G, I], uc[G, I, 0]) + _total_enthalpy_ratio(p, Qe[G - 1, I], uc[G - 1, I, 0]))
+            want = (p.gamma - 1) * h * h / dy ** 2 * hb * x[i] * dx * dy
+        assert d[0, i] == pytest.approx(want, rel=1e-10, abs=1e-13 * abs(b[i]))
+
+
+def _enthalpy_ratio(p, qe, rho):
+    pri = orc.con2pri(p, qe[None, :])[0]
+    e = pri[4] / ((p.gamma - 1) * pri[0])
+    return (e + pri[4] / pri[0]) * qe[0] / rho
+
+
+def _total_enthalpy_ratio(p, qe, rho):
+    pri = orc.con2pri(p, qe[None, :])[0]
+    return (qe[4] + pri[4]) / pri[0] * qe[0] / rho
+
+
+def test_jet_run_injects_mass_symmetrically():
+    """Whole runs under the inflow: the injected mass follows the jet's mass flux rho v (strip width) t, the
+    solution stays mirror-symmetric about x = 0, the inflow ghosts keep the jet state and the staggered CT
+    keeps div B at round-off."""
+    for form in (im.FORMULA_PRESSURE, im.FORMULA_ENERGY):
+        p, u, bx, by = make_case("jet", 32, 48, formula=form)
+        o = orc.Oracle(p)
+        o.set_state(u, bx, by)
+        m0 = orc.pri2con(p, u)[G:-G, G:-G, 0].sum()
+        o.step(4)
+        uc, up, bx1, by1, sc = o.get_state()
+        dx, dy = 1.0 / p.nx, 1.5 / p.ny
+        cells, _ = _strip(p)
+        inj = (uc[G:-G, G:-G, 0].sum() - m0) * dx * dy
+        assert inj == pytest.approx(p.jet_rho * p.jet_v * len(cells) * dx * sc["t"], rel=0.05)
+        assert np.array_equal(up[G:-G, G:-G, 0], up[G:-G, G:-G, 0][:, ::-1])
+        assert np.allclose(up[G:-G, G:-G, 1], -up[G:-G, G:-G, 1][:, ::-1], rtol=0, atol=1e-9 * np.abs(up[..., 1]).max())
+        assert np.all(up[:G, cells][..., 2] == p.jet_v)
+        assert o.stats()["div_b"] <= 1e-15 * p.jet_B0 / dx
