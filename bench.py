@@ -27,6 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # IMEX_RECON_* values of the C ABI
 RECON_NAMES = {"none": 0, "minmod": 1, "vanleer": 2, "eno3": 3, "weno5": 4}
 RECON_LABEL = {0: "NoRecon", 1: "Minmod", 2: "VanLeer", 3: "ENO3", 4: "WENO5"}
+WORKLOAD_NAME = {"ot": "orszag_tang", "vortex": "mhd_vortex", "blast": "magnetic_blast", "jet": "mhd_jet"}
 
 
 def log(*a):
@@ -452,6 +453,8 @@ def main():
                     help="reconstruct the conservative (default) or the primitive state")
     ap.add_argument("--recon-steps", type=int, default=20,
                     help="timed FP64 steps of each other reconstruction among the secondary workloads")
+    ap.add_argument("--jet-steps", type=int, default=20,
+                    help="timed steps of the 400x600 MHD jet FP64 / mixed pair among the secondary workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -487,6 +490,8 @@ def main():
         runs[f] = run_gpu(args.workload, args.steps, args.warmup, prec, sched, args.mixed_tols if f == "mixed" else None,
                           formula, magfield, over)
     variant = args.magfield if args.magfield != "stagger" else args.formula
+    if not args.workload.startswith("ot1024"):
+        variant = "other"                 # the committed traffic summaries are per-dispatch figures of the OT 1024^2 runs
     res = {f: summarize(runs[f], args.steps, f, variant) for f in forms}
     head = "mixed" if "mixed" in res else "fp64"
     p = runs[head]["p"]
@@ -503,7 +508,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32-operators/f64-krylov",
         "data": "synthetic (Orszag-Tang initial data generated on the host)",
-        "config": {"workload": f"orszag_tang_{p.nx}x{p.ny}_weno5_staggerct_rk2", "nx": p.nx, "ny": p.ny,
+        "config": {"workload": f"{WORKLOAD_NAME.get(args.workload.rstrip('0123456789'), args.workload)}_{p.nx}x{p.ny}_"
+                               f"{RECON_LABEL[p.recon].lower()}_staggerct_rk2", "nx": p.nx, "ny": p.ny,
                    "recon": RECON_LABEL[p.recon], "recon_state": args.recon_state, "p_floor": p.p_floor,
                    "gamma": p.gamma, "form": head,
                    "formula": args.formula, "magfield": args.magfield, "schedule": args.schedule},
@@ -553,6 +559,9 @@ def main():
                      "recon_state": im.RECONSTATE_PRIM if st == "prim" else im.RECONSTATE_CONS}
                 out["reconstructions"][f"{rn}_{st}"] = run_pair(args.workload, args.recon_steps, 3, sched, formula, magfield,
                                                                 forms=("fp64",), over=o)["fp64"]
+        out["jet"] = dict(workload="MHD jet 400x600 (testMHDJet's grid): jet inflow strip |x| < 0.05 on the bottom "
+                                   "boundary (rho 1.4, v 1e4, B0 141.4, p 1), outflow elsewhere, Minmod, StaggerCT",
+                          steps=args.jet_steps, warmup=3, **run_pair("jet400", args.jet_steps, 3, sched))
         out["diagnostics"] = dict(workload="Orszag-Tang 1024x1024, FP64: per-step diagnostics recorded on the device, "
                                            "Diagnose_CT_EMF, VTK and RK_*.dat output", steps=20,
                                   **run_diag_cost(args.workload, 20, 3, sched))
