@@ -11,7 +11,7 @@ import numpy as np
 GC = 3
 NVAR = 8
 
-BC_TRANSMISSIVE, BC_PERIODIC, BC_JET_INFLOW = 0, 2, 4
+BC_TRANSMISSIVE, BC_REFLECTIVE, BC_PERIODIC, BC_JET_INFLOW = 0, 1, 2, 4
 RECON_MINMOD, RECON_VANLEER, RECON_WENO5 = 1, 2, 4
 EOS_IDEAL, EOS_RK = 0, 1
 CT_CONSISTENT, CT_PLAIN = 0, 1

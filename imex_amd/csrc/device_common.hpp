@@ -27,6 +27,7 @@ struct Dims {
   int N;                  // nx*ny
   double dx, dy;
   int perx, pery;         // 1 = periodic, 0 = transmissive
+  int refx, refy;         // 1 = reflective (x_Reflective / y_Reflective)
   double x0, y0;          // domain origin (cell centres at x0 + (i - g + 1/2) dx)
   int jet;                // 1 = bottom boundary is the MHD jet inflow (y_JetInflow)
   double jet_hw, jet_rho, jet_v, jet_B0, jet_p, jet_E;   // strip half-width and inflow state; jet_E = jet_E_val()
@@ -65,15 +66,23 @@ struct Phys {
 
 // ---------------------------------------------------------------- index maps
 // bound_cond_u / bound_cond_array / M_P map of a padded index to its interior source.
+// ghost -> image cell (M_P, implicit_operators.cpp:5-139): periodic wrap, reflective mirror, clamp otherwise
 __device__ __forceinline__ int map_x(const Dims& d, int i) {
-  if (i < GC) return d.perx ? d.nx + i : GC;
-  if (i > d.nx + GC - 1) return d.perx ? i - d.nx : d.nx + GC - 1;
+  if (i < GC) return d.perx ? d.nx + i : (d.refx ? 2 * GC - 1 - i : GC);
+  if (i > d.nx + GC - 1) return d.perx ? i - d.nx : (d.refx ? 2 * (d.nx + GC) - i - 1 : d.nx + GC - 1);
   return i;
 }
 __device__ __forceinline__ int map_y(const Dims& d, int j) {
-  if (j < GC) return d.pery ? d.ny + j : GC;
-  if (j > d.ny + GC - 1) return d.pery ? j - d.ny : d.ny + GC - 1;
+  if (j < GC) return d.pery ? d.ny + j : (d.refy ? 2 * GC - 1 - j : GC);
+  if (j > d.ny + GC - 1) return d.pery ? j - d.ny : (d.refy ? 2 * (d.ny + GC) - j - 1 : d.ny + GC - 1);
   return j;
+}
+// Reflective ghost fills negate (boundary.cpp:17-54, 89-126; timestep.cpp:175-214): the factor a ghost
+// value of sign class `neg` (1: momenta, B and every bound_cond_array array) takes at padded (i, j).
+__device__ __forceinline__ double refl_factor(const Dims& d, int i, int j, bool neg) {
+  if (!neg) return 1.0;
+  bool gx = d.refx && (i < GC || i > d.nx + GC - 1), gy = d.refy && (j < GC || j > d.ny + GC - 1);
+  return (gx != gy) ? -1.0 : 1.0;
 }
 __device__ __forceinline__ long long pidx(const Dims& d, int i, int j) { return (long long)i + (long long)j * d.NX; }
 // compact Krylov index of padded cell (i,j) after the boundary map (M_P)

@@ -243,6 +243,7 @@ static Dims make_dims(const imex_params& p) {
   d.dx = (p.x1 - p.x0) / ((p.nx + 1) - 1);     // main.cpp:226-227
   d.dy = (p.y1 - p.y0) / ((p.ny + 1) - 1);
   d.perx = p.bc_x == IMEX_BC_PERIODIC; d.pery = p.bc_y == IMEX_BC_PERIODIC;
+  d.refx = p.bc_x == IMEX_BC_REFLECTIVE; d.refy = p.bc_y == IMEX_BC_REFLECTIVE;
   d.x0 = p.x0; d.y0 = p.y0;
   d.jet = p.bc_y == IMEX_BC_JET_INFLOW;
   d.jet_hw = p.jet_halfwidth; d.jet_rho = p.jet_rho; d.jet_v = p.jet_v; d.jet_B0 = p.jet_B0; d.jet_p = p.jet_p;
@@ -391,8 +392,12 @@ static int build_maps(imex_ctx* c) {
 // solve at budget key k
 static inline int op_prec(const imex_ctx* c, int op) { return (c->mixed_ops >> op) & 1; }
 static inline int key_prec(const imex_ctx* c, int k) { return op_prec(c, (k / NIT) % 3 == 0 ? 0 : 1); }
-static inline MagIds mag_ids(const imex_ctx* c) { MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; m.pot = 0; return m; }
-static inline MagIds pot_ids(const imex_ctx* c) { MagIds m = mag_ids(c); m.pot = 1; return m; }
+static inline MagIds mag_ids(const imex_ctx* c) {
+  MagIds m; m.Qe = c->cur.Qe; m.u = c->cur.u; m.pot = 0;
+  m.refl = c->prm.bc_x == IMEX_BC_REFLECTIVE || c->prm.bc_y == IMEX_BC_REFLECTIVE;
+  return m;
+}
+static inline MagIds pot_ids(const imex_ctx* c) { MagIds m = mag_ids(c); m.pot = 1; m.refl = 0; return m; }
 static inline PIds p_ids(const imex_ctx* c, int jacobian, int precond) {
   PIds q; q.u = c->cur.u; q.jacobian = jacobian; q.precond = precond; q.energy = 0; return q;
 }
@@ -1223,13 +1228,15 @@ static int pull_scalars(imex_ctx* c, int i) {
 
 // ---------------------------------------------------------------- host boundary fills for initial data
 static int hmapx(const imex_params& p, int i) {
-  if (i < GC) return p.bc_x == IMEX_BC_PERIODIC ? p.nx + i : GC;
-  if (i > p.nx + GC - 1) return p.bc_x == IMEX_BC_PERIODIC ? i - p.nx : p.nx + GC - 1;
+  const bool per = p.bc_x == IMEX_BC_PERIODIC, ref = p.bc_x == IMEX_BC_REFLECTIVE;
+  if (i < GC) return per ? p.nx + i : (ref ? 2 * GC - 1 - i : GC);
+  if (i > p.nx + GC - 1) return per ? i - p.nx : (ref ? 2 * (p.nx + GC) - i - 1 : p.nx + GC - 1);
   return i;
 }
 static int hmapy(const imex_params& p, int j) {
-  if (j < GC) return p.bc_y == IMEX_BC_PERIODIC ? p.ny + j : GC;
-  if (j > p.ny + GC - 1) return p.bc_y == IMEX_BC_PERIODIC ? j - p.ny : p.ny + GC - 1;
+  const bool per = p.bc_y == IMEX_BC_PERIODIC, ref = p.bc_y == IMEX_BC_REFLECTIVE;
+  if (j < GC) return per ? p.ny + j : (ref ? 2 * GC - 1 - j : GC);
+  if (j > p.ny + GC - 1) return per ? j - p.ny : (ref ? 2 * (p.ny + GC) - j - 1 : p.ny + GC - 1);
   return j;
 }
 // Ghost fill with the reference's pass order (x over all rows, then y over all columns), then the jet
@@ -1237,25 +1244,29 @@ static int hmapy(const imex_params& p, int j) {
 // g_bc_prim), the staggered overrides of bound_cond_staggered (constrained_transport.cpp:50-63).
 static void hfill(const imex_params& p, double* a, int nvar, int kind, int prim = 0) {
   const int NX = p.nx + 2 * GC, NY = p.ny + 2 * GC;
+  // cells mirror across a reflective boundary with momenta and B negated; staggered faces there
+  // are filled as for an outflow (bound_cond_staggered's non-periodic branch)
   auto mx = [&](int i) {
-    if (p.bc_x == IMEX_BC_PERIODIC) return hmapx(p, i);
+    if (p.bc_x == IMEX_BC_PERIODIC || (kind == 0 && p.bc_x == IMEX_BC_REFLECTIVE)) return hmapx(p, i);
     int lo = kind == 1 ? GC - 1 : GC;
     return i < lo ? lo : (i > p.nx + GC - 1 ? p.nx + GC - 1 : i);
   };
   auto my = [&](int j) {
-    if (p.bc_y == IMEX_BC_PERIODIC) return hmapy(p, j);
+    if (p.bc_y == IMEX_BC_PERIODIC || (kind == 0 && p.bc_y == IMEX_BC_REFLECTIVE)) return hmapy(p, j);
     int lo = kind == 2 ? GC - 1 : GC;
     return j < lo ? lo : (j > p.ny + GC - 1 ? p.ny + GC - 1 : j);
   };
+  auto sg = [&](int k, bool ref) { return (ref && kind == 0 && nvar == NVAR && k != 0 && k != 4) ? -1.0 : 1.0; };
+  const bool rx = p.bc_x == IMEX_BC_REFLECTIVE, ry = p.bc_y == IMEX_BC_REFLECTIVE;
   for (int j = 0; j < NY; j++)
     for (int i = 0; i < NX; i++) {
       int si = mx(i);
-      if (si != i) for (int k = 0; k < nvar; k++) a[(size_t)nvar * (i + j * NX) + k] = a[(size_t)nvar * (si + j * NX) + k];
+      if (si != i) for (int k = 0; k < nvar; k++) a[(size_t)nvar * (i + j * NX) + k] = sg(k, rx) * a[(size_t)nvar * (si + j * NX) + k];
     }
   for (int i = 0; i < NX; i++)
     for (int j = 0; j < NY; j++) {
       int sj = my(j);
-      if (sj != j) for (int k = 0; k < nvar; k++) a[(size_t)nvar * (i + j * NX) + k] = a[(size_t)nvar * (i + sj * NX) + k];
+      if (sj != j) for (int k = 0; k < nvar; k++) a[(size_t)nvar * (i + j * NX) + k] = sg(k, ry) * a[(size_t)nvar * (i + sj * NX) + k];
     }
   if (p.bc_y != IMEX_BC_JET_INFLOW) return;
   const Dims d = make_dims(p);
@@ -1471,9 +1482,10 @@ int imex_init_potential(const imex_params* p, int case_id, double amp, double* A
 
 static int check_params(const imex_params* prm) {
   if (prm->gmres_restart > MAXM) return set_err(IMEX_ERR_ARG, "gmres_restart exceeds 32");
-  if ((prm->bc_x != IMEX_BC_PERIODIC && prm->bc_x != IMEX_BC_TRANSMISSIVE) ||
-      (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE && prm->bc_y != IMEX_BC_JET_INFLOW))
-    return set_err(IMEX_ERR_UNSUPPORTED, "boundaries: periodic or transmissive, and the jet inflow on y");
+  if ((prm->bc_x != IMEX_BC_PERIODIC && prm->bc_x != IMEX_BC_TRANSMISSIVE && prm->bc_x != IMEX_BC_REFLECTIVE) ||
+      (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE && prm->bc_y != IMEX_BC_REFLECTIVE &&
+       prm->bc_y != IMEX_BC_JET_INFLOW))
+    return set_err(IMEX_ERR_UNSUPPORTED, "boundaries: periodic, transmissive or reflective, and the jet inflow on y");
   if (prm->bc_y == IMEX_BC_JET_INFLOW && !(prm->jet_rho > 0 && prm->jet_p > 0 && prm->jet_halfwidth > 0))
     return set_err(IMEX_ERR_ARG, "jet inflow state needs jet_rho, jet_p, jet_halfwidth > 0");
   if (prm->recon < IMEX_RECON_NONE || prm->recon > IMEX_RECON_WENO5)

@@ -52,7 +52,7 @@ struct PotOp {
 };
 // What the host passes: buffer ids and flags (pot: the vector-potential operator on the magnetic
 // slots, 3 unknowns per cell).
-struct MagIds { int Qe, u, pot; };
+struct MagIds { int Qe, u, pot, refl; };   // refl: the magnetic operator with reflective-wall flags
 struct PIds { int u, jacobian, precond, energy; };   // energy: Energy_Subsystem_AX on the same slots
 
 __device__ __forceinline__ MagOp make_mag_op(const Inst& I, const MagIds& m) {

@@ -118,8 +118,8 @@ __device__ __forceinline__ void mag_load(const MagOp& op, MagSmem<T>& sm, const 
 // Corner (a, b) lies between LDS cells a..a+1 and b..b+1.  Its six terms: n0 b1, n0 b2, n1 b0,
 // n1 b2, PC0 = |n.b|/2 - n0 b0, PC1 = |n.b|/2 - n1 b1 with n, b the four-cell averages of B^e
 // and of the unknown.
-template <class T>
-__device__ __forceinline__ void mag_corner(const MagSmem<T>& sm, int a, int b, T* cn) {
+template <class T, class SM>
+__device__ __forceinline__ void mag_corner(const SM& sm, int a, int b, T* cn) {
   T n[3], bb[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -136,8 +136,8 @@ __device__ __forceinline__ void mag_corner(const MagSmem<T>& sm, int a, int b, T
 }
 // Per-cell products of LDS cell (a, b): PM - q5 x5, PM - q6 x6, q6 x5, q6 x7, q5 x6, q5 x7
 // with PM = (q . x)/2.
-template <class T>
-__device__ __forceinline__ void mag_dv(const MagSmem<T>& sm, int a, int b, T* dv) {
+template <class T, class SM>
+__device__ __forceinline__ void mag_dv(const SM& sm, int a, int b, T* dv) {
   T q5 = sm.q[0][b][a], q6 = sm.q[1][b][a], q7 = sm.q[2][b][a];
   T x5 = sm.x[0][b][a], x6 = sm.x[1][b][a], x7 = sm.x[2][b][a];
   T q5x5 = q5 * x5, q6x6 = q6 * x6;
@@ -151,8 +151,8 @@ __device__ __forceinline__ void mag_dv(const MagSmem<T>& sm, int a, int b, T* dv
 }
 // Fluxes (gx, gz) through the y-face between LDS rows yb and yb+1 at column a; cL, cR are the
 // corners (a-1, yb), (a, yb) and D, U the cell products of (a, yb), (a, yb+1).
-template <class T>
-__device__ __forceinline__ void mag_yface(const MagSmem<T>& sm, int a, int yb, const T* cL, const T* cR,
+template <class T, class SM>
+__device__ __forceinline__ void mag_yface(const SM& sm, int a, int yb, const T* cL, const T* cR,
                                           const T* D, const T* U, T idx, T idy, T midx, T& gx, T& gz) {
   T rf = T(0.5) * (sm.r[yb][a] + sm.r[yb + 1][a]);
   T nf0 = T(0.5) * (sm.q[0][yb][a] + sm.q[0][yb + 1][a]);
@@ -164,8 +164,8 @@ __device__ __forceinline__ void mag_yface(const MagSmem<T>& sm, int a, int yb, c
 }
 // Fluxes (fy, fz) through the x-face between LDS columns xa and xa+1 at row b; cB, cT are the
 // corners (xa, b-1), (xa, b) and L, R the cell products of (xa, b), (xa+1, b).
-template <class T>
-__device__ __forceinline__ void mag_xface(const MagSmem<T>& sm, int xa, int b, const T* cB, const T* cT,
+template <class T, class SM>
+__device__ __forceinline__ void mag_xface(const SM& sm, int xa, int b, const T* cB, const T* cT,
                                           const T* L, const T* R, T idy, T midx, T& fy, T& fz) {
   T rf = T(0.5) * (sm.r[b][xa] + sm.r[b][xa + 1]);
   T nf0 = T(0.5) * (sm.q[0][b][xa] + sm.q[0][b][xa + 1]);
@@ -215,6 +215,74 @@ __device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem<T>& s
   out[0] = ax5 - h * lap[0];
   out[1] = ax6 - h * lap[1];
   out[2] = ax7 - h * lap[2];
+}
+
+// The same for a tile type SM (the reflective path's local neighbourhood).
+template <class T, class SM>
+__device__ __forceinline__ void mag_compute_at(const MagOp& op, const SM& sm, int a, int b, T* out) {
+  const Dims& d = op.d;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
+  const T idx = T(1) / dx, idy = T(1) / dy, midx = T(-1) / dx;
+  T c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
+  mag_corner(sm, a - 1, b - 1, c00);
+  mag_corner(sm, a, b - 1, c10);
+  mag_corner(sm, a - 1, b, c01);
+  mag_corner(sm, a, b, c11);
+  T dC[6], dN[6];
+  mag_dv(sm, a, b, dC);
+  T gxu, gzu, gxl, gzl, fyr, fzr, fyl, fzl;
+  mag_dv(sm, a, b + 1, dN);
+  mag_yface(sm, a, b, c01, c11, dC, dN, idx, idy, midx, gxu, gzu);
+  mag_dv(sm, a, b - 1, dN);
+  mag_yface(sm, a, b - 1, c00, c10, dN, dC, idx, idy, midx, gxl, gzl);
+  mag_dv(sm, a + 1, b, dN);
+  mag_xface(sm, a, b, c10, c11, dC, dN, idy, midx, fyr, fzr);
+  mag_dv(sm, a - 1, b, dN);
+  mag_xface(sm, a - 1, b, c00, c01, dN, dC, idy, midx, fyl, fzl);
+  auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
+  T ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
+  T ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
+  T ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
+  const T nu_x = (T)op.S[S_BMAXX] * dx, nu_y = (T)op.S[S_BMAXY] * dy;
+  T lap[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - T(2) * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - T(2) * X(c, 0, 0) + X(c, 0, 1));
+  out[0] = ax5 - h * lap[0];
+  out[1] = ax6 - h * lap[1];
+  out[2] = ax7 - h * lap[2];
+}
+
+// 3x3 neighbourhood of one cell, the unknown's neighbours multiplied by the reflective flags
+template <class T> struct MagLocal {
+  T x[3][3][3];
+  T q[3][3][3];
+  T r[3][3];
+};
+// Reflective boundaries (implicit_operators.cpp:146-211): the reference negates a neighbour's unknown
+// whenever the clamped neighbour index is the first (last) interior cell, i.e. at the boundary cell and
+// at the cell next to it; the centre is never negated.  Cells away from reflective walls take the tile
+// as it is.
+template <class T>
+__device__ __forceinline__ void mag_compute_refl(const MagOp& op, const MagSmem<T>& sm, int tx, int ty, int i, int j, bool valid,
+                                                 T* out) {
+  if (!valid) return;
+  const Dims& d = op.d;
+  const int a = tx + 1, b = ty + 1;
+  const T fxl = (d.refx && i <= GC + 1) ? T(-1) : T(1), fxr = (d.refx && i >= d.nx + GC - 2) ? T(-1) : T(1);
+  const T fyl = (d.refy && j <= GC + 1) ? T(-1) : T(1), fyr = (d.refy && j >= d.ny + GC - 2) ? T(-1) : T(1);
+  if (fxl > T(0) && fxr > T(0) && fyl > T(0) && fyr > T(0)) { mag_compute_at(op, sm, a, b, out); return; }
+  MagLocal<T> L;
+  const T fx[3] = {fxl, T(1), fxr}, fy[3] = {fyl, T(1), fyr};
+  for (int dj = 0; dj < 3; dj++)
+    for (int di = 0; di < 3; di++) {
+      for (int c = 0; c < 3; c++) {
+        L.x[c][dj][di] = sm.x[c][b + dj - 1][a + di - 1] * (fx[di] * fy[dj]);
+        L.q[c][dj][di] = sm.q[c][b + dj - 1][a + di - 1];
+      }
+      L.r[dj][di] = sm.r[b + dj - 1][a + di - 1];
+    }
+  mag_compute_at(op, L, 1, 1, out);
 }
 
 // ---------------------------------------------------------------- pressure operator
@@ -301,6 +369,14 @@ template <class T> struct OpSel<0, T> {
   __device__ static T center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
   __device__ static void precond(const MagOp&, int, int, int, T*) {}
   __device__ static const Dims& dims(const MagOp& op) { return op.d; }
+};
+// The magnetic operator with the reflective-wall neighbour flags: its own instantiation, so that the
+// flag-free operator keeps its register allocation
+template <class T> struct OpSel<4, T> : OpSel<0, T> {
+  typedef typename OpSel<0, T>::Smem Smem;
+  __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
+    mag_compute_refl(op, sm, tx, ty, i, j, valid, y);
+  }
 };
 template <class T> struct OpSel<1, T> {
   static constexpr int NC = 1;
@@ -495,7 +571,7 @@ __device__ __forceinline__ void final_sums(const double* partial, int stride, in
 // Always in FP64 (it is the outer residual of the mixed form's iterative refinement); r0 is
 // stored in the basis type VT, its norm is taken in FP64 before rounding.
 template <int OPK, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 || OPK == 4 ? IMEX_MAG_WAVES : 4)))
 k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int ib,
                  int outer, int x_is_zero, int skip, int mixed) {
   BATCH_BLOCK(bm);
@@ -594,7 +670,7 @@ __global__ void __launch_bounds__(256) k_gmres_residual_fin(const Inst* __restri
 // The dots of each tile are reduced over the wave right away (a uniform loop over i, so no
 // accumulator array is live across the operator) and summed per wave in LDS, in tile order.
 template <int OPK, class T, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 ? IMEX_MAG_WAVES : (sizeof(T) == 4 ? 8 : 4))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 || OPK == 4 ? IMEX_MAG_WAVES : (sizeof(T) == 4 ? 8 : 4))))
 k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -772,7 +848,7 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ 
 
 // ---------------------------------------------------------------- plain operator application (tests, adjoint)
 template <int OPK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 || OPK == 4 ? IMEX_MAG_WAVES : 4)))
 k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int iy) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -987,11 +1063,13 @@ void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const P
   const BMap& bm = L.bm[sh];
   if (prec == 0) {
     if (op == 0 && mo.pot) k_gmres_residual<3, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
+    else if (op == 0 && mo.refl) k_gmres_residual<4, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
     else if (op == 0) k_gmres_residual<0, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
     else if (po.energy) k_gmres_residual<2, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
     else k_gmres_residual<1, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 0);
   } else {
     if (op == 0 && mo.pot) k_gmres_residual<3, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
+    else if (op == 0 && mo.refl) k_gmres_residual<4, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
     else if (op == 0) k_gmres_residual<0, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
     else if (po.energy) k_gmres_residual<2, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
     else k_gmres_residual<1, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, ix, ib, outer, x_is_zero, skip, 1);
@@ -1006,11 +1084,13 @@ void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PI
   const BMap& bm = L.bm[sh];
   if (prec == 0) {
     if (op == 0 && mo.pot) k_arnoldi_apply<3, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else if (op == 0 && mo.refl) k_arnoldi_apply<4, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
     else if (op == 0) k_arnoldi_apply<0, double, double><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
     else if (po.energy) k_arnoldi_apply<2, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
     else k_arnoldi_apply<1, double, double><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
   } else {
     if (op == 0 && mo.pot) k_arnoldi_apply<3, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
+    else if (op == 0 && mo.refl) k_arnoldi_apply<4, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
     else if (op == 0) k_arnoldi_apply<0, float, float><<<GRID(L, sh), 256, 0, L.st>>>(mo, P, L.IT, bm, j);
     else if (po.energy) k_arnoldi_apply<2, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
     else k_arnoldi_apply<1, float, float><<<GRID(L, sh), 256, 0, L.st>>>(po, P, L.IT, bm, j);
@@ -1049,6 +1129,7 @@ void gmres_xupdate(const Launch& L, int op, int prec, int ix) {
 
 void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy) {
   if (op == 0 && mo.pot) k_op_apply<3><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
+  else if (op == 0 && mo.refl) k_op_apply<4><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
   else if (op == 0) k_op_apply<0><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(mo, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
   else if (po.energy) k_op_apply<2><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);
   else k_op_apply<1><<<GRID(L, SH_TILES_ALL), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_ALL], ix, iy);

@@ -15,17 +15,18 @@ struct FillMap { int lox, hix, loy, hiy; };   // identity range of the map
 // planes: buffer id b[p], plane k[p] of that buffer; jv[p] the plane's value in the jet inflow strip's
 // ghost rows: -1 = the copy (arrays), -2 = left as the producing kernel wrote it, 0..7 = conservative
 // slot, 8..15 = primitive slot (k - 8)
-struct FillPlanes { int b[8], k[8], n; signed char jv[8]; };
+// rn[p] = 1: the plane's reflective ghost is the negated mirror value (momenta, B, bound_cond_array arrays)
+struct FillPlanes { int b[8], k[8], n; signed char jv[8], rn[8]; };
 
 __device__ __forceinline__ int fill_mx(const Dims& d, int kind, int i) {
-  if (d.perx) return map_x(d, i);
+  if (d.perx || (kind == 0 && d.refx)) return map_x(d, i);
   int lo = (kind == 1) ? GC - 1 : GC;           // staggered x-normal keeps face g-1
   if (i < lo) return lo;
   if (i > d.nx + GC - 1) return d.nx + GC - 1;
   return i;
 }
 __device__ __forceinline__ int fill_my(const Dims& d, int kind, int j) {
-  if (d.pery) return map_y(d, j);
+  if (d.pery || (kind == 0 && d.refy)) return map_y(d, j);
   int lo = (kind == 2) ? GC - 1 : GC;           // staggered y-normal keeps face g-1
   if (j < lo) return lo;
   if (j > d.ny + GC - 1) return d.ny + GC - 1;
@@ -75,7 +76,7 @@ __global__ void k_bc_fill(const Inst* __restrict__ IT, BMap bm, FillPlanes fp, i
     if (jet_mode == 1 && fp.jv[p] >= 0) a[dst] = jet_state(d, fp.jv[p] >= 8, fp.jv[p] & 7);
     else if (jet_mode == 2) a[dst] = 0.0;
     else if (jet_mode == 3) a[dst] = d.jet_B0;
-    else a[dst] = a[src];
+    else a[dst] = kind == 0 ? refl_factor(d, i, j, fp.rn[p]) * a[src] : a[src];
   }
 }
 
@@ -544,7 +545,10 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
       if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, Lc, Rc, &ef);
       else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, Lc, Rc, &ef);
       for (int k = 0; k < 5; k++)
-        if (k < nrec) { wl[k * EX_NW + it] = Lc[k]; wr[k * EX_NW + it] = Rc[k]; }
+        if (k < nrec) {
+          const double rf = refl_factor(d, gi, gj, k != 0 && k != 4);   // bound_cond_u of the face states
+          wl[k * EX_NW + it] = rf * Lc[k]; wr[k * EX_NW + it] = rf * Rc[k];
+        }
       if (ef) atomicOr(eflag, 1);
     } else {
       for (int k = 0; k < nrec; k++) {
@@ -556,8 +560,9 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
         const int sel = std::is_same<T, float>::value && P.recon == IMEX_RECON_ENO3 ? eno3_stencil(a64) : -1;
         T L, R;
         recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R, sel);
-        wl[k * EX_NW + it] = L;
-        wr[k * EX_NW + it] = R;
+        const T rf = (T)refl_factor(d, gi, gj, k != 0 && k != 4);        // bound_cond_u of the face states
+        wl[k * EX_NW + it] = rf * L;
+        wr[k * EX_NW + it] = rf * R;
       }
     }
   }
@@ -698,9 +703,10 @@ __global__ void k_mag_writeback(const Inst* __restrict__ IT, BMap bm, int iu) {
     return;
   }
   int id = kidx(d, i, j);
-  u[5 * d.PS + c] = x[id];
-  u[6 * d.PS + c] = x[id + d.N];
-  u[7 * d.PS + c] = x[id + 2 * d.N];
+  const double rf = refl_factor(d, i, j, true);        // the bound_cond_u that follows: B negated
+  u[5 * d.PS + c] = rf * x[id];
+  u[6 * d.PS + c] = rf * x[id + d.N];
+  u[7 * d.PS + c] = rf * x[id + 2 * d.N];
 }
 
 // CollectMagFluxes (ct_flux.cpp:5-130): Fx at x faces (i+1/2, j), Fy at y faces (i, j+1/2),
@@ -751,13 +757,14 @@ __global__ void __launch_bounds__(256) k_emf_flux(const Inst* __restrict__ IT, B
   double By_2 = Bxn_p0 / rho_p0 * (-1.0 / dx * (Q(i + 1, j, 5) * U(i + 1, j, 6) - Q(i, j, 5) * U(i, j, 6)) + 1.0 / dy * (PC(n_pp, b_pp, 1) - PC(n_pm, b_pm, 1)));
   double nu_x = S[S_BMAXX] * dx;
   double Fdiff_x = -nu_x * (U(i + 1, j, 6) - U(i, j, 6)) / dx;
-  Fx[pidx(d, ii, jj)] = Fn_x + h * (By_1 + By_2) + Fdiff_x;
+  const double rf = refl_factor(d, ii, jj, true);      // bound_cond_array of the flux arrays: reflective negation
+  Fx[pidx(d, ii, jj)] = rf * (Fn_x + h * (By_1 + By_2) + Fdiff_x);
   double Fn_y = Bxn_0p / rho_0p * (rvy_0p - (h / dy) * (PE(i, j + 1) - p00)) - Byn_0p / rho_0p * (rvx_0p - (h / dx) * (p_pp - p_mp));
   double Bx_1 = Bxn_0p / rho_0p * (1.0 / dx * (n_pp[0] * b_pp[1] - n_mp[0] * b_mp[1]) - 1.0 / dy * ((PM(i, j + 1) - Q(i, j + 1, 6) * U(i, j + 1, 6)) - (PM(i, j) - Q(i, j, 6) * U(i, j, 6))));
   double Bx_2 = Byn_0p / rho_0p * (1.0 / dx * (PC(n_pp, b_pp, 0) - PC(n_mp, b_mp, 0)) - 1.0 / dy * (Q(i, j + 1, 6) * U(i, j + 1, 5) - Q(i, j, 6) * U(i, j, 5)));
   double nu_y = S[S_BMAXY] * dy;
   double Fdiff_y = -nu_y * (U(i, j + 1, 5) - U(i, j, 5)) / dy;
-  Fy[pidx(d, ii, jj)] = Fn_y + h * (Bx_1 + Bx_2) + Fdiff_y;
+  Fy[pidx(d, ii, jj)] = rf * (Fn_y + h * (Bx_1 + Bx_2) + Fdiff_y);
 }
 
 // ================================================================ pressure subsystem pointwise/5-point kernels
@@ -944,7 +951,8 @@ __global__ void k_ct_corner(const Inst* __restrict__ IT, BMap bm, int iex, int i
   auto EX = [&](int a, int b) { return sx * ezx[pidx(d, a, b)]; };
   auto EY = [&](int a, int b) { return sy * ezy[pidx(d, a, b)]; };
   int I = map_x(d, i), J = map_y(d, j);
-  double bar = 0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J));
+  // Ez_bar is formed inside and filled by bound_cond_array (constrained_transport.cpp:148-154)
+  double bar = refl_factor(d, i, j, true) * (0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J)));
   const double dx = d.dx, dy = d.dy;
   double e00 = ez_tilde(d, u, i, j), e10 = ez_tilde(d, u, i + 1, j), e01 = ez_tilde(d, u, i, j + 1), e11 = ez_tilde(d, u, i + 1, j + 1);
   double t1 = 0.5 * ((EX(i, j) - e00) / (dx / 2.0) + (EX(i, j + 1) - e01) / (dx / 2.0));
@@ -989,8 +997,9 @@ __global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int 
   by[c] = (j < GC - 1 && jet_inlet(d, i)) ? d.jet_B0 : BY(i, j);
   if (j < GC && jet_inlet(d, i)) { u[5 * d.PS + c] = 0.0; u[6 * d.PS + c] = d.jet_B0; return; }
   int I = map_x(d, i), J = map_y(d, j);
-  u[5 * d.PS + c] = 0.5 * (BX(I - 1, J) + BX(I, J));
-  u[6 * d.PS + c] = 0.5 * (BY(I, J - 1) + BY(I, J));
+  const double rf = refl_factor(d, i, j, true);        // bound_cond_u: B negated across a reflective wall
+  u[5 * d.PS + c] = rf * (0.5 * (BX(I - 1, J) + BX(I, J)));
+  u[6 * d.PS + c] = rf * (0.5 * (BY(I, J - 1) + BY(I, J)));
 }
 
 // Reconstructed face states of rho, m_x, m_y, B_x, B_y (the EMF terms read no other component) on
@@ -1038,8 +1047,9 @@ __global__ void __launch_bounds__(256) k_ct_plain_emf(Phys P, const Inst* __rest
   // components in w: 0 rho, 1 m_x, 2 m_y, 3 B_x, 4 B_y
   auto fx6 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[4] * vx - vy * Q[3]; };
   auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[3] * vy - vx * Q[4]; };
-  ezx[pidx(d, ii, jj)] = 0.5 * (fx6(w[0][0]) + fx6(w[0][1]));   // Ez_p0 = -ezx
-  ezy[pidx(d, ii, jj)] = 0.5 * (gy5(w[1][0]) + gy5(w[1][1]));   // Ez_0p = +ezy
+  const double rf = refl_factor(d, ii, jj, true);              // bound_cond_array of the face EMFs
+  ezx[pidx(d, ii, jj)] = rf * (0.5 * (fx6(w[0][0]) + fx6(w[0][1])));   // Ez_p0 = -ezx
+  ezy[pidx(d, ii, jj)] = rf * (0.5 * (gy5(w[1][0]) + gy5(w[1][1])));   // Ez_0p = +ezy
 }
 
 // ================================================================ per-step diagnostics (main.cpp:562-592)
@@ -1144,11 +1154,12 @@ __global__ void __launch_bounds__(256) k_ctd_faces(Phys P, const Inst* __restric
   auto gy5 = [](const double* Q) { double vx = Q[1] / Q[0], vy = Q[2] / Q[0]; return Q[3] * vy - vx * Q[4]; };
   const double *L = w[0][0], *R = w[0][1];
   long long c = pidx(d, ii, jj);
-  gmem(I_.buf[B_EZX])[c] = -(0.5 * (fx6(L) + fx6(R)));
-  gmem(I_.buf[B_MXS])[c] = 0.5 * fmax(fabs(L[1] / L[0]), fabs(R[1] / R[0])) * (R[4] - L[4]);
+  const double rf = refl_factor(d, ii, jj, true);              // bound_cond_array of the face arrays
+  gmem(I_.buf[B_EZX])[c] = rf * (-(0.5 * (fx6(L) + fx6(R))));
+  gmem(I_.buf[B_MXS])[c] = rf * (0.5 * fmax(fabs(L[1] / L[0]), fabs(R[1] / R[0])) * (R[4] - L[4]));
   L = w[1][0]; R = w[1][1];
-  gmem(I_.buf[B_EZY])[c] = 0.5 * (gy5(L) + gy5(R));
-  gmem(I_.buf[B_MYS])[c] = -0.5 * fmax(fabs(L[2] / L[0]), fabs(R[2] / R[0])) * (R[3] - L[3]);
+  gmem(I_.buf[B_EZY])[c] = rf * (0.5 * (gy5(L) + gy5(R)));
+  gmem(I_.buf[B_MYS])[c] = rf * (-0.5 * fmax(fabs(L[2] / L[0]), fabs(R[2] / R[0])) * (R[3] - L[3]));
 }
 
 // CT_CornerAssemble (constrained_transport.cpp:147-164) of the three parts: central and Rusanov faces
@@ -1166,7 +1177,7 @@ __global__ void k_ctd_corners(const Inst* __restrict__ IT, BMap bm, int iu) {
     auto EX = [&](int a, int b) { return ex ? ex[pidx(d, a, b)] : 0.0; };
     auto EY = [&](int a, int b) { return ey ? ey[pidx(d, a, b)] : 0.0; };
     auto ET = [&](int a, int b) { return tilde ? ez_tilde(d, u, a, b) : 0.0; };
-    double bar = 0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J));
+    double bar = refl_factor(d, i, j, true) * (0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J)));
     double e00 = ET(i, j), e10 = ET(i + 1, j), e01 = ET(i, j + 1), e11 = ET(i + 1, j + 1);
     double t1 = 0.5 * ((EX(i, j) - e00) / (dx / 2.0) + (EX(i, j + 1) - e01) / (dx / 2.0));
     double t2 = 0.5 * ((e10 - EX(i, j)) / (dx / 2.0) + (e11 - EX(i, j + 1)) / (dx / 2.0));
@@ -1313,6 +1324,10 @@ void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind
     fp.b[p] = b[p]; fp.k[p] = k[p];
     // Grids take the inflow state (primitive in B_UPRI), arrays the copy, unless the caller says otherwise
     fp.jv[p] = jv ? jv[p] : (b[p] >= B_UPRI && b[p] <= B_QS ? (signed char)(k[p] + (b[p] == B_UPRI ? 8 : 0)) : (signed char)-1);
+    // reflective sign class: grid slots other than rho and E (p); arrays negate; a plane formed from each
+    // cell's own state (jv -2: the enthalpy ratio, even in B and v) does not
+    const bool grid = b[p] >= B_UPRI && b[p] <= B_QS;
+    fp.rn[p] = fp.jv[p] == -2 ? 0 : (grid ? (k[p] != 0 && k[p] != 4) : 1);
   }
   const int sh = kind == 0 ? SH_FILL0 : (kind == 1 ? SH_FILL1 : SH_FILL2);
   k_bc_fill<<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], fp, kind);

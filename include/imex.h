@@ -29,8 +29,8 @@ extern "C" {
 #endif
 
 /* ---- enumerations (values follow the reference enums in include/config.hpp:27-41) ---- */
-enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_PERIODIC = 2,              /* x/y_boundary_condition (config.hpp:36-37) */
-       IMEX_BC_JET_INFLOW = 4 };                                     /* y only: y_JetInflow */
+enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_REFLECTIVE = 1,           /* x/y_boundary_condition (config.hpp:36-37) */
+       IMEX_BC_PERIODIC = 2, IMEX_BC_JET_INFLOW = 4 };               /* JET_INFLOW: y only (y_JetInflow) */
 enum { IMEX_RECON_NONE = 0, IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_ENO3 = 3,
        IMEX_RECON_WENO5 = 4 };                                        /* Reconstruction (config.hpp:30) */
 enum { IMEX_RECONSTATE_PRIM = 0, IMEX_RECONSTATE_CONS = 1 };         /* ReconState (config.hpp:41) */

@@ -218,12 +218,17 @@ void grid_pri2con(Ctx& C, Vec& con, const Vec& pri) {               // state.cpp
 // ---------------------------------------------------------------- boundary conditions
 // bound_cond_u (boundary.cpp:5-175), periodic and transmissive branches: x pass over all rows, then y pass over all columns.
 // bound_cond_u (boundary.cpp:5-175); prim = g_bc_prim (the grid holds primitives: slot 4 is p)
+// Reflective (:17-54, 89-126): the mirror cell, momenta and B negated (primitive velocities alike).
+inline double refl_sign(int k) { return (k == 0 || k == 4) ? 1.0 : -1.0; }
 void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
   const int nx = C.nx, ny = C.ny;
   for (int j = 0; j < C.NY; j++)
     for (int g = 0; g < GC; g++)
       for (int k = 0; k < NV; k++) {
-        if (C.P.bc_x == IMEX_BC_PERIODIC) {
+        if (C.P.bc_x == IMEX_BC_REFLECTIVE) {
+          G(u, C, g, j, k) = refl_sign(k) * G(u, C, 2 * GC - 1 - g, j, k);
+          G(u, C, nx + GC + g, j, k) = refl_sign(k) * G(u, C, nx + GC - 1 - g, j, k);
+        } else if (C.P.bc_x == IMEX_BC_PERIODIC) {
           G(u, C, g, j, k) = G(u, C, nx + g, j, k);
           G(u, C, nx + GC + g, j, k) = G(u, C, GC + g, j, k);
         } else {
@@ -234,7 +239,10 @@ void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
   for (int i = 0; i < C.NX; i++)
     for (int g = 0; g < GC; g++)
       for (int k = 0; k < NV; k++) {
-        if (C.P.bc_y == IMEX_BC_PERIODIC) {
+        if (C.P.bc_y == IMEX_BC_REFLECTIVE) {
+          G(u, C, i, g, k) = refl_sign(k) * G(u, C, i, 2 * GC - 1 - g, k);
+          G(u, C, i, ny + GC + g, k) = refl_sign(k) * G(u, C, i, ny + GC - 1 - g, k);
+        } else if (C.P.bc_y == IMEX_BC_PERIODIC) {
           G(u, C, i, g, k) = G(u, C, i, ny + g, k);
           G(u, C, i, ny + GC + g, k) = G(u, C, i, GC + g, k);
         } else {
@@ -253,17 +261,19 @@ void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
       }
     }
 }
-// bound_cond_array (timestep.cpp:166-222)
+// bound_cond_array (timestep.cpp:166-222); reflective: the mirror value negated, for every array
 void bc_array(const Ctx& C, Vec& u) {
   const int nx = C.nx, ny = C.ny;
   for (int j = 0; j < C.NY; j++)
     for (int g = 0; g < GC; g++) {
-      if (C.P.bc_x == IMEX_BC_PERIODIC) { A(u, C, g, j) = A(u, C, nx + g, j); A(u, C, nx + GC + g, j) = A(u, C, GC + g, j); }
+      if (C.P.bc_x == IMEX_BC_REFLECTIVE) { A(u, C, g, j) = -A(u, C, 2 * GC - 1 - g, j); A(u, C, nx + GC + g, j) = -A(u, C, nx + GC - 1 - g, j); }
+      else if (C.P.bc_x == IMEX_BC_PERIODIC) { A(u, C, g, j) = A(u, C, nx + g, j); A(u, C, nx + GC + g, j) = A(u, C, GC + g, j); }
       else { A(u, C, g, j) = A(u, C, GC, j); A(u, C, nx + GC + g, j) = A(u, C, nx + GC - 1, j); }
     }
   for (int i = 0; i < C.NX; i++)
     for (int g = 0; g < GC; g++) {
-      if (C.P.bc_y == IMEX_BC_PERIODIC) { A(u, C, i, g) = A(u, C, i, ny + g); A(u, C, i, ny + GC + g) = A(u, C, i, GC + g); }
+      if (C.P.bc_y == IMEX_BC_REFLECTIVE) { A(u, C, i, g) = -A(u, C, i, 2 * GC - 1 - g); A(u, C, i, ny + GC + g) = -A(u, C, i, ny + GC - 1 - g); }
+      else if (C.P.bc_y == IMEX_BC_PERIODIC) { A(u, C, i, g) = A(u, C, i, ny + g); A(u, C, i, ny + GC + g) = A(u, C, i, GC + g); }
       else { A(u, C, i, g) = A(u, C, i, GC); A(u, C, i, ny + GC + g) = A(u, C, i, ny + GC - 1); }
     }
 }
@@ -306,15 +316,17 @@ void bc_staggered(const Ctx& C, Vec& u, int axis) {
     }
   }
 }
-// M_P / M_B index maps (implicit_operators.cpp:5-139), periodic and transmissive branches.
+// M_P / M_B index maps (implicit_operators.cpp:5-139): periodic wrap, reflective mirror, clamp otherwise.
 inline int mapx(const Ctx& C, int i) {
-  if (i < GC) return C.P.bc_x == IMEX_BC_PERIODIC ? C.nx + i : GC;
-  if (i > C.nx + GC - 1) return C.P.bc_x == IMEX_BC_PERIODIC ? i - C.nx : C.nx + GC - 1;
+  if (i < GC) return C.P.bc_x == IMEX_BC_PERIODIC ? C.nx + i : (C.P.bc_x == IMEX_BC_REFLECTIVE ? 2 * GC - 1 - i : GC);
+  if (i > C.nx + GC - 1)
+    return C.P.bc_x == IMEX_BC_PERIODIC ? i - C.nx : (C.P.bc_x == IMEX_BC_REFLECTIVE ? 2 * (C.nx + GC) - i - 1 : C.nx + GC - 1);
   return i;
 }
 inline int mapy(const Ctx& C, int j) {
-  if (j < GC) return C.P.bc_y == IMEX_BC_PERIODIC ? C.ny + j : GC;
-  if (j > C.ny + GC - 1) return C.P.bc_y == IMEX_BC_PERIODIC ? j - C.ny : C.ny + GC - 1;
+  if (j < GC) return C.P.bc_y == IMEX_BC_PERIODIC ? C.ny + j : (C.P.bc_y == IMEX_BC_REFLECTIVE ? 2 * GC - 1 - j : GC);
+  if (j > C.ny + GC - 1)
+    return C.P.bc_y == IMEX_BC_PERIODIC ? j - C.ny : (C.P.bc_y == IMEX_BC_REFLECTIVE ? 2 * (C.ny + GC) - j - 1 : C.ny + GC - 1);
   return j;
 }
 inline int M_P(const Ctx& C, int i, int j) { return (mapx(C, i) - GC) + C.nx * (mapy(C, j) - GC); }
@@ -573,11 +585,23 @@ void flux_update(Ctx& C, const imex_scalars& S, Vec& u_con, const Vec& Qs, const
 void mag_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, const double* x, double* Ax) {
   const double h = S.adt, dx = C.dx, dy = C.dy;
   if (C.faithful) { Vec pri(Qe.size()); grid_con2pri(C, pri, Qe); }   // loop-invariant work the reference repeats (:143-144)
-  auto X = [&](int a, int b, int k) { return x[M_B(C, a, b, k)]; };
+  // reflective boundaries (:146-211): a neighbour's unknown is negated when the clamped neighbour index
+  // is the first or last interior cell -- at the boundary cell and at the cell next to it
+  const bool rx = C.P.bc_x == IMEX_BC_REFLECTIVE, ry = C.P.bc_y == IMEX_BC_REFLECTIVE;
+  int i = 0, j = 0;
+  double fxl = 1, fxr = 1, fyl = 1, fyr = 1;
+  auto X = [&](int a, int b, int k) {
+    double f = (a < i ? fxl : a > i ? fxr : 1.0) * (b < j ? fyl : b > j ? fyr : 1.0);
+    return x[M_B(C, a, b, k)] * f;
+  };
   auto Q = [&](int a, int b, int k) { return G(Qe, C, a, b, k); };
   auto R = [&](int a, int b) { return G(u_con, C, a, b, 0); };
-  for (int i = GC; i < C.nx + GC; i++)
-    for (int j = GC; j < C.ny + GC; j++) {
+  for (i = GC; i < C.nx + GC; i++)
+    for (j = GC; j < C.ny + GC; j++) {
+      fxl = (rx && std::max(GC, i - 1) == GC) ? -1.0 : 1.0;
+      fxr = (rx && std::min(C.nx + GC - 1, i + 1) == C.nx + GC - 1) ? -1.0 : 1.0;
+      fyl = (ry && std::max(GC, j - 1) == GC) ? -1.0 : 1.0;
+      fyr = (ry && std::min(C.ny + GC - 1, j + 1) == C.ny + GC - 1) ? -1.0 : 1.0;
       double rho_p0 = 0.5 * (R(i, j) + R(i + 1, j)), rho_m0 = 0.5 * (R(i, j) + R(i - 1, j));
       double rho_0p = 0.5 * (R(i, j) + R(i, j + 1)), rho_0m = 0.5 * (R(i, j) + R(i, j - 1));
       double n_p0[3], n_m0[3], n_0p[3], n_0m[3], n_pp[3], n_pm[3], n_mp[3], n_mm[3];

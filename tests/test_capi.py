@@ -134,3 +134,18 @@ def test_jet_initial_data():
     A = im.init_potential(p, im.CASE_JET, 1.0)
     xc = -0.5 + (np.arange(p.nx + 2 * G) - G + 0.5) * dx
     assert np.allclose(A[..., 2], np.broadcast_to(-p.jet_B0 * xc, A[..., 2].shape), rtol=0, atol=1e-12)
+
+
+def test_reflective_initial_ghosts():
+    """The library's host fill of the initial primitives under reflective walls (boundary.cpp:17-54,
+    89-126): mirror cells with the velocities and B negated, x pass then y pass."""
+    p = im.blast_params(12)
+    p.bc_x = p.bc_y = im.BC_REFLECTIVE
+    u, bx, by = im.init_case(p, im.CASE_BLAST, 100.0, 0.0, 0.0)
+    G, n = im.GC, 12
+    m = lambda i: 2 * G - 1 - i if i < G else (2 * (n + G) - 1 - i if i >= n + G else i)
+    g = lambda i: i < G or i >= n + G
+    sgn = np.array([1, -1, -1, -1, 1, -1, -1, -1.0])
+    for j in range(n + 2 * G):
+        for i in range(n + 2 * G):
+            assert np.array_equal(u[j, i], sgn ** (int(g(i)) + int(g(j))) * u[m(j), m(i)]), (i, j)

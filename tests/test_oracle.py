@@ -1073,3 +1073,91 @@ def test_jet_run_injects_mass_symmetrically():
         assert np.allclose(up[G:-G, G:-G, 1], -up[G:-G, G:-G, 1][:, ::-1], rtol=0, atol=1e-9 * np.abs(up[..., 1]).max())
         assert np.all(up[:G, cells][..., 2] == p.jet_v)
         assert o.stats()["div_b"] <= 1e-15 * p.jet_B0 / dx
+
+
+# ---------------------------------------------------------------- reflective walls (x_Reflective / y_Reflective)
+def test_reflective_ghost_fills_and_maps_brute_force():
+    """bound_cond_u / bound_cond_array (boundary.cpp:17-54, 89-126; timestep.cpp:175-214) and M_P
+    (implicit_operators.cpp:14-28, 42-56): mirror cells, momenta and B negated in a Grid, every Array
+    negated, x pass then y pass (a corner takes both signs); the staggered fill and bound_cond_A are those
+    of an outflow."""
+    p = ideal(10, 8, bc=im.BC_REFLECTIVE)
+    q = ideal(10, 8, bc=im.BC_TRANSMISSIVE)
+    NX, NY = p.nx + 2 * G, p.ny + 2 * G
+    rng = np.random.default_rng(12)
+    a = rng.uniform(0.5, 2.0, (NY, NX, 8))
+    mx = lambda i: 2 * G - 1 - i if i < G else (2 * (p.nx + G) - 1 - i if i >= p.nx + G else i)
+    my = lambda j: 2 * G - 1 - j if j < G else (2 * (p.ny + G) - 1 - j if j >= p.ny + G else j)
+    gx = lambda i: i < G or i >= p.nx + G
+    gy = lambda j: j < G or j >= p.ny + G
+    sgn = np.array([1, -1, -1, -1, 1, -1, -1, -1.0])
+    for kind in (0, 4):
+        got = orc.bc(p, a, kind)
+        for j in range(NY):
+            for i in range(NX):
+                s = sgn ** (int(gx(i)) + int(gy(j)))
+                assert np.array_equal(got[j, i], s * a[my(j), mx(i)]), (i, j)
+    b = rng.uniform(0.5, 2.0, (NY, NX))
+    got = orc.bc(p, b, 1)
+    for j in range(NY):
+        for i in range(NX):
+            assert got[j, i] == (-1.0) ** (int(gx(i)) + int(gy(j))) * b[my(j), mx(i)]
+    for kind in (2, 3):
+        assert np.array_equal(orc.bc(p, b, kind), orc.bc(q, b, kind))
+    A = rng.uniform(-1, 1, (NY, NX, 8))
+    assert np.array_equal(orc.bc_potential(p, A), orc.bc_potential(q, A))
+    for j in range(NY):
+        for i in range(NX):
+            assert orc.map_index(p, i, j) == (mx(i) - G) + p.nx * (my(j) - G)
+
+
+def test_reflective_magnetic_operator_flags():
+    """Magnetic_Subsystem_AX's reflective flags (implicit_operators.cpp:146-211) negate only neighbours whose
+    clamped index is a boundary cell: with the unknown zero on the first and last interior rows and columns
+    the operator equals the outflow one, and in general the two differ only on the two outer layers."""
+    p, u, bx, by = make_case("blast", 16, 12)
+    p.bc_x = p.bc_y = im.BC_REFLECTIVE
+    con = orc.bc(p, orc.pri2con(p, u), 0)
+    st = _stage_from(p, con)
+    sc = dict(adt=0.002, a_max=30.0, dt=0.004, B_max_x=20.0, B_max_y=20.0, vx_max=1.0, vy_max=1.0)
+    q = im.blast_params(16)
+    q.ny = 12
+    q.bc_x = q.bc_y = im.BC_TRANSMISSIVE
+    rng = np.random.default_rng(2)
+    N = p.nx * p.ny
+    x = rng.standard_normal(3 * N)
+    a, b = orc.apply_op(p, sc, st, im.OP_MAG, x), orc.apply_op(q, sc, st, im.OP_MAG, x)
+    d = np.abs(a - b).reshape(3, p.ny, p.nx)
+    inner = np.zeros((p.ny, p.nx), bool)
+    inner[2:-2, 2:-2] = True
+    assert np.all(d[:, inner] == 0) and d.max() > 0
+    xz = x.reshape(3, p.ny, p.nx).copy()
+    xz[:, 0, :] = xz[:, -1, :] = xz[:, :, 0] = xz[:, :, -1] = 0
+    xz = xz.ravel()
+    assert np.array_equal(orc.apply_op(p, sc, st, im.OP_MAG, xz), orc.apply_op(q, sc, st, im.OP_MAG, xz))
+
+
+def test_reflective_box_keeps_the_mirror_symmetries():
+    """A field-free blast in a reflective box keeps its mirror symmetries (x -> -x, y -> -y, x <-> y) to
+    round-off after its wave has reached the walls, and a state at rest with B = 0 stays unchanged.  (With a
+    field the reference's staggered CT is not mirror symmetric at reflective walls: the corner of the left
+    wall is a ghost index, filled by bound_cond_array's negation, that of the right wall an interior one.)"""
+    p, u, bx, by = make_case("blast", 16, 16, amp=0.0)
+    p.bc_x = p.bc_y = im.BC_REFLECTIVE
+    o = orc.Oracle(p)
+    o.set_state(orc.bc(p, u, 4), bx, by)
+    o.step(10)
+    r = o.get_state()[1][G:-G, G:-G]
+    s0, s1 = np.abs(r[..., 0]).max(), np.abs(r[..., 1]).max()
+    assert np.abs(r[:, 0, 1]).max() > 1e-3 * s1                              # the wave reached the walls
+    assert np.allclose(r[..., 0], r[:, ::-1, 0], rtol=0, atol=1e-13 * s0)
+    assert np.allclose(r[..., 0], r[::-1, :, 0], rtol=0, atol=1e-13 * s0)
+    assert np.allclose(r[..., 0], r[..., 0].T, rtol=0, atol=1e-13 * s0)
+    assert np.allclose(r[..., 1], -r[:, ::-1, 1], rtol=0, atol=1e-12 * s1)
+    assert np.allclose(r[..., 1], r[..., 2].T, rtol=0, atol=1e-12 * s1)
+    q = ideal(12, 10, bc=im.BC_REFLECTIVE)
+    pri = np.tile([1.3, 0.0, 0.0, 0.0, 0.7, 0.0, 0.0, 0.0], (q.ny + 2 * G, q.nx + 2 * G, 1))
+    o = orc.Oracle(q)
+    o.set_state(pri, np.zeros(pri.shape[:2]), np.zeros(pri.shape[:2]))
+    o.step(2)
+    assert np.array_equal(o.get_state()[1][G:-G, G:-G], pri[G:-G, G:-G])
