@@ -10,7 +10,7 @@ from .capi import (  # noqa: F401
     CT_CONSISTENT, CT_PLAIN, PREC_FP64, PREC_MIXED, OP_MAG, OP_P_CARRIER, OP_P_JACOBIAN, OP_ENERGY,
     FORMULA_ENERGY, FORMULA_PRESSURE, MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A,
     MAGFIELD_NODIVFREE, OP_POTENTIAL, init_potential, RECON_NONE, RECON_ENO3, RECONSTATE_PRIM, RECONSTATE_CONS, write_vtk_host, write_fields_host, write_history, output_path,
-    CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, CASE_JET, BC_JET_INFLOW, BC_REFLECTIVE, jet_params,
+    CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, CASE_JET, BC_JET_INFLOW, BC_REFLECTIVE, BC_DIRICHLET, jet_params,
     SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST, GC, NVAR,
     TUNE_EXPLICIT, TUNE_KRYLOV,
 )

@@ -11,7 +11,7 @@ import numpy as np
 GC = 3
 NVAR = 8
 
-BC_TRANSMISSIVE, BC_REFLECTIVE, BC_PERIODIC, BC_JET_INFLOW = 0, 1, 2, 4
+BC_TRANSMISSIVE, BC_REFLECTIVE, BC_PERIODIC, BC_DIRICHLET, BC_JET_INFLOW = 0, 1, 2, 3, 4
 RECON_MINMOD, RECON_VANLEER, RECON_WENO5 = 1, 2, 4
 EOS_IDEAL, EOS_RK = 0, 1
 CT_CONSISTENT, CT_PLAIN = 0, 1
@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("schedule", C.c_int), ("budget_margin", C.c_int), ("mixed_pressure", C.c_int),
         ("formula", C.c_int), ("recon_state", C.c_int), ("magfield", C.c_int),
         ("jet_rho", C.c_double), ("jet_v", C.c_double), ("jet_B0", C.c_double), ("jet_p", C.c_double),
-        ("jet_halfwidth", C.c_double),
+        ("jet_halfwidth", C.c_double), ("rho_amb", C.c_double), ("p_amb", C.c_double),
     ]
 
 

@@ -28,6 +28,8 @@ struct Dims {
   double dx, dy;
   int perx, pery;         // 1 = periodic, 0 = transmissive
   int refx, refy;         // 1 = reflective (x_Reflective / y_Reflective)
+  int dirx, diry;         // 1 = Dirichlet ambient (x_Dirichlet / y_Dirichlet)
+  double rho_amb, p_amb, e_amb;   // ambient density and pressure; e_amb = p_amb / (gamma - 1)
   double x0, y0;          // domain origin (cell centres at x0 + (i - g + 1/2) dx)
   int jet;                // 1 = bottom boundary is the MHD jet inflow (y_JetInflow)
   double jet_hw, jet_rho, jet_v, jet_B0, jet_p, jet_E;   // strip half-width and inflow state; jet_E = jet_E_val()
@@ -76,6 +78,14 @@ __device__ __forceinline__ int map_y(const Dims& d, int j) {
   if (j < GC) return d.pery ? d.ny + j : (d.refy ? 2 * GC - 1 - j : GC);
   if (j > d.ny + GC - 1) return d.pery ? j - d.ny : (d.refy ? 2 * (d.ny + GC) - j - 1 : d.ny + GC - 1);
   return j;
+}
+// A ghost cell on a Dirichlet side (x_Dirichlet / y_Dirichlet): it holds the ambient state
+__host__ __device__ inline bool dir_ghost(const Dims& d, int i, int j) {
+  return (d.dirx && (i < GC || i > d.nx + GC - 1)) || (d.diry && (j < GC || j > d.ny + GC - 1));
+}
+// A boundary cell of the pressure solve's strong Dirichlet rows (imex_subsystems.cpp:161, implicit_operators.cpp:550)
+__host__ __device__ inline bool dir_bnd(const Dims& d, int i, int j) {
+  return (d.dirx && (i == GC || i == d.nx + GC - 1)) || (d.diry && (j == GC || j == d.ny + GC - 1));
 }
 // Reflective ghost fills negate (boundary.cpp:17-54, 89-126; timestep.cpp:175-214): the factor a ghost
 // value of sign class `neg` (1: momenta, B and every bound_cond_array array) takes at padded (i, j).

@@ -244,6 +244,8 @@ static Dims make_dims(const imex_params& p) {
   d.dy = (p.y1 - p.y0) / ((p.ny + 1) - 1);
   d.perx = p.bc_x == IMEX_BC_PERIODIC; d.pery = p.bc_y == IMEX_BC_PERIODIC;
   d.refx = p.bc_x == IMEX_BC_REFLECTIVE; d.refy = p.bc_y == IMEX_BC_REFLECTIVE;
+  d.dirx = p.bc_x == IMEX_BC_DIRICHLET; d.diry = p.bc_y == IMEX_BC_DIRICHLET;
+  d.rho_amb = p.rho_amb; d.p_amb = p.p_amb; d.e_amb = p.p_amb / (p.gamma - 1.0);   // config.hpp:247, boundary.cpp:67-69
   d.x0 = p.x0; d.y0 = p.y0;
   d.jet = p.bc_y == IMEX_BC_JET_INFLOW;
   d.jet_hw = p.jet_halfwidth; d.jet_rho = p.jet_rho; d.jet_v = p.jet_v; d.jet_B0 = p.jet_B0; d.jet_p = p.jet_p;
@@ -419,6 +421,14 @@ static void fill_planes(imex_ctx* c, std::initializer_list<std::pair<int, int>> 
                         const signed char* jv = nullptr) {
   int b[8], k[8], n = 0;
   for (auto& pk : planes) { b[n] = pk.first; k[n] = pk.second; n++; }
+  // a Dirichlet ghost's E follows its B (bound_cond_u refills the whole cell): a fill of a Grid's B
+  // planes refills its E plane too
+  const bool dirichlet = c->prm.bc_x == IMEX_BC_DIRICHLET || c->prm.bc_y == IMEX_BC_DIRICHLET;
+  if (dirichlet && !jv && n < 8 && b[0] >= B_UPRI && b[0] <= B_QS) {
+    bool hasB = false, hasE = false;
+    for (int q = 0; q < n; q++) { hasB |= k[q] >= 5; hasE |= k[q] == 4; }
+    if (hasB && !hasE) { b[n] = b[0]; k[n] = 4; n++; }
+  }
   int e0 = prof_begin(c);
   launch_bc_fill(c->L, n, b, k, kind, jv);
   prof_end(c, e0, KC_BC, 0.0);
@@ -1268,8 +1278,19 @@ static void hfill(const imex_params& p, double* a, int nvar, int kind, int prim 
       int sj = my(j);
       if (sj != j) for (int k = 0; k < nvar; k++) a[(size_t)nvar * (i + j * NX) + k] = sg(k, ry) * a[(size_t)nvar * (i + sj * NX) + k];
     }
-  if (p.bc_y != IMEX_BC_JET_INFLOW) return;
   const Dims d = make_dims(p);
+  if (kind == 0 && nvar == NVAR && (d.dirx || d.diry)) {      // Dirichlet ambient (boundary.cpp:56-76, 128-147)
+    auto amb = [&](int i, int j) {
+      double* q = &a[(size_t)NVAR * (i + j * NX)];
+      q[0] = d.rho_amb; q[1] = q[2] = q[3] = 0.0;
+      q[4] = prim ? d.p_amb : d.e_amb + 0.5 * (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]);
+    };
+    // the passes above copied B (outflow rule); rho, v and slot 4 of the ghosts take the ambient state
+    for (int j = 0; j < NY; j++)
+      for (int i = 0; i < NX; i++)
+        if (dir_ghost(d, i, j)) amb(i, j);
+  }
+  if (p.bc_y != IMEX_BC_JET_INFLOW) return;
   for (int i = 0; i < NX; i++) {
     if (kind == 0 && nvar == NVAR && jet_inlet(d, i))
       for (int j = 0; j < GC; j++)
@@ -1376,6 +1397,7 @@ int imex_default_params(imex_params* p) {
   p->recon_state = IMEX_RECONSTATE_CONS;                                 // config.hpp:61
   p->jet_rho = 1.4; p->jet_v = 10000.0; p->jet_B0 = sqrt(20000.0); p->jet_p = 1.0;   // config.hpp:257-260
   p->jet_halfwidth = 0.05;                                               // config.hpp:261-269
+  p->rho_amb = 1.0; p->p_amb = 0.1;                                      // config.hpp:247
   return 0;
 }
 
@@ -1482,10 +1504,11 @@ int imex_init_potential(const imex_params* p, int case_id, double amp, double* A
 
 static int check_params(const imex_params* prm) {
   if (prm->gmres_restart > MAXM) return set_err(IMEX_ERR_ARG, "gmres_restart exceeds 32");
-  if ((prm->bc_x != IMEX_BC_PERIODIC && prm->bc_x != IMEX_BC_TRANSMISSIVE && prm->bc_x != IMEX_BC_REFLECTIVE) ||
-      (prm->bc_y != IMEX_BC_PERIODIC && prm->bc_y != IMEX_BC_TRANSMISSIVE && prm->bc_y != IMEX_BC_REFLECTIVE &&
-       prm->bc_y != IMEX_BC_JET_INFLOW))
-    return set_err(IMEX_ERR_UNSUPPORTED, "boundaries: periodic, transmissive or reflective, and the jet inflow on y");
+  if (prm->bc_x < IMEX_BC_TRANSMISSIVE || prm->bc_x > IMEX_BC_DIRICHLET ||
+      prm->bc_y < IMEX_BC_TRANSMISSIVE || prm->bc_y > IMEX_BC_JET_INFLOW)
+    return set_err(IMEX_ERR_ARG, "bad boundary (the jet inflow is a y boundary)");
+  if ((prm->bc_x == IMEX_BC_DIRICHLET || prm->bc_y == IMEX_BC_DIRICHLET) && !(prm->rho_amb > 0 && prm->p_amb > 0))
+    return set_err(IMEX_ERR_ARG, "the Dirichlet ambient needs rho_amb, p_amb > 0");
   if (prm->bc_y == IMEX_BC_JET_INFLOW && !(prm->jet_rho > 0 && prm->jet_p > 0 && prm->jet_halfwidth > 0))
     return set_err(IMEX_ERR_ARG, "jet inflow state needs jet_rho, jet_p, jet_halfwidth > 0");
   if (prm->recon < IMEX_RECON_NONE || prm->recon > IMEX_RECON_WENO5)

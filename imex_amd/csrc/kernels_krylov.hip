@@ -303,6 +303,7 @@ __device__ __forceinline__ void p_apply(const POp& op, const T* xs, int lx, int 
   auto E = [&](int di, int dj) { return (T)op.eta[pidx(d, i + di, j + dj)]; };
   T h00 = H(0, 0), hp0 = H(1, 0), hm0 = H(-1, 0), h0p = H(0, 1), h0m = H(0, -1);
   T pii = X(0, 0), xp0 = X(1, 0), xm0 = X(-1, 0), x0p = X(0, 1), x0m = X(0, -1);
+  if (dir_bnd(d, i, j)) { out[0] = pii; return; }     // strong Dirichlet: identity row (implicit_operators.cpp:549-550)
   // jet inflow strip: the ghost below the first row is a homogeneous Dirichlet unknown in the enthalpy
   // term only; its known value is lifted into the right-hand side (implicit_operators.cpp:559-561)
   const T p_0m = (j == GC && jet_inlet(d, i)) ? T(0) : x0m;
@@ -319,7 +320,7 @@ __device__ __forceinline__ void p_apply(const POp& op, const T* xs, int lx, int 
 }
 template <class T>
 __device__ __forceinline__ void p_precond(const POp& op, int cidx, int i, int j, T* y) {
-  if (!op.precond) return;
+  if (!op.precond || dir_bnd(op.d, i, j)) return;     // identity rows are left alone (implicit_operators.cpp:652-655)
   double rho = op.u[pidx(op.d, i, j)];
   double Vd = op.jacobian ? p_vslope(op, cidx, rho) : 1.0 / (op.P.gamma - 1.0);
   y[0] = y[0] / (T)(Vd + op.Tdiag[cidx]);
@@ -339,9 +340,12 @@ __device__ __forceinline__ void e_apply(const POp& op, const T* xs, int lx, int 
   T hhp_x = T(0.5) * (h00 + H(1, 0)), hhm_x = T(0.5) * (h00 + H(-1, 0));
   T hhp_y = T(0.5) * (h00 + H(0, 1)), hhm_y = T(0.5) * (h00 + H(0, -1));
   T Ei = X(0, 0);
-  const T E_0m = (j == GC && jet_inlet(d, i)) ? T(0) : X(0, -1);   // getE: the jet strip's ghost (:772-779)
-  T ax = Ei - gm1 * h * h * (T(1) / (dx * dx) * (hhp_x * (X(1, 0) - Ei) - hhm_x * (Ei - X(-1, 0)))
-                           + T(1) / (dy * dy) * (hhp_y * (X(0, 1) - Ei) - hhm_y * (Ei - E_0m)));
+  // getE (:772-779): Dirichlet ghosts and the jet strip's ghost are homogeneous
+  auto GE = [&](int di, int dj) {
+    return (dir_ghost(d, i + di, j + dj) || (dj < 0 && j == GC && jet_inlet(d, i))) ? T(0) : X(di, dj);
+  };
+  T ax = Ei - gm1 * h * h * (T(1) / (dx * dx) * (hhp_x * (GE(1, 0) - Ei) - hhm_x * (Ei - GE(-1, 0)))
+                           + T(1) / (dy * dy) * (hhp_y * (GE(0, 1) - Ei) - hhm_y * (Ei - GE(0, -1))));
   out[0] = ax * dx * dy;
 }
 template <class T>
@@ -908,7 +912,7 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
       p_apply(op, xs, tx + 1, ty + 1, cx + GC, cy + GC, cidx, &Lp);
       double pf = xs[(ty + 1) * (TX + 2) + tx + 1];
       double rho = op.u[pidx(d, cx + GC, cy + GC)];
-      double g = Lp - a * pf + V_mod(P, pf, rho, Mbig, &ef) - b[cidx];
+      double g = dir_bnd(d, cx + GC, cy + GC) ? Lp - b[cidx] : Lp - a * pf + V_mod(P, pf, rho, Mbig, &ef) - b[cidx];   // :697-699
       ng[cidx] = -g;
       acc += g * g;
     }

@@ -70,8 +70,26 @@ __global__ void k_bc_fill(const Inst* __restrict__ IT, BMap bm, FillPlanes fp, i
     else if (kind == 1 && j < GC && i < d.NX - 1 && jet_inlet_face(d, i)) jet_mode = 2;
     else if (kind == 2 && j < GC - 1 && jet_inlet(d, i)) jet_mode = 3;
   }
+  const bool dg = kind == 0 && dir_ghost(d, i, j);   // Dirichlet ambient ghost (boundary.cpp:56-76, 128-147)
   for (int p = 0; p < fp.n; p++) {
     double* a = gmem(I_.buf[fp.b[p]]) + fp.k[p] * d.PS;
+    if ((jet_mode == 1 || dg) && fp.jv[p] == -2) continue;            // formed from the ghost's own state already
+    if (dg && fp.jv[p] >= 0) {
+      const int slot = fp.jv[p] & 7;
+      const double rf = refl_factor(d, i, j, true);
+      double v;
+      if (slot == 0) v = d.rho_amb;
+      else if (slot < 4) v = 0.0;
+      else if (slot > 4) v = rf * a[src];                           // B: the boundary cell's
+      else if (fp.jv[p] >= 8) v = d.p_amb;
+      else {
+        const double* g0 = gmem(I_.buf[fp.b[p]]);
+        double b5 = rf * g0[5 * d.PS + src], b6 = rf * g0[6 * d.PS + src], b7 = rf * g0[7 * d.PS + src];
+        v = d.e_amb + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
+      }
+      a[dst] = v;
+      continue;
+    }
     if (jet_mode == 1 && fp.jv[p] == -2) continue;                    // formed from the strip's state already
     if (jet_mode == 1 && fp.jv[p] >= 0) a[dst] = jet_state(d, fp.jv[p] >= 8, fp.jv[p] & 7);
     else if (jet_mode == 2) a[dst] = 0.0;
@@ -105,6 +123,10 @@ __global__ void __launch_bounds__(256) k_step_start(Phys P, const Inst* __restri
     for (int k = 0; k < NVAR; k++) q[k] = uin[k * d.PS + c];
     if (mode == 0) { for (int k = 0; k < NVAR; k++) pri[k] = q[k]; }
     else if (j < GC && jet_inlet(d, i)) { for (int k = 0; k < NVAR; k++) pri[k] = jet_state(d, 1, k); }   // g_bc_prim fill
+    else if (dir_ghost(d, i, j)) {                       // g_bc_prim fill of a Dirichlet ghost
+      pri[0] = d.rho_amb; pri[1] = pri[2] = pri[3] = 0.0; pri[4] = d.p_amb;
+      pri[5] = q[5]; pri[6] = q[6]; pri[7] = q[7];
+    }
     else con2pri(P, pri, q, &ef);
     double rho = pri[0], p = pri[4], Bx = pri[5], By = pri[6], Bz = pri[7];
     double B2 = Bx * Bx + By * By + Bz * Bz;
@@ -464,16 +486,22 @@ template <int EX_TX> struct ExTile {
 // Loop bounds are compile-time so the stencil stays in registers.
 // Stencil cells o < raw are ghost cells of the jet inflow strip: the reference fills the primitive grid's
 // ghosts with bound_cond_u outside g_bc_prim, so they hold the conservative inflow state, taken as is.
+// The ghosts of a Dirichlet side of the stencil (`amb`) hold, for the same reason, the conservative ambient
+// state: rho_amb, 0, 0, 0, e_amb + |B|^2/2 and the boundary cell's B.
+struct DirAmb { int lo, hi; double lo_v[NVAR], hi_v[NVAR]; };
 template <int NC>
 __device__ __forceinline__ void prim_faces(const Phys& P, int central, const double* __restrict__ q, long long PS,
-                                           long long step, int raw, double* Lc, double* Rc, int* ef) {
+                                           long long step, int raw, const DirAmb& amb, double* Lc, double* Rc, int* ef) {
   double s[5][NC];
 #pragma unroll
   for (int o = 0; o < 5; o++) {
     double c[NVAR];
 #pragma unroll
     for (int k = 0; k < NC; k++) c[k] = q[k * PS + (o - 2) * step];
-    if (o < raw) {
+    if (o < amb.lo || o >= 5 - amb.hi) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) s[o][k] = o < amb.lo ? amb.lo_v[k] : amb.hi_v[k];
+    } else if (o < raw) {
 #pragma unroll
       for (int k = 0; k < NC; k++) s[o][k] = c[k];
     } else if constexpr (NC == 4) {
@@ -537,13 +565,52 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     int mj = DIR == 0 ? gj : map_y(d, gj);
     const long long base = pidx(d, mi, mj);
     const long long step = DIR == 0 ? 1 : d.NX;
+    // Dirichlet sides (boundary.cpp:56-76, 128-147): TVDReconstruction's ghost fill puts the conservative
+    // ambient state, with the boundary cell's B, in the stencil cells beyond the boundary, and its face-state
+    // fill puts the ambient state, with E from the face state's own B, in the ghost items
+    DirAmb amb;
+    amb.lo = amb.hi = 0;
+    const bool dpass = DIR == 0 ? d.dirx : d.diry;
+    if (dpass) {
+      const int t0 = DIR == 0 ? mi : mj, n = DIR == 0 ? d.nx : d.ny;
+      amb.lo = max(0, GC + 2 - t0);
+      amb.hi = max(0, t0 + 2 - (n + GC - 1));
+      for (int side = 0; side < 2; side++) {
+        if ((side == 0 ? amb.lo : amb.hi) == 0) continue;
+        const int tb = side == 0 ? GC : n + GC - 1;
+        const long long cb = DIR == 0 ? pidx(d, tb, mj) : pidx(d, mi, tb);
+        double* v = side == 0 ? amb.lo_v : amb.hi_v;
+        const double b5 = Qe[5 * d.PS + cb], b6 = Qe[6 * d.PS + cb], b7 = Qe[7 * d.PS + cb];
+        v[0] = d.rho_amb; v[1] = v[2] = v[3] = 0.0;
+        v[4] = d.e_amb + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
+        v[5] = b5; v[6] = b6; v[7] = b7;
+      }
+    }
+    const bool dgi = dpass && (DIR == 0 ? (gi < GC || gi > d.nx + GC - 1) : (gj < GC || gj > d.ny + GC - 1));
+    if (dgi) {
+      T Lr[5] = {(T)d.rho_amb, T(0), T(0), T(0), T(0)}, Rr[5] = {(T)d.rho_amb, T(0), T(0), T(0), T(0)};
+      if (nrec == 5) {                                    // E of each face state from its own reconstructed B
+        double bl[3], br[3];
+        for (int c = 0; c < 3; c++) {
+          double a[5];
+#pragma unroll
+          for (int o = 0; o < 5; o++) a[o] = o < amb.lo ? amb.lo_v[5 + c] : (o >= 5 - amb.hi ? amb.hi_v[5 + c] : Qe[(5 + c) * d.PS + base + (o - 2) * step]);
+          recon5(P.recon, DIR == 1 ? P.central_y : 0, a, bl[c], br[c]);
+        }
+        Lr[4] = (T)(d.e_amb + 0.5 * (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]));
+        Rr[4] = (T)(d.e_amb + 0.5 * (br[0] * br[0] + br[1] * br[1] + br[2] * br[2]));
+      }
+      for (int k = 0; k < 5; k++)
+        if (k < nrec) { wl[k * EX_NW + it] = Lr[k]; wr[k * EX_NW + it] = Rr[k]; }
+      continue;
+    }
     if constexpr (PRIM) {
       static_assert(std::is_same<T, double>::value, "primitive reconstruction is FP64 only");
       int ef = 0;
       double Lc[NVAR], Rc[NVAR];
       const int raw = (DIR == 1 && jet_inlet(d, gi)) ? GC + 2 - gj : 0;   // stencil rows below GC in the strip
-      if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, Lc, Rc, &ef);
-      else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, Lc, Rc, &ef);
+      if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, amb, Lc, Rc, &ef);
+      else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, amb, Lc, Rc, &ef);
       for (int k = 0; k < 5; k++)
         if (k < nrec) {
           const double rf = refl_factor(d, gi, gj, k != 0 && k != 4);   // bound_cond_u of the face states
@@ -556,7 +623,10 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
         double a64[5];
         T a[5];
 #pragma unroll
-        for (int o = 0; o < 5; o++) { a64[o] = q[(o - 2) * step]; a[o] = (T)a64[o]; }
+        for (int o = 0; o < 5; o++) {
+          a64[o] = o < amb.lo ? amb.lo_v[k] : (o >= 5 - amb.hi ? amb.hi_v[k] : q[(o - 2) * step]);
+          a[o] = (T)a64[o];
+        }
         const int sel = std::is_same<T, float>::value && P.recon == IMEX_RECON_ENO3 ? eno3_stencil(a64) : -1;
         T L, R;
         recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R, sel);
@@ -704,9 +774,11 @@ __global__ void k_mag_writeback(const Inst* __restrict__ IT, BMap bm, int iu) {
   }
   int id = kidx(d, i, j);
   const double rf = refl_factor(d, i, j, true);        // the bound_cond_u that follows: B negated
-  u[5 * d.PS + c] = rf * x[id];
-  u[6 * d.PS + c] = rf * x[id + d.N];
-  u[7 * d.PS + c] = rf * x[id + 2 * d.N];
+  const double b5 = rf * x[id], b6 = rf * x[id + d.N], b7 = rf * x[id + 2 * d.N];
+  u[5 * d.PS + c] = b5;
+  u[6 * d.PS + c] = b6;
+  u[7 * d.PS + c] = b7;
+  if (dir_ghost(d, i, j)) u[4 * d.PS + c] = d.e_amb + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);   // and E of the ambient
 }
 
 // CollectMagFluxes (ct_flux.cpp:5-130): Fx at x faces (i+1/2, j), Fy at y faces (i, j+1/2),
@@ -804,6 +876,7 @@ __global__ void __launch_bounds__(256) k_p_rhs(const Inst* __restrict__ IT, BMap
                                  + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (my(i, j + 1) + my(i, j)) - 0.5 * (h0m + h00) * 0.5 * (my(i, j - 1) + my(i, j)))));
   int idx = (i - GC) + d.nx * (j - GC);
   b[idx] = RHS;
+  if (dir_bnd(d, i, j)) b[idx] = d.p_amb;              // strong Dirichlet row (imex_subsystems.cpp:160-161, 263-264)
   // jet inflow strip: the Dirichlet lifting of the ghost pressure (imex_subsystems.cpp:163-165, 266-267)
   if (j == GC && jet_inlet(d, i)) b[idx] += h * h / (dy * dy) * 0.5 * (h00 + h0m) * d.jet_p;
   if (pass == 0) {
@@ -894,11 +967,21 @@ __global__ void __launch_bounds__(256) k_e_rhs(Phys P, const Inst* __restrict__ 
   b[idx] = U_star * dx * dy;
   x0[idx] = e;
   hE[c] = h00;
-  if (j == GC && jet_inlet(d, i)) hE[pidx(d, i, j - 1)] = h0m;   // the jet strip's ghost: from its own state
+  // ghost coefficients the operator reads that are not copies: the jet strip's and the Dirichlet sides'
+  if (j == GC && jet_inlet(d, i)) hE[pidx(d, i, j - 1)] = h0m;
+  if (dir_ghost(d, i + 1, j)) hE[pidx(d, i + 1, j)] = hp0;
+  if (dir_ghost(d, i - 1, j)) hE[pidx(d, i - 1, j)] = hm0;
+  if (dir_ghost(d, i, j + 1)) hE[pidx(d, i, j + 1)] = h0p;
+  if (dir_ghost(d, i, j - 1)) hE[pidx(d, i, j - 1)] = h0m;
   double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0), hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
-  if (d.jet) {   // Dirichlet lifting (imex_subsystems.cpp:453-475): the jet strip's ghost energy, zero elsewhere
-    const double ec = (j == GC && jet_inlet(d, i)) ? d.jet_E : 0.0;
-    double L_const = -(g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * 0.0 + hhm_x * 0.0) + 1.0 / (dy * dy) * (hhp_y * 0.0 + hhm_y * ec));
+  if (d.jet || d.dirx || d.diry) {   // Dirichlet lifting (imex_subsystems.cpp:453-475): getEc of the ghosts
+    auto ec = [&](int a, int bb) {
+      if (bb < GC && jet_inlet(d, a)) return d.jet_E;
+      if (!dir_ghost(d, a, bb)) return 0.0;
+      return d.e_amb + 0.5 * (U(a, bb, 5) * U(a, bb, 5) + U(a, bb, 6) * U(a, bb, 6) + U(a, bb, 7) * U(a, bb, 7));
+    };
+    double L_const = -(g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * ec(i + 1, j) + hhm_x * ec(i - 1, j))
+                                       + 1.0 / (dy * dy) * (hhp_y * ec(i, j + 1) + hhm_y * ec(i, j - 1)));
     L_const *= dx * dy;
     b[idx] -= L_const;
   }
@@ -920,11 +1003,17 @@ __global__ void k_e_update(Phys P, const Inst* __restrict__ IT, BMap bm, int iu)
   if (i >= d.nx + GC || j >= d.ny + GC) return;
   const double h = S[S_ADT], g = P.gamma;
   long long c = pidx(d, i, j);
+  // a Dirichlet ghost holds the ambient energy of its B after bound_cond_u (boundary.cpp:66-70)
+  auto EG = [&](int a, int bb) {
+    if (!dir_ghost(d, a, bb)) return E[kidx(d, a, bb)];
+    const long long q = pidx(d, a, bb);
+    return d.e_amb + 0.5 * (u[5 * d.PS + q] * u[5 * d.PS + q] + u[6 * d.PS + q] * u[6 * d.PS + q] + u[7 * d.PS + q] * u[7 * d.PS + q]);
+  };
   u[4 * d.PS + c] = E[kidx(d, i, j)];
-  u[1 * d.PS + c] -= h * (1 / (2 * d.dx)) * (g - 1) * (E[kidx(d, i + 1, j)] - E[kidx(d, i - 1, j)]);
+  u[1 * d.PS + c] -= h * (1 / (2 * d.dx)) * (g - 1) * (EG(i + 1, j) - EG(i - 1, j));
   // the ghost below the jet strip holds the inflow energy after bound_cond_u (imex_subsystems.cpp:508-519)
-  const double E_0m = (j == GC && jet_inlet(d, i)) ? d.jet_E : E[kidx(d, i, j - 1)];
-  u[2 * d.PS + c] -= h * (1 / (2 * d.dy)) * (g - 1) * (E[kidx(d, i, j + 1)] - E_0m);
+  const double E_0m = (j == GC && jet_inlet(d, i)) ? d.jet_E : EG(i, j - 1);
+  u[2 * d.PS + c] -= h * (1 / (2 * d.dy)) * (g - 1) * (EG(i, j + 1) - E_0m);
 }
 
 // ================================================================ constrained transport
@@ -998,8 +1087,13 @@ __global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int 
   if (j < GC && jet_inlet(d, i)) { u[5 * d.PS + c] = 0.0; u[6 * d.PS + c] = d.jet_B0; return; }
   int I = map_x(d, i), J = map_y(d, j);
   const double rf = refl_factor(d, i, j, true);        // bound_cond_u: B negated across a reflective wall
-  u[5 * d.PS + c] = rf * (0.5 * (BX(I - 1, J) + BX(I, J)));
-  u[6 * d.PS + c] = rf * (0.5 * (BY(I, J - 1) + BY(I, J)));
+  const double b5 = rf * (0.5 * (BX(I - 1, J) + BX(I, J))), b6 = rf * (0.5 * (BY(I, J - 1) + BY(I, J)));
+  u[5 * d.PS + c] = b5;
+  u[6 * d.PS + c] = b6;
+  if (dir_ghost(d, i, j)) {                             // the Dirichlet ambient's E follows its B
+    const double b7 = u[7 * d.PS + c];
+    u[4 * d.PS + c] = d.e_amb + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
+  }
 }
 
 // Reconstructed face states of rho, m_x, m_y, B_x, B_y (the EMF terms read no other component) on

@@ -30,7 +30,8 @@ extern "C" {
 
 /* ---- enumerations (values follow the reference enums in include/config.hpp:27-41) ---- */
 enum { IMEX_BC_TRANSMISSIVE = 0, IMEX_BC_REFLECTIVE = 1,           /* x/y_boundary_condition (config.hpp:36-37) */
-       IMEX_BC_PERIODIC = 2, IMEX_BC_JET_INFLOW = 4 };               /* JET_INFLOW: y only (y_JetInflow) */
+       IMEX_BC_PERIODIC = 2, IMEX_BC_DIRICHLET = 3,
+       IMEX_BC_JET_INFLOW = 4 };                                     /* JET_INFLOW: y only (y_JetInflow) */
 enum { IMEX_RECON_NONE = 0, IMEX_RECON_MINMOD = 1, IMEX_RECON_VANLEER = 2, IMEX_RECON_ENO3 = 3,
        IMEX_RECON_WENO5 = 4 };                                        /* Reconstruction (config.hpp:30) */
 enum { IMEX_RECONSTATE_PRIM = 0, IMEX_RECONSTATE_CONS = 1 };         /* ReconState (config.hpp:41) */
@@ -110,6 +111,9 @@ typedef struct imex_params {
                                      p, 0, B0, 0) held in the bottom ghost cells of the strip |x_c| < jet_halfwidth;
                                      outflow (transmissive) elsewhere.  Defaults: config.hpp:257-260 */
   double jet_halfwidth;           /* 0.05 (config.hpp:261-269) */
+  double rho_amb, p_amb;          /* bc = DIRICHLET (config.hpp:247): the ambient density and pressure its ghosts
+                                     hold (velocity 0, B copied from the boundary cell); the pressure solve pins
+                                     its boundary cells to p_amb.  Defaults 1, 0.1 */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */

@@ -73,6 +73,12 @@ inline bool jet_inlet_face(const Ctx& C, int i) {                    // x face i
   double xf = C.P.x0 + (i - GC + 1.0) * C.dx;
   return std::fabs(xf) < C.P.jet_halfwidth - 1e-12;
 }
+// Dirichlet ambient (config.hpp:246-250): boundary cells of the pressure solve and ghost cells
+inline bool dir_x(const Ctx& C) { return C.P.bc_x == IMEX_BC_DIRICHLET; }
+inline bool dir_y(const Ctx& C) { return C.P.bc_y == IMEX_BC_DIRICHLET; }
+inline bool dir_bnd(const Ctx& C, int i, int j) {                    // imex_subsystems.cpp:161, implicit_operators.cpp:550
+  return (dir_x(C) && (i == GC || i == C.nx + GC - 1)) || (dir_y(C) && (j == GC || j == C.ny + GC - 1));
+}
 inline double jet_E_val(const Ctx& C) {                              // :271-273 (ideal-gas form, as written)
   return C.P.jet_p / (C.P.gamma - 1.0) + 0.5 * C.P.jet_rho * C.P.jet_v * C.P.jet_v + 0.5 * C.P.jet_B0 * C.P.jet_B0;
 }
@@ -225,7 +231,9 @@ void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
   for (int j = 0; j < C.NY; j++)
     for (int g = 0; g < GC; g++)
       for (int k = 0; k < NV; k++) {
-        if (C.P.bc_x == IMEX_BC_REFLECTIVE) {
+        if (C.P.bc_x == IMEX_BC_DIRICHLET) {
+          continue;                                                   // below, per cell
+        } else if (C.P.bc_x == IMEX_BC_REFLECTIVE) {
           G(u, C, g, j, k) = refl_sign(k) * G(u, C, 2 * GC - 1 - g, j, k);
           G(u, C, nx + GC + g, j, k) = refl_sign(k) * G(u, C, nx + GC - 1 - g, j, k);
         } else if (C.P.bc_x == IMEX_BC_PERIODIC) {
@@ -236,10 +244,26 @@ void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
           G(u, C, nx + GC + g, j, k) = G(u, C, nx + GC - 1, j, k);
         }
       }
+  // Dirichlet (:56-76, 128-147): rho_amb, v = 0, B copied from the boundary cell, slot 4 = p_amb (primitive
+  // grid) or E = p_amb/(gamma-1) + |B|^2/2 of the ghost's B
+  auto dirichlet = [&](int gi, int gj, int si, int sj) {
+    G(u, C, gi, gj, 0) = C.P.rho_amb;
+    G(u, C, gi, gj, 1) = 0; G(u, C, gi, gj, 2) = 0; G(u, C, gi, gj, 3) = 0;
+    for (int k = 5; k < NV; k++) G(u, C, gi, gj, k) = G(u, C, si, sj, k);
+    G(u, C, gi, gj, 4) = prim ? C.P.p_amb
+                              : C.P.p_amb / (C.P.gamma - 1.0) + 0.5 * (G(u, C, gi, gj, 5) * G(u, C, gi, gj, 5) +
+                                                                       G(u, C, gi, gj, 6) * G(u, C, gi, gj, 6) +
+                                                                       G(u, C, gi, gj, 7) * G(u, C, gi, gj, 7));
+  };
+  if (dir_x(C))
+    for (int j = 0; j < C.NY; j++)
+      for (int g = 0; g < GC; g++) { dirichlet(g, j, GC, j); dirichlet(nx + GC + g, j, nx + GC - 1, j); }
   for (int i = 0; i < C.NX; i++)
     for (int g = 0; g < GC; g++)
       for (int k = 0; k < NV; k++) {
-        if (C.P.bc_y == IMEX_BC_REFLECTIVE) {
+        if (C.P.bc_y == IMEX_BC_DIRICHLET) {
+          continue;
+        } else if (C.P.bc_y == IMEX_BC_REFLECTIVE) {
           G(u, C, i, g, k) = refl_sign(k) * G(u, C, i, 2 * GC - 1 - g, k);
           G(u, C, i, ny + GC + g, k) = refl_sign(k) * G(u, C, i, ny + GC - 1 - g, k);
         } else if (C.P.bc_y == IMEX_BC_PERIODIC) {
@@ -250,6 +274,9 @@ void bc_grid(const Ctx& C, Vec& u, bool prim = false) {
           G(u, C, i, ny + GC + g, k) = G(u, C, i, ny + GC - 1, k);
         }
       }
+  if (dir_y(C))
+    for (int i = 0; i < C.NX; i++)
+      for (int g = 0; g < GC; g++) { dirichlet(i, g, i, GC); dirichlet(i, ny + GC + g, i, ny + GC - 1); }
   if (jet_on(C))                                                      // y_JetInflow (:148-172): the strip
     for (int i = 0; i < C.NX; i++) {
       if (!jet_inlet(C, i)) continue;
@@ -696,12 +723,26 @@ void p_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, PCo
       double pii = X(i, j);
       double rho = G(u_con, C, i, j, 0);
       double Vdiag = jacobian ? V_slope(C, p_freeze[M_P(C, i, j)], rho) : (1.0 / (C.P.gamma - 1.0));
+      if (dir_bnd(C, i, j)) { Ax[M_P(C, i, j)] = pii; continue; }       // strong Dirichlet: identity row (:549-550)
       double p_0m = (j == GC && jet_inlet(C, i)) ? 0.0 : X(i, j - 1);   // jet strip: homogeneous ghost (:559-561)
       double ax = Vdiag * pii - h * h * (1 / (dx * dx) * (0.5 * (hp0 + h00) * (X(i + 1, j) - X(i, j)) - 0.5 * (h00 + hm0) * (X(i, j) - X(i - 1, j)))
                                        + 1 / (dy * dy) * (0.5 * (h0p + h00) * (X(i, j + 1) - X(i, j)) - 0.5 * (h0m + h00) * (X(i, j) - p_0m)));
       double nu_x = S.vx_max * dx, nu_y = S.vy_max * dy;
-      double lap_p = nu_x / (dx * dx) * (0.5 * (E(i + 1, j) + E(i, j)) * (X(i + 1, j) - pii) - 0.5 * (E(i, j) + E(i - 1, j)) * (pii - X(i - 1, j)))
-                   + nu_y / (dy * dy) * (0.5 * (E(i, j + 1) + E(i, j)) * (X(i, j + 1) - pii) - 0.5 * (E(i, j) + E(i, j - 1)) * (pii - X(i, j - 1)));
+      // getp (:533-547): a Dirichlet ghost is the negated mirror value in the viscosity term
+      auto GP = [&](int a, int b) {
+        int II = a, JJ = b; double sg = 1.0;
+        if (dir_x(C)) {
+          if (a < GC) { II = 2 * GC - 1 - a; sg = -sg; }
+          else if (a >= C.nx + GC) { II = 2 * (C.nx + GC) - 1 - a; sg = -sg; }
+        }
+        if (dir_y(C)) {
+          if (b < GC) { JJ = 2 * GC - 1 - b; sg = -sg; }
+          else if (b >= C.ny + GC) { JJ = 2 * (C.ny + GC) - 1 - b; sg = -sg; }
+        }
+        return sg * X(II, JJ);
+      };
+      double lap_p = nu_x / (dx * dx) * (0.5 * (E(i + 1, j) + E(i, j)) * (GP(i + 1, j) - pii) - 0.5 * (E(i, j) + E(i - 1, j)) * (pii - GP(i - 1, j)))
+                   + nu_y / (dy * dy) * (0.5 * (E(i, j + 1) + E(i, j)) * (GP(i, j + 1) - pii) - 0.5 * (E(i, j) + E(i, j - 1)) * (pii - GP(i, j - 1)));
       double visc_scale = Vbase_slope(C, A(pc.pe, C, i, j), rho);
       Ax[M_P(C, i, j)] = ax - visc_scale * h * lap_p;
     }
@@ -808,6 +849,7 @@ int pressure_newton(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& 
     for (int i = GC; i < C.nx + GC; i++)
       for (int j = GC; j < C.ny + GC; j++) {
         int idx = M_P(C, i, j);
+        if (dir_bnd(C, i, j)) { out[idx] = in[idx]; continue; }     // identity row (:652-655)
         double Vd = jac ? V_slope(C, p_freeze[idx], G(u_con, C, i, j, 0)) : a;
         out[idx] = in[idx] / (Vd + Tdiag[idx]);
       }
@@ -846,7 +888,8 @@ int pressure_newton(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& 
     for (int i = GC; i < C.nx + GC; i++)
       for (int j = GC; j < C.ny + GC; j++) {
         int idx = M_P(C, i, j);
-        g[idx] = Lp[idx] - a * p_freeze[idx] + V_mod(C, p_freeze[idx], G(u_con, C, i, j, 0)) - b[idx];
+        if (dir_bnd(C, i, j)) g[idx] = Lp[idx] - b[idx];             // identity row: p - p_amb (:697-699)
+        else g[idx] = Lp[idx] - a * p_freeze[idx] + V_mod(C, p_freeze[idx], G(u_con, C, i, j, 0)) - b[idx];
       }
     double res = normv(g);
     if (it == 0) res0 = res;
@@ -1007,6 +1050,7 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
         double RHS = p_star - ke - h * (1 / dx * (0.5 * (hp0 + h00) * 0.5 * (A(mx, C, i + 1, j) + A(mx, C, i, j)) - 0.5 * (hm0 + h00) * 0.5 * (A(mx, C, i - 1, j) + A(mx, C, i, j)))
                                         + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (A(my, C, i, j + 1) + A(my, C, i, j)) - 0.5 * (h0m + h00) * 0.5 * (A(my, C, i, j - 1) + A(my, C, i, j)))));
         b[M_P(C, i, j)] = RHS;
+        if (dir_bnd(C, i, j)) b[M_P(C, i, j)] = C.P.p_amb;          // strong Dirichlet (:160-161, 263-264)
         if (j == GC && jet_inlet(C, i)) b[M_P(C, i, j)] += h * h / (dy * dy) * 0.5 * (h00 + h0m) * C.P.jet_p;   // lifting (:163-165, 266-267)
         if (pass == 0) p[M_P(C, i, j)] = A(pc.pe, C, i, j);
       }
@@ -1063,9 +1107,14 @@ void e_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, Vec
       double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0);
       double hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
       double Ei = X(i, j);
-      double E_0m = (j == GC && jet_inlet(C, i)) ? 0.0 : X(i, j - 1);   // getE: jet strip ghost homogeneous (:772-779)
-      double ax = Ei - (g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * (X(i + 1, j) - Ei) - hhm_x * (Ei - X(i - 1, j)))
-                                        + 1.0 / (dy * dy) * (hhp_y * (X(i, j + 1) - Ei) - hhm_y * (Ei - E_0m)));
+      // getE (:772-779): Dirichlet ghosts and the jet strip's ghost are homogeneous (zero)
+      auto GE = [&](int a, int b) {
+        bool xg = a < GC || a >= C.nx + GC, yg = b < GC || b >= C.ny + GC;
+        if ((dir_x(C) && xg) || (dir_y(C) && yg) || (b < GC && jet_inlet(C, a))) return 0.0;
+        return X(a, b);
+      };
+      double ax = Ei - (g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * (GE(i + 1, j) - Ei) - hhm_x * (Ei - GE(i - 1, j)))
+                                        + 1.0 / (dy * dy) * (hhp_y * (GE(i, j + 1) - Ei) - hhm_y * (Ei - GE(i, j - 1))));
       Ax[M_P(C, i, j)] = ax * dx * dy;
     }
   C.st.op_applies_p++;
@@ -1098,10 +1147,16 @@ int pre_subsystem_energy(Ctx& C, const imex_scalars& S, Stage& W) {
       double U_star = G(u, C, i, j, 4) - 0.5 * h / dx * (hp0 * G(u, C, i + 1, j, 1) - hm0 * G(u, C, i - 1, j, 1))
                     - 0.5 * h / dy * (h0p * G(u, C, i, j + 1, 2) - h0m * G(u, C, i, j - 1, 2));
       b[M_P(C, i, j)] = U_star * dx * dy;
-      if (jet_on(C)) {                                                // Dirichlet lifting (:453-475): getEc is
-        double h00 = A(hE, C, i, j);                                  // jet_E_val on the strip's ghost row only
+      if (jet_on(C) || dir_x(C) || dir_y(C)) {                        // Dirichlet lifting (:453-475): getEc
+        double h00 = A(hE, C, i, j);
         double hhp_x = 0.5 * (h00 + hp0), hhm_x = 0.5 * (h00 + hm0), hhp_y = 0.5 * (h00 + h0p), hhm_y = 0.5 * (h00 + h0m);
-        auto getEc = [&](int ii, int jj) { return (jj < GC && jet_inlet(C, ii)) ? jet_E_val(C) : 0.0; };
+        auto getEc = [&](int ii, int jj) {
+          bool xg = ii < GC || ii >= C.nx + GC, yg = jj < GC || jj >= C.ny + GC;
+          if (jj < GC && jet_inlet(C, ii)) return jet_E_val(C);
+          if (!((dir_x(C) && xg) || (dir_y(C) && yg))) return 0.0;
+          return C.P.p_amb / (C.P.gamma - 1.0) + 0.5 * (G(u, C, ii, jj, 5) * G(u, C, ii, jj, 5) + G(u, C, ii, jj, 6) * G(u, C, ii, jj, 6) +
+                                                       G(u, C, ii, jj, 7) * G(u, C, ii, jj, 7));
+        };
         double L_const = -(g - 1) * h * h * (1.0 / (dx * dx) * (hhp_x * getEc(i + 1, j) + hhm_x * getEc(i - 1, j))
                                            + 1.0 / (dy * dy) * (hhp_y * getEc(i, j + 1) + hhm_y * getEc(i, j - 1)));
         L_const *= dx * dy;
@@ -1982,6 +2037,7 @@ int orc_gmres(const imex_params* p, const imex_scalars* sc, const imex_stage_hos
     for (int i = GC; i < C.nx + GC; i++)
       for (int j = GC; j < C.ny + GC; j++) {
         int idx = M_P(C, i, j);
+        if (dir_bnd(C, i, j)) { out[idx] = in[idx]; continue; }     // identity rows (implicit_operators.cpp:652-655)
         double Vd = op == IMEX_OP_P_JACOBIAN ? V_slope(C, p_freeze[idx], G(W.u_con, C, i, j, 0)) : a;
         out[idx] = in[idx] / (Vd + Tdiag[idx]);
       }

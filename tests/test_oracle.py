@@ -1161,3 +1161,98 @@ def test_reflective_box_keeps_the_mirror_symmetries():
     o.set_state(pri, np.zeros(pri.shape[:2]), np.zeros(pri.shape[:2]))
     o.step(2)
     assert np.array_equal(o.get_state()[1][G:-G, G:-G], pri[G:-G, G:-G])
+
+
+# ---------------------------------------------------------------- Dirichlet ambient (x_Dirichlet / y_Dirichlet)
+def _dirichlet_blast(n=16, m=12, walls=(im.BC_DIRICHLET, im.BC_DIRICHLET), **over):
+    p, u, bx, by = make_case("blast", n, m, amp=10.0, **over)
+    p.bc_x, p.bc_y = walls
+    p.rho_amb, p.p_amb = 1.0, 0.1
+    return p, orc.bc(p, u, 4), bx, by
+
+
+def test_dirichlet_ghost_fills():
+    """bound_cond_u under x/y_Dirichlet (boundary.cpp:56-76, 128-147): every ghost of a Dirichlet side holds
+    rho_amb, v = 0, the B of its outflow image (the boundary cell) and p_amb (primitive grid) or
+    p_amb/(gamma-1) + |B|^2/2 (conservative grid); arrays and faces are filled as at an outflow."""
+    for walls in ((im.BC_DIRICHLET, im.BC_DIRICHLET), (im.BC_DIRICHLET, im.BC_PERIODIC), (im.BC_PERIODIC, im.BC_DIRICHLET)):
+        p = ideal(10, 8)
+        p.bc_x, p.bc_y = walls
+        q = ideal(10, 8)
+        q.bc_x, q.bc_y = [im.BC_TRANSMISSIVE if w == im.BC_DIRICHLET else w for w in walls]
+        NX, NY = p.nx + 2 * G, p.ny + 2 * G
+        rng = np.random.default_rng(21)
+        a = rng.uniform(0.5, 2.0, (NY, NX, 8))
+        out_flow = orc.bc(q, a, 0)
+        for kind, prim in ((0, False), (4, True)):
+            got = orc.bc(p, a, kind)
+            for j in range(NY):
+                for i in range(NX):
+                    gx = walls[0] == im.BC_DIRICHLET and (i < G or i >= p.nx + G)
+                    gy = walls[1] == im.BC_DIRICHLET and (j < G or j >= p.ny + G)
+                    if not (gx or gy):
+                        assert np.array_equal(got[j, i], out_flow[j, i])
+                        continue
+                    B = out_flow[j, i, 5:]
+                    e = p.p_amb if prim else p.p_amb / (p.gamma - 1) + 0.5 * (B ** 2).sum()
+                    assert np.array_equal(got[j, i, :5], [p.rho_amb, 0, 0, 0, e]) and np.array_equal(got[j, i, 5:], B), (walls, i, j)
+        b = rng.uniform(0.5, 2.0, (NY, NX))
+        for kind in (1, 2, 3):
+            assert np.array_equal(orc.bc(p, b, kind), orc.bc(q, b, kind))
+
+
+def test_dirichlet_pressure_rows_pin_the_boundary_pressure():
+    """Strong Dirichlet rows (implicit_operators.cpp:549-550, 652-655, 697-699; imex_subsystems.cpp:160-161):
+    the pressure operator is the identity on the boundary cells, and PreSubsystem_Pressure leaves p = p_amb
+    there (read back from E), also with the nested Newton of an active floor."""
+    for floor in (0, 1):
+        p, u, bx, by = _dirichlet_blast(floor_on=floor)
+        o = orc.Oracle(p)
+        o.set_state(u, bx, by)
+        o.step(1)
+        st, sc, _ = o.step_capture(3)
+        N = p.nx * p.ny
+        x = np.random.default_rng(5).standard_normal(N)
+        ax = orc.apply_op(p, sc, st, im.OP_P_CARRIER, x).reshape(p.ny, p.nx)
+        xr = x.reshape(p.ny, p.nx)
+        bnd = np.zeros((p.ny, p.nx), bool)
+        bnd[0, :] = bnd[-1, :] = bnd[:, 0] = bnd[:, -1] = True
+        assert np.array_equal(ax[bnd], xr[bnd]) and not np.allclose(ax[~bnd], xr[~bnd])
+        out, rc, _ = orc.sub(p, sc, st, "pre_subsystem_pressure")
+        pri = orc.con2pri(p, out["u_con"])[G:-G, G:-G, 4]
+        assert np.allclose(pri[bnd], p.p_amb, rtol=1e-8, atol=0), floor
+
+
+def test_dirichlet_energy_operator_drops_the_ghost_coupling():
+    """Energy_Subsystem_AX's getE (implicit_operators.cpp:772-779): a Dirichlet ghost is homogeneous, so on the
+    boundary cells the operator differs from the outflow one exactly by the dropped couplings."""
+    p, u, bx, by = _dirichlet_blast(formula=im.FORMULA_ENERGY)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
+    o.step(1)
+    st, sc, _ = o.step_capture(3)
+    q = im.blast_params(16)
+    q.ny, q.formula = 12, im.FORMULA_ENERGY
+    q.bc_x = q.bc_y = im.BC_TRANSMISSIVE
+    x = np.random.default_rng(6).standard_normal(p.nx * p.ny)
+    d = (orc.apply_op(p, sc, st, im.OP_ENERGY, x) - orc.apply_op(q, sc, st, im.OP_ENERGY, x)).reshape(p.ny, p.nx)
+    inner = np.zeros((p.ny, p.nx), bool)
+    inner[1:-1, 1:-1] = True
+    assert np.all(d[inner] == 0) and np.all(d[~inner] != 0)
+    dx, dy, h, g = (p.x1 - p.x0) / p.nx, (p.y1 - p.y0) / p.ny, sc["adt"], p.gamma
+    uc, Qe = st["u_con"], st["Qe"]
+    hE = lambda jj, ii: _total_enthalpy_ratio(p, Qe[jj, ii], uc[jj, ii, 0])
+    j, i = G + 4, G                                         # a left-boundary cell: only the x- ghost is dropped
+    want = (g - 1) * h * h / dx ** 2 * 0.5 * (hE(j, i) + hE(j, i - 1)) * x[(i - G) + p.nx * (j - G)] * dx * dy
+    assert d[j - G, i - G] == pytest.approx(want, rel=1e-10)
+
+
+def test_dirichlet_ambient_at_rest_is_stationary():
+    q = ideal(12, 10)
+    q.bc_x = q.bc_y = im.BC_DIRICHLET
+    q.rho_amb, q.p_amb = 1.3, 0.7
+    pri = np.tile([1.3, 0.0, 0.0, 0.0, 0.7, 0.0, 0.0, 0.0], (q.ny + 2 * G, q.nx + 2 * G, 1))
+    o = orc.Oracle(q)
+    o.set_state(pri, np.zeros(pri.shape[:2]), np.zeros(pri.shape[:2]))
+    o.step(2)
+    assert np.array_equal(o.get_state()[1][G:-G, G:-G], pri[G:-G, G:-G])
