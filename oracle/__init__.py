@@ -310,6 +310,8 @@ def cfl(p, u_pri, sc=None):
 
 
 def bc(p, a, kind):
+    """Ghost fill of a copy of `a`: kind 0 Grid (conservative), 1 Array, 2 / 3 staggered x / y faces,
+    4 Grid of primitives (the reference's g_bc_prim)."""
     a = np.array(a, dtype=np.float64, copy=True)
     lib().orc_bc(C.byref(p), ptr(a), kind)
     return a

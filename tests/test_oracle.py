@@ -38,6 +38,16 @@ term, a wrong sign or index, or a transposed operand fails a check:
   CT (ct_flux.cpp:150-191)                  discrete div B of the faces unchanged by any EMF
   div B metric (imex_subsystems.cpp:630-660)  closed form on analytic faces
   CFL (timestep.cpp:5-67, 89-120, 254-273)  brute-force numpy maxima and the dt rule
+  jet inflow (boundary.cpp:148-172, 279-291, constrained_transport.cpp:50-63, imex_subsystems.cpp:107-118,
+    163-165, 202-203, 453-475, implicit_operators.cpp:559-561, 772-779)  brute-force fills; the operators differ
+                                            from the outflow ones by exactly the dropped ghost coupling; injected
+                                            mass follows the inflow flux, mirror symmetry, div B
+  reflective walls (boundary.cpp:17-54, 89-126, timestep.cpp:175-214, implicit_operators.cpp:14-56, 146-211)
+                                            brute-force fills and maps; the magnetic operator's flags touch only
+                                            boundary-cell neighbours; a field-free box keeps its symmetries
+  Dirichlet ambient (boundary.cpp:56-76, 128-147, implicit_operators.cpp:549-550, 652-655, 697-699, 772-779)
+                                            ambient ghosts with the image's B; identity rows pin p = p_amb; the
+                                            energy operator drops exactly the ghost couplings; rest state kept
   whole step (main.cpp:334-595)             mass conservation, div B preservation, vortex advection (every
                                             reconstruction and ReconState; dissipation ordered NoRecon > Minmod > ENO3); energy
                                             formula: mass, momentum and total energy conservation; UnStaggerCT /
