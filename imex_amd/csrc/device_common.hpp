@@ -64,6 +64,7 @@ struct Phys {
   double cfl, time_stop;
   int recon, central_y;
   int prim;               // reconstruct primitives (ReconState PrimState)
+  int bnd;                // a jet inflow, reflective or Dirichlet boundary is present
 };
 
 // ---------------------------------------------------------------- index maps

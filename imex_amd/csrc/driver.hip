@@ -260,6 +260,8 @@ static Phys make_phys(const imex_params& p) {
   P.kappa1 = p.shock_kappa1; P.cfl = p.cfl; P.time_stop = p.time_stop;
   P.recon = p.recon; P.central_y = p.minmod_central_y;
   P.prim = p.recon_state == IMEX_RECONSTATE_PRIM;
+  P.bnd = p.bc_y == IMEX_BC_JET_INFLOW || p.bc_x == IMEX_BC_REFLECTIVE || p.bc_y == IMEX_BC_REFLECTIVE ||
+          p.bc_x == IMEX_BC_DIRICHLET || p.bc_y == IMEX_BC_DIRICHLET;
   return P;
 }
 

@@ -536,7 +536,7 @@ __device__ __forceinline__ void prim_faces(const Phys& P, int central, const dou
 // the primitives of the five stencil cells and converts the two face states back (ReconState
 // PrimState, reconstruction.cpp:11-24, 272-281); kind 0 needs rho and v only, kinds 1 and 2 the
 // whole state, as the face energy rho e(rho, p) + kinetic + |B|^2/2 uses the reconstructed p and B.
-template <int DIR, int EX_TX, class T, bool PRIM>
+template <int DIR, int EX_TX, class T, bool PRIM, bool BND>
 __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int kind, const double* __restrict__ Qe,
                                              const double* __restrict__ Qs, double* __restrict__ u, double h, T s_max,
                                              int i0, int j0, T* wl, T* wr, T* F, double* acc, int* eflag) {
@@ -556,7 +556,7 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     int gi = DIR == 0 ? i0 + lx - 1 : i0 + lx;
     int gj = DIR == 0 ? j0 + ly : j0 + ly - 1;
     if (gi > d.nx + GC || gj > d.ny + GC) continue;      // beyond a partial tile: unused
-    if (DIR == 1 && gj < GC && jet_inlet(d, gi)) {        // bound_cond_u of the face states: the jet inflow
+    if (BND && DIR == 1 && gj < GC && jet_inlet(d, gi)) {  // bound_cond_u of the face states: the jet inflow
       for (int k = 0; k < 5; k++)
         if (k < nrec) { wl[k * EX_NW + it] = (T)jet_state(d, 0, k); wr[k * EX_NW + it] = (T)jet_state(d, 0, k); }
       continue;
@@ -570,7 +570,7 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     // fill puts the ambient state, with E from the face state's own B, in the ghost items
     DirAmb amb;
     amb.lo = amb.hi = 0;
-    const bool dpass = DIR == 0 ? d.dirx : d.diry;
+    const bool dpass = BND && (DIR == 0 ? d.dirx : d.diry);
     if (dpass) {
       const int t0 = DIR == 0 ? mi : mj, n = DIR == 0 ? d.nx : d.ny;
       amb.lo = max(0, GC + 2 - t0);
@@ -608,12 +608,12 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
       static_assert(std::is_same<T, double>::value, "primitive reconstruction is FP64 only");
       int ef = 0;
       double Lc[NVAR], Rc[NVAR];
-      const int raw = (DIR == 1 && jet_inlet(d, gi)) ? GC + 2 - gj : 0;   // stencil rows below GC in the strip
+      const int raw = (BND && DIR == 1 && jet_inlet(d, gi)) ? GC + 2 - gj : 0;   // stencil rows below GC in the strip
       if (kind == 0) prim_faces<4>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, amb, Lc, Rc, &ef);
       else prim_faces<NVAR>(P, DIR == 1 ? P.central_y : 0, Qe + base, d.PS, step, raw, amb, Lc, Rc, &ef);
       for (int k = 0; k < 5; k++)
         if (k < nrec) {
-          const double rf = refl_factor(d, gi, gj, k != 0 && k != 4);   // bound_cond_u of the face states
+          const double rf = BND ? refl_factor(d, gi, gj, k != 0 && k != 4) : 1.0;   // bound_cond_u of the face states
           wl[k * EX_NW + it] = rf * Lc[k]; wr[k * EX_NW + it] = rf * Rc[k];
         }
       if (ef) atomicOr(eflag, 1);
@@ -624,15 +624,19 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
         T a[5];
 #pragma unroll
         for (int o = 0; o < 5; o++) {
-          a64[o] = o < amb.lo ? amb.lo_v[k] : (o >= 5 - amb.hi ? amb.hi_v[k] : q[(o - 2) * step]);
+          if constexpr (BND) a64[o] = o < amb.lo ? amb.lo_v[k] : (o >= 5 - amb.hi ? amb.hi_v[k] : q[(o - 2) * step]);
+          else a64[o] = q[(o - 2) * step];
           a[o] = (T)a64[o];
         }
         const int sel = std::is_same<T, float>::value && P.recon == IMEX_RECON_ENO3 ? eno3_stencil(a64) : -1;
         T L, R;
         recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R, sel);
-        const T rf = (T)refl_factor(d, gi, gj, k != 0 && k != 4);        // bound_cond_u of the face states
-        wl[k * EX_NW + it] = rf * L;
-        wr[k * EX_NW + it] = rf * R;
+        if constexpr (BND) {
+          const T rf = (T)refl_factor(d, gi, gj, k != 0 && k != 4);      // bound_cond_u of the face states
+          L = rf * L; R = rf * R;
+        }
+        wl[k * EX_NW + it] = L;
+        wr[k * EX_NW + it] = R;
       }
     }
   }
@@ -671,7 +675,9 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
   }
 }
 
-template <int EX_TX, class T, bool PRIM>
+// BND: the boundary treatments other than periodic and outflow (jet inflow, reflective, Dirichlet) are
+// compiled in; without them the kernel carries none of their code.
+template <int EX_TX, class T, bool PRIM, bool BND>
 __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict__ IT, BMap bm, int kind, int iQe, int iQs, int iu) {
   constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
   BATCH_BLOCK(bm);
@@ -689,8 +695,8 @@ __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict
   const T s_max = (T)S[S_AMAX];
   int* eflag = gmem(I_.eflag);
   double acc[5];
-  explicit_dir<0, EX_TX, T, PRIM>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
-  explicit_dir<1, EX_TX, T, PRIM>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
+  explicit_dir<0, EX_TX, T, PRIM, BND>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
+  explicit_dir<1, EX_TX, T, PRIM, BND>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
 }
 
 // ================================================================ magnetic subsystem
@@ -1446,9 +1452,15 @@ void launch_shock(const Launch& L, const Phys& P, int iQe) {
 }
 template <int TXE>
 static void launch_explicit_t(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed) {
-  if (P.prim) k_explicit<TXE, double, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-  else if (mixed) k_explicit<TXE, float, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-  else k_explicit<TXE, double, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  if (P.bnd) {
+    if (P.prim) k_explicit<TXE, double, true, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else if (mixed) k_explicit<TXE, float, false, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else k_explicit<TXE, double, false, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  } else {
+    if (P.prim) k_explicit<TXE, double, true, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else if (mixed) k_explicit<TXE, float, false, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else k_explicit<TXE, double, false, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+  }
 }
 // tile: the explicit tile width (16, 32 or 64 cells); the SH_EXPL block map must be built for it
 void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile) {
