@@ -349,8 +349,8 @@ KERNEL_OF_CLASS = {   # (class, form) -> kernel symbol in the committed rocprof 
     ("p_arnoldi_apply", "mixed"): "k_arnoldi_apply<1, double, double>",
     ("e_arnoldi_apply", "fp64"): "k_arnoldi_apply<2, double, double>",
     ("e_arnoldi_apply", "mixed"): "k_arnoldi_apply<2, double, double>",
-    ("explicit_energy", "fp64"): "k_explicit<32, double>",
-    ("explicit_energy", "mixed"): "k_explicit<32, float>",
+    ("explicit_energy", "fp64"): "k_explicit<32, double, false, false>",
+    ("explicit_energy", "mixed"): "k_explicit<32, float, false, false>",
 }
 TRAFFIC_SUMMARY = {"pressure": os.path.join(ROOT, "profiles", "imex_step_ot1024_kernels.md"),
                    "energy": os.path.join(ROOT, "profiles", "imex_step_ot1024_energy_kernels.md"),
