@@ -717,14 +717,36 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict_
         y[c] = (T)as;
       }
     }
-    for (int i = 0; i <= j; i++) {
-      double part = 0.0;
-      if (valid) {
-        const VT* vi = KV<VT>::V(I, i);
-        for (int c = 0; c < NC; c++) part += (i < j ? (double)vi[(long long)c * d.N + cidx] : vc[c]) * (double)y[c];
+    if constexpr (NC == 1) {
+      // Scalar operators: the stored v_i are fetched eight at a time before any of them is used,
+      // so their loads are in flight together instead of one HBM round trip per i.
+      constexpr int CH = 8;
+      const long long cl = valid ? cidx : 0;
+      for (int i0 = 0; i0 <= j; i0 += CH) {
+        VT vv[CH];
+#pragma unroll
+        for (int q = 0; q < CH; q++)
+          if (i0 + q < j) vv[q] = KV<VT>::V(I, i0 + q)[cl];
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+          const int i = i0 + q;
+          if (i > j) break;
+          double part = 0.0;
+          if (valid) part += (i < j ? (double)vv[q] : vc[0]) * (double)y[0];
+          double s = wave_sum(part);
+          if (lane == 0) wsum[wid][i] += s;
+        }
       }
-      double s = wave_sum(part);
-      if (lane == 0) wsum[wid][i] += s;
+    } else {
+      for (int i = 0; i <= j; i++) {
+        double part = 0.0;
+        if (valid) {
+          const VT* vi = KV<VT>::V(I, i);
+          for (int c = 0; c < NC; c++) part += (i < j ? (double)vi[(long long)c * d.N + cidx] : vc[c]) * (double)y[c];
+        }
+        double s = wave_sum(part);
+        if (lane == 0) wsum[wid][i] += s;
+      }
     }
   }
   __syncthreads();
