@@ -814,35 +814,46 @@ __global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const Inst* __restri
   KrylovDev* K = gmem(I.K);
   if (K->done) return;
   __shared__ double tot[1];
+  // Column j of H, the rotations so far and gam[j] are staged in LDS by parallel loads, so the
+  // rotation chain below runs on LDS instead of one global round trip per rotation.
+  __shared__ double hc[MAXM + 2], cs[MAXM + 2], sn[MAXM + 2], g0[1];
+  const int L = MAXM + 1;
+  double* H = K->H;
+  const int t = threadIdx.x;
+  if (t <= j) hc[t] = H[t * L + j];
+  if (t >= 1 && t <= j) { cs[t] = K->c[t]; sn[t] = K->s[t]; }
+  if (t == 0) g0[0] = K->gam[j];
   final_sums<1>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int L = MAXM + 1;
-    double* H = K->H;
-    H[(j + 1) * L + j] = sqrt(tot[0]);
+  if (t == 0) {
+    hc[j + 1] = sqrt(tot[0]);
     for (int i = 0; i < j; i++) {
-      double t1 = K->c[i + 1] * H[i * L + j] + K->s[i + 1] * H[(i + 1) * L + j];
-      double t2 = K->s[i + 1] * H[i * L + j] - K->c[i + 1] * H[(i + 1) * L + j];
-      H[i * L + j] = t1;
-      H[(i + 1) * L + j] = t2;
+      double t1 = cs[i + 1] * hc[i] + sn[i + 1] * hc[i + 1];
+      double t2 = sn[i + 1] * hc[i] - cs[i + 1] * hc[i + 1];
+      hc[i] = t1;
+      hc[i + 1] = t2;
     }
-    double beta = sqrt(H[j * L + j] * H[j * L + j] + H[(j + 1) * L + j] * H[(j + 1) * L + j]);
-    K->s[j + 1] = H[(j + 1) * L + j] / beta;
-    K->c[j + 1] = H[j * L + j] / beta;
-    H[j * L + j] = beta;
-    K->gam[j + 1] = K->s[j + 1] * K->gam[j];
-    K->gam[j] = K->c[j + 1] * K->gam[j];
-    K->res = fabs(K->gam[j + 1]);
+    double beta = sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
+    const double sj = hc[j + 1] / beta, cj = hc[j] / beta;
+    K->s[j + 1] = sj;
+    K->c[j + 1] = cj;
+    hc[j] = beta;
+    const double gj1 = sj * g0[0];
+    K->gam[j + 1] = gj1;
+    K->gam[j] = cj * g0[0];
+    K->res = fabs(gj1);
     K->steps = j + 1;
     K->steps_total += 1;
     if (K->outer < MAXSWEEP) K->sweep_steps[K->outer] = j + 1;
-    if (K->gam[j + 1] > K->tol_eff) {
-      K->hcur = H[(j + 1) * L + j];
+    if (gj1 > K->tol_eff) {
+      K->hcur = hc[j + 1];
       if (j + 1 == m) { K->done = 1; K->converged = 0; K->jlast = j; }
     } else {
       K->done = 1; K->converged = mixed ? 0 : 1; K->jlast = j;
     }
   }
+  __syncthreads();
+  if (t <= j + 1) H[t * L + j] = hc[t];
 }
 
 // ---------------------------------------------------------------- x += sum alpha_i v_i (back substitution per block)
@@ -852,14 +863,21 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ 
   const Inst& I = IT[inst_];
   KrylovDev* K = gmem(I.K);
   __shared__ double alpha[MAXM + 1];
+  __shared__ double Hs[(MAXM + 1) * (MAXM + 1)], gs[MAXM + 1];   // H and gam staged by parallel loads
   const int jl = K->jlast;
   if (jl < 0) return;
+  const int L = MAXM + 1;
+  for (int q = threadIdx.x; q < (jl + 1) * (jl + 1); q += blockDim.x) {
+    const int i = q / (jl + 1), k = q % (jl + 1);
+    Hs[i * L + k] = K->H[i * L + k];
+  }
+  if ((int)threadIdx.x <= jl) gs[threadIdx.x] = K->gam[threadIdx.x];
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const int L = MAXM + 1;
     for (int i = jl; i >= 0; i--) {
-      double a = K->gam[i];
-      for (int k = i + 1; k <= jl; k++) a -= K->H[i * L + k] * alpha[k];
-      alpha[i] = a / K->H[i * L + i];
+      double a = gs[i];
+      for (int k = i + 1; k <= jl; k++) a -= Hs[i * L + k] * alpha[k];
+      alpha[i] = a / Hs[i * L + i];
     }
   }
   __syncthreads();
