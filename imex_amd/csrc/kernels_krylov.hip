@@ -540,12 +540,16 @@ __device__ __forceinline__ void block_sums(const double* acc, int nv, double* re
     out[threadIdx.x] = s;
   }
 }
+// The one-block finishes run FIN_THREADS threads: their cost is the chain of dependent loads each
+// thread makes over the partial rows (nblk / FIN_THREADS of them), so wide blocks keep it short.
+constexpr int FIN_THREADS = 1024;
+
 // One block: out[v] = sum over blocks b < nblk of partial[b*stride + v], v < nv <= NV.  Thread t
-// sums rows t, t + 256, ... with all their values in flight at once; the per-thread sums are
+// sums rows t, t + blockDim.x, ... with all their values in flight at once; the per-thread sums are
 // then reduced wave by wave in a fixed order, so the result does not depend on timing.
 template <int NV>
 __device__ __forceinline__ void final_sums(const double* partial, int stride, int nblk, int nv, double* out) {
-  __shared__ double red[4][NV];
+  __shared__ double red[FIN_THREADS / 64][NV];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double acc[NV];
 #pragma unroll
@@ -638,7 +642,7 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
 // One-block finish of the residual norm: gam0, tolerance, convergence (implicit_operators.cpp:829-836).
 // Mixed form: the solve is converged when the FP64 residual is below outer_tol ||M^-1 b|| +
 // gmres_tol; otherwise the inner solve of this sweep runs to inner_tol relative to this residual.
-__global__ void __launch_bounds__(256) k_gmres_residual_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int outer,
+__global__ void __launch_bounds__(FIN_THREADS) k_gmres_residual_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int outer,
                                                             int skip, int mixed, double outer_tol, double inner_tol) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -760,7 +764,7 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict_
 
 // One-block finish of the dots: H[i][j] for i <= j.
 template <int JB>
-__global__ void __launch_bounds__(256) k_arnoldi_dots_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j) {
+__global__ void __launch_bounds__(FIN_THREADS) k_arnoldi_dots_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   KrylovDev* K = gmem(I.K);
@@ -807,7 +811,7 @@ __global__ void __launch_bounds__(256) k_arnoldi_update(const Inst* __restrict__
 // One-block finish of the update: H[j+1][j] = ||w||, Givens rotations, convergence (implicit_operators.cpp:857-897).
 // In the mixed form an inner solve that reaches its tolerance ends the sweep but not the solve:
 // convergence is decided by the next FP64 residual.
-__global__ void __launch_bounds__(256) k_arnoldi_givens_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j, int m,
+__global__ void __launch_bounds__(FIN_THREADS) k_arnoldi_givens_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int j, int m,
                                                             int mixed) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -962,7 +966,7 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
   if (threadIdx.x == 0) gmem(I.partial)[bx_ * (MAXM + 2)] = res[0];
 }
 
-__global__ void __launch_bounds__(256) k_newton_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int it, double ntol) {
+__global__ void __launch_bounds__(FIN_THREADS) k_newton_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int it, double ntol) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   KrylovDev* K = gmem(I.K);
@@ -1120,7 +1124,7 @@ void gmres_residual(const Launch& L, int op, int prec, const MagIds& mo, const P
   }
 }
 void gmres_residual_fin(const Launch& L, int op, int prec, int outer, int skip, double outer_tol, double inner_tol) {
-  k_gmres_residual_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[tiles_shape(op)], outer, skip, prec, outer_tol, inner_tol);
+  k_gmres_residual_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[tiles_shape(op)], outer, skip, prec, outer_tol, inner_tol);
 }
 
 void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PIds& po, const Phys& P, int j) {
@@ -1143,9 +1147,9 @@ void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PI
 void arnoldi_apply_fin(const Launch& L, int op, int j) {
   const BMap& one = L.bm[SH_ONE];
   const BMap& src = L.bm[tiles_shape(op)];
-  if (j < 8) k_arnoldi_dots_fin<8><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
-  else if (j < 16) k_arnoldi_dots_fin<16><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
-  else k_arnoldi_dots_fin<32><<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, one, src, j);
+  if (j < 8) k_arnoldi_dots_fin<8><<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, one, src, j);
+  else if (j < 16) k_arnoldi_dots_fin<16><<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, one, src, j);
+  else k_arnoldi_dots_fin<32><<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, one, src, j);
 }
 
 template <class VT>
@@ -1161,7 +1165,7 @@ void arnoldi_update(const Launch& L, int op, int prec, int j, int m) {
   else launch_update<float>(L, vec_shape(op), j, m, nc);
 }
 void arnoldi_update_fin(const Launch& L, int op, int prec, int j, int m) {
-  k_arnoldi_givens_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[vec_shape(op)], j, m, prec);
+  k_arnoldi_givens_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[vec_shape(op)], j, m, prec);
 }
 
 void gmres_xupdate(const Launch& L, int op, int prec, int ix) {
@@ -1183,7 +1187,7 @@ void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int
   k_newton_residual<<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
 }
 void newton_residual_fin(const Launch& L, int it, double ntol) {
-  k_newton_fin<<<GRID(L, SH_ONE), 256, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], it, ntol);
+  k_newton_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], it, ntol);
 }
 
 void newton_mbig(const Launch& L, const Phys& P, int ib, int iu) {
