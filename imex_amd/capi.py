@@ -99,7 +99,7 @@ def struct_dict(s):
     return {n: getattr(s, n) for n, _ in s._fields_}
 
 
-# ---------------------------------------------------------------- problem presets
+# ---------------------------------------------------------------- problem presets (the reference's built-in cases)
 def default_params():
     p = Params()
     lib().imex_default_params(C.byref(p))
