@@ -545,8 +545,8 @@ __device__ __forceinline__ void block_sums(const double* acc, int nv, double* re
 constexpr int FIN_THREADS = 1024;
 
 // One block: out[v] = sum over blocks b < nblk of partial[b*stride + v], v < nv <= NV.  Thread t
-// sums rows t, t + blockDim.x, ... with all their values in flight at once; the per-thread sums are
-// then reduced wave by wave in a fixed order, so the result does not depend on timing.
+// sums rows t, t + blockDim.x, ... in that order; the per-thread sums are then reduced wave by
+// wave in a fixed order, so the result does not depend on timing.
 template <int NV>
 __device__ __forceinline__ void final_sums(const double* partial, int stride, int nblk, int nv, double* out) {
   __shared__ double red[FIN_THREADS / 64][NV];
