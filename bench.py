@@ -340,9 +340,10 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}), flush=True)
 
 
-KERNEL_OF_CLASS = {   # (class, form) -> kernel symbol in the committed rocprof summary
+KERNEL_OF_CLASS = {   # (class, form) or (variant, class, form) -> kernel symbol in the committed rocprof summary
     ("potential_a", "mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<3, double, double>",
     ("potential_a", "mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<3, float, float>",
+    ("energy", "explicit_energy", "mixed"): "k_explicit<32, double, false, false>",   # the energy form keeps FP64 fluxes
     ("mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<0, double, double>",
     ("mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<0, float, float>",
     ("p_arnoldi_apply", "fp64"): "k_arnoldi_apply<1, double, double>",

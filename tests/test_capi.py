@@ -115,6 +115,23 @@ def test_bench_tables_follow_the_abi():
     assert set(b.RECON_LABEL) == set(b.RECON_NAMES.values())
 
 
+def test_bench_traffic_lookups_resolve():
+    """Every kernel class that can dominate a bench line finds its per-dispatch HBM traffic in the committed
+    rocprofv3 summary of that line's workload (the kernel names there follow the current build)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    lines = {"pressure": ["mag_arnoldi_apply", "p_arnoldi_apply"],
+             "energy": ["mag_arnoldi_apply", "e_arnoldi_apply", "explicit_energy"],
+             "potential_a": ["mag_arnoldi_apply", "p_arnoldi_apply"]}
+    for variant, classes in lines.items():
+        for cls in classes:
+            for form in ("fp64", "mixed"):
+                t = b.measured_traffic(cls, form, variant)
+                assert t is not None and t > 0, (variant, cls, form, b.kernel_of(cls, form, variant))
+
+
 def test_jet_initial_data():
     """testMHDJet (initial_conditions.cpp:671-689, constrained_transport.cpp:240-243): ambient
     (0.1 gamma, 0, 0, 0, 1, 0, B0, 0) in primitives, the inflow state in the strip's bottom ghosts, faces
