@@ -488,7 +488,15 @@ template <int EX_TX> struct ExTile {
 // ghosts with bound_cond_u outside g_bc_prim, so they hold the conservative inflow state, taken as is.
 // The ghosts of a Dirichlet side of the stencil (`amb`) hold, for the same reason, the conservative ambient
 // state: rho_amb, 0, 0, 0, e_amb + |B|^2/2 and the boundary cell's B.
-struct DirAmb { int lo, hi; double lo_v[NVAR], hi_v[NVAR]; };
+// Held as scalars and formed per component on demand, so no array in it is indexed at run time.
+struct DirAmb {
+  int lo, hi;                  // stencil cells beyond the low / high boundary
+  double rho, e, lb[3], hb[3];  // rho_amb, e_amb, and the B of the low / high boundary cell
+  __device__ __forceinline__ double val(bool low, int k) const {
+    const double b5 = low ? lb[0] : hb[0], b6 = low ? lb[1] : hb[1], b7 = low ? lb[2] : hb[2];
+    return k == 0 ? rho : k < 4 ? 0.0 : k == 4 ? e + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7) : k == 5 ? b5 : k == 6 ? b6 : b7;
+  }
+};
 template <int NC>
 __device__ __forceinline__ void prim_faces(const Phys& P, int central, const double* __restrict__ q, long long PS,
                                            long long step, int raw, const DirAmb& amb, double* Lc, double* Rc, int* ef) {
@@ -500,7 +508,7 @@ __device__ __forceinline__ void prim_faces(const Phys& P, int central, const dou
     for (int k = 0; k < NC; k++) c[k] = q[k * PS + (o - 2) * step];
     if (o < amb.lo || o >= 5 - amb.hi) {
 #pragma unroll
-      for (int k = 0; k < NC; k++) s[o][k] = o < amb.lo ? amb.lo_v[k] : amb.hi_v[k];
+      for (int k = 0; k < NC; k++) s[o][k] = amb.val(o < amb.lo, k);
     } else if (o < raw) {
 #pragma unroll
       for (int k = 0; k < NC; k++) s[o][k] = c[k];
@@ -570,20 +578,20 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     // fill puts the ambient state, with E from the face state's own B, in the ghost items
     DirAmb amb;
     amb.lo = amb.hi = 0;
+    amb.rho = d.rho_amb; amb.e = d.e_amb;
+    amb.lb[0] = amb.lb[1] = amb.lb[2] = amb.hb[0] = amb.hb[1] = amb.hb[2] = 0.0;
     const bool dpass = BND && (DIR == 0 ? d.dirx : d.diry);
     if (dpass) {
       const int t0 = DIR == 0 ? mi : mj, n = DIR == 0 ? d.nx : d.ny;
       amb.lo = max(0, GC + 2 - t0);
       amb.hi = max(0, t0 + 2 - (n + GC - 1));
-      for (int side = 0; side < 2; side++) {
-        if ((side == 0 ? amb.lo : amb.hi) == 0) continue;
-        const int tb = side == 0 ? GC : n + GC - 1;
-        const long long cb = DIR == 0 ? pidx(d, tb, mj) : pidx(d, mi, tb);
-        double* v = side == 0 ? amb.lo_v : amb.hi_v;
-        const double b5 = Qe[5 * d.PS + cb], b6 = Qe[6 * d.PS + cb], b7 = Qe[7 * d.PS + cb];
-        v[0] = d.rho_amb; v[1] = v[2] = v[3] = 0.0;
-        v[4] = d.e_amb + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
-        v[5] = b5; v[6] = b6; v[7] = b7;
+      if (amb.lo) {
+        const long long cb = DIR == 0 ? pidx(d, GC, mj) : pidx(d, mi, GC);
+        amb.lb[0] = Qe[5 * d.PS + cb]; amb.lb[1] = Qe[6 * d.PS + cb]; amb.lb[2] = Qe[7 * d.PS + cb];
+      }
+      if (amb.hi) {
+        const long long cb = DIR == 0 ? pidx(d, n + GC - 1, mj) : pidx(d, mi, n + GC - 1);
+        amb.hb[0] = Qe[5 * d.PS + cb]; amb.hb[1] = Qe[6 * d.PS + cb]; amb.hb[2] = Qe[7 * d.PS + cb];
       }
     }
     const bool dgi = dpass && (DIR == 0 ? (gi < GC || gi > d.nx + GC - 1) : (gj < GC || gj > d.ny + GC - 1));
@@ -594,7 +602,7 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
         for (int c = 0; c < 3; c++) {
           double a[5];
 #pragma unroll
-          for (int o = 0; o < 5; o++) a[o] = o < amb.lo ? amb.lo_v[5 + c] : (o >= 5 - amb.hi ? amb.hi_v[5 + c] : Qe[(5 + c) * d.PS + base + (o - 2) * step]);
+          for (int o = 0; o < 5; o++) a[o] = o < amb.lo ? amb.val(true, 5 + c) : (o >= 5 - amb.hi ? amb.val(false, 5 + c) : Qe[(5 + c) * d.PS + base + (o - 2) * step]);
           recon5(P.recon, DIR == 1 ? P.central_y : 0, a, bl[c], br[c]);
         }
         Lr[4] = (T)(d.e_amb + 0.5 * (bl[0] * bl[0] + bl[1] * bl[1] + bl[2] * bl[2]));
@@ -624,7 +632,7 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
         T a[5];
 #pragma unroll
         for (int o = 0; o < 5; o++) {
-          if constexpr (BND) a64[o] = o < amb.lo ? amb.lo_v[k] : (o >= 5 - amb.hi ? amb.hi_v[k] : q[(o - 2) * step]);
+          if constexpr (BND) a64[o] = o < amb.lo ? amb.val(true, k) : (o >= 5 - amb.hi ? amb.val(false, k) : q[(o - 2) * step]);
           else a64[o] = q[(o - 2) * step];
           a[o] = (T)a64[o];
         }
