@@ -672,7 +672,7 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
   }
   block_sums<2>(acc, 2, red, res);
   __syncthreads();
-  if (threadIdx.x < 2) gmem(I.partial)[bx_ * (MAXM + 2) + threadIdx.x] = res[threadIdx.x];
+  if (threadIdx.x < 2) gmem(I.partial)[bx_ * 2 + threadIdx.x] = res[threadIdx.x];   // packed pairs: coalesced finish
 }
 
 // local block count of instance i in the map of the kernel whose partials a finish reduces
@@ -689,7 +689,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_gmres_residual_fin(const Inst* 
   if (skipped(I, skip)) return;
   if (mixed && outer > 0 && K->outer_conv) return;
   __shared__ double tot[2];
-  final_sums<2>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 2, tot);
+  final_sums<2>(gmem(I.partial), 2, SRC_BLOCKS(src), 2, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double g0 = sqrt(tot[0]);
@@ -844,7 +844,7 @@ __global__ void __launch_bounds__(256) k_arnoldi_update(const Inst* __restrict__
     acc += (double)ws * (double)ws;
   }
   block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) gmem(I.partial)[bx_ * (MAXM + 2)] = res[0];
+  if (threadIdx.x == 0) gmem(I.partial)[bx_] = res[0];   // packed: the finish's loads coalesce
 }
 
 // One-block finish of the update: H[j+1][j] = ||w||, Givens rotations, convergence (implicit_operators.cpp:857-897).
@@ -866,7 +866,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_arnoldi_givens_fin(const Inst* 
   if (t <= j) hc[t] = H[t * L + j];
   if (t >= 1 && t <= j) { cs[t] = K->c[t]; sn[t] = K->s[t]; }
   if (t == 0) g0[0] = K->gam[j];
-  final_sums<1>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 1, tot);
+  final_sums<1>(gmem(I.partial), 1, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
   if (t == 0) {
     hc[j + 1] = sqrt(tot[0]);
@@ -1002,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
   }
   if (ef) atomicOr(gmem(I.eflag), 1);
   block_sums<1>(&acc, 1, red, res);
-  if (threadIdx.x == 0) gmem(I.partial)[bx_ * (MAXM + 2)] = res[0];
+  if (threadIdx.x == 0) gmem(I.partial)[bx_] = res[0];   // packed: the finish's loads coalesce
 }
 
 __global__ void __launch_bounds__(FIN_THREADS) k_newton_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int it, double ntol) {
@@ -1011,7 +1011,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_newton_fin(const Inst* __restri
   KrylovDev* K = gmem(I.K);
   if (it > 0 && K->newton_conv) return;
   __shared__ double tot[1];
-  final_sums<1>(gmem(I.partial), MAXM + 2, SRC_BLOCKS(src), 1, tot);
+  final_sums<1>(gmem(I.partial), 1, SRC_BLOCKS(src), 1, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = sqrt(tot[0]);
