@@ -507,7 +507,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32-operators/f64-krylov",
+        "dtype": "f64" if head == "fp64" else "f32-magnetic-operator/f64",
         "data": "synthetic (Orszag-Tang initial data generated on the host)",
         "config": {"workload": f"{WORKLOAD_NAME.get(args.workload.rstrip('0123456789'), args.workload)}_{p.nx}x{p.ny}_"
                                f"{RECON_LABEL[p.recon].lower()}_staggerct_rk2", "nx": p.nx, "ny": p.ny,
@@ -522,6 +522,8 @@ def main():
     if head == "mixed":
         out["config"]["mixed_inner_tol"] = runs["mixed"]["p"].mixed_inner_tol
         out["config"]["mixed_outer_tol"] = runs["mixed"]["p"].mixed_outer_tol
+        out["config"]["mixed_flux"] = runs["mixed"]["p"].mixed_flux
+        out["config"]["mixed_pressure"] = runs["mixed"]["p"].mixed_pressure
     if "fp64" in res and head != "fp64":
         out["fp64"] = res["fp64"]
         a, b = runs["mixed"]["final"][0], runs["fp64"]["final"][0]
@@ -532,6 +534,12 @@ def main():
                                  "steps_compared": args.warmup + args.steps}
         out["speedup_mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
     if not args.no_extra:
+        if args.formula == "pressure" and args.recon_state == "cons":
+            out["mixed_variants"] = {"workload": "the headline workload with the opt-in FP32 parts of the mixed form",
+                                     "steps": args.steps, "warmup": args.warmup}
+            for vname, vover in (("fp32_fluxes", {"mixed_flux": 1}), ("fp32_pressure_solves", {"mixed_pressure": 1})):
+                out["mixed_variants"][vname] = run_pair(args.workload, args.steps, args.warmup, sched, formula, magfield,
+                                                        forms=("mixed",), over=dict(over, **vover))["mixed"]
         out["blast"] = dict(workload="magnetic blast 1024x1024 VanLeer, transmissive, nested-Newton p_floor 1e-10",
                             steps=args.blast_steps, **run_pair("blast1024", args.blast_steps, 0, sched))
         out["batch"] = {"workload": f"{args.batch_instances} MHD vortices, sizes from {list(BATCH_SIZES)}^2, "

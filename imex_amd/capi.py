@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("formula", C.c_int), ("recon_state", C.c_int), ("magfield", C.c_int),
         ("jet_rho", C.c_double), ("jet_v", C.c_double), ("jet_B0", C.c_double), ("jet_p", C.c_double),
         ("jet_halfwidth", C.c_double), ("rho_amb", C.c_double), ("p_amb", C.c_double),
+        ("mixed_flux", C.c_int),
     ]
 
 

@@ -1624,10 +1624,15 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   c->P = make_phys(*prm);
   c->mixed = prm->precision == IMEX_PREC_MIXED ? 1 : 0;
   c->mixed_ops = c->mixed ? (1 | (prm->mixed_pressure ? 2 : 0)) : 0;
-  // Under the energy formula the explicit fluxes stay FP64: the energy-form discretisation has a
-  // slowly growing mode (about x1.1 per step on Orszag-Tang) that FP32 flux rounding would seed
-  // at 1e-7, while the pressure formula resets E from p each stage and keeps such errors flat.
-  c->mixed_flux = c->mixed && prm->formula == IMEX_FORMULA_PRESSURE && prm->recon_state == IMEX_RECONSTATE_CONS;
+  // FP32 explicit fluxes are opt-in (mixed_flux): their rounding, 1e-7 relative and uncorrelated from
+  // cell to cell, enters the right-hand sides of the magnetic and pressure solves, and the solves'
+  // tolerances then take more Arnoldi steps than the FP32 fluxes save (Orszag-Tang 1024^2: the FP64
+  // pressure solves need 470 instead of 267 Arnoldi steps over 20 steps).  Under the energy formula
+  // they stay FP64 in any case: the energy-form discretisation has a slowly growing mode (about x1.1
+  // per step on Orszag-Tang) that FP32 flux rounding would seed, while the pressure formula resets E
+  // from p each stage and keeps such errors flat.
+  c->mixed_flux = c->mixed && prm->mixed_flux && prm->formula == IMEX_FORMULA_PRESSURE &&
+                  prm->recon_state == IMEX_RECONSTATE_CONS;
   c->needA = prm->magfield == IMEX_MAGFIELD_UNSTAGGER || prm->magfield == IMEX_MAGFIELD_POTENTIAL_A;
   c->cap_tiles_p = op_prec(c, 1) ? 2048 : 1536;
   c->ni = n;

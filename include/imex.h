@@ -114,6 +114,12 @@ typedef struct imex_params {
   double rho_amb, p_amb;          /* bc = DIRICHLET (config.hpp:247): the ambient density and pressure its ghosts
                                      hold (velocity 0, B copied from the boundary cell); the pressure solve pins
                                      its boundary cells to p_amb.  Defaults 1, 0.1 */
+  int mixed_flux;                 /* mixed form: 1 = the explicit fluxes (TVDReconstruction + RusanovC_Update /
+                                     Pressure_flux_Update) also run in FP32 (pressure formula, conservative
+                                     reconstruction only); 0 (default) = they stay FP64.  FP32 flux rounding
+                                     (1e-7 relative, cell to cell) enters the right-hand sides of the magnetic and
+                                     pressure solves, whose tolerances then cost more Arnoldi steps than the FP32
+                                     fluxes save */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */

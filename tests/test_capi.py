@@ -100,9 +100,9 @@ def test_batch_create_rejects_bad_arguments_before_touching_a_device():
     assert L.imex_batch_size(None) == 0
 
 
-def test_mixed_pressure_defaults_off():
+def test_mixed_pressure_and_flux_default_off():
     p = im.default_params()
-    assert p.mixed_pressure == 0 and (p.mixed_inner_tol, p.mixed_outer_tol) == (1e-6, 1e-9)
+    assert p.mixed_pressure == 0 and p.mixed_flux == 0 and (p.mixed_inner_tol, p.mixed_outer_tol) == (1e-6, 1e-9)
 
 
 def test_bench_tables_follow_the_abi():

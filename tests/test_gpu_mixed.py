@@ -1,9 +1,10 @@
 """Mixed-precision form against the FP64 CPU oracle.
 
-The mixed form reconstructs and forms the explicit fluxes in FP32 (the conservative
-update stays FP64) and applies the magnetic operator (and, with mixed_pressure = 1, the
-pressure operator) in FP32 inside GMRES solves whose basis is stored in FP32; dot products, norms, the Hessenberg/Givens solve and the outer
-residual of the iterative refinement are FP64, and so are CT and the Newton residual.
+The mixed form applies the magnetic operator (and, with mixed_pressure = 1, the pressure
+operator) in FP32 inside GMRES solves whose basis is stored in FP32; dot products, norms,
+the Hessenberg/Givens solve and the outer residual of the iterative refinement are FP64,
+and so are CT and the Newton residual.  With mixed_flux = 1 it also reconstructs and forms
+the explicit fluxes in FP32 (the conservative update stays FP64).
 Bound: relative L2 at most 1e-6 per conserved field after N steps, div B at most 1e-12
 (CT is unchanged).  The accuracy reached is printed for each case.
 """
@@ -30,15 +31,17 @@ def run_gpu(p, u, bx, by, steps):
     return out, stats
 
 
+@pytest.mark.parametrize("flux", [0, 1])
 @pytest.mark.parametrize("case,n,m,steps", CASES)
-def test_mixed_form_within_1e6_of_oracle(case, n, m, steps):
-    print(f"[mixed] {case} {n}x{m} steps={steps}", flush=True)
+def test_mixed_form_within_1e6_of_oracle(case, n, m, steps, flux):
+    print(f"[mixed] {case} {n}x{m} steps={steps} mixed_flux={flux}", flush=True)
     p, u, bx, by = make_case(case, n, m)
     o = orc.Oracle(p)
     o.set_state(u, bx, by)
     o.step(steps)
     uc_r, up_r, bx_r, by_r, sc_r = o.get_state()
     p.precision = im.PREC_MIXED
+    p.mixed_flux = flux
     (uc, up, bxg, byg, sc), stats = run_gpu(p, u, bx, by, steps)
     errs = field_errors(uc, uc_r)
     print("  rel L2 per field:", " ".join(f"{e:.2e}" for e in errs),
@@ -67,13 +70,14 @@ def test_mixed_fp32_pressure_solves_within_1e6(case, n, m, steps):
     assert stats["div_b"] <= 1e-12
 
 
-@pytest.mark.parametrize("case,n,m,steps", [("vortex", 64, 64, 12), ("ot", 64, 64, 8)])
-def test_mixed_schedules_bitwise_identical(case, n, m, steps):
-    print(f"[mixed-schedule] {case} {n}x{m} steps={steps}", flush=True)
+@pytest.mark.parametrize("case,n,m,steps,flux", [("vortex", 64, 64, 12, 0), ("ot", 64, 64, 8, 0), ("ot", 64, 64, 8, 1)])
+def test_mixed_schedules_bitwise_identical(case, n, m, steps, flux):
+    print(f"[mixed-schedule] {case} {n}x{m} steps={steps} mixed_flux={flux}", flush=True)
     outs = []
     for sched in (im.SCHED_HOST, im.SCHED_BUDGET, im.SCHED_GRAPH):
         p, u, bx, by = make_case(case, n, m)
         p.precision = im.PREC_MIXED
+        p.mixed_flux = flux
         p.schedule = sched
         (uc, up, bxg, byg, sc), stats = run_gpu(p, u, bx, by, steps)
         print(f"  sched {sched}: budget steps {stats['budget_steps']} reruns {stats['budget_reruns']} "

@@ -100,15 +100,18 @@ def test_recon_full_run_parity(recon, state, case, n, m, steps, amp, over):
     assert np.allclose(ct, ct_ref, rtol=1e-8, atol=1e-12 * np.abs(ct_ref).max())
 
 
-@pytest.mark.parametrize("recon,state", [(im.RECON_ENO3, im.RECONSTATE_CONS), (im.RECON_WENO5, im.RECONSTATE_PRIM)])
-def test_recon_mixed_form_within_1e6(recon, state):
-    print(f"[recon mixed] recon={recon} state={state} ot 64x64 steps=6", flush=True)
+@pytest.mark.parametrize("recon,state,flux", [(im.RECON_ENO3, im.RECONSTATE_CONS, 1), (im.RECON_ENO3, im.RECONSTATE_CONS, 0),
+                                              (im.RECON_WENO5, im.RECONSTATE_PRIM, 1)])
+def test_recon_mixed_form_within_1e6(recon, state, flux):
+    """ENO3 with the opt-in FP32 fluxes chooses its stencil in FP64 and evaluates it in FP32."""
+    print(f"[recon mixed] recon={recon} state={state} mixed_flux={flux} ot 64x64 steps=6", flush=True)
     p, u, bx, by = make_case("ot", 64, 64, recon=recon, recon_state=state)
     o = orc.Oracle(p)
     o.set_state(u, bx, by)
     o.step(6)
     uc_r = o.get_state()[0]
     p.precision = im.PREC_MIXED
+    p.mixed_flux = flux
     s = im.Solver(p)
     try:
         s.set_state(u, bx, by)
