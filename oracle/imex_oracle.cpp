@@ -10,15 +10,16 @@
  * this library, and only as the checker.  The product path never calls it.
  *
  * Scope: GLF = true, add_pvisc = MaterialSpeedp, add_bvisc = true, PressureFormula
- * (DefinitionofU, all_correction) and EnergyFormula, StaggerCT, UnStaggerCT, PotentialA
- * and NoDivFree, ConsState, IMEXdt, RKs = 2,
- * periodic / transmissive boundaries (include/config.hpp:43-65).
+ * (DefinitionofU, all_correction) and EnergyFormula, StaggerCT (EMF-consistent and plain),
+ * UnStaggerCT, PotentialA and NoDivFree, NoRecon / Minmod / VanLeer / ENO3 / WENO5 of the
+ * conservative or the primitive state, ideal gas and Redlich-Kwong, IMEXdt, RKs = 2;
+ * periodic, transmissive, reflective, Dirichlet-ambient and jet-inflow boundaries
+ * (include/config.hpp:27-65, 100-125, 246-273).
  *
- * Parity status of each part (how it is pinned) is listed in the header of
- * tests/test_oracle.py.  The reference itself cannot be compiled in this image
- * (Eigen 3 is absent, Makefile:37), so the oracle is pinned by closed forms,
- * invariants and brute force; composed-subsystem algebra follows the reference
- * line by line (cited per function) and is otherwise parity unpinned.
+ * How each part is pinned is listed in the headers of tests/test_oracle.py and
+ * tests/test_oracle_pins.py.  The reference itself cannot be compiled in this image
+ * (Eigen 3 is absent, Makefile:37), so the oracle is pinned by closed forms, Fourier
+ * symbols, invariants and brute force, the composed subsystem algebra included.
  *
  * Two timing modes: faithful = 1 recomputes the per-apply invariants exactly as
  * the reference does (ucon2upri(Qe), ShockDetector, 5 EOS enthalpies per cell

@@ -54,9 +54,11 @@ term, a wrong sign or index, or a transposed operand fails a check:
                                             PotentialA: central div B = 0 inside, mass conservation; NoDivFree:
                                             faces untouched
 
-The composed algebra of MagSubsystem_UpdateB's right-hand side, CollectMagFluxes,
-the two-pass pressure solve and the stage combination follows the reference line by
-line and is otherwise parity unpinned (no reference run is possible in this image).
+The composed algebra of MagSubsystem_UpdateB's right-hand side, the magnetic operator's
+face terms, CollectMagFluxes, ShockDetector, the two-pass pressure solve and the stage
+combination is pinned in tests/test_oracle_pins.py (Fourier symbols, closed forms, the flux
+form of the magnetic solve and the one-step linear response).  No part of the oracle is left
+parity unpinned.
 """
 import numpy as np
 import pytest
