@@ -25,6 +25,7 @@ CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, CASE_JET = 0, 1, 2, 3
 SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
 TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
 WARN_GMRES_MAXOUTER, WARN_NEWTON_MAXITER, WARN_POSITIVITY, WARN_EOS_MAXITER = 1, 2, 4, 8
+ERR_ARG, ERR_HIP, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_NOMEM = -1, -2, -3, -4, -5
 
 
 class Params(C.Structure):
@@ -205,6 +206,7 @@ def lib():
         "imex_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
         "imex_reset_stats": (C.c_int, [vp]),
         "imex_div_b": (C.c_double, [vp]),
+        "imex_div_b_noalloc": (C.c_int, [C.POINTER(Params), vp, vp, vp, vp]),
         "imex_set_stage": (C.c_int, [vp, C.POINTER(StageHost), C.POINTER(Scalars)]),
         "imex_get_stage": (C.c_int, [vp, C.POINTER(StageHost), C.POINTER(Scalars)]),
         "imex_cfl": (C.c_int, [vp, C.POINTER(Scalars)]),

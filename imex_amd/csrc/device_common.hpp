@@ -247,6 +247,21 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// This thread's share of cal_divergence_error's sum (imex_subsystems.cpp:630-660, StaggerCT branch):
+// sum of dx dy (div B)^2 over interior cells t = b * blockDim + tid, t += g * blockDim, from the staggered
+// faces in the Array layout (face i+1/2 of row j at i + j * NX).
+__device__ __forceinline__ double divb_thread_sum(const double* __restrict__ bx, const double* __restrict__ by, int nx, int NX,
+                                                  long long N, double dx, double dy, long long b, long long g) {
+  double acc = 0.0;
+  for (long long t = b * blockDim.x + threadIdx.x; t < N; t += g * blockDim.x) {
+    int i = (int)(t % nx) + GC, j = (int)(t / nx) + GC;
+    long long c = (long long)i + (long long)j * NX;
+    double v = (bx[c] - bx[c - 1]) / dx + (by[c] - by[c - NX]) / dy;
+    acc += dx * dy * v * v;
+  }
+  return acc;
+}
+
 // "Last block" detection for single-pass grid reductions.  Every block calls
 // it after writing its partial; the block that returns true sees all partials.
 __device__ __forceinline__ bool last_block_arrive(unsigned* counter) {

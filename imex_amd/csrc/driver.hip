@@ -21,6 +21,7 @@
 
 static thread_local std::string g_err;
 static int set_err(int code, const std::string& m) { g_err = m; return code; }
+int imex_internal_error(int code, const std::string& msg) { return set_err(code, msg); }
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return set_err(IMEX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } while (0)
 

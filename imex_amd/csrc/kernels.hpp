@@ -3,6 +3,7 @@
 // Every kernel takes the device instance table (`const Inst*`) and a block map (`BMap`) and
 // addresses its buffers by id (batch.hpp), so one launch covers every instance of a context.
 #pragma once
+#include <string>
 #include "device_common.hpp"
 #include "batch.hpp"
 
@@ -149,3 +150,6 @@ void vslope_launch(const Launch& L, const Phys& P, int ipf, int iu);
 void axpy_launch(const Launch& L, int iy, int ix, int skip);
 void fill_launch(const Launch& L, int ibuf, double v, int skip);
 void pstats_launch(const Launch& L, int ip, double pfloor, double thr);
+
+// Records the message returned by imex_last_error and returns `code` (entry points outside driver.hip).
+int imex_internal_error(int code, const std::string& msg);

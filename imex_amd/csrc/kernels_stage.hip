@@ -1398,12 +1398,7 @@ __global__ void k_divb(const Inst* __restrict__ IT, BMap bm, int ibx, int iby) {
   double* partial = gmem(I_.partial);
 
   __shared__ double sh[16];
-  double acc = 0.0;
-  for (long long t = bx_ * (long long)blockDim.x + threadIdx.x; t < d.N; t += (long long)gx_ * blockDim.x) {
-    int i = (int)(t % d.nx) + GC, j = (int)(t / d.nx) + GC;
-    double v = (bx[pidx(d, i, j)] - bx[pidx(d, i - 1, j)]) / d.dx + (by[pidx(d, i, j)] - by[pidx(d, i, j - 1)]) / d.dy;
-    acc += d.dx * d.dy * v * v;
-  }
+  double acc = divb_thread_sum(bx, by, d.nx, d.NX, d.N, d.dx, d.dy, bx_, gx_);
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[bx_] = s;
 }

@@ -367,6 +367,19 @@ int imex_write_history(const char* file, int n, const int* steps, const double* 
  * an error if it does not fit in `cap` bytes with the terminating zero. */
 int imex_output_path(const imex_params* p, const char* root, const char* testname, char* out, int cap);
 
+/* ---- no allocation (no counterpart in the reference) ----
+ * cal_divergence_error of device-resident staggered faces, for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops.  Same metric and summation order as
+ * imex_div_b (StaggerCT branch, imex_subsystems.cpp:630-660) without a context.  bx, by: device arrays in
+ * the Array layout ((ny+6)*(nx+6) doubles, face i+1/2 of row j at i + j*(nx+6)); out: device pointer,
+ * receives sqrt(sum over interior cells of dx dy (div B)^2); p supplies nx, ny and the domain; stream: a
+ * hipStream_t (null = the default stream).  Enqueues two kernels and returns without synchronising.
+ * Allocates nothing: its partial sums live in a fixed 256-double device array of the library (one copy per
+ * GPU, held until the process ends, its address resolved on the first call on each GPU, which should
+ * therefore not itself be captured).  Calls that overlap on one GPU share that array and race.  Grids up
+ * to 32768 cells per side; larger ones return IMEX_ERR_ARG. */
+int imex_div_b_noalloc(const imex_params* p, const double* bx, const double* by, double* out, void* stream);
+
 /* ---- profiling ---- */
 /* Enable (1) or disable (0) per-launch event timing; enabling clears the totals. */
 int imex_profile_enable(imex_ctx* ctx, int on);
