@@ -71,6 +71,7 @@ def lib():
         "orc_cfl": (C.c_int, [PP, dp, SP]),
         "orc_bc": (C.c_int, [PP, dp, C.c_int]),
         "orc_map_index": (C.c_int, [PP, C.c_int, C.c_int]),
+        "orc_set_threads": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -78,6 +79,11 @@ def lib():
         f.argtypes = args
     _LIB = L
     return L
+
+
+def set_threads(n):
+    """Threads of the oracle's per-cell loops (0 keeps the current count); returns the count in force."""
+    return lib().orc_set_threads(int(n))
 
 
 def _sc(d):

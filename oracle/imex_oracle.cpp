@@ -37,6 +37,7 @@
 #include <vector>
 #include <algorithm>
 #include <functional>
+#include <omp.h>
 #include "../include/imex.h"
 
 namespace {
@@ -97,7 +98,7 @@ double rk_T_from_rho_p(Ctx& C, double rho, double p) {              // config.hp
     double f = mR * T_old / (v - mb) - ma / std::sqrt(T_old) / v / (v + mb) - p;           // :130
     double df = mR / (v - mb) + 0.5 * ma / T_old / std::sqrt(T_old) / v / (v + mb);         // :131
     T_new = T_old - f / df;
-    if (++it > 1000) { C.eos_warn = 1; break; }
+    if (++it > 1000) { __atomic_store_n(&C.eos_warn, 1, __ATOMIC_RELAXED); break; }
   } while (std::fabs(T_new - T_old) / std::fabs(T_old) > 1e-14);
   return T_new;
 }
@@ -118,7 +119,7 @@ double rk_p_from_rho_e(Ctx& C, double rho, double e) {              // config.hp
     double f = mcv * T_old + 1.5 * ma / mb * U / std::sqrt(T_old) - e;                      // :132
     double df = mcv - 0.75 * ma / mb * U / T_old / std::sqrt(T_old);                        // :133
     T_new = T_old - f / df;
-    if (++it > 1000) { C.eos_warn = 1; break; }
+    if (++it > 1000) { __atomic_store_n(&C.eos_warn, 1, __ATOMIC_RELAXED); break; }
   } while (std::fabs(T_new - T_old) / std::fabs(T_old) > 1e-14);
   double a = ma / std::sqrt(T_new);
   return mR * T_new / (v - mb) - a / v / (v + mb);
@@ -214,10 +215,12 @@ inline double calkinetic(const double* con) {                       // timestep.
   return 0.5 * rho * (vx * vx + vy * vy + vz * vz);
 }
 void grid_con2pri(Ctx& C, Vec& pri, const Vec& con) {               // state.cpp:257-270
+  #pragma omp parallel for schedule(static)
   for (int j = 0; j < C.NY; j++)
     for (int i = 0; i < C.NX; i++) con2pri(C, &pri[NV * (i + j * C.NX)], &con[NV * (i + j * C.NX)]);
 }
 void grid_pri2con(Ctx& C, Vec& con, const Vec& pri) {               // state.cpp:242-255
+  #pragma omp parallel for schedule(static)
   for (int j = 0; j < C.NY; j++)
     for (int i = 0; i < C.NX; i++) pri2con(C, &con[NV * (i + j * C.NX)], &pri[NV * (i + j * C.NX)]);
 }
@@ -376,6 +379,7 @@ void tvd_reconstruction(Ctx& C, const Vec& ucon, Recon& w) {
   bc_grid(C, u);
   w.xL.assign(ucon.size(), 0.0); w.xR.assign(ucon.size(), 0.0);
   w.yL.assign(ucon.size(), 0.0); w.yR.assign(ucon.size(), 0.0);
+  #pragma omp parallel for schedule(static)
   for (int j = GC; j < ny + GC; j++)
     for (int i = GC - 1; i < nx + GC + 1; i++)
       for (int k = 0; k < NV; k++) {
@@ -520,6 +524,7 @@ void shock_detector(Ctx& C, Vec& eta, const Vec& Qe) {
   Vec pri(Qe.size());
   grid_con2pri(C, pri, Qe);
   Vec et(C.NX * C.NY, 0.0);
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {
       double cf_min = 1.0e300;
@@ -593,6 +598,7 @@ void rusanov(const imex_scalars& S, double gam, int dir, int kind, const double*
 // Energy_flux_Update (flux.cpp:165-200).
 void flux_update(Ctx& C, const imex_scalars& S, Vec& u_con, const Vec& Qs, const Recon& w, int kind) {
   const double h = S.adt;
+  #pragma omp parallel for schedule(static)
   for (int j = GC; j < C.ny + GC; j++)
     for (int i = GC; i < C.nx + GC; i++) {
       double QL[NV], QR[NV], fp[5], fm[5], gp[5], gm[5];
@@ -616,7 +622,9 @@ void mag_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, c
   // reflective boundaries (:146-211): a neighbour's unknown is negated when the clamped neighbour index
   // is the first or last interior cell -- at the boundary cell and at the cell next to it
   const bool rx = C.P.bc_x == IMEX_BC_REFLECTIVE, ry = C.P.bc_y == IMEX_BC_REFLECTIVE;
-  int i = 0, j = 0;
+  #pragma omp parallel
+  {
+  int i = 0, j = 0;                   // per thread: the lambdas below read the current cell
   double fxl = 1, fxr = 1, fyl = 1, fyr = 1;
   auto X = [&](int a, int b, int k) {
     double f = (a < i ? fxl : a > i ? fxr : 1.0) * (b < j ? fyl : b > j ? fyr : 1.0);
@@ -624,6 +632,7 @@ void mag_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, c
   };
   auto Q = [&](int a, int b, int k) { return G(Qe, C, a, b, k); };
   auto R = [&](int a, int b) { return G(u_con, C, a, b, 0); };
+  #pragma omp for schedule(static)
   for (i = GC; i < C.nx + GC; i++)
     for (j = GC; j < C.ny + GC; j++) {
       fxl = (rx && std::max(GC, i - 1) == GC) ? -1.0 : 1.0;
@@ -692,6 +701,7 @@ void mag_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, c
       Ax[M_B(C, i, j, 6)] = ax6 - h * lap[1];
       Ax[M_B(C, i, j, 7)] = ax7 - h * lap[2];
     }
+  }
   C.st.op_applies_mag++;
 }
 
@@ -704,6 +714,7 @@ void p_coef(Ctx& C, const Vec& u_con, const Vec& Qe, PCoef& pc) {
   shock_detector(C, pc.eta, Qe);
   pc.hhat.assign(C.NX * C.NY, 0.0);
   pc.pe.assign(C.NX * C.NY, 0.0);
+  #pragma omp parallel for schedule(static)
   for (int j = 0; j < C.NY; j++)
     for (int i = 0; i < C.NX; i++) {
       A(pc.hhat, C, i, j) = calenthalpy(C, &Qe[NV * (i + j * C.NX)]) * G(Qe, C, i, j, 0) / G(u_con, C, i, j, 0);
@@ -718,6 +729,7 @@ void p_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, PCo
   auto H = [&](int a, int b) { return A(pc.hhat, C, a, b); };
   auto E = [&](int a, int b) { return A(pc.eta, C, a, b); };
   auto X = [&](int a, int b) { return x[M_P(C, a, b)]; };
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {
       double h00 = H(i, j), hp0 = H(i + 1, j), hm0 = H(i - 1, j), h0p = H(i, j + 1), h0m = H(i, j - 1);
@@ -751,7 +763,21 @@ void p_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, PCo
 }
 
 // ---------------------------------------------------------------- GMRES (implicit_operators.cpp:793-919)
-double dotv(const Vec& a, const Vec& b) { double s = 0; for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i]; return s; }
+// Sums in fixed chunks of 4096 elements, each in index order, then the chunk sums in order: the same
+// bits for any thread count.
+double dotv(const Vec& a, const Vec& b) {
+  const long long n = (long long)a.size(), CH = 4096, nc = (n + CH - 1) / CH;
+  std::vector<double> part(nc, 0.0);
+  #pragma omp parallel for schedule(static)
+  for (long long c = 0; c < nc; c++) {
+    double s = 0;
+    for (long long i = c * CH; i < std::min(n, (c + 1) * CH); i++) s += a[i] * b[i];
+    part[c] = s;
+  }
+  double s = 0;
+  for (long long c = 0; c < nc; c++) s += part[c];
+  return s;
+}
 double normv(const Vec& a) { return std::sqrt(dotv(a, a)); }
 
 int gmres(Ctx& C, Vec& x, const Vec& b, const std::function<void(const double*, double*)>& Aop,
@@ -770,12 +796,14 @@ int gmres(Ctx& C, Vec& x, const Vec& b, const std::function<void(const double*, 
   for (int outer = 0; outer < outmax; outer++) {
     std::fill(Ax.begin(), Ax.end(), 0.0);
     Aop(x.data(), Ax.data());
+    #pragma omp parallel for schedule(static)
     for (int n = 0; n < dim; n++) tmp[n] = b[n] - Ax[n];
     applyM(r0, tmp);
     gam[0] = normv(r0);
     if (outer == 0) { tol_eff = C.P.gmres_tol * gam[0] + C.P.gmres_tol; if (info) info->res0 = gam[0]; }
     if (info) info->res = gam[0];
     if (gam[0] <= tol_eff) { if (iters) *iters += total; if (restarts) *restarts += outer; if (info) { info->iters = total; info->restarts = outer; } return 0; }
+    #pragma omp parallel for schedule(static)
     for (int n = 0; n < dim; n++) v[0][n] = r0[n] / gam[0];
     std::fill(Ax.begin(), Ax.end(), 0.0);
     Aop(v[0].data(), Ax.data());
@@ -785,7 +813,11 @@ int gmres(Ctx& C, Vec& x, const Vec& b, const std::function<void(const double*, 
       total++;
       for (int i = 0; i <= j; i++) H[i][j] = dotv(v[i], Av[j]);         // classical Gram-Schmidt (:850-852)
       w = Av[j];
-      for (int i = 0; i <= j; i++) { double hij = H[i][j]; for (int n = 0; n < dim; n++) w[n] -= hij * v[i][n]; }
+      for (int i = 0; i <= j; i++) {
+        double hij = H[i][j];
+        #pragma omp parallel for schedule(static)
+        for (int n = 0; n < dim; n++) w[n] -= hij * v[i][n];
+      }
       H[j + 1][j] = normv(w);
       for (int i = 0; i < j; i++) {
         double t1 = c[i + 1] * H[i][j] + s[i + 1] * H[i + 1][j];
@@ -801,6 +833,7 @@ int gmres(Ctx& C, Vec& x, const Vec& b, const std::function<void(const double*, 
       if (info) info->res = std::fabs(gam[j + 1]);
       if (gam[j + 1] > tol_eff) {
         double hn = H[j + 1][j];
+        #pragma omp parallel for schedule(static)
         for (int n = 0; n < dim; n++) v[j + 1][n] = w[n] / hn;
         std::fill(Ax.begin(), Ax.end(), 0.0);
         Aop(v[j + 1].data(), Ax.data());
@@ -839,6 +872,7 @@ int pressure_newton(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& 
   const double h = S.adt, dx = C.dx, dy = C.dy;
   const double a = 1.0 / (C.P.gamma - 1.0);
   Vec g(N), dp(N), neg_g(N), Lp(N), Tdiag(N), p_freeze;
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {
       double h00 = A(pc.hhat, C, i, j), hp0 = A(pc.hhat, C, i + 1, j), hm0 = A(pc.hhat, C, i - 1, j);
@@ -847,6 +881,7 @@ int pressure_newton(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& 
     }
   bool jac = false;
   std::function<void(double*, const double*)> precP = [&](double* out, const double* in) {
+    #pragma omp parallel for schedule(static)
     for (int i = GC; i < C.nx + GC; i++)
       for (int j = GC; j < C.ny + GC; j++) {
         int idx = M_P(C, i, j);
@@ -886,6 +921,7 @@ int pressure_newton(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& 
   for (it = 0; it < C.P.newton_max; it++) {
     jac = false;
     Aop(p_freeze.data(), Lp.data());
+    #pragma omp parallel for schedule(static)
     for (int i = GC; i < C.nx + GC; i++)
       for (int j = GC; j < C.ny + GC; j++) {
         int idx = M_P(C, i, j);
@@ -924,6 +960,7 @@ void collect_mag_fluxes(Ctx& C, const imex_scalars& S, const Vec& u, const Vec& 
   auto Q = [&](int a, int b, int k) { return G(Qe, C, a, b, k); };
   auto U = [&](int a, int b, int k) { return G(u, C, a, b, k); };
   auto PE = [&](int a, int b) { return A(pe, C, a, b); };
+  #pragma omp parallel for schedule(static)
   for (int i = 1; i < C.NX - 1; i++)
     for (int j = 1; j < C.NY - 1; j++) {       // ct_flux.cpp:14-127
       double rho_p0 = 0.5 * (U(i, j, 0) + U(i + 1, j, 0)), rho_0p = 0.5 * (U(i, j, 0) + U(i, j + 1, 0));
@@ -976,6 +1013,7 @@ int mag_update_b(Ctx& C, const imex_scalars& S, Stage& W) {
   auto Q = [&](int a, int bb, int k) { return G(W.Qe, C, a, bb, k); };
   auto U = [&](int a, int bb, int k) { return G(u, C, a, bb, k); };
   auto PE = [&](int a, int bb) { return A(pe, C, a, bb); };
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {      // imex_subsystems.cpp:549-613
       double rho_p0 = 0.5 * (U(i, j, 0) + U(i + 1, j, 0)), rho_m0 = 0.5 * (U(i, j, 0) + U(i - 1, j, 0));
@@ -1038,6 +1076,7 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
   p_coef(C, W.u_con, W.Qe, pc);
   Vec b(N), p(N);
   auto rhs = [&](int pass) {
+    #pragma omp parallel for schedule(static)
     for (int i = GC; i < C.nx + GC; i++)
       for (int j = GC; j < C.ny + GC; j++) {
         const Vec& u = W.u_con;
@@ -1058,6 +1097,7 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
   };
   rhs(0);
   int rc = pressure_newton(C, S, W.u_con, W.Qe, pc, p, b, nullptr);
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {      // momentum update :186-215
       double flux_xm = 0.5 * (p[M_P(C, i - 1, j)] + p[M_P(C, i, j)]);
@@ -1071,6 +1111,7 @@ int pre_subsystem_pressure(Ctx& C, const imex_scalars& S, Stage& W) {
   bc_grid(C, W.u_con);
   rhs(1);                                       // all_correction second pass :223-284
   rc |= pressure_newton(C, S, W.u_con, W.Qe, pc, p, b, nullptr);
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {      // DefinitionofU :286-302
       Vec& u = W.u_con;
@@ -1092,6 +1133,7 @@ inline double caltotalenthalpy(Ctx& C, const double* con) {        // timestep.c
 // its four neighbours, ghost cells included, inside each use)
 void e_coef(Ctx& C, const Vec& u_con, const Vec& Qe, Vec& hE) {
   hE.assign(C.NX * C.NY, 0.0);
+  #pragma omp parallel for schedule(static)
   for (int j = 0; j < C.NY; j++)
     for (int i = 0; i < C.NX; i++)
       A(hE, C, i, j) = caltotalenthalpy(C, &Qe[NV * (i + j * C.NX)]) * G(Qe, C, i, j, 0) / G(u_con, C, i, j, 0);
@@ -1101,6 +1143,7 @@ void e_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, Vec
   if (C.faithful) e_coef(C, u_con, Qe, hE);   // the reference recomputes these inside every apply (:742-767)
   const double h = S.adt, dx = C.dx, dy = C.dy, g = C.P.gamma;
   auto X = [&](int a, int b) { return x[M_P(C, a, b)]; };
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {
       double h00 = A(hE, C, i, j), hp0 = A(hE, C, i + 1, j), hm0 = A(hE, C, i - 1, j);
@@ -1194,6 +1237,7 @@ void corner_assemble(const Ctx& C, Vec& Ez_pp, const Vec& Ez_p0, const Vec& Ez_0
       A(bar, C, i, j) = 0.25 * (A(Ez_p0, C, i, j) + A(Ez_p0, C, i, j + 1) + A(Ez_0p, C, i, j) + A(Ez_0p, C, i + 1, j));
   bc_array(C, bar);
   const double dx = C.dx, dy = C.dy;
+  #pragma omp parallel for schedule(static)
   for (int j = 1; j < C.NY - 2; j++)
     for (int i = 1; i < C.NX - 2; i++) {
       double t1 = 0.5 * ((A(Ez_p0, C, i, j) - A(Et, C, i, j)) / (dx / 2.0) + (A(Ez_p0, C, i, j + 1) - A(Et, C, i, j + 1)) / (dx / 2.0));
@@ -1326,6 +1370,7 @@ void pot_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, c
   bc_A(C, AG);
   auto a = [&](int ii, int jj, int k) { return AG[NV * (ii + jj * C.NX) + k]; };
   auto Q = [&](int ii, int jj, int k) { return G(Qe, C, ii, jj, k); };
+  #pragma omp parallel for schedule(static)
   for (int i = GC; i < C.nx + GC; i++)
     for (int j = GC; j < C.ny + GC; j++) {
       double rho_00 = G(u_con, C, i, j, 0);
@@ -1662,6 +1707,7 @@ int run_step(Run& R) {
     W.bx_qs = R.bx_n; W.by_qs = R.by_n;
     for (int r = 0; r < rks; r++) {
       double ce = Ae[rks][r] * S.dt, ci = Ai[rks][r] * S.dt;
+      #pragma omp parallel for schedule(static)
       for (size_t n = 0; n < NG; n++) { W.Qe[n] = W.Qe[n] + ce * R.Hs[r][n]; W.Qs[n] = W.Qs[n] + ci * R.Hs[r][n]; }
       for (size_t n = 0; n < NA; n++) { W.bx_qs[n] += ci * R.HsB[r][0][n]; W.by_qs[n] += ci * R.HsB[r][1][n]; }
     }
@@ -1710,6 +1756,7 @@ int run_step(Run& R) {
     }
     grid_con2pri(C, R.u_pri, R.u_con);
     if (R.Hs[rks].size() != NG) R.Hs[rks].assign(NG, 0.0);
+    #pragma omp parallel for schedule(static)
     for (size_t n = 0; n < NG; n++) R.Hs[rks][n] = (R.u_con[n] - W.Qs[n]) / S.adt;
     for (int c = 0; c < 2; c++) if (R.HsB[rks][c].size() != NA) R.HsB[rks][c].assign(NA, 0.0);
     for (size_t n = 0; n < NA; n++) { R.HsB[rks][0][n] = (R.bx[n] - W.bx_qs[n]) / S.adt; R.HsB[rks][1][n] = (R.by[n] - W.by_qs[n]) / S.adt; }
@@ -1808,6 +1855,12 @@ int orc_capture_potential(void* h, double* A, double* As) {
   return 0;
 }
 void orc_destroy(void* h) { delete (Run*)h; }
+
+// Threads of the per-cell loops (results do not depend on it); returns the count now in force.
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 
 int orc_set_state(void* h, const double* u_pri, const double* bx, const double* by, double t, int timestep, double prev_dt) {
   Run& R = *(Run*)h;
