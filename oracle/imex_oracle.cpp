@@ -632,9 +632,10 @@ void mag_apply(Ctx& C, const imex_scalars& S, const Vec& u_con, const Vec& Qe, c
   };
   auto Q = [&](int a, int b, int k) { return G(Qe, C, a, b, k); };
   auto R = [&](int a, int b) { return G(u_con, C, a, b, 0); };
+  // the worksharing loop runs over its own index: the lambdas above hold references to this thread's i, j
   #pragma omp for schedule(static)
-  for (i = GC; i < C.nx + GC; i++)
-    for (j = GC; j < C.ny + GC; j++) {
+  for (int ii = GC; ii < C.nx + GC; ii++)
+    for (i = ii, j = GC; j < C.ny + GC; j++) {
       fxl = (rx && std::max(GC, i - 1) == GC) ? -1.0 : 1.0;
       fxr = (rx && std::min(C.nx + GC - 1, i + 1) == C.nx + GC - 1) ? -1.0 : 1.0;
       fyl = (ry && std::max(GC, j - 1) == GC) ? -1.0 : 1.0;
