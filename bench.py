@@ -311,21 +311,73 @@ def run_adjoint(n, steps, every, ndir, seed, eps=1e-6, noise=1e-3):
             "worst_solve_residual": info["worst_solve_residual"]}
 
 
-def cpu_baseline(name, state, budget_steps=1, formula=None, magfield=None, potential=None, over=None):
-    """FP64 oracle (faithful: per-apply recomputation like the reference), 1 core, from the GPU's post-warmup state."""
+def cpu_baseline(name, state, budget_steps=1, formula=None, magfield=None, potential=None, over=None, threads=1,
+                 faithful=1):
+    """FP64 oracle from the GPU's post-warmup state: faithful mode (per-apply recomputation like the reference)
+    on one core, or the hoisted mode on `threads` cores."""
     import imex_amd as im
     import oracle as orc
     p, kw = im.case_of(name)
     apply_over(p, formula, magfield, over)
     uc, up, bx, by, sc = state
-    o = orc.Oracle(p, 1)
-    o.set_state(up, bx, by, sc["t"], sc["timestep"], sc["prev_dt"])
-    if potential is not None:
-        o.set_potential(potential)
+    used = orc.set_threads(threads)
+    try:
+        o = orc.Oracle(p, faithful)
+        o.set_state(up, bx, by, sc["t"], sc["timestep"], sc["prev_dt"])
+        if potential is not None:
+            o.set_potential(potential)
+        t0 = time.perf_counter()
+        o.step(budget_steps)
+        t1 = time.perf_counter()
+    finally:
+        orc.set_threads(os.cpu_count() or 1)
+    return p.nx * p.ny * budget_steps / (t1 - t0) / 1e6, t1 - t0, used
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_accuracy(name, steps, formula=None, magfield=None, over=None):
+    """Relative L2 per conserved field of the FP64 and mixed GPU forms against the oracle (all host cores,
+    hoisted mode) after `steps` steps from the initial data."""
+    import numpy as np
+    import imex_amd as im
+    import oracle as orc
+    p, kw = im.case_of(name)
+    apply_over(p, formula, magfield, over)
+    u, bx, by = im.init_case(p, kw["case"], kw.get("amp", 1.0), kw.get("u0", 0.0), kw.get("v0", 0.0))
+    orc.set_threads(os.cpu_count() or 1)
+    o = orc.Oracle(p)
+    o.set_state(u, bx, by)
     t0 = time.perf_counter()
-    o.step(budget_steps)
-    t1 = time.perf_counter()
-    return p.nx * p.ny * budget_steps / (t1 - t0) / 1e6, t1 - t0
+    o.step(steps)
+    t_orc = time.perf_counter() - t0
+    ref = o.get_state()[0]
+    g = im.GC
+    out = {"steps": steps, "oracle_s": round(t_orc, 1)}
+    for f, prec in (("fp64", im.PREC_FP64), ("mixed", im.PREC_MIXED)):
+        q, _ = im.case_of(name)
+        apply_over(q, formula, magfield, over)
+        q.precision = prec
+        s = im.Solver(q)
+        try:
+            s.set_state(u, bx, by)
+            s.step(steps)
+            uc = s.get_state()[0]
+        finally:
+            s.close()
+        errs = [float(np.linalg.norm(uc[g:-g, g:-g, k] - ref[g:-g, g:-g, k]) / max(np.linalg.norm(ref[g:-g, g:-g, k]), 1e-300))
+                for k in range(8)]
+        out[f"{f}_rel_l2_vs_oracle_max"] = float(f"{max(errs):.3e}")
+        out[f"{f}_rel_l2_vs_oracle_per_field"] = [float(f"{e:.3e}") for e in errs]
+    return out
 
 
 def copy_calibration(mib):
@@ -428,7 +480,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", default="graph", choices=["graph", "budget", "host"])
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--oracle-steps", type=int, default=3,
+                    help="steps of the headline workload from its initial data for the forms' error against the oracle")
     ap.add_argument("--mixed-tols", type=float, nargs=2, default=None, metavar=("INNER", "OUTER"),
                     help="override the mixed form's inner / outer refinement tolerances")
     ap.add_argument("--no-extra", action="store_true",
@@ -576,13 +630,23 @@ def main():
                                   **run_diag_cost(args.workload, 20, 3, sched))
         out["adjoint"] = run_adjoint(256, 20, 5, args.adjoint_dirs, 1)
     if not args.no_cpu:
+        log("[bench] oracle accuracy of both forms")
+        out["oracle_accuracy"] = dict(workload="the headline workload from its initial data, both GPU forms against "
+                                               "the FP64 oracle", **oracle_accuracy(args.workload, args.oracle_steps,
+                                                                                    formula, magfield, over))
         log("[bench] CPU oracle sample")
-        v, secs = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula, magfield, runs[head]["potential"],
-                               over)
-        out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{args.cpu_steps} step(s) of {p.nx}x{p.ny} Orszag-Tang ({args.formula} formula, "
-                                         f"{args.magfield}) "
-                                         f"from the post-warmup state, faithful mode, {secs:.1f} s"}
+        v, secs, used = cpu_baseline(args.workload, runs[head]["state"], args.cpu_steps, formula, magfield,
+                                     runs[head]["potential"], over)
+        out["cpu_baseline"] = {"value": round(v, 5), "unit": "Mcell-steps/s", "cores": used, "kind": "port",
+                               "sample": f"{args.cpu_steps} steps of {p.nx}x{p.ny} Orszag-Tang ({args.formula} formula, "
+                                         f"{args.magfield}) from the post-warmup state, faithful mode (the reference's "
+                                         f"per-apply recomputation), {secs:.1f} s on {used} core",
+                               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+        va, secs_a, used_a = cpu_baseline(args.workload, runs[head]["state"], 1, formula, magfield, runs[head]["potential"],
+                                          over, threads=os.cpu_count() or 1, faithful=0)
+        out["cpu_all_cores"] = {"value": round(va, 5), "unit": "Mcell-steps/s", "cores": used_a,
+                                "sample": f"1 step, hoisted mode (invariants computed once per stage), OpenMP over the "
+                                          f"per-cell loops, {secs_a:.1f} s"}
     print(json.dumps(out), flush=True)
 
 
