@@ -59,18 +59,15 @@ __global__ void k_budget_gmres_check(const Inst* __restrict__ IT, BMap bm, int s
   const KrylovDev* K = gmem(I.K);
   int* ctl = gmem(I.ctl);
   const int t = threadIdx.x;
+  // sweeps: the mixed form's refinement sweeps (the residual of sweep `outer` found it converged),
+  // or the FP64 form's restart cycles (cycle `outer` converged)
+  const int sweeps = mixed ? K->outer : K->outer + 1;
   if (t == 0) {
-    if (mixed) {
-      ctl[slot] = K->steps_total;
-      ctl[slot + 1] = K->outer;
-      if (!K->outer_conv) ctl[0] |= 1;
-    } else {
-      ctl[slot] = K->steps;
-      ctl[slot + 1] = 1;
-      if (!K->converged) ctl[0] |= 1;
-    }
+    ctl[slot] = K->steps_total;
+    ctl[slot + 1] = sweeps;
+    if (!K->outer_conv) ctl[0] |= 1;
   }
-  if (t < nsw) ctl[slot + 2 + t] = mixed ? (t < K->outer ? K->sweep_steps[t] : 0) : K->steps;
+  if (t < nsw) ctl[slot + 2 + t] = t < sweeps && t < MAXSWEEP ? K->sweep_steps[t] : 0;
 }
 __global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int slot) {
   BATCH_BLOCK(bm);
@@ -467,7 +464,8 @@ static void cur_gmres(imex_ctx* c, int key, int steps, int sweeps) {
 
 // ---------------------------------------------------------------- GMRES, budgeted schedule
 // GMRES_Solver (implicit_operators.cpp:823-911) with a fixed number of launches: the FP64 form
-// runs one restart cycle of B Arnoldi steps; the mixed form runs nsw refinement sweeps (FP64
+// runs nsw restart cycles (residual, B Arnoldi steps, x update; implicit_operators.cpp:899-910),
+// the last of which must have converged; the mixed form runs nsw refinement sweeps (FP64
 // residual, B FP32 Arnoldi steps, x update) and the FP64 residual that must have converged.
 // The kernels are the ones the host-polled loop below uses, so a solve that converges within
 // its budget gives the same result bit for bit.  Launches past convergence exit at once; a solve
@@ -481,7 +479,7 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
   BudgetRec r;
   r.kind = 0; r.op = op; r.parent = parent; r.it = it; r.key = gkey(c);
   r.B = std::min(m, std::max(1, c->tab.B[r.key]));
-  r.nsw = prec ? std::min(MAXSWEEP - 1, std::max(1, c->tab.S[r.key])) : 1;
+  r.nsw = std::min(MAXSWEEP - 1, std::max(1, c->tab.S[r.key]));
   r.slot = c->nslot; c->nslot += 2 + r.nsw;
   for (int sw = 0; sw <= r.nsw; sw++) {
     if (sw == r.nsw && !prec) break;
@@ -617,15 +615,11 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
   if (prec) c->stats.mixed_refinements += outer;
   else if (rs_acc) *rs_acc += outer;
   if (total > 0 && outer == 0) c->pred[op] = total;
-  if (prec) {
-    c->step_mx[op] = std::max(c->step_mx[op], maxsweep);
-    c->step_mx[3 + op] = std::max(c->step_mx[3 + op], (int)outer);
-    cur_gmres(c, gkey(c), maxsweep, (int)outer);
-  } else {
-    c->step_mx[op] = std::max(c->step_mx[op], outer == 0 ? total : MAXM + 1);
-    c->step_mx[3 + op] = std::max(c->step_mx[3 + op], 1);
-    cur_gmres(c, gkey(c), outer == 0 ? total : MAXM + 1, 1);
-  }
+  // budgets learn the longest sweep (or restart cycle) and the number of sweeps (cycles)
+  const int nsweeps = prec ? (int)outer : (int)std::min<long long>(outer + 1, MAXSWEEP);
+  c->step_mx[op] = std::max(c->step_mx[op], maxsweep);
+  c->step_mx[3 + op] = std::max(c->step_mx[3 + op], nsweeps);
+  cur_gmres(c, gkey(c), maxsweep, nsweeps);
   if (info) { info->iters = total; info->restarts = (int)outer; info->res0 = res0; info->res = res; info->status = rc; }
   return rc;
 }
@@ -940,9 +934,10 @@ static void hist_push(imex_ctx* c, const int mx[NBK]) {
 
 // Budgets, per solve position: the largest count that position needed in the recent steps
 // (or, for a position not seen lately, the largest of its kind), plus budget_margin Arnoldi
-// steps per GMRES sweep, plus the extra margins that recent overruns have raised.  Returns false (host scheduling) until there is history, or when a
-// recent solve needed a restart.  A batch has no host scheduling: it starts from generous
-// budgets and grows them when a step outruns them.
+// steps per GMRES sweep (restart cycle), plus the extra margins that recent overruns have raised.
+// Returns false (host scheduling) until there is history, or when a recent solve needed more
+// sweeps or restart cycles than a budget can hold.  A batch has no host scheduling: it starts
+// from generous budgets and grows them when a step outruns them.
 static bool choose_budgets(imex_ctx* c) {
   if (c->taping) return false;
   if (c->ni == 1 && (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0)) return false;
@@ -950,8 +945,7 @@ static bool choose_budgets(imex_ctx* c) {
   int mx[NBK] = {0, 0, 0, 0, 0};
   if (c->hist_n == 0) { mx[0] = mx[1] = std::min(m - 1, 11); mx[2] = 2; mx[3] = mx[4] = 3; }
   for (int i = 0; i < c->hist_n; i++) for (int k = 0; k < NBK; k++) mx[k] = std::max(mx[k], c->hist[k][i]);
-  if (c->ni == 1 && (mx[0] >= m || mx[1] >= m)) return false;              // a recent solve needed a restart
-  if (c->ni == 1 && (mx[3] >= MAXSWEEP - 1 || mx[4] >= MAXSWEEP - 1)) return false;
+  if (c->ni == 1 && (mx[3] >= MAXSWEEP - 1 || mx[4] >= MAXSWEEP - 1)) return false;   // too many sweeps or cycles
   for (int k = 0; k < NGKEY; k++) {
     const int op = (k / NIT) % 3 == 0 ? 0 : 1;
     int st = -1, sw = -1;
@@ -996,8 +990,9 @@ static int budget_finish(imex_ctx* c, long long gen0) {
       }
       const int steps = ctl[r.slot], sweeps = ctl[r.slot + 1];
       const bool skipped = r.parent >= 0 && r.it >= ctl[c->brecs[r.parent].slot];
+      const int rp0 = op_prec(c, r.op);
       for (int sw = 0; sw < (int)r.rres.size(); sw++) {
-        const bool live = !skipped && sw <= sweeps;                       // residuals that ran
+        const bool live = !skipped && (rp0 ? sw <= sweeps : sw < sweeps);   // residuals that ran
         if (!live) { idle(r.rres[sw]); idle(r.rresf[sw]); }
       }
       for (int sw = 0; sw < r.nsw; sw++) {
@@ -1012,8 +1007,10 @@ static int budget_finish(imex_ctx* c, long long gen0) {
       if (skipped) continue;
       mx[3 + r.op] = std::max(mx[3 + r.op], sweeps);
       const int rp = op_prec(c, r.op);
-      cur_gmres(c, r.key, rp ? 0 : steps, sweeps);
+      cur_gmres(c, r.key, 0, sweeps);
       if (rp) c->stats.mixed_refinements += sweeps;
+      else if (r.op == 0) c->stats.mag_gmres_restarts += std::max(0, sweeps - 1);
+      else c->stats.p_gmres_restarts += std::max(0, sweeps - 1);
       if (r.op == 0) { c->stats.mag_solves++; c->stats.mag_gmres_iters += steps; }
       else { c->stats.p_solves++; c->stats.p_gmres_iters += steps; }
     }
@@ -1030,10 +1027,10 @@ static bool tag_enabled(const imex_ctx* c, const NodeTag& t, const BudTab& tb) {
   const bool live = sub == 0 || it < tb.N[t.key / NIT];      // the Newton iteration holding the solve runs
   if (!live) return false;
   const int mixed = key_prec(c, t.key);
-  const int S = mixed ? tb.S[t.key] : 1;
+  const int S = tb.S[t.key];
   switch (t.kind) {
     case TG_ARN: return t.sw < S && t.j < tb.B[t.key];
-    case TG_RES: return !mixed || t.sw <= S;
+    case TG_RES: return mixed ? t.sw <= S : t.sw < S;       // the mixed form's closing residual
     case TG_XUPD: return t.sw < S;
     default: return true;                                     // TG_GCHK
   }
@@ -1042,7 +1039,7 @@ static bool tag_enabled(const imex_ctx* c, const NodeTag& t, const BudTab& tb) {
 static bool tab_covers(const imex_ctx* c, const BudTab& cap, const BudTab& t) {
   for (int k = 0; k < NGKEY; k++) {
     if (t.B[k] > cap.B[k]) return false;
-    if (key_prec(c, k) && t.S[k] > cap.S[k]) return false;
+    if (t.S[k] > cap.S[k]) return false;
   }
   for (int k = 0; k < NNKEY; k++) if (t.N[k] > cap.N[k]) return false;
   return true;
@@ -1077,7 +1074,9 @@ static int budget_graph_launch(imex_ctx* c) {
       const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
       for (int k = 0; k < NGKEY; k++) {
         cap.B[k] = std::min(m, cur.B[k] + 2);
-        cap.S[k] = key_prec(c, k) ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : 1;
+        // a refinement sweep more for the mixed form; the FP64 form gets a restart cycle more only
+        // where one has been needed
+        cap.S[k] = (key_prec(c, k) || cur.S[k] > 1) ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : cur.S[k];
       }
       for (int k = 0; k < NNKEY; k++) cap.N[k] = std::min(std::min(c->prm.newton_max, NIT - 1), cur.N[k] + 1);
     }
@@ -1169,7 +1168,7 @@ static int one_step(imex_ctx* c) {
     c->fit_streak = 0;
     if (c->ni > 1 && (f & 1)) {     // a single instance reruns the step host-scheduled instead
       c->margin_extra = std::min(8, c->margin_extra + 2);
-      if (c->mixed_ops) c->sweep_extra = std::min(4, c->sweep_extra + 1);
+      c->sweep_extra = std::min(4, c->sweep_extra + 1);
     }
     if (c->ni > 1 && (f & 2)) c->newton_extra = std::min(4, c->newton_extra + 1);
     // a solve outran its budget: restore the step's scalars and rerun it (host scheduling for a

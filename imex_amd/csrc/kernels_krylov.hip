@@ -631,7 +631,10 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
     }
     return;
   }
-  if (mixed && outer > 0 && K->outer_conv) return;    // refinement finished in an earlier sweep
+  if (outer > 0 && K->outer_conv) {      // solved in an earlier sweep or restart cycle: nothing to update
+    if (bx_ == 0 && threadIdx.x == 0) K->jlast = -1;
+    return;
+  }
   typedef OpSel<OPK, double> S;
   constexpr int NC = S::NC;
   const typename S::Op op = S::make(I, P, ids);
@@ -687,7 +690,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_gmres_residual_fin(const Inst* 
   const Inst& I = IT[inst_];
   KrylovDev* K = gmem(I.K);
   if (skipped(I, skip)) return;
-  if (mixed && outer > 0 && K->outer_conv) return;
+  if (outer > 0 && K->outer_conv) return;
   __shared__ double tot[2];
   final_sums<2>(gmem(I.partial), 2, SRC_BLOCKS(src), 2, tot);
   __syncthreads();
@@ -893,6 +896,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_arnoldi_givens_fin(const Inst* 
       if (j + 1 == m) { K->done = 1; K->converged = 0; K->jlast = j; }
     } else {
       K->done = 1; K->converged = mixed ? 0 : 1; K->jlast = j;
+      if (!mixed) K->outer_conv = 1;       // later restart cycles of a budgeted solve find it solved
     }
   }
   __syncthreads();

@@ -85,6 +85,48 @@ def test_batch_matches_single_runs_bitwise(eos, precision, schedule):
         assert st[k] == tot[k], (k, st[k], tot[k])
 
 
+def test_batch_with_restarting_instances_matches_single_runs():
+    """Magnetic blasts of several strengths: the strong ones need GMRES restart cycles, which
+    the batch runs inside its budgets (implicit_operators.cpp:899-910) while the weak ones
+    have stopped; every instance still ends where its host-scheduled single run ends."""
+    steps = 5
+    inst = [(48, 40, 100.0), (40, 40, 10.0), (32, 48, 60.0)]
+    print(f"[batch] magnetic blasts {inst} steps={steps}", flush=True)
+    p = im.blast_params(48)
+    p.schedule = im.SCHED_GRAPH
+    data = []
+    for nx, ny, amp in inst:
+        q = im.Params.from_buffer_copy(p)
+        q.nx, q.ny = nx, ny
+        data.append((q,) + im.init_case(q, im.CASE_BLAST, amp, 0.0, 0.0, 0))
+    b = im.Batch(p, [(nx, ny) for nx, ny, _ in inst])
+    try:
+        for i, (q, u, bx, by) in enumerate(data):
+            b.set_instance(i, u, bx, by)
+        _, st = b.step(steps)
+        outs = [b.get_instance(i) for i in range(len(inst))]
+    finally:
+        b.close()
+    print(f"  batch: budget steps {st['budget_steps']} reruns {st['budget_reruns']} graphs {st['graph_builds']}; "
+          + " ".join(f"{k}={st[k]}" for k in COUNTS + ("mag_gmres_restarts", "p_gmres_restarts")), flush=True)
+    assert st["budget_steps"] >= steps
+    restarts = []
+    tot = dict.fromkeys(COUNTS + ("mag_gmres_restarts", "p_gmres_restarts"), 0)
+    for i, (q, u, bx, by) in enumerate(data):
+        uc, bxg, byg, sc, sst = run_single(q, u, bx, by, steps)
+        restarts.append(sst["mag_gmres_restarts"] + sst["p_gmres_restarts"])
+        bu, _, bbx, bby, bsc = outs[i]
+        assert np.array_equal(bu, uc), f"instance {i} state differs"
+        assert np.array_equal(bbx, bxg) and np.array_equal(bby, byg), f"instance {i} faces differ"
+        assert bsc["t"] == sc["t"] and bsc["dt"] == sc["dt"]
+        for k in tot:
+            tot[k] += sst[k]
+    print(f"  restart cycles of the single runs: {restarts}", flush=True)
+    assert max(restarts) > 0
+    for k in tot:
+        assert st[k] == tot[k], (k, st[k], tot[k])
+
+
 def test_batch_of_one_equals_solver():
     print("[batch] one instance", flush=True)
     p = base_params(im.EOS_IDEAL, im.PREC_FP64)
