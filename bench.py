@@ -479,7 +479,9 @@ def main():
     ap.add_argument("--workload", default="ot1024")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--schedule", default="graph", choices=["graph", "budget", "host"])
+    ap.add_argument("--schedule", default=None, choices=["auto", "graph", "budget", "host"],
+                    help="step schedule; by default graph for the Orszag-Tang 1024^2 runs (fastest there: "
+                         "steady iteration counts) and the library's automatic choice for the others")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--oracle-steps", type=int, default=3,
                     help="steps of the headline workload from its initial data for the forms' error against the oracle")
@@ -522,12 +524,15 @@ def main():
         sys.exit(2)
     import numpy as np
     import imex_amd as im
-    sched = {"graph": im.SCHED_GRAPH, "budget": im.SCHED_BUDGET, "host": im.SCHED_HOST}[args.schedule]
+    SCHEDS = {"auto": im.SCHED_AUTO, "graph": im.SCHED_GRAPH, "budget": im.SCHED_BUDGET, "host": im.SCHED_HOST}
+    sched_name = args.schedule or ("graph" if args.workload.startswith("ot1024") else "auto")
+    sched = SCHEDS[sched_name]
+    sched_other = SCHEDS[args.schedule or "auto"]      # the blast, jet and batch workloads
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(256, 20, 5, args.adjoint_dirs, 1)}), flush=True)
         return
     if args.batch_only:
-        res = {e: run_batch(args.batch_instances, args.batch_steps, 1, eos, 2024, sched)
+        res = {e: run_batch(args.batch_instances, args.batch_steps, 1, eos, 2024, sched_other)
                for e, eos in (("ideal", im.EOS_IDEAL), ("redlich_kwong", im.EOS_RK))}
         print(json.dumps({"batch": res}), flush=True)
         return
@@ -567,7 +572,7 @@ def main():
                                f"{RECON_LABEL[p.recon].lower()}_staggerct_rk2", "nx": p.nx, "ny": p.ny,
                    "recon": RECON_LABEL[p.recon], "recon_state": args.recon_state, "p_floor": p.p_floor,
                    "gamma": p.gamma, "form": head,
-                   "formula": args.formula, "magfield": args.magfield, "schedule": args.schedule},
+                   "formula": args.formula, "magfield": args.magfield, "schedule": sched_name},
         "roofline": res[head]["roofline"],
         "gmres": res[head]["gmres"],
         "schedule": res[head]["schedule"],
@@ -595,12 +600,13 @@ def main():
                 out["mixed_variants"][vname] = run_pair(args.workload, args.steps, args.warmup, sched, formula, magfield,
                                                         forms=("mixed",), over=dict(over, **vover))["mixed"]
         out["blast"] = dict(workload="magnetic blast 1024x1024 VanLeer, transmissive, nested-Newton p_floor 1e-10",
-                            steps=args.blast_steps, **run_pair("blast1024", args.blast_steps, 0, sched))
+                            steps=args.blast_steps, schedule=args.schedule or "auto",
+                            **run_pair("blast1024", args.blast_steps, 0, sched_other))
         out["batch"] = {"workload": f"{args.batch_instances} MHD vortices, sizes from {list(BATCH_SIZES)}^2, "
                                     "amplitude and background velocity U[0.5,2], Minmod, StaggerCT, FP64",
                         "steps": args.batch_steps, "warmup": 1,
-                        "ideal": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_IDEAL, 2024, sched),
-                        "redlich_kwong": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_RK, 2024, sched)}
+                        "ideal": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_IDEAL, 2024, sched_other),
+                        "redlich_kwong": run_batch(args.batch_instances, args.batch_steps, 1, im.EOS_RK, 2024, sched_other)}
         if args.magfield == "stagger":
             out["magfield"] = {"workload": "Orszag-Tang 1024x1024 WENO5, FP64, with the other magnetic treatments "
                                            "(UnStaggerCT and PotentialA evolve the vector potential A_z)",
@@ -624,7 +630,8 @@ def main():
                                                                 forms=("fp64",), over=o)["fp64"]
         out["jet"] = dict(workload="MHD jet 400x600 (testMHDJet's grid): jet inflow strip |x| < 0.05 on the bottom "
                                    "boundary (rho 1.4, v 1e4, B0 141.4, p 1), outflow elsewhere, Minmod, StaggerCT",
-                          steps=args.jet_steps, warmup=3, **run_pair("jet400", args.jet_steps, 3, sched))
+                          steps=args.jet_steps, warmup=3, schedule=args.schedule or "auto",
+                          **run_pair("jet400", args.jet_steps, 3, sched_other))
         out["diagnostics"] = dict(workload="Orszag-Tang 1024x1024, FP64: per-step diagnostics recorded on the device, "
                                            "Diagnose_CT_EMF, VTK and RK_*.dat output", steps=20,
                                   **run_diag_cost(args.workload, 20, 3, sched))

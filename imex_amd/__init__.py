@@ -11,6 +11,6 @@ from .capi import (  # noqa: F401
     FORMULA_ENERGY, FORMULA_PRESSURE, MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A,
     MAGFIELD_NODIVFREE, OP_POTENTIAL, init_potential, RECON_NONE, RECON_ENO3, RECONSTATE_PRIM, RECONSTATE_CONS, write_vtk_host, write_fields_host, write_history, output_path,
     CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, CASE_JET, BC_JET_INFLOW, BC_REFLECTIVE, BC_DIRICHLET, jet_params,
-    SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST, GC, NVAR,
+    SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST, SCHED_AUTO, GC, NVAR,
     TUNE_EXPLICIT, TUNE_KRYLOV, ERR_ARG, ERR_HIP, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_NOMEM,
 )

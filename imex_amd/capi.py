@@ -22,7 +22,7 @@ MAGFIELD_STAGGER, MAGFIELD_UNSTAGGER, MAGFIELD_POTENTIAL_A, MAGFIELD_NODIVFREE =
 RECON_NONE, RECON_ENO3 = 0, 3
 RECONSTATE_PRIM, RECONSTATE_CONS = 0, 1
 CASE_VORTEX, CASE_ORSZAG_TANG, CASE_BLAST, CASE_JET = 0, 1, 2, 3
-SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST = 0, 1, 2
+SCHED_GRAPH, SCHED_BUDGET, SCHED_HOST, SCHED_AUTO = 0, 1, 2, 3
 TUNE_EXPLICIT, TUNE_KRYLOV = 1, 2
 WARN_GMRES_MAXOUTER, WARN_NEWTON_MAXITER, WARN_POSITIVITY, WARN_EOS_MAXITER = 1, 2, 4, 8
 ERR_ARG, ERR_HIP, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_NOMEM = -1, -2, -3, -4, -5

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <chrono>
 #include "kernels.hpp"
 #include "adjoint.hpp"
 
@@ -141,6 +142,7 @@ struct GraphEntry {
   std::vector<BudgetRec> brecs;
   int nslot = 1;
   long long used = 0;
+  bool gk[NGKEY] = {};       // the GMRES budget positions that have nodes in the graph
 };
 #define BHIST 8
 #define MAX_GRAPHS 6
@@ -205,9 +207,9 @@ struct imex_ctx {
   // solve, 2 Jacobian solves per Newton solve, 3/4 sweeps per magnetic / pressure solve
   int hist[NBK][BHIST];        // per-step maxima of the recent steps
   int hist_n = 0, hist_pos = 0;
-  // extra budgets of a batch after an overrun: Arnoldi steps (+2 per GMRES overrun), refinement
-  // sweeps and Newton iterations (+1 per overrun of their kind); each decays by 1 every 16 steps
-  // that fit
+  // extra budgets after an overrun: Arnoldi steps (+2 per GMRES overrun of a batch, +1 of a
+  // single instance), refinement sweeps (a batch) and Newton iterations (+1 per overrun of their
+  // kind); each decays by 1 every 16 steps that fit
   int margin_extra = 0, sweep_extra = 0, newton_extra = 0, fit_streak = 0;
   int step_mx[NBK] = {0, 0, 0, 0, 0};   // maxima of the step being run under host scheduling
   // the same per solve position (-1: no such solve in that step)
@@ -218,6 +220,12 @@ struct imex_ctx {
   // mixed form: FP32 fluxes (pressure formula only, mixed_flux); per operator (bit 0 magnetic,
   // bit 1 pressure / energy) FP32 inner solves
   int mixed = 0, mixed_ops = 0, mixed_flux = 0;
+  // IMEX_SCHED_AUTO (single instance): timed trials of host-polled and budgeted (graph) steps;
+  // phase 0 host trial, 1 budget trial, 2 the faster schedule for AUTO_RUN steps
+  int auto_phase = 2, auto_left = 0;
+  bool auto_budget = false, auto_win = true;
+  double auto_ms[2] = {0.0, 0.0};
+  std::chrono::steady_clock::time_point auto_t0;
   std::vector<GraphEntry> graphs;
   long long graph_clock = 0;
   bool capturing = false;      // launches are being captured into a step graph
@@ -941,6 +949,7 @@ static void hist_push(imex_ctx* c, const int mx[NBK]) {
 static bool choose_budgets(imex_ctx* c) {
   if (c->taping) return false;
   if (c->ni == 1 && (c->prm.schedule == IMEX_SCHED_HOST || c->hist_n == 0)) return false;
+  if (c->ni == 1 && c->prm.schedule == IMEX_SCHED_AUTO && !c->auto_budget) return false;
   const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
   int mx[NBK] = {0, 0, 0, 0, 0};
   if (c->hist_n == 0) { mx[0] = mx[1] = std::min(m - 1, 11); mx[2] = 2; mx[3] = mx[4] = 3; }
@@ -950,6 +959,11 @@ static bool choose_budgets(imex_ctx* c) {
     const int op = (k / NIT) % 3 == 0 ? 0 : 1;
     int st = -1, sw = -1;
     for (int i = 0; i < c->hist_n; i++) { st = std::max(st, c->khist[i][k][0]); sw = std::max(sw, c->khist[i][k][1]); }
+    if (st < 0) {            // a Newton iteration not reached lately: the largest of its Newton solve
+      const int k0 = k - k % NIT;
+      for (int i = 0; i < c->hist_n; i++)
+        for (int q = k0; q < k0 + NIT; q++) { st = std::max(st, c->khist[i][q][0]); sw = std::max(sw, c->khist[i][q][1]); }
+    }
     if (st < 0) { st = mx[op]; sw = mx[3 + op]; }
     c->tab.B[k] = std::min(m, std::max(1, st + c->prm.budget_margin + c->margin_extra));
     c->tab.S[k] = std::min(MAXSWEEP - 1, std::max(1, sw + c->sweep_extra));
@@ -1036,12 +1050,14 @@ static bool tag_enabled(const imex_ctx* c, const NodeTag& t, const BudTab& tb) {
   }
 }
 
-static bool tab_covers(const imex_ctx* c, const BudTab& cap, const BudTab& t) {
+// Whether graph e holds every launch that table t enables.  GMRES positions without nodes in
+// the graph do not count: the Newton capacities decide which of them the step reaches.
+static bool tab_covers(const GraphEntry& e, const BudTab& t) {
   for (int k = 0; k < NGKEY; k++) {
-    if (t.B[k] > cap.B[k]) return false;
-    if (t.S[k] > cap.S[k]) return false;
+    if (!e.gk[k]) continue;
+    if (t.B[k] > e.cap.B[k] || t.S[k] > e.cap.S[k]) return false;
   }
-  for (int k = 0; k < NNKEY; k++) if (t.N[k] > cap.N[k]) return false;
+  for (int k = 0; k < NNKEY; k++) if (t.N[k] > e.cap.N[k]) return false;
   return true;
 }
 
@@ -1059,7 +1075,7 @@ static int budget_graph_launch(imex_ctx* c) {
   GraphEntry* g = nullptr;
   for (GraphEntry& e : c->graphs) {
     if (e.parity != parity || e.state_mode != c->state_mode) continue;
-    if (c->no_toggle ? std::memcmp(&e.cap, &cur, sizeof(BudTab)) == 0 : tab_covers(c, e.cap, cur)) { g = &e; break; }
+    if (c->no_toggle ? std::memcmp(&e.cap, &cur, sizeof(BudTab)) == 0 : tab_covers(e, cur)) { g = &e; break; }
   }
   if (!g) {
     if ((int)c->graphs.size() >= MAX_GRAPHS) {     // drop the least recently used graph
@@ -1073,12 +1089,25 @@ static int budget_graph_launch(imex_ctx* c) {
     if (!c->no_toggle) {
       const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
       for (int k = 0; k < NGKEY; k++) {
-        cap.B[k] = std::min(m, cur.B[k] + 2);
+        // every Arnoldi step of a restart cycle: the nodes past the budget stay disabled, and
+        // Arnoldi budgets, which move most from step to step, never force a capture
+        cap.B[k] = m;
         // a refinement sweep more for the mixed form; the FP64 form gets a restart cycle more only
         // where one has been needed
         cap.S[k] = (key_prec(c, k) || cur.S[k] > 1) ? std::min(MAXSWEEP - 1, cur.S[k] + 1) : cur.S[k];
       }
       for (int k = 0; k < NNKEY; k++) cap.N[k] = std::min(std::min(c->prm.newton_max, NIT - 1), cur.N[k] + 1);
+      // the new graph also covers the graphs of this parity it replaces, so capacities only grow
+      // and budgets that move up and down stop causing captures
+      for (size_t i = c->graphs.size(); i-- > 0;) {
+        const GraphEntry& e = c->graphs[i];
+        if (e.parity != parity || e.state_mode != c->state_mode) continue;
+        for (int k = 0; k < NGKEY; k++) { cap.B[k] = std::max(cap.B[k], e.cap.B[k]); cap.S[k] = std::max(cap.S[k], e.cap.S[k]); }
+        for (int k = 0; k < NNKEY; k++) cap.N[k] = std::max(cap.N[k], e.cap.N[k]);
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+        c->graphs.erase(c->graphs.begin() + (long)i);
+      }
     }
     c->tab = cap;
     c->tags.clear();
@@ -1100,6 +1129,7 @@ static int budget_graph_launch(imex_ctx* c) {
     if (!c->tag_err) ne.tags = c->tags;
     ne.brecs = c->brecs;
     ne.nslot = c->nslot;
+    for (const BudgetRec& r : c->brecs) if (r.kind == 0 && r.key >= 0 && r.key < NGKEY) ne.gk[r.key] = true;
     c->graphs.push_back(ne);
     g = &c->graphs.back();
     c->stats.graph_builds++;
@@ -1166,11 +1196,15 @@ static int one_step(imex_ctx* c) {
       return rc;
     }
     c->fit_streak = 0;
-    if (c->ni > 1 && (f & 1)) {     // a single instance reruns the step host-scheduled instead
-      c->margin_extra = std::min(8, c->margin_extra + 2);
-      c->sweep_extra = std::min(4, c->sweep_extra + 1);
+    // the margins grow after an overrun and shrink again after 16 steps that fit: a rerun costs
+    // a whole step, a unit of margin the launches of one Arnoldi step (or Newton iteration) that
+    // exit at once.  A single instance's rerun is host-scheduled and its counts enter the
+    // history, so its sweep (restart cycle) budgets need no extra margin.
+    if (f & 1) {
+      c->margin_extra = std::min(8, c->margin_extra + (c->ni > 1 ? 2 : 1));
+      if (c->ni > 1) c->sweep_extra = std::min(4, c->sweep_extra + 1);
     }
-    if (c->ni > 1 && (f & 2)) c->newton_extra = std::min(4, c->newton_extra + 1);
+    if (f & 2) c->newton_extra = std::min(4, c->newton_extra + 1);
     // a solve outran its budget: restore the step's scalars and rerun it (host scheduling for a
     // single instance, larger budgets for a batch)
     c->stats.budget_reruns++;
@@ -1393,7 +1427,7 @@ int imex_default_params(imex_params* p) {
   p->precision = IMEX_PREC_FP64;
   p->mixed_inner_tol = 1e-6;
   p->mixed_outer_tol = 1e-9;
-  p->schedule = IMEX_SCHED_GRAPH;
+  p->schedule = IMEX_SCHED_AUTO;
   p->budget_margin = 1;
   p->formula = IMEX_FORMULA_PRESSURE;                                    // config.hpp:44
   p->recon_state = IMEX_RECONSTATE_CONS;                                 // config.hpp:61
@@ -1519,7 +1553,8 @@ static int check_params(const imex_params* prm) {
     return set_err(IMEX_ERR_ARG, "bad recon_state");
   if (prm->precision != IMEX_PREC_FP64 && prm->precision != IMEX_PREC_MIXED)
     return set_err(IMEX_ERR_ARG, "bad precision");
-  if (prm->schedule != IMEX_SCHED_GRAPH && prm->schedule != IMEX_SCHED_BUDGET && prm->schedule != IMEX_SCHED_HOST)
+  if (prm->schedule != IMEX_SCHED_GRAPH && prm->schedule != IMEX_SCHED_BUDGET && prm->schedule != IMEX_SCHED_HOST &&
+      prm->schedule != IMEX_SCHED_AUTO)
     return set_err(IMEX_ERR_ARG, "bad schedule");
   if (prm->formula != IMEX_FORMULA_PRESSURE && prm->formula != IMEX_FORMULA_ENERGY)
     return set_err(IMEX_ERR_ARG, "bad formula");
@@ -1776,6 +1811,39 @@ int imex_get_state(imex_ctx* c, double* u_con, double* u_pri, double* bx, double
   return imex_batch_get_state(c, 0, u_con, u_pri, bx, by, sc);
 }
 
+// ---------------------------------------------------------------- automatic schedule
+// Budgeted steps save the host round trips of the polled loop but pay for the launches that
+// find their solve converged and for the reruns of steps that outrun their budgets; which wins
+// depends on the problem (small grids and steady iteration counts favour budgets, large grids
+// whose counts swing favour polling).  The schedules give the same results bit for bit, so a
+// single instance can pick by measurement: a trial of AUTO_WARM + AUTO_TIMED host-polled steps
+// (the first trial also gives the budgets their history), one of as many budgeted graph steps
+// (the untimed ones absorb the other schedule's queued tail and the captures of both state
+// parities), then the faster schedule for AUTO_RUN steps, then trials again.  A batch always
+// runs budgeted.
+#define AUTO_WARM 2
+#define AUTO_TIMED 3
+#define AUTO_RUN 48
+static bool auto_before(imex_ctx* c) {
+  if (c->ni != 1 || c->prm.schedule != IMEX_SCHED_AUTO) return false;
+  if (c->taping || c->prof.on) { c->auto_budget = c->auto_win; return false; }
+  if (c->auto_left == 0) {
+    if (c->auto_phase == 1) c->auto_win = c->auto_ms[1] < c->auto_ms[0];
+    c->auto_phase = (c->auto_phase + 1) % 3;
+    c->auto_left = c->auto_phase < 2 ? AUTO_WARM + AUTO_TIMED : AUTO_RUN;
+  }
+  c->auto_budget = c->auto_phase == 0 ? false : c->auto_phase == 1 ? true : c->auto_win;
+  if (c->auto_phase < 2 && c->auto_left == AUTO_TIMED) c->auto_t0 = std::chrono::steady_clock::now();
+  return true;
+}
+static int auto_after(imex_ctx* c) {
+  if (--c->auto_left > 0 || c->auto_phase == 2) return 0;
+  HIPCHK(hipStreamSynchronize(c->st));       // the trial's last step has finished on the device
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c->auto_t0).count();
+  c->auto_ms[c->auto_phase] = ms / AUTO_TIMED;
+  return 0;
+}
+
 int imex_step_async(imex_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return set_err(IMEX_ERR_ARG, "bad argument");
   if (c->needA)
@@ -1783,8 +1851,10 @@ int imex_step_async(imex_ctx* c, int nsteps) {
       if (!c->a_set[i]) return set_err(IMEX_ERR_ARG, "UNSTAGGER / POTENTIAL_A need the vector potential (imex_set_potential)");
   int rc = 0;
   for (int s = 0; s < nsteps; s++) {
+    const bool timed = auto_before(c);
     int r = one_step(c);
     if (r < 0) return r;
+    if (timed && (r = auto_after(c)) < 0) return r;
     rc |= r;
     if (c->diag_cap > 0) diag_launch(c, c->L, 1);
   }

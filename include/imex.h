@@ -43,13 +43,15 @@ enum { IMEX_MAGFIELD_STAGGER = 0, IMEX_MAGFIELD_UNSTAGGER = 1, IMEX_MAGFIELD_POT
        IMEX_MAGFIELD_NODIVFREE = 3 };                              /* MagField (config.hpp:28) */
 enum { IMEX_OP_MAG = 0, IMEX_OP_P_CARRIER = 1, IMEX_OP_P_JACOBIAN = 2, IMEX_OP_ENERGY = 3, IMEX_OP_POTENTIAL = 4 };
 enum { IMEX_CASE_VORTEX = 0, IMEX_CASE_ORSZAG_TANG = 1, IMEX_CASE_BLAST = 2, IMEX_CASE_JET = 3 };
-/* How a step is scheduled.  All three give identical results.
+/* How a step is scheduled.  All four give identical results.
  *   GRAPH:  every solve gets an iteration budget learned from earlier steps; the whole step is
  *           launched as one captured hipGraph with no host synchronisation inside it; a step whose
  *           solve outran its budget is discarded and rerun with HOST scheduling.
  *   BUDGET: as GRAPH, launched kernel by kernel (no graph).
- *   HOST:   the host polls GMRES/Newton convergence between launches, as the reference's loops do. */
-enum { IMEX_SCHED_GRAPH = 0, IMEX_SCHED_BUDGET = 1, IMEX_SCHED_HOST = 2 };
+ *   HOST:   the host polls GMRES/Newton convergence between launches, as the reference's loops do.
+ *   AUTO:   (default) a single instance times a few GRAPH and HOST steps now and then and runs the
+ *           faster of the two; a batch runs GRAPH. */
+enum { IMEX_SCHED_GRAPH = 0, IMEX_SCHED_BUDGET = 1, IMEX_SCHED_HOST = 2, IMEX_SCHED_AUTO = 3 };
 
 enum {
   IMEX_WARN_GMRES_MAXOUTER = 1,   /* src/implicit_operators.cpp:918 */

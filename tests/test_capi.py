@@ -33,7 +33,7 @@ def test_default_params_match_reference_config():
     assert (p.cfl, p.gmres_tol, p.gmres_restart, p.shock_kappa1) == (0.8, 1e-14, 25, 0.001)
     assert (p.floor_on, p.p_floor, p.theta, p.mbig_fac, p.newton_max, p.newton_tol) == (1, 1e-7, 0.9, 1e6, 20, 1e-8)
     assert (p.rk_alfa, p.rk_b, p.rk_R, p.rk_cv) == (0.05, 0.1, 0.4, 1.0)
-    assert (p.schedule, p.budget_margin) == (im.SCHED_GRAPH, 1)
+    assert (p.schedule, p.budget_margin) == (im.SCHED_AUTO, 1)
 
 
 def test_vortex_initial_data():

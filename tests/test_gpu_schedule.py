@@ -1,9 +1,10 @@
-"""The three step schedules give the same state bit for bit.
+"""The four step schedules give the same state bit for bit.
 
 HOST polls GMRES/Newton convergence between launches (the reference's control
 flow); BUDGET queues every solve with an iteration budget learned from earlier
 steps, so no host synchronisation happens inside a step; GRAPH replays the
-budgeted step as a captured hipGraph.  A negative budget margin forces solves
+budgeted step as a captured hipGraph; AUTO times a few steps of GRAPH and HOST
+and runs the faster.  A negative budget margin forces solves
 to outrun their budgets, which exercises the discard-and-rerun path.
 """
 import numpy as np
@@ -40,13 +41,16 @@ def test_schedules_bitwise_identical(case, n, m, steps):
     print(f"  host: restarts mag {ref[4]['mag_gmres_restarts']} p {ref[4]['p_gmres_restarts']}", flush=True)
     if case == "blast":
         assert ref[4]["mag_gmres_restarts"] + ref[4]["p_gmres_restarts"] > 0
-    for sched in (im.SCHED_BUDGET, im.SCHED_GRAPH):
+    for sched in (im.SCHED_BUDGET, im.SCHED_GRAPH, im.SCHED_AUTO):
         out = run(case, n, m, steps, sched)
         st = out[4]
         print(f"  sched {sched}: budget steps {st['budget_steps']} reruns {st['budget_reruns']} "
               f"graphs {st['graph_builds']}; {dict((k, st[k]) for k in COUNTS)}", flush=True)
-        assert st["budget_steps"] >= steps - 1      # the blast's restart cycles are budgeted too
-        if sched == im.SCHED_GRAPH:
+        if sched == im.SCHED_AUTO:      # five host-polled steps, then a trial of five budgeted ones
+            assert st["budget_steps"] >= min(5, steps - 5)
+        else:
+            assert st["budget_steps"] >= steps - 1   # the blast's restart cycles are budgeted too
+        if sched != im.SCHED_BUDGET:
             assert st["graph_builds"] >= 1
         for a, b in zip(out[:3], ref[:3]):
             assert np.array_equal(a, b)
