@@ -10,10 +10,12 @@
 //
 // GMRES (implicit_operators.cpp:793-919): classical Gram-Schmidt as in the reference —
 // all dots of step j are formed in the operator kernel from the freshly computed
-// M^-1 A v_j, then one update pass forms w = Av_j - sum H_ij v_i and ||w||.  Each of
-// the two kernels finishes its grid reduction in the last block to arrive, which also
-// runs the Givens / Hessenberg bookkeeping in FP64 and raises `done` on convergence or at
-// the restart length; later kernels of the same cycle see `done` and exit at once.
+// M^-1 A v_j, then one update pass forms w = Av_j - sum H_ij v_i and ||w||.  Each block
+// writes its partial sums packed per block; a one-block finish kernel after each of the
+// two reduces them in a fixed order (so results do not depend on block timing), and the
+// second finish runs the Givens / Hessenberg bookkeeping in FP64 and raises `done` on
+// convergence or at the restart length; later kernels of the same cycle see `done` and
+// exit at once.
 #include "kernels.hpp"
 
 #ifndef IMEX_MAG_WAVES

@@ -381,6 +381,10 @@ __device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL
     // Jiang-Shu weights w_k = (d_k/e_k) / sum_m (d_m/e_m), e_k = (eps + beta_k)^2.  FP64: written
     // with the common denominator e0 e1 e2 so that each face value costs one division.  FP32:
     // hardware reciprocals (the products e_i e_j could overflow the FP32 range).
+    // Multiply-adds are fused here (the weights are smooth in the data, so rounding moves the face
+    // values by an ulp and decides nothing); the limiter and ENO branches above, whose choices
+    // switch at ties, keep separate roundings.
+#pragma clang fp contract(fast)
     T t0 = um2 - T(2) * um1 + u0, t1 = um2 - T(4) * um1 + T(3) * u0;
     T beta0 = T(13.0 / 12.0) * (t0 * t0) + T(1.0 / 4.0) * (t1 * t1);
     T t2 = um1 - T(2) * u0 + up1, t3 = um1 - up1;
@@ -626,16 +630,25 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
         }
       if (ef) atomicOr(eflag, 1);
     } else {
-      for (int k = 0; k < nrec; k++) {
+      // all stencil loads are issued before any reconstruction (the fifth variable is loaded for
+      // kind 0 too, where it goes unused), so their latencies overlap
+      double s64[5][5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
         const double* q = Qe + k * d.PS + base;
-        double a64[5];
-        T a[5];
 #pragma unroll
         for (int o = 0; o < 5; o++) {
-          if constexpr (BND) a64[o] = o < amb.lo ? amb.val(true, k) : (o >= 5 - amb.hi ? amb.val(false, k) : q[(o - 2) * step]);
-          else a64[o] = q[(o - 2) * step];
-          a[o] = (T)a64[o];
+          if constexpr (BND) s64[k][o] = o < amb.lo ? amb.val(true, k) : (o >= 5 - amb.hi ? amb.val(false, k) : q[(o - 2) * step]);
+          else s64[k][o] = q[(o - 2) * step];
         }
+      }
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        if (k >= nrec) continue;
+        const double* a64 = s64[k];
+        T a[5];
+#pragma unroll
+        for (int o = 0; o < 5; o++) a[o] = (T)a64[o];
         const int sel = std::is_same<T, float>::value && P.recon == IMEX_RECON_ENO3 ? eno3_stencil(a64) : -1;
         T L, R;
         recon5(P.recon, DIR == 1 ? P.central_y : 0, a, L, R, sel);
@@ -649,7 +662,15 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     }
   }
   __syncthreads();
-  for (int f = threadIdx.x; f < NF; f += blockDim.x) {
+  // at most two faces per thread; their fluxes are held in registers until every face state has
+  // been read, then written over the face states (F aliases wl: a smaller LDS footprint, more
+  // blocks per CU)
+  static_assert(EX_NF <= EX_NW && NF <= 512, "flux tile layout");
+  T Fo[2][5];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int f = threadIdx.x + r * 256;
+    if (f >= NF) continue;
     int fx = f % FW, fy = f / FW;
     // face between local cells (left, right) along DIR; item indices into wl/wr
     int itL, itR, gli, glj, gri, grj;
@@ -666,9 +687,14 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
     } else {
       for (int k = 5; k < 8; k++) { QL[k] = T(0); QR[k] = T(0); }
     }
-    T Fo[5];
-    gl_flux(DIR, kind, (T)P.gamma, QL, QR, s_max, Fo);
-    for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[k];
+    gl_flux(DIR, kind, (T)P.gamma, QL, QR, s_max, Fo[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int f = threadIdx.x + r * 256;
+    if (f >= NF) continue;
+    for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[r][k];
   }
   __syncthreads();
   if (own) {
@@ -687,7 +713,7 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
 // compiled in; without them the kernel carries none of their code.
 template <int EX_TX, class T, bool PRIM, bool BND>
 __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict__ IT, BMap bm, int kind, int iQe, int iQs, int iu) {
-  constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
+  constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW;
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims d = I_.d;
@@ -697,7 +723,7 @@ __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict
   const double* S = gmem(I_.S);
 
   __shared__ T wl[5 * EX_NW], wr[5 * EX_NW];
-  __shared__ T F[5 * EX_NF];
+  T* F = wl;                  // face fluxes, written once the face states have been read
   const int i0 = GC + bx_ * EX_TX, j0 = GC + by_ * EX_TY;
   const double h = S[S_ADT];
   const T s_max = (T)S[S_AMAX];
