@@ -95,8 +95,10 @@ def interior_direction(rng, shape_u, shape_f):
     return du, dbx, dby
 
 
-def fd_check(n, steps, every, ndir, seed, eps=1e-6, scale=1e-2, m=None):
+def fd_check(n, steps, every, ndir, seed, eps=1e-6, scale=1e-2, m=None, time_stop=None):
     p, u, bx, by = vortex(n, m)
+    if time_stop is not None:
+        p.time_stop = time_stop
     ut = torch.tensor(u, requires_grad=True)
     bxt = torch.tensor(bx, requires_grad=True)
     byt = torch.tensor(by, requires_grad=True)
@@ -149,6 +151,28 @@ def test_adjoint_one_step_uneven_grid():
     print("[adjoint] vortex 32x20, 1 step", flush=True)
     errs = fd_check(32, 1, 1, 3, seed=5, m=20)
     assert errs.max() <= 1e-5, errs
+
+
+def test_adjoint_last_step_clipped_to_time_stop():
+    """dt = time_stop - t on the last step (main.cpp:341): its cotangent reaches every earlier dt
+    through t, and the clipped step's own CFL dt gets none."""
+    n, steps = 24, 4
+    p, u, bx, by = vortex(n)
+    s = im.Solver(p)
+    try:
+        s.set_state(u, bx, by)
+        s.step(steps - 1)
+        t3 = s.get_state()[4]["t"]
+        s.step(1)
+        dt4 = s.get_state()[4]["dt"]
+    finally:
+        s.close()
+    ts = t3 + 0.6 * dt4
+    print(f"[adjoint] vortex {n}x{n}, {steps} steps, time_stop {ts:.9e} clips the last step (t3 {t3:.9e}, dt4 {dt4:.3e})",
+          flush=True)
+    errs = fd_check(n, steps, 2, 4, seed=17, time_stop=ts)
+    assert np.median(errs) <= 1e-5, errs
+    assert errs.max() <= 1e-4, errs
 
 
 def test_adjoint_unsupported_settings():
