@@ -27,6 +27,20 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_caller_program_is_built_and_checks_its_arguments(tmp_path):
+    """The C++ caller links the library (no GPU call happens before imex_create) and rejects bad input."""
+    import subprocess
+    exe = os.path.join(ROOT, "imex_amd", "imex_caller")
+    assert os.path.exists(exe), "make -C imex_amd builds imex_amd/imex_caller"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, "stage", str(tmp_path), "no_such_subsystem"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown subsystem" in r.stderr
+    (tmp_path / "params.bin").write_bytes(b"\0" * 8)              # too short for an imex_params
+    r = subprocess.run([exe, "step", str(tmp_path), "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "fewer than" in r.stderr
+
+
 def test_default_params_match_reference_config():
     p = im.default_params()
     # include/config.hpp:57, 71, 77, 100-107, 125; implicit_operators.cpp:801
