@@ -21,6 +21,11 @@ void adj_mag_rhs(hipStream_t st, const AD& a, const AdjScal& sc, const double* u
 // cotangents of the coefficients (Qe B, rho of u, scalars) of y . A x
 void adj_mag_op(hipStream_t st, const AD& a, const AdjScal& sc, int mode, const double* x, const double* Qe,
                 const double* u, const double* y, double* xbar, double* Qebar, double* ubar, double* sbar);
+// the magnetic operator's Jacobian in the unknown (81 planes of nx ny, compact interior order) and
+// its transpose applied to y (3 planes, written to out)
+#define ADJ_MAG_JAC 81
+void adj_mag_jac(hipStream_t st, const AD& a, const AdjScal& sc, const double* Qe, const double* u, double* J);
+void adj_mag_op_t(hipStream_t st, const AD& a, const double* J, const double* y, double* out);
 // EMF fluxes (ct_flux.cpp:5-127): obar = (Fx bar, Fy bar) planes
 void adj_emf(hipStream_t st, const AD& a, const AdjScal& sc, const double* u, const double* Qe, const double* pe,
              const double* fbar, double* ubar, double* Qebar, double* pebar, double* sbar);
@@ -62,6 +67,37 @@ void adj_max_grad(hipStream_t st, const AD& a, const Phys& P, const double* U, c
                   double* Ubar);
 // primitive cotangent from a conservative one: pribar = (d con / d pri)^T conbar
 void adj_pri2con(hipStream_t st, const AD& a, const Phys& P, const double* pri, const double* conbar, double* pribar);
+
+// ---------------------------------------------------------------- transposed GMRES, device side
+// Restart length of the transposed solves and their device control block: the Hessenberg column
+// work, the Givens rotations and the convergence decision live on the device, so the host queues
+// Arnoldi steps without waiting on each dot product (it polls `cdone` a step behind).
+#define ADJ_M 30
+struct AdjKry {
+  double H[(ADJ_M + 1) * ADJ_M];  // rotated Hessenberg columns: H[i * ADJ_M + j], i <= j
+  double cs[ADJ_M], sn[ADJ_M], g[ADJ_M + 1], y[ADJ_M];
+  double d1[ADJ_M + 2], d2[ADJ_M + 2];   // dot sums of the two Gram-Schmidt passes of the current step
+  double bnorm, tol, res, hn, beta;
+  int jlast, cdone, solved, iters;   // last column, cycle ended, solve converged, Arnoldi steps so far
+};
+struct AdjVecs { const double* p[ADJ_M + 1]; };
+// Per-block partial sums: `part` holds akr_blocks x (ADJ_M + 1) doubles, `pn` akr_blocks doubles.
+int akr_blocks(const AD& a, int nc);
+// bnorm = ||b||, res = bnorm; solved when b = 0
+void akr_init(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, double* pn, double tol);
+// start of a cycle: r = b - t, beta = ||r||; v0 = r / beta (zero once the solve is done)
+void akr_residual(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, const double* t, double* r,
+                  double* v0, double* pn);
+// one classical Gram-Schmidt pass against V_0..V_{nv-1}: d (0: d1, 1: d2) = V^T w, w -= V d; with
+// pn, the blocks' shares of ||w||^2 after it
+void akr_gs_pass(hipStream_t st, const AD& a, int nc, int nv, const AdjVecs& V, double* w, AdjKry* K, int d,
+                 double* part, double* pn);
+// column j: H = d1 + d2, hn = ||w||, rotations, residual estimate, end of cycle
+void akr_column(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* pn, int j, int m);
+// v_{j+1} = w / hn (zero once the cycle has ended)
+void akr_next(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* w, double* v);
+// back substitution and x += sum_i y_i V_i over the cycle's columns
+void akr_update_x(hipStream_t st, const AD& a, int nc, const AdjVecs& V, AdjKry* K, double* x);
 
 // vector helpers over interior cells of nc planes
 void adj_dots(hipStream_t st, const AD& a, int nc, int nv, const double* const* v, const double* w, double* out);

@@ -189,6 +189,64 @@ __global__ void __launch_bounds__(128) k_vjp(Fn fn, AD a, AFields F, const doubl
   }
 }
 
+// Jacobian of an element function that is linear in the inputs it differentiates (the magnetic
+// operator in the unknown B): J[((s NOUT + o) K + q) N + e] = d out_o(e) / d input_q(cell s of e's
+// stencil), N = nx ny, e the compact interior index.  Formed once per solve by the same dual-number
+// evaluations as one k_vjp; each transposed application is then the gather below.
+template <class Fn>
+__global__ void __launch_bounds__(128) k_jac(Fn fn, AD a, AFields F, Sc<double> sc, double* __restrict__ J) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = a.nx * a.ny;
+  if (n >= N) return;
+  const int i = n % a.nx, j = n / a.nx;
+  typedef DN<Fn::K> T;
+  Sc<T> st;
+#pragma unroll
+  for (int q = 0; q < SC_N; q++) st.s[q] = T(sc.s[q]);
+  for (int s = 0; s < Fn::NS; s++) {
+    int oi, oj;
+    Fn::off(s, oi, oj);
+    Rd<T, Fn> rd{&a, &F, wrapi(i + oi, a.nx), wrapi(j + oj, a.ny)};
+    T out[Fn::NOUT];
+    fn.eval(rd, st, i, j, out);
+#pragma unroll
+    for (int o = 0; o < Fn::NOUT; o++)
+#pragma unroll
+      for (int q = 0; q < Fn::K; q++) J[((long long)(s * Fn::NOUT + o) * Fn::K + q) * N + n] = out[o].g[q];
+  }
+}
+
+// (J^T y) at cell c: the elements e whose stencil holds c (c = e + off(s)), each contributing
+// sum_o J_e[s][o][q] y_o(e); written to plane q of `out` (not added).  No atomics: one thread per
+// cell, a fixed summation order.
+template <class Fn>
+__global__ void __launch_bounds__(128) k_jac_t(AD a, const double* __restrict__ J, const double* __restrict__ y,
+                                               double* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = a.nx * a.ny;
+  if (n >= N) return;
+  const int i = n % a.nx, j = n / a.nx;
+  double acc[Fn::K];
+#pragma unroll
+  for (int q = 0; q < Fn::K; q++) acc[q] = 0.0;
+  for (int s = 0; s < Fn::NS; s++) {
+    int oi, oj;
+    Fn::off(s, oi, oj);
+    const int ei = wrapi(i - oi, a.nx), ej = wrapi(j - oj, a.ny);
+    const int e = ei + a.nx * ej;
+    const long long te = tix(a, ei, ej);
+#pragma unroll
+    for (int o = 0; o < Fn::NOUT; o++) {
+      const double yo = y[o * a.PS + te];
+#pragma unroll
+      for (int q = 0; q < Fn::K; q++) acc[q] += J[((long long)(s * Fn::NOUT + o) * Fn::K + q) * N + e] * yo;
+    }
+  }
+  const long long c = tix(a, i, j);
+#pragma unroll
+  for (int q = 0; q < Fn::K; q++) out[q * a.PS + c] = acc[q];
+}
+
 // Scalar cotangents: sbar[q] += sum_e sum_o obar[o] d out_o / d scalar_q.
 template <class Fn>
 __global__ void __launch_bounds__(128) k_vjp_scalars(Fn fn, AD a, AFields F, const double* __restrict__ obar, Sc<double> sc,

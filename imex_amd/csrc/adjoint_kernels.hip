@@ -746,6 +746,239 @@ __global__ void __launch_bounds__(AB) k_adj_compact(AD a, int nc, int dir, const
   else out[n] = in[c];
 }
 
+// ---------------------------------------------------------------- transposed GMRES, device side
+// The vector kernels run on a 2-D grid: x over AB-cell slabs of a row, y over groups of AKR_ROWS
+// rows of all nc planes, so an element's padded index needs no division.  Sums are written per
+// block (no atomics) and reduced in a fixed order by the one-block kernels that use them.
+#define AKR_ROWS 8
+__device__ __forceinline__ long long akr_index(const AD& a, int nc, int r, bool& ok) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y * AKR_ROWS + r;
+  ok = i < a.nx && row < nc * a.ny;
+  const int rr = ok ? row : 0, comp = rr / a.ny, j = rr - comp * a.ny;
+  return comp * a.PS + (long long)((ok ? i : 0) + GC) + (long long)(j + GC) * a.NX;
+}
+__device__ __forceinline__ int akr_block() { return blockIdx.x + gridDim.x * blockIdx.y; }
+static inline dim3 akr_grid(const AD& a, int nc) { return dim3((a.nx + AB - 1) / AB, (nc * a.ny + AKR_ROWS - 1) / AKR_ROWS); }
+
+// the block's sum of v (all threads call it); thread 0 gets the result
+__device__ __forceinline__ double akr_block_sum(double v) {
+  __shared__ double sh[AB / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < AB / 64; q++) t += sh[q];
+  __syncthreads();
+  return t;
+}
+// sum of part[b * stride] over b < nb by one block (fixed order per lane, then over the waves)
+__device__ __forceinline__ double akr_fin_sum(const double* part, int stride, int nb) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long long)b * stride];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += sh[q];
+  __syncthreads();
+  return t;
+}
+
+__global__ void __launch_bounds__(1024) k_akr_init(AdjKry* K, const double* __restrict__ pn, int nb, double tol) {
+  const double b2 = akr_fin_sum(pn, 1, nb);
+  if (threadIdx.x != 0) return;
+  K->bnorm = sqrt(b2);
+  K->tol = tol;
+  K->res = K->bnorm;
+  K->solved = K->bnorm == 0.0 ? 1 : 0;
+  K->cdone = K->solved;
+  K->iters = 0;
+  K->jlast = -1;
+}
+
+__global__ void __launch_bounds__(1024) k_akr_cycle(AdjKry* K, const double* __restrict__ pn, int nb) {
+  const int t = threadIdx.x;
+  const double r2 = akr_fin_sum(pn, 1, nb);
+  if (t == 0) K->jlast = -1;
+  if (K->solved) { if (t == 0) K->cdone = 1; return; }
+  for (int i = t; i <= ADJ_M; i += blockDim.x) K->g[i] = 0.0;
+  __syncthreads();
+  if (t == 0) {
+    const double beta = sqrt(r2);
+    K->beta = beta;
+    K->res = beta;
+    if (beta <= K->tol * K->bnorm) { K->solved = 1; K->cdone = 1; }
+    else { K->cdone = 0; K->g[0] = beta; }
+  }
+}
+
+// out = in / s (s = beta for the first basis vector, hn for the next ones); zero once the cycle has ended
+__global__ void __launch_bounds__(AB) k_akr_scale(AD a, int nc, const AdjKry* __restrict__ K, int which,
+                                                  const double* __restrict__ in, double* __restrict__ out) {
+  const double sc = 1.0 / (which == 0 ? K->beta : K->hn);
+  const bool zero = K->cdone;
+#pragma unroll
+  for (int r = 0; r < AKR_ROWS; r++) {
+    bool ok;
+    const long long c = akr_index(a, nc, r, ok);
+    if (ok) out[c] = zero ? 0.0 : in[c] * sc;
+  }
+}
+
+// part[block][q] = the block's share of V_q . w, four basis vectors per pass so that the
+// accumulators stay in registers (past nv the last vector is read again and its product dropped)
+__global__ void __launch_bounds__(AB) k_akr_dots(AD a, int nc, int nv, AdjVecs V, const double* __restrict__ w,
+                                                 const AdjKry* __restrict__ K, double* __restrict__ part) {
+  if (K->cdone) return;
+  constexpr int CH = 4;
+  long long cs[AKR_ROWS];
+  double wv[AKR_ROWS];
+#pragma unroll
+  for (int r = 0; r < AKR_ROWS; r++) {
+    bool ok;
+    cs[r] = akr_index(a, nc, r, ok);
+    wv[r] = ok ? w[cs[r]] : 0.0;
+  }
+  double* pb = part + (long long)akr_block() * (ADJ_M + 1);
+  for (int q0 = 0; q0 < nv; q0 += CH) {
+    double acc[CH] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const double* vp = V.p[min(q0 + u, nv - 1)];
+#pragma unroll
+      for (int r = 0; r < AKR_ROWS; r++) acc[u] += vp[cs[r]] * wv[r];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const double t = akr_block_sum(acc[u]);
+      if (threadIdx.x == 0 && q0 + u < nv) pb[q0 + u] = t;
+    }
+  }
+}
+
+// d (d1 or d2)[q] = sum over the blocks of part[.][q], q < nv
+__global__ void __launch_bounds__(1024) k_akr_dsum(AdjKry* K, const double* __restrict__ part, int nb, int nv, int d) {
+  if (K->cdone) return;
+  double* dst = d == 0 ? K->d1 : K->d2;
+  for (int q = 0; q < nv; q++) {
+    const double t = akr_fin_sum(part + q, ADJ_M + 1, nb);
+    if (threadIdx.x == 0) dst[q] = t;
+  }
+}
+
+// w -= sum_q d[q] V_q (q < nv); with norm, pn[block] = the block's share of ||w||^2
+__global__ void __launch_bounds__(AB) k_akr_project(AD a, int nc, int nv, AdjVecs V, double* __restrict__ w,
+                                                    const AdjKry* __restrict__ K, int d, double* __restrict__ pn) {
+  if (K->cdone) return;
+  __shared__ double cf[ADJ_M + 1];
+  const double* src = d == 0 ? K->d1 : K->d2;
+  for (int q = threadIdx.x; q < nv; q += blockDim.x) cf[q] = src[q];
+  __syncthreads();
+  double sq = 0.0;
+#pragma unroll
+  for (int r = 0; r < AKR_ROWS; r++) {
+    bool ok;
+    const long long c = akr_index(a, nc, r, ok);
+    if (!ok) continue;
+    double s = 0.0;
+    for (int q = 0; q < nv; q++) s += cf[q] * V.p[q][c];
+    const double wv = w[c] - s;
+    w[c] = wv;
+    sq += wv * wv;
+  }
+  if (pn) {
+    const double t = akr_block_sum(sq);
+    if (threadIdx.x == 0) pn[akr_block()] = t;
+  }
+}
+
+// Column j of the Hessenberg matrix from the two Gram-Schmidt passes, the rotations, and whether
+// the cycle ends: converged (|g_{j+1}| <= tol ||b||), breakdown (hn = 0) or restart (j + 1 = m).
+__global__ void __launch_bounds__(1024) k_akr_column(AdjKry* K, const double* __restrict__ pn, int nb, int j, int m) {
+  if (K->cdone) return;
+  __shared__ double hcol[ADJ_M + 2];
+  const int t = threadIdx.x;
+  const double w2 = akr_fin_sum(pn, 1, nb);
+  for (int i = t; i <= j; i += blockDim.x) hcol[i] = K->d1[i] + K->d2[i];
+  __syncthreads();
+  if (t == 0) {
+    const double hn = sqrt(w2);
+    for (int i = 0; i < j; i++) {
+      const double a0 = K->cs[i] * hcol[i] + K->sn[i] * hcol[i + 1];
+      hcol[i + 1] = -K->sn[i] * hcol[i] + K->cs[i] * hcol[i + 1];
+      hcol[i] = a0;
+    }
+    const double den = sqrt(hcol[j] * hcol[j] + hn * hn);
+    K->cs[j] = den > 0 ? hcol[j] / den : 1.0;
+    K->sn[j] = den > 0 ? hn / den : 0.0;
+    hcol[j] = den;
+    K->g[j + 1] = -K->sn[j] * K->g[j];
+    K->g[j] = K->cs[j] * K->g[j];
+    for (int i = 0; i <= j; i++) K->H[i * ADJ_M + j] = hcol[i];
+    K->hn = hn;
+    K->iters += 1;
+    K->jlast = j;
+    K->res = fabs(K->g[j + 1]);
+    const bool conv = K->res <= K->tol * K->bnorm;
+    if (conv) K->solved = 1;
+    if (conv || hn == 0.0 || j + 1 == m) K->cdone = 1;
+  }
+}
+
+// y = R^-1 g over the cycle's columns (each block repeats the small back substitution), x += V y.
+__global__ void __launch_bounds__(AB) k_akr_update_x(AD a, int nc, AdjVecs V, const AdjKry* __restrict__ K,
+                                                     double* __restrict__ x) {
+  const int jl = K->jlast;
+  if (jl < 0) return;
+  __shared__ double y[ADJ_M];
+  if (threadIdx.x == 0) {
+    for (int i = jl; i >= 0; i--) {
+      double s = K->g[i];
+      for (int k = i + 1; k <= jl; k++) s -= K->H[i * ADJ_M + k] * y[k];
+      y[i] = s / K->H[i * ADJ_M + i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < AKR_ROWS; r++) {
+    bool ok;
+    const long long c = akr_index(a, nc, r, ok);
+    if (!ok) continue;
+    double s = 0.0;
+    for (int q = 0; q <= jl; q++) s += y[q] * V.p[q][c];
+    x[c] += s;
+  }
+}
+
+// pn[block] = the block's share of ||v||^2
+__global__ void __launch_bounds__(AB) k_akr_norm(AD a, int nc, const double* __restrict__ v, double* __restrict__ pn) {
+  double sq = 0.0;
+#pragma unroll
+  for (int r = 0; r < AKR_ROWS; r++) {
+    bool ok;
+    const long long c = akr_index(a, nc, r, ok);
+    if (ok) { const double vv = v[c]; sq += vv * vv; }
+  }
+  const double t = akr_block_sum(sq);
+  if (threadIdx.x == 0) pn[akr_block()] = t;
+}
+
+// r = b - t, and pn[block] = the block's share of ||r||^2
+__global__ void __launch_bounds__(AB) k_akr_sub(AD a, int nc, const double* __restrict__ b, const double* __restrict__ t,
+                                                double* __restrict__ r, double* __restrict__ pn) {
+  double sq = 0.0;
+#pragma unroll
+  for (int q = 0; q < AKR_ROWS; q++) {
+    bool ok;
+    const long long c = akr_index(a, nc, q, ok);
+    if (ok) { const double rv = b[c] - t[c]; r[c] = rv; sq += rv * rv; }
+  }
+  const double s = akr_block_sum(sq);
+  if (threadIdx.x == 0) pn[akr_block()] = s;
+}
+
 // ================================================================ launchers
 void adj_explicit(hipStream_t st, const AD& a, const Phys& P, int kind, const AdjScal& sc,
                   const double* Qe, const double* Qs, const double* uB, const double* uout, const double* ubar,
@@ -773,6 +1006,14 @@ void adj_mag_op(hipStream_t st, const AD& a, const AdjScal& sc, int mode, const 
     FnMagOp<1> fn{a.dx, a.dy};
     vjp(st, fn, a, fields({x, Qe, u}, {nullptr, Qebar, ubar}), y, sc, sbar);
   }
+}
+void adj_mag_jac(hipStream_t st, const AD& a, const AdjScal& sc, const double* Qe, const double* u, double* J) {
+  FnMagOp<0> fn{a.dx, a.dy};
+  // field 0 (the unknown) only needs finite values: the operator is linear in it
+  k_jac<FnMagOp<0>><<<nblk(a), AB, 0, st>>>(fn, a, fields({Qe + 5 * a.PS, Qe, u}, {nullptr, nullptr, nullptr}), to_sc(sc), J);
+}
+void adj_mag_op_t(hipStream_t st, const AD& a, const double* J, const double* y, double* out) {
+  k_jac_t<FnMagOp<0>><<<nblk(a), AB, 0, st>>>(a, J, y, out);
 }
 void adj_emf(hipStream_t st, const AD& a, const AdjScal& sc, const double* u, const double* Qe, const double* pe,
              const double* fbar, double* ubar, double* Qebar, double* pebar, double* sbar) {
@@ -860,6 +1101,35 @@ void adj_pri2con(hipStream_t st, const AD& a, const Phys& P, const double* pri, 
   FnPri2Con fn{P.gamma - 1.0};
   AdjScal z{};
   vjp(st, fn, a, fields({pri}, {pribar}), conbar, z, nullptr);
+}
+
+static inline int nblkc(const AD& a, int nc);
+int akr_blocks(const AD& a, int nc) { const dim3 g = akr_grid(a, nc); return (int)(g.x * g.y); }
+void akr_init(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, double* pn, double tol) {
+  k_akr_norm<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, b, pn);
+  k_akr_init<<<1, 1024, 0, st>>>(K, pn, akr_blocks(a, nc), tol);
+}
+void akr_residual(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, const double* t, double* r,
+                  double* v0, double* pn) {
+  k_akr_sub<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, b, t, r, pn);
+  k_akr_cycle<<<1, 1024, 0, st>>>(K, pn, akr_blocks(a, nc));
+  k_akr_scale<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, K, 0, r, v0);
+}
+void akr_gs_pass(hipStream_t st, const AD& a, int nc, int nv, const AdjVecs& V, double* w, AdjKry* K, int d,
+                 double* part, double* pn) {
+  const int nb = akr_blocks(a, nc);
+  k_akr_dots<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, nv, V, w, K, part);
+  k_akr_dsum<<<1, 1024, 0, st>>>(K, part, nb, nv, d);
+  k_akr_project<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, nv, V, w, K, d, pn);
+}
+void akr_column(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* pn, int j, int m) {
+  k_akr_column<<<1, 1024, 0, st>>>(K, pn, akr_blocks(a, nc), j, m);
+}
+void akr_next(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* w, double* v) {
+  k_akr_scale<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, K, 1, w, v);
+}
+void akr_update_x(hipStream_t st, const AD& a, int nc, const AdjVecs& V, AdjKry* K, double* x) {
+  k_akr_update_x<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, V, K, x);
 }
 
 void adj_dots(hipStream_t st, const AD& a, int nc, int nv, const double* const* v, const double* w, double* out) {
