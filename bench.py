@@ -300,7 +300,8 @@ def run_adjoint(n, steps, every, ndir, seed, eps=1e-6, noise=1e-3):
     ad, fd = np.array(ad), np.array(fd)
     rel = np.abs(ad - fd) / np.abs(fd)
     return {"workload": f"MHD vortex {n}x{n} (seeded 1e-3 relative perturbation), FP64, {steps} steps, "
-                        f"J = total kinetic energy at the final step, checkpoint every {every} steps",
+                        f"J = total kinetic energy at the final step, checkpoint every {every} steps "
+                        f"(tapes of all steps kept in HBM: {info['recomputed_steps'] == 0})",
             "directions": ndir, "fd_eps": eps, "direction_seed": seed,
             "rel_error_median": float(f"{np.median(rel):.3e}"), "rel_error_max": float(f"{rel.max():.3e}"),
             "rel_error_all_directions": float(f"{np.linalg.norm(ad - fd) / np.linalg.norm(fd):.3e}"),

@@ -81,8 +81,10 @@ struct AdjKry {
   int jlast, cdone, solved, iters;   // last column, cycle ended, solve converged, Arnoldi steps so far
 };
 struct AdjVecs { const double* p[ADJ_M + 1]; };
-// Per-block partial sums: `part` holds akr_blocks x (ADJ_M + 1) doubles, `pn` akr_blocks doubles.
+// Per-block partial sums: `part` holds akr_blocks x (ADJ_M + 1) doubles (dot products), `pn`
+// akr_blocks1 doubles (norms).
 int akr_blocks(const AD& a, int nc);
+int akr_blocks1(const AD& a, int nc);
 // bnorm = ||b||, res = bnorm; solved when b = 0
 void akr_init(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, double* pn, double tol);
 // start of a cycle: r = b - t, beta = ||r||; v0 = r / beta (zero once the solve is done)

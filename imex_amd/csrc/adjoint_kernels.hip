@@ -722,13 +722,13 @@ __global__ void __launch_bounds__(AB) k_adj_axpys(AD a, int nc, int nv, VCoef cf
   for (int q = 0; q < nv; q++) s += cf.c[q] * v.p[q][c];
   w[c] += sign * s;
 }
+// one grid row of one plane per block row (x over the row's cells): no division per element
 __global__ void __launch_bounds__(AB) k_adj_lin(AD a, int nc, int op, const double* __restrict__ in, const double* __restrict__ d,
                                                 double s, double* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = a.nx * a.ny;
-  if (n >= N * nc) return;
-  const int comp = n / N, m = n % N;
-  const long long c = comp * a.PS + tix(a, m % a.nx, m / a.nx);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y;
+  if (i >= a.nx || row >= nc * a.ny) return;
+  const int comp = row / a.ny, jr = row - comp * a.ny;
+  const long long c = comp * a.PS + (long long)(i + GC) + (long long)(jr + GC) * a.NX;
   switch (op) {
     case 0: out[c] = s * in[c]; break;            // scale-copy
     case 1: out[c] = in[c] / d[c]; break;         // divide by a diagonal (one plane)
@@ -751,14 +751,18 @@ __global__ void __launch_bounds__(AB) k_adj_compact(AD a, int nc, int dir, const
 // rows of all nc planes, so an element's padded index needs no division.  Sums are written per
 // block (no atomics) and reduced in a fixed order by the one-block kernels that use them.
 #define AKR_ROWS 8
+// element r of the thread in a grid of `rows` rows per block
+template <int ROWS = AKR_ROWS>
 __device__ __forceinline__ long long akr_index(const AD& a, int nc, int r, bool& ok) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y * AKR_ROWS + r;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y * ROWS + r;
   ok = i < a.nx && row < nc * a.ny;
   const int rr = ok ? row : 0, comp = rr / a.ny, j = rr - comp * a.ny;
   return comp * a.PS + (long long)((ok ? i : 0) + GC) + (long long)(j + GC) * a.NX;
 }
 __device__ __forceinline__ int akr_block() { return blockIdx.x + gridDim.x * blockIdx.y; }
 static inline dim3 akr_grid(const AD& a, int nc) { return dim3((a.nx + AB - 1) / AB, (nc * a.ny + AKR_ROWS - 1) / AKR_ROWS); }
+// one row per block: the elementwise kernels, whose many blocks keep more loads in flight
+static inline dim3 akr_grid1(const AD& a, int nc) { return dim3((a.nx + AB - 1) / AB, nc * a.ny); }
 
 // the block's sum of v (all threads call it); thread 0 gets the result
 __device__ __forceinline__ double akr_block_sum(double v) {
@@ -818,13 +822,9 @@ __global__ void __launch_bounds__(1024) k_akr_cycle(AdjKry* K, const double* __r
 __global__ void __launch_bounds__(AB) k_akr_scale(AD a, int nc, const AdjKry* __restrict__ K, int which,
                                                   const double* __restrict__ in, double* __restrict__ out) {
   const double sc = 1.0 / (which == 0 ? K->beta : K->hn);
-  const bool zero = K->cdone;
-#pragma unroll
-  for (int r = 0; r < AKR_ROWS; r++) {
-    bool ok;
-    const long long c = akr_index(a, nc, r, ok);
-    if (ok) out[c] = zero ? 0.0 : in[c] * sc;
-  }
+  bool ok;
+  const long long c = akr_index<1>(a, nc, 0, ok);
+  if (ok) out[c] = K->cdone ? 0.0 : in[c] * sc;
 }
 
 // part[block][q] = the block's share of V_q . w, four basis vectors per pass so that the
@@ -858,13 +858,16 @@ __global__ void __launch_bounds__(AB) k_akr_dots(AD a, int nc, int nv, AdjVecs V
   }
 }
 
-// d (d1 or d2)[q] = sum over the blocks of part[.][q], q < nv
+// d (d1 or d2)[q] = sum over the blocks of part[.][q], q < nv: one wave per q, lanes over blocks
 __global__ void __launch_bounds__(1024) k_akr_dsum(AdjKry* K, const double* __restrict__ part, int nb, int nv, int d) {
   if (K->cdone) return;
   double* dst = d == 0 ? K->d1 : K->d2;
-  for (int q = 0; q < nv; q++) {
-    const double t = akr_fin_sum(part + q, ADJ_M + 1, nb);
-    if (threadIdx.x == 0) dst[q] = t;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int q = wv; q < nv; q += nw) {
+    double s = 0.0;
+    for (int b = lane; b < nb; b += 64) s += part[(long long)b * (ADJ_M + 1) + q];
+    s = wave_sum(s);
+    if (lane == 0) dst[q] = s;
   }
 }
 
@@ -876,17 +879,15 @@ __global__ void __launch_bounds__(AB) k_akr_project(AD a, int nc, int nv, AdjVec
   const double* src = d == 0 ? K->d1 : K->d2;
   for (int q = threadIdx.x; q < nv; q += blockDim.x) cf[q] = src[q];
   __syncthreads();
+  bool ok;
+  const long long c = akr_index<1>(a, nc, 0, ok);
   double sq = 0.0;
-#pragma unroll
-  for (int r = 0; r < AKR_ROWS; r++) {
-    bool ok;
-    const long long c = akr_index(a, nc, r, ok);
-    if (!ok) continue;
+  if (ok) {
     double s = 0.0;
     for (int q = 0; q < nv; q++) s += cf[q] * V.p[q][c];
     const double wv = w[c] - s;
     w[c] = wv;
-    sq += wv * wv;
+    sq = wv * wv;
   }
   if (pn) {
     const double t = akr_block_sum(sq);
@@ -932,35 +933,32 @@ __global__ void __launch_bounds__(AB) k_akr_update_x(AD a, int nc, AdjVecs V, co
                                                      double* __restrict__ x) {
   const int jl = K->jlast;
   if (jl < 0) return;
-  __shared__ double y[ADJ_M];
+  __shared__ double y[ADJ_M], Hs[ADJ_M * ADJ_M], gs[ADJ_M];   // H and g staged by parallel loads
+  for (int q = threadIdx.x; q < (jl + 1) * ADJ_M; q += blockDim.x) Hs[q] = K->H[q];
+  for (int q = threadIdx.x; q <= jl; q += blockDim.x) gs[q] = K->g[q];
+  __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = jl; i >= 0; i--) {
-      double s = K->g[i];
-      for (int k = i + 1; k <= jl; k++) s -= K->H[i * ADJ_M + k] * y[k];
-      y[i] = s / K->H[i * ADJ_M + i];
+      double s = gs[i];
+      for (int k = i + 1; k <= jl; k++) s -= Hs[i * ADJ_M + k] * y[k];
+      y[i] = s / Hs[i * ADJ_M + i];
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int r = 0; r < AKR_ROWS; r++) {
-    bool ok;
-    const long long c = akr_index(a, nc, r, ok);
-    if (!ok) continue;
-    double s = 0.0;
-    for (int q = 0; q <= jl; q++) s += y[q] * V.p[q][c];
-    x[c] += s;
-  }
+  bool ok;
+  const long long c = akr_index<1>(a, nc, 0, ok);
+  if (!ok) return;
+  double s = 0.0;
+  for (int q = 0; q <= jl; q++) s += y[q] * V.p[q][c];
+  x[c] += s;
 }
 
 // pn[block] = the block's share of ||v||^2
 __global__ void __launch_bounds__(AB) k_akr_norm(AD a, int nc, const double* __restrict__ v, double* __restrict__ pn) {
+  bool ok;
+  const long long c = akr_index<1>(a, nc, 0, ok);
   double sq = 0.0;
-#pragma unroll
-  for (int r = 0; r < AKR_ROWS; r++) {
-    bool ok;
-    const long long c = akr_index(a, nc, r, ok);
-    if (ok) { const double vv = v[c]; sq += vv * vv; }
-  }
+  if (ok) { const double vv = v[c]; sq = vv * vv; }
   const double t = akr_block_sum(sq);
   if (threadIdx.x == 0) pn[akr_block()] = t;
 }
@@ -968,13 +966,10 @@ __global__ void __launch_bounds__(AB) k_akr_norm(AD a, int nc, const double* __r
 // r = b - t, and pn[block] = the block's share of ||r||^2
 __global__ void __launch_bounds__(AB) k_akr_sub(AD a, int nc, const double* __restrict__ b, const double* __restrict__ t,
                                                 double* __restrict__ r, double* __restrict__ pn) {
+  bool ok;
+  const long long c = akr_index<1>(a, nc, 0, ok);
   double sq = 0.0;
-#pragma unroll
-  for (int q = 0; q < AKR_ROWS; q++) {
-    bool ok;
-    const long long c = akr_index(a, nc, q, ok);
-    if (ok) { const double rv = b[c] - t[c]; r[c] = rv; sq += rv * rv; }
-  }
+  if (ok) { const double rv = b[c] - t[c]; r[c] = rv; sq = rv * rv; }
   const double s = akr_block_sum(sq);
   if (threadIdx.x == 0) pn[akr_block()] = s;
 }
@@ -1031,7 +1026,7 @@ void adj_shock(hipStream_t st, const AD& a, const Phys& P, const double* Qe, dou
   FnChi fn{P.gamma, P.kappa1, fmin(a.dx, a.dy), a.dx, a.dy};
   const AFields F = fields({Qe}, {Qebar});
   k_eval<FnChi><<<nblk(a), AB, 0, st>>>(fn, a, F, chi, Sc<double>{});
-  k_adj_lin<<<nblk(a), AB, 0, st>>>(a, 1, 2, nullptr, nullptr, 0.0, chibar);
+  k_adj_lin<<<akr_grid1(a, 1), AB, 0, st>>>(a, 1, 2, nullptr, nullptr, 0.0, chibar);
   k_adj_eta<<<nblk(a), AB, 0, st>>>(a, chi, etabar, chibar);
   AdjScal z{};
   vjp(st, fn, a, F, chibar, z, nullptr);
@@ -1105,31 +1100,31 @@ void adj_pri2con(hipStream_t st, const AD& a, const Phys& P, const double* pri, 
 
 static inline int nblkc(const AD& a, int nc);
 int akr_blocks(const AD& a, int nc) { const dim3 g = akr_grid(a, nc); return (int)(g.x * g.y); }
+int akr_blocks1(const AD& a, int nc) { const dim3 g = akr_grid1(a, nc); return (int)(g.x * g.y); }
 void akr_init(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, double* pn, double tol) {
-  k_akr_norm<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, b, pn);
-  k_akr_init<<<1, 1024, 0, st>>>(K, pn, akr_blocks(a, nc), tol);
+  k_akr_norm<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, b, pn);
+  k_akr_init<<<1, 1024, 0, st>>>(K, pn, akr_blocks1(a, nc), tol);
 }
 void akr_residual(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* b, const double* t, double* r,
                   double* v0, double* pn) {
-  k_akr_sub<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, b, t, r, pn);
-  k_akr_cycle<<<1, 1024, 0, st>>>(K, pn, akr_blocks(a, nc));
-  k_akr_scale<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, K, 0, r, v0);
+  k_akr_sub<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, b, t, r, pn);
+  k_akr_cycle<<<1, 1024, 0, st>>>(K, pn, akr_blocks1(a, nc));
+  k_akr_scale<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, K, 0, r, v0);
 }
 void akr_gs_pass(hipStream_t st, const AD& a, int nc, int nv, const AdjVecs& V, double* w, AdjKry* K, int d,
                  double* part, double* pn) {
-  const int nb = akr_blocks(a, nc);
   k_akr_dots<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, nv, V, w, K, part);
-  k_akr_dsum<<<1, 1024, 0, st>>>(K, part, nb, nv, d);
-  k_akr_project<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, nv, V, w, K, d, pn);
+  k_akr_dsum<<<1, 1024, 0, st>>>(K, part, akr_blocks(a, nc), nv, d);
+  k_akr_project<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, nv, V, w, K, d, pn);
 }
 void akr_column(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* pn, int j, int m) {
-  k_akr_column<<<1, 1024, 0, st>>>(K, pn, akr_blocks(a, nc), j, m);
+  k_akr_column<<<1, 1024, 0, st>>>(K, pn, akr_blocks1(a, nc), j, m);
 }
 void akr_next(hipStream_t st, const AD& a, int nc, AdjKry* K, const double* w, double* v) {
-  k_akr_scale<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, K, 1, w, v);
+  k_akr_scale<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, K, 1, w, v);
 }
 void akr_update_x(hipStream_t st, const AD& a, int nc, const AdjVecs& V, AdjKry* K, double* x) {
-  k_akr_update_x<<<akr_grid(a, nc), AB, 0, st>>>(a, nc, V, K, x);
+  k_akr_update_x<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, V, K, x);
 }
 
 void adj_dots(hipStream_t st, const AD& a, int nc, int nv, const double* const* v, const double* w, double* out) {
@@ -1146,16 +1141,16 @@ void adj_axpys(hipStream_t st, const AD& a, int nc, int nv, const double* coef, 
 }
 static inline int nblkc(const AD& a, int nc) { return (a.nx * a.ny * nc + AB - 1) / AB; }
 void adj_scale_copy(hipStream_t st, const AD& a, int nc, const double* in, double s, double* out) {
-  k_adj_lin<<<nblkc(a, nc), AB, 0, st>>>(a, nc, 0, in, nullptr, s, out);
+  k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 0, in, nullptr, s, out);
 }
 void adj_div(hipStream_t st, const AD& a, int nc, const double* in, const double* diag, double* out) {
-  k_adj_lin<<<nblkc(a, nc), AB, 0, st>>>(a, nc, 1, in, diag, 0.0, out);
+  k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 1, in, diag, 0.0, out);
 }
 void adj_zero(hipStream_t st, const AD& a, int nc, double* p) {
-  k_adj_lin<<<nblkc(a, nc), AB, 0, st>>>(a, nc, 2, nullptr, nullptr, 0.0, p);
+  k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 2, nullptr, nullptr, 0.0, p);
 }
 void adj_add(hipStream_t st, const AD& a, int nc, const double* in, double s, double* out) {
-  k_adj_lin<<<nblkc(a, nc), AB, 0, st>>>(a, nc, 3, in, nullptr, s, out);
+  k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 3, in, nullptr, s, out);
 }
 void adj_from_compact(hipStream_t st, const AD& a, int nc, const double* in, double* out) {
   k_adj_compact<<<nblkc(a, nc), AB, 0, st>>>(a, nc, 0, in, out);
