@@ -222,9 +222,26 @@ __device__ __forceinline__ double enthalpy_of(const Phys& P, const double* c, in
 }
 
 // ---------------------------------------------------------------- block reductions (wave64)
+// Wave-wide sum with DPP lane moves (register to register, no LDS): an inclusive scan within each
+// row of 16 lanes (row_shr 1, 2, 4, 8, shifted-in lanes read zero), then the row totals carried
+// across rows (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3); lane 63 ends with
+// the total, which is returned to every lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+  v += dpp_move<0x111, 0xf>(v);      // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);      // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);      // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);      // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);      // row_bcast:15
+  v += dpp_move<0x143, 0xc>(v);      // row_bcast:31
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
