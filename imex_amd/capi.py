@@ -50,6 +50,7 @@ class Params(C.Structure):
         ("jet_rho", C.c_double), ("jet_v", C.c_double), ("jet_B0", C.c_double), ("jet_p", C.c_double),
         ("jet_halfwidth", C.c_double), ("rho_amb", C.c_double), ("p_amb", C.c_double),
         ("mixed_flux", C.c_int),
+        ("adjoint_recompute", C.c_int),
     ]
 
 

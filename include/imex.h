@@ -122,6 +122,10 @@ typedef struct imex_params {
                                      (1e-7 relative, cell to cell) enters the right-hand sides of the magnetic and
                                      pressure solves, whose tolerances then cost more Arnoldi steps than the FP32
                                      fluxes save */
+  int adjoint_recompute;          /* imex_adjoint_forward / _backward: 0 (default) = keep every step's tape in
+                                     device memory when the tapes fit in half of the free memory (the backward pass
+                                     then recomputes nothing), else keep checkpoints only; 1 = checkpoints only,
+                                     the backward pass re-runs each segment of `every` steps (least memory) */
 } imex_params;
 
 /* Step-global scalars: the mutable globals read by the operators (config.hpp:83-94). */

@@ -48,6 +48,8 @@ def test_default_params_match_reference_config():
     assert (p.floor_on, p.p_floor, p.theta, p.mbig_fac, p.newton_max, p.newton_tol) == (1, 1e-7, 0.9, 1e6, 20, 1e-8)
     assert (p.rk_alfa, p.rk_b, p.rk_R, p.rk_cv) == (0.05, 0.1, 0.4, 1.0)
     assert (p.schedule, p.budget_margin) == (im.SCHED_AUTO, 1)
+    assert (p.mixed_flux, p.mixed_pressure, p.adjoint_recompute) == (0, 0, 0)
+    assert C.sizeof(im.Params) == C.sizeof(im.Params.from_buffer_copy(bytes(p)))
 
 
 def test_vortex_initial_data():
