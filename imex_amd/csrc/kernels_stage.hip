@@ -400,14 +400,20 @@ __device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL
     T q1 = T(1.0 / 3.0) * um1 + T(5.0 / 6.0) * u0 - T(1.0 / 6.0) * up1;
     T q2 = T(11.0 / 6.0) * u0 - T(7.0 / 6.0) * up1 + T(1.0 / 3.0) * up2;
     T a0 = T(0.3) * P0, a1 = T(0.6) * P1, a2 = T(0.1) * P2;
-    wL = std::is_same<T, float>::value ? (a0 * q0 + a1 * q1 + a2 * q2) * rcp_t(a0 + a1 + a2)
-                                       : (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+    const T numL = a0 * q0 + a1 * q1 + a2 * q2, denL = a0 + a1 + a2;
     q0 = T(1.0 / 3.0) * um2 - T(7.0 / 6.0) * um1 + T(11.0 / 6.0) * u0;
     q1 = T(-1.0 / 6.0) * um1 + T(5.0 / 6.0) * u0 + T(1.0 / 3.0) * up1;
     q2 = T(1.0 / 3.0) * u0 + T(5.0 / 6.0) * up1 - T(1.0 / 6.0) * up2;
     a0 = T(0.1) * P0; a1 = T(0.6) * P1; a2 = T(0.3) * P2;
-    wR = std::is_same<T, float>::value ? (a0 * q0 + a1 * q1 + a2 * q2) * rcp_t(a0 + a1 + a2)
-                                       : (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+    const T numR = a0 * q0 + a1 * q1 + a2 * q2, denR = a0 + a1 + a2;
+    if (std::is_same<T, float>::value) {
+      wL = numL * rcp_t(denL);
+      wR = numR * rcp_t(denR);
+    } else {                    // one division for both faces (denL denR stays far inside the FP64 range)
+      const T inv = T(1) / (denL * denR);
+      wL = numL * (denR * inv);
+      wR = numR * (denL * inv);
+    }
   }
 }
 
