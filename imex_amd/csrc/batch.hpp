@@ -34,6 +34,7 @@ struct Inst {
   float* fbuf[NFBUF];
   double* S;             // step scalars (ScalarSlot)
   double* partial;       // per-block partial sums
+  double* partial2;      // two per-block values a producer leaves for a later finish (Mbig, positivity)
   double* pst;           // positivity statistics (min p, max floored, violation)
   int* eflag;            // EOS iteration cap reached
   KrylovDev* K;          // GMRES / Newton control block

@@ -70,13 +70,17 @@ __global__ void k_budget_gmres_check(const Inst* __restrict__ IT, BMap bm, int s
   }
   if (t < nsw) ctl[slot + 2 + t] = t < sweeps && t < MAXSWEEP ? K->sweep_steps[t] : 0;
 }
-__global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int slot) {
+// Newton log: ctl[slot] = Jacobian solves the solve needed, or -1 when it did not converge; with the
+// floor active also the positivity statistics of the final p, from the last Newton residual's
+// partials (grid src).
+__global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int slot, BMap src, int pst, double thr) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   if (threadIdx.x == 0) {
     gmem(I.ctl)[slot] = gmem(I.K)->newton_conv ? gmem(I.K)->newton_first : -1;
     if (!gmem(I.K)->newton_conv) gmem(I.ctl)[0] |= 2;
   }
+  if (pst) pstats_reduce(I, bmap_count(src, inst_), thr);
 }
 
 // ---------------------------------------------------------------- per-kernel-class event profiler
@@ -123,7 +127,7 @@ struct BudgetRec {
   int nsw = 1, B = 0;                     // budgeted sweeps and Arnoldi steps per sweep
   std::vector<int> ra, ru, rfa, rfu;      // profiler records of the Arnoldi launches and finishes, sweep-major
   std::vector<int> rres, rresf, rx;       // per sweep (+ the closing residual of a mixed-form solve)
-  std::vector<int> rnres, rnresf, rfill, raxpy;   // Newton: residual (+finish) / dp fill / update per iteration
+  std::vector<int> rnres, rnresf, raxpy;          // Newton: residual (+finish) / update per iteration
 };
 // Launches of a captured step whose execution depends on the budgets: Arnoldi step j of sweep
 // sw of GMRES key `key` (4 kernels), the residual (+finish) opening sweep sw, the x update of
@@ -524,7 +528,7 @@ static int run_gmres_budget(imex_ctx* c, int op, const MagIds& mo, const PIds& p
       tag_last(c, TG_ARN, k, sw, j);
     }
     e0 = prof_begin(c);
-    gmres_xupdate(L, op, prec, ix);
+    gmres_xupdate(L, op, prec, ix, x_is_zero && sw == 0, skip);
     r.rx.push_back(prof_end(c, e0, kcls(op, po, KC_MAG_XUPD, KC_P_XUPD, KC_E_XUPD), (double)nc * (2 + vb * r.B)));
     tag_last(c, TG_XUPD, k, sw, 0);
   }
@@ -603,13 +607,13 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
     int rx;
     {
       int e0 = prof_begin(c);
-      gmres_xupdate(L, op, prec, ix);
+      gmres_xupdate(L, op, prec, ix, xz, sk);
       rx = prof_end(c, e0, kcls(op, po, KC_MAG_XUPD, KC_P_XUPD, KC_E_XUPD), (double)nc * (2 + vb * (ks.jlast + 1)));
     }
     if (c->prof.on && c->prof.gen == gen0) {      // launches queued past convergence exited at once
       auto idle = [&](int r) { if (r >= 0) { c->prof.recs[r].cls = KC_EARLY_EXIT; c->prof.recs[r].bytes = 0; } };
       for (int jj = ks.steps; jj < j; jj++) { idle(recs_a[jj]); idle(recs_u[jj]); idle(recs_fa[jj]); idle(recs_fu[jj]); }
-      if (ks.jlast < 0) idle(rx);
+      if (ks.jlast < 0 && (!xz || (sk == SKIP_NEWTON && ks.newton_conv))) idle(rx);   // x = 0 is written
     }
     total += ks.steps;
     maxsweep = std::max(maxsweep, ks.steps);
@@ -632,16 +636,25 @@ static int run_gmres(imex_ctx* c, int op, const MagIds& mo, const PIds& po, int 
   return rc;
 }
 
+// ---------------------------------------------------------------- Newton helpers
+// Mbig (Cor. 1) before a Newton solve: from k_p_rhs's per-block partials when that kernel has just
+// formed b (mbig_ready), else from a reduction over b.
+static void mbig_launch(imex_ctx* c, int ib, bool mbig_ready) {
+  if (mbig_ready) newton_mbig_fin_prhs(c->L, c->P);
+  else newton_mbig(c->L, c->P, ib, c->cur.u);
+}
+static inline double pst_threshold(const imex_ctx* c) { return c->prm.theta * c->prm.p_floor - 1e-14; }
+
 // ---------------------------------------------------------------- nested Newton, budgeted schedule
 // Pressure_NestedNewton (implicit_operators.cpp:625-738) with a fixed number of Jacobian solves:
 // residual, solve, update per iteration, then the residual that must have converged.  Once a
 // residual has converged, the later residuals, fills, solves and updates of this Newton solve
 // skip on the device.
-static int run_newton_budget(imex_ctx* c, int ib, int ip) {
+static int run_newton_budget(imex_ctx* c, int ib, int ip, bool mbig_ready) {
   const Launch& L = c->L;
   if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL)
     return run_gmres_budget(c, 1, mag_ids(c), p_ids(c, 0, 1), ip, ib, 0, SKIP_NONE, -1, -1);
-  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(L, c->P, ib, c->cur.u));
+  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, mbig_launch(c, ib, mbig_ready));
   const int NB = std::min(c->prm.newton_max, std::max(1, c->tab.N[nkey(c)]));
   const int self = (int)c->brecs.size();
   {
@@ -667,11 +680,7 @@ static int run_newton_budget(imex_ctx* c, int ib, int ip) {
       vslope_launch(L, c->P, ip, c->cur.u);
       tag_last(c, TG_NIT, nk, 0, it);
     }
-    e0 = prof_begin(c);
-    fill_launch(L, B_DP, 0.0, SKIP_NEWTON);
-    rr = prof_end(c, e0, KC_MISC, 1);
-    c->brecs[self].rfill.push_back(rr);
-    tag_last(c, TG_NIT, nk, 0, it);
+    // the solve starts from dp = 0: its first x update writes dp instead of reading it
     run_gmres_budget(c, 1, mag_ids(c), p_ids(c, 1, 1), B_DP, B_NG, 1, SKIP_NEWTON, self, it);
     c->newton_it = 0;
     e0 = prof_begin(c);
@@ -680,15 +689,15 @@ static int run_newton_budget(imex_ctx* c, int ib, int ip) {
     c->brecs[self].raxpy.push_back(rr);
     tag_last(c, TG_NIT, nk, 0, it);
   }
-  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot);
-  if (c->prm.floor_on)
-    LAUNCH(KC_MISC, 1, pstats_launch(L, ip, c->prm.p_floor, c->prm.theta * c->prm.p_floor - 1e-14));
+  // the last residual that ran (a converged one, or the closing residual NB) saw the final p
+  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot, L.bm[SH_TILES],
+                                                                   c->prm.floor_on, pst_threshold(c));
   return 0;
 }
 
 // ---------------------------------------------------------------- nested Newton (implicit_operators.cpp:625-738)
-static int run_newton(imex_ctx* c, int ib, int ip, imex_solve_info* info) {
-  if (c->budget) return run_newton_budget(c, ib, ip);
+static int run_newton(imex_ctx* c, int ib, int ip, imex_solve_info* info, bool mbig_ready) {
+  if (c->budget) return run_newton_budget(c, ib, ip, mbig_ready);
   int rc = 0;
   const Launch& L = c->L;
   if (!c->prm.floor_on && c->prm.eos == IMEX_EOS_IDEAL) {
@@ -696,7 +705,7 @@ static int run_newton(imex_ctx* c, int ib, int ip, imex_solve_info* info) {
     c->stats.p_solves++;
     return rc;
   }
-  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, newton_mbig(L, c->P, ib, c->cur.u));
+  if (c->prm.floor_on) LAUNCH(KC_MISC, 1, mbig_launch(c, ib, mbig_ready));
   int it;
   double r0 = 0, r = 0;
   for (it = 0; it < c->prm.newton_max; it++) {
@@ -706,8 +715,7 @@ static int run_newton(imex_ctx* c, int ib, int ip, imex_solve_info* info) {
     LAUNCH(KC_NEWTON_RES, 6, newton_residual(L, p_ids(c, 0, 0), c->P, ip, ib, it, c->prm.newton_tol));
     LAUNCH(KC_KFIN, 0, newton_residual_fin(L, it, c->prm.newton_tol));
     if (c->P.eos != IMEX_EOS_IDEAL) vslope_launch(L, c->P, ip, c->cur.u);
-    LAUNCH(KC_MISC, 1, fill_launch(L, B_DP, 0.0, SKIP_NONE));
-    long long its = 0, rss = 0;
+    long long its = 0, rss = 0;     // dp = 0 is written by the solve's first x update
     // Newton usually needs pred_newton Jacobian solves; past that the solve is expected to be
     // skipped, so only a one-step chunk is queued speculatively
     int first = it < c->pred_newton ? c->pred[1] : 1;
@@ -725,8 +733,11 @@ static int run_newton(imex_ctx* c, int ib, int ip, imex_solve_info* info) {
   c->pred_newton = std::max(1, it);
   c->step_mx[2] = std::max(c->step_mx[2], it);
   if (it == c->prm.newton_max) { rc |= IMEX_WARN_NEWTON_MAXITER; c->stats.warnings |= IMEX_WARN_NEWTON_MAXITER; }
-  if (c->prm.floor_on)
-    LAUNCH(KC_MISC, 1, pstats_launch(L, ip, c->prm.p_floor, c->prm.theta * c->prm.p_floor - 1e-14));
+  // a converged solve's last residual saw the final p; at the iteration cap p moved after it
+  if (c->prm.floor_on) {
+    if (it == c->prm.newton_max) LAUNCH(KC_MISC, 1, pstats_launch(L, ip, c->prm.p_floor, pst_threshold(c)));
+    else LAUNCH(KC_MISC, 0, pstats_fin_newton(L, pst_threshold(c)));
+  }
   if (info) { info->iters = it; info->res0 = r0; info->res = r; info->status = rc; info->restarts = 0; }
   return rc;
 }
@@ -810,18 +821,18 @@ static int sub_pressure(imex_ctx* c) {
   ensure_hhat(c);
   const int u = c->cur.u;
   if (c->P.eos != IMEX_EOS_IDEAL) launch_visc_rk(c->L, c->P, u);
-  LAUNCH(KC_P_RHS, 15, launch_p_rhs(c->L, 0, u));
+  LAUNCH(KC_P_RHS, 15, launch_p_rhs(c->L, c->P, 0, u));
   static const signed char mom_jv[2] = {1, 2};       // rhov*_star in the jet strip: (0, rho v) (imex_subsystems.cpp:107-118)
   fill_planes(c, {{B_MXS, 0}, {B_MYS, 0}}, 0, mom_jv);
   c->sub_idx = 1;
-  int rc = run_newton(c, B_PB, B_P, nullptr);
+  int rc = run_newton(c, B_PB, B_P, nullptr, true);      // k_p_rhs left the Mbig partials of b
   if (c->hook) c->hook(c, HK_P0);
   LAUNCH(KC_P_MOM, 5, launch_p_momentum(c->L, u));
   fill_planes(c, {{u, 1}, {u, 2}});
   if (c->hook) c->hook(c, HK_MOM);
-  LAUNCH(KC_P_RHS, 12, launch_p_rhs(c->L, 1, u));
+  LAUNCH(KC_P_RHS, 12, launch_p_rhs(c->L, c->P, 1, u));
   c->sub_idx = 2;
-  rc |= run_newton(c, B_PB, B_P, nullptr);
+  rc |= run_newton(c, B_PB, B_P, nullptr, true);
   if (c->hook) c->hook(c, HK_P1);
   LAUNCH(KC_P_ENERGY, 9, launch_p_energy(c->L, c->P, u));
   fill_planes(c, {{u, 4}});
@@ -999,7 +1010,7 @@ static int budget_finish(imex_ctx* c, long long gen0) {
         mx[2] = std::max(mx[2], nf);
         c->ncur[r.key] = std::max(c->ncur[r.key], nf);
         for (size_t q = 0; q < r.rnres.size(); q++) if ((int)q > nf) { idle(r.rnres[q]); idle(r.rnresf[q]); }
-        for (size_t q = 0; q < r.rfill.size(); q++) if ((int)q >= nf) { idle(r.rfill[q]); idle(r.raxpy[q]); }
+        for (size_t q = 0; q < r.raxpy.size(); q++) if ((int)q >= nf) idle(r.raxpy[q]);
         continue;
       }
       const int steps = ctl[r.slot], sweeps = ctl[r.slot + 1];
@@ -1589,6 +1600,7 @@ static int alloc_instance(imex_ctx* c, int i) {
   }
   P8(&I.S, S_NSLOTS);
   P8(&I.partial, (size_t)rows * (MAXM + 2));
+  P8(&I.partial2, 2 * (size_t)std::max<long long>(rows, cdiv(d.nx, 64) * cdiv(d.ny, 4)));
   P8(&I.pst, 4);
   P8(&I.bsave, S_NSLOTS + 4);
   parts.push_back({(void**)&I.eflag, sizeof(int)});
@@ -2035,7 +2047,7 @@ static int prep_pressure_operator(imex_ctx* c, int op, const double* p_freeze, d
   const Dims& d = c->d0();
   if (c->P.eos != IMEX_EOS_IDEAL) launch_visc_rk(c->L, c->P, c->cur.u);
   // Tdiag via the first-pass RHS kernel (its b and p outputs are overwritten by the caller)
-  launch_p_rhs(c->L, 0, c->cur.u);
+  launch_p_rhs(c->L, c->P, 0, c->cur.u);
   if (op == IMEX_OP_P_JACOBIAN) {
     if (!p_freeze) return set_err(IMEX_ERR_ARG, "p_freeze required");
     HIPCHK(hipStreamSynchronize(c->st));
@@ -2103,7 +2115,7 @@ int imex_pressure_newton(imex_ctx* c, const double* b, double* p, imex_solve_inf
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_PB], b, d.N * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->hI[0].buf[B_P], p, d.N * sizeof(double), hipMemcpyHostToDevice));
-  rc = run_newton(c, B_PB, B_P, info);
+  rc = run_newton(c, B_PB, B_P, info, false);
   int r2 = finish(c, rc);
   if (r2 < 0) return r2;
   HIPCHK(hipMemcpy(p, c->hI[0].buf[B_P], d.N * sizeof(double), hipMemcpyDeviceToHost));

@@ -10,6 +10,24 @@
 #define TX 64              // operator tile (cells) in x
 #define TY 4               // operator tile (cells) in y
 
+// Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
+// device: pst[0] = min p ever, pst[1] = max floored count, pst[2] = 1 if min p < theta p_floor - 1e-14.
+// The nblk per-block (min p, floored count) pairs in partial2 come from k_pstats or from the last
+// k_newton_residual of the solve; the first wave of the finishing block reduces them.
+__device__ __forceinline__ void pstats_reduce(const Inst& I, int nblk, double thr) {
+  if (threadIdx.x >= 64) return;
+  // min, and a sum of small integers held in doubles: both exact in any order
+  double m = 1e300, c = 0;
+  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, gmem(I.partial2)[2 * b]); c += gmem(I.partial2)[2 * b + 1]; }
+  m = wave_min(m); c = wave_sum(c);
+  if (threadIdx.x == 0) {
+    double* acc = gmem(I.pst);
+    acc[0] = fmin(acc[0], m);
+    acc[1] = fmax(acc[1], c);
+    if (m < thr) acc[2] = 1.0;
+  }
+}
+
 // Device-resident GMRES / Newton control block.
 struct KrylovDev {
   double H[(MAXM + 2) * (MAXM + 1)];   // H[i*(MAXM+1) + j]
@@ -95,7 +113,7 @@ void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs,
 void launch_mag_rhs(const Launch& L, const Phys& P, int iu, int iQe, int iQs, int ib, int ix0);
 void launch_mag_writeback(const Launch& L, int iu);
 void launch_emf_flux(const Launch& L, int iu, int iQe);
-void launch_p_rhs(const Launch& L, int pass, int iu);
+void launch_p_rhs(const Launch& L, const Phys& P, int pass, int iu);
 void launch_p_momentum(const Launch& L, int iu);
 void launch_p_energy(const Launch& L, const Phys& P, int iu);
 // energy formula: RHS, coefficient plane, initial guess and precE divisor; then E and momentum update
@@ -141,15 +159,16 @@ void arnoldi_apply(const Launch& L, int op, int prec, const MagIds& mo, const PI
 void arnoldi_apply_fin(const Launch& L, int op, int j);
 void arnoldi_update(const Launch& L, int op, int prec, int j, int m);
 void arnoldi_update_fin(const Launch& L, int op, int prec, int j, int m);
-void gmres_xupdate(const Launch& L, int op, int prec, int ix);
+void gmres_xupdate(const Launch& L, int op, int prec, int ix, int x_zero, int skip);
 void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy);
 void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int ib, int it, double ntol);
 void newton_residual_fin(const Launch& L, int it, double ntol);
 void newton_mbig(const Launch& L, const Phys& P, int ib, int iu);
+void newton_mbig_fin_prhs(const Launch& L, const Phys& P);   // Mbig from k_p_rhs's partials
 void vslope_launch(const Launch& L, const Phys& P, int ipf, int iu);
 void axpy_launch(const Launch& L, int iy, int ix, int skip);
-void fill_launch(const Launch& L, int ibuf, double v, int skip);
 void pstats_launch(const Launch& L, int ip, double pfloor, double thr);
+void pstats_fin_newton(const Launch& L, double thr);           // from the last Newton residual's partials
 
 // Records the message returned by imex_last_error and returns `code` (entry points outside driver.hip).
 int imex_internal_error(int code, const std::string& msg);

@@ -907,14 +907,18 @@ __global__ void __launch_bounds__(FIN_THREADS) k_arnoldi_givens_fin(const Inst* 
 
 // ---------------------------------------------------------------- x += sum alpha_i v_i (back substitution per block)
 template <class VT>
-__global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ IT, BMap bm, int ix, int nc) {
+// x_zero: the first sweep of a solve from x = 0, whose x is written (0 + V alpha, or 0 when the
+// sweep took no Arnoldi step) rather than read, so the caller need not clear it beforehand.
+__global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ IT, BMap bm, int ix, int nc, int x_zero,
+                                                       int skip) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   KrylovDev* K = gmem(I.K);
+  if (skipped(I, skip)) return;
   __shared__ double alpha[MAXM + 1];
   __shared__ double Hs[(MAXM + 1) * (MAXM + 1)], gs[MAXM + 1];   // H and gam staged by parallel loads
   const int jl = K->jlast;
-  if (jl < 0) return;
+  if (jl < 0 && !x_zero) return;
   const int L = MAXM + 1;
   for (int q = threadIdx.x; q < (jl + 1) * (jl + 1); q += blockDim.x) {
     const int i = q / (jl + 1), k = q % (jl + 1);
@@ -933,7 +937,7 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ 
   double* __restrict__ x = gmem(I.buf[ix]);
   const long long n = (long long)nc * I.d.N;
   for (long long e = bx_ * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gx_ * blockDim.x) {
-    double xv = x[e];
+    double xv = x_zero ? 0.0 : x[e];
     for (int i = 0; i <= jl; i++) xv += alpha[i] * (double)KV<VT>::V(I, i)[e];
     x[e] = xv;
   }
@@ -987,6 +991,7 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
   const double Mbig = op.S[S_MBIG];
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   double acc = 0.0;
+  double pmin = 1e300, nfl = 0.0;     // positivity statistics of p (with the floor active)
   int ef = 0;
   for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
@@ -1004,11 +1009,25 @@ __global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const
       double g = dir_bnd(d, cx + GC, cy + GC) ? Lp - b[cidx] : Lp - a * pf + V_mod(P, pf, rho, Mbig, &ef) - b[cidx];   // :697-699
       ng[cidx] = -g;
       acc += g * g;
+      pmin = fmin(pmin, pf);
+      if (pf < P.p_floor) nfl += 1;
     }
   }
   if (ef) atomicOr(gmem(I.eflag), 1);
   block_sums<1>(&acc, 1, red, res);
   if (threadIdx.x == 0) gmem(I.partial)[bx_] = res[0];   // packed: the finish's loads coalesce
+  if (P.floor_on) {
+    // min p and the floored count of this p (exact in any order), for the positivity statistics:
+    // the last residual a Newton solve runs is the one of its final p
+    __shared__ double sp[8];
+    pmin = wave_min(pmin); nfl = wave_sum(nfl);
+    if ((threadIdx.x & 63) == 0) { sp[threadIdx.x >> 6] = pmin; sp[4 + (threadIdx.x >> 6)] = nfl; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++) { pmin = fmin(pmin, sp[w]); nfl += sp[4 + w]; }
+      gmem(I.partial2)[2 * bx_] = pmin; gmem(I.partial2)[2 * bx_ + 1] = nfl;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(FIN_THREADS) k_newton_fin(const Inst* __restrict__ IT, BMap bm, BMap src, int it, double ntol) {
@@ -1055,17 +1074,19 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Phys P, const Inst* __restr
   if (threadIdx.x == 0) {
     double a = sh[0], c = sh[4];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sh[w]); c = fmax(c, sh[4 + w]); }
-    gmem(I.partial)[bx_ * 2] = a; gmem(I.partial)[bx_ * 2 + 1] = c;
+    gmem(I.partial2)[bx_ * 2] = a; gmem(I.partial2)[bx_ * 2 + 1] = c;
   }
 }
 
+// Reduces the (min b, max rho e) pairs in partial2 of the grid src: k_newton_mbig's, or k_p_rhs's
+// when that kernel has just formed b.
 __global__ void k_newton_mbig_fin(Phys P, const Inst* __restrict__ IT, BMap bm, BMap src) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   const int nblk = SRC_BLOCKS(src);
   // min and max are order independent: one wave reduces the block partials
   double a = 1e300, c = -1e300;
-  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, gmem(I.partial)[bl * 2]); c = fmax(c, gmem(I.partial)[bl * 2 + 1]); }
+  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, gmem(I.partial2)[bl * 2]); c = fmax(c, gmem(I.partial2)[bl * 2 + 1]); }
   a = wave_min(a); c = wave_max(c);
   if (threadIdx.x == 0) {
     double Vpe = P.p_floor / (P.gamma - 1.0);
@@ -1096,16 +1117,9 @@ __global__ void k_axpy1(const Inst* __restrict__ IT, BMap bm, int iy, int ix, in
   if (e < I.d.N) gmem(I.buf[iy])[e] += gmem(I.buf[ix])[e];
 }
 
-__global__ void k_fill_n(const Inst* __restrict__ IT, BMap bm, int ib, double v, int skip) {
-  BATCH_BLOCK(bm);
-  const Inst& I = IT[inst_];
-  if (skipped(I, skip)) return;
-  long long e = bx_ * (long long)blockDim.x + threadIdx.x;
-  if (e < I.d.N) gmem(I.buf[ib])[e] = v;
-}
 
-// Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
-// device: pst[0] = min p ever, pst[1] = max floored count, pst[2] = 1 if min p < theta p_floor - 1e-14.
+// Per-block (min p, floored count) of p for pstats_reduce, when no Newton residual has seen the
+// final p (the host-polled Newton loop stopped at its iteration cap after an update).
 __global__ void k_pstats(const Inst* __restrict__ IT, BMap bm, int ip, double pfloor) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -1123,24 +1137,13 @@ __global__ void k_pstats(const Inst* __restrict__ IT, BMap bm, int ip, double pf
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) { mn = fmin(mn, smin[w]); cnt += scnt[w]; }
-    gmem(I.partial)[2 * bx_] = smin[0] < mn ? smin[0] : mn;
-    gmem(I.partial)[2 * bx_ + 1] = cnt;
+    gmem(I.partial2)[2 * bx_] = smin[0] < mn ? smin[0] : mn;
+    gmem(I.partial2)[2 * bx_ + 1] = cnt;
   }
 }
 __global__ void k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, double thr) {
   BATCH_BLOCK(bm);
-  const Inst& I = IT[inst_];
-  const int nblk = SRC_BLOCKS(src);
-  // min, and a sum of small integers held in doubles: both exact in any order, one wave reduces
-  double m = 1e300, c = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, gmem(I.partial)[2 * b]); c += gmem(I.partial)[2 * b + 1]; }
-  m = wave_min(m); c = wave_sum(c);
-  if (threadIdx.x == 0) {
-    double* acc = gmem(I.pst);
-    acc[0] = fmin(acc[0], m);
-    acc[1] = fmax(acc[1], c);
-    if (m < thr) acc[2] = 1.0;
-  }
+  pstats_reduce(IT[inst_], SRC_BLOCKS(src), thr);
 }
 
 // ================================================================ launchers
@@ -1213,11 +1216,11 @@ void arnoldi_update_fin(const Launch& L, int op, int prec, int j, int m) {
   k_arnoldi_givens_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[vec_shape(op)], j, m, prec);
 }
 
-void gmres_xupdate(const Launch& L, int op, int prec, int ix) {
+void gmres_xupdate(const Launch& L, int op, int prec, int ix, int x_zero, int skip) {
   const int nc = op == 0 ? 3 : 1;
   const int sh = vec_shape(op);
-  if (prec == 0) k_gmres_xupdate<double><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], ix, nc);
-  else k_gmres_xupdate<float><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], ix, nc);
+  if (prec == 0) k_gmres_xupdate<double><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], ix, nc, x_zero, skip);
+  else k_gmres_xupdate<float><<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], ix, nc, x_zero, skip);
 }
 
 void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy) {
@@ -1239,6 +1242,9 @@ void newton_mbig(const Launch& L, const Phys& P, int ib, int iu) {
   k_newton_mbig<<<GRID(L, SH_MBIG), 256, 0, L.st>>>(P, L.IT, L.bm[SH_MBIG], ib, iu);
   k_newton_mbig_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_MBIG]);
 }
+void newton_mbig_fin_prhs(const Launch& L, const Phys& P) {
+  k_newton_mbig_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_INT64x4]);
+}
 
 void vslope_launch(const Launch& L, const Phys& P, int ipf, int iu) {
   k_vslope<<<GRID(L, SH_LIN_N), 256, 0, L.st>>>(P, L.IT, L.bm[SH_LIN_N], ipf, iu);
@@ -1248,11 +1254,11 @@ void axpy_launch(const Launch& L, int iy, int ix, int skip) {
   k_axpy1<<<GRID(L, SH_LIN_N), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_N], iy, ix, skip);
 }
 
-void fill_launch(const Launch& L, int ib, double v, int skip) {
-  k_fill_n<<<GRID(L, SH_LIN_N), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_N], ib, v, skip);
-}
 
 void pstats_launch(const Launch& L, int ip, double pfloor, double thr) {
   k_pstats<<<GRID(L, SH_PSTATS), 256, 0, L.st>>>(L.IT, L.bm[SH_PSTATS], ip, pfloor);
   k_pstats_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_PSTATS], thr);
+}
+void pstats_fin_newton(const Launch& L, double thr) {
+  k_pstats_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], thr);
 }

@@ -889,7 +889,10 @@ __global__ void __launch_bounds__(256) k_emf_flux(const Inst* __restrict__ IT, B
 // RHS of PreSubsystem_Pressure (imex_subsystems.cpp:123-168 first pass, :226-270 second pass).
 // pass 0 also stores rho v* (mxs, mys), the initial pressure p0 = p(Qe) and the Jacobi diagonal
 // part Tdiag (implicit_operators.cpp:631-649).
-__global__ void __launch_bounds__(256) k_p_rhs(const Inst* __restrict__ IT, BMap bm, int pass, int iu) {
+// With the pressure floor active it also leaves, per block, min b and (Redlich-Kwong) max rho
+// e(rho, p_floor) in partial2 for the Mbig finish of the Newton solve that follows
+// (implicit_operators.cpp:670-686): the values k_newton_mbig would reduce from the same b.
+__global__ void __launch_bounds__(256) k_p_rhs(Phys P, const Inst* __restrict__ IT, BMap bm, int pass, int iu) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
@@ -904,31 +907,54 @@ __global__ void __launch_bounds__(256) k_p_rhs(const Inst* __restrict__ IT, BMap
   const double* S = gmem(I_.S);
 
   int i = bx_ * blockDim.x + threadIdx.x + GC, j = by_ * blockDim.y + threadIdx.y + GC;
-  if (i >= d.nx + GC || j >= d.ny + GC) return;
-  const double h = S[S_ADT], dx = d.dx, dy = d.dy;
-  long long c = pidx(d, i, j);
-  auto H = [&](int a, int bb) { return hhat[pidx(d, a, bb)]; };
-  const double* MX = pass == 0 ? u + 1 * d.PS : mxs;
-  const double* MY = pass == 0 ? u + 2 * d.PS : mys;
-  auto mx = [&](int a, int bb) { return MX[pidx(d, a, bb)]; };
-  auto my = [&](int a, int bb) { return MY[pidx(d, a, bb)]; };
-  double u5 = u[5 * d.PS + c], u6 = u[6 * d.PS + c], u7 = u[7 * d.PS + c], rho = u[c];
-  double B2 = u5 * u5 + u6 * u6 + u7 * u7;
-  double h00 = H(i, j), hp0 = H(i + 1, j), hm0 = H(i - 1, j), h0p = H(i, j + 1), h0m = H(i, j - 1);
-  double p_star = u[4 * d.PS + c] - 0.5 * B2;
-  double u1 = u[1 * d.PS + c], u2 = u[2 * d.PS + c], u3 = u[3 * d.PS + c];
-  double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3) / rho;   // pass 0: rho v* == u; pass 1: updated momentum
-  double RHS = p_star - ke - h * (1 / dx * (0.5 * (hp0 + h00) * 0.5 * (mx(i + 1, j) + mx(i, j)) - 0.5 * (hm0 + h00) * 0.5 * (mx(i - 1, j) + mx(i, j)))
-                                 + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (my(i, j + 1) + my(i, j)) - 0.5 * (h0m + h00) * 0.5 * (my(i, j - 1) + my(i, j)))));
-  int idx = (i - GC) + d.nx * (j - GC);
-  b[idx] = RHS;
-  if (dir_bnd(d, i, j)) b[idx] = d.p_amb;              // strong Dirichlet row (imex_subsystems.cpp:160-161, 263-264)
-  // jet inflow strip: the Dirichlet lifting of the ghost pressure (imex_subsystems.cpp:163-165, 266-267)
-  if (j == GC && jet_inlet(d, i)) b[idx] += h * h / (dy * dy) * 0.5 * (h00 + h0m) * d.jet_p;
-  if (pass == 0) {
-    mxs[c] = u1; mys[c] = u2;
-    p0[idx] = pe[c];
-    Tdiag[idx] = h * h * ((0.5 * (hp0 + h00) + 0.5 * (hm0 + h00)) / (dx * dx) + (0.5 * (h0p + h00) + 0.5 * (h0m + h00)) / (dy * dy));
+  const bool valid = i < d.nx + GC && j < d.ny + GC;
+  double bmin = 1e300, vmax = -1e300;
+  if (valid) {
+    const double h = S[S_ADT], dx = d.dx, dy = d.dy;
+    long long c = pidx(d, i, j);
+    auto H = [&](int a, int bb) { return hhat[pidx(d, a, bb)]; };
+    const double* MX = pass == 0 ? u + 1 * d.PS : mxs;
+    const double* MY = pass == 0 ? u + 2 * d.PS : mys;
+    auto mx = [&](int a, int bb) { return MX[pidx(d, a, bb)]; };
+    auto my = [&](int a, int bb) { return MY[pidx(d, a, bb)]; };
+    double u5 = u[5 * d.PS + c], u6 = u[6 * d.PS + c], u7 = u[7 * d.PS + c], rho = u[c];
+    double B2 = u5 * u5 + u6 * u6 + u7 * u7;
+    double h00 = H(i, j), hp0 = H(i + 1, j), hm0 = H(i - 1, j), h0p = H(i, j + 1), h0m = H(i, j - 1);
+    double p_star = u[4 * d.PS + c] - 0.5 * B2;
+    double u1 = u[1 * d.PS + c], u2 = u[2 * d.PS + c], u3 = u[3 * d.PS + c];
+    double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3) / rho;   // pass 0: rho v* == u; pass 1: updated momentum
+    double RHS = p_star - ke - h * (1 / dx * (0.5 * (hp0 + h00) * 0.5 * (mx(i + 1, j) + mx(i, j)) - 0.5 * (hm0 + h00) * 0.5 * (mx(i - 1, j) + mx(i, j)))
+                                   + (1 / dy * (0.5 * (h0p + h00) * 0.5 * (my(i, j + 1) + my(i, j)) - 0.5 * (h0m + h00) * 0.5 * (my(i, j - 1) + my(i, j)))));
+    int idx = (i - GC) + d.nx * (j - GC);
+    double bv = dir_bnd(d, i, j) ? d.p_amb : RHS;      // strong Dirichlet row (imex_subsystems.cpp:160-161, 263-264)
+    // jet inflow strip: the Dirichlet lifting of the ghost pressure (imex_subsystems.cpp:163-165, 266-267)
+    if (j == GC && jet_inlet(d, i)) bv += h * h / (dy * dy) * 0.5 * (h00 + h0m) * d.jet_p;
+    b[idx] = bv;
+    if (pass == 0) {
+      mxs[c] = u1; mys[c] = u2;
+      p0[idx] = pe[c];
+      Tdiag[idx] = h * h * ((0.5 * (hp0 + h00) + 0.5 * (hm0 + h00)) / (dx * dx) + (0.5 * (h0p + h00) + 0.5 * (h0m + h00)) / (dy * dy));
+    }
+    if (P.floor_on) {
+      bmin = bv;
+      if (P.eos != IMEX_EOS_IDEAL) {
+        int ef = 0;
+        double v = Vbase(P, P.p_floor, rho, &ef);
+        if (isfinite(v)) vmax = v;
+        if (ef) atomicOr(gmem(I_.eflag), 1);
+      }
+    }
+  }
+  if (!P.floor_on) return;
+  // min and max are order independent: the same Mbig as a reduction over the whole plane
+  __shared__ double sh[8];
+  bmin = wave_min(bmin); vmax = wave_max(vmax);
+  if (threadIdx.x == 0) { sh[threadIdx.y] = bmin; sh[4 + threadIdx.y] = vmax; }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    double a = sh[0], v = sh[4];
+    for (int w = 1; w < (int)blockDim.y; w++) { a = fmin(a, sh[w]); v = fmax(v, sh[4 + w]); }
+    gmem(I_.partial2)[2 * lb_] = a; gmem(I_.partial2)[2 * lb_ + 1] = v;
   }
 }
 
@@ -1512,8 +1538,8 @@ void launch_mag_writeback(const Launch& L, int iu) {
 void launch_emf_flux(const Launch& L, int iu, int iQe) {
   k_emf_flux<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x4], iu, iQe);
 }
-void launch_p_rhs(const Launch& L, int pass, int iu) {
-  k_p_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT64x4], pass, iu);
+void launch_p_rhs(const Launch& L, const Phys& P, int pass, int iu) {
+  k_p_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], pass, iu);
 }
 void launch_p_momentum(const Launch& L, int iu) {
   k_p_momentum<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_INT64x4], iu);
