@@ -73,7 +73,7 @@ __global__ void k_budget_gmres_check(const Inst* __restrict__ IT, BMap bm, int s
 // Newton log: ctl[slot] = Jacobian solves the solve needed, or -1 when it did not converge; with the
 // floor active also the positivity statistics of the final p, from the last Newton residual's
 // partials (grid src).
-__global__ void k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int slot, BMap src, int pst, double thr) {
+__global__ void __launch_bounds__(1024) k_budget_newton_check(const Inst* __restrict__ IT, BMap bm, int slot, BMap src, int pst, double thr) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
   if (threadIdx.x == 0) {
@@ -690,7 +690,7 @@ static int run_newton_budget(imex_ctx* c, int ib, int ip, bool mbig_ready) {
     tag_last(c, TG_NIT, nk, 0, it);
   }
   // the last residual that ran (a converged one, or the closing residual NB) saw the final p
-  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot, L.bm[SH_TILES],
+  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), c->prm.floor_on ? 1024 : 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot, L.bm[SH_TILES],
                                                                    c->prm.floor_on, pst_threshold(c));
   return 0;
 }

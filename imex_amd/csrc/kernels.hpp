@@ -10,21 +10,40 @@
 #define TX 64              // operator tile (cells) in x
 #define TY 4               // operator tile (cells) in y
 
+// Reduces nblk per-block pairs (a, b) at part[2k], part[2k+1] with every thread of the block (up to
+// 1024): min of a, and the sum (SUM) or max of b, all exact in any order; thread 0 gets them in out.
+// Many threads keep each thread's chain of loads short when the producer ran thousands of blocks.
+template <bool SUM>
+__device__ __forceinline__ void pair_reduce(const double* part, int nblk, double* out) {
+  __shared__ double sa[16], sb[16];
+  double a = 1e300, b = SUM ? 0.0 : -1e300;
+  for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+    a = fmin(a, part[2 * k]);
+    b = SUM ? b + part[2 * k + 1] : fmax(b, part[2 * k + 1]);
+  }
+  a = wave_min(a);
+  b = SUM ? wave_sum(b) : wave_max(b);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < nw; q++) { a = fmin(a, sa[q]); b = SUM ? b + sb[q] : fmax(b, sb[q]); }
+    out[0] = a; out[1] = b;
+  }
+}
+
 // Positivity statistics of one Newton solution (implicit_operators.cpp:724-737), accumulated on the
 // device: pst[0] = min p ever, pst[1] = max floored count, pst[2] = 1 if min p < theta p_floor - 1e-14.
 // The nblk per-block (min p, floored count) pairs in partial2 come from k_pstats or from the last
-// k_newton_residual of the solve; the first wave of the finishing block reduces them.
+// k_newton_residual of the solve.
 __device__ __forceinline__ void pstats_reduce(const Inst& I, int nblk, double thr) {
-  if (threadIdx.x >= 64) return;
-  // min, and a sum of small integers held in doubles: both exact in any order
-  double m = 1e300, c = 0;
-  for (int b = threadIdx.x; b < nblk; b += 64) { m = fmin(m, gmem(I.partial2)[2 * b]); c += gmem(I.partial2)[2 * b + 1]; }
-  m = wave_min(m); c = wave_sum(c);
+  double mc[2];
+  pair_reduce<true>(gmem(I.partial2), nblk, mc);
   if (threadIdx.x == 0) {
     double* acc = gmem(I.pst);
-    acc[0] = fmin(acc[0], m);
-    acc[1] = fmax(acc[1], c);
-    if (m < thr) acc[2] = 1.0;
+    acc[0] = fmin(acc[0], mc[0]);
+    acc[1] = fmax(acc[1], mc[1]);
+    if (mc[0] < thr) acc[2] = 1.0;
   }
 }
 

@@ -1080,15 +1080,13 @@ __global__ void __launch_bounds__(256) k_newton_mbig(Phys P, const Inst* __restr
 
 // Reduces the (min b, max rho e) pairs in partial2 of the grid src: k_newton_mbig's, or k_p_rhs's
 // when that kernel has just formed b.
-__global__ void k_newton_mbig_fin(Phys P, const Inst* __restrict__ IT, BMap bm, BMap src) {
+__global__ void __launch_bounds__(FIN_THREADS) k_newton_mbig_fin(Phys P, const Inst* __restrict__ IT, BMap bm, BMap src) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
-  const int nblk = SRC_BLOCKS(src);
-  // min and max are order independent: one wave reduces the block partials
-  double a = 1e300, c = -1e300;
-  for (int bl = threadIdx.x; bl < nblk; bl += 64) { a = fmin(a, gmem(I.partial2)[bl * 2]); c = fmax(c, gmem(I.partial2)[bl * 2 + 1]); }
-  a = wave_min(a); c = wave_max(c);
+  double ac[2];
+  pair_reduce<false>(gmem(I.partial2), SRC_BLOCKS(src), ac);
   if (threadIdx.x == 0) {
+    const double a = ac[0], c = ac[1];
     double Vpe = P.p_floor / (P.gamma - 1.0);
     if (P.eos != IMEX_EOS_IDEAL && c > Vpe) Vpe = c;
     double M_cor = fmax(0.0, Vpe - a) / ((1.0 - P.theta) * P.p_floor);
@@ -1141,7 +1139,7 @@ __global__ void k_pstats(const Inst* __restrict__ IT, BMap bm, int ip, double pf
     gmem(I.partial2)[2 * bx_ + 1] = cnt;
   }
 }
-__global__ void k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, double thr) {
+__global__ void __launch_bounds__(FIN_THREADS) k_pstats_fin(const Inst* __restrict__ IT, BMap bm, BMap src, double thr) {
   BATCH_BLOCK(bm);
   pstats_reduce(IT[inst_], SRC_BLOCKS(src), thr);
 }
@@ -1240,10 +1238,10 @@ void newton_residual_fin(const Launch& L, int it, double ntol) {
 
 void newton_mbig(const Launch& L, const Phys& P, int ib, int iu) {
   k_newton_mbig<<<GRID(L, SH_MBIG), 256, 0, L.st>>>(P, L.IT, L.bm[SH_MBIG], ib, iu);
-  k_newton_mbig_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_MBIG]);
+  k_newton_mbig_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_MBIG]);
 }
 void newton_mbig_fin_prhs(const Launch& L, const Phys& P) {
-  k_newton_mbig_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_INT64x4]);
+  k_newton_mbig_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_INT64x4]);
 }
 
 void vslope_launch(const Launch& L, const Phys& P, int ipf, int iu) {
@@ -1257,8 +1255,8 @@ void axpy_launch(const Launch& L, int iy, int ix, int skip) {
 
 void pstats_launch(const Launch& L, int ip, double pfloor, double thr) {
   k_pstats<<<GRID(L, SH_PSTATS), 256, 0, L.st>>>(L.IT, L.bm[SH_PSTATS], ip, pfloor);
-  k_pstats_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_PSTATS], thr);
+  k_pstats_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_PSTATS], thr);
 }
 void pstats_fin_newton(const Launch& L, double thr) {
-  k_pstats_fin<<<GRID(L, SH_ONE), 64, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], thr);
+  k_pstats_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], thr);
 }
