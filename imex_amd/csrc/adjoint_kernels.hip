@@ -34,6 +34,31 @@ __device__ __forceinline__ void rc_cell(int recon, bool central, const T& um, co
   }
 }
 
+// ---- WENO5 reconstruction of one cell (reconstruction.cpp:184-266, Jiang-Shu weights, eps 1e-6,
+// pow(x, 2) = x x): left / right face values from the five cells um2 .. up2
+template <class T>
+__device__ __forceinline__ void rc5_cell(const T& um2, const T& um1, const T& u0, const T& up1, const T& up2, T& wl, T& wr) {
+  const T t0 = um2 - 2.0 * um1 + u0, t1 = um2 - 4.0 * um1 + 3.0 * u0;
+  const T t2 = um1 - 2.0 * u0 + up1, t3 = um1 - up1;
+  const T t4 = u0 - 2.0 * up1 + up2, t5 = 3.0 * u0 - 4.0 * up1 + up2;
+  const T b0 = (13.0 / 12.0) * (t0 * t0) + 0.25 * (t1 * t1);
+  const T b1 = (13.0 / 12.0) * (t2 * t2) + 0.25 * (t3 * t3);
+  const T b2 = (13.0 / 12.0) * (t4 * t4) + 0.25 * (t5 * t5);
+  const T e0 = (1e-6 + b0) * (1e-6 + b0), e1 = (1e-6 + b1) * (1e-6 + b1), e2 = (1e-6 + b2) * (1e-6 + b2);
+  // left face: linear weights 0.3, 0.6, 0.1 on the three candidate stencils
+  T a0 = 0.3 / e0, a1 = 0.6 / e1, a2 = 0.1 / e2;
+  T q0 = (-1.0 / 6.0) * um2 + (5.0 / 6.0) * um1 + (1.0 / 3.0) * u0;
+  T q1 = (1.0 / 3.0) * um1 + (5.0 / 6.0) * u0 - (1.0 / 6.0) * up1;
+  T q2 = (11.0 / 6.0) * u0 - (7.0 / 6.0) * up1 + (1.0 / 3.0) * up2;
+  wl = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+  // right face: linear weights 0.1, 0.6, 0.3
+  a0 = 0.1 / e0; a1 = 0.6 / e1; a2 = 0.3 / e2;
+  q0 = (1.0 / 3.0) * um2 - (7.0 / 6.0) * um1 + (11.0 / 6.0) * u0;
+  q1 = (-1.0 / 6.0) * um1 + (5.0 / 6.0) * u0 + (1.0 / 3.0) * up1;
+  q2 = (1.0 / 3.0) * u0 + (5.0 / 6.0) * up1 - (1.0 / 6.0) * up2;
+  wr = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
 // ---- physical flux (state.cpp:27-44, 86-106, 298-322)
 template <int DIR, class T>
 __device__ __forceinline__ void phys_flux(int kind, const T* Q, T* f) {
@@ -56,11 +81,15 @@ __device__ __forceinline__ void phys_flux(int kind, const T* Q, T* f) {
 
 // ---- Rusanov face flux (flux.cpp:202-243, 329-367); element (i, j) = face between cells
 // (i-1, j) and (i, j) (DIR 0) or (i, j-1) and (i, j) (DIR 1).  Fields: 0 Qe (0..4), 1 u (B 5..7).
-template <int DIR>
+// W5: WENO5 reconstruction (stencil i-3 .. i+2), else Minmod / VanLeer (i-2 .. i+1).
+template <int DIR, bool W5 = false>
 struct FnFlux {
-  static constexpr int K = 8, NOUT = 5, NS = 4;
+  static constexpr int K = 8, NOUT = 5, NS = W5 ? 6 : 4;
   int kind, recon, central;
-  __device__ static void off(int s, int& oi, int& oj) { oi = DIR == 0 ? s - 2 : 0; oj = DIR == 0 ? 0 : s - 2; }
+  __device__ static void off(int s, int& oi, int& oj) {
+    const int o = W5 ? s - 3 : s - 2;
+    oi = DIR == 0 ? o : 0; oj = DIR == 0 ? 0 : o;
+  }
   __device__ static int slot(int fid, int k) { return fid == 0 ? (k < 5 ? k : -1) : (k >= 5 ? k : -1); }
   __device__ static void which(int q, int& fid, int& k) { fid = q < 5 ? 0 : 1; k = q; }
   template <class T, class R>
@@ -71,11 +100,18 @@ struct FnFlux {
     T QL[8], QR[8];
     for (int k = 0; k < 5; k++) {
       if (k < nrec) {
-        const T a0 = rd(0, k, i - 2 * di, j - 2 * dj), a1 = rd(0, k, i - di, j - dj), a2 = rd(0, k, i, j),
-                a3 = rd(0, k, i + di, j + dj);
         T wl, wr, wl2, wr2;
-        rc_cell(recon, cen, a0, a1, a2, wl, wr);
-        rc_cell(recon, cen, a1, a2, a3, wl2, wr2);
+        if constexpr (W5) {
+          T a[6];
+          for (int o = 0; o < 6; o++) a[o] = rd(0, k, i + (o - 3) * di, j + (o - 3) * dj);
+          rc5_cell(a[0], a[1], a[2], a[3], a[4], wl, wr);
+          rc5_cell(a[1], a[2], a[3], a[4], a[5], wl2, wr2);
+        } else {
+          const T a0 = rd(0, k, i - 2 * di, j - 2 * dj), a1 = rd(0, k, i - di, j - dj), a2 = rd(0, k, i, j),
+                  a3 = rd(0, k, i + di, j + dj);
+          rc_cell(recon, cen, a0, a1, a2, wl, wr);
+          rc_cell(recon, cen, a1, a2, a3, wl2, wr2);
+        }
         QL[k] = wr; QR[k] = wl2;
       } else { QL[k] = T(0.0); QR[k] = T(0.0); }
     }
@@ -981,10 +1017,17 @@ void adj_explicit(hipStream_t st, const AD& a, const Phys& P, int kind, const Ad
   const int k0 = kind == 0 ? 0 : 1, k1 = kind == 0 ? 4 : 5;
   k_adj_fluxdiv<<<nblk(a), AB, 0, st>>>(a, k0, k1, sc.s[SC_H], ubar, uout, Qs, Qsbar, fbx, fby, sbar);
   const AFields F = fields({Qe, uB}, {Qebar, uBbar});
-  FnFlux<0> fx{kind, P.recon, P.central_y};
-  FnFlux<1> fy{kind, P.recon, P.central_y};
-  vjp(st, fx, a, F, fbx, sc, sbar);
-  vjp(st, fy, a, F, fby, sc, sbar);
+  if (P.recon == IMEX_RECON_WENO5) {
+    FnFlux<0, true> fx{kind, P.recon, P.central_y};
+    FnFlux<1, true> fy{kind, P.recon, P.central_y};
+    vjp(st, fx, a, F, fbx, sc, sbar);
+    vjp(st, fy, a, F, fby, sc, sbar);
+  } else {
+    FnFlux<0> fx{kind, P.recon, P.central_y};
+    FnFlux<1> fy{kind, P.recon, P.central_y};
+    vjp(st, fx, a, F, fbx, sc, sbar);
+    vjp(st, fy, a, F, fby, sc, sbar);
+  }
 }
 void adj_mag_rhs(hipStream_t st, const AD& a, const AdjScal& sc, const double* u, const double* Qe, const double* pe,
                  const double* Qs, const double* bbar, double* ubar, double* Qebar, double* pebar, double* Qsbar,

@@ -95,10 +95,12 @@ def interior_direction(rng, shape_u, shape_f):
     return du, dbx, dby
 
 
-def fd_check(n, steps, every, ndir, seed, eps=1e-6, scale=1e-2, m=None, time_stop=None):
+def fd_check(n, steps, every, ndir, seed, eps=1e-6, scale=1e-2, m=None, time_stop=None, setup=None):
     p, u, bx, by = vortex(n, m)
     if time_stop is not None:
         p.time_stop = time_stop
+    if setup is not None:
+        setup(p)
     ut = torch.tensor(u, requires_grad=True)
     bxt = torch.tensor(bx, requires_grad=True)
     byt = torch.tensor(by, requires_grad=True)
@@ -151,6 +153,17 @@ def test_adjoint_one_step_uneven_grid():
     print("[adjoint] vortex 32x20, 1 step", flush=True)
     errs = fd_check(32, 1, 1, 3, seed=5, m=20)
     assert errs.max() <= 1e-5, errs
+
+
+def test_adjoint_weno5():
+    """WENO5 reconstruction (reconstruction.cpp:184-266): the flux element's stencil is six cells."""
+    print("[adjoint] vortex 24x24 WENO5, 3 steps, checkpoint every 2", flush=True)
+
+    def weno5(p):
+        p.recon = im.RECON_WENO5
+    errs = fd_check(24, 3, 2, 4, seed=21, setup=weno5)
+    assert np.median(errs) <= 1e-5, errs
+    assert errs.max() <= 1e-4, errs
 
 
 def test_adjoint_last_step_clipped_to_time_stop():
