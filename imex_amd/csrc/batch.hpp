@@ -87,6 +87,7 @@ enum Shape {
   SH_INT64x4,     // (nx, ny) in 64x4 tiles
   SH_EXPL,        // (nx, ny) in explicit tiles (16x16, 32x8 or 64x4, autotuned)
   SH_PAD64x4,     // (nx+6, ny+6) in 64x4 tiles
+  SH_PAD64x8,     // (nx+6, ny+6) in 64x8 tiles
   SH_INT2_64x4,   // (nx+2, ny+2) in 64x4 tiles
   SH_LIN_TOT,     // (nx+6)(ny+6) cells, 256 per block
   SH_LIN_N,       // nx ny cells, 256 per block

@@ -333,6 +333,7 @@ static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& n
       gx = (int)cdiv(d.nx, tx); nb = gx * (int)cdiv(d.ny, ty); break;
     }
     case SH_PAD64x4: gx = (int)cdiv(d.NX, 64); nb = gx * (int)cdiv(d.NY, 4); break;
+    case SH_PAD64x8: gx = (int)cdiv(d.NX, 64); nb = gx * (int)cdiv(d.NY, 8); break;
     case SH_INT2_64x4: gx = (int)cdiv(d.nx + 2, 64); nb = gx * (int)cdiv(d.ny + 2, 4); break;
     case SH_LIN_TOT: nb = (int)cdiv(tot, 256); gx = nb; break;
     case SH_LIN_N: nb = (int)cdiv(d.N, 256); gx = nb; break;

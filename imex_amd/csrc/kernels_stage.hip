@@ -829,17 +829,21 @@ __global__ void k_mag_writeback(const Inst* __restrict__ IT, BMap bm, int iu) {
 
 // CollectMagFluxes (ct_flux.cpp:5-130): Fx at x faces (i+1/2, j), Fy at y faces (i, j+1/2),
 // evaluated at the ghost-map image of every padded index (= compute on [1, n+2g-1) then bound_cond_array).
-// The ten input planes (rho, rho v_x, rho v_y and B~ of u; B of Qe; p(Qe)) of a 64x4 tile with a
-// one-cell frame are staged in LDS; positions whose ghost-map image is another cell (the ghost
-// frame) read their stencil from global memory.  The formula is written once for both readers.
+// The ten input planes (rho, rho v_x, rho v_y and B~ of u; B of Qe; p(Qe)) of a 64x8 tile with a
+// one-cell frame are staged in LDS (each of the 64x4 threads forms two positions, rows ty and
+// ty + 4); positions whose ghost-map image is another cell (the ghost frame) read their stencil
+// from global memory.  The formula is written once for both readers.
 #define EMF_NF 10
-struct EmfSmem { double v[EMF_NF][6][66]; };
+#define EMF_TY 8
+struct EmfSmem { double v[EMF_NF][EMF_TY + 2][66]; };
 // LDS plane of each input: u slots 0, 1, 2, 5, 6, 7, Qe slots 5, 6, 7, p(Qe)
 __device__ __forceinline__ int emf_plane_u(int k) { return k < 3 ? k : k - 2; }
 template <class RU, class RQ, class RP>
 __device__ __forceinline__ void emf_fluxes(const RU& U, const RQ& Q, const RP& PE, int i, int j, double h, double dx, double dy,
                                            double nu_x, double nu_y, double& fx, double& fy) {
   double rho_p0 = 0.5 * (U(i, j, 0) + U(i + 1, j, 0)), rho_0p = 0.5 * (U(i, j, 0) + U(i, j + 1, 0));
+  // one division per face density; the quotients below multiply by its reciprocal
+  const double irp0 = 1.0 / rho_p0, ir0p = 1.0 / rho_0p;
   double rvx_p0 = 0.5 * (U(i, j, 1) + U(i + 1, j, 1)), rvy_p0 = 0.5 * (U(i, j, 2) + U(i + 1, j, 2));
   double rvx_0p = 0.5 * (U(i, j, 1) + U(i, j + 1, 1)), rvy_0p = 0.5 * (U(i, j, 2) + U(i, j + 1, 2));
   double p00 = PE(i, j);
@@ -862,15 +866,15 @@ __device__ __forceinline__ void emf_fluxes(const RU& U, const RQ& Q, const RP& P
   }
   auto PM = [&](int a, int b) { return 0.5 * (Q(a, b, 5) * U(a, b, 5) + Q(a, b, 6) * U(a, b, 6) + Q(a, b, 7) * U(a, b, 7)); };
   auto PC = [&](const double* nn, const double* bb, int mm) { return 0.5 * (nn[0] * bb[0] + nn[1] * bb[1] + nn[2] * bb[2]) - nn[mm] * bb[mm]; };
-  double Fn_x = Byn_p0 / rho_p0 * (rvx_p0 - (h / dx) * (PE(i + 1, j) - p00)) - Bxn_p0 / rho_p0 * (rvy_p0 - (h / dy) * (p_pp - p_pm));
-  double By_1 = Byn_p0 / rho_p0 * (-1.0 / dx * ((PM(i + 1, j) - Q(i + 1, j, 5) * U(i + 1, j, 5)) - (PM(i, j) - Q(i, j, 5) * U(i, j, 5))) + 1.0 / dy * (n_pp[0] * b_pp[1] - n_pm[0] * b_pm[1]));
-  double By_2 = Bxn_p0 / rho_p0 * (-1.0 / dx * (Q(i + 1, j, 5) * U(i + 1, j, 6) - Q(i, j, 5) * U(i, j, 6)) + 1.0 / dy * (PC(n_pp, b_pp, 1) - PC(n_pm, b_pm, 1)));
-  double Fdiff_x = -nu_x * (U(i + 1, j, 6) - U(i, j, 6)) / dx;
+  double Fn_x = Byn_p0 * irp0 * (rvx_p0 - (h / dx) * (PE(i + 1, j) - p00)) - Bxn_p0 * irp0 * (rvy_p0 - (h / dy) * (p_pp - p_pm));
+  double By_1 = Byn_p0 * irp0 * (-1.0 / dx * ((PM(i + 1, j) - Q(i + 1, j, 5) * U(i + 1, j, 5)) - (PM(i, j) - Q(i, j, 5) * U(i, j, 5))) + 1.0 / dy * (n_pp[0] * b_pp[1] - n_pm[0] * b_pm[1]));
+  double By_2 = Bxn_p0 * irp0 * (-1.0 / dx * (Q(i + 1, j, 5) * U(i + 1, j, 6) - Q(i, j, 5) * U(i, j, 6)) + 1.0 / dy * (PC(n_pp, b_pp, 1) - PC(n_pm, b_pm, 1)));
+  double Fdiff_x = -nu_x * (U(i + 1, j, 6) - U(i, j, 6)) * (1.0 / dx);
   fx = Fn_x + h * (By_1 + By_2) + Fdiff_x;
-  double Fn_y = Bxn_0p / rho_0p * (rvy_0p - (h / dy) * (PE(i, j + 1) - p00)) - Byn_0p / rho_0p * (rvx_0p - (h / dx) * (p_pp - p_mp));
-  double Bx_1 = Bxn_0p / rho_0p * (1.0 / dx * (n_pp[0] * b_pp[1] - n_mp[0] * b_mp[1]) - 1.0 / dy * ((PM(i, j + 1) - Q(i, j + 1, 6) * U(i, j + 1, 6)) - (PM(i, j) - Q(i, j, 6) * U(i, j, 6))));
-  double Bx_2 = Byn_0p / rho_0p * (1.0 / dx * (PC(n_pp, b_pp, 0) - PC(n_mp, b_mp, 0)) - 1.0 / dy * (Q(i, j + 1, 6) * U(i, j + 1, 5) - Q(i, j, 6) * U(i, j, 5)));
-  double Fdiff_y = -nu_y * (U(i, j + 1, 5) - U(i, j, 5)) / dy;
+  double Fn_y = Bxn_0p * ir0p * (rvy_0p - (h / dy) * (PE(i, j + 1) - p00)) - Byn_0p * ir0p * (rvx_0p - (h / dx) * (p_pp - p_mp));
+  double Bx_1 = Bxn_0p * ir0p * (1.0 / dx * (n_pp[0] * b_pp[1] - n_mp[0] * b_mp[1]) - 1.0 / dy * ((PM(i, j + 1) - Q(i, j + 1, 6) * U(i, j + 1, 6)) - (PM(i, j) - Q(i, j, 6) * U(i, j, 6))));
+  double Bx_2 = Byn_0p * ir0p * (1.0 / dx * (PC(n_pp, b_pp, 0) - PC(n_mp, b_mp, 0)) - 1.0 / dy * (Q(i, j + 1, 6) * U(i, j + 1, 5) - Q(i, j, 6) * U(i, j, 5)));
+  double Fdiff_y = -nu_y * (U(i, j + 1, 5) - U(i, j, 5)) * (1.0 / dy);
   fy = Fn_y + h * (Bx_1 + Bx_2) + Fdiff_y;
 }
 
@@ -886,45 +890,59 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const double* S = gmem(I_.S);
   __shared__ EmfSmem sm;
 
-  // stage the tile [i0 - 1, i0 + 64] x [j0 - 1, j0 + 4] (clamped to the padded grid; clamped
+  // stage the tile [i0 - 1, i0 + 64] x [j0 - 1, j0 + EMF_TY] (clamped to the padded grid; clamped
   // entries are read only by positions that take the global path)
-  const int i0 = bx_ * 64, j0 = by_ * 4;
+  const int i0 = bx_ * 64, j0 = by_ * EMF_TY;
   const int t = threadIdx.x + 64 * threadIdx.y;
-  for (int e = t; e < 66 * 6; e += 256) {
+  // every load of the thread's (up to three) staged positions is issued before the first LDS store
+  constexpr int NSTG = 66 * (EMF_TY + 2), NPT = (NSTG + 255) / 256;
+  double v[NPT][EMF_NF];
+#pragma unroll
+  for (int q = 0; q < NPT; q++) {
+    const int e = t + 256 * q;
+    if (e >= NSTG) break;
     const int lx = e % 66, ly = e / 66;
     const int a = min(max(i0 + lx - 1, 0), d.NX - 1), b = min(max(j0 + ly - 1, 0), d.NY - 1);
     const long long c = pidx(d, a, b);
-    double v[EMF_NF];
-    v[0] = u[c]; v[1] = u[d.PS + c]; v[2] = u[2 * d.PS + c];
-    v[3] = u[5 * d.PS + c]; v[4] = u[6 * d.PS + c]; v[5] = u[7 * d.PS + c];
-    v[6] = Qe[5 * d.PS + c]; v[7] = Qe[6 * d.PS + c]; v[8] = Qe[7 * d.PS + c];
-    v[9] = pe[c];
+    v[q][0] = u[c]; v[q][1] = u[d.PS + c]; v[q][2] = u[2 * d.PS + c];
+    v[q][3] = u[5 * d.PS + c]; v[q][4] = u[6 * d.PS + c]; v[q][5] = u[7 * d.PS + c];
+    v[q][6] = Qe[5 * d.PS + c]; v[q][7] = Qe[6 * d.PS + c]; v[q][8] = Qe[7 * d.PS + c];
+    v[q][9] = pe[c];
+  }
 #pragma unroll
-    for (int f = 0; f < EMF_NF; f++) sm.v[f][ly][lx] = v[f];
+  for (int q = 0; q < NPT; q++) {
+    const int e = t + 256 * q;
+    if (e >= NSTG) break;
+    const int lx = e % 66, ly = e / 66;
+#pragma unroll
+    for (int f = 0; f < EMF_NF; f++) sm.v[f][ly][lx] = v[q][f];
   }
   __syncthreads();
 
-  int ii = i0 + threadIdx.x, jj = j0 + threadIdx.y;
-  if (ii >= d.NX || jj >= d.NY) return;
-  const int i = map_x(d, ii), j = map_y(d, jj);
   const double h = S[S_ADT], dx = d.dx, dy = d.dy;
   const double nu_x = S[S_BMAXX] * dx, nu_y = S[S_BMAXY] * dy;
-  double fx, fy;
-  if (i == ii && j == jj && i >= 1 && j >= 1 && i < d.NX - 1 && j < d.NY - 1) {
-    const int ox = i0 - 1, oy = j0 - 1;
-    auto U = [&](int a, int b, int k) { return sm.v[emf_plane_u(k)][b - oy][a - ox]; };
-    auto Q = [&](int a, int b, int k) { return sm.v[k + 1][b - oy][a - ox]; };
-    auto PE = [&](int a, int b) { return sm.v[9][b - oy][a - ox]; };
-    emf_fluxes(U, Q, PE, i, j, h, dx, dy, nu_x, nu_y, fx, fy);
-  } else {
-    auto U = [&](int a, int b, int k) { return u[k * d.PS + pidx(d, a, b)]; };
-    auto Q = [&](int a, int b, int k) { return Qe[k * d.PS + pidx(d, a, b)]; };
-    auto PE = [&](int a, int b) { return pe[pidx(d, a, b)]; };
-    emf_fluxes(U, Q, PE, i, j, h, dx, dy, nu_x, nu_y, fx, fy);
+#pragma unroll 1
+  for (int r = 0; r < EMF_TY; r += 4) {
+    const int ii = i0 + threadIdx.x, jj = j0 + threadIdx.y + r;
+    if (ii >= d.NX || jj >= d.NY) break;
+    const int i = map_x(d, ii), j = map_y(d, jj);
+    double fx, fy;
+    if (i == ii && j == jj) {
+      const int ox = i0 - 1, oy = j0 - 1;
+      auto U = [&](int a, int b, int k) { return sm.v[emf_plane_u(k)][b - oy][a - ox]; };
+      auto Q = [&](int a, int b, int k) { return sm.v[k + 1][b - oy][a - ox]; };
+      auto PE = [&](int a, int b) { return sm.v[9][b - oy][a - ox]; };
+      emf_fluxes(U, Q, PE, i, j, h, dx, dy, nu_x, nu_y, fx, fy);
+    } else {
+      auto U = [&](int a, int b, int k) { return u[k * d.PS + pidx(d, a, b)]; };
+      auto Q = [&](int a, int b, int k) { return Qe[k * d.PS + pidx(d, a, b)]; };
+      auto PE = [&](int a, int b) { return pe[pidx(d, a, b)]; };
+      emf_fluxes(U, Q, PE, i, j, h, dx, dy, nu_x, nu_y, fx, fy);
+    }
+    const double rf = refl_factor(d, ii, jj, true);      // bound_cond_array of the flux arrays: reflective negation
+    Fx[pidx(d, ii, jj)] = rf * fx;
+    Fy[pidx(d, ii, jj)] = rf * fy;
   }
-  const double rf = refl_factor(d, ii, jj, true);      // bound_cond_array of the flux arrays: reflective negation
-  Fx[pidx(d, ii, jj)] = rf * fx;
-  Fy[pidx(d, ii, jj)] = rf * fy;
 }
 
 // ================================================================ pressure subsystem pointwise/5-point kernels
@@ -1578,7 +1596,7 @@ void launch_mag_writeback(const Launch& L, int iu) {
   k_mag_writeback<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_TOT], iu);
 }
 void launch_emf_flux(const Launch& L, int iu, int iQe) {
-  k_emf_flux<<<GRID(L, SH_PAD64x4), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x4], iu, iQe);
+  k_emf_flux<<<GRID(L, SH_PAD64x8), B64x4, 0, L.st>>>(L.IT, L.bm[SH_PAD64x8], iu, iQe);
 }
 void launch_p_rhs(const Launch& L, const Phys& P, int pass, int iu) {
   k_p_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], pass, iu);
