@@ -1157,7 +1157,10 @@ __device__ __forceinline__ double ez_tilde(const Dims& d, const double* u, int i
   double r = u[c];
   return u[2 * d.PS + c] / r * u[5 * d.PS + c] - u[1 * d.PS + c] / r * u[6 * d.PS + c];
 }
-__global__ void k_ct_corner(const Inst* __restrict__ IT, BMap bm, int iex, int iey, double sx, double sy, int iu) {
+// Corner EMFs of a 64x4 tile: the cell EMFs Ez~ (each formed once, not once per corner that uses
+// it) and the face EMFs of the tile's 65x5 cells are staged in LDS; the ghost frame's corners read
+// their averaged term at the ghost-map image from global memory.
+__global__ void __launch_bounds__(256) k_ct_corner(const Inst* __restrict__ IT, BMap bm, int iex, int iey, double sx, double sy, int iu) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
@@ -1165,21 +1168,51 @@ __global__ void k_ct_corner(const Inst* __restrict__ IT, BMap bm, int iex, int i
   const double* __restrict__ ezy = gmem(I_.buf[iey]);
   const double* __restrict__ u = gmem(I_.buf[iu]);
   double* __restrict__ Ezpp = gmem(I_.buf[B_EZPP]);
+  __shared__ double se[3][5][65];     // Ez~, sx ezx, sy ezy
 
   // Ez_p0 = sx * ezx, Ez_0p = sy * ezy (sx = -1, sy = +1 for the EMF-consistent form)
-  int i = bx_ * blockDim.x + threadIdx.x + GC - 2, j = by_ * blockDim.y + threadIdx.y + GC - 2;
+  const int i0 = bx_ * 64 + GC - 2, j0 = by_ * 4 + GC - 2;
+  const int t = threadIdx.x + 64 * threadIdx.y;
+  double v[2][7];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = t + 256 * q;
+    if (e >= 65 * 5) break;
+    const int a = min(i0 + e % 65, d.NX - 1), b = min(j0 + e / 65, d.NY - 1);
+    const long long c = pidx(d, a, b);
+    v[q][0] = u[c]; v[q][1] = u[d.PS + c]; v[q][2] = u[2 * d.PS + c]; v[q][3] = u[5 * d.PS + c]; v[q][4] = u[6 * d.PS + c];
+    v[q][5] = ezx[c]; v[q][6] = ezy[c];
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = t + 256 * q;
+    if (e >= 65 * 5) break;
+    const int lx = e % 65, ly = e / 65;
+    const double r = v[q][0];
+    se[0][ly][lx] = v[q][2] / r * v[q][3] - v[q][1] / r * v[q][4];       // ez_tilde
+    se[1][ly][lx] = sx * v[q][5];
+    se[2][ly][lx] = sy * v[q][6];
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   if (i >= d.nx + GC || j >= d.ny + GC) return;
+  const int lx = threadIdx.x, ly = threadIdx.y;
   auto EX = [&](int a, int b) { return sx * ezx[pidx(d, a, b)]; };
   auto EY = [&](int a, int b) { return sy * ezy[pidx(d, a, b)]; };
   int I = map_x(d, i), J = map_y(d, j);
   // Ez_bar is formed inside and filled by bound_cond_array (constrained_transport.cpp:148-154)
-  double bar = refl_factor(d, i, j, true) * (0.25 * (EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J)));
+  const double sum = (I == i && J == j) ? se[1][ly][lx] + se[1][ly + 1][lx] + se[2][ly][lx] + se[2][ly][lx + 1]
+                                        : EX(I, J) + EX(I, J + 1) + EY(I, J) + EY(I + 1, J);
+  double bar = refl_factor(d, i, j, true) * (0.25 * sum);
   const double dx = d.dx, dy = d.dy;
-  double e00 = ez_tilde(d, u, i, j), e10 = ez_tilde(d, u, i + 1, j), e01 = ez_tilde(d, u, i, j + 1), e11 = ez_tilde(d, u, i + 1, j + 1);
-  double t1 = 0.5 * ((EX(i, j) - e00) / (dx / 2.0) + (EX(i, j + 1) - e01) / (dx / 2.0));
-  double t2 = 0.5 * ((e10 - EX(i, j)) / (dx / 2.0) + (e11 - EX(i, j + 1)) / (dx / 2.0));
-  double t3 = 0.5 * ((EY(i, j) - e00) / (dy / 2.0) + (EY(i + 1, j) - e10) / (dy / 2.0));
-  double t4 = 0.5 * ((e01 - EY(i, j)) / (dy / 2.0) + (e11 - EY(i + 1, j)) / (dy / 2.0));
+  const double e00 = se[0][ly][lx], e10 = se[0][ly][lx + 1], e01 = se[0][ly + 1][lx], e11 = se[0][ly + 1][lx + 1];
+  const double x00 = se[1][ly][lx], x01 = se[1][ly + 1][lx], y00 = se[2][ly][lx], y10 = se[2][ly][lx + 1];
+  // the differences over half a cell multiply by the reciprocals of dx/2, dy/2 (one division each per block)
+  const double rhx = 1.0 / (dx / 2.0), rhy = 1.0 / (dy / 2.0);
+  double t1 = 0.5 * ((x00 - e00) * rhx + (x01 - e01) * rhx);
+  double t2 = 0.5 * ((e10 - x00) * rhx + (e11 - x01) * rhx);
+  double t3 = 0.5 * ((y00 - e00) * rhy + (y10 - e10) * rhy);
+  double t4 = 0.5 * ((e01 - y00) * rhy + (e11 - y10) * rhy);
   Ezpp[pidx(d, i, j)] = bar + dx / 8.0 * (t1 - t2) + dy / 8.0 * (t3 - t4);
 }
 
@@ -1195,7 +1228,10 @@ __device__ __forceinline__ int smap_y(const Dims& d, int axis, int j) {
   int lo = axis == 1 ? GC - 1 : GC;
   return j < lo ? lo : (j > d.ny + GC - 1 ? d.ny + GC - 1 : j);
 }
-__global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int ibyqs, int ibx, int iby, int iu) {
+// The new face values of the 64x4 tile (and of the faces just below and left of it) are kept in
+// LDS, so each is formed once; the cell average of an interior cell reads them there, the ghost
+// frame's cells form theirs at the ghost-map image.
+__global__ void __launch_bounds__(256) k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int ibyqs, int ibx, int iby, int iu) {
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
@@ -1206,20 +1242,39 @@ __global__ void k_ct_faces(const Inst* __restrict__ IT, BMap bm, int ibxqs, int 
   double* __restrict__ by = gmem(I_.buf[iby]);
   double* __restrict__ u = gmem(I_.buf[iu]);
   const double* S = gmem(I_.S);
+  __shared__ double sbx[4][65], sby[5][64];   // BX at faces i0-1 .. i0+63, BY at faces j0-1 .. j0+3
 
-  int i = bx_ * blockDim.x + threadIdx.x, j = by_ * blockDim.y + threadIdx.y;
-  if (i >= d.NX || j >= d.NY) return;
+  const int i0 = bx_ * blockDim.x, j0 = by_ * blockDim.y;
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const double h = S[S_ADT];
   auto BX = [&](int a, int b) { int I = smap_x(d, 0, a), J = smap_y(d, 0, b); return bxqs[pidx(d, I, J)] - h / d.dy * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I, J - 1)]); };
   auto BY = [&](int a, int b) { int I = smap_x(d, 1, a), J = smap_y(d, 1, b); return byqs[pidx(d, I, J)] + h / d.dx * (Ezpp[pidx(d, I, J)] - Ezpp[pidx(d, I - 1, J)]); };
-  long long c = pidx(d, i, j);
-  // the jet inflow strip's ghost faces and cells (constrained_transport.cpp:50-63, boundary.cpp:148-172)
-  bx[c] = (j < GC && i < d.NX - 1 && jet_inlet_face(d, i)) ? 0.0 : BX(i, j);
-  by[c] = (j < GC - 1 && jet_inlet(d, i)) ? d.jet_B0 : BY(i, j);
+  const bool in = i < d.NX && j < d.NY;
+  if (in) {
+    const double fbx = BX(i, j), fby = BY(i, j);
+    sbx[threadIdx.y][threadIdx.x + 1] = fbx;
+    sby[threadIdx.y + 1][threadIdx.x] = fby;
+    const long long c = pidx(d, i, j);
+    // the jet inflow strip's ghost faces and cells (constrained_transport.cpp:50-63, boundary.cpp:148-172)
+    bx[c] = (j < GC && i < d.NX - 1 && jet_inlet_face(d, i)) ? 0.0 : fbx;
+    by[c] = (j < GC - 1 && jet_inlet(d, i)) ? d.jet_B0 : fby;
+    if (threadIdx.x == 0 && i > 0) sbx[threadIdx.y][0] = BX(i - 1, j);
+    if (threadIdx.y == 0 && j > 0) sby[0][threadIdx.x] = BY(i, j - 1);
+  }
+  __syncthreads();
+  if (!in) return;
+  const long long c = pidx(d, i, j);
   if (j < GC && jet_inlet(d, i)) { u[5 * d.PS + c] = 0.0; u[6 * d.PS + c] = d.jet_B0; return; }
   int I = map_x(d, i), J = map_y(d, j);
   const double rf = refl_factor(d, i, j, true);        // bound_cond_u: B negated across a reflective wall
-  const double b5 = rf * (0.5 * (BX(I - 1, J) + BX(I, J))), b6 = rf * (0.5 * (BY(I, J - 1) + BY(I, J)));
+  double b5, b6;
+  if (I == i && J == j) {
+    b5 = rf * (0.5 * (sbx[threadIdx.y][threadIdx.x] + sbx[threadIdx.y][threadIdx.x + 1]));
+    b6 = rf * (0.5 * (sby[threadIdx.y][threadIdx.x] + sby[threadIdx.y + 1][threadIdx.x]));
+  } else {
+    b5 = rf * (0.5 * (BX(I - 1, J) + BX(I, J)));
+    b6 = rf * (0.5 * (BY(I, J - 1) + BY(I, J)));
+  }
   u[5 * d.PS + c] = b5;
   u[6 * d.PS + c] = b6;
   if (dir_ghost(d, i, j)) {                             // the Dirichlet ambient's E follows its B
