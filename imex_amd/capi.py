@@ -78,7 +78,7 @@ class KernelProfile(C.Structure):
 
 class TuneInfo(C.Structure):
     _fields_ = [("explicit_tile_x", C.c_int), ("explicit_tile_y", C.c_int), ("krylov_blocks", C.c_int),
-                ("explicit_us", C.c_double * 3), ("krylov_us", C.c_double * 3),
+                ("explicit_us", C.c_double * 5), ("krylov_us", C.c_double * 3),
                 ("krylov_p_blocks", C.c_int), ("krylov_p_us", C.c_double * 4)]
 
 

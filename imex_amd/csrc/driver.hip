@@ -183,7 +183,7 @@ struct imex_ctx {
   // keeps resident avoids a partly filled last round (magnetic FP64 apply: 3 waves/SIMD, 768
   // blocks; pressure FP64: 6, 1536; pressure FP32 apply: 8, 2048)
   int cap_tiles = 2048, cap_tiles_p = 2048, cap_vec = 2048;
-  int ex_tile = 32;                       // explicit-kernel tile width (16, 32 or 64; autotuned)
+  int ex_tile = 32, ex_rows = 8;          // explicit-kernel tile (16x16, 32x8, 64x4, 32x16 or 64x8; autotuned)
   // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo;
   // faces likewise (bxn/byn read, bx2/by2 written)
   int iUs = B_U2, iUo = B_U3, iBxn = B_BX0, iByn = B_BY0, iBx2 = B_BX2, iBy2 = B_BY2;
@@ -321,6 +321,10 @@ static int dalloc(imex_ctx* c, T** out, size_t count) {
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
+// explicit-kernel tile candidates of the autotuner (cells: width, rows), in imex_tune_info order
+constexpr int NEXT = 5;
+static const int EX_TILES[NEXT][2] = {{16, 16}, {32, 8}, {64, 4}, {32, 16}, {64, 8}};
+
 // Per-instance grid of a launch shape: (blocks per row, blocks).
 static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& nb) {
   long long tiles = cdiv(d.nx, TX) * cdiv(d.ny, TY);
@@ -329,7 +333,7 @@ static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& n
   switch (sh) {
     case SH_INT64x4: gx = (int)cdiv(d.nx, 64); nb = gx * (int)cdiv(d.ny, 4); break;
     case SH_EXPL: {
-      const int tx = c->ex_tile, ty = 256 / tx;
+      const int tx = c->ex_tile, ty = c->ex_rows;
       gx = (int)cdiv(d.nx, tx); nb = gx * (int)cdiv(d.ny, ty); break;
     }
     case SH_PAD64x4: gx = (int)cdiv(d.NX, 64); nb = gx * (int)cdiv(d.NY, 4); break;
@@ -752,7 +756,7 @@ static void ensure_qe(imex_ctx* c) {
 }
 
 static int sub_explicit_conv(imex_ctx* c) {
-  LAUNCH(KC_EXPL_CONV, 13, launch_explicit(c->L, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
+  LAUNCH(KC_EXPL_CONV, 13, launch_explicit(c->L, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile, c->ex_rows));
   const int u = c->cur.u;
   fill_planes(c, {{u, 0}, {u, 1}, {u, 2}, {u, 3}});
   c->hhat_ready = false;
@@ -801,7 +805,7 @@ static int sub_unstaggered_ct(imex_ctx* c, int ia01) {
 
 static int sub_explicit_press(imex_ctx* c) {
   // algorithmic planes: Qe 0..4 (all eight with PrimState), Qs 1..4, the cell B, u 1..4 written
-  LAUNCH(KC_EXPL_PRESS, c->P.prim ? 19 : 16, launch_explicit(c->L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
+  LAUNCH(KC_EXPL_PRESS, c->P.prim ? 19 : 16, launch_explicit(c->L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile, c->ex_rows));
   const int u = c->cur.u;
   fill_planes(c, {{u, 1}, {u, 2}, {u, 3}, {u, 4}});
   return 0;
@@ -843,7 +847,7 @@ static int sub_pressure(imex_ctx* c) {
 
 // B slots <- cell B~, then Energy_flux_Update (main.cpp:448-474, flux.cpp:165-200)
 static int sub_explicit_energy(imex_ctx* c) {
-  LAUNCH(KC_EXPL_ENERGY, c->P.prim ? 19 : 16, launch_explicit(c->L, c->P, 2, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile));
+  LAUNCH(KC_EXPL_ENERGY, c->P.prim ? 19 : 16, launch_explicit(c->L, c->P, 2, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile, c->ex_rows));
   const int u = c->cur.u;
   fill_planes(c, {{u, 1}, {u, 2}, {u, 3}, {u, 4}});
   return 0;
@@ -1701,10 +1705,13 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
   if (hipMemcpy(c->IT, c->hI.data(), (size_t)n * sizeof(Inst), hipMemcpyHostToDevice) != hipSuccess) return fail(set_err(IMEX_ERR_HIP, "table"));
   if ((rc = dalloc(c, &c->bmap_dev, (size_t)NSHAPE * (2 * n + 1)))) return fail(rc);
   if (n > 1) {          // block table sized for every launch shape the autotuner may choose
-    const int ct = c->cap_tiles, ctp = c->cap_tiles_p, cv = c->cap_vec, et = c->ex_tile;
+    const int ct = c->cap_tiles, ctp = c->cap_tiles_p, cv = c->cap_vec, et = c->ex_tile, er = c->ex_rows;
     c->cap_tiles = c->cap_tiles_p = c->cap_vec = 4096;
-    for (int t : {16, 32, 64}) { c->ex_tile = t; c->blk_cap = std::max(c->blk_cap, map_blocks(c)); }
-    c->cap_tiles = ct; c->cap_tiles_p = ctp; c->cap_vec = cv; c->ex_tile = et;
+    for (int q = 0; q < NEXT; q++) {
+      c->ex_tile = EX_TILES[q][0]; c->ex_rows = EX_TILES[q][1];
+      c->blk_cap = std::max(c->blk_cap, map_blocks(c));
+    }
+    c->cap_tiles = ct; c->cap_tiles_p = ctp; c->cap_vec = cv; c->ex_tile = et; c->ex_rows = er;
     if ((rc = dalloc(c, &c->blk_dev, (size_t)c->blk_cap))) return fail(rc);
   }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { c->st = nullptr; return fail(set_err(IMEX_ERR_HIP, "stream")); }
@@ -2177,7 +2184,6 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
   bind_stage0(c);
   c->qe_ready = false;
   ensure_qe(c);
-  const int tiles[3] = {16, 32, 64};
   const int caps[3] = {1024, 2048, 4096};
   // Each candidate is timed in ROUNDS interleaved rounds and keeps its fastest, so that a slow
   // moment of the device does not decide between candidates a few percent apart.
@@ -2186,16 +2192,16 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
   if (flags & IMEX_TUNE_EXPLICIT) {
     int best = 1;
     for (int rd = 0; rd < ROUNDS && !rc; rd++)
-      for (int q = 0; q < 3 && !rc; q++) {
-        c->ex_tile = tiles[q];
+      for (int q = 0; q < NEXT && !rc; q++) {
+        c->ex_tile = EX_TILES[q][0]; c->ex_rows = EX_TILES[q][1];
         if ((rc = build_maps(c))) break;
         keep_min(info.explicit_us[q], timed(5, [&] {
-          launch_explicit(L, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile);
-          launch_explicit(L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile);
+          launch_explicit(L, c->P, 0, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile, c->ex_rows);
+          launch_explicit(L, c->P, 1, c->cur.Qe, c->cur.Qs, c->cur.u, c->mixed_flux, c->ex_tile, c->ex_rows);
         }), rd);
       }
-    for (int q = 0; q < 3; q++) if (info.explicit_us[q] < info.explicit_us[best]) best = q;
-    c->ex_tile = tiles[best];
+    for (int q = 0; q < NEXT; q++) if (info.explicit_us[q] < info.explicit_us[best]) best = q;
+    c->ex_tile = EX_TILES[best][0]; c->ex_rows = EX_TILES[best][1];
   }
   if (!rc && (flags & IMEX_TUNE_KRYLOV)) {
     const int m = std::min(std::max(c->prm.gmres_restart, 1), MAXM);
@@ -2241,7 +2247,7 @@ int imex_autotune(imex_ctx* c, int flags, imex_tune_info* out) {
   if (rc) return rc;
   HIPCHK(hipGetLastError());
   info.explicit_tile_x = c->ex_tile;
-  info.explicit_tile_y = 256 / c->ex_tile;
+  info.explicit_tile_y = c->ex_rows;
   info.krylov_blocks = c->cap_tiles;
   info.krylov_p_blocks = c->cap_tiles_p;
   if (out) *out = info;

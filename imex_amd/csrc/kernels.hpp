@@ -128,7 +128,7 @@ void launch_stage_combine(const Launch& L, int ibxn, int ibyn);
 void launch_qe_prep(const Launch& L, const Phys& P, int iQe);
 void launch_shock(const Launch& L, const Phys& P, int iQe);
 // tile: width of the explicit tile (16, 32 or 64 cells; 256 cells per tile); SH_EXPL is mapped for it
-void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile);
+void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile, int rows);
 void launch_mag_rhs(const Launch& L, const Phys& P, int iu, int iQe, int iQs, int ib, int ix0);
 void launch_mag_writeback(const Launch& L, int iu);
 void launch_emf_flux(const Launch& L, int iu, int iQe);

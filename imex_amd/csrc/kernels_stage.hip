@@ -423,6 +423,8 @@ __device__ __forceinline__ void recon5(int recon, int central, const T* a, T& wL
 // normal-momentum and v_n B^2/2 energy parts, and -(gamma - 1) |m|^2 / (2 rho) on the normal momentum.
 template <class T>
 __device__ __forceinline__ void gl_flux(int dir, int kind, T gam, const T* QL, const T* QR, T s_max, T* F) {
+  // Multiply-adds are fused (a smooth function of the face states: no branch depends on its rounding).
+#pragma clang fp contract(fast)
   T fl[5], fr[5];
   const T* Qs2[2] = {QL, QR};
   T* fo[2] = {fl, fr};
@@ -480,10 +482,12 @@ __device__ __forceinline__ void gl_flux(int dir, int kind, T gam, const T* QL, c
 // the reconstruction at the ghost-map image of c, as the reference's bound_cond_u on w_*
 // produces.  kind 0 writes u[0..3] (the energy slot of the convective predictor is never
 // read); kinds 1 and 2 write u[1..4] (their density equals the convective one bit for bit).
-// The tile is EX_TX x (256 / EX_TX) cells (EX_TX = 16, 32 or 64, chosen by the autotuner); the
-// tile shape changes which lane forms which face value, not the value.
-template <int EX_TX> struct ExTile {
-  static constexpr int ROWS = 256 / EX_TX;
+// The tile is EX_TX x EX_TY cells, one thread per cell (16x16, 32x8 and 64x4 with 256 threads, 32x16
+// and 64x8 with 512, chosen by the autotuner); the tile shape changes which lane forms which face
+// value, not the value.  Larger tiles reconstruct fewer frame cells and form fewer tile-edge faces
+// twice per cell of the grid.
+template <int EX_TX, int EX_TY> struct ExTile {
+  static constexpr int ROWS = EX_TY, NT = EX_TX * EX_TY;
   static constexpr int NWX = (EX_TX + 2) * ROWS, NWY = EX_TX * (ROWS + 2);
   static constexpr int NFX = (EX_TX + 1) * ROWS, NFY = EX_TX * (ROWS + 1);
   static constexpr int NW = NWX > NWY ? NWX : NWY;     // cells reconstructed per direction, at most
@@ -554,11 +558,12 @@ __device__ __forceinline__ void prim_faces(const Phys& P, int central, const dou
 // the primitives of the five stencil cells and converts the two face states back (ReconState
 // PrimState, reconstruction.cpp:11-24, 272-281); kind 0 needs rho and v only, kinds 1 and 2 the
 // whole state, as the face energy rho e(rho, p) + kinetic + |B|^2/2 uses the reconstructed p and B.
-template <int DIR, int EX_TX, class T, bool PRIM, bool BND>
+template <int DIR, int EX_TX, int EX_TYT, class T, bool PRIM, bool BND>
 __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int kind, const double* __restrict__ Qe,
                                              const double* __restrict__ Qs, double* __restrict__ u, double h, T s_max,
                                              int i0, int j0, T* wl, T* wr, T* F, double* acc, int* eflag) {
-  constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW, EX_NF = ExTile<EX_TX>::NF;
+  constexpr int EX_TY = ExTile<EX_TX, EX_TYT>::ROWS, EX_NW = ExTile<EX_TX, EX_TYT>::NW, EX_NF = ExTile<EX_TX, EX_TYT>::NF;
+  constexpr int NT = ExTile<EX_TX, EX_TYT>::NT;
   constexpr int W = DIR == 0 ? EX_TX + 2 : EX_TX;                  // reconstructed cells per row
   constexpr int NITEMS = DIR == 0 ? (EX_TX + 2) * EX_TY : EX_TX * (EX_TY + 2);
   constexpr int NF = DIR == 0 ? (EX_TX + 1) * EX_TY : EX_TX * (EX_TY + 1);
@@ -671,11 +676,11 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
   // at most two faces per thread; their fluxes are held in registers until every face state has
   // been read, then written over the face states (F aliases wl: a smaller LDS footprint, more
   // blocks per CU)
-  static_assert(EX_NF <= EX_NW && NF <= 512, "flux tile layout");
+  static_assert(EX_NF <= EX_NW && NF <= 2 * NT, "flux tile layout");
   T Fo[2][5];
 #pragma unroll
   for (int r = 0; r < 2; r++) {
-    const int f = threadIdx.x + r * 256;
+    const int f = threadIdx.x + r * NT;
     if (f >= NF) continue;
     int fx = f % FW, fy = f / FW;
     // face between local cells (left, right) along DIR; item indices into wl/wr
@@ -698,12 +703,13 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 2; r++) {
-    const int f = threadIdx.x + r * 256;
+    const int f = threadIdx.x + r * NT;
     if (f >= NF) continue;
     for (int k = 0; k < 5; k++) F[k * EX_NF + f] = Fo[r][k];
   }
   __syncthreads();
   if (own) {
+#pragma clang fp contract(fast)
     long long c = pidx(d, ci, cj);
     if (DIR == 0) {
       int fm = ty * FW + tx, fp = fm + 1;
@@ -717,9 +723,9 @@ __device__ __forceinline__ void explicit_dir(const Dims& d, const Phys& P, int k
 
 // BND: the boundary treatments other than periodic and outflow (jet inflow, reflective, Dirichlet) are
 // compiled in; without them the kernel carries none of their code.
-template <int EX_TX, class T, bool PRIM, bool BND>
-__global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict__ IT, BMap bm, int kind, int iQe, int iQs, int iu) {
-  constexpr int EX_TY = ExTile<EX_TX>::ROWS, EX_NW = ExTile<EX_TX>::NW;
+template <int EX_TX, int EX_TYT, class T, bool PRIM, bool BND>
+__global__ void __launch_bounds__(EX_TX * EX_TYT) k_explicit(Phys P, const Inst* __restrict__ IT, BMap bm, int kind, int iQe, int iQs, int iu) {
+  constexpr int EX_TY = ExTile<EX_TX, EX_TYT>::ROWS, EX_NW = ExTile<EX_TX, EX_TYT>::NW;
   BATCH_BLOCK(bm);
   const Inst& I_ = IT[inst_];
   const Dims d = I_.d;
@@ -735,8 +741,8 @@ __global__ void __launch_bounds__(256) k_explicit(Phys P, const Inst* __restrict
   const T s_max = (T)S[S_AMAX];
   int* eflag = gmem(I_.eflag);
   double acc[5];
-  explicit_dir<0, EX_TX, T, PRIM, BND>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
-  explicit_dir<1, EX_TX, T, PRIM, BND>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
+  explicit_dir<0, EX_TX, EX_TYT, T, PRIM, BND>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
+  explicit_dir<1, EX_TX, EX_TYT, T, PRIM, BND>(d, P, kind, Qe, Qs, u, h, s_max, i0, j0, wl, wr, F, acc, eflag);
 }
 
 // ================================================================ magnetic subsystem
@@ -1626,23 +1632,27 @@ void launch_shock(const Launch& L, const Phys& P, int iQe) {
   const int eta[1] = {B_ETA};
   launch_bc_fill(L, 1, eta, k0, 0);
 }
-template <int TXE>
+template <int TXE, int TYE>
 static void launch_explicit_t(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed) {
+  constexpr int NT = TXE * TYE;
   if (P.bnd) {
-    if (P.prim) k_explicit<TXE, double, true, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-    else if (mixed) k_explicit<TXE, float, false, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-    else k_explicit<TXE, double, false, true><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    if (P.prim) k_explicit<TXE, TYE, double, true, true><<<GRID(L, SH_EXPL), NT, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else if (mixed) k_explicit<TXE, TYE, float, false, true><<<GRID(L, SH_EXPL), NT, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else k_explicit<TXE, TYE, double, false, true><<<GRID(L, SH_EXPL), NT, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
   } else {
-    if (P.prim) k_explicit<TXE, double, true, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-    else if (mixed) k_explicit<TXE, float, false, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
-    else k_explicit<TXE, double, false, false><<<GRID(L, SH_EXPL), 256, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    if (P.prim) k_explicit<TXE, TYE, double, true, false><<<GRID(L, SH_EXPL), NT, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else if (mixed) k_explicit<TXE, TYE, float, false, false><<<GRID(L, SH_EXPL), NT, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
+    else k_explicit<TXE, TYE, double, false, false><<<GRID(L, SH_EXPL), NT, 0, L.st>>>(P, L.IT, L.bm[SH_EXPL], kind, iQe, iQs, iu);
   }
 }
-// tile: the explicit tile width (16, 32 or 64 cells); the SH_EXPL block map must be built for it
-void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile) {
-  if (tile == 16) launch_explicit_t<16>(L, P, kind, iQe, iQs, iu, mixed);
-  else if (tile == 64) launch_explicit_t<64>(L, P, kind, iQe, iQs, iu, mixed);
-  else launch_explicit_t<32>(L, P, kind, iQe, iQs, iu, mixed);
+// (tile, rows): the explicit tile (16x16, 32x8, 64x4, 32x16 or 64x8 cells); the SH_EXPL block map must
+// be built for it
+void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs, int iu, int mixed, int tile, int rows) {
+  if (tile == 16) launch_explicit_t<16, 16>(L, P, kind, iQe, iQs, iu, mixed);
+  else if (tile == 64 && rows == 8) launch_explicit_t<64, 8>(L, P, kind, iQe, iQs, iu, mixed);
+  else if (tile == 64) launch_explicit_t<64, 4>(L, P, kind, iQe, iQs, iu, mixed);
+  else if (rows == 16) launch_explicit_t<32, 16>(L, P, kind, iQe, iQs, iu, mixed);
+  else launch_explicit_t<32, 8>(L, P, kind, iQe, iQs, iu, mixed);
 }
 void launch_mag_rhs(const Launch& L, const Phys& P, int iu, int iQe, int iQs, int ib, int ix0) {
   k_mag_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu, iQe, iQs, ib, ix0);

@@ -241,7 +241,7 @@ enum { IMEX_TUNE_EXPLICIT = 1, IMEX_TUNE_KRYLOV = 2 };
 typedef struct imex_tune_info {
   int explicit_tile_x, explicit_tile_y;   /* chosen explicit tile (cells) */
   int krylov_blocks;                      /* chosen Krylov grid cap (blocks per instance) */
-  double explicit_us[3];                  /* both explicit kernels, per candidate 16x16, 32x8, 64x4 */
+  double explicit_us[5];                  /* both explicit kernels, per candidate 16x16, 32x8, 64x4, 32x16, 64x8 */
   double krylov_us[3];                    /* one Arnoldi step (apply + update), per cap 1024, 2048, 4096 */
   int krylov_p_blocks;                    /* chosen grid cap of the pressure-operator kernels */
   double krylov_p_us[4];                  /* pressure Arnoldi apply, per cap 1024, 1536, 2048, 4096 */

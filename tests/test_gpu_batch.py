@@ -180,7 +180,7 @@ def test_autotune_keeps_results():
             if flags:
                 info = b.autotune(flags)
                 print(f"  flags {flags}: {info}", flush=True)
-                assert info["explicit_tile"] in ((16, 16), (32, 8), (64, 4))
+                assert info["explicit_tile"] in ((16, 16), (32, 8), (64, 4), (32, 16), (64, 8))
                 assert info["krylov_blocks"] in (1024, 2048, 4096)
                 if flags & im.TUNE_EXPLICIT:
                     assert all(t > 0 for t in info["explicit_us"])
