@@ -45,6 +45,13 @@ void adj_p_op(hipStream_t st, const AD& a, const Phys& P, const AdjScal& sc, int
               double* sbar);
 // Jacobi diagonal of the pressure operator (V' + T diag, implicit_operators.cpp:656-668)
 void adj_p_diag(hipStream_t st, const AD& a, const Phys& P, const AdjScal& sc, const double* hhat, double* diag);
+// Active pressure floor (ideal gas): dv = V_slope(p) - 1/(gamma-1) from the solution p and the solve's
+// Mbig (one device double); out += dv * in (in null: out += dv); and the Mbig path of the cotangent of b:
+// lam (the cotangent of b from the transposed solve) gets the term through M_cor(min b).
+void adj_floor_dv(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* mb, double* dv);
+void adj_diag_add(hipStream_t st, const AD& a, const double* dv, const double* in, double* out);
+void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* mb,
+                    double* lam);
 // momentum update m = m_in - h grad p (imex_subsystems.cpp:186-215)
 void adj_momentum(hipStream_t st, const AD& a, const AdjScal& sc, const double* umid, const double* uin,
                   const double* ubar, double* pbar, double* sbar);

@@ -772,6 +772,63 @@ __global__ void __launch_bounds__(AB) k_adj_lin(AD a, int nc, int op, const doub
     default: out[c] += s * in[c]; break;          // add
   }
 }
+// ---- active pressure floor (implicit_operators.cpp:625-738, config.hpp:231-239), ideal gas.
+// The converged Newton solution of V_mod(p) + T p = b has the Jacobian T + diag V_slope(p): V_slope is
+// 1/(gamma - 1) above p_floor (the linear carrier's diagonal) and Mbig below it.  dv = V_slope - 1/(gamma-1).
+__global__ void __launch_bounds__(AB) k_adj_floor_dv(AD a, const double* __restrict__ p, const double* __restrict__ mb,
+                                                     double pfloor, double av, double* __restrict__ dv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.nx * a.ny) return;
+  const long long c = tix(a, n % a.nx, n / a.nx);
+  dv[c] = p[c] >= pfloor ? 0.0 : mb[0] - av;
+}
+// out += dv * in (the diagonal correction of the transposed Jacobian)
+__global__ void __launch_bounds__(AB) k_adj_diag_add(AD a, const double* __restrict__ dv, const double* __restrict__ in,
+                                                     double* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.nx * a.ny) return;
+  const long long c = tix(a, n % a.nx, n / a.nx);
+  out[c] += dv[c] * (in ? in[c] : 1.0);
+}
+// Mbig = max(M_base, M_cor) with M_cor = max(0, Vpe - min b) / ((1 - theta) p_floor) (:670-686), and
+// V_mod = Vpe + Mbig (p - p_floor) below the floor: the cotangent of Mbig is -sum_c lam_c (p_c - p_floor)
+// over the floored cells (lam = J^-T pbar), and where M_cor decides Mbig it reaches b at the cell that
+// attains min b (lowest index on ties): lam[argmin] += Mbig_bar * dM_cor/d(min b).  One block.
+__global__ void __launch_bounds__(1024) k_adj_floor_mbig(AD a, const double* __restrict__ p, const double* __restrict__ b,
+                                                         const double* __restrict__ mb, double pfloor, double av, double mbase,
+                                                         double theta, double* __restrict__ lam) {
+  __shared__ double sm[16], sb[16];
+  __shared__ int si[16];
+  const int N = a.nx * a.ny;
+  double acc = 0.0, bmin = 1e300;
+  int imin = N;
+  for (int n = threadIdx.x; n < N; n += blockDim.x) {
+    const long long c = tix(a, n % a.nx, n / a.nx);
+    if (p[c] < pfloor) acc += lam[c] * (p[c] - pfloor);
+    if (b[c] < bmin) { bmin = b[c]; imin = n; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_down(acc, o, 64);
+    const double ob = __shfl_down(bmin, o, 64);
+    const int oi = __shfl_down(imin, o, 64);
+    if (ob < bmin || (ob == bmin && oi < imin)) { bmin = ob; imin = oi; }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { sm[wid] = acc; sb[wid] = bmin; si[wid] = imin; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int w = 1; w < nw; w++) {
+      acc += sm[w];
+      if (sb[w] < bmin || (sb[w] == bmin && si[w] < imin)) { bmin = sb[w]; imin = si[w]; }
+    }
+    const double mbar = -acc;
+    const double vpe = pfloor * av;
+    const double mcor = fmax(0.0, vpe - bmin) / ((1.0 - theta) * pfloor);
+    if (mcor > mbase && imin < N) lam[tix(a, imin % a.nx, imin / a.nx)] += mbar * (-1.0 / ((1.0 - theta) * pfloor));
+  }
+}
+
 __global__ void __launch_bounds__(AB) k_adj_compact(AD a, int nc, int dir, const double* __restrict__ in, double* __restrict__ out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = a.nx * a.ny;
@@ -1188,6 +1245,17 @@ void adj_scale_copy(hipStream_t st, const AD& a, int nc, const double* in, doubl
 }
 void adj_div(hipStream_t st, const AD& a, int nc, const double* in, const double* diag, double* out) {
   k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 1, in, diag, 0.0, out);
+}
+void adj_floor_dv(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* mb, double* dv) {
+  k_adj_floor_dv<<<nblk(a), AB, 0, st>>>(a, p, mb, P.p_floor, 1.0 / (P.gamma - 1.0), dv);
+}
+void adj_diag_add(hipStream_t st, const AD& a, const double* dv, const double* in, double* out) {
+  k_adj_diag_add<<<nblk(a), AB, 0, st>>>(a, dv, in, out);
+}
+void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* mb,
+                    double* lam) {
+  const double av = 1.0 / (P.gamma - 1.0);
+  k_adj_floor_mbig<<<1, 1024, 0, st>>>(a, p, b, mb, P.p_floor, av, P.mbig_fac * av, P.theta, lam);
 }
 void adj_zero(hipStream_t st, const AD& a, int nc, double* p) {
   k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 2, nullptr, nullptr, 0.0, p);

@@ -166,6 +166,39 @@ def test_adjoint_weno5():
     assert errs.max() <= 1e-4, errs
 
 
+def floored_cells(n, steps, setup):
+    """Largest number of cells the nested Newton floors in one solve over the run, and min p."""
+    p, u, bx, by = vortex(n)
+    setup(p)
+    s = im.Solver(p)
+    try:
+        s.set_state(u, bx, by)
+        s.step(steps)
+        st = s.stats()
+    finally:
+        s.close()
+    return st["max_floored"], st["min_p"]
+
+
+@pytest.mark.parametrize("mbig_fac", [10.0, 1e-3])
+def test_adjoint_active_pressure_floor(mbig_fac):
+    """p_floor above the vortex's lowest pressures: the nested Newton floors an annulus of cells
+    (implicit_operators.cpp:625-738).  The transposed solves use the Newton Jacobian (V_slope = Mbig
+    below the floor) and Mbig's cotangent reaches the floored cells' p; with a small Mbig_fac the Cor. 1
+    slope M_cor(min b) decides Mbig, so the cotangent also reaches b at the cell of min b."""
+    def floor(p):
+        p.floor_on = 1
+        p.p_floor = 0.99
+        p.mbig_fac = mbig_fac
+    nfl, pmin = floored_cells(24, 3, floor)
+    print(f"[adjoint] vortex 24x24 with p_floor 0.99, Mbig_fac {mbig_fac}: up to {nfl:.0f} floored cells, min p {pmin:.6f}",
+          flush=True)
+    assert nfl > 0
+    errs = fd_check(24, 3, 2, 4, seed=31, setup=floor)
+    assert np.median(errs) <= 3e-5, errs
+    assert errs.max() <= 3e-4, errs
+
+
 def test_adjoint_last_step_clipped_to_time_stop():
     """dt = time_stop - t on the last step (main.cpp:341): its cotangent reaches every earlier dt
     through t, and the clipped step's own CFL dt gets none."""
