@@ -38,20 +38,30 @@ void adj_shock(hipStream_t st, const AD& a, const Phys& P, const double* Qe, dou
 // pressure right-hand side (imex_subsystems.cpp:123-175, 223-284), pass 0 or 1
 void adj_p_rhs(hipStream_t st, const AD& a, const AdjScal& sc, int pass, const double* ucur, const double* min,
                const double* hhat, const double* bbar, double* ucurbar, double* minbar, double* hhatbar, double* sbar);
-// pressure operator (implicit_operators.cpp:505-616), ideal gas: mode 0 A^T y into xbar; mode 1
-// coefficient cotangents (hhat, eta, scalars) of y . A x
+// pressure operator (implicit_operators.cpp:505-616), carrier mode: mode 0 A^T y into xbar; mode 1
+// coefficient cotangents (hhat, eta, the Redlich-Kwong viscosity plane visc, scalars) of y . A x; visc null
+// for the ideal gas
 void adj_p_op(hipStream_t st, const AD& a, const Phys& P, const AdjScal& sc, int mode, const double* x,
-              const double* hhat, const double* eta, const double* y, double* xbar, double* hhatbar, double* etabar,
-              double* sbar);
+              const double* hhat, const double* eta, const double* visc, const double* y, double* xbar, double* hhatbar,
+              double* etabar, double* viscbar, double* sbar);
 // Jacobi diagonal of the pressure operator (V' + T diag, implicit_operators.cpp:656-668)
 void adj_p_diag(hipStream_t st, const AD& a, const Phys& P, const AdjScal& sc, const double* hhat, double* diag);
-// Active pressure floor (ideal gas): dv = V_slope(p) - 1/(gamma-1) from the solution p and the solve's
-// Mbig (one device double); out += dv * in (in null: out += dv); and the Mbig path of the cotangent of b:
-// lam (the cotangent of b from the transposed solve) gets the term through M_cor(min b).
-void adj_floor_dv(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* mb, double* dv);
+// The nested Newton's solution (V_mod(p, rho) + T p = b): dv = dV_mod/dp - 1/(gamma-1) at the solution
+// p (mb: the solve's Mbig, one device double); its rho cotangent ubar[rho] -= lam dV_mod/drho; out += dv * in
+// (in null: out += dv); and, with the floor on, the Mbig path of the cotangents of b (into lam, the cotangent
+// of b from the transposed solve) and, for Redlich-Kwong, of rho.
+void adj_newton_dv(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* u, const double* mb, double* dv);
+void adj_newton_rho(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* u, const double* mb,
+                    const double* lam, double* ubar);
 void adj_diag_add(hipStream_t st, const AD& a, const double* dv, const double* in, double* out);
-void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* mb,
-                    double* lam);
+// dv = vs - 1/(gamma-1) for the forward's Jacobian-mode diagonal vs (a padded plane)
+void adj_slope_dv(hipStream_t st, const AD& a, const Phys& P, const double* vs, double* dv);
+void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* u,
+                    const double* mb, double* lam, double* ubar);
+// Redlich-Kwong viscosity scale of the pressure operator, Vbase_slope(p(Qe), rho), and its cotangents
+void adj_visc(hipStream_t st, const AD& a, const Phys& P, const double* pe, const double* u, double* visc);
+void adj_visc_vjp(hipStream_t st, const AD& a, const Phys& P, const double* pe, const double* u, const double* viscbar,
+                  double* pebar, double* ubar);
 // momentum update m = m_in - h grad p (imex_subsystems.cpp:186-215)
 void adj_momentum(hipStream_t st, const AD& a, const AdjScal& sc, const double* umid, const double* uin,
                   const double* ubar, double* pbar, double* sbar);

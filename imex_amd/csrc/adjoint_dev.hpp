@@ -1,6 +1,7 @@
 // adjoint_dev.hpp — device side of the discrete adjoint of the IMEX-RK2 step.
 //
-// Scope: periodic grids, ideal gas, Minmod or VanLeer reconstruction, EMF-consistent CT, FP64.
+// Scope: periodic grids, ideal gas or Redlich-Kwong, Minmod, VanLeer or WENO5 reconstruction of the
+// conservative state, the nested Newton with the pressure floor active or not, EMF-consistent CT, FP64.
 // On a periodic grid every ghost value is a copy of an interior value, so each field is treated
 // as a field on the torus of interior cells, read through a wrapped index (0-based interior
 // coordinates).
@@ -101,10 +102,99 @@ template <int K> __device__ __forceinline__ DN<K> sqrt(const DN<K>& a) {
   return r;
 }
 template <int K> __device__ __forceinline__ DN<K> fabs(const DN<K>& a) { return a.v < 0.0 ? -a : a; }
+template <int K> __device__ __forceinline__ DN<K> log(const DN<K>& a) {
+  DN<K> r; r.v = ::log(a.v);
+  const double s = 1.0 / a.v;
+#pragma unroll
+  for (int k = 0; k < K; k++) r.g[k] = a.g[k] * s;
+  return r;
+}
 
 // branch helpers that work for both scalar types
 template <class T> __device__ __forceinline__ T tmax(const T& a, const T& b) { return val(a) >= val(b) ? a : b; }
 template <class T> __device__ __forceinline__ T tmin(const T& a, const T& b) { return val(a) <= val(b) ? a : b; }
+
+// ---------------------------------------------------------------- EOS for both scalar types
+// (include/config.hpp:130-198): ideal gas, or Redlich-Kwong with its temperature solves (Newton to a
+// relative step of 1e-14, as the forward's).  With dual numbers the Newton iteration carries the
+// derivatives along; they converge with the value (each step solves the linearised equation again).
+template <class T>
+__device__ __forceinline__ T rkT_from_rho_p(const Phys& P, const T& rho, const T& p) {
+  const double mR = P.rk_R, ma = P.rk_alfa, mb = P.rk_b;
+  const T v = 1.0 / rho;
+  T T_new = (1.0 / rho - mb) * (p + ma * rho * rho) / mR, T_old;
+  int it = 0;
+  do {
+    T_old = T_new;
+    const T f = mR * T_old / (v - mb) - ma / sqrt(T_old) / v / (v + mb) - p;
+    const T df = mR / (v - mb) + 0.5 * ma / T_old / sqrt(T_old) / v / (v + mb);
+    T_new = T_old - f / df;
+  } while (++it <= RK_MAXIT && ::fabs(val(T_new) - val(T_old)) / ::fabs(val(T_old)) > 1e-14);
+  return T_new;
+}
+template <class T>
+__device__ __forceinline__ T rkT_from_rho_e(const Phys& P, const T& rho, const T& e) {
+  const double ma = P.rk_alfa, mb = P.rk_b, mcv = P.rk_cv;
+  const T v = 1.0 / rho;
+  const T U = log(v / (v + mb));
+  T T_new = (e + rho * ma) / mcv, T_old;
+  int it = 0;
+  do {
+    T_old = T_new;
+    const T f = mcv * T_old + 1.5 * ma / mb * U / sqrt(T_old) - e;
+    const T df = mcv - 0.75 * ma / mb * U / T_old / sqrt(T_old);
+    T_new = T_old - f / df;
+  } while (++it <= RK_MAXIT && ::fabs(val(T_new) - val(T_old)) / ::fabs(val(T_old)) > 1e-14);
+  return T_new;
+}
+template <class T>
+__device__ __forceinline__ T eos_e_t(const Phys& P, const T& rho, const T& p) {
+  if (P.eos == IMEX_EOS_IDEAL) return p / ((P.gamma - 1.0) * rho);
+  const T v = 1.0 / rho;
+  const T Tm = rkT_from_rho_p(P, rho, p);
+  const T U = log(v / (v + P.rk_b));
+  return P.rk_cv * Tm + 1.5 * P.rk_alfa / P.rk_b * U / sqrt(Tm);
+}
+template <class T>
+__device__ __forceinline__ T eos_p_t(const Phys& P, const T& rho, const T& e) {
+  if (P.eos == IMEX_EOS_IDEAL) return (P.gamma - 1.0) * rho * e;
+  const T v = 1.0 / rho;
+  const T Tm = rkT_from_rho_e(P, rho, e);
+  const T a = P.rk_alfa / sqrt(Tm);
+  return P.rk_R * Tm / (v - P.rk_b) - a / v / (v + P.rk_b);
+}
+template <class T>
+__device__ __forceinline__ T eos_a_t(const Phys& P, const T& rho, const T& p) {
+  if (P.eos == IMEX_EOS_IDEAL) return sqrt(P.gamma * p / rho);
+  const double mR = P.rk_R, ma = P.rk_alfa, mb = P.rk_b, mcv = P.rk_cv;
+  const T v = 1.0 / rho;
+  const T Tm = rkT_from_rho_p(P, rho, p);
+  const T U = log(v / (v + mb));
+  const T p_T = mR / (v - mb) + ma / (2.0 * Tm * sqrt(Tm) * v * (v + mb));
+  const T e_T = mcv - 0.75 * ma / mb * U / (Tm * sqrt(Tm));
+  const T p_v = -mR * Tm / ((v - mb) * (v - mb)) + ma * (2.0 * v + mb) / (sqrt(Tm) * v * v * ((v + mb) * (v + mb)));
+  const T e_v = 1.5 * ma / mb * (1.0 / v - 1.0 / (v + mb)) / sqrt(Tm);
+  const T a2 = p / (rho * rho) * (p_T / e_T) + v * v * (e_v - e_T * (p_v / p_T));
+  return sqrt(a2);
+}
+// Vbase (config.hpp:206-217): rho e(rho, p); its slope is the forward's central difference in p
+template <class T>
+__device__ __forceinline__ T vbase_t(const Phys& P, const T& p, const T& rho) {
+  if (P.eos == IMEX_EOS_IDEAL) return p / (P.gamma - 1.0);
+  return rho * eos_e_t(P, rho, p);
+}
+template <class T>
+__device__ __forceinline__ T vbase_slope_t(const Phys& P, const T& p, const T& rho) {
+  if (P.eos == IMEX_EOS_IDEAL) return T(1.0 / (P.gamma - 1.0));
+  const double d = 1e-8;
+  return rho * (eos_e_t(P, rho, p + d) - eos_e_t(P, rho, p - d)) / (2.0 * d);
+}
+// V_mod (config.hpp:231-234) with the floor on
+template <class T>
+__device__ __forceinline__ T vmod_t(const Phys& P, const T& p, const T& rho, double Mbig) {
+  if (val(p) >= P.p_floor) return vbase_t(P, p, rho);
+  return vbase_t(P, T(P.p_floor), rho) + Mbig * (p - P.p_floor);
+}
 
 // ---------------------------------------------------------------- torus geometry and fields
 struct AD {

@@ -318,7 +318,7 @@ struct FnEMF {
 // Fields: 0 Qe, 1 u (rho).
 struct FnPeHhat {
   static constexpr int K = 9, NOUT = 2, NS = 1;
-  double g1;
+  Phys P;
   __device__ static void off(int, int& oi, int& oj) { oi = 0; oj = 0; }
   __device__ static int slot(int fid, int k) { return fid == 0 ? k : (k == 0 ? 8 : -1); }
   __device__ static void which(int q, int& fid, int& k) { if (q < 8) { fid = 0; k = q; } else { fid = 1; k = 0; } }
@@ -330,7 +330,7 @@ struct FnPeHhat {
     const T B2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
     const T v2 = vx * vx + vy * vy + vz * vz;
     const T e = (q[4] - 0.5 * rho * v2 - 0.5 * B2) / rho;
-    const T p = g1 * rho * e;
+    const T p = eos_p_t(P, rho, e);
     out[0] = p;
     out[1] = (e + p / rho) * q[0] / rd(1, 0, i, j);
   }
@@ -339,7 +339,8 @@ struct FnPeHhat {
 // ---- shock indicator before the 3x3 max (flux.cpp:5-51).  Field: 0 Qe.
 struct FnChi {
   static constexpr int K = 8, NOUT = 1, NS = 9;
-  double gamma, kappa, hmin, dx, dy;
+  Phys P;
+  double kappa, hmin, dx, dy;
   __device__ static void off(int s, int& oi, int& oj) { oi = s % 3 - 1; oj = s / 3 - 1; }
   __device__ static int slot(int fid, int k) { return fid == 0 ? k : -1; }
   __device__ static void which(int q, int& fid, int& k) { fid = 0; k = q; }
@@ -353,8 +354,9 @@ struct FnChi {
         const T rho = q[0], vx = q[1] / q[0], vy = q[2] / q[0], vz = q[3] / q[0];
         const T B2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
         const T e = (q[4] - 0.5 * rho * (vx * vx + vy * vy + vz * vz) - 0.5 * B2) / rho;
-        const T p = (gamma - 1.0) * rho * e;
-        const T cf = sqrt(gamma * p / rho + B2 / rho);
+        const T p = eos_p_t(P, rho, e);
+        const T a = eos_a_t(P, rho, p);
+        const T cf = sqrt(a * a + B2 / rho);
         if (val(cf) < val(cf_min)) cf_min = cf;
       }
     const T div_u = (rd(0, 1, i + 1, j) / rd(0, 0, i + 1, j) - rd(0, 1, i - 1, j) / rd(0, 0, i - 1, j)) / (2.0 * dx)
@@ -410,12 +412,14 @@ struct FnPRHS {
   }
 };
 
-// ---- pressure operator row (implicit_operators.cpp:505-616), ideal gas (V' = visc_scale = 1/(gamma-1)).
-// Fields: 0 x, 1 hhat, 2 eta.  MODE 0 seeds x; MODE 1 seeds hhat and eta.
+// ---- pressure operator row (implicit_operators.cpp:505-616), carrier mode (V' = 1/(gamma-1)); the
+// viscosity scale is 1/(gamma-1) for the ideal gas, the plane Vbase_slope(p(Qe), rho) for Redlich-Kwong.
+// Fields: 0 x, 1 hhat, 2 eta, 3 visc (rk).  MODE 0 seeds x; MODE 1 seeds hhat, eta and visc.
 template <int MODE>
 struct FnPOp {
-  static constexpr int K = MODE == 0 ? 1 : 2, NOUT = 1, NS = 5;
+  static constexpr int K = MODE == 0 ? 1 : 3, NOUT = 1, NS = 5;
   double a, dx, dy;
+  int rk;
   __device__ static void off(int s, int& oi, int& oj) {
     const int ti[5] = {0, 1, -1, 0, 0}, tj[5] = {0, 0, 0, 1, -1};
     oi = ti[s]; oj = tj[s];
@@ -423,7 +427,7 @@ struct FnPOp {
   __device__ static int slot(int fid, int k) {
     if (k != 0) return -1;
     if (MODE == 0) return fid == 0 ? 0 : -1;
-    return fid == 1 ? 0 : (fid == 2 ? 1 : -1);
+    return fid >= 1 && fid <= 3 ? fid - 1 : -1;
   }
   __device__ static void which(int q, int& fid, int& k) { k = 0; fid = MODE == 0 ? 0 : 1 + q; }
   template <class T, class R>
@@ -441,14 +445,30 @@ struct FnPOp {
     const T e00 = E(i, j);
     const T lap = nu_x / (dx_ * dx_) * (0.5 * (E(i + 1, j) + e00) * (xp0 - pii) - 0.5 * (e00 + E(i - 1, j)) * (pii - xm0))
                 + nu_y / (dy_ * dy_) * (0.5 * (E(i, j + 1) + e00) * (x0p - pii) - 0.5 * (e00 + E(i, j - 1)) * (pii - x0m));
-    out[0] = ax - a * h * lap;
+    const T visc = rk ? rd(3, 0, i, j) : T(a);
+    out[0] = ax - visc * h * lap;
   }
 };
 
-// ---- energy from the pressure (imex_subsystems.cpp:286-302), ideal gas.  Fields: 0 p, 1 u.
+// ---- viscosity scale of the Redlich-Kwong pressure operator, Vbase_slope(p(Qe), rho) (config.hpp:211-217).
+// Fields: 0 p(Qe), 1 u (rho).
+struct FnVisc {
+  static constexpr int K = 2, NOUT = 1, NS = 1;
+  Phys P;
+  __device__ static void off(int, int& oi, int& oj) { oi = 0; oj = 0; }
+  __device__ static int slot(int fid, int k) { return k == 0 && fid <= 1 ? fid : -1; }
+  __device__ static void which(int q, int& fid, int& k) { fid = q; k = 0; }
+  template <class T, class R>
+  __device__ void eval(const R& rd, const Sc<T>&, int i, int j, T* out) const {
+    out[0] = vbase_slope_t(P, rd(0, 0, i, j), rd(1, 0, i, j));
+  }
+};
+
+// ---- energy from the pressure (imex_subsystems.cpp:286-302): Vbase(p, rho) + kinetic + magnetic.
+// Fields: 0 p, 1 u.
 struct FnEnergy {
   static constexpr int K = 8, NOUT = 1, NS = 1;
-  double a;
+  Phys P;
   __device__ static void off(int, int& oi, int& oj) { oi = 0; oj = 0; }
   __device__ static int slot(int fid, int k) {
     if (fid == 0) return k == 0 ? 0 : -1;
@@ -461,7 +481,7 @@ struct FnEnergy {
   __device__ void eval(const R& rd, const Sc<T>&, int i, int j, T* out) const {
     const T rho = rd(1, 0, i, j), m1 = rd(1, 1, i, j), m2 = rd(1, 2, i, j), m3 = rd(1, 3, i, j);
     const T b5 = rd(1, 5, i, j), b6 = rd(1, 6, i, j), b7 = rd(1, 7, i, j);
-    out[0] = a * rd(0, 0, i, j) + 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
+    out[0] = vbase_t(P, rd(0, 0, i, j), rho) + 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho + 0.5 * (b5 * b5 + b6 * b6 + b7 * b7);
   }
 };
 
@@ -482,7 +502,7 @@ struct FnEt {
 // the conservative state: v + c_f(|B|), |v|, the two magnetic eigenvalue bounds, |vx|, |vy|.
 struct FnMax {
   static constexpr int K = 8, NOUT = 6, NS = 1;
-  double gamma;
+  Phys P;
   __device__ static void off(int, int& oi, int& oj) { oi = 0; oj = 0; }
   __device__ static int slot(int fid, int k) { return fid == 0 ? k : -1; }
   __device__ static void which(int q, int& fid, int& k) { fid = 0; k = q; }
@@ -493,10 +513,11 @@ struct FnMax {
     const T rho = q[0], vx = q[1] / q[0], vy = q[2] / q[0], vz = q[3] / q[0];
     const T B2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
     const T e = (q[4] - 0.5 * rho * (vx * vx + vy * vy + vz * vz) - 0.5 * B2) / rho;
-    const T p = (gamma - 1.0) * rho * e;
+    const T p = eos_p_t(P, rho, e);
     const T Bm = sqrt(B2);
     const T v = sqrt(vx * vx + vy * vy + vz * vz);
-    const T a2 = gamma * p / rho;
+    const T a = eos_a_t(P, rho, p);
+    const T a2 = a * a;
     const T t = a2 + B2 / rho;
     T disc = t * t - 4.0 * a2 * Bm * Bm / rho;
     if (val(disc) < 0.0) disc = T(0.0);
@@ -510,10 +531,10 @@ struct FnMax {
   }
 };
 
-// ---- primitive -> conservative (state.cpp:242-255), ideal gas.  Field: 0 primitives.
+// ---- primitive -> conservative (state.cpp:242-255).  Field: 0 primitives.
 struct FnPri2Con {
   static constexpr int K = 8, NOUT = 8, NS = 1;
-  double g1;
+  Phys P;
   __device__ static void off(int, int& oi, int& oj) { oi = 0; oj = 0; }
   __device__ static int slot(int fid, int k) { return fid == 0 ? k : -1; }
   __device__ static void which(int q, int& fid, int& k) { fid = 0; k = q; }
@@ -521,7 +542,7 @@ struct FnPri2Con {
   __device__ void eval(const R& rd, const Sc<T>&, int i, int j, T* c) const {
     T p[8];
     for (int k = 0; k < 8; k++) p[k] = rd(0, k, i, j);
-    const T e = p[4] / (g1 * p[0]);
+    const T e = eos_e_t(P, p[0], p[4]);
     const T B2 = p[5] * p[5] + p[6] * p[6] + p[7] * p[7];
     const T v2 = p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
     c[0] = p[0]; c[1] = p[0] * p[1]; c[2] = p[0] * p[2]; c[3] = p[0] * p[3];
@@ -772,17 +793,33 @@ __global__ void __launch_bounds__(AB) k_adj_lin(AD a, int nc, int op, const doub
     default: out[c] += s * in[c]; break;          // add
   }
 }
-// ---- active pressure floor (implicit_operators.cpp:625-738, config.hpp:231-239), ideal gas.
-// The converged Newton solution of V_mod(p) + T p = b has the Jacobian T + diag V_slope(p): V_slope is
-// 1/(gamma - 1) above p_floor (the linear carrier's diagonal) and Mbig below it.  dv = V_slope - 1/(gamma-1).
-__global__ void __launch_bounds__(AB) k_adj_floor_dv(AD a, const double* __restrict__ p, const double* __restrict__ mb,
-                                                     double pfloor, double av, double* __restrict__ dv) {
+// ---- the nested Newton's converged solution (implicit_operators.cpp:625-738, config.hpp:206-239):
+// V_mod(p, rho) + T p = b, T the carrier operator less its 1/(gamma-1) diagonal.  Its Jacobian in p
+// is T + diag dV_mod/dp: the transposed solve adds dv = dV_mod/dp - 1/(gamma-1) to the carrier's
+// diagonal (Mbig below the floor; for Redlich-Kwong the derivative of rho e(rho, p), not the forward's
+// difference quotient, which only steers the Newton iteration).  mode 0: dv; mode 1: the cotangent of
+// rho, rhobar -= lam dV_mod/drho.
+__global__ void __launch_bounds__(AB) k_adj_newton(AD a, Phys P, int mode, const double* __restrict__ p,
+                                                   const double* __restrict__ u, const double* __restrict__ mb,
+                                                   const double* __restrict__ lam, double* __restrict__ out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.nx * a.ny) return;
   const long long c = tix(a, n % a.nx, n / a.nx);
-  dv[c] = p[c] >= pfloor ? 0.0 : mb[0] - av;
+  typedef DN<2> T;
+  T pv(p[c]), rv(u[c]);
+  pv.g[0] = 1.0; rv.g[1] = 1.0;
+  const T V = P.floor_on ? vmod_t(P, pv, rv, mb[0]) : vbase_t(P, pv, rv);
+  if (mode == 0) out[c] = V.g[0] - 1.0 / (P.gamma - 1.0);
+  else out[c] -= lam[c] * V.g[1];
 }
-// out += dv * in (the diagonal correction of the transposed Jacobian)
+// dv = vs - 1/(gamma-1) for a given slope plane vs (the forward's Jacobian-mode diagonal)
+__global__ void __launch_bounds__(AB) k_adj_slope_dv(AD a, double av, const double* __restrict__ vs, double* __restrict__ dv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.nx * a.ny) return;
+  const long long c = tix(a, n % a.nx, n / a.nx);
+  dv[c] = vs[c] - av;
+}
+// out += dv * in (the diagonal correction of the transposed Jacobian; in null: out += dv)
 __global__ void __launch_bounds__(AB) k_adj_diag_add(AD a, const double* __restrict__ dv, const double* __restrict__ in,
                                                      double* __restrict__ out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -790,42 +827,64 @@ __global__ void __launch_bounds__(AB) k_adj_diag_add(AD a, const double* __restr
   const long long c = tix(a, n % a.nx, n / a.nx);
   out[c] += dv[c] * (in ? in[c] : 1.0);
 }
-// Mbig = max(M_base, M_cor) with M_cor = max(0, Vpe - min b) / ((1 - theta) p_floor) (:670-686), and
-// V_mod = Vpe + Mbig (p - p_floor) below the floor: the cotangent of Mbig is -sum_c lam_c (p_c - p_floor)
-// over the floored cells (lam = J^-T pbar), and where M_cor decides Mbig it reaches b at the cell that
-// attains min b (lowest index on ties): lam[argmin] += Mbig_bar * dM_cor/d(min b).  One block.
-__global__ void __launch_bounds__(1024) k_adj_floor_mbig(AD a, const double* __restrict__ p, const double* __restrict__ b,
-                                                         const double* __restrict__ mb, double pfloor, double av, double mbase,
-                                                         double theta, double* __restrict__ lam) {
-  __shared__ double sm[16], sb[16];
-  __shared__ int si[16];
+// Mbig = max(M_base, M_cor) with M_cor = max(0, Vpe - min b) / ((1 - theta) p_floor) and Vpe =
+// max(p_floor/(gamma-1), max rho e(rho, p_floor)) (the second for Redlich-Kwong; :670-686), and V_mod =
+// Vbase(p_floor) + Mbig (p - p_floor) below the floor: the cotangent of Mbig is -sum_c lam_c (p_c - p_floor)
+// over the floored cells (lam = J^-T pbar).  Where M_cor decides Mbig it reaches b at the cell that attains
+// min b (lam[argmin] += Mbig_bar dM_cor/d(min b)) and, when the Redlich-Kwong term decides Vpe, rho at the
+// cell that attains it (lowest indices on ties).  One block.
+__global__ void __launch_bounds__(1024) k_adj_floor_mbig(AD a, Phys P, const double* __restrict__ p,
+                                                         const double* __restrict__ b, const double* __restrict__ u,
+                                                         const double* __restrict__ mb, double* __restrict__ lam,
+                                                         double* __restrict__ ubar) {
+  __shared__ double sm[16], sb[16], sv[16];
+  __shared__ int si[16], sj[16];
   const int N = a.nx * a.ny;
-  double acc = 0.0, bmin = 1e300;
-  int imin = N;
+  const double pf = P.p_floor, av = 1.0 / (P.gamma - 1.0);
+  const bool rk = P.eos != IMEX_EOS_IDEAL;
+  double acc = 0.0, bmin = 1e300, vmax = -1e300;
+  int imin = N, imax = N;
   for (int n = threadIdx.x; n < N; n += blockDim.x) {
     const long long c = tix(a, n % a.nx, n / a.nx);
-    if (p[c] < pfloor) acc += lam[c] * (p[c] - pfloor);
+    if (p[c] < pf) acc += lam[c] * (p[c] - pf);
     if (b[c] < bmin) { bmin = b[c]; imin = n; }
+    if (rk) {
+      const double v = vbase_t(P, pf, u[c]);
+      if (v > vmax) { vmax = v; imax = n; }
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     acc += __shfl_down(acc, o, 64);
-    const double ob = __shfl_down(bmin, o, 64);
-    const int oi = __shfl_down(imin, o, 64);
+    const double ob = __shfl_down(bmin, o, 64), ov = __shfl_down(vmax, o, 64);
+    const int oi = __shfl_down(imin, o, 64), oj = __shfl_down(imax, o, 64);
     if (ob < bmin || (ob == bmin && oi < imin)) { bmin = ob; imin = oi; }
+    if (ov > vmax || (ov == vmax && oj < imax)) { vmax = ov; imax = oj; }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) { sm[wid] = acc; sb[wid] = bmin; si[wid] = imin; }
+  if (lane == 0) { sm[wid] = acc; sb[wid] = bmin; si[wid] = imin; sv[wid] = vmax; sj[wid] = imax; }
   __syncthreads();
   if (threadIdx.x == 0) {
     const int nw = blockDim.x >> 6;
     for (int w = 1; w < nw; w++) {
       acc += sm[w];
       if (sb[w] < bmin || (sb[w] == bmin && si[w] < imin)) { bmin = sb[w]; imin = si[w]; }
+      if (sv[w] > vmax || (sv[w] == vmax && sj[w] < imax)) { vmax = sv[w]; imax = sj[w]; }
     }
     const double mbar = -acc;
-    const double vpe = pfloor * av;
-    const double mcor = fmax(0.0, vpe - bmin) / ((1.0 - theta) * pfloor);
-    if (mcor > mbase && imin < N) lam[tix(a, imin % a.nx, imin / a.nx)] += mbar * (-1.0 / ((1.0 - theta) * pfloor));
+    double vpe = pf * av;
+    const bool vrk = rk && vmax > vpe;
+    if (vrk) vpe = vmax;
+    const double mcor = fmax(0.0, vpe - bmin) / ((1.0 - P.theta) * pf);
+    if (mcor > P.mbig_fac * av && vpe - bmin > 0.0 && imin < N) {
+      const double s = mbar / ((1.0 - P.theta) * pf);      // Mbar dM_cor/dVpe; dM_cor/d(min b) = -that
+      lam[tix(a, imin % a.nx, imin / a.nx)] -= s;
+      if (vrk && imax < N) {
+        const long long c = tix(a, imax % a.nx, imax / a.nx);
+        DN<1> r(u[c]);
+        r.g[0] = 1.0;
+        ubar[c] += s * vbase_t(P, DN<1>(pf), r).g[0];
+      }
+    }
   }
 }
 
@@ -1117,13 +1176,13 @@ void adj_emf(hipStream_t st, const AD& a, const AdjScal& sc, const double* u, co
 }
 void adj_pe_hhat(hipStream_t st, const AD& a, const Phys& P, const double* Qe, const double* u, const double* obar,
                  double* Qebar, double* ubar) {
-  FnPeHhat fn{P.gamma - 1.0};
+  FnPeHhat fn{P};
   AdjScal z{};
   vjp(st, fn, a, fields({Qe, u}, {Qebar, ubar}), obar, z, nullptr);
 }
 void adj_shock(hipStream_t st, const AD& a, const Phys& P, const double* Qe, double* chi, const double* etabar,
                double* chibar, double* Qebar) {
-  FnChi fn{P.gamma, P.kappa1, fmin(a.dx, a.dy), a.dx, a.dy};
+  FnChi fn{P, P.kappa1, fmin(a.dx, a.dy), a.dx, a.dy};
   const AFields F = fields({Qe}, {Qebar});
   k_eval<FnChi><<<nblk(a), AB, 0, st>>>(fn, a, F, chi, Sc<double>{});
   k_adj_lin<<<akr_grid1(a, 1), AB, 0, st>>>(a, 1, 2, nullptr, nullptr, 0.0, chibar);
@@ -1137,16 +1196,27 @@ void adj_p_rhs(hipStream_t st, const AD& a, const AdjScal& sc, int pass, const d
   vjp(st, fn, a, fields({ucur, min, hhat}, {ucurbar, minbar, hhatbar}), bbar, sc, sbar);
 }
 void adj_p_op(hipStream_t st, const AD& a, const Phys& P, const AdjScal& sc, int mode, const double* x,
-              const double* hhat, const double* eta, const double* y, double* xbar, double* hhatbar, double* etabar,
-              double* sbar) {
+              const double* hhat, const double* eta, const double* visc, const double* y, double* xbar, double* hhatbar,
+              double* etabar, double* viscbar, double* sbar) {
   const double av = 1.0 / (P.gamma - 1.0);
+  const int rk = visc != nullptr;
   if (mode == 0) {
-    FnPOp<0> fn{av, a.dx, a.dy};
-    vjp(st, fn, a, fields({x, hhat, eta}, {xbar, nullptr, nullptr}), y, sc, sbar);
+    FnPOp<0> fn{av, a.dx, a.dy, rk};
+    vjp(st, fn, a, fields({x, hhat, eta, visc}, {xbar, nullptr, nullptr, nullptr}), y, sc, sbar);
   } else {
-    FnPOp<1> fn{av, a.dx, a.dy};
-    vjp(st, fn, a, fields({x, hhat, eta}, {nullptr, hhatbar, etabar}), y, sc, sbar);
+    FnPOp<1> fn{av, a.dx, a.dy, rk};
+    vjp(st, fn, a, fields({x, hhat, eta, visc}, {nullptr, hhatbar, etabar, rk ? viscbar : nullptr}), y, sc, sbar);
   }
+}
+void adj_visc(hipStream_t st, const AD& a, const Phys& P, const double* pe, const double* u, double* visc) {
+  FnVisc fn{P};
+  k_eval<FnVisc><<<nblk(a), AB, 0, st>>>(fn, a, fields({pe, u}, {}), visc, Sc<double>{});
+}
+void adj_visc_vjp(hipStream_t st, const AD& a, const Phys& P, const double* pe, const double* u, const double* viscbar,
+                  double* pebar, double* ubar) {
+  FnVisc fn{P};
+  AdjScal z{};
+  vjp(st, fn, a, fields({pe, u}, {pebar, ubar}), viscbar, z, nullptr);
 }
 void adj_p_diag(hipStream_t st, const AD& a, const Phys& P, const AdjScal& sc, const double* hhat, double* diag) {
   k_adj_pdiag<<<nblk(a), AB, 0, st>>>(a, sc.s[SC_H], 1.0 / (P.gamma - 1.0), hhat, diag);
@@ -1157,7 +1227,7 @@ void adj_momentum(hipStream_t st, const AD& a, const AdjScal& sc, const double* 
 }
 void adj_energy(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* u, const double* u4bar,
                 double* pbar, double* ubar) {
-  FnEnergy fn{1.0 / (P.gamma - 1.0)};
+  FnEnergy fn{P};
   AdjScal z{};
   vjp(st, fn, a, fields({p, u}, {pbar, ubar}), u4bar, z, nullptr);
 }
@@ -1177,7 +1247,7 @@ void adj_ct(hipStream_t st, const AD& a, const AdjScal& sc, const double* u, con
   vjp(st, fn, a, fields({u}, {ubar}), et, z, nullptr);
 }
 void adj_max_values(hipStream_t st, const AD& a, const Phys& P, const double* U, double* vals) {
-  FnMax fn{P.gamma};
+  FnMax fn{P};
   k_eval<FnMax><<<nblk(a), AB, 0, st>>>(fn, a, fields({U}, {}), vals, Sc<double>{});
 }
 void adj_argmax(hipStream_t st, const AD& a, const double* vals, int* out, double* work) {
@@ -1189,11 +1259,11 @@ void adj_argmax(hipStream_t st, const AD& a, const double* vals, int* out, doubl
 }
 void adj_max_grad(hipStream_t st, const AD& a, const Phys& P, const double* U, const int* argmax, const double* qbar,
                   double* Ubar) {
-  FnMax fn{P.gamma};
+  FnMax fn{P};
   k_max_grad<<<1, 64, 0, st>>>(fn, a, fields({U}, {}), argmax, qbar, Ubar);
 }
 void adj_pri2con(hipStream_t st, const AD& a, const Phys& P, const double* pri, const double* conbar, double* pribar) {
-  FnPri2Con fn{P.gamma - 1.0};
+  FnPri2Con fn{P};
   AdjScal z{};
   vjp(st, fn, a, fields({pri}, {pribar}), conbar, z, nullptr);
 }
@@ -1246,16 +1316,22 @@ void adj_scale_copy(hipStream_t st, const AD& a, int nc, const double* in, doubl
 void adj_div(hipStream_t st, const AD& a, int nc, const double* in, const double* diag, double* out) {
   k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 1, in, diag, 0.0, out);
 }
-void adj_floor_dv(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* mb, double* dv) {
-  k_adj_floor_dv<<<nblk(a), AB, 0, st>>>(a, p, mb, P.p_floor, 1.0 / (P.gamma - 1.0), dv);
+void adj_newton_dv(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* u, const double* mb, double* dv) {
+  k_adj_newton<<<nblk(a), AB, 0, st>>>(a, P, 0, p, u, mb, nullptr, dv);
+}
+void adj_newton_rho(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* u, const double* mb,
+                    const double* lam, double* ubar) {
+  k_adj_newton<<<nblk(a), AB, 0, st>>>(a, P, 1, p, u, mb, lam, ubar);
+}
+void adj_slope_dv(hipStream_t st, const AD& a, const Phys& P, const double* vs, double* dv) {
+  k_adj_slope_dv<<<nblk(a), AB, 0, st>>>(a, 1.0 / (P.gamma - 1.0), vs, dv);
 }
 void adj_diag_add(hipStream_t st, const AD& a, const double* dv, const double* in, double* out) {
   k_adj_diag_add<<<nblk(a), AB, 0, st>>>(a, dv, in, out);
 }
-void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* mb,
-                    double* lam) {
-  const double av = 1.0 / (P.gamma - 1.0);
-  k_adj_floor_mbig<<<1, 1024, 0, st>>>(a, p, b, mb, P.p_floor, av, P.mbig_fac * av, P.theta, lam);
+void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* u,
+                    const double* mb, double* lam, double* ubar) {
+  k_adj_floor_mbig<<<1, 1024, 0, st>>>(a, P, p, b, u, mb, lam, ubar);
 }
 void adj_zero(hipStream_t st, const AD& a, int nc, double* p) {
   k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 2, nullptr, nullptr, 0.0, p);

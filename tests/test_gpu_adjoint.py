@@ -121,10 +121,16 @@ def fd_check(n, steps, every, ndir, seed, eps=1e-6, scale=1e-2, m=None, time_sto
     return np.array(errs)
 
 
-@pytest.mark.parametrize("op,nc", [(im.OP_MAG, 3), (im.OP_P_CARRIER, 1)])
-def test_transposed_operator_dot_product(op, nc):
-    print(f"[adjoint] dot-product test op={op}", flush=True)
+@pytest.mark.parametrize("op,nc,eos", [(im.OP_MAG, 3, im.EOS_IDEAL), (im.OP_P_CARRIER, 1, im.EOS_IDEAL),
+                                        (im.OP_P_JACOBIAN, 1, im.EOS_IDEAL), (im.OP_P_CARRIER, 1, im.EOS_RK),
+                                        (im.OP_P_JACOBIAN, 1, im.EOS_RK)])
+def test_transposed_operator_dot_product(op, nc, eos):
+    """<y, A x> = <A^T y, x>; the Jacobian mode with p_freeze on both sides of p_floor (V_slope = Mbig
+    below it), the Redlich-Kwong operators with their viscosity plane and V_slope."""
+    print(f"[adjoint] dot-product test op={op} eos={eos}", flush=True)
     p, u, bx, by = vortex(32, 24)
+    p.eos = eos
+    p.p_floor = 0.99
     s = im.Solver(p)
     try:
         s.set_state(u, bx, by)
@@ -133,8 +139,13 @@ def test_transposed_operator_dot_product(op, nc):
         rng = np.random.default_rng(3)
         x = rng.standard_normal(nc * N)
         y = rng.standard_normal(nc * N)
-        Ax = s.apply_op(op, x)
-        ATy = s.apply_op_T(op, y)
+        pf, mbig = None, 0.0
+        if op == im.OP_P_JACOBIAN:
+            pf = 1.0 + 0.02 * rng.standard_normal(N)
+            mbig = 40.0
+            assert (pf < p.p_floor).any() and (pf >= p.p_floor).any()
+        Ax = s.apply_op(op, x, pf, mbig)
+        ATy = s.apply_op_T(op, y, pf, mbig)
     finally:
         s.close()
     lhs, rhs = float(y @ Ax), float(ATy @ x)
@@ -197,6 +208,40 @@ def test_adjoint_active_pressure_floor(mbig_fac):
     errs = fd_check(24, 3, 2, 4, seed=31, setup=floor)
     assert np.median(errs) <= 3e-5, errs
     assert errs.max() <= 3e-4, errs
+
+
+def test_adjoint_redlich_kwong():
+    """Redlich-Kwong EOS (config.hpp:130-173): temperature solves inside p(Qe), the sound speeds and the
+    energy; the pressure solve is the nested Newton's on Vbase(p, rho) = rho e(rho, p), and the
+    operator's viscosity scale is Vbase_slope(p(Qe), rho)."""
+    print("[adjoint] vortex 24x24 Redlich-Kwong, 3 steps, checkpoint every 2", flush=True)
+
+    def rk(p):
+        p.eos = im.EOS_RK
+    errs = fd_check(24, 3, 2, 4, seed=41, setup=rk)
+    assert np.median(errs) <= 3e-5, errs
+    assert errs.max() <= 3e-4, errs
+
+
+def test_adjoint_redlich_kwong_active_floor():
+    """Redlich-Kwong with the floor active and Mbig decided by Cor. 1, whose Vpe is max rho e(rho, p_floor)
+    over the cells: Mbig's cotangent reaches rho at the cell that attains it.  The nested Newton stops at a
+    relative residual of 1e-8 (implicit_operators.cpp:703-706); with the floored cells and the
+    Redlich-Kwong energy its last iterate keeps a truncation error that differences of eps = 1e-6 amplify
+    to ~1e-4 (measured: max 3.5e-4 at eps = 1e-6, 3.5e-5 at 1e-5, 7e-4 at 1e-4 where the differences cross
+    the floor's kink), so this check differences with eps = 1e-5."""
+    def rkf(p):
+        p.eos = im.EOS_RK
+        p.floor_on = 1
+        p.p_floor = 0.99
+        p.mbig_fac = 1e-3
+    nfl, pmin = floored_cells(24, 3, rkf)
+    print(f"[adjoint] vortex 24x24 Redlich-Kwong with p_floor 0.99: up to {nfl:.0f} floored cells, min p {pmin:.6f}",
+          flush=True)
+    assert nfl > 0
+    errs = fd_check(24, 3, 2, 4, seed=43, eps=1e-5, setup=rkf)
+    assert np.median(errs) <= 3e-5, errs
+    assert errs.max() <= 2e-4, errs
 
 
 def test_adjoint_last_step_clipped_to_time_stop():
