@@ -974,9 +974,13 @@ k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict_
 
 // ---------------------------------------------------------------- Newton residual (implicit_operators.cpp:690-710)
 // g = L p - a p + V_mod(p, rho) - b with L the carrier operator; writes -g and ||g||.
-__global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ip,
+// RK: the Redlich-Kwong instantiation; the ideal-gas one carries no temperature solves (fewer registers)
+template <bool RK>
+__global__ void __launch_bounds__(256) k_newton_residual(PIds ids, Phys P_, const Inst* __restrict__ IT, BMap bm, int ip,
                                                          int ib, int it, double ntol) {
   BATCH_BLOCK(bm);
+  Phys P = P_;
+  if constexpr (!RK) P.eos = IMEX_EOS_IDEAL;
   const Inst& I = IT[inst_];
   if (it > 0 && gmem(I.K)->newton_conv) return;     // an earlier iteration of this Newton solve has converged
   const POp op = make_p_op(I, P, ids);
@@ -1230,7 +1234,8 @@ void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const P
 }
 
 void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int ib, int it, double ntol) {
-  k_newton_residual<<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
+  if (P.eos != IMEX_EOS_IDEAL) k_newton_residual<true><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
+  else k_newton_residual<false><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
 }
 void newton_residual_fin(const Launch& L, int it, double ntol) {
   k_newton_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], it, ntol);

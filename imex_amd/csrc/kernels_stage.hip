@@ -104,8 +104,12 @@ __global__ void k_bc_fill(const Inst* __restrict__ IT, BMap bm, FillPlanes fp, i
 // calch + caldt + MagneticEigenvalues (timestep.cpp:5-67, 89-120, 254-273) fused with
 // the con -> pri -> con round trip of main.cpp:356/561: u_n = pri2con(pri).
 // mode 0: `uin` holds primitives; mode 1: `uin` holds conservatives (pri = con2pri).
-__global__ void __launch_bounds__(256) k_step_start(Phys P, const Inst* __restrict__ IT, BMap bm, int mode, int iin) {
+// RK: the Redlich-Kwong instantiation; the ideal-gas one carries no temperature solves (fewer registers)
+template <bool RK>
+__global__ void __launch_bounds__(256) k_step_start(Phys P_, const Inst* __restrict__ IT, BMap bm, int mode, int iin) {
   BATCH_BLOCK(bm);
+  Phys P = P_;
+  if constexpr (!RK) P.eos = IMEX_EOS_IDEAL;
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
   const double* __restrict__ uin = gmem(I_.buf[iin]);
@@ -748,9 +752,12 @@ __global__ void __launch_bounds__(EX_TX * EX_TYT) k_explicit(Phys P, const Inst*
 // ================================================================ magnetic subsystem
 // RHS of MagSubsystem_UpdateB (imex_subsystems.cpp:549-613), initial guess x0 = B(Qe),
 // and the enthalpy ratio hhat = calenthalpy(Qe) rho^e / rho (used by the pressure subsystem).
-__global__ void __launch_bounds__(256) k_mag_rhs(Phys P, const Inst* __restrict__ IT, BMap bm, int iu, int iQe, int iQs,
+template <bool RK>
+__global__ void __launch_bounds__(256) k_mag_rhs(Phys P_, const Inst* __restrict__ IT, BMap bm, int iu, int iQe, int iQs,
                                                  int ib, int ix0) {
   BATCH_BLOCK(bm);
+  Phys P = P_;
+  if constexpr (!RK) P.eos = IMEX_EOS_IDEAL;
   const Inst& I_ = IT[inst_];
   const Dims& d = I_.d;
   const double* __restrict__ u = gmem(I_.buf[iu]);
@@ -1615,7 +1622,8 @@ void launch_bc_fill(const Launch& L, int n, const int* b, const int* k, int kind
   k_bc_fill<<<GRID(L, sh), 256, 0, L.st>>>(L.IT, L.bm[sh], fp, kind);
 }
 void launch_step_start(const Launch& L, const Phys& P, int mode, int iin) {
-  k_step_start<<<GRID(L, SH_START), 256, 0, L.st>>>(P, L.IT, L.bm[SH_START], mode, iin);
+  if (P.eos != IMEX_EOS_IDEAL) k_step_start<true><<<GRID(L, SH_START), 256, 0, L.st>>>(P, L.IT, L.bm[SH_START], mode, iin);
+  else k_step_start<false><<<GRID(L, SH_START), 256, 0, L.st>>>(P, L.IT, L.bm[SH_START], mode, iin);
   k_step_finish<<<GRID(L, SH_ONE), 256, 0, L.st>>>(P, L.IT, L.bm[SH_ONE], L.bm[SH_START]);
 }
 void launch_stage_combine(const Launch& L, int ibxn, int ibyn) {
@@ -1655,7 +1663,8 @@ void launch_explicit(const Launch& L, const Phys& P, int kind, int iQe, int iQs,
   else launch_explicit_t<32, 8>(L, P, kind, iQe, iQs, iu, mixed);
 }
 void launch_mag_rhs(const Launch& L, const Phys& P, int iu, int iQe, int iQs, int ib, int ix0) {
-  k_mag_rhs<<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu, iQe, iQs, ib, ix0);
+  if (P.eos != IMEX_EOS_IDEAL) k_mag_rhs<true><<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu, iQe, iQs, ib, ix0);
+  else k_mag_rhs<false><<<GRID(L, SH_INT64x4), B64x4, 0, L.st>>>(P, L.IT, L.bm[SH_INT64x4], iu, iQe, iQs, ib, ix0);
 }
 void launch_mag_writeback(const Launch& L, int iu) {
   k_mag_writeback<<<GRID(L, SH_LIN_TOT), 256, 0, L.st>>>(L.IT, L.bm[SH_LIN_TOT], iu);
