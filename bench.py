@@ -396,30 +396,32 @@ def copy_calibration(mib):
 KERNEL_OF_CLASS = {   # (class, form) or (variant, class, form) -> kernel symbol in the committed rocprof summary
     ("potential_a", "mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<3, double, double>",
     ("potential_a", "mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<3, float, float>",
-    ("energy", "explicit_energy", "mixed"): "k_explicit<32, double, false, false>",   # the energy form keeps FP64 fluxes
+    ("energy", "explicit_energy", "mixed"): "k_explicit<{tx}, {ty}, double, false, false>",   # the energy form keeps FP64 fluxes
     ("mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<0, double, double>",
     ("mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<0, float, float>",
     ("p_arnoldi_apply", "fp64"): "k_arnoldi_apply<1, double, double>",
     ("p_arnoldi_apply", "mixed"): "k_arnoldi_apply<1, double, double>",
     ("e_arnoldi_apply", "fp64"): "k_arnoldi_apply<2, double, double>",
     ("e_arnoldi_apply", "mixed"): "k_arnoldi_apply<2, double, double>",
-    ("explicit_energy", "fp64"): "k_explicit<32, double, false, false>",
-    ("explicit_energy", "mixed"): "k_explicit<32, float, false, false>",
+    ("explicit_energy", "fp64"): "k_explicit<{tx}, {ty}, double, false, false>",
+    ("explicit_energy", "mixed"): "k_explicit<{tx}, {ty}, float, false, false>",
 }
 TRAFFIC_SUMMARY = {"pressure": os.path.join(ROOT, "profiles", "imex_step_ot1024_kernels.md"),
                    "energy": os.path.join(ROOT, "profiles", "imex_step_ot1024_energy_kernels.md"),
                    "potential_a": os.path.join(ROOT, "profiles", "imex_step_ot1024_potential_kernels.md")}
 
 
-def kernel_of(cls, form, variant):
-    return KERNEL_OF_CLASS.get((variant, cls, form), KERNEL_OF_CLASS.get((cls, form)))
+def kernel_of(cls, form, variant, tile=(32, 16)):
+    """Kernel symbol behind a profiler class; explicit kernels are named by their (autotuned) tile."""
+    k = KERNEL_OF_CLASS.get((variant, cls, form), KERNEL_OF_CLASS.get((cls, form)))
+    return k.format(tx=tile[0], ty=tile[1]) if k else None
 
 
-def measured_traffic(cls, form, variant="pressure"):
+def measured_traffic(cls, form, variant="pressure", tile=(32, 16)):
     """HBM bytes per active dispatch of the kernel behind `cls`, from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of the run variant (calibrated against a plain copy); None if it is not listed there."""
     import re
-    name = kernel_of(cls, form, variant)
+    name = kernel_of(cls, form, variant, tile)
     path = TRAFFIC_SUMMARY.get(variant)
     if not name or not path or not os.path.exists(path):
         return None
@@ -452,14 +454,15 @@ def summarize(r, steps, label, variant="pressure"):
     log(f"[bench] {label} profiled pass: kernel time {tot_ms:.1f} ms of {r['prof_elapsed'] * 1e3:.1f} ms wall; "
         f"timed pass {r['elapsed'] * 1e3:.1f} ms; stats {r['stats']}")
     st = r["stats"]
-    traffic = measured_traffic(dom["name"], label, variant)
+    tile = tuple(r["tune"]["explicit_tile"])
+    traffic = measured_traffic(dom["name"], label, variant, tile)
     return {
         "value": round(value, 3),
         "ms_per_step": round(r["elapsed"] / steps * 1e3, 3),
         "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per active dispatch of "
-                                        f"{kernel_of(dom['name'], label, variant)}, calibrated, "
+                                        f"{kernel_of(dom['name'], label, variant, tile)}, calibrated, "
                                         f"{os.path.relpath(TRAFFIC_SUMMARY[variant], ROOT)}") if traffic else None,
                      "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "gmres": {k: st[k] for k in ("mag_gmres_iters", "mag_solves", "p_gmres_iters", "p_solves", "newton_iters",
