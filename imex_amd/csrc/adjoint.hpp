@@ -57,7 +57,7 @@ void adj_diag_add(hipStream_t st, const AD& a, const double* dv, const double* i
 // dv = vs - 1/(gamma-1) for the forward's Jacobian-mode diagonal vs (a padded plane)
 void adj_slope_dv(hipStream_t st, const AD& a, const Phys& P, const double* vs, double* dv);
 void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* u,
-                    const double* mb, double* lam, double* ubar);
+                    double* lam, double* ubar, double* work);   // work: >= 256 doubles of scratch
 // Redlich-Kwong viscosity scale of the pressure operator, Vbase_slope(p(Qe), rho), and its cotangents
 void adj_visc(hipStream_t st, const AD& a, const Phys& P, const double* pe, const double* u, double* visc);
 void adj_visc_vjp(hipStream_t st, const AD& a, const Phys& P, const double* pe, const double* u, const double* viscbar,

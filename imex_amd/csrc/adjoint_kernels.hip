@@ -833,18 +833,18 @@ __global__ void __launch_bounds__(AB) k_adj_diag_add(AD a, const double* __restr
 // over the floored cells (lam = J^-T pbar).  Where M_cor decides Mbig it reaches b at the cell that attains
 // min b (lam[argmin] += Mbig_bar dM_cor/d(min b)) and, when the Redlich-Kwong term decides Vpe, rho at the
 // cell that attains it (lowest indices on ties).  One block.
-__global__ void __launch_bounds__(1024) k_adj_floor_mbig(AD a, Phys P, const double* __restrict__ p,
-                                                         const double* __restrict__ b, const double* __restrict__ u,
-                                                         const double* __restrict__ mb, double* __restrict__ lam,
-                                                         double* __restrict__ ubar) {
-  __shared__ double sm[16], sb[16], sv[16];
-  __shared__ int si[16], sj[16];
+#define FLB 64          // blocks of the floor reduction
+__global__ void __launch_bounds__(256) k_adj_floor_part(AD a, Phys P, const double* __restrict__ p, const double* __restrict__ b,
+                                                        const double* __restrict__ u, const double* __restrict__ lam,
+                                                        double* __restrict__ work) {
+  __shared__ double sm[4], sb[4], sv[4];
+  __shared__ int si[4], sj[4];
   const int N = a.nx * a.ny;
-  const double pf = P.p_floor, av = 1.0 / (P.gamma - 1.0);
+  const double pf = P.p_floor;
   const bool rk = P.eos != IMEX_EOS_IDEAL;
   double acc = 0.0, bmin = 1e300, vmax = -1e300;
   int imin = N, imax = N;
-  for (int n = threadIdx.x; n < N; n += blockDim.x) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
     const long long c = tix(a, n % a.nx, n / a.nx);
     if (p[c] < pf) acc += lam[c] * (p[c] - pf);
     if (b[c] < bmin) { bmin = b[c]; imin = n; }
@@ -864,26 +864,42 @@ __global__ void __launch_bounds__(1024) k_adj_floor_mbig(AD a, Phys P, const dou
   if (lane == 0) { sm[wid] = acc; sb[wid] = bmin; si[wid] = imin; sv[wid] = vmax; sj[wid] = imax; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
-    for (int w = 1; w < nw; w++) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
       acc += sm[w];
       if (sb[w] < bmin || (sb[w] == bmin && si[w] < imin)) { bmin = sb[w]; imin = si[w]; }
       if (sv[w] > vmax || (sv[w] == vmax && sj[w] < imax)) { vmax = sv[w]; imax = sj[w]; }
     }
-    const double mbar = -acc;
-    double vpe = pf * av;
-    const bool vrk = rk && vmax > vpe;
-    if (vrk) vpe = vmax;
-    const double mcor = fmax(0.0, vpe - bmin) / ((1.0 - P.theta) * pf);
-    if (mcor > P.mbig_fac * av && vpe - bmin > 0.0 && imin < N) {
-      const double s = mbar / ((1.0 - P.theta) * pf);      // Mbar dM_cor/dVpe; dM_cor/d(min b) = -that
-      lam[tix(a, imin % a.nx, imin / a.nx)] -= s;
-      if (vrk && imax < N) {
-        const long long c = tix(a, imax % a.nx, imax / a.nx);
-        DN<1> r(u[c]);
-        r.g[0] = 1.0;
-        ubar[c] += s * vbase_t(P, DN<1>(pf), r).g[0];
-      }
+    int* wi = (int*)(work + 3 * FLB);
+    work[blockIdx.x] = acc; work[FLB + blockIdx.x] = bmin; work[2 * FLB + blockIdx.x] = vmax;
+    wi[blockIdx.x] = imin; wi[FLB + blockIdx.x] = imax;
+  }
+}
+__global__ void k_adj_floor_fin(AD a, Phys P, const double* __restrict__ u, const double* __restrict__ work,
+                                double* __restrict__ lam, double* __restrict__ ubar) {
+  if (threadIdx.x != 0) return;
+  const int N = a.nx * a.ny;
+  const int* wi = (const int*)(work + 3 * FLB);
+  double acc = 0.0, bmin = 1e300, vmax = -1e300;
+  int imin = N, imax = N;
+  for (int q = 0; q < FLB; q++) {                 // block order: the same sums on every run
+    acc += work[q];
+    if (work[FLB + q] < bmin || (work[FLB + q] == bmin && wi[q] < imin)) { bmin = work[FLB + q]; imin = wi[q]; }
+    if (work[2 * FLB + q] > vmax || (work[2 * FLB + q] == vmax && wi[FLB + q] < imax)) { vmax = work[2 * FLB + q]; imax = wi[FLB + q]; }
+  }
+  const double pf = P.p_floor, av = 1.0 / (P.gamma - 1.0);
+  const double mbar = -acc;
+  double vpe = pf * av;
+  const bool vrk = P.eos != IMEX_EOS_IDEAL && vmax > vpe;
+  if (vrk) vpe = vmax;
+  const double mcor = fmax(0.0, vpe - bmin) / ((1.0 - P.theta) * pf);
+  if (mcor > P.mbig_fac * av && vpe - bmin > 0.0 && imin < N) {
+    const double s = mbar / ((1.0 - P.theta) * pf);      // Mbar dM_cor/dVpe; dM_cor/d(min b) = -that
+    lam[tix(a, imin % a.nx, imin / a.nx)] -= s;
+    if (vrk && imax < N) {
+      const long long c = tix(a, imax % a.nx, imax / a.nx);
+      DN<1> r(u[c]);
+      r.g[0] = 1.0;
+      ubar[c] += s * vbase_t(P, DN<1>(pf), r).g[0];
     }
   }
 }
@@ -1330,8 +1346,9 @@ void adj_diag_add(hipStream_t st, const AD& a, const double* dv, const double* i
   k_adj_diag_add<<<nblk(a), AB, 0, st>>>(a, dv, in, out);
 }
 void adj_floor_mbig(hipStream_t st, const AD& a, const Phys& P, const double* p, const double* b, const double* u,
-                    const double* mb, double* lam, double* ubar) {
-  k_adj_floor_mbig<<<1, 1024, 0, st>>>(a, P, p, b, u, mb, lam, ubar);
+                    double* lam, double* ubar, double* work) {
+  k_adj_floor_part<<<FLB, 256, 0, st>>>(a, P, p, b, u, lam, work);
+  k_adj_floor_fin<<<1, 64, 0, st>>>(a, P, u, work, lam, ubar);
 }
 void adj_zero(hipStream_t st, const AD& a, int nc, double* p) {
   k_adj_lin<<<akr_grid1(a, nc), AB, 0, st>>>(a, nc, 2, nullptr, nullptr, 0.0, p);
