@@ -18,9 +18,12 @@
 // exit at once.
 #include "kernels.hpp"
 
-#ifndef IMEX_MAG_WAVES
-#define IMEX_MAG_WAVES 3
-#endif
+// Waves per SIMD the operator kernels are compiled for (their register budget: 512 / waves VGPRs):
+// the corner-staged magnetic operator fits 128 VGPRs, the vector-potential and reflective-wall
+// magnetic operators need up to 168, the scalar operators fit 128 (FP32 pressure Arnoldi: 64).
+template <int OPK, class T> constexpr int op_waves() {
+  return OPK == 0 ? 4 : (OPK == 3 || OPK == 4) ? 3 : (sizeof(T) == 4 ? 8 : 4);
+}
 
 // Krylov vectors of one storage type (double for the FP64 form, float for the inner solves of
 // the mixed form): the basis V_i, the unnormalised next vector w and the operator image Av.
@@ -219,6 +222,69 @@ __device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem<T>& s
   out[2] = ax7 - h * lap[2];
 }
 
+// The periodic / outflow operator stages the six corner terms of every corner of the tile once
+// (a corner is shared by four cells) before the cells form their face fluxes: the same arithmetic,
+// a quarter of the corner work.
+template <class T> struct MagSmemC : MagSmem<T> {
+  T c[6][TY + 1][TX + 1];     // corner (a, b) between LDS cells a..a+1, b..b+1
+};
+// Corner terms of the (TX+1) x (TY+1) corners of the tile: thread t takes corner (t % TX + 1,
+// t / TX + 1); threads t < TX + TY + 1 also take a corner of the left column or the bottom row.
+template <class T>
+__device__ __forceinline__ void mag_corners(MagSmemC<T>& sm) {
+  const int t = threadIdx.x;
+  auto put = [&](int a, int b) {
+    T cn[6];
+    mag_corner(sm, a, b, cn);
+#pragma unroll
+    for (int k = 0; k < 6; k++) sm.c[k][b][a] = cn[k];
+  };
+  put(t % TX + 1, t / TX + 1);
+  if (t < TX + TY + 1) {
+    if (t <= TY) put(0, t);
+    else put(t - TY, 0);
+  }
+}
+template <class T>
+__device__ __forceinline__ void mag_compute_c(const MagOp& op, const MagSmemC<T>& sm, int tx, int ty, bool valid, T* out) {
+  if (!valid) return;
+  const Dims& d = op.d;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
+  const T idx = T(1) / dx, idy = T(1) / dy, midx = T(-1) / dx;
+  const int a = tx + 1, b = ty + 1;
+  T c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    c00[k] = sm.c[k][b - 1][a - 1];
+    c10[k] = sm.c[k][b - 1][a];
+    c01[k] = sm.c[k][b][a - 1];
+    c11[k] = sm.c[k][b][a];
+  }
+  T dC[6], dN[6];
+  mag_dv(sm, a, b, dC);
+  T gxu, gzu, gxl, gzl, fyr, fzr, fyl, fzl;
+  mag_dv(sm, a, b + 1, dN);
+  mag_yface(sm, a, b, c01, c11, dC, dN, idx, idy, midx, gxu, gzu);
+  mag_dv(sm, a, b - 1, dN);
+  mag_yface(sm, a, b - 1, c00, c10, dN, dC, idx, idy, midx, gxl, gzl);
+  mag_dv(sm, a + 1, b, dN);
+  mag_xface(sm, a, b, c10, c11, dC, dN, idy, midx, fyr, fzr);
+  mag_dv(sm, a - 1, b, dN);
+  mag_xface(sm, a - 1, b, c00, c01, dN, dC, idy, midx, fyl, fzl);
+  auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
+  T ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
+  T ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
+  T ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
+  const T nu_x = (T)op.S[S_BMAXX] * dx, nu_y = (T)op.S[S_BMAXY] * dy;
+  T lap[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - T(2) * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - T(2) * X(c, 0, 0) + X(c, 0, 1));
+  out[0] = ax5 - h * lap[0];
+  out[1] = ax6 - h * lap[1];
+  out[2] = ax7 - h * lap[2];
+}
+
 // The same for a tile type SM (the reflective path's local neighbourhood).
 template <class T, class SM>
 __device__ __forceinline__ void mag_compute_at(const MagOp& op, const SM& sm, int a, int b, T* out) {
@@ -359,33 +425,46 @@ __device__ __forceinline__ void e_precond(const POp& op, int cidx, T* y) {
 // operator stages), then compute M^-1 A x for the block's cells in arithmetic type T.
 template <class T> struct PSmem { T x[(TY + 2) * (TX + 2)]; };
 template <int OPK, class T> struct OpSel;
+// Every operator: load (stage the tile), prep (a second pass over the staged tile between two
+// block barriers, when PREP), compute (M^-1 A x of one cell).
 template <class T> struct OpSel<0, T> {
   static constexpr int NC = 3;
+  static constexpr bool PREP = true;
   typedef MagOp Op;
   typedef MagIds Ids;
   __device__ static Op make(const Inst& I, const Phys&, const Ids& ids) { return make_mag_op(I, ids); }
-  typedef MagSmem<T> Smem;
+  typedef MagSmemC<T> Smem;
   template <class VT>
   __device__ static void load(const MagOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     mag_load(op, sm, x, h, scale, gx0, gy0);
   }
+  __device__ static void prep(const MagOp&, Smem& sm) { mag_corners(sm); }
   __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int, int, int, bool valid, T* y) {
-    mag_compute(op, sm, tx, ty, valid, y);
+    mag_compute_c(op, sm, tx, ty, valid, y);
   }
   __device__ static T center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
   __device__ static void precond(const MagOp&, int, int, int, T*) {}
   __device__ static const Dims& dims(const MagOp& op) { return op.d; }
 };
 // The magnetic operator with the reflective-wall neighbour flags: its own instantiation, so that the
-// flag-free operator keeps its register allocation
+// flag-free operator keeps its register allocation; each cell forms its own corners (a cell next to
+// a wall sees its neighbours with the wall's signs)
 template <class T> struct OpSel<4, T> : OpSel<0, T> {
-  typedef typename OpSel<0, T>::Smem Smem;
+  static constexpr bool PREP = false;
+  typedef MagSmem<T> Smem;
+  template <class VT>
+  __device__ static void load(const MagOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
+    mag_load(op, sm, x, h, scale, gx0, gy0);
+  }
+  __device__ static void prep(const MagOp&, Smem&) {}
   __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
     mag_compute_refl(op, sm, tx, ty, i, j, valid, y);
   }
+  __device__ static T center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
 };
 template <class T> struct OpSel<1, T> {
   static constexpr int NC = 1;
+  static constexpr bool PREP = false;
   typedef POp Op;
   typedef PIds Ids;
   __device__ static Op make(const Inst& I, const Phys& P, const Ids& ids) { return make_p_op(I, P, ids); }
@@ -397,6 +476,7 @@ template <class T> struct OpSel<1, T> {
   __device__ static void compute(const POp& op, Smem& sm, int tx, int ty, int i, int j, int cidx, bool valid, T* y) {
     if (valid) p_apply(op, sm.x, tx + 1, ty + 1, i, j, cidx, y);
   }
+  __device__ static void prep(const POp&, Smem&) {}
   __device__ static T center(const Smem& sm, int, int tx, int ty) { return sm.x[(ty + 1) * (TX + 2) + tx + 1]; }
   __device__ static void precond(const POp& op, int cidx, int i, int j, T* y) { p_precond(op, cidx, i, j, y); }
   __device__ static const Dims& dims(const POp& op) { return op.d; }
@@ -507,6 +587,7 @@ __device__ __forceinline__ void pot_compute(const PotOp& op, const PotSmem<T>& s
 
 template <class T> struct OpSel<3, T> {
   static constexpr int NC = 3;
+  static constexpr bool PREP = false;
   typedef PotOp Op;
   typedef MagIds Ids;
   __device__ static Op make(const Inst& I, const Phys&, const Ids& ids) { return make_pot_op(I, ids); }
@@ -515,6 +596,7 @@ template <class T> struct OpSel<3, T> {
   __device__ static void load(const PotOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     pot_load(op, sm, x, h, scale, gx0, gy0);
   }
+  __device__ static void prep(const PotOp&, Smem&) {}
   __device__ static void compute(const PotOp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
     pot_compute(op, sm, tx, ty, i, j, valid, y);
   }
@@ -620,7 +702,7 @@ __device__ __forceinline__ void final_sums(const double* partial, int stride, in
 // Always in FP64 (it is the outer residual of the mixed form's iterative refinement); r0 is
 // stored in the basis type VT, its norm is taken in FP64 before rounding.
 template <int OPK, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 || OPK == 4 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(op_waves<OPK, double>())))
 k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int ib,
                  int outer, int x_is_zero, int skip, int mixed) {
   BATCH_BLOCK(bm);
@@ -660,6 +742,7 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
       __syncthreads();
       S::load(op, sm, x, 1.0, false, gx0, gy0);
       __syncthreads();
+      if constexpr (S::PREP) { S::prep(op, sm); __syncthreads(); }
       S::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
     }
     if (valid) {
@@ -722,7 +805,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_gmres_residual_fin(const Inst* 
 // The dots of each tile are reduced over the wave right away (a uniform loop over i, so no
 // accumulator array is live across the operator) and summed per wave in LDS, in tile order.
 template <int OPK, class T, class VT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 || OPK == 4 ? IMEX_MAG_WAVES : (sizeof(T) == 4 ? 8 : 4))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(op_waves<OPK, T>())))
 k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int j) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -750,6 +833,7 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict_
     __syncthreads();
     S::load(op, sm, w, h, true, gx0, gy0);
     __syncthreads();
+    if constexpr (S::PREP) { S::prep(op, sm); __syncthreads(); }
     T y[NC];
     double vc[NC];
     S::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
@@ -945,7 +1029,7 @@ __global__ void __launch_bounds__(256) k_gmres_xupdate(const Inst* __restrict__ 
 
 // ---------------------------------------------------------------- plain operator application (tests, adjoint)
 template <int OPK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OPK == 0 || OPK == 3 || OPK == 4 ? IMEX_MAG_WAVES : 4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(op_waves<OPK, double>())))
 k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict__ IT, BMap bm, int ix, int iy) {
   BATCH_BLOCK(bm);
   const Inst& I = IT[inst_];
@@ -966,6 +1050,7 @@ k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict_
     __syncthreads();
     S::load(op, sm, x, 1.0, false, gx0, gy0);
     __syncthreads();
+    if constexpr (S::PREP) { S::prep(op, sm); __syncthreads(); }
     double y[NC];
     S::compute(op, sm, tx, ty, cx + GC, cy + GC, cidx, valid, y);
     if (valid) for (int c = 0; c < NC; c++) Ax[(long long)c * d.N + cidx] = y[c];
