@@ -227,7 +227,20 @@ __device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem<T>& s
 // a quarter of the corner work.
 template <class T> struct MagSmemC : MagSmem<T> {
   T c[6][TY + 1][TX + 1];     // corner (a, b) between LDS cells a..a+1, b..b+1
+  T k[8];                     // the operator's uniform coefficients (mag_coef), formed once per block
 };
+enum { MK_IDX, MK_IDY, MK_MIDX, MK_HHDX, MK_HHDY, MK_LX, MK_LY, MK_H };
+// 1/dx, 1/dy, -1/dx, h^2/dx, h^2/dy, nu_x/dx^2, nu_y/dy^2 and h, each formed as mag_compute forms it
+template <class T>
+__device__ __forceinline__ void mag_coef(const MagOp& op, T* k) {
+  const Dims& d = op.d;
+  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
+  const T nu_x = (T)op.S[S_BMAXX] * dx, nu_y = (T)op.S[S_BMAXY] * dy;
+  k[MK_IDX] = T(1) / dx; k[MK_IDY] = T(1) / dy; k[MK_MIDX] = T(-1) / dx;
+  k[MK_HHDX] = h * h / dx; k[MK_HHDY] = h * h / dy;
+  k[MK_LX] = nu_x / (dx * dx); k[MK_LY] = nu_y / (dy * dy);
+  k[MK_H] = h;
+}
 // Corner terms of the (TX+1) x (TY+1) corners of the tile: thread t takes corner (t % TX + 1,
 // t / TX + 1); threads t < TX + TY + 1 also take a corner of the left column or the bottom row.
 template <class T>
@@ -246,11 +259,9 @@ __device__ __forceinline__ void mag_corners(MagSmemC<T>& sm) {
   }
 }
 template <class T>
-__device__ __forceinline__ void mag_compute_c(const MagOp& op, const MagSmemC<T>& sm, int tx, int ty, bool valid, T* out) {
+__device__ __forceinline__ void mag_compute_c(const MagSmemC<T>& sm, int tx, int ty, bool valid, T* out) {
   if (!valid) return;
-  const Dims& d = op.d;
-  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
-  const T idx = T(1) / dx, idy = T(1) / dy, midx = T(-1) / dx;
+  const T idx = sm.k[MK_IDX], idy = sm.k[MK_IDY], midx = sm.k[MK_MIDX], h = sm.k[MK_H];
   const int a = tx + 1, b = ty + 1;
   T c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
 #pragma unroll
@@ -272,14 +283,15 @@ __device__ __forceinline__ void mag_compute_c(const MagOp& op, const MagSmemC<T>
   mag_dv(sm, a - 1, b, dN);
   mag_xface(sm, a - 1, b, c00, c01, dN, dC, idy, midx, fyl, fzl);
   auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
-  T ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
-  T ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
-  T ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
-  const T nu_x = (T)op.S[S_BMAXX] * dx, nu_y = (T)op.S[S_BMAXY] * dy;
+  const T hhdx = sm.k[MK_HHDX], hhdy = sm.k[MK_HHDY];
+  T ax5 = X(0, 0, 0) + hhdy * (gxu - gxl);
+  T ax6 = X(1, 0, 0) + hhdx * (fyr - fyl);
+  T ax7 = X(2, 0, 0) + hhdx * (fzr - fzl) + hhdy * (gzu - gzl);
+  const T lx = sm.k[MK_LX], ly = sm.k[MK_LY];
   T lap[3];
 #pragma unroll
   for (int c = 0; c < 3; c++)
-    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - T(2) * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - T(2) * X(c, 0, 0) + X(c, 0, 1));
+    lap[c] = lx * (X(c, -1, 0) - T(2) * X(c, 0, 0) + X(c, 1, 0)) + ly * (X(c, 0, -1) - T(2) * X(c, 0, 0) + X(c, 0, 1));
   out[0] = ax5 - h * lap[0];
   out[1] = ax6 - h * lap[1];
   out[2] = ax7 - h * lap[2];
@@ -425,8 +437,9 @@ __device__ __forceinline__ void e_precond(const POp& op, int cidx, T* y) {
 // operator stages), then compute M^-1 A x for the block's cells in arithmetic type T.
 template <class T> struct PSmem { T x[(TY + 2) * (TX + 2)]; };
 template <int OPK, class T> struct OpSel;
-// Every operator: load (stage the tile), prep (a second pass over the staged tile between two
-// block barriers, when PREP), compute (M^-1 A x of one cell).
+// Every operator: init (once per block, before the first tile's barrier), load (stage the tile),
+// prep (a second pass over the staged tile between two block barriers, when PREP), compute
+// (M^-1 A x of one cell).
 template <class T> struct OpSel<0, T> {
   static constexpr int NC = 3;
   static constexpr bool PREP = true;
@@ -438,9 +451,10 @@ template <class T> struct OpSel<0, T> {
   __device__ static void load(const MagOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     mag_load(op, sm, x, h, scale, gx0, gy0);
   }
+  __device__ static void init(const MagOp& op, Smem& sm) { if (threadIdx.x == 0) mag_coef(op, sm.k); }
   __device__ static void prep(const MagOp&, Smem& sm) { mag_corners(sm); }
-  __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int, int, int, bool valid, T* y) {
-    mag_compute_c(op, sm, tx, ty, valid, y);
+  __device__ static void compute(const MagOp&, Smem& sm, int tx, int ty, int, int, int, bool valid, T* y) {
+    mag_compute_c(sm, tx, ty, valid, y);
   }
   __device__ static T center(const Smem& sm, int c, int tx, int ty) { return sm.x[c][ty + 1][tx + 1]; }
   __device__ static void precond(const MagOp&, int, int, int, T*) {}
@@ -456,6 +470,7 @@ template <class T> struct OpSel<4, T> : OpSel<0, T> {
   __device__ static void load(const MagOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     mag_load(op, sm, x, h, scale, gx0, gy0);
   }
+  __device__ static void init(const MagOp&, Smem&) {}
   __device__ static void prep(const MagOp&, Smem&) {}
   __device__ static void compute(const MagOp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
     mag_compute_refl(op, sm, tx, ty, i, j, valid, y);
@@ -476,6 +491,7 @@ template <class T> struct OpSel<1, T> {
   __device__ static void compute(const POp& op, Smem& sm, int tx, int ty, int i, int j, int cidx, bool valid, T* y) {
     if (valid) p_apply(op, sm.x, tx + 1, ty + 1, i, j, cidx, y);
   }
+  __device__ static void init(const POp&, Smem&) {}
   __device__ static void prep(const POp&, Smem&) {}
   __device__ static T center(const Smem& sm, int, int tx, int ty) { return sm.x[(ty + 1) * (TX + 2) + tx + 1]; }
   __device__ static void precond(const POp& op, int cidx, int i, int j, T* y) { p_precond(op, cidx, i, j, y); }
@@ -596,6 +612,7 @@ template <class T> struct OpSel<3, T> {
   __device__ static void load(const PotOp& op, Smem& sm, const VT* x, double h, bool scale, int gx0, int gy0) {
     pot_load(op, sm, x, h, scale, gx0, gy0);
   }
+  __device__ static void init(const PotOp&, Smem&) {}
   __device__ static void prep(const PotOp&, Smem&) {}
   __device__ static void compute(const PotOp& op, Smem& sm, int tx, int ty, int i, int j, int, bool valid, T* y) {
     pot_compute(op, sm, tx, ty, i, j, valid, y);
@@ -731,6 +748,7 @@ k_gmres_residual(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __res
   const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
   double acc[2] = {0.0, 0.0};     // ||M^-1 r||^2, and ||M^-1 b||^2 on the first sweep of the mixed form
+  S::init(op, sm);
   for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -824,6 +842,7 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict_
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int i = lane; i <= j; i += 64) wsum[wid][i] = 0.0;
   VT* vj = KV<VT>::V(I, j);
+  S::init(op, sm);
   for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -870,14 +889,28 @@ k_arnoldi_apply(typename OpSel<OPK, T>::Ids ids, Phys P, const Inst* __restrict_
         }
       }
     } else {
-      for (int i = 0; i <= j; i++) {
-        double part = 0.0;
-        if (valid) {
-          const VT* vi = KV<VT>::V(I, i);
-          for (int c = 0; c < NC; c++) part += (i < j ? (double)vi[(long long)c * d.N + cidx] : vc[c]) * (double)y[c];
+      // the same for the three-component operators, four stored vectors (all components) at a time
+      constexpr int CH = 4;
+      const long long cl = valid ? cidx : 0;
+      for (int i0 = 0; i0 <= j; i0 += CH) {
+        VT vv[CH][NC];
+#pragma unroll
+        for (int q = 0; q < CH; q++)
+          if (i0 + q < j) {
+            const VT* vi = KV<VT>::V(I, i0 + q);
+#pragma unroll
+            for (int c = 0; c < NC; c++) vv[q][c] = vi[(long long)c * d.N + cl];
+          }
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+          const int i = i0 + q;
+          if (i > j) break;
+          double part = 0.0;
+          if (valid)
+            for (int c = 0; c < NC; c++) part += (i < j ? (double)vv[q][c] : vc[c]) * (double)y[c];
+          double s = wave_sum(part);
+          if (lane == 0) wsum[wid][i] += s;
         }
-        double s = wave_sum(part);
-        if (lane == 0) wsum[wid][i] += s;
       }
     }
   }
@@ -1041,6 +1074,7 @@ k_op_apply(typename OpSel<OPK, double>::Ids ids, Phys P, const Inst* __restrict_
   __shared__ typename S::Smem sm;
   const Dims& d = S::dims(op);
   const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
+  S::init(op, sm);
   for (int tile = bx_; tile < ntx * nty; tile += gx_) {
     int gx0 = (tile % ntx) * TX, gy0 = (tile / ntx) * TY;
     int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
