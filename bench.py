@@ -393,6 +393,34 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}), flush=True)
 
 
+def stream_ceiling():
+    """Practical streaming rates on this GPU, the yardstick beside the 8 TB/s peak: torch elementwise kernels
+    with one or two input streams and one output stream (negation, addition) over vectors of 1 Mi doubles
+    (8 MB each, the size of one field at 1024^2; such working sets can stay in the 256 MB last-level cache
+    across launches) and of 64 Mi doubles (512 MB, beyond it).  Each rate is bytes moved over the event-timed
+    average of 50 back-to-back launches after 5 untimed ones."""
+    import torch
+    out = {}
+    for n, tag in ((1 << 20, "8MB_vectors"), (64 << 20, "512MB_vectors")):
+        a = torch.rand(n, dtype=torch.float64, device="cuda")
+        b = torch.rand(n, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        cases = {"copy_1r1w": (lambda: torch.neg(a, out=c), 2), "add_2r1w": (lambda: torch.add(a, b, out=c), 3)}
+        for name, (fn, streams) in cases.items():
+            for _ in range(5):
+                fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(50):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / 50
+            out[f"{tag}_{name}"] = {"us": round(us, 2), "GB/s": round(streams * n * 8 / us / 1e3, 1)}
+        del a, b, c
+    print(json.dumps({"stream_ceiling": out}), flush=True)
+
+
 KERNEL_OF_CLASS = {   # (class, form) or (variant, class, form) -> kernel symbol in the committed rocprof summary
     ("potential_a", "mag_arnoldi_apply", "fp64"): "k_arnoldi_apply<3, double, double>",
     ("potential_a", "mag_arnoldi_apply", "mixed"): "k_arnoldi_apply<3, float, float>",
@@ -516,9 +544,14 @@ def main():
                     help="timed FP64 steps of each other reconstruction among the secondary workloads")
     ap.add_argument("--jet-steps", type=int, default=20,
                     help="timed steps of the 400x600 MHD jet FP64 / mixed pair among the secondary workloads")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="print the practical streaming rates of plain copy / add kernels (8 MB and 512 MB vectors) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (reference point for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.stream_ceiling:
+        stream_ceiling()
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
