@@ -92,8 +92,9 @@ enum Shape {
   SH_LIN_TOT,     // (nx+6)(ny+6) cells, 256 per block
   SH_LIN_N,       // nx ny cells, 256 per block
   SH_START,       // start-of-step reduction grid
-  SH_TILES,       // operator tiles (64x4), capped: magnetic operator, Newton residual
+  SH_TILES,       // operator tiles (64x4), capped: magnetic operator
   SH_TILES_P,     // operator tiles (64x4), capped: pressure operator
+  SH_TILES_N,     // operator tiles (64x4), one round of resident blocks: Newton residual
   SH_TILES_ALL,   // operator tiles, one block each up to 4096
   SH_VEC3,        // 3 nx ny vector elements, capped
   SH_VEC1,        // nx ny vector elements, capped

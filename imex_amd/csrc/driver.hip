@@ -183,6 +183,7 @@ struct imex_ctx {
   // keeps resident avoids a partly filled last round (magnetic FP64 apply: 3 waves/SIMD, 768
   // blocks; pressure FP64: 6, 1536; pressure FP32 apply: 8, 2048)
   int cap_tiles = 2048, cap_tiles_p = 2048, cap_vec = 2048;
+  int cap_tiles_n = 2048;            // Newton residual: the blocks resident at once (one round)
   int ex_tile = 32, ex_rows = 8;          // explicit-kernel tile (16x16, 32x8, 64x4, 32x16 or 64x8; autotuned)
   // state buffers: the step reads Us (or u_pri before the first step) and its last stage writes Uo;
   // faces likewise (bxn/byn read, bx2/by2 written)
@@ -344,6 +345,7 @@ static void shape_grid(const imex_ctx* c, int sh, const Dims& d, int& gx, int& n
     case SH_START: nb = (int)std::min<long long>(cdiv(tot, 256), 1024); gx = nb; break;
     case SH_TILES: nb = (int)std::min<long long>(tiles, c->cap_tiles); gx = nb; break;
     case SH_TILES_P: nb = (int)std::min<long long>(tiles, c->cap_tiles_p); gx = nb; break;
+    case SH_TILES_N: nb = (int)std::min<long long>(tiles, c->cap_tiles_n); gx = nb; break;
     case SH_TILES_ALL: nb = (int)std::min<long long>(tiles, 4096); gx = nb; break;
     case SH_VEC3: nb = (int)std::min<long long>(cdiv(3LL * d.N, 256), c->cap_vec); gx = nb; break;
     case SH_VEC1: nb = (int)std::min<long long>(cdiv(d.N, 256), c->cap_vec); gx = nb; break;
@@ -695,7 +697,7 @@ static int run_newton_budget(imex_ctx* c, int ib, int ip, bool mbig_ready) {
     tag_last(c, TG_NIT, nk, 0, it);
   }
   // the last residual that ran (a converged one, or the closing residual NB) saw the final p
-  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), c->prm.floor_on ? 1024 : 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot, L.bm[SH_TILES],
+  k_budget_newton_check<<<dim3(L.nblocks[SH_ONE]), c->prm.floor_on ? 1024 : 64, 0, c->st>>>(L.IT, L.bm[SH_ONE], c->brecs[self].slot, L.bm[SH_TILES_N],
                                                                    c->prm.floor_on, pst_threshold(c));
   return 0;
 }
@@ -1687,6 +1689,12 @@ int imex_batch_create(const imex_params* prm, int n, const int* nx, const int* n
                   prm->recon_state == IMEX_RECONSTATE_CONS;
   c->needA = prm->magfield == IMEX_MAGFIELD_UNSTAGGER || prm->magfield == IMEX_MAGFIELD_POTENTIAL_A;
   c->cap_tiles_p = op_prec(c, 1) ? 2048 : 1536;
+  // A tile-looping grid larger than the blocks resident at once ends in a partly filled round: the Newton
+  // residual runs one round of resident blocks
+  {
+    const int res = newton_residual_resident(prm->eos != IMEX_EOS_IDEAL);
+    c->cap_tiles_n = res > 0 ? std::min(res, 4096) : 2048;
+  }
   c->ni = n;
   c->iprm.assign(n, c->prm);
   c->hI.resize(n);

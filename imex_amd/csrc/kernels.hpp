@@ -180,6 +180,9 @@ void arnoldi_update(const Launch& L, int op, int prec, int j, int m);
 void arnoldi_update_fin(const Launch& L, int op, int prec, int j, int m);
 void gmres_xupdate(const Launch& L, int op, int prec, int ix, int x_zero, int skip);
 void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const Phys& P, int ix, int iy);
+// blocks of the Newton residual (ideal-gas or Redlich-Kwong instantiation) resident on the current
+// device at once, 0 if the occupancy query fails
+int newton_residual_resident(bool rk);
 void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int ib, int it, double ntol);
 void newton_residual_fin(const Launch& L, int it, double ntol);
 void newton_mbig(const Launch& L, const Phys& P, int ib, int iu);

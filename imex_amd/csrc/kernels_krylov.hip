@@ -1353,11 +1353,18 @@ void op_apply(const Launch& L, int op, const MagIds& mo, const PIds& po, const P
 }
 
 void newton_residual(const Launch& L, const PIds& po, const Phys& P, int ip, int ib, int it, double ntol) {
-  if (P.eos != IMEX_EOS_IDEAL) k_newton_residual<true><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
-  else k_newton_residual<false><<<GRID(L, SH_TILES), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES], ip, ib, it, ntol);
+  if (P.eos != IMEX_EOS_IDEAL) k_newton_residual<true><<<GRID(L, SH_TILES_N), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_N], ip, ib, it, ntol);
+  else k_newton_residual<false><<<GRID(L, SH_TILES_N), 256, 0, L.st>>>(po, P, L.IT, L.bm[SH_TILES_N], ip, ib, it, ntol);
+}
+int newton_residual_resident(bool rk) {
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const hipError_t e = rk ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_newton_residual<true>, 256, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_newton_residual<false>, 256, 0);
+  return e == hipSuccess ? per * ncu : 0;
 }
 void newton_residual_fin(const Launch& L, int it, double ntol) {
-  k_newton_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], it, ntol);
+  k_newton_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES_N], it, ntol);
 }
 
 void newton_mbig(const Launch& L, const Phys& P, int ib, int iu) {
@@ -1382,5 +1389,5 @@ void pstats_launch(const Launch& L, int ip, double pfloor, double thr) {
   k_pstats_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_PSTATS], thr);
 }
 void pstats_fin_newton(const Launch& L, double thr) {
-  k_pstats_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES], thr);
+  k_pstats_fin<<<GRID(L, SH_ONE), FIN_THREADS, 0, L.st>>>(L.IT, L.bm[SH_ONE], L.bm[SH_TILES_N], thr);
 }
