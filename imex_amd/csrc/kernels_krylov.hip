@@ -182,49 +182,11 @@ __device__ __forceinline__ void mag_xface(const SM& sm, int xa, int b, const T* 
   fz = nf2 / rf * (midx * dpm + idy * (cT[2] - cB[2])) + nf0 / rf * (midx * (R[5] - L[5]) - idy * (cT[3] - cB[3]));
 }
 
-// A x for the cell of a valid thread (no block synchronisation: each thread forms the four face
-// fluxes of its own cell from the staged tiles; a face shared by two cells is formed by both
-// with the same arithmetic).  T is the arithmetic type (double, or float in the mixed form).
-template <class T>
-__device__ __forceinline__ void mag_compute(const MagOp& op, const MagSmem<T>& sm, int tx, int ty, bool valid, T* out) {
-  if (!valid) return;
-  const Dims& d = op.d;
-  const T h = (T)op.S[S_ADT], dx = (T)d.dx, dy = (T)d.dy;
-  const T idx = T(1) / dx, idy = T(1) / dy, midx = T(-1) / dx;
-  const int a = tx + 1, b = ty + 1;
-  T c00[6], c10[6], c01[6], c11[6];      // corners (a-1, b-1), (a, b-1), (a-1, b), (a, b)
-  mag_corner(sm, a - 1, b - 1, c00);
-  mag_corner(sm, a, b - 1, c10);
-  mag_corner(sm, a - 1, b, c01);
-  mag_corner(sm, a, b, c11);
-  T dC[6], dN[6];
-  mag_dv(sm, a, b, dC);
-  T gxu, gzu, gxl, gzl, fyr, fzr, fyl, fzl;
-  mag_dv(sm, a, b + 1, dN);
-  mag_yface(sm, a, b, c01, c11, dC, dN, idx, idy, midx, gxu, gzu);
-  mag_dv(sm, a, b - 1, dN);
-  mag_yface(sm, a, b - 1, c00, c10, dN, dC, idx, idy, midx, gxl, gzl);
-  mag_dv(sm, a + 1, b, dN);
-  mag_xface(sm, a, b, c10, c11, dC, dN, idy, midx, fyr, fzr);
-  mag_dv(sm, a - 1, b, dN);
-  mag_xface(sm, a - 1, b, c00, c01, dN, dC, idy, midx, fyl, fzl);
-  auto X = [&](int c, int di, int dj) { return sm.x[c][b + dj][a + di]; };
-  T ax5 = X(0, 0, 0) + (h * h / dy) * (gxu - gxl);
-  T ax6 = X(1, 0, 0) + (h * h / dx) * (fyr - fyl);
-  T ax7 = X(2, 0, 0) + h * h / dx * (fzr - fzl) + h * h / dy * (gzu - gzl);
-  const T nu_x = (T)op.S[S_BMAXX] * dx, nu_y = (T)op.S[S_BMAXY] * dy;
-  T lap[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-    lap[c] = nu_x / (dx * dx) * (X(c, -1, 0) - T(2) * X(c, 0, 0) + X(c, 1, 0)) + nu_y / (dy * dy) * (X(c, 0, -1) - T(2) * X(c, 0, 0) + X(c, 0, 1));
-  out[0] = ax5 - h * lap[0];
-  out[1] = ax6 - h * lap[1];
-  out[2] = ax7 - h * lap[2];
-}
-
-// The periodic / outflow operator stages the six corner terms of every corner of the tile once
-// (a corner is shared by four cells) before the cells form their face fluxes: the same arithmetic,
-// a quarter of the corner work.
+// A x for the cell of a valid thread.  The periodic / outflow operator stages the six corner terms
+// of every corner of the tile once (a corner is shared by four cells) between two block barriers;
+// each thread then forms the four face fluxes of its own cell from the staged tiles (a face shared
+// by two cells is formed by both with the same arithmetic).  T is the arithmetic type (double, or
+// float in the mixed form).
 template <class T> struct MagSmemC : MagSmem<T> {
   T c[6][TY + 1][TX + 1];     // corner (a, b) between LDS cells a..a+1, b..b+1
   T k[8];                     // the operator's uniform coefficients (mag_coef), formed once per block
